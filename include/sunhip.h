@@ -1,0 +1,140 @@
+/*
+ * sunhip.h -- the thin C ABI of the gfx950 kernels (libsunhip.so).
+ *
+ * Plain pointers and sizes only.  A grid context owns the device copy of one
+ * SUNTANS grid (numprocs=1, the whole grid on one GPU) in a tiled,
+ * layer-contiguous layout: 64 spatially ordered columns per tile (one
+ * wavefront, one lane per column) and, inside a tile, level k of all 64
+ * columns contiguous.  Edge rows are tiled the same way.
+ *
+ * Host arrays crossing this ABI are "flat": a per-cell field stores cell i's
+ * column (Nk[i] values, Nk[i]+1 for w-type fields) contiguously, cells in
+ * order; a per-edge field stores Nkc[j] values per edge; 2-D fields one value
+ * per cell or edge.  Every function returns 0 on success and a negative code
+ * on failure (sunhip_error() describes the last failure); none exits.
+ *
+ * The reference-signature drop-ins in suntans_hip.h are written on top of
+ * this ABI.
+ */
+#ifndef SUNHIP_H
+#define SUNHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sunhip_grid sunhip_grid;
+
+/* Planform topology and metrics.  Arrays are the reference's gridT members
+ * with the same meaning (main/grid.h:32-149). */
+typedef struct {
+  int Nc, Ne, maxfaces, Nkmax;
+  const int *nfaces;   /* [Nc] */
+  const int *neigh;    /* [Nc*maxfaces], -1 on the boundary */
+  const int *face;     /* [Nc*maxfaces] */
+  const int *normal;   /* [Nc*maxfaces], +-1 */
+  const int *grad;     /* [2*Ne], -1 on the boundary side */
+  const int *mark;     /* [Ne] */
+  const int *Nk;       /* [Nc] */
+  const int *Nkc;      /* [Ne] */
+  const int *cellp;    /* [Nc] */
+  const int *celldist; /* [3] */
+  const int *edgep;    /* [Ne] */
+  const int *edgedist; /* [6] */
+  const double *Ac;    /* [Nc] */
+  const double *df;    /* [Ne] */
+  const double *dg;    /* [Ne] */
+  const double *xv, *yv; /* [Nc], used only to order columns spatially */
+} sunhip_topology;
+
+/* Vertical extents that move with the free surface (UpdateDZ output). */
+typedef struct {
+  const int *ctop;     /* [Nc] */
+  const int *ctopold;  /* [Nc] */
+  const int *Nke;      /* [Ne] */
+  const int *etop;     /* [Ne] */
+} sunhip_extents;
+
+/* Solve controls (propT members of the same names). */
+typedef struct {
+  int maxiters;        /* qmaxiters or maxiters */
+  int precond;         /* qprecond (0 none, 2 vertical line) or hprecond (0/1) */
+  int resnorm;
+  double epsilon;      /* qepsilon or epsilon */
+  int check_every;     /* iterations between host convergence polls (0: auto) */
+} sunhip_cg_params;
+
+typedef struct {
+  int niters;          /* iterations run (the reference's loop counter n) */
+  int converged;
+  double resid;        /* sqrt(eps/eps0) at exit */
+  double eps0;
+  double device_ms;    /* device time of the iteration loop (HIP events) */
+  double setup_ms;     /* device time of coefficients + initial residual */
+} sunhip_cg_stats;
+
+const char *sunhip_error(void);
+int sunhip_device_count(void);
+
+int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *topo, int device);
+int sunhip_grid_destroy(sunhip_grid *g);
+int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *ext);
+/* Device bytes held by the context. */
+int64_t sunhip_grid_bytes(const sunhip_grid *g);
+/* Sizes: N cell-layers (sum Nk), N flat w values, edge-layer values (sum Nkc),
+ * padded cell-layer slots and tiles. */
+int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
+
+/* Named device fields.  Cell fields: "q","src","dzz","dzzold","s","wnew",
+ * "wtmp2","kappa_tv","Cn","gradSx","gradSy","x","r","z","p","rtmp",
+ * "stmp"...; w-type fields take Nk+1 values per column (names starting
+ * with 'w'); edge fields: "dzf","u","utmp2","SfHp","SfHm"; 2-D cell
+ * fields: "h","htmp","hcoef"...; 2-D edge fields: "D".  Upload/download
+ * move between a flat host array and the device layout. */
+int sunhip_upload(sunhip_grid *g, const char *name, const double *flat);
+int sunhip_download(sunhip_grid *g, const char *name, double *flat);
+/* Device-side copy of one field into another of the same kind. */
+int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src);
+
+/* ---- nonhydrostatic pressure (main/phys.c:2411-2550) ------------------- */
+/* Solve on the device-resident fields: "q" (warm start in, solution out),
+ * "src" (RHS, clobbered as the reference clobbers it), "dzz", "dzf", "D".
+ * Requires sunhip_set_extents.  theta/dt are not needed by this solve. */
+int sunhip_q_solve(sunhip_grid *g, const sunhip_cg_params *prm, sunhip_cg_stats *st);
+/* y = OperatorQ(x) on named device fields (QCoefficients from "dzz"). */
+int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y);
+/* xc = Preconditioner(x) (vertical-line Thomas solve per column). */
+int sunhip_q_precondition(sunhip_grid *g, const char *x, const char *xc);
+/* Run exactly n iterations regardless of the tolerance (parity at a fixed
+ * iteration count). */
+int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sunhip_cg_stats *st);
+
+/* ---- free surface (main/phys.c:3086-3217) ------------------------------ */
+/* Fields: "h" (in: boundary values; out: solution), "htmp" (RHS, clobbered),
+ * "D".  grav, theta, dt enter through HCoefficients. */
+int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                   sunhip_cg_stats *st);
+int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y);
+
+/* ---- TVD scalar transport (main/tvd.c:36-120, main/scalars.c:35-325) ---- */
+typedef struct {
+  double theta, dt, kappa;
+  int TVD;             /* 0 none (upwind), 1..4 Psi limiter */
+  int step;            /* prop->n */
+  int alpha_top, alpha_bot;
+} sunhip_scalar_params;
+/* SfHp/SfHm from "s", "u", "utmp2", "dzf" (gradSx/gradSy receive sumQ/sumQC). */
+int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
+/* One UpdateScalars step of field scal with "wnew","wtmp2","dzz","dzzold",
+ * "kappa_tv","u","utmp2","dzf"; writes scal and "Cn". */
+int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
+
+/* Synchronise the context's stream. */
+int sunhip_sync(sunhip_grid *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,80 @@
+/*
+ * sunstate.h -- host-side construction of SUNTANS grid/physics/property
+ * state for the drop-in solvers: a triangulated-rectangle grid builder with
+ * the reference's geometry conventions, allocation of the jagged per-cell and
+ * per-edge rows in the reference's shapes, the vertical-grid and flux-height
+ * updates the solvers consume, and flat (column-compact) access to fields.
+ *
+ * These are set-up utilities that produce the solvers' inputs; they are not
+ * part of the accelerated path.  Every function here is plain C.
+ */
+#ifndef SUNSTATE_H
+#define SUNSTATE_H
+
+#include "sun_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Rectangle [0,Lx]x[0,Ly] of nx*ny quads, each split into two right
+ * triangles along its lower-left/upper-right diagonal.  Voronoi points are
+ * centroids (with circumcentres the two triangles of a quad would share a
+ * point and dg would vanish).  All boundary edges get mark 1 (closed).
+ * Cell, edge and face conventions follow main/grid.c:654-720 (face/neigh),
+ * :992-1007 (normal), :3493-3533 (n1, n2, dg), :1628-1685 (cellp/edgep,
+ * celldist/edgedist). */
+gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly);
+
+/* Vertical grid: dz[Nkmax] reference layer thicknesses and dv[Nc] depths.
+ * Sets Nk (main/grid.c:1466-1498), Nke/Nkc (:1530-1563), ctop/etop = 0, and
+ * allocates dzz/dzzold (Nk) and dzf (Nkc) rows as main/phys.c:765-799. */
+int sun_grid_vertical(gridT *g, int Nkmax, const double *dz, const double *dv, int stairstep);
+
+/* Every array the path and its callers read, in the reference's shapes
+ * (main/phys.c:115-355): cell rows Nk long, w-type rows Nk+1, edge rows Nkc,
+ * column scratch Nkmax+1.  All zero. */
+physT *sun_phys_alloc(const gridT *g);
+propT *sun_prop_default(void);
+
+/* UpdateDZ (main/phys.c:816-968) and SetFluxHeight (main/phys.c:4338-4435)
+ * semantics for the linear free surface state in phys->h and phys->u. */
+void sun_update_dz(gridT *g, physT *p, const propT *prop, int option);
+void sun_set_flux_height(gridT *g, physT *p, const propT *prop);
+
+/* ComputeQSource (main/phys.c:2345-2395): RHS of the pressure Poisson
+ * equation from phys->u/w/utmp2/wtmp2; also sets D[j] = df/dg. */
+void sun_compute_qsource(REAL **src, gridT *g, physT *p, const propT *prop);
+
+physT *sun_phys_clone(const gridT *g, const physT *p);
+void sun_phys_copy(const gridT *g, const physT *from, physT *to);
+void sun_grid_free(gridT *g);
+void sun_phys_free(const gridT *g, physT *p);
+void sun_prop_free(propT *prop);
+
+/* Flat access.  Field names are the physT/gridT member names.  Cell fields
+ * are column-compact (cell i occupies Nk[i] values, w-type fields Nk[i]+1),
+ * edge fields occupy Nkc[j] values per edge, 2-D fields one value per cell
+ * or edge.  Returns the number of values, or -1 for an unknown name. */
+long sun_field_len(const gridT *g, const char *name);
+long sun_field_get(const gridT *g, const physT *p, const char *name, double *out);
+long sun_field_set(gridT *g, physT *p, const char *name, const double *in);
+long sun_grid_int(const gridT *g, const char *name, int *out);
+long sun_grid_set_int(gridT *g, const char *name, const int *in);
+/* Scalars: Nc, Ne, Np, Nkmax, maxfaces, N (= sum Nk), Ek (edge-layers with two
+ * wet neighbours). */
+long sun_grid_scalar(const gridT *g, const char *name);
+/* The member itself (a REAL** or REAL* row table) of physT / gridT by name,
+ * for passing to functions with the reference's signatures. */
+void *sun_phys_ptr(const physT *p, const char *name);
+void *sun_grid_ptr(const gridT *g, const char *name);
+size_t sun_prop_size(void);
+void sun_prop_copy(const propT *from, propT *to);
+/* propT members by name (doubles and ints). */
+int sun_prop_set(propT *prop, const char *name, double value);
+double sun_prop_get(const propT *prop, const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

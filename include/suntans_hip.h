@@ -1,0 +1,61 @@
+/*
+ * suntans_hip.h -- drop-in replacements for the SUNTANS elliptic solvers and
+ * TVD scalar transport, with the reference's own C signatures, running on one
+ * MI355X (numprocs=1, the whole grid on one GPU).  Link libsuntans.so in
+ * place of the reference's definitions; see INTEGRATION.md.
+ *
+ * Semantics follow the reference function each declaration replaces.  The
+ * caller owns every array; results are written to the same rows the
+ * reference writes (computational cells / edges).  Scratch rows the
+ * reference clobbers as a side effect (src, stmp2, stmp3, uold, wtmp, qtmp,
+ * a..d, hold, htmp*, hcoef, hfcoef) are left untouched: the reference reads
+ * none of them afterwards.  Unsupported configurations (numprocs>1,
+ * qprecond=1, UpdateScalars with src1/src2/Ftop/Fbot) print an error and
+ * exit, as the reference does on fatal conditions; nonconvergence is not an
+ * error (query it with sunhip_last_solve).
+ */
+#ifndef SUNTANS_HIP_H
+#define SUNTANS_HIP_H
+
+#include "sun_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* main/phys.c:2411 (static CGSolveQ; prototype main/phys.c:50-51) */
+void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *prop, int myproc,
+              int numprocs, MPI_Comm comm);
+/* main/phys.c:3086 (static CGSolve; prototype main/phys.c:45-46) */
+void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
+/* main/phys.c:3486 (static OperatorQ; prototype main/phys.c:71-72) */
+void OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *grid, physT *phys, propT *prop);
+/* main/phys.c:3577 (static Preconditioner; prototype main/phys.c:55-56) */
+void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, propT *prop);
+/* main/phys.c:3335 (static OperatorH; prototype main/phys.c:65-66) */
+void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *phys, propT *prop);
+/* main/tvd.h:27-28 (definition main/tvd.c:36) */
+void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal,
+                           int TVD, MPI_Comm comm, int myproc);
+/* main/scalars.h:19-22 (definition main/scalars.c:35) */
+void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **scal, REAL **boundary_scal,
+                   REAL **Cn, REAL kappa, REAL kappaH, REAL **kappa_tv, REAL theta, REAL **src1,
+                   REAL **src2, REAL *Ftop, REAL *Fbot, int alpha_top, int alpha_bot, MPI_Comm comm,
+                   int myproc, int checkflag, int TVDscheme);
+
+/* No counterpart in the reference: its solvers report convergence only on
+ * stdout (main/phys.c:2527-2536, :3204-3213).  which = 0 for the last
+ * CGSolveQ, 1 for the last CGSolve.  niters is the reference's loop counter
+ * n at exit; resid is sqrt(eps/eps0).  Returns 0, or -1 before any solve. */
+int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms);
+/* No counterpart in the reference: drop the cached device copies of all
+ * grids (call after freeing or rebuilding a gridT). */
+void sunhip_dropin_reset(void);
+/* No counterpart in the reference: select the HIP device used by the
+ * drop-ins (default 0). */
+void sunhip_dropin_set_mode(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

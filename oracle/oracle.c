@@ -1,0 +1,646 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the SUNTANS elliptic solves and TVD
+ * scalar transport.  TEST INFRASTRUCTURE ONLY (see oracle.h): this file is
+ * the checker for the HIP path and is never linked into the product.
+ *
+ * Every expression keeps the reference's evaluation order, so that with
+ * gcc -O2 on x86-64 the results are bit-identical to the reference.  Where
+ * the reference reads outside a row (documented per function) the oracle
+ * reads 0 instead; those cases are excluded from comparisons with the real
+ * reference.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define TVD_EPS 1e-12 /* main/tvd.c:21 */
+
+oracle_cg_stats oracle_last_q, oracle_last_h;
+
+static double fmax_ref(double x1, double x2) { return x2 > x1 ? x2 : x1; } /* main/util.c:212 */
+static double fmin_ref(double x, double y) { return x < y ? x : y; }       /* main/util.c:240 */
+static double upwind_ref(double u, double a, double b) {                    /* main/util.c:219-232 */
+  if (fabs(u) > 0) return u > 0 ? b : a;
+  return fmax_ref(a, b);
+}
+static int kmin_face(const gridT *g, int i, int nc) {
+  return g->ctop[nc] > g->ctop[i] ? g->ctop[nc] : g->ctop[i];
+}
+
+/* main/util.c:185-203: Thomas elimination without pivoting; b and d are
+ * overwritten by the forward sweep. */
+void oracle_TriSolve(REAL *a, REAL *b, REAL *c, REAL *d, REAL *u, int N) {
+  for (int k = 1; k < N; k++) {
+    b[k] -= a[k] * c[k - 1] / b[k - 1];
+    d[k] -= a[k] * d[k - 1] / b[k - 1];
+  }
+  u[N - 1] = d[N - 1] / b[N - 1];
+  for (int k = N - 2; k >= 0; k--) u[k] = d[k] / b[k] - c[k] * u[k + 1] / b[k];
+}
+
+/* main/tvd.c:130-151.  Case 4 is minmod (min(r,1) for r>0), whatever its
+ * comment says. */
+double oracle_Psi(double r, int TVD) {
+  if (TVD == 2) return 1;
+  if (TVD == 3) return fmax_ref(0, fmax_ref(fmin_ref(2 * (r), 1), fmin_ref(r, 2)));
+  if (TVD == 4) return r > 0 ? fmin_ref(r, 1) : 0;
+  return 0;
+}
+
+/* main/phys.c:3420-3471 */
+void oracle_QCoefficients(REAL **coef, REAL **fcoef, REAL **c, gridT *g, physT *p, propT *prop) {
+  const int mf = g->maxfaces;
+  if (prop->qprecond == 1) {
+    for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+      const int i = g->cellp[ip], kt = g->ctop[i];
+      coef[i][kt] = g->Ac[i] / g->dzz[i][kt] / c[i][kt];
+      for (int k = kt + 1; k < g->Nk[i]; k++)
+        coef[i][k] = 2 * g->Ac[i] / (g->dzz[i][k] + g->dzz[i][k - 1]) / (c[i][k] * c[i][k - 1]);
+      for (int nf = 0; nf < g->nfaces[i]; nf++) {
+        const int nc = g->neigh[i * mf + nf];
+        if (nc == -1) continue;
+        const int ne = g->face[i * mf + nf];
+        for (int k = kmin_face(g, i, nc); k < g->Nke[ne]; k++)
+          fcoef[i * mf + nf][k] = g->dzz[i][k] * p->D[ne] / (c[i][k] * c[nc][k]);
+      }
+    }
+    return;
+  }
+  /* unpreconditioned coefficients are built for every cell, not just cellp */
+  for (int i = 0; i < g->Nc; i++) {
+    const int kt = g->ctop[i];
+    coef[i][kt] = g->Ac[i] / g->dzz[i][kt];
+    for (int k = kt + 1; k < g->Nk[i]; k++) coef[i][k] = 2 * g->Ac[i] / (g->dzz[i][k] + g->dzz[i][k - 1]);
+  }
+}
+
+/* main/phys.c:3486-3536: y = L x, horizontal faces first (face order), then
+ * the vertical three-point part with q=0 above the top (ghost -x) and
+ * dq/dz=0 below the bottom. */
+void oracle_OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *g, physT *p, propT *prop) {
+  (void)c; (void)prop;
+  const int mf = g->maxfaces;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], kt = g->ctop[i], nk = g->Nk[i];
+    REAL *yi = y[i], *xi = x[i], *ci = coef[i];
+    for (int k = kt; k < nk; k++) yi[k] = 0;
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int nc = g->neigh[i * mf + nf];
+      if (nc == -1) continue;
+      const int ne = g->face[i * mf + nf];
+      for (int k = kmin_face(g, i, nc); k < g->Nke[ne]; k++)
+        yi[k] += (x[nc][k] - xi[k]) * g->dzf[ne][k] * p->D[ne];
+    }
+    for (int k = kt + 1; k < nk - 1; k++)
+      yi[k] += ci[k] * xi[k - 1] - (ci[k] + ci[k + 1]) * xi[k] + ci[k + 1] * xi[k + 1];
+    if (kt < nk - 1) {
+      yi[kt] += (-2 * ci[kt] - ci[kt + 1]) * xi[kt] + ci[kt + 1] * xi[kt + 1];
+      yi[nk - 1] += ci[nk - 1] * xi[nk - 2] - ci[nk - 1] * xi[nk - 1];
+    } else {
+      yi[kt] -= 2.0 * ci[kt] * xi[kt];
+    }
+  }
+}
+
+/* main/phys.c:3364-3409 (qprecond=1: symmetrically scaled operator with unit
+ * diagonal) */
+void oracle_OperatorQC(REAL **coef, REAL **fcoef, REAL **x, REAL **y, REAL **c, gridT *g, physT *p,
+                       propT *prop) {
+  (void)c; (void)p; (void)prop;
+  const int mf = g->maxfaces;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], kt = g->ctop[i], nk = g->Nk[i];
+    for (int k = kt; k < nk; k++) y[i][k] = -x[i][k];
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int nc = g->neigh[i * mf + nf];
+      if (nc == -1) continue;
+      const int ne = g->face[i * mf + nf];
+      for (int k = kmin_face(g, i, nc); k < g->Nke[ne]; k++) y[i][k] += x[nc][k] * fcoef[i * mf + nf][k];
+    }
+    for (int k = kt + 1; k < nk - 1; k++) y[i][k] += coef[i][k] * x[i][k - 1] + coef[i][k + 1] * x[i][k + 1];
+    if (kt < nk - 1) {
+      y[i][kt] += coef[i][kt + 1] * x[i][kt + 1];
+      y[i][nk - 1] += coef[i][nk - 1] * x[i][nk - 2];
+    }
+  }
+}
+
+/* main/phys.c:3629-3687: c = sqrt(|diag L|), 1 where the diagonal is not
+ * positive. */
+void oracle_ConditionQ(REAL **x, gridT *g, physT *p, propT *prop) {
+  (void)prop;
+  const int mf = g->maxfaces;
+  REAL *a = p->a;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], kt = g->ctop[i], nk = g->Nk[i];
+    for (int k = kt; k < nk; k++) x[i][k] = 0;
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int nc = g->neigh[i * mf + nf];
+      if (nc == -1) continue;
+      const int ne = g->face[i * mf + nf];
+      for (int k = kmin_face(g, i, nc); k < g->Nke[ne]; k++) x[i][k] += g->dzz[i][k] * p->D[ne];
+    }
+    a[kt] = g->Ac[i] / g->dzz[i][kt];
+    for (int k = kt + 1; k < nk; k++) a[k] = 2 * g->Ac[i] / (g->dzz[i][k] + g->dzz[i][k - 1]);
+    for (int k = kt + 1; k < nk - 1; k++) x[i][k] += (a[k] + a[k + 1]);
+    if (kt < nk - 1) {
+      x[i][kt] += 2 * a[kt] + a[kt + 1];
+      x[i][nk - 1] += a[nk - 1];
+    }
+  }
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) {
+      if (x[i][k] <= 0) x[i][k] = 1;
+      x[i][k] = sqrt(x[i][k]);
+    }
+  }
+}
+
+/* main/phys.c:3577-3619: per column, invert the vertical part of L with the
+ * Thomas algorithm (scratch rows phys->a..d). */
+void oracle_Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *g, physT *p, propT *prop) {
+  (void)prop;
+  REAL *a = p->a, *b = p->b, *c = p->c, *d = p->d;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], kt = g->ctop[i], nk = g->Nk[i];
+    const REAL *ci = coef[i];
+    if (kt >= nk - 1) {
+      xc[i][kt] = -0.5 * x[i][kt] / ci[kt];
+      continue;
+    }
+    for (int k = kt + 1; k < nk - 1; k++) {
+      a[k] = ci[k];
+      b[k] = -ci[k] - ci[k + 1];
+      c[k] = ci[k + 1];
+      d[k] = x[i][k];
+    }
+    b[kt] = -2 * ci[kt] - ci[kt + 1];
+    c[kt] = ci[kt + 1];
+    d[kt] = x[i][kt];
+    a[nk - 1] = ci[nk - 1];
+    b[nk - 1] = -ci[nk - 1];
+    d[nk - 1] = x[i][nk - 1];
+    oracle_TriSolve(a + kt, b + kt, c + kt, d + kt, xc[i] + kt, nk - kt);
+  }
+}
+
+/* main/phys.c:3304-3319 (numprocs=1: the reduce is a copy) */
+double oracle_InnerProduct3(REAL **x, REAL **y, gridT *g) {
+  double s = 0;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) s += x[i][k] * y[i][k];
+  }
+  return s;
+}
+
+/* main/phys.c:3280-3294 */
+double oracle_InnerProduct(REAL *x, REAL *y, gridT *g) {
+  double s = 0;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    s += x[i] * y[i];
+  }
+  return s;
+}
+
+#define FOR_CELLS_K(g, i, k)                                        \
+  for (int ip_ = (g)->celldist[0]; ip_ < (g)->celldist[1]; ip_++)   \
+    for (int i = (g)->cellp[ip_], k = (g)->ctop[i]; k < (g)->Nk[i]; k++)
+
+/* main/phys.c:2411-2550.  x aliases q (warm start), p aliases src, r is
+ * phys->stmp3, z phys->stmp2, rtmp phys->uold.  With qprecond=1 the caller
+ * in Solve passes c=phys->stmp3, the same rows as r, so the closing x/=c
+ * divides by the final residual; that aliasing is reproduced because the
+ * same rows are used. */
+void oracle_CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *g, physT *p, propT *prop) {
+  REAL **z = p->stmp2, **x = q, **r = p->stmp3, **rtmp = p->uold, **pp = src;
+  double mu, nu, alpha, alpha0, eps, eps0;
+  const int pc = prop->qprecond;
+  if (pc == 1) {
+    oracle_ConditionQ(c, g, p, prop);
+    FOR_CELLS_K(g, i, k) { pp[i][k] /= c[i][k]; x[i][k] *= c[i][k]; }
+  }
+  const int niters = prop->qmaxiters;
+  oracle_QCoefficients(p->wtmp, p->qtmp, c, g, p, prop);
+  if (pc == 1) oracle_OperatorQC(p->wtmp, p->qtmp, x, z, c, g, p, prop);
+  else oracle_OperatorQ(p->wtmp, x, z, c, g, p, prop);
+  FOR_CELLS_K(g, i, k) r[i][k] = pp[i][k] - z[i][k];
+  if (pc == 2) {
+    oracle_Preconditioner(r, rtmp, p->wtmp, g, p, prop);
+    FOR_CELLS_K(g, i, k) pp[i][k] = rtmp[i][k];
+    alpha = alpha0 = oracle_InnerProduct3(r, rtmp, g);
+  } else {
+    FOR_CELLS_K(g, i, k) pp[i][k] = r[i][k];
+    alpha = alpha0 = oracle_InnerProduct3(r, r, g);
+  }
+  if (!prop->resnorm) alpha0 = 1;
+  if (pc == 2) eps = eps0 = oracle_InnerProduct3(r, r, g);
+  else eps = eps0 = alpha0;
+
+  int n;
+  for (n = 0; n < niters && eps != 0; n++) {
+    if (pc == 1) oracle_OperatorQC(p->wtmp, p->qtmp, pp, z, c, g, p, prop);
+    else oracle_OperatorQ(p->wtmp, pp, z, c, g, p, prop);
+    mu = 1 / alpha;
+    nu = alpha / oracle_InnerProduct3(pp, z, g);
+    FOR_CELLS_K(g, i, k) { x[i][k] += nu * pp[i][k]; r[i][k] -= nu * z[i][k]; }
+    if (pc == 2) {
+      oracle_Preconditioner(r, rtmp, p->wtmp, g, p, prop);
+      alpha = oracle_InnerProduct3(r, rtmp, g);
+      mu *= alpha;
+      FOR_CELLS_K(g, i, k) pp[i][k] = rtmp[i][k] + mu * pp[i][k];
+    } else {
+      alpha = oracle_InnerProduct3(r, r, g);
+      mu *= alpha;
+      FOR_CELLS_K(g, i, k) pp[i][k] = r[i][k] + mu * pp[i][k];
+    }
+    eps = (pc == 2) ? oracle_InnerProduct3(r, r, g) : alpha;
+    if (sqrt(eps / eps0) < prop->qepsilon) break;
+  }
+  oracle_last_q.niters = n;
+  oracle_last_q.resid = sqrt(eps / eps0);
+  oracle_last_q.eps0 = eps0;
+  if (pc == 1) FOR_CELLS_K(g, i, k) x[i][k] /= c[i][k];
+}
+
+/* main/phys.c:3252-3269 */
+void oracle_HCoefficients(REAL *coef, REAL *fcoef, gridT *g, physT *p, propT *prop) {
+  const int mf = g->maxfaces;
+  const double tmp = prop->grav * pow(prop->theta * prop->dt, 2);
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    coef[i] = g->Ac[i];
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      if (g->neigh[i * mf + nf] == -1) continue;
+      const int ne = g->face[i * mf + nf];
+      fcoef[i * mf + nf] = tmp * p->D[ne] * g->df[ne] / g->dg[ne];
+      coef[i] += fcoef[i * mf + nf];
+    }
+  }
+}
+
+/* main/phys.c:3335-3349 */
+void oracle_OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *g, physT *p, propT *prop) {
+  (void)p; (void)prop;
+  const int mf = g->maxfaces;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    y[i] = coef[i] * x[i];
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int nb = g->neigh[i * mf + nf];
+      if (nb != -1) y[i] -= fcoef[i * mf + nf] * x[nb];
+    }
+  }
+}
+
+/* main/phys.c:3227-3235 */
+void oracle_HPreconditioner(REAL *x, REAL *y, gridT *g, physT *p, propT *prop) {
+  (void)prop;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    y[i] = x[i] / p->hcoef[i];
+  }
+}
+
+#define FOR_CELLS(g, i, lo, hi) \
+  for (int ip_ = (g)->celldist[lo], i; ip_ < (g)->celldist[hi] && ((i = (g)->cellp[ip_]), 1); ip_++)
+
+/* main/phys.c:3086-3217.  x=h (zeroed on computational cells, boundary
+ * values lifted into the RHS), r=hold, rtmp=htmp2, z=htmp3, p=htmp. */
+void oracle_CGSolve(gridT *g, physT *p, propT *prop) {
+  REAL *x = p->h, *r = p->hold, *rtmp = p->htmp2, *z = p->htmp3, *pp = p->htmp;
+  double mu, nu, eps, eps0, alpha, alpha0;
+  const int niters = prop->maxiters;
+  oracle_HCoefficients(p->hcoef, p->hfcoef, g, p, prop);
+  FOR_CELLS(g, i, 0, 1) x[i] = 0;
+  oracle_OperatorH(x, z, p->hcoef, p->hfcoef, g, p, prop);
+  FOR_CELLS(g, i, 0, 1) { pp[i] = pp[i] - z[i]; r[i] = pp[i]; x[i] = 0; }
+  FOR_CELLS(g, i, 1, 2) pp[i] = 0;
+  if (prop->hprecond == 1) {
+    oracle_HPreconditioner(r, rtmp, g, p, prop);
+    FOR_CELLS(g, i, 0, 1) pp[i] = rtmp[i];
+    alpha = alpha0 = oracle_InnerProduct(r, rtmp, g);
+  } else {
+    FOR_CELLS(g, i, 0, 1) pp[i] = r[i];
+    alpha = alpha0 = oracle_InnerProduct(r, r, g);
+  }
+  if (!prop->resnorm) alpha0 = 1;
+  if (prop->hprecond == 1) eps = eps0 = oracle_InnerProduct(r, r, g);
+  else eps = eps0 = alpha0;
+  int n;
+  for (n = 0; n < niters && eps != 0 && alpha != 0; n++) {
+    oracle_OperatorH(pp, z, p->hcoef, p->hfcoef, g, p, prop);
+    mu = 1 / alpha;
+    nu = alpha / oracle_InnerProduct(pp, z, g);
+    FOR_CELLS(g, i, 0, 1) { x[i] += nu * pp[i]; r[i] -= nu * z[i]; }
+    if (prop->hprecond == 1) {
+      oracle_HPreconditioner(r, rtmp, g, p, prop);
+      alpha = oracle_InnerProduct(r, rtmp, g);
+      mu *= alpha;
+      FOR_CELLS(g, i, 0, 1) pp[i] = rtmp[i] + mu * pp[i];
+    } else {
+      alpha = oracle_InnerProduct(r, r, g);
+      mu *= alpha;
+      FOR_CELLS(g, i, 0, 1) pp[i] = r[i] + mu * pp[i];
+    }
+    eps = (prop->hprecond == 1) ? oracle_InnerProduct(r, r, g) : alpha;
+    if (sqrt(eps / eps0) < prop->epsilon) break;
+  }
+  oracle_last_h.niters = n;
+  oracle_last_h.resid = sqrt(eps / eps0);
+  oracle_last_h.eps0 = eps0;
+}
+
+/* main/phys.c:2345-2395 */
+void oracle_ComputeQSource(REAL **src, gridT *g, physT *p, propT *prop) {
+  const double tf = (1 - prop->theta) / prop->theta;
+  const int mf = g->maxfaces;
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) src[i][k] = 0;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++)
+      src[i][k] = g->Ac[i] * (p->w[i][k] - p->w[i][k + 1]) + tf * g->Ac[i] * (p->wtmp2[i][k] - p->wtmp2[i][k + 1]);
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int ne = g->face[i * mf + nf];
+      for (int k = g->ctop[i]; k < g->Nke[ne]; k++)
+        src[i][k] += (p->u[ne][k] + tf * p->utmp2[ne][k]) * g->dzf[ne][k] * g->normal[i * mf + nf] * g->df[ne];
+    }
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) src[i][k] /= prop->dt;
+  }
+  for (int j = 0; j < g->Ne; j++) p->D[j] = g->df[j] / g->dg[j];
+}
+
+/* main/tvd.c:36-120.  Pass 1 builds, per cell-layer, the inflow volume flux
+ * sumQ and the inflow-weighted scalar jump sumQC (in gradSx/gradSy); pass 2
+ * limits the face values of each wet interior edge-layer by Psi of the
+ * upwind cell's ratio; type-2 edges take the boundary value.  The reference
+ * reads scal[neigh][k] below the neighbour's bottom (multiplied by zero);
+ * the oracle skips that term. */
+void oracle_HorizontalFaceScalars(gridT *g, physT *p, propT *prop, REAL **scal, REAL **boundary_scal,
+                                  int TVD) {
+  REAL **sumQ = p->gradSx, **sumQC = p->gradSy;
+  const int mf = g->maxfaces;
+  const double th = prop->theta;
+  for (int ip = g->celldist[0]; ip < g->celldist[2]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) {
+      sumQ[i][k] = 0;
+      sumQC[i][k] = 0;
+      for (int f = 0; f < g->nfaces[i]; f++) {
+        const int ne = g->face[i * mf + f], nb = g->neigh[i * mf + f], nrm = g->normal[i * mf + f];
+        const double un = nrm * (th * p->u[ne][k] + (1 - th) * p->utmp2[ne][k]);
+        const double Qm = 0.5 * g->dzf[ne][k] * g->df[ne] * fabs(un - fabs(un));
+        if (nb != -1 && k < g->Nk[nb]) sumQC[i][k] += Qm * (scal[i][k] - scal[nb][k]);
+        sumQ[i][k] += Qm;
+      }
+    }
+  }
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp], nc1 = g->grad[2 * j], nc2 = g->grad[2 * j + 1];
+    for (int k = 0; k < g->etop[j]; k++) p->SfHp[j][k] = p->SfHm[j][k] = 0;
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) {
+      const double un = th * p->u[j][k] + (1 - th) * p->utmp2[j][k];
+      const int up = un > 0 ? nc2 : nc1;
+      const double si = un > 0 ? scal[nc2][k] : scal[nc1][k];
+      const double sm = un > 0 ? scal[nc1][k] : scal[nc2][k];
+      double r = 0;
+      if (sumQ[up][k] != 0 && sm != si) r = sumQC[up][k] / (sumQ[up][k] * (sm - si));
+      p->SfHp[j][k] = scal[nc2][k] + 0.5 * oracle_Psi(r, TVD) * (scal[nc1][k] - scal[nc2][k]);
+      p->SfHm[j][k] = scal[nc1][k] - 0.5 * oracle_Psi(r, TVD) * (scal[nc1][k] - scal[nc2][k]);
+    }
+  }
+  for (int jp = g->edgedist[2]; jp < g->edgedist[3]; jp++) {
+    const int j = g->edgep[jp], ib = g->grad[2 * j];
+    for (int k = 0; k < g->etop[j]; k++) p->SfHp[j][k] = p->SfHm[j][k] = 0;
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) {
+      p->SfHp[j][k] = boundary_scal[jp - g->edgedist[2]][k];
+      p->SfHm[j][k] = scal[ib][k];
+    }
+  }
+}
+
+/* Column read with the out-of-row cases of GetApAm mapped to 0. */
+static double col(REAL **s, int i, int k, int nk) { return (k >= 0 && k < nk) ? s[i][k] : 0.0; }
+
+/* main/tvd.c:161-204: vertical TVD flux coefficients for one column.  For
+ * columns with fewer than three wet layers the reference reads outside the
+ * row; the oracle reads 0 there. */
+void oracle_GetApAm(REAL *ap, REAL *am, REAL *wp, REAL *wm, REAL *Cp, REAL *Cm, REAL *rp, REAL *rm,
+                    REAL **w, REAL **dzz, REAL **scal, int i, int Nk, int ktop, REAL dt, int TVD) {
+  for (int k = 0; k < Nk + 1; k++) {
+    wp[k] = 0.5 * (w[i][k] + fabs(w[i][k]));
+    wm[k] = 0.5 * (w[i][k] - fabs(w[i][k]));
+  }
+  for (int k = 1; k < Nk; k++) {
+    Cp[k] = 2 * wp[k] * dt / (dzz[i][k] + dzz[i][k - 1]);
+    Cm[k] = 2 * wm[k] * dt / (dzz[i][k] + dzz[i][k - 1]);
+  }
+  Cp[Nk] = wp[Nk] * dt / dzz[i][Nk - 1];
+  Cm[Nk] = wm[Nk] * dt / dzz[i][Nk - 1];
+#define S(k) col(scal, i, (k), Nk)
+  for (int k = ktop + 2; k < Nk - 1; k++) {
+    rp[k - ktop] = (S(k) - S(k + 1) + TVD_EPS) / (S(k - 1) - S(k) + TVD_EPS);
+    rm[k - ktop] = (S(k - 2) - S(k - 1) + TVD_EPS) / (S(k - 1) - S(k) + TVD_EPS);
+  }
+  rp[1] = (S(ktop + 1) - S(ktop + 2) + TVD_EPS) / (S(ktop) - S(ktop + 1) + TVD_EPS);
+  rm[1] = TVD_EPS / (S(ktop) - S(ktop + 1) + TVD_EPS);
+  {
+    const int k = Nk - 1;
+    rp[k - ktop] = TVD_EPS / (S(k - 1) - S(k) + TVD_EPS);
+    rm[k - ktop] = (S(k - 2) - S(k - 1) + TVD_EPS) / (S(k - 1) - S(k) + TVD_EPS);
+  }
+  rp[Nk - ktop] = 1;
+  rm[Nk - ktop] = (S(Nk - 2) - S(Nk - 1) + TVD_EPS) / TVD_EPS;
+#undef S
+  for (int k = ktop + 1; k < Nk + 1; k++) {
+    am[k] = 0.5 * wp[k] * oracle_Psi(rp[k - ktop], TVD) * (1 - Cp[k]) +
+            wm[k] * (1 - 0.5 * oracle_Psi(rm[k - ktop], TVD) * (1 + Cm[k]));
+    ap[k] = wp[k] * (1 - 0.5 * oracle_Psi(rp[k - ktop], TVD) * (1 - Cp[k])) +
+            0.5 * wm[k] * oracle_Psi(rm[k - ktop], TVD) * (1 + Cm[k]);
+  }
+}
+
+/* main/scalars.c:35-325 with CHECKCONSISTENCY=0 (main/suntans.h:30).
+ * Adams-Bashforth is disabled in the reference (fab=1, Cn zeroed first). */
+void oracle_UpdateScalars(gridT *g, physT *p, propT *prop, REAL **wnew, REAL **scal,
+                          REAL **boundary_scal, REAL **Cn, REAL kappa, REAL kappaH, REAL **kappa_tv,
+                          REAL theta, REAL **src1, REAL **src2, REAL *Ftop, REAL *Fbot, int alpha_top,
+                          int alpha_bot, int checkflag, int TVDscheme) {
+  (void)kappaH; (void)checkflag;
+  const int mf = g->maxfaces;
+  const double dt = prop->dt, fab = 1;
+  REAL *ap = p->ap, *am = p->am, *bd = p->bp, *temp = p->bm;
+  REAL *a = p->a, *b = p->b, *c = p->c, *d = p->d;
+  prop->TVD = TVDscheme;
+  prop->horiTVD = 1;
+  prop->vertTVD = 1;
+  const int tvd_h = prop->TVD && prop->horiTVD, tvd_v = prop->TVD && prop->vertTVD;
+
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = 0; k < g->Nk[i]; k++) Cn[i][k] = 0;
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = 0; k < g->Nk[i]; k++) p->stmp[i][k] = scal[i][k];
+  FOR_CELLS(g, i, 0, 1) for (int k = g->ctop[i]; k < g->Nk[i]; k++) p->stmp2[i][k] = 0;
+  if (boundary_scal)
+    for (int jp = g->edgedist[2]; jp < g->edgedist[5]; jp++) {
+      const int j = g->edgep[jp], ib = g->grad[2 * j];
+      for (int k = g->ctop[ib]; k < g->Nk[ib]; k++) p->stmp2[ib][k] = boundary_scal[jp - g->edgedist[2]][k];
+    }
+  if (tvd_h) oracle_HorizontalFaceScalars(g, p, prop, scal, boundary_scal, prop->TVD);
+
+  FOR_CELLS(g, i, 0, 1) {
+    const double Ac = g->Ac[i];
+    const int nk = g->Nk[i], ct = g->ctop[i], cto = g->ctopold[i];
+    int ktop;
+    double dznew;
+    if (ct >= cto) {
+      ktop = ct;
+      dznew = g->dzz[i][ktop];
+    } else {
+      ktop = cto;
+      dznew = 0;
+      for (int k = ct; k <= cto; k++) dznew += g->dzz[i][k];
+    }
+    const int nb = nk - 1 - ktop; /* row of the bottom cell */
+
+    /* implicit (new time level) vertical advection */
+    if (!tvd_v)
+      for (int k = 0; k < nk + 1; k++) {
+        ap[k] = 0.5 * (wnew[i][k] + fabs(wnew[i][k]));
+        am[k] = 0.5 * (wnew[i][k] - fabs(wnew[i][k]));
+      }
+    else
+      oracle_GetApAm(ap, am, p->wp, p->wm, p->Cp, p->Cm, p->rp, p->rm, wnew, g->dzz, scal, i, nk, ktop, dt,
+                     prop->TVD);
+    for (int k = ktop + 1; k < nk; k++) {
+      a[k - ktop] = theta * dt * am[k];
+      b[k - ktop] = g->dzz[i][k] + theta * dt * (ap[k] - am[k + 1]);
+      c[k - ktop] = -theta * dt * ap[k + 1];
+    }
+    a[0] = 0;
+    b[0] = dznew - theta * dt * am[ktop + 1];
+    c[0] = -theta * dt * ap[ktop + 1];
+    b[nb] += c[nb];
+
+    /* implicit vertical diffusion */
+    for (int k = ktop + 1; k < nk; k++)
+      bd[k] = (2.0 * kappa + kappa_tv[i][k - 1] + kappa_tv[i][k]) / (g->dzz[i][k - 1] + g->dzz[i][k]);
+    for (int k = ktop + 1; k < nk - 1; k++) {
+      a[k - ktop] -= theta * dt * bd[k];
+      b[k - ktop] += theta * dt * (bd[k] + bd[k + 1]);
+      c[k - ktop] -= theta * dt * bd[k + 1];
+    }
+    if (src1)
+      for (int k = ktop; k < nk; k++) b[k - ktop] += g->dzz[i][k] * src1[i][k] * theta * dt;
+    if (ktop < nk - 1) {
+      b[0] += theta * dt * (bd[ktop + 1] + 2 * alpha_top * bd[ktop + 1]);
+      c[0] -= theta * dt * bd[ktop + 1];
+      a[nb] -= theta * dt * bd[nk - 1];
+      b[nb] += theta * dt * (bd[nk - 1] + 2 * alpha_bot * bd[nk - 1]);
+    }
+
+    /* explicit (old time level) part */
+    for (int k = ktop + 1; k < nk; k++) d[k - ktop] = g->dzzold[i][k] * p->stmp[i][k];
+    if (src1)
+      for (int k = ktop + 1; k < nk; k++)
+        d[k - ktop] -= src1[i][k] * (1 - theta) * dt * g->dzzold[i][k] * p->stmp[i][k];
+    d[0] = 0;
+    if (cto <= ct) {
+      for (int k = cto; k <= ct; k++) d[0] += g->dzzold[i][k] * p->stmp[i][k];
+      if (src1)
+        for (int k = cto; k <= ct; k++) d[0] -= src1[i][k] * (1 - theta) * dt * g->dzzold[i][k] * p->stmp[i][k];
+    } else {
+      d[0] = g->dzzold[i][ktop] * p->stmp[i][ktop];
+      /* the reference multiplies by stmp at a stale loop index here; the
+       * oracle uses the top cell */
+      if (src1) d[0] -= src1[i][ktop] * (1 - theta) * dt * g->dzzold[i][ktop] * p->stmp[i][ktop];
+    }
+    if (!tvd_v)
+      for (int k = 0; k < nk + 1; k++) {
+        ap[k] = 0.5 * (p->wtmp2[i][k] + fabs(p->wtmp2[i][k]));
+        am[k] = 0.5 * (p->wtmp2[i][k] - fabs(p->wtmp2[i][k]));
+      }
+    else
+      oracle_GetApAm(ap, am, p->wp, p->wm, p->Cp, p->Cm, p->rp, p->rm, p->wtmp2, g->dzzold, p->stmp, i, nk,
+                     ktop, dt, prop->TVD);
+    REAL *s = p->stmp[i];
+    for (int k = ktop + 1; k < nk - 1; k++)
+      d[k - ktop] -= (1 - theta) * dt * (am[k] * s[k - 1] + (ap[k] - am[k + 1]) * s[k] - ap[k + 1] * s[k + 1]) -
+                     (1 - theta) * dt * (bd[k] * s[k - 1] - (bd[k] + bd[k + 1]) * s[k] + bd[k + 1] * s[k + 1]);
+    if (ktop < nk - 1) {
+      int k = ktop;
+      d[0] = d[0] - (1 - theta) * dt * (-am[k + 1] * s[k] - ap[k + 1] * s[k + 1]) +
+             (1 - theta) * dt * (-(2 * alpha_top * bd[k + 1] + bd[k + 1]) * s[k] + bd[k + 1] * s[k + 1]);
+      if (Ftop) d[0] += dt * (1 - alpha_top + 2 * alpha_top * bd[k + 1]) * Ftop[i];
+      k = nk - 1;
+      d[k - ktop] -= (1 - theta) * dt * (am[k] * s[k - 1] + ap[k] * s[k]) -
+                     (1 - theta) * dt * (bd[k] * s[k - 1] - (bd[k] + 2 * alpha_bot * bd[k]) * s[k]);
+      if (Fbot) d[k - ktop] += dt * (-1 + alpha_bot + 2 * alpha_bot * bd[k]) * Fbot[i];
+    }
+
+    /* previous-step AB term: weight 1-fab = 0 */
+    if (ct <= cto) {
+      for (int k = ct; k <= cto; k++) d[0] += (1 - fab) * Cn[i][cto] / (1 + abs(ct - cto));
+      for (int k = cto + 1; k < nk; k++) d[k - cto] += (1 - fab) * Cn[i][k];
+    } else {
+      for (int k = cto; k <= ct; k++) d[0] += (1 - fab) * Cn[i][k];
+      for (int k = ct + 1; k < nk; k++) d[k - ct] += (1 - fab) * Cn[i][k];
+    }
+    for (int k = 0; k < ct; k++) Cn[i][k] = 0;
+    if (src2)
+      for (int k = ct; k < nk; k++) Cn[i][k - ktop] = dt * src2[i][k] * g->dzzold[i][k];
+    else
+      for (int k = ct; k < nk; k++) Cn[i][k] = 0;
+
+    /* horizontal flux divergence with the TVD face values */
+    for (int k = 0; k < nk; k++) ap[k] = 0;
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int ne = g->face[i * mf + nf], nrm = g->normal[i * mf + nf];
+      const double df = g->df[ne];
+      int nc1 = g->grad[2 * ne], nc2 = g->grad[2 * ne + 1];
+      REAL *sp;
+      if (nc1 == -1) nc1 = nc2;
+      if (nc2 == -1) {
+        nc2 = nc1;
+        sp = (boundary_scal && (g->mark[ne] == 2 || g->mark[ne] == 3)) ? p->stmp2[nc1] : p->stmp[nc1];
+      } else {
+        sp = p->stmp[nc2];
+      }
+      if (!tvd_h)
+        for (int k = 0; k < g->Nke[ne]; k++) temp[k] = upwind_ref(p->utmp2[ne][k], p->stmp[nc1][k], sp[k]);
+      else
+        for (int k = 0; k < g->Nke[ne]; k++) temp[k] = p->utmp2[ne][k] > 0 ? p->SfHp[ne][k] : p->SfHm[ne][k];
+      for (int k = 0; k < g->Nke[ne]; k++)
+        ap[k] += dt * df * nrm / Ac * (theta * p->u[ne][k] + (1 - theta) * p->utmp2[ne][k]) * temp[k] *
+                 g->dzf[ne][k];
+    }
+    for (int k = ktop + 1; k < nk; k++) Cn[i][k - ktop] -= ap[k];
+    for (int k = 0; k <= ktop; k++) Cn[i][0] -= ap[k];
+    for (int k = 0; k < nk - ktop; k++) d[k] += fab * Cn[i][k];
+    for (int k = ktop; k < nk; k++) ap[k] = Cn[i][k - ktop];
+    for (int k = 0; k <= ktop; k++) Cn[i][k] = 0;
+    for (int k = ktop + 1; k < nk; k++) Cn[i][k] = ap[k];
+    for (int k = ct; k <= ktop; k++) Cn[i][k] = ap[ktop] / (1 + abs(ct - ktop));
+
+    if (nk - ktop > 1) {
+      oracle_TriSolve(a, b, c, d, scal[i] + ktop, nk - ktop);
+    } else if (prop->n > 1) {
+      scal[i][ktop] = (b[0] > 0 && p->active[i]) ? d[0] / b[0] : 0;
+    }
+    for (int k = 0; k < ct; k++) scal[i][k] = 0;
+    for (int k = ct; k < cto; k++) scal[i][k] = scal[i][ktop];
+  }
+}
+
+void oracle_solve_q(gridT *g, physT *p, propT *prop) {
+  oracle_CGSolveQ(p->qc, p->stmp, p->stmp3, g, p, prop);
+}
+void oracle_solve_h(gridT *g, physT *p, propT *prop) { oracle_CGSolve(g, p, prop); }
+void oracle_update_salinity(gridT *g, physT *p, propT *prop) {
+  oracle_UpdateScalars(g, p, prop, p->wnew, p->s, p->boundary_s, p->Cn_R, prop->kappa_s, prop->kappa_sH,
+                       p->kappa_tv, prop->theta, NULL, NULL, NULL, NULL, 0, 0, 1, prop->TVDsalt);
+}

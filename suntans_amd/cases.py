@@ -1,0 +1,204 @@
+"""Seeded synthetic workloads for the elliptic solves and scalar transport.
+
+* :func:`tiny`          small ragged grids with moving free surface (parity)
+* :func:`lock_exchange` 10 km x 1 km channel, 100x10 quads as 2000 triangles,
+                        50 uniform levels over 20 m, salinity step at x=5 km
+* :func:`large_solve`   100 km x 50 km, 1000x500 quads as 1.0M triangles,
+                        depth 10-200 m from 8 Gaussian bumps, 100 levels of
+                        2 m (~50M pressure unknowns)
+* :func:`ensemble_member` one member of the 64-member ensemble
+
+Every random number comes from ``numpy.random.default_rng(seed)``.  Velocity
+fields are sums of a few low-wavenumber modes, zero on closed boundary edges
+and outside each edge's wet layers; vertical velocity follows continuity
+where a step needs a divergence-free field.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .state import State
+
+
+def _modes(rng, n, amp):
+    return [(rng.normal(0, amp), rng.uniform(0.5, 3.0), rng.uniform(0.5, 3.0), rng.uniform(0.2, 1.5),
+             rng.uniform(0, 2 * np.pi)) for _ in range(n)]
+
+
+def smooth(rng, x, y, z, Lx, Ly, H, amp=1.0, n=5):
+    """Sum of n random low-wavenumber modes at (x, y, z)."""
+    out = np.zeros(np.broadcast(x, y, z).shape)
+    for a, kx, ky, kz, ph in _modes(rng, n, amp):
+        out += a * np.sin(kx * np.pi * x / Lx + ph) * np.cos(ky * np.pi * y / Ly + 0.5 * ph) * \
+            np.cos(kz * np.pi * z / H + 0.3 * ph)
+    return out
+
+
+def bumps_depth(rng, Lx, Ly, dmin, dmax, nb=8, mean=None):
+    """Depth function: clip(base + sum of nb Gaussian bumps, dmin, dmax)."""
+    cx = rng.uniform(0, Lx, nb)
+    cy = rng.uniform(0, Ly, nb)
+    sx = rng.uniform(0.08, 0.25, nb) * Lx
+    sy = rng.uniform(0.08, 0.25, nb) * Ly
+    amp = rng.uniform(-1.0, 1.0, nb)
+
+    def raw(x, y):
+        v = np.zeros_like(x)
+        for i in range(nb):
+            v += amp[i] * np.exp(-((x - cx[i]) / sx[i]) ** 2 - ((y - cy[i]) / sy[i]) ** 2)
+        return v
+
+    def depth(x, y):
+        r = raw(x, y)
+        r = (r - r.min()) / max(r.max() - r.min(), 1e-30)
+        target = 0.5 if mean is None else (mean - dmin) / (dmax - dmin)
+        # power law that puts the mean of the normalised field at target
+        lo, hi = 0.05, 20.0
+        for _ in range(60):
+            g = np.sqrt(lo * hi)
+            if np.mean(r ** g) > target:
+                lo = g
+            else:
+                hi = g
+        d = dmin + (dmax - dmin) * r ** np.sqrt(lo * hi)
+        d = np.clip(d, dmin, dmax)
+        d[np.argmax(d)] = dmax  # the deepest column uses every level
+        return d
+
+    return depth
+
+
+def fill_velocities(st: State, rng, amp_u=0.1, amp_w=1e-3, zero_w_bottom=True):
+    """u, utmp2 on wet edge layers (zero on closed boundaries), w, wtmp2, wnew."""
+    Lx, Ly = st.Lx, st.Ly
+    H = float(np.max(st.get("dv"))) + 1e-9
+    edge, k = st.edge_layers()
+    xe, ye = st.get("xe"), st.get("ye")
+    dz = st.get("dz")
+    zc = np.concatenate([[0.0], np.cumsum(dz)])
+    zmid = 0.5 * (zc[:-1] + zc[1:])
+    mark = st.ints("mark")
+    wet = st.edge_wet_mask() & (mark[edge] == 0)
+    for name in ("u", "utmp2"):
+        v = smooth(rng, xe[edge], ye[edge], zmid[np.minimum(k, len(zmid) - 1)], Lx, Ly, H, amp_u)
+        st.set(name, np.where(wet, v, 0.0))
+    cell, kw = st.cell_layers(extra=1)
+    xv, yv = st.get("xv"), st.get("yv")
+    nk = st.ints("Nk")
+    for name in ("w", "wtmp2", "wnew"):
+        v = smooth(rng, xv[cell], yv[cell], zc[np.minimum(kw, len(zc) - 1)], Lx, Ly, H, amp_w)
+        if zero_w_bottom:
+            v = np.where(kw == nk[cell], 0.0, v)
+        st.set(name, v)
+
+
+def fill_scalar(st: State, rng, kind="random", noise=1e-3):
+    cell, k = st.cell_layers()
+    xv = st.get("xv")
+    if kind == "step":
+        s = (xv[cell] >= 0.5 * st.Lx).astype(float) + rng.normal(0, noise, len(cell))
+    else:
+        s = rng.uniform(0, 1, len(cell))
+    st.set("s", s)
+    return s
+
+
+def _finish(st: State, rng, dt, warm_q=True, h_amp=0.0):
+    st.set_prop(dt=dt)
+    st.set_flux_height()
+    # pressure source from the predicted velocities (ComputeQSource)
+    st.compute_qsource()
+    if warm_q:
+        cell, k = st.cell_layers()
+        st.set("qc", rng.normal(0, 1e-3, len(cell)) * st.wet_mask())
+    # free-surface right-hand side and coefficient D = sum_k dzf (inviscid)
+    Ac, xv, yv = st.get("Ac"), st.get("xv"), st.get("yv")
+    st.set("htmp", Ac * (st.get("h") + smooth(rng, xv, yv, 0.0, st.Lx, st.Ly, 1.0, 1e-2)))
+    st.set("h", rng.normal(0, 1e-3, st.Nc))
+    st.set("kappa_tv", 0.0)
+
+
+def depth_integrated_D(st: State) -> np.ndarray:
+    edge, k = st.edge_layers()
+    dzf = st.get("dzf")
+    return np.bincount(edge, weights=dzf * st.edge_wet_mask(), minlength=st.Ne)
+
+
+def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3) -> State:
+    """Small grid with ragged columns, dry top layers and a moved surface."""
+    rng = np.random.default_rng(seed)
+    dz = np.full(Nkmax, 1.0)
+    H = Nkmax * 1.0
+
+    def depth(x, y):
+        d = rng.uniform(min_layers + 2.1, H, x.shape)
+        d[rng.integers(len(d))] = H
+        return d
+
+    st = State(nx, ny, 400.0 * nx, 300.0 * ny, Nkmax, dz, depth)
+    st.update_dz(-1)
+    h1 = rng.uniform(-1.6, 0.3, st.Nc) if move_surface else np.zeros(st.Nc)
+    st.set("h", h1)
+    st.update_dz(1)
+    if move_surface:
+        st.set("h", h1 + rng.uniform(-1.0, 1.0, st.Nc))
+        st.update_dz(0)
+    fill_velocities(st, rng)
+    fill_scalar(st, rng)
+    _finish(st, rng, dt)
+    return st
+
+
+def lock_exchange(seed=0, nx=100, ny=10, Nkmax=50, depth=20.0, dt=5.0) -> State:
+    rng = np.random.default_rng(seed)
+    st = State(nx, ny, 10e3, 1e3, Nkmax, np.full(Nkmax, depth / Nkmax), depth)
+    st.update_dz(-1)
+    st.update_dz(1)
+    fill_velocities(st, rng, amp_u=0.05, amp_w=5e-4)
+    fill_scalar(st, rng, kind="step")
+    _finish(st, rng, dt)
+    st.set_prop(beta=1.0, TVDsalt=4)
+    return st
+
+
+def large_solve(seed=0, nx=1000, ny=500, Nkmax=100, dz=2.0, mean_depth=None, dt=10.0,
+                target_N=5.0e7) -> State:
+    """The ~50M-unknown solve: depth tuned so that sum(Nk) is close to target_N."""
+    rng = np.random.default_rng(seed)
+    Nc = 2 * nx * ny
+    mean = mean_depth if mean_depth is not None else target_N / Nc * dz - 0.5 * dz
+    depth = bumps_depth(rng, 100e3 * nx / 1000, 50e3 * ny / 500, 10.0, Nkmax * dz, mean=mean)
+    st = State(nx, ny, 100e3 * nx / 1000, 50e3 * ny / 500, Nkmax, np.full(Nkmax, dz), depth)
+    st.update_dz(-1)
+    st.update_dz(1)
+    fill_velocities(st, rng, amp_u=0.1, amp_w=1e-3)
+    st.set("utmp2", 0.0)
+    st.set("wtmp2", 0.0)
+    fill_scalar(st, rng)
+    st.set_prop(dt=dt)
+    st.set_flux_height()
+    st.compute_qsource()
+    st.set("qc", 0.0)
+    Ac, xv, yv = st.get("Ac"), st.get("xv"), st.get("yv")
+    st.set("htmp", Ac * smooth(rng, xv, yv, 0.0, st.Lx, st.Ly, 1.0, 1e-2))
+    st.set("h", 0.0)
+    return st
+
+
+def ensemble_member(seed: int, Nc_target: int, aspect: float, Nkmax: int, dt=10.0) -> State:
+    """One member: a triangulated rectangle of ~Nc_target cells, Gaussian-bump
+    bathymetry with Nkmax levels of 2 m."""
+    rng = np.random.default_rng(seed)
+    nquad = max(Nc_target // 2, 4)
+    ny = max(int(round(np.sqrt(nquad / aspect))), 2)
+    nx = max(int(round(nquad / ny)), 2)
+    dz = 2.0
+    Lx, Ly = 100.0 * nx, 100.0 * ny
+    depth = bumps_depth(rng, Lx, Ly, 3.2 * dz, Nkmax * dz, mean=0.55 * Nkmax * dz)
+    st = State(nx, ny, Lx, Ly, Nkmax, np.full(Nkmax, dz), depth)
+    st.update_dz(-1)
+    st.update_dz(1)
+    fill_velocities(st, rng, amp_u=0.1, amp_w=1e-3)
+    fill_scalar(st, rng)
+    _finish(st, rng, dt, warm_q=False)
+    return st

@@ -1,0 +1,315 @@
+/*
+ * dropin.c -- the reference-signature entry points (include/suntans_hip.h)
+ * over the thin device ABI (include/sunhip.h).  Plain C.
+ *
+ * Each call packs the jagged REAL** rows it needs into one flat buffer, hands
+ * them to the device, runs the kernels and writes back the rows the
+ * reference writes.  The device copy of a grid's topology is built on the
+ * first call with that gridT and reused; the moving vertical extents
+ * (ctop, ctopold, Nke, etop) and all field data are refreshed on every call,
+ * since the caller may have changed them between calls.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sunhip.h"
+#include "suntans_hip.h"
+
+#define MAXCTX 16
+
+typedef struct {
+  const gridT *key;
+  int Nc, Ne;
+  sunhip_grid *dev;
+  double *flat;      /* host packing buffer */
+  size_t flat_n;
+} ctx_t;
+
+static ctx_t g_ctx[MAXCTX];
+static int g_nctx = 0;
+static int g_device = 0;
+static struct { int valid; sunhip_cg_stats st; } g_last[2];
+
+static void fatal(const char *where, const char *msg) {
+  fprintf(stderr, "suntans_hip: %s: %s\n", where, msg);
+  exit(EXIT_FAILURE);
+}
+
+static void check(int rc, const char *where) {
+  if (rc) fatal(where, sunhip_error());
+}
+
+static ctx_t *get_ctx(gridT *g, const char *where) {
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i].key == g && g_ctx[i].Nc == g->Nc && g_ctx[i].Ne == g->Ne) return &g_ctx[i];
+  if (g_nctx == MAXCTX) fatal(where, "too many grids; call sunhip_dropin_reset()");
+  ctx_t *c = &g_ctx[g_nctx];
+  sunhip_topology T;
+  T.Nc = g->Nc; T.Ne = g->Ne; T.maxfaces = g->maxfaces; T.Nkmax = g->Nkmax;
+  T.nfaces = g->nfaces; T.neigh = g->neigh; T.face = g->face; T.normal = g->normal;
+  T.grad = g->grad; T.mark = g->mark; T.Nk = g->Nk; T.Nkc = g->Nkc;
+  T.cellp = g->cellp; T.celldist = g->celldist; T.edgep = g->edgep; T.edgedist = g->edgedist;
+  T.Ac = g->Ac; T.df = g->df; T.dg = g->dg; T.xv = g->xv; T.yv = g->yv;
+  check(sunhip_grid_create(&c->dev, &T, g_device), where);
+  int64_t sz[6];
+  sunhip_grid_sizes(c->dev, sz);
+  size_t n = (size_t)sz[1];
+  if ((size_t)sz[2] > n) n = (size_t)sz[2];
+  if ((size_t)g->maxfaces * g->Nc > n) n = (size_t)g->maxfaces * g->Nc;
+  c->flat = (double *)malloc(n * sizeof(double));
+  if (!c->flat) fatal(where, "out of host memory");
+  c->flat_n = n;
+  c->key = g; c->Nc = g->Nc; c->Ne = g->Ne;
+  g_nctx++;
+  return c;
+}
+
+static void extents(ctx_t *c, gridT *g, const char *where) {
+  sunhip_extents X = {g->ctop, g->ctopold, g->Nke, g->etop};
+  check(sunhip_set_extents(c->dev, &X), where);
+}
+
+/* flat <-> jagged */
+static double *pack_cells(ctx_t *c, const gridT *g, REAL **rows, int extra) {
+  size_t o = 0;
+  for (int i = 0; i < g->Nc; i++) {
+    size_t n = (size_t)(g->Nk[i] + extra);
+    memcpy(c->flat + o, rows[i], n * sizeof(double));
+    o += n;
+  }
+  return c->flat;
+}
+static double *pack_edges(ctx_t *c, const gridT *g, REAL **rows) {
+  size_t o = 0;
+  for (int j = 0; j < g->Ne; j++) {
+    memcpy(c->flat + o, rows[j], (size_t)g->Nkc[j] * sizeof(double));
+    o += (size_t)g->Nkc[j];
+  }
+  return c->flat;
+}
+/* write back cells of cellp[celldist[lo]..celldist[hi]) */
+static void unpack_cells(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  static size_t *start = NULL;
+  static int nstart = 0;
+  if (nstart < g->Nc + 1) {
+    free(start);
+    start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Nc + 1));
+    nstart = g->Nc + 1;
+  }
+  start[0] = 0;
+  for (int i = 0; i < g->Nc; i++) start[i + 1] = start[i] + (size_t)g->Nk[i];
+  for (int ip = g->celldist[lo]; ip < g->celldist[hi]; ip++) {
+    const int i = g->cellp[ip];
+    memcpy(rows[i], c->flat + start[i], (size_t)g->Nk[i] * sizeof(double));
+  }
+}
+
+static void upload_cells(ctx_t *c, gridT *g, const char *name, REAL **rows, int extra, const char *where) {
+  check(sunhip_upload(c->dev, name, pack_cells(c, g, rows, extra)), where);
+}
+static void upload_edges(ctx_t *c, gridT *g, const char *name, REAL **rows, const char *where) {
+  check(sunhip_upload(c->dev, name, pack_edges(c, g, rows)), where);
+}
+
+void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs,
+              MPI_Comm comm) {
+  static const char *W = "CGSolveQ";
+  (void)myproc; (void)comm; (void)c;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  if (prop->qprecond != 0 && prop->qprecond != 2)
+    fatal(W, "qprecond must be 0 or 2 (qprecond=1 is not built for the device)");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "q", q, 0, W);
+  upload_cells(C, grid, "src", src, 0, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  check(sunhip_upload(C->dev, "D", phys->D), W);
+  sunhip_cg_params P = {prop->qmaxiters, prop->qprecond, (int)prop->resnorm, prop->qepsilon, 0};
+  check(sunhip_q_solve(C->dev, &P, &g_last[0].st), W);
+  g_last[0].valid = 1;
+  check(sunhip_download(C->dev, "q", C->flat), W);
+  unpack_cells(C, grid, q, 0, 1);
+}
+
+void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
+  static const char *W = "CGSolve";
+  (void)myproc; (void)comm;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  if (prop->hprecond != 0 && prop->hprecond != 1) fatal(W, "hprecond must be 0 or 1");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  check(sunhip_upload(C->dev, "h", phys->h), W);
+  check(sunhip_upload(C->dev, "htmp", phys->htmp), W);
+  check(sunhip_upload(C->dev, "D", phys->D), W);
+  sunhip_cg_params P = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0};
+  check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
+  g_last[1].valid = 1;
+  check(sunhip_download(C->dev, "h", C->flat), W);
+  for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
+    const int i = grid->cellp[ip];
+    phys->h[i] = C->flat[i];
+  }
+}
+
+int sunhip_q_coefficients(sunhip_grid *g);
+
+void OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *grid, physT *phys, propT *prop) {
+  static const char *W = "OperatorQ";
+  (void)c; (void)prop;
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "coef", coef, 0, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  check(sunhip_upload(C->dev, "D", phys->D), W);
+  upload_cells(C, grid, "opx", x, 0, W);
+  check(sunhip_q_operator(C->dev, "opx", "opy"), W);
+  check(sunhip_download(C->dev, "opy", C->flat), W);
+  unpack_cells(C, grid, y, 0, 1);
+}
+
+void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, propT *prop) {
+  static const char *W = "Preconditioner";
+  (void)phys; (void)prop;
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "coef", coef, 0, W);
+  upload_cells(C, grid, "opx", x, 0, W);
+  check(sunhip_q_precondition(C->dev, "opx", "opy"), W);
+  check(sunhip_download(C->dev, "opy", C->flat), W);
+  unpack_cells(C, grid, xc, 0, 1);
+}
+
+int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char *x, const char *y);
+
+void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *phys, propT *prop) {
+  static const char *W = "OperatorH";
+  (void)phys; (void)prop;
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  check(sunhip_upload(C->dev, "hcoef", coef), W);
+  check(sunhip_upload(C->dev, "hx", x), W);
+  check(sunhip_h_operator_coef(C->dev, fcoef, "hx", "hy"), W);
+  check(sunhip_download(C->dev, "hy", C->flat), W);
+  for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
+    const int i = grid->cellp[ip];
+    y[i] = C->flat[i];
+  }
+}
+
+/* open-boundary rows (edges of edgedist[2..5)) into an edge field */
+static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) {
+  const int lo = g->edgedist[2], hi = g->edgedist[5];
+  if (!bscal || hi <= lo) return;
+  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
+  start[0] = 0;
+  for (int j = 0; j < g->Ne; j++) start[j + 1] = start[j] + (size_t)g->Nkc[j];
+  memset(C->flat, 0, start[g->Ne] * sizeof(double));
+  for (int jp = lo; jp < hi; jp++) {
+    const int j = g->edgep[jp];
+    memcpy(C->flat + start[j], bscal[jp - lo], (size_t)g->Nke[j] * sizeof(double));
+  }
+  free(start);
+  check(sunhip_upload(C->dev, "bscal", C->flat), W);
+}
+
+static void download_face_values(ctx_t *C, gridT *g, physT *p, const char *W) {
+  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
+  start[0] = 0;
+  for (int j = 0; j < g->Ne; j++) start[j + 1] = start[j] + (size_t)g->Nkc[j];
+  const char *names[2] = {"SfHp", "SfHm"};
+  REAL **rows[2] = {p->SfHp, p->SfHm};
+  for (int f = 0; f < 2; f++) {
+    check(sunhip_download(C->dev, names[f], C->flat), W);
+    for (int r = 0; r < 2; r++) {
+      const int lo = g->edgedist[r == 0 ? 0 : 2], hi = g->edgedist[r == 0 ? 1 : 3];
+      for (int jp = lo; jp < hi; jp++) {
+        const int j = g->edgep[jp];
+        memcpy(rows[f][j], C->flat + start[j], (size_t)g->Nke[j] * sizeof(double));
+      }
+    }
+  }
+  free(start);
+}
+
+static void scalar_inputs(ctx_t *C, gridT *grid, physT *phys, REAL **scal, REAL **bscal, const char *W) {
+  extents(C, grid, W);
+  upload_cells(C, grid, "s", scal, 0, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_edges(C, grid, "utmp2", phys->utmp2, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  upload_bscal(C, grid, bscal, W);
+}
+
+void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal, int TVD,
+                           MPI_Comm comm, int myproc) {
+  static const char *W = "HorizontalFaceScalars";
+  (void)comm; (void)myproc;
+  ctx_t *C = get_ctx(grid, W);
+  scalar_inputs(C, grid, phys, scal, boundary_scal, W);
+  sunhip_scalar_params S = {prop->theta, prop->dt, 0.0, TVD, prop->n, 0, 0};
+  check(sunhip_face_scalars(C->dev, &S, "s"), W);
+  download_face_values(C, grid, phys, W);
+  check(sunhip_download(C->dev, "gradSx", C->flat), W);
+  unpack_cells(C, grid, phys->gradSx, 0, 2);
+  check(sunhip_download(C->dev, "gradSy", C->flat), W);
+  unpack_cells(C, grid, phys->gradSy, 0, 2);
+}
+
+void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **scal, REAL **boundary_scal,
+                   REAL **Cn, REAL kappa, REAL kappaH, REAL **kappa_tv, REAL theta, REAL **src1, REAL **src2,
+                   REAL *Ftop, REAL *Fbot, int alpha_top, int alpha_bot, MPI_Comm comm, int myproc,
+                   int checkflag, int TVDscheme) {
+  static const char *W = "UpdateScalars";
+  (void)kappaH; (void)comm; (void)myproc; (void)checkflag;
+  if (src1 || src2 || Ftop || Fbot)
+    fatal(W, "source terms (src1, src2, Ftop, Fbot) are not built for the device");
+  if (!TVDscheme) {
+    for (int jp = grid->edgedist[2]; jp < grid->edgedist[5]; jp++)
+      if (boundary_scal) fatal(W, "first-order upwind with open boundaries is not built for the device");
+  }
+  prop->TVD = TVDscheme;
+  prop->horiTVD = 1;
+  prop->vertTVD = 1;
+  ctx_t *C = get_ctx(grid, W);
+  scalar_inputs(C, grid, phys, scal, boundary_scal, W);
+  upload_cells(C, grid, "wnew", wnew, 1, W);
+  upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_cells(C, grid, "dzzold", grid->dzzold, 0, W);
+  upload_cells(C, grid, "kappa_tv", kappa_tv, 0, W);
+  for (int i = 0; i < grid->Nc; i++) C->flat[i] = phys->active[i] ? 1.0 : 0.0;
+  check(sunhip_upload(C->dev, "active", C->flat), W);
+  sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
+  check(sunhip_update_scalars(C->dev, &S, "s"), W);
+  check(sunhip_download(C->dev, "s", C->flat), W);
+  unpack_cells(C, grid, scal, 0, 1);
+  /* Cn: zero everywhere, then the computational columns */
+  for (int i = 0; i < grid->Nc; i++) memset(Cn[i], 0, (size_t)grid->Nk[i] * sizeof(double));
+  check(sunhip_download(C->dev, "Cn", C->flat), W);
+  unpack_cells(C, grid, Cn, 0, 1);
+  if (TVDscheme) download_face_values(C, grid, phys, W);
+}
+
+int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) {
+  if (which < 0 || which > 1 || !g_last[which].valid) return -1;
+  if (niters) *niters = g_last[which].st.niters;
+  if (resid) *resid = g_last[which].st.resid;
+  if (device_ms) *device_ms = g_last[which].st.device_ms;
+  return 0;
+}
+
+void sunhip_dropin_reset(void) {
+  for (int i = 0; i < g_nctx; i++) {
+    sunhip_grid_destroy(g_ctx[i].dev);
+    free(g_ctx[i].flat);
+  }
+  memset(g_ctx, 0, sizeof g_ctx);
+  g_nctx = 0;
+}
+
+void sunhip_dropin_set_mode(int device) {
+  if (device != g_device) sunhip_dropin_reset();
+  g_device = device;
+}

@@ -1,0 +1,724 @@
+/*
+ * sunstate.c -- host construction of the state the drop-in solvers consume.
+ * See include/sunstate.h for the contract of each function.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+#include "sunstate.h"
+
+#define DRY_HEIGHT 1e-10 /* main/suntans.h:27 (DRYCELLHEIGHT) */
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) { fprintf(stderr, "sunstate: out of host memory\n"); exit(1); }
+  return p;
+}
+
+/* One contiguous block carved into rows of len[i] (+extra) values. */
+static REAL **rows_alloc(int nrows, const int *len, int extra) {
+  size_t total = 0;
+  for (int i = 0; i < nrows; i++) total += (size_t)(len[i] + extra);
+  REAL **r = (REAL **)xcalloc((size_t)nrows, sizeof(REAL *));
+  REAL *blk = (REAL *)xcalloc(total, sizeof(REAL));
+  size_t off = 0;
+  for (int i = 0; i < nrows; i++) { r[i] = blk + off; off += (size_t)(len[i] + extra); }
+  return r;
+}
+static void rows_free(REAL **r) { if (r) { free(r[0]); free(r); } }
+static size_t rows_total(int nrows, const int *len, int extra) {
+  size_t t = 0;
+  for (int i = 0; i < nrows; i++) t += (size_t)(len[i] + extra);
+  return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* planform                                                                  */
+
+gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly) {
+  gridT *g = (gridT *)xcalloc(1, sizeof(gridT));
+  const int Nc = 2 * nx * ny, Np = (nx + 1) * (ny + 1);
+  const int Ne = 3 * nx * ny + nx + ny;
+  const int rowE = 3 * nx + 1;
+  const double dx = Lx / nx, dy = Ly / ny;
+  g->Nc = g->Nc_comp = Nc; g->Ne = g->Ne_comp = Ne; g->Np = Np;
+  g->maxfaces = 3; g->Nge = 0;
+
+#define PT(ix, iy) ((iy) * (nx + 1) + (ix))
+#define T0(ix, iy) (2 * ((iy) * nx + (ix)))
+#define T1(ix, iy) (2 * ((iy) * nx + (ix)) + 1)
+#define EH(ix, iy) ((iy) < ny ? (iy) * rowE + 3 * (ix) : ny * rowE + (ix))
+#define EV(ix, iy) ((ix) < nx ? (iy) * rowE + 3 * (ix) + 1 : (iy) * rowE + 3 * nx)
+#define ED(ix, iy) ((iy) * rowE + 3 * (ix) + 2)
+
+  g->xp = (REAL *)xcalloc(Np, sizeof(REAL));
+  g->yp = (REAL *)xcalloc(Np, sizeof(REAL));
+  for (int iy = 0; iy <= ny; iy++)
+    for (int ix = 0; ix <= nx; ix++) { g->xp[PT(ix, iy)] = ix * dx; g->yp[PT(ix, iy)] = iy * dy; }
+
+  g->cells = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
+  g->face = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
+  g->neigh = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
+  g->normal = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
+  g->nfaces = (int *)xcalloc(Nc, sizeof(int));
+  g->xv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->yv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->Ac = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->def = (REAL *)xcalloc(3 * (size_t)Nc, sizeof(REAL));
+  g->dv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->edges = (int *)xcalloc(3 * (size_t)Ne, sizeof(int));
+  g->grad = (int *)xcalloc(2 * (size_t)Ne, sizeof(int));
+  g->gradf = (int *)xcalloc(2 * (size_t)Ne, sizeof(int));
+  g->mark = (int *)xcalloc(Ne, sizeof(int));
+  g->eneigh = (int *)xcalloc(2 * (size_t)(3 - 1) * Ne, sizeof(int));
+  g->df = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->dg = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->n1 = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->n2 = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->xe = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->ye = (REAL *)xcalloc(Ne, sizeof(REAL));
+
+  for (int iy = 0; iy < ny; iy++)
+    for (int ix = 0; ix < nx; ix++) {
+      int c0 = T0(ix, iy), c1 = T1(ix, iy);
+      int *v0 = g->cells + 3 * c0, *v1 = g->cells + 3 * c1;
+      v0[0] = PT(ix, iy); v0[1] = PT(ix + 1, iy); v0[2] = PT(ix + 1, iy + 1);
+      v1[0] = PT(ix, iy); v1[1] = PT(ix + 1, iy + 1); v1[2] = PT(ix, iy + 1);
+      /* face nf joins corner nf to corner nf+1 */
+      int *f0 = g->face + 3 * c0, *f1 = g->face + 3 * c1;
+      int *n0 = g->neigh + 3 * c0, *n1 = g->neigh + 3 * c1;
+      f0[0] = EH(ix, iy);     n0[0] = iy > 0 ? T1(ix, iy - 1) : -1;
+      f0[1] = EV(ix + 1, iy); n0[1] = ix + 1 < nx ? T1(ix + 1, iy) : -1;
+      f0[2] = ED(ix, iy);     n0[2] = c1;
+      f1[0] = ED(ix, iy);     n1[0] = c0;
+      f1[1] = EH(ix, iy + 1); n1[1] = iy + 1 < ny ? T0(ix, iy + 1) : -1;
+      f1[2] = EV(ix, iy);     n1[2] = ix > 0 ? T0(ix - 1, iy) : -1;
+      g->nfaces[c0] = g->nfaces[c1] = 3;
+      g->xv[c0] = (ix + 2.0 / 3.0) * dx; g->yv[c0] = (iy + 1.0 / 3.0) * dy;
+      g->xv[c1] = (ix + 1.0 / 3.0) * dx; g->yv[c1] = (iy + 2.0 / 3.0) * dy;
+      g->Ac[c0] = g->Ac[c1] = 0.5 * dx * dy;
+    }
+
+  /* edges: end points and the cells on either side (second is -1 on the
+   * boundary); ordering of the two sides is below/above, left/right, T0/T1 */
+  for (int iy = 0; iy <= ny; iy++)
+    for (int ix = 0; ix < nx; ix++) {
+      int e = EH(ix, iy);
+      int below = iy > 0 ? T1(ix, iy - 1) : -1, above = iy < ny ? T0(ix, iy) : -1;
+      g->edges[3 * e] = PT(ix, iy); g->edges[3 * e + 1] = PT(ix + 1, iy);
+      if (below >= 0 && above >= 0) { g->grad[2 * e] = below; g->grad[2 * e + 1] = above; }
+      else { g->grad[2 * e] = below >= 0 ? below : above; g->grad[2 * e + 1] = -1; g->mark[e] = 1; }
+    }
+  for (int iy = 0; iy < ny; iy++)
+    for (int ix = 0; ix <= nx; ix++) {
+      int e = EV(ix, iy);
+      int left = ix > 0 ? T0(ix - 1, iy) : -1, right = ix < nx ? T1(ix, iy) : -1;
+      g->edges[3 * e] = PT(ix, iy); g->edges[3 * e + 1] = PT(ix, iy + 1);
+      if (left >= 0 && right >= 0) { g->grad[2 * e] = left; g->grad[2 * e + 1] = right; }
+      else { g->grad[2 * e] = left >= 0 ? left : right; g->grad[2 * e + 1] = -1; g->mark[e] = 1; }
+    }
+  for (int iy = 0; iy < ny; iy++)
+    for (int ix = 0; ix < nx; ix++) {
+      int e = ED(ix, iy);
+      g->edges[3 * e] = PT(ix, iy); g->edges[3 * e + 1] = PT(ix + 1, iy + 1);
+      g->grad[2 * e] = T0(ix, iy); g->grad[2 * e + 1] = T1(ix, iy);
+    }
+  for (int e = 0; e < Ne; e++) g->edges[3 * e + 2] = g->mark[e];
+
+  /* metrics, main/grid.c:3457-3533 conventions */
+  for (int e = 0; e < Ne; e++) {
+    int p1 = g->edges[3 * e], p2 = g->edges[3 * e + 1];
+    double tx = g->xp[p2] - g->xp[p1], ty = g->yp[p2] - g->yp[p1];
+    double tm = sqrt(tx * tx + ty * ty);
+    g->df[e] = tm;
+    tx /= tm; ty /= tm;
+    int c = g->grad[2 * e] >= 0 ? g->grad[2 * e] : g->grad[2 * e + 1];
+    double s = (g->xv[c] - g->xp[p1]) * tx + (g->yv[c] - g->yp[p1]) * ty;
+    g->xe[e] = g->xp[p1] + s * tx; g->ye[e] = g->yp[p1] + s * ty;
+    int a = g->grad[2 * e], b = g->grad[2 * e + 1];
+    double nx_, ny_;
+    if (a >= 0 && b >= 0) {
+      nx_ = g->xv[a] - g->xv[b]; ny_ = g->yv[a] - g->yv[b];
+      g->dg[e] = sqrt(nx_ * nx_ + ny_ * ny_);
+      g->n1[e] = nx_ / g->dg[e]; g->n2[e] = ny_ / g->dg[e];
+    } else {
+      int in = a >= 0 ? a : b;
+      nx_ = a >= 0 ? g->xv[in] - g->xe[e] : g->xe[e] - g->xv[in];
+      ny_ = a >= 0 ? g->yv[in] - g->ye[e] : g->ye[e] - g->yv[in];
+      double d = sqrt(nx_ * nx_ + ny_ * ny_);
+      g->n1[e] = nx_ / d; g->n2[e] = ny_ / d; g->dg[e] = 2.0 * d;
+    }
+  }
+  for (int i = 0; i < Nc; i++)
+    for (int nf = 0; nf < 3; nf++) {
+      int e = g->face[3 * i + nf];
+      g->normal[3 * i + nf] = (i == g->grad[2 * e]) ? -1 : 1;
+      if (g->grad[2 * e] == i) g->gradf[2 * e] = nf; else g->gradf[2 * e + 1] = nf;
+      int p1 = g->edges[3 * e];
+      g->def[3 * i + nf] = -((g->xv[i] - g->xp[p1]) * g->n1[e] + (g->yv[i] - g->yp[p1]) * g->n2[e]) *
+                           g->normal[3 * i + nf];
+    }
+
+  /* computational ordering (single process): every cell is computational;
+   * edges sorted by mark 0,1,2,3,4 */
+  g->cellp = (int *)xcalloc(Nc, sizeof(int));
+  g->celldist = (int *)xcalloc(3, sizeof(int));
+  for (int i = 0; i < Nc; i++) g->cellp[i] = i;
+  g->celldist[0] = 0; g->celldist[1] = Nc; g->celldist[2] = Nc;
+  g->edgep = (int *)xcalloc(Ne, sizeof(int));
+  g->edgedist = (int *)xcalloc(6, sizeof(int));
+  int k = 0;
+  for (int e = 0; e < Ne; e++) if (g->mark[e] == 0 || g->mark[e] == 5) g->edgep[k++] = e;
+  g->edgedist[1] = k;
+  for (int m = 1; m < 5; m++) {
+    for (int e = 0; e < Ne; e++) if (g->mark[e] == m) g->edgep[k++] = e;
+    g->edgedist[m + 1] = k;
+  }
+  g->mnptr = (int *)xcalloc(Nc, sizeof(int));
+  g->eptr = (int *)xcalloc(Ne, sizeof(int));
+  for (int i = 0; i < Nc; i++) g->mnptr[i] = i;
+  for (int e = 0; e < Ne; e++) g->eptr[e] = e;
+  g->Nneighs = 0;
+  g->myneighs = (int *)xcalloc(1, sizeof(int));
+  return g;
+#undef PT
+#undef T0
+#undef T1
+#undef EH
+#undef EV
+#undef ED
+}
+
+int sun_grid_vertical(gridT *g, int Nkmax, const double *dz, const double *dv, int stairstep) {
+  const int Nc = g->Nc, Ne = g->Ne;
+  g->Nkmax = Nkmax;
+  g->stairstep = stairstep;
+  g->dz = (REAL *)xcalloc(Nkmax, sizeof(REAL));
+  memcpy(g->dz, dz, sizeof(REAL) * (size_t)Nkmax);
+  memcpy(g->dv, dv, sizeof(REAL) * (size_t)Nc);
+  double dmax = 0;
+  for (int i = 0; i < Nc; i++) if (dv[i] > dmax) dmax = dv[i];
+  g->Nk = (int *)xcalloc(Nc, sizeof(int));
+  /* layer count: the first k whose cumulative thickness reaches the depth */
+  for (int i = 0; i < Nc; i++) {
+    if (Nkmax <= 1 || g->dv[i] == dmax) { g->Nk[i] = Nkmax; continue; }
+    double z = 0;
+    for (int k = 0; k < Nkmax; k++) {
+      z += g->dz[k];
+      g->Nk[i] = k + 1;
+      if (z >= g->dv[i]) break;
+    }
+  }
+  if (stairstep)
+    for (int i = 0; i < Nc; i++) {
+      double z = 0;
+      for (int k = 0; k < g->Nk[i]; k++) z += g->dz[k];
+      g->dv[i] = z;
+    }
+  g->ctop = (int *)xcalloc(Nc, sizeof(int));
+  g->ctopold = (int *)xcalloc(Nc, sizeof(int));
+  g->dztop = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->dzbot = (REAL *)xcalloc(Nc, sizeof(REAL));
+  for (int i = 0; i < Nc; i++) g->dztop[i] = g->dz[0];
+  g->etop = (int *)xcalloc(Ne, sizeof(int));
+  g->etopold = (int *)xcalloc(Ne, sizeof(int));
+  g->Nke = (int *)xcalloc(Ne, sizeof(int));
+  g->Nkc = (int *)xcalloc(Ne, sizeof(int));
+  for (int e = 0; e < Ne; e++) {
+    int a = g->grad[2 * e], b = g->grad[2 * e + 1];
+    if (a < 0 || b < 0) {
+      int c = a >= 0 ? a : b;
+      g->Nke[e] = g->Nkc[e] = g->Nk[c];
+    } else {
+      int na = g->Nk[a], nb = g->Nk[b];
+      g->Nke[e] = na < nb ? na : nb;
+      g->Nkc[e] = na < nb ? nb : na;
+    }
+  }
+  g->Nkp = (int *)xcalloc(g->Np, sizeof(int));
+  for (int p = 0; p < g->Np; p++) g->Nkp[p] = Nkmax;
+  g->dzz = rows_alloc(Nc, g->Nk, 0);
+  g->dzzold = rows_alloc(Nc, g->Nk, 0);
+  for (int i = 0; i < Nc; i++)
+    for (int k = 0; k < g->Nk[i]; k++) g->dzz[i][k] = g->dzzold[i][k] = g->dz[k];
+  g->dzf = rows_alloc(Ne, g->Nkc, 0);
+  g->dzfB = (REAL *)xcalloc(Ne, sizeof(REAL));
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* physical state                                                            */
+
+struct fdesc { const char *name; int kind; size_t off; };
+/* kinds: 1 cell rows (Nk), 2 w rows (Nk+1), 3 edge rows (Nkc), 4 cell 2-D,
+ * 5 edge 2-D, 6 column scratch (Nkmax+1) */
+#define PF(f, k) {#f, k, offsetof(physT, f)}
+static const struct fdesc phys_fields[] = {
+  PF(uc, 1), PF(vc, 1), PF(wc, 1), PF(uold, 1), PF(vold, 1), PF(q, 1), PF(qc, 1),
+  PF(s, 1), PF(T, 1), PF(s0, 1), PF(rho, 1), PF(stmp, 1), PF(stmp2, 1), PF(stmp3, 1),
+  PF(nu_tv, 1), PF(kappa_tv, 1), PF(nu_lax, 1), PF(Cn_R, 1), PF(Cn_T, 1),
+  PF(gradSx, 1), PF(gradSy, 1), PF(Ttmp, 1),
+  PF(w, 2), PF(wnew, 2), PF(wtmp, 2), PF(wtmp2, 2), PF(Cn_W, 2), PF(Cn_W2, 2),
+  PF(u, 3), PF(utmp, 3), PF(utmp2, 3), PF(ut, 3), PF(Cn_U, 3), PF(Cn_U2, 3),
+  PF(SfHp, 3), PF(SfHm, 3),
+  PF(h, 4), PF(hcorr, 4), PF(hold, 4), PF(htmp, 4), PF(htmp2, 4), PF(htmp3, 4),
+  PF(hcoef, 4), PF(Tsurf, 4), PF(dT, 4),
+  PF(D, 5), PF(tau_T, 5), PF(tau_B, 5), PF(CdT, 5), PF(CdB, 5),
+  PF(ap, 6), PF(am, 6), PF(bp, 6), PF(bm, 6), PF(a, 6), PF(b, 6), PF(c, 6), PF(d, 6),
+  PF(Cp, 6), PF(Cm, 6), PF(rp, 6), PF(rm, 6), PF(wp, 6), PF(wm, 6),
+};
+#undef PF
+#define NPF (int)(sizeof(phys_fields) / sizeof(phys_fields[0]))
+
+physT *sun_phys_alloc(const gridT *g) {
+  physT *p = (physT *)xcalloc(1, sizeof(physT));
+  const int Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
+  for (int f = 0; f < NPF; f++) {
+    void *slot = (char *)p + phys_fields[f].off;
+    switch (phys_fields[f].kind) {
+      case 1: *(REAL ***)slot = rows_alloc(Nc, g->Nk, 0); break;
+      case 2: *(REAL ***)slot = rows_alloc(Nc, g->Nk, 1); break;
+      case 3: *(REAL ***)slot = rows_alloc(Ne, g->Nkc, 0); break;
+      case 4: *(REAL **)slot = (REAL *)xcalloc(Nc, sizeof(REAL)); break;
+      case 5: *(REAL **)slot = (REAL *)xcalloc(Ne, sizeof(REAL)); break;
+      case 6: *(REAL **)slot = (REAL *)xcalloc((size_t)g->Nkmax + 1, sizeof(REAL)); break;
+    }
+  }
+  /* htmp is 10*Nc long in the reference (main/phys.c:177) */
+  free(p->htmp);
+  p->htmp = (REAL *)xcalloc(10 * (size_t)Nc, sizeof(REAL));
+  p->hfcoef = (REAL *)xcalloc((size_t)mf * Nc, sizeof(REAL));
+  /* qtmp: one Nk row per cell face (main/phys.c:194,261-262) */
+  {
+    int *len = (int *)xcalloc((size_t)mf * Nc, sizeof(int));
+    for (int i = 0; i < Nc; i++)
+      for (int nf = 0; nf < g->nfaces[i]; nf++) len[i * mf + nf] = g->Nk[i];
+    p->qtmp = rows_alloc(mf * Nc, len, 0);
+    free(len);
+  }
+  p->active = (unsigned char *)xcalloc(Nc, 1);
+  for (int i = 0; i < Nc; i++) p->active[i] = 1;
+  /* open-boundary rows: one per edge of marks 2..4 (main/phys.c:285-305) */
+  {
+    int nb = g->edgedist[5] - g->edgedist[2];
+    int *len = (int *)xcalloc(nb, sizeof(int));
+    for (int j = 0; j < nb; j++) len[j] = g->Nke[g->edgep[g->edgedist[2] + j]];
+    p->boundary_s = rows_alloc(nb, len, 0);
+    p->boundary_T = rows_alloc(nb, len, 0);
+    p->boundary_u = rows_alloc(nb, len, 0);
+    p->boundary_v = rows_alloc(nb, len, 0);
+    p->boundary_rho = rows_alloc(nb, len, 0);
+    p->boundary_w = rows_alloc(nb, len, 1);
+    p->boundary_tmp = rows_alloc(nb, len, 1);
+    p->boundary_h = (REAL *)xcalloc(nb, sizeof(REAL));
+    p->boundary_flag = (REAL *)xcalloc(nb, sizeof(REAL));
+    free(len);
+  }
+  return p;
+}
+
+void sun_phys_free(const gridT *g, physT *p) {
+  (void)g;
+  if (!p) return;
+  for (int f = 0; f < NPF; f++) {
+    void *slot = (char *)p + phys_fields[f].off;
+    if (phys_fields[f].kind <= 3) rows_free(*(REAL ***)slot); else free(*(REAL **)slot);
+  }
+  free(p->hfcoef);
+  rows_free(p->qtmp);
+  free(p->active);
+  rows_free(p->boundary_s); rows_free(p->boundary_T); rows_free(p->boundary_u);
+  rows_free(p->boundary_v); rows_free(p->boundary_rho); rows_free(p->boundary_w);
+  rows_free(p->boundary_tmp);
+  free(p->boundary_h); free(p->boundary_flag);
+  free(p);
+}
+
+static size_t kind_len(const gridT *g, int kind) {
+  switch (kind) {
+    case 1: return rows_total(g->Nc, g->Nk, 0);
+    case 2: return rows_total(g->Nc, g->Nk, 1);
+    case 3: return rows_total(g->Ne, g->Nkc, 0);
+    case 4: return (size_t)g->Nc;
+    case 5: return (size_t)g->Ne;
+    case 6: return (size_t)g->Nkmax + 1;
+  }
+  return 0;
+}
+
+void sun_phys_copy(const gridT *g, const physT *from, physT *to) {
+  for (int f = 0; f < NPF; f++) {
+    size_t off = phys_fields[f].off, n = kind_len(g, phys_fields[f].kind);
+    if (phys_fields[f].kind <= 3)
+      memcpy((*(REAL ***)((char *)to + off))[0], (*(REAL ***)((char *)from + off))[0], n * sizeof(REAL));
+    else
+      memcpy(*(REAL **)((char *)to + off), *(REAL **)((char *)from + off), n * sizeof(REAL));
+  }
+  memcpy(to->htmp, from->htmp, 10 * (size_t)g->Nc * sizeof(REAL));
+  memcpy(to->hfcoef, from->hfcoef, (size_t)g->maxfaces * g->Nc * sizeof(REAL));
+  memcpy(to->active, from->active, (size_t)g->Nc);
+  {
+    int nb = g->edgedist[5] - g->edgedist[2];
+    size_t n = 0;
+    for (int j = 0; j < nb; j++) n += (size_t)g->Nke[g->edgep[g->edgedist[2] + j]];
+    if (n) memcpy(to->boundary_s[0], from->boundary_s[0], n * sizeof(REAL));
+  }
+  to->mass = from->mass; to->volume = from->volume;
+}
+
+physT *sun_phys_clone(const gridT *g, const physT *p) {
+  physT *q = sun_phys_alloc(g);
+  sun_phys_copy(g, p, q);
+  return q;
+}
+
+propT *sun_prop_default(void) {
+  propT *pr = (propT *)xcalloc(1, sizeof(propT));
+  /* examples/lockexchange/rundata/suntans.dat values where the path reads them */
+  pr->dt = 5.0; pr->theta = 0.55; pr->thetaS = 0.55; pr->thetaB = 0.55;
+  pr->nu = 0.025; pr->nu_H = 0.016; pr->CdB = 0.025; pr->grav = 9.81;
+  pr->epsilon = 1e-10; pr->qepsilon = 1e-10; pr->resnorm = 0;
+  pr->beta = 1.0; pr->kappa_s = 0; pr->kappa_sH = 0;
+  pr->nonhydrostatic = 1; pr->cgsolver = 1; pr->maxiters = 1000; pr->qmaxiters = 2000;
+  pr->qprecond = 2; pr->hprecond = 1; pr->nonlinear = 2; pr->TVDsalt = 4;
+  pr->n = 1; pr->nstart = 0; pr->nsteps = 1;
+  return pr;
+}
+void sun_prop_free(propT *prop) { free(prop); }
+
+void sun_grid_free(gridT *g) {
+  if (!g) return;
+  REAL *r1[] = {g->xp, g->yp, g->xv, g->yv, g->xe, g->ye, g->dv, g->dz, g->dzfB, g->dztop,
+                g->dzbot, g->df, g->dg, g->def, g->Ac, g->n1, g->n2};
+  for (size_t i = 0; i < sizeof(r1) / sizeof(r1[0]); i++) free(r1[i]);
+  int *i1[] = {g->face, g->edges, g->cells, g->neigh, g->eneigh, g->grad, g->gradf, g->mark,
+               g->normal, g->celldist, g->edgedist, g->cellp, g->edgep, g->mnptr, g->eptr,
+               g->myneighs, g->Nk, g->ctop, g->ctopold, g->etop, g->etopold, g->Nkp, g->Nke,
+               g->Nkc, g->nfaces};
+  for (size_t i = 0; i < sizeof(i1) / sizeof(i1[0]); i++) free(i1[i]);
+  rows_free(g->dzz); rows_free(g->dzzold); rows_free(g->dzf);
+  free(g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* vertical grid updates                                                     */
+
+void sun_update_dz(gridT *g, physT *p, const propT *prop, int option) {
+  const int Nc = g->Nc, Ne = g->Ne;
+  if (prop->linearFS) return;
+  if (!option) {
+    for (int j = 0; j < Ne; j++) g->etopold[j] = g->etop[j];
+    for (int i = 0; i < Nc; i++) {
+      g->ctopold[i] = g->ctop[i];
+      for (int k = 0; k < g->Nk[i]; k++) g->dzzold[i][k] = k < g->ctop[i] ? 0 : g->dzz[i][k];
+    }
+  }
+  if (option == -1) for (int i = 0; i < Nc; i++) p->h[i] = 0;
+  /* partial-step bottom layer */
+  for (int i = 0; i < Nc; i++) {
+    double z = 0;
+    for (int k = 0; k < g->Nk[i]; k++) z -= g->dz[k];
+    g->dzz[i][g->Nk[i] - 1] = g->dz[g->Nk[i] - 1] + g->dv[i] + z;
+  }
+  /* layer thicknesses cut by the free surface */
+  if (g->Nkmax > 1) {
+    for (int i = 0; i < Nc; i++) {
+      double z = 0;
+      int found = 0;
+      const int last = g->Nk[i] - 1;
+      for (int k = 0; k <= last; k++) {
+        z -= g->dz[k];
+        if (p->h[i] < z) { g->dzz[i][k] = 0; continue; }
+        if (!found) {
+          found = 1;
+          if (k == last) { g->dzz[i][k] = p->h[i] + g->dv[i]; g->ctop[i] = k; }
+          else if (p->h[i] == z) { g->dzz[i][k] = 0; g->ctop[i] = k + 1; }
+          else { g->dzz[i][k] = p->h[i] - z; g->ctop[i] = k; }
+        } else if (k == last) {
+          g->dzz[i][k] = g->dz[k] + g->dv[i] + z;
+        } else {
+          g->dzz[i][k] = (z < -g->dv[i]) ? 0 : g->dz[k];
+        }
+      }
+    }
+  } else {
+    for (int i = 0; i < Nc; i++) g->dzz[i][0] = g->dv[i] + p->h[i];
+  }
+  for (int j = 0; j < Ne; j++) {
+    int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+    if (a == -1) g->etop[j] = g->ctop[b];
+    else if (b == -1) g->etop[j] = g->ctop[a];
+    else g->etop[j] = g->ctop[a] < g->ctop[b] ? g->ctop[a] : g->ctop[b];
+  }
+  if (option == -1) {
+    for (int i = 0; i < Nc; i++) {
+      int k = g->Nk[i] - 1;
+      g->dzbot[i] = g->dzz[i][k];
+      if (!g->stairstep && g->fixdzz && g->dzz[i][k] < g->dz[k] * g->dzsmall)
+        g->dv[i] += g->dz[k] * g->dzsmall - g->dzz[i][k];
+    }
+  }
+  if (option == 1) {
+    for (int j = 0; j < Ne; j++) g->etopold[j] = g->etop[j];
+    for (int i = 0; i < Nc; i++) {
+      g->ctopold[i] = g->ctop[i];
+      for (int k = 0; k < g->Nk[i]; k++) g->dzzold[i][k] = g->dzz[i][k];
+    }
+    for (int j = 0; j < Ne; j++) {
+      int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+      if (a == -1) a = b;
+      if (b == -1) b = a;
+      double d1 = g->dzz[a][g->Nk[a] - 1], d2 = g->dzz[b][g->Nk[b] - 1], z = 0;
+      for (int k = 0; k < g->Nke[j] - 1; k++) z -= g->dz[k];
+      if (p->h[a] < z) d1 = d1 - z + p->h[a];
+      if (p->h[b] < z) d2 = d2 - z + p->h[b];
+      g->dzfB[j] = d1 < d2 ? d1 : d2;
+    }
+  }
+}
+
+/* flux height chosen by the sign of u: downstream-of-flow cell's thickness
+ * (main/util.c:219-232) */
+static double upwind(double u, double a, double b) {
+  if (fabs(u) > 0) return u > 0 ? b : a;
+  return b > a ? b : a;
+}
+
+void sun_set_flux_height(gridT *g, physT *p, const propT *prop) {
+  (void)prop;
+  for (int j = 0; j < g->Ne; j++)
+    for (int k = 0; k < g->Nkc[j]; k++) g->dzf[j][k] = 0;
+  if (g->smoothbot)
+    for (int i = 0; i < g->Nc; i++) {
+      double lo = g->smoothbot * g->dz[g->Nk[i] - 1];
+      g->dzz[i][g->Nk[i] - 1] = g->dzbot[i] > lo ? g->dzbot[i] : lo;
+    }
+  for (int j = 0; j < g->Ne; j++) {
+    int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+    if (a == -1) a = b;
+    if (b == -1) b = a;
+    for (int k = g->etop[j]; k < g->Nke[j]; k++)
+      g->dzf[j][k] = upwind(p->u[j][k], g->dzz[a][k], g->dzz[b][k]);
+    int kb = g->Nke[j] - 1;
+    if (g->etop[j] == kb) {
+      double dmin = g->dv[a] < g->dv[b] ? g->dv[a] : g->dv[b];
+      double v = upwind(p->u[j][kb], p->h[a], p->h[b]) + dmin;
+      g->dzf[j][kb] = v > 0 ? v : 0;
+    } else {
+      g->dzf[j][kb] = g->dzz[a][kb] < g->dzz[b][kb] ? g->dzz[a][kb] : g->dzz[b][kb];
+    }
+    for (int k = g->etop[j]; k < g->Nke[j]; k++)
+      if (g->dzf[j][k] <= DRY_HEIGHT) g->dzf[j][k] = 0;
+  }
+  if (g->smoothbot)
+    for (int i = 0; i < g->Nc; i++) {
+      double lo = g->dzsmall * g->dz[g->Nk[i] - 1];
+      double *bot = &g->dzz[i][g->Nk[i] - 1];
+      if (lo > *bot) *bot = lo;
+    }
+}
+
+void sun_compute_qsource(REAL **src, gridT *g, physT *p, const propT *prop) {
+  const double tf = (1 - prop->theta) / prop->theta;
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) src[i][k] = 0;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    const double Ac = g->Ac[i];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++)
+      src[i][k] = Ac * (p->w[i][k] - p->w[i][k + 1]) + tf * Ac * (p->wtmp2[i][k] - p->wtmp2[i][k + 1]);
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      const int ne = g->face[i * g->maxfaces + nf];
+      const double nrm = g->normal[i * g->maxfaces + nf];
+      for (int k = g->ctop[i]; k < g->Nke[ne]; k++)
+        src[i][k] += (p->u[ne][k] + tf * p->utmp2[ne][k]) * g->dzf[ne][k] * nrm * g->df[ne];
+    }
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) src[i][k] /= prop->dt;
+  }
+  for (int j = 0; j < g->Ne; j++) p->D[j] = g->df[j] / g->dg[j];
+}
+
+/* ------------------------------------------------------------------------ */
+/* flat access                                                               */
+
+static const struct fdesc *find_phys(const char *name) {
+  for (int f = 0; f < NPF; f++)
+    if (!strcmp(phys_fields[f].name, name)) return &phys_fields[f];
+  return NULL;
+}
+
+/* grid-owned real fields */
+static int grid_real(const gridT *g, const char *name, int *kind, REAL ***rows, REAL **vec) {
+  *rows = NULL; *vec = NULL;
+  if (!strcmp(name, "dzz")) { *kind = 1; *rows = g->dzz; }
+  else if (!strcmp(name, "dzzold")) { *kind = 1; *rows = g->dzzold; }
+  else if (!strcmp(name, "dzf")) { *kind = 3; *rows = g->dzf; }
+  else if (!strcmp(name, "Ac")) { *kind = 4; *vec = g->Ac; }
+  else if (!strcmp(name, "dv")) { *kind = 4; *vec = g->dv; }
+  else if (!strcmp(name, "xv")) { *kind = 4; *vec = g->xv; }
+  else if (!strcmp(name, "yv")) { *kind = 4; *vec = g->yv; }
+  else if (!strcmp(name, "df")) { *kind = 5; *vec = g->df; }
+  else if (!strcmp(name, "dg")) { *kind = 5; *vec = g->dg; }
+  else if (!strcmp(name, "n1")) { *kind = 5; *vec = g->n1; }
+  else if (!strcmp(name, "n2")) { *kind = 5; *vec = g->n2; }
+  else if (!strcmp(name, "xe")) { *kind = 5; *vec = g->xe; }
+  else if (!strcmp(name, "ye")) { *kind = 5; *vec = g->ye; }
+  else if (!strcmp(name, "dzfB")) { *kind = 5; *vec = g->dzfB; }
+  else if (!strcmp(name, "dz")) { *kind = 7; *vec = g->dz; }
+  else return 0;
+  return 1;
+}
+
+static long xfer(const gridT *g, int kind, REAL **rows, REAL *vec, double *flat, int to_flat) {
+  size_t n = 0;
+  if (kind <= 3) {
+    const int nr = kind == 3 ? g->Ne : g->Nc;
+    const int *len = kind == 3 ? g->Nkc : g->Nk;
+    const int extra = kind == 2;
+    for (int i = 0; i < nr; i++) {
+      size_t m = (size_t)(len[i] + extra);
+      if (to_flat) memcpy(flat + n, rows[i], m * sizeof(REAL));
+      else memcpy(rows[i], flat + n, m * sizeof(REAL));
+      n += m;
+    }
+  } else {
+    n = kind == 7 ? (size_t)g->Nkmax : kind_len(g, kind);
+    if (to_flat) memcpy(flat, vec, n * sizeof(REAL)); else memcpy(vec, flat, n * sizeof(REAL));
+  }
+  return (long)n;
+}
+
+long sun_field_len(const gridT *g, const char *name) {
+  const struct fdesc *f = find_phys(name);
+  if (f) return (long)kind_len(g, f->kind);
+  if (!strcmp(name, "hfcoef")) return (long)g->maxfaces * g->Nc;
+  int kind; REAL **rows; REAL *vec;
+  if (grid_real(g, name, &kind, &rows, &vec)) return kind == 7 ? g->Nkmax : (long)kind_len(g, kind);
+  return -1;
+}
+
+long sun_field_get(const gridT *g, const physT *p, const char *name, double *out) {
+  const struct fdesc *f = find_phys(name);
+  if (f) {
+    void *slot = (char *)p + f->off;
+    return xfer(g, f->kind, f->kind <= 3 ? *(REAL ***)slot : NULL, f->kind > 3 ? *(REAL **)slot : NULL, out, 1);
+  }
+  if (!strcmp(name, "hfcoef")) {
+    memcpy(out, p->hfcoef, sizeof(REAL) * (size_t)g->maxfaces * g->Nc);
+    return (long)g->maxfaces * g->Nc;
+  }
+  int kind; REAL **rows; REAL *vec;
+  if (grid_real(g, name, &kind, &rows, &vec)) return xfer(g, kind, rows, vec, out, 1);
+  return -1;
+}
+
+long sun_field_set(gridT *g, physT *p, const char *name, const double *in) {
+  const struct fdesc *f = find_phys(name);
+  if (f) {
+    void *slot = (char *)p + f->off;
+    return xfer(g, f->kind, f->kind <= 3 ? *(REAL ***)slot : NULL, f->kind > 3 ? *(REAL **)slot : NULL,
+                (double *)in, 0);
+  }
+  int kind; REAL **rows; REAL *vec;
+  if (grid_real(g, name, &kind, &rows, &vec)) return xfer(g, kind, rows, vec, (double *)in, 0);
+  return -1;
+}
+
+static int *grid_ivec(const gridT *g, const char *name, long *n) {
+  const long Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
+  struct { const char *nm; int *p; long n; } t[] = {
+    {"Nk", g->Nk, Nc}, {"ctop", g->ctop, Nc}, {"ctopold", g->ctopold, Nc}, {"nfaces", g->nfaces, Nc},
+    {"cellp", g->cellp, Nc}, {"neigh", g->neigh, Nc * mf}, {"face", g->face, Nc * mf},
+    {"normal", g->normal, Nc * mf}, {"cells", g->cells, Nc * mf}, {"Nke", g->Nke, Ne},
+    {"Nkc", g->Nkc, Ne}, {"etop", g->etop, Ne}, {"etopold", g->etopold, Ne}, {"mark", g->mark, Ne},
+    {"edgep", g->edgep, Ne}, {"grad", g->grad, 2 * Ne}, {"gradf", g->gradf, 2 * Ne},
+    {"edges", g->edges, 3 * Ne}, {"celldist", g->celldist, 3}, {"edgedist", g->edgedist, 6},
+  };
+  for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
+    if (!strcmp(t[i].nm, name)) { *n = t[i].n; return t[i].p; }
+  return NULL;
+}
+
+long sun_grid_int(const gridT *g, const char *name, int *out) {
+  long n; int *v = grid_ivec(g, name, &n);
+  if (!v) return -1;
+  if (out) memcpy(out, v, sizeof(int) * (size_t)n);
+  return n;
+}
+
+long sun_grid_set_int(gridT *g, const char *name, const int *in) {
+  long n; int *v = grid_ivec(g, name, &n);
+  if (!v) return -1;
+  memcpy(v, in, sizeof(int) * (size_t)n);
+  return n;
+}
+
+void *sun_phys_ptr(const physT *p, const char *name) {
+  const struct fdesc *f = find_phys(name);
+  if (f) return *(void **)((char *)p + f->off);
+  if (!strcmp(name, "hfcoef")) return p->hfcoef;
+  if (!strcmp(name, "qtmp")) return p->qtmp;
+  if (!strcmp(name, "boundary_s")) return p->boundary_s;
+  if (!strcmp(name, "active")) return p->active;
+  return NULL;
+}
+
+void *sun_grid_ptr(const gridT *g, const char *name) {
+  if (!strcmp(name, "dzz")) return g->dzz;
+  if (!strcmp(name, "dzzold")) return g->dzzold;
+  if (!strcmp(name, "dzf")) return g->dzf;
+  return NULL;
+}
+
+size_t sun_prop_size(void) { return sizeof(propT); }
+void sun_prop_copy(const propT *from, propT *to) { memcpy(to, from, sizeof(propT)); }
+
+long sun_grid_scalar(const gridT *g, const char *name) {
+  if (!strcmp(name, "Nc")) return g->Nc;
+  if (!strcmp(name, "Ne")) return g->Ne;
+  if (!strcmp(name, "Np")) return g->Np;
+  if (!strcmp(name, "Nkmax")) return g->Nkmax;
+  if (!strcmp(name, "maxfaces")) return g->maxfaces;
+  if (!strcmp(name, "N")) {
+    long n = 0;
+    for (int i = 0; i < g->Nc; i++) n += g->Nk[i] - g->ctop[i];
+    return n;
+  }
+  if (!strcmp(name, "Ek")) {
+    long n = 0;
+    for (int j = 0; j < g->Ne; j++) {
+      int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+      if (a < 0 || b < 0) continue;
+      int kt = g->ctop[a] > g->ctop[b] ? g->ctop[a] : g->ctop[b];
+      if (g->Nke[j] > kt) n += g->Nke[j] - kt;
+    }
+    return n;
+  }
+  return -1;
+}
+
+#define PROP_D(f) if (!strcmp(name, #f)) { if (set) prop->f = value; return prop->f; }
+#define PROP_I(f) if (!strcmp(name, #f)) { if (set) prop->f = (int)value; return prop->f; }
+static double prop_access(propT *prop, const char *name, double value, int set, int *ok) {
+  *ok = 1;
+  PROP_D(dt) PROP_D(theta) PROP_D(thetaS) PROP_D(thetaB) PROP_D(epsilon) PROP_D(qepsilon)
+  PROP_D(resnorm) PROP_D(grav) PROP_D(beta) PROP_D(kappa_s) PROP_D(kappa_sH) PROP_D(gamma)
+  PROP_D(nu) PROP_D(nu_H) PROP_D(CdB)
+  PROP_I(n) PROP_I(nstart) PROP_I(nsteps) PROP_I(maxiters) PROP_I(qmaxiters) PROP_I(hprecond)
+  PROP_I(qprecond) PROP_I(TVD) PROP_I(horiTVD) PROP_I(vertTVD) PROP_I(TVDsalt) PROP_I(TVDtemp)
+  PROP_I(nonhydrostatic) PROP_I(cgsolver) PROP_I(linearFS) PROP_I(stairstep) PROP_I(wetdry)
+  PROP_I(nonlinear)
+  *ok = 0;
+  return 0;
+}
+#undef PROP_D
+#undef PROP_I
+
+int sun_prop_set(propT *prop, const char *name, double value) {
+  int ok; prop_access(prop, name, value, 1, &ok);
+  return ok ? 0 : -1;
+}
+double sun_prop_get(const propT *prop, const char *name) {
+  int ok; return prop_access((propT *)prop, name, 0, 0, &ok);
+}

@@ -1,0 +1,557 @@
+// sunhip_grid.hip -- grid context: spatial tiling, device metadata, field
+// staging between the flat host layout and the tiled device layout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "sunhip_internal.h"
+
+namespace sunhip {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return -2;
+  }
+  g->bytes += (int64_t)bytes;
+  return 0;
+}
+
+Kind kind_of(const char *name) {
+  static const char *edge[] = {"dzf", "u", "utmp2", "SfHp", "SfHm", "ef0", "ef1"};
+  static const char *edge2[] = {"D", "df", "dg"};
+  static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1", "hx", "hy"};
+  for (auto n : edge) if (!strcmp(name, n)) return EDGE;
+  for (auto n : edge2) if (!strcmp(name, n)) return EDGE2;
+  for (auto n : cell2) if (!strcmp(name, n)) return CELL2;
+  if (name[0] == 'w') return CELLW;
+  return CELL;
+}
+
+int field(sunhip_grid *g, const char *name, Field **out) {
+  auto it = g->fields.find(name);
+  if (it == g->fields.end()) {
+    Field f;
+    f.kind = kind_of(name);
+    switch (f.kind) {
+      case CELL: case CELLW: f.n = (size_t)g->NT; break;
+      case EDGE: f.n = (size_t)g->NE; break;
+      case CELL2: f.n = (size_t)g->NS; break;
+      case EDGE2: f.n = (size_t)g->NES; break;
+    }
+    int rc = dev_alloc(g, (void **)&f.d, f.n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(f.d, 0, f.n * sizeof(double), g->stream));
+    it = g->fields.emplace(name, f).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+}  // namespace sunhip
+
+using namespace sunhip;
+
+sunhip::Geo sunhip_grid::geo() const {
+  Geo G;
+  G.Nt = Nt; G.NS = NS; G.Net = Net; G.NES = NES; G.mf = mf;
+  G.toff = d_toff; G.tdep = d_tdep; G.eoff = d_eoff; G.edep = d_edep;
+  G.nk = d_nk; G.ctop = d_ctop; G.ctopold = d_ctopold; G.comp = d_comp; G.nf = d_nf;
+  G.nbb = d_nbb; G.nbs = d_nbs; G.eb = d_eb; G.es = d_es; G.nrm = d_nrm; G.kr = d_kr; G.Ac = d_Ac;
+  G.e_c1 = d_e_c1; G.e_c2 = d_e_c2; G.e_s1 = d_e_s1; G.e_s2 = d_e_s2;
+  G.e_nke = d_e_nke; G.e_nkc = d_e_nkc; G.e_etop = d_e_etop; G.e_mark = d_e_mark; G.e_comp = d_e_comp;
+  G.df = d_df; G.dg = d_dg;
+  return G;
+}
+
+// ---------------------------------------------------------------------------
+// staging kernels: one block per tile, one thread per lane
+
+__global__ void k_scatter_cols(const double *__restrict__ flat, double *__restrict__ dst,
+                               const int64_t *__restrict__ fstart, const int *__restrict__ ncol,
+                               int extra, const int *__restrict__ toff, const int *__restrict__ tdep) {
+  const int t = blockIdx.x, l = threadIdx.x, s = t * SUNHIP_TILE + l;
+  const int64_t f0 = fstart[s];
+  const int n = f0 < 0 ? 0 : ncol[s] + extra;
+  double *col = dst + toff[t] + l;
+  for (int k = 0; k < tdep[t]; k++) col[(int64_t)k * SUNHIP_TILE] = k < n ? flat[f0 + k] : 0.0;
+}
+
+__global__ void k_gather_cols(double *__restrict__ flat, const double *__restrict__ src,
+                              const int64_t *__restrict__ fstart, const int *__restrict__ ncol, int extra,
+                              const int *__restrict__ toff) {
+  const int t = blockIdx.x, l = threadIdx.x, s = t * SUNHIP_TILE + l;
+  const int64_t f0 = fstart[s];
+  if (f0 < 0) return;
+  const int n = ncol[s] + extra;
+  const double *col = src + toff[t] + l;
+  for (int k = 0; k < n; k++) flat[f0 + k] = col[(int64_t)k * SUNHIP_TILE];
+}
+
+__global__ void k_scatter_2d(const double *__restrict__ flat, double *__restrict__ dst,
+                             const int *__restrict__ slot, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[slot[i]] = flat[i];
+}
+
+__global__ void k_gather_2d(double *__restrict__ flat, const double *__restrict__ src,
+                            const int *__restrict__ slot, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flat[i] = src[slot[i]];
+}
+
+// Per-call horizontal coupling range of every face: [max(ctop_i, ctop_nc), Nke).
+__global__ void k_face_ranges(int NS, int mf, const int *__restrict__ nbs, const int *__restrict__ es,
+                              const int *__restrict__ ctop, const int *__restrict__ nke_e,
+                              int *__restrict__ kr) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= NS) return;
+  for (int f = 0; f < mf; f++) {
+    const int nb = nbs[f * NS + s], e = es[f * NS + s];
+    int v = 0;
+    if (nb >= 0 && e >= 0) {
+      const int kmin = ctop[nb] > ctop[s] ? ctop[nb] : ctop[s];
+      v = kmin | (nke_e[e] << 16);
+    } else if (e >= 0) {
+      v = ctop[s] | (nke_e[e] << 16);
+    }
+    kr[f * NS + s] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+static uint32_t spread16(uint32_t v) {
+  v &= 0xffff;
+  v = (v | (v << 8)) & 0x00ff00ff;
+  v = (v | (v << 4)) & 0x0f0f0f0f;
+  v = (v | (v << 2)) & 0x33333333;
+  v = (v | (v << 1)) & 0x55555555;
+  return v;
+}
+
+template <class T>
+static int upload_vec(sunhip_grid *g, T **d, const std::vector<T> &h) {
+  int rc = dev_alloc(g, (void **)d, h.size() * sizeof(T));
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" {
+
+const char *sunhip_error(void) { return g_err; }
+
+int sunhip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) {
+  *out = nullptr;
+  if (!T || T->Nc <= 0 || T->Ne <= 0 || T->maxfaces <= 0) {
+    set_error("sunhip_grid_create: empty topology");
+    return -1;
+  }
+  if (T->maxfaces > SUNHIP_MAXF) {
+    set_error("sunhip_grid_create: maxfaces=%d exceeds %d", T->maxfaces, SUNHIP_MAXF);
+    return -1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("sunhip_grid_create: no HIP device");
+    return -4;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("sunhip_grid_create: device %d out of range (%d devices)", device, ndev);
+    return -1;
+  }
+  HIPCHK(hipSetDevice(device));
+  sunhip_grid *g = new sunhip_grid();
+  g->device = device;
+  const int Nc = T->Nc, Ne = T->Ne, mf = T->maxfaces;
+  g->Nc = Nc; g->Ne = Ne; g->mf = mf; g->Nkmax = T->Nkmax;
+  g->Nk.assign(T->Nk, T->Nk + Nc);
+  g->Nkc.assign(T->Nkc, T->Nkc + Ne);
+  g->face.assign(T->face, T->face + (size_t)Nc * mf);
+  g->neigh.assign(T->neigh, T->neigh + (size_t)Nc * mf);
+  g->normal.assign(T->normal, T->normal + (size_t)Nc * mf);
+  g->nfaces.assign(T->nfaces, T->nfaces + Nc);
+  g->grad.assign(T->grad, T->grad + 2 * (size_t)Ne);
+  g->mark.assign(T->mark, T->mark + Ne);
+  g->cellp.assign(T->cellp, T->cellp + Nc);
+  g->edgep.assign(T->edgep, T->edgep + Ne);
+  g->celldist.assign(T->celldist, T->celldist + 3);
+  g->edgedist.assign(T->edgedist, T->edgedist + 6);
+  for (int i = 0; i < Nc; i++)
+    if (T->Nk[i] < 1 || T->nfaces[i] < 1 || T->nfaces[i] > mf) {
+      set_error("sunhip_grid_create: cell %d has Nk=%d nfaces=%d", i, T->Nk[i], T->nfaces[i]);
+      delete g;
+      return -1;
+    }
+
+  // --- cell order: Morton curve of the Voronoi points
+  double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+  for (int i = 0; i < Nc; i++) {
+    xmin = std::min(xmin, T->xv[i]); xmax = std::max(xmax, T->xv[i]);
+    ymin = std::min(ymin, T->yv[i]); ymax = std::max(ymax, T->yv[i]);
+  }
+  const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-30);
+  std::vector<uint32_t> key(Nc);
+  for (int i = 0; i < Nc; i++) {
+    uint32_t ix = (uint32_t)std::min(65535.0, std::floor((T->xv[i] - xmin) / span * 65535.0));
+    uint32_t iy = (uint32_t)std::min(65535.0, std::floor((T->yv[i] - ymin) / span * 65535.0));
+    key[i] = spread16(ix) | (spread16(iy) << 1);
+  }
+  std::vector<int> order(Nc);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  g->Nt = (Nc + SUNHIP_TILE - 1) / SUNHIP_TILE;
+  g->NS = g->Nt * SUNHIP_TILE;
+  g->slot_of_cell.assign(Nc, -1);
+  g->cell_of_slot.assign(g->NS, -1);
+  for (int s = 0; s < Nc; s++) { g->slot_of_cell[order[s]] = s; g->cell_of_slot[s] = order[s]; }
+  g->tdep.assign(g->Nt, 1);
+  g->toff.assign(g->Nt + 1, 0);
+  for (int t = 0; t < g->Nt; t++) {
+    int d = 0;
+    for (int l = 0; l < SUNHIP_TILE; l++) {
+      int c = g->cell_of_slot[t * SUNHIP_TILE + l];
+      if (c >= 0) d = std::max(d, T->Nk[c]);
+    }
+    g->tdep[t] = d + 1;
+    g->maxdepth = std::max(g->maxdepth, d);
+  }
+  int64_t acc = 0;
+  for (int t = 0; t < g->Nt; t++) { g->toff[t] = (int)acc; acc += (int64_t)g->tdep[t] * SUNHIP_TILE; }
+  if (acc >= (int64_t)1 << 31) {
+    set_error("sunhip_grid_create: %lld cell-layer slots exceed the 32-bit layout", (long long)acc);
+    delete g;
+    return -1;
+  }
+  g->toff[g->Nt] = (int)acc;
+  g->NT = acc;
+
+  // --- edge order: follow the first adjacent cell's slot
+  std::vector<int64_t> ekey(Ne);
+  for (int j = 0; j < Ne; j++) {
+    int a = T->grad[2 * j], b = T->grad[2 * j + 1];
+    int sa = a >= 0 ? g->slot_of_cell[a] : INT32_MAX, sb = b >= 0 ? g->slot_of_cell[b] : INT32_MAX;
+    ekey[j] = (int64_t)std::min(sa, sb) * 4 + (sa < sb ? 0 : 1);
+  }
+  std::vector<int> eorder(Ne);
+  std::iota(eorder.begin(), eorder.end(), 0);
+  std::stable_sort(eorder.begin(), eorder.end(), [&](int a, int b) { return ekey[a] < ekey[b]; });
+  g->Net = (Ne + SUNHIP_TILE - 1) / SUNHIP_TILE;
+  g->NES = g->Net * SUNHIP_TILE;
+  g->slot_of_edge.assign(Ne, -1);
+  g->edge_of_slot.assign(g->NES, -1);
+  for (int s = 0; s < Ne; s++) { g->slot_of_edge[eorder[s]] = s; g->edge_of_slot[s] = eorder[s]; }
+  g->edep.assign(g->Net, 1);
+  g->eoff.assign(g->Net + 1, 0);
+  acc = 0;
+  for (int t = 0; t < g->Net; t++) {
+    int d = 1;
+    for (int l = 0; l < SUNHIP_TILE; l++) {
+      int e = g->edge_of_slot[t * SUNHIP_TILE + l];
+      if (e >= 0) d = std::max(d, T->Nkc[e]);
+    }
+    g->edep[t] = d;
+    g->eoff[t] = (int)acc;
+    acc += (int64_t)d * SUNHIP_TILE;
+  }
+  if (acc >= (int64_t)1 << 31) {
+    set_error("sunhip_grid_create: edge layout exceeds 32 bits");
+    delete g;
+    return -1;
+  }
+  g->eoff[g->Net] = (int)acc;
+  g->NE = acc;
+
+  // --- flat offsets
+  g->cstart.assign(Nc + 1, 0);
+  for (int i = 0; i < Nc; i++) g->cstart[i + 1] = g->cstart[i] + T->Nk[i];
+  g->Nflat = g->cstart[Nc];
+  g->NflatW = g->Nflat + Nc;
+  g->estart.assign(Ne + 1, 0);
+  for (int j = 0; j < Ne; j++) g->estart[j + 1] = g->estart[j] + T->Nkc[j];
+  g->NflatE = g->estart[Ne];
+
+  // --- per-slot metadata
+  const int NS = g->NS, NES = g->NES;
+  auto cell_elem = [&](int c) { int s = g->slot_of_cell[c]; return g->toff[s / 64] + (s % 64); };
+  auto edge_elem = [&](int e) { int s = g->slot_of_edge[e]; return g->eoff[s / 64] + (s % 64); };
+  std::vector<int> nk(NS, 0), nf(NS, 0), comp(NS, 0), nbb(mf * NS, -1), nbs(mf * NS, -1), eb(mf * NS, -1),
+      es(mf * NS, -1), nrm(mf * NS, 0);
+  std::vector<double> Ac(NS, 0.0);
+  std::vector<int64_t> cst(NS, -1), cstw(NS, -1);
+  for (int s = 0; s < NS; s++) {
+    int c = g->cell_of_slot[s];
+    if (c < 0) continue;
+    nk[s] = T->Nk[c];
+    nf[s] = T->nfaces[c];
+    Ac[s] = T->Ac[c];
+    cst[s] = g->cstart[c];
+    cstw[s] = g->cstart[c] + c;
+    for (int f = 0; f < T->nfaces[c]; f++) {
+      int nb = T->neigh[c * mf + f], e = T->face[c * mf + f];
+      if (nb >= 0) { nbb[f * NS + s] = cell_elem(nb); nbs[f * NS + s] = g->slot_of_cell[nb]; }
+      if (e >= 0) { eb[f * NS + s] = edge_elem(e); es[f * NS + s] = g->slot_of_edge[e]; }
+      nrm[f * NS + s] = T->normal[c * mf + f];
+    }
+  }
+  for (int ip = T->celldist[0]; ip < T->celldist[1]; ip++) comp[g->slot_of_cell[T->cellp[ip]]] = 1;
+  for (int ip = T->celldist[1]; ip < T->celldist[2]; ip++) comp[g->slot_of_cell[T->cellp[ip]]] = 2;
+  std::vector<int> ec1(NES, -1), ec2(NES, -1), es1(NES, -1), es2(NES, -1), enkc(NES, 0), emark(NES, 0),
+      ecomp(NES, 0);
+  std::vector<double> df(NES, 0.0), dg(NES, 0.0);
+  std::vector<int64_t> est(NES, -1);
+  for (int s = 0; s < NES; s++) {
+    int e = g->edge_of_slot[s];
+    if (e < 0) continue;
+    int a = T->grad[2 * e], b = T->grad[2 * e + 1];
+    if (a >= 0) { ec1[s] = cell_elem(a); es1[s] = g->slot_of_cell[a]; }
+    if (b >= 0) { ec2[s] = cell_elem(b); es2[s] = g->slot_of_cell[b]; }
+    enkc[s] = T->Nkc[e];
+    emark[s] = T->mark[e];
+    df[s] = T->df[e];
+    dg[s] = T->dg[e];
+    est[s] = g->estart[e];
+  }
+  for (int jp = T->edgedist[0]; jp < T->edgedist[1]; jp++) ecomp[g->slot_of_edge[T->edgep[jp]]] = 1;
+  for (int jp = T->edgedist[2]; jp < T->edgedist[3]; jp++) ecomp[g->slot_of_edge[T->edgep[jp]]] = 2;
+
+  int rc = 0;
+  HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  rc |= upload_vec(g, &g->d_toff, g->toff);
+  rc |= upload_vec(g, &g->d_tdep, g->tdep);
+  rc |= upload_vec(g, &g->d_eoff, g->eoff);
+  rc |= upload_vec(g, &g->d_edep, g->edep);
+  rc |= upload_vec(g, &g->d_nk, nk);
+  rc |= upload_vec(g, &g->d_nf, nf);
+  rc |= upload_vec(g, &g->d_comp, comp);
+  rc |= upload_vec(g, &g->d_nbb, nbb);
+  rc |= upload_vec(g, &g->d_nbs, nbs);
+  rc |= upload_vec(g, &g->d_eb, eb);
+  rc |= upload_vec(g, &g->d_es, es);
+  rc |= upload_vec(g, &g->d_nrm, nrm);
+  rc |= upload_vec(g, &g->d_Ac, Ac);
+  rc |= upload_vec(g, &g->d_cstart, cst);
+  rc |= upload_vec(g, &g->d_cstartw, cstw);
+  rc |= upload_vec(g, &g->d_e_c1, ec1);
+  rc |= upload_vec(g, &g->d_e_c2, ec2);
+  rc |= upload_vec(g, &g->d_e_s1, es1);
+  rc |= upload_vec(g, &g->d_e_s2, es2);
+  rc |= upload_vec(g, &g->d_e_nkc, enkc);
+  rc |= upload_vec(g, &g->d_e_mark, emark);
+  rc |= upload_vec(g, &g->d_e_comp, ecomp);
+  rc |= upload_vec(g, &g->d_df, df);
+  rc |= upload_vec(g, &g->d_dg, dg);
+  rc |= upload_vec(g, &g->d_estart, est);
+  {
+    std::vector<int> cs(g->slot_of_cell), esv(g->slot_of_edge);
+    rc |= upload_vec(g, &g->d_cellslot, cs);
+    rc |= upload_vec(g, &g->d_edgeslot, esv);
+  }
+  std::vector<int> zeros(NS, 0), ezeros(NES, 0), kr0(mf * NS, 0);
+  rc |= upload_vec(g, &g->d_ctop, zeros);
+  rc |= upload_vec(g, &g->d_ctopold, zeros);
+  rc |= upload_vec(g, &g->d_kr, kr0);
+  rc |= upload_vec(g, &g->d_e_nke, ezeros);
+  rc |= upload_vec(g, &g->d_e_etop, ezeros);
+  // reductions: one partial pair per block of the widest launch
+  const int nblk = std::max(g->Nt, g->Net) + 64;
+  rc |= dev_alloc(g, (void **)&g->d_part, sizeof(double) * 4 * (size_t)nblk);
+  rc |= dev_alloc(g, (void **)&g->d_count, sizeof(unsigned) * 64);
+  rc |= dev_alloc(g, (void **)&g->d_qs, sizeof(CGScal));
+  rc |= dev_alloc(g, (void **)&g->d_hs, sizeof(CGScal));
+  if (rc) { sunhip_grid_destroy(g); return rc; }
+  HIPCHK(hipMemset(g->d_count, 0, sizeof(unsigned) * 64));
+  HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CGScal), hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&g->ev0));
+  HIPCHK(hipEventCreate(&g->ev1));
+  HIPCHK(hipEventCreate(&g->ev2));
+  *out = g;
+  return 0;
+}
+
+int sunhip_grid_destroy(sunhip_grid *g) {
+  if (!g) return 0;
+  hipSetDevice(g->device);
+  if (g->stream) hipStreamSynchronize(g->stream);
+  for (auto &kv : g->fields) hipFree(kv.second.d);
+  void *ptrs[] = {g->d_toff, g->d_tdep, g->d_eoff, g->d_edep, g->d_nk, g->d_ctop, g->d_ctopold, g->d_comp,
+                  g->d_nf, g->d_nbb, g->d_nbs, g->d_eb, g->d_es, g->d_nrm, g->d_kr, g->d_Ac, g->d_df,
+                  g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
+                  g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
+                  g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs};
+  for (void *p : ptrs) if (p) hipFree(p);
+  if (g->h_stage) hipHostFree(g->h_stage);
+  if (g->h_sc) hipHostFree(g->h_sc);
+  if (g->ev0) hipEventDestroy(g->ev0);
+  if (g->ev1) hipEventDestroy(g->ev1);
+  if (g->ev2) hipEventDestroy(g->ev2);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+  return 0;
+}
+
+int64_t sunhip_grid_bytes(const sunhip_grid *g) { return g ? g->bytes : 0; }
+
+int sunhip_grid_sizes(const sunhip_grid *g, int64_t *o) {
+  o[0] = g->Nflat; o[1] = g->NflatW; o[2] = g->NflatE; o[3] = g->NT; o[4] = g->Nt; o[5] = g->NE;
+  return 0;
+}
+
+int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
+  HIPCHK(hipSetDevice(g->device));
+  const int NS = g->NS, NES = g->NES;
+  std::vector<int> ct(NS, 0), cto(NS, 0), nke(NES, 0), et(NES, 0);
+  for (int s = 0; s < g->Nc; s++) {
+    int c = g->cell_of_slot[s];
+    ct[s] = X->ctop[c];
+    cto[s] = X->ctopold ? X->ctopold[c] : X->ctop[c];
+    if (ct[s] < 0 || ct[s] > g->Nk[c]) {
+      set_error("sunhip_set_extents: cell %d has ctop=%d outside [0,Nk=%d]", c, ct[s], g->Nk[c]);
+      return -1;
+    }
+  }
+  for (int s = 0; s < g->Ne; s++) {
+    int e = g->edge_of_slot[s];
+    nke[s] = X->Nke[e];
+    et[s] = X->etop ? X->etop[e] : 0;
+    if (nke[s] < 0 || nke[s] > g->Nkc[e]) {
+      set_error("sunhip_set_extents: edge %d has Nke=%d outside [0,Nkc=%d]", e, nke[s], g->Nkc[e]);
+      return -1;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(g->d_ctop, ct.data(), NS * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_ctopold, cto.data(), NS * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_e_nke, nke.data(), NES * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->d_e_etop, et.data(), NES * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  k_face_ranges<<<(NS + 255) / 256, 256, 0, g->stream>>>(NS, g->mf, g->d_nbs, g->d_es, g->d_ctop, g->d_e_nke,
+                                                          g->d_kr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g->stream));  // the host vectors go out of scope
+  g->have_extents = true;
+  return 0;
+}
+
+static int ensure_stage(sunhip_grid *g, size_t n) {
+  if (g->stage_n < n) {
+    if (g->d_stage) { hipFree(g->d_stage); g->bytes -= (int64_t)(g->stage_n * sizeof(double)); }
+    g->d_stage = nullptr;
+    g->stage_n = 0;
+    int rc = dev_alloc(g, (void **)&g->d_stage, n * sizeof(double));
+    if (rc) return rc;
+    g->stage_n = n;
+  }
+  return 0;
+}
+
+static size_t flat_len(const sunhip_grid *g, Kind k) {
+  switch (k) {
+    case CELL: return (size_t)g->Nflat;
+    case CELLW: return (size_t)g->NflatW;
+    case EDGE: return (size_t)g->NflatE;
+    case CELL2: return (size_t)g->Nc;
+    case EDGE2: return (size_t)g->Ne;
+  }
+  return 0;
+}
+
+int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  const size_t n = flat_len(g, f->kind);
+  if ((rc = ensure_stage(g, n))) return rc;
+  HIPCHK(hipMemcpyAsync(g->d_stage, flat, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  switch (f->kind) {
+    case CELL: case CELLW:
+      k_scatter_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
+                                                           g->d_nk, f->kind == CELLW, g->d_toff, g->d_tdep);
+      break;
+    case EDGE:
+      k_scatter_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff,
+                                                            g->d_edep);
+      break;
+    case CELL2:
+      k_scatter_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_cellslot, g->Nc);
+      break;
+    case EDGE2:
+      k_scatter_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
+      break;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g->stream));  // the caller may reuse its buffer
+  return 0;
+}
+
+int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  const size_t n = flat_len(g, f->kind);
+  if ((rc = ensure_stage(g, n))) return rc;
+  switch (f->kind) {
+    case CELL: case CELLW:
+      k_gather_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
+                                                          g->d_nk, f->kind == CELLW, g->d_toff);
+      break;
+    case EDGE:
+      k_gather_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff);
+      break;
+    case CELL2:
+      k_gather_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_cellslot, g->Nc);
+      break;
+    case EDGE2:
+      k_gather_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
+      break;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(flat, g->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *a, *b;
+  int rc = field(g, dst, &a);
+  if (rc) return rc;
+  if ((rc = field(g, src, &b))) return rc;
+  if (a->n != b->n) {
+    set_error("sunhip_copy_field: %s and %s differ in size", dst, src);
+    return -1;
+  }
+  HIPCHK(hipMemcpyAsync(a->d, b->d, a->n * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+  return 0;
+}
+
+int sunhip_sync(sunhip_grid *g) {
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// sunhip_internal.h -- device context and layout shared by the sunhip kernels.
+//
+// Layout.  Cells are ordered along a Morton curve of their Voronoi points and
+// cut into tiles of 64 consecutive cells, one wavefront each, one lane per
+// column.  A tile stores level k of its 64 columns contiguously; its depth is
+// the deepest column (+1 so w-type fields with Nk+1 values fit), so the value
+// of cell slot s = 64*t + lane at level k lives at toff[t] + 64*k + lane.
+// Edges get the same treatment with their own tiles.  Every horizontal gather
+// (neighbour column, face edge) is then a 64-lane access at one level, and
+// the vertical coupling of a column stays in the lane's registers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sunhip.h"
+
+#define SUNHIP_MAXF 4   // faces per cell supported (triangles 3, quadrilaterals 4)
+#define SUNHIP_TILE 64
+
+namespace sunhip {
+
+void set_error(const char *fmt, ...);
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::sunhip::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                    \
+      return -3;                                                                  \
+    }                                                                             \
+  } while (0)
+
+// Kinds of named fields.
+enum Kind { CELL = 1, CELLW = 2, EDGE = 3, CELL2 = 4, EDGE2 = 5 };
+
+struct Field {
+  Kind kind;
+  double *d = nullptr;
+  size_t n = 0;  // device doubles
+};
+
+// Device-resident CG state (one per solver kind).
+struct CGScal {
+  double alpha, alpha0, eps, eps0, nu, mu, muinv, pz, tol, resid;
+  int iter;       // loop iterations completed
+  int nloop;      // the reference's loop counter n at exit
+  int maxit, done, converged, precond, resnorm, pad;
+};
+
+// Per-call geometry the kernels read, gathered into one argument block.
+struct Geo {
+  int Nt, NS, Net, NES, mf;
+  const int *toff;     // [Nt+1] element offset of each cell tile
+  const int *tdep;     // [Nt] tile depth (levels stored)
+  const int *eoff;     // [Net+1]
+  const int *edep;     // [Net]
+  const int *nk;       // [NS] Nk per slot (0 = padding)
+  const int *ctop;     // [NS]
+  const int *ctopold;  // [NS]
+  const int *comp;     // [NS] 1 computational (celldist[0..1)), 2 type-3 boundary, 0 none
+  const int *nf;       // [NS]
+  const int *nbb;      // [MAXF*NS] neighbour element base, -1 none
+  const int *nbs;      // [MAXF*NS] neighbour slot, -1 none
+  const int *eb;       // [MAXF*NS] face edge element base
+  const int *es;       // [MAXF*NS] face edge slot
+  const int *nrm;      // [MAXF*NS] normal sign
+  const int *kr;       // [MAXF*NS] kmin | kmax << 16 for the horizontal coupling
+  const double *Ac;    // [NS]
+  // edges, by edge slot
+  const int *e_c1, *e_c2;  // element base of grad[0], grad[1] (-1 none)
+  const int *e_s1, *e_s2;  // slots of grad[0], grad[1] (-1 none)
+  const int *e_nke, *e_nkc, *e_etop, *e_mark, *e_comp;  // e_comp: 1 for edgedist[0..1)
+  const double *df, *dg;
+};
+
+}  // namespace sunhip
+
+struct sunhip_grid {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int Nc = 0, Ne = 0, mf = 0, Nkmax = 0;
+  int Nt = 0, Net = 0, NS = 0, NES = 0;
+  int64_t NT = 0, NE = 0;           // padded element counts (cells, edges)
+  int64_t Nflat = 0, NflatW = 0, NflatE = 0;
+  int maxdepth = 0;                 // deepest tile
+  std::vector<int> slot_of_cell, cell_of_slot, slot_of_edge, edge_of_slot;
+  std::vector<int> toff, tdep, eoff, edep;
+  std::vector<int> Nk, Nkc;         // host copies
+  std::vector<int64_t> cstart, estart;  // flat offsets per cell / edge
+  std::vector<int> face, neigh, normal, nfaces, grad, mark, cellp, edgep, celldist, edgedist;
+  // device metadata
+  int *d_toff = nullptr, *d_tdep = nullptr, *d_eoff = nullptr, *d_edep = nullptr;
+  int *d_nk = nullptr, *d_ctop = nullptr, *d_ctopold = nullptr, *d_comp = nullptr, *d_nf = nullptr;
+  int *d_nbb = nullptr, *d_nbs = nullptr, *d_eb = nullptr, *d_es = nullptr, *d_nrm = nullptr, *d_kr = nullptr;
+  double *d_Ac = nullptr, *d_df = nullptr, *d_dg = nullptr;
+  int *d_e_c1 = nullptr, *d_e_c2 = nullptr, *d_e_s1 = nullptr, *d_e_s2 = nullptr;
+  int *d_e_nke = nullptr, *d_e_nkc = nullptr, *d_e_etop = nullptr, *d_e_mark = nullptr, *d_e_comp = nullptr;
+  int64_t *d_cstart = nullptr, *d_cstartw = nullptr, *d_estart = nullptr;  // per slot flat start, -1 padding
+  int *d_cellslot = nullptr, *d_edgeslot = nullptr;  // host index -> slot
+  double *d_stage = nullptr;        // flat staging buffer (device)
+  size_t stage_n = 0;
+  double *h_stage = nullptr;        // pinned host staging
+  size_t h_stage_n = 0;
+  bool have_extents = false;
+  // reductions
+  double *d_part = nullptr;         // block partials
+  unsigned *d_count = nullptr;      // last-block tickets (one per reduction site)
+  sunhip::CGScal *d_qs = nullptr, *d_hs = nullptr;
+  sunhip::CGScal *h_sc = nullptr;   // pinned mirror
+  std::map<std::string, sunhip::Field> fields;
+  int64_t bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+
+  sunhip::Geo geo() const;
+};
+
+namespace sunhip {
+int field(sunhip_grid *g, const char *name, Field **out);  // create on demand
+Kind kind_of(const char *name);
+int dev_alloc(sunhip_grid *g, void **p, size_t bytes);
+}  // namespace sunhip

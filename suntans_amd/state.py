@@ -1,0 +1,187 @@
+"""Python handle on a SUNTANS gridT/physT/propT triple built by libsunstate.
+
+The structs live in C memory with the reference's layout, so the same
+pointers can be handed to the drop-in solvers (libsuntans.so) and to any
+checker.  Fields move in and out as flat numpy arrays in the column-compact
+order described in include/sunstate.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+
+
+def _bind(L):
+    if getattr(L, "_state_py_bound", False):
+        return
+    L.sun_phys_ptr.restype = C.c_void_p
+    L.sun_phys_ptr.argtypes = [C.c_void_p, C.c_char_p]
+    L.sun_grid_ptr.restype = C.c_void_p
+    L.sun_grid_ptr.argtypes = [C.c_void_p, C.c_char_p]
+    L.sun_prop_copy.argtypes = [C.c_void_p, C.c_void_p]
+    L._state_py_bound = True
+
+
+class State:
+    """One grid with its physical state and run properties."""
+
+    def __init__(self, nx: int, ny: int, Lx: float, Ly: float, Nkmax: int, dz, depth,
+                 stairstep: int = 0):
+        L = native.state_lib()
+        _bind(L)
+        self._L = L
+        self.g = L.sun_grid_tri(int(nx), int(ny), float(Lx), float(Ly))
+        if not self.g:
+            raise RuntimeError("sun_grid_tri failed")
+        self.nx, self.ny, self.Lx, self.Ly = nx, ny, Lx, Ly
+        self.p = None
+        self.prop = None
+        self._owns_grid = True
+        dz = np.ascontiguousarray(np.broadcast_to(np.asarray(dz, np.float64), (Nkmax,)))
+        Nc = 2 * nx * ny
+        if callable(depth):
+            depth = depth(self.get("xv"), self.get("yv"))
+        dv = np.ascontiguousarray(np.broadcast_to(np.asarray(depth, np.float64), (Nc,)))
+        if L.sun_grid_vertical(self.g, int(Nkmax), native.dptr(dz), native.dptr(dv), int(stairstep)):
+            raise RuntimeError("sun_grid_vertical failed")
+        self.p = L.sun_phys_alloc(self.g)
+        self.prop = L.sun_prop_default()
+        self._owns_grid = True
+
+    # -- lifetime ---------------------------------------------------------
+    def clone(self) -> "State":
+        """A second physT and propT on the same grid, copies of this one's."""
+        other = State.__new__(State)
+        other._L = self._L
+        other.g = self.g
+        other.nx, other.ny, other.Lx, other.Ly = self.nx, self.ny, self.Lx, self.Ly
+        other.p = self._L.sun_phys_clone(self.g, self.p)
+        other.prop = self._L.sun_prop_default()
+        self._L.sun_prop_copy(self.prop, other.prop)
+        other._owns_grid = False
+        other._parent = self
+        return other
+
+    def copy_phys_from(self, other: "State") -> None:
+        self._L.sun_phys_copy(self.g, other.p, self.p)
+        self._L.sun_prop_copy(other.prop, self.prop)
+
+    def __del__(self):
+        try:
+            L = self._L
+            if getattr(self, "p", None):
+                L.sun_phys_free(self.g, self.p)
+            if getattr(self, "prop", None):
+                L.sun_prop_free(self.prop)
+            if getattr(self, "_owns_grid", False) and getattr(self, "g", None):
+                L.sun_grid_free(self.g)
+        except Exception:
+            pass
+
+    # -- raw members for calls with the reference's signatures ----------------
+    def ptr(self, name: str) -> int:
+        """physT member (row table) by name, or gridT dzz/dzzold/dzf."""
+        v = self._L.sun_phys_ptr(self.p, name.encode())
+        if not v:
+            v = self._L.sun_grid_ptr(self.g, name.encode())
+        if not v:
+            raise KeyError(name)
+        return v
+
+    # -- scalars / props ----------------------------------------------------
+    def scalar(self, name: str) -> int:
+        v = self._L.sun_grid_scalar(self.g, name.encode())
+        if v < 0:
+            raise KeyError(name)
+        return int(v)
+
+    @property
+    def Nc(self) -> int:
+        return self.scalar("Nc")
+
+    @property
+    def Ne(self) -> int:
+        return self.scalar("Ne")
+
+    @property
+    def N(self) -> int:
+        return self.scalar("N")
+
+    def set_prop(self, **kw) -> None:
+        for k, v in kw.items():
+            if self._L.sun_prop_set(self.prop, k.encode(), float(v)) != 0:
+                raise KeyError(k)
+
+    def get_prop(self, name: str) -> float:
+        return float(self._L.sun_prop_get(self.prop, name.encode()))
+
+    # -- fields ---------------------------------------------------------------
+    def _len(self, name: str) -> int:
+        n = self._L.sun_field_len(self.g, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        return int(n)
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.empty(self._len(name), np.float64)
+        if self._L.sun_field_get(self.g, self.p, name.encode(), native.dptr(out)) < 0:
+            raise KeyError(name)
+        return out
+
+    def set(self, name: str, values) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), (self._len(name),)))
+        if self._L.sun_field_set(self.g, self.p, name.encode(), native.dptr(a)) < 0:
+            raise KeyError(name)
+
+    def ints(self, name: str) -> np.ndarray:
+        n = self._L.sun_grid_int(self.g, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, np.int32)
+        self._L.sun_grid_int(self.g, name.encode(), native.iptr(out))
+        return out
+
+    def set_ints(self, name: str, values) -> None:
+        n = self._L.sun_grid_int(self.g, name.encode(), None)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.int32), (n,)))
+        self._L.sun_grid_set_int(self.g, name.encode(), native.iptr(a))
+
+    # -- index helpers --------------------------------------------------------
+    def cell_layers(self, extra: int = 0):
+        """(cell index, k) of every value of a flat cell field."""
+        nk = self.ints("Nk").astype(np.int64) + extra
+        cell = np.repeat(np.arange(len(nk)), nk)
+        start = np.concatenate([[0], np.cumsum(nk)[:-1]])
+        k = np.arange(int(nk.sum())) - np.repeat(start, nk)
+        return cell, k
+
+    def edge_layers(self):
+        nkc = self.ints("Nkc").astype(np.int64)
+        edge = np.repeat(np.arange(len(nkc)), nkc)
+        start = np.concatenate([[0], np.cumsum(nkc)[:-1]])
+        k = np.arange(int(nkc.sum())) - np.repeat(start, nkc)
+        return edge, k
+
+    def wet_mask(self) -> np.ndarray:
+        """Flat cell-field mask of the wet layers [ctop, Nk)."""
+        cell, k = self.cell_layers()
+        return k >= self.ints("ctop")[cell]
+
+    def edge_wet_mask(self) -> np.ndarray:
+        """Flat edge-field mask of the layers [etop, Nke)."""
+        edge, k = self.edge_layers()
+        return (k >= self.ints("etop")[edge]) & (k < self.ints("Nke")[edge])
+
+    # -- reference set-up steps -----------------------------------------------
+    def update_dz(self, option: int) -> None:
+        self._L.sun_update_dz(self.g, self.p, self.prop, int(option))
+
+    def set_flux_height(self) -> None:
+        self._L.sun_set_flux_height(self.g, self.p, self.prop)
+
+    def compute_qsource(self) -> None:
+        """stmp <- ComputeQSource(u, w, utmp2, wtmp2); D <- df/dg."""
+        self._L.sun_compute_qsource(self.ptr("stmp"), self.g, self.p, self.prop)

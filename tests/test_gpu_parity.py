@@ -1,0 +1,174 @@
+"""HIP path (through the reference-signature drop-ins in libsuntans.so) against
+the oracle, on the same seeded inputs.
+
+Tolerances:
+* OperatorQ, Preconditioner, OperatorH, HorizontalFaceScalars, UpdateScalars:
+  bit-identical (element-wise arithmetic keeps the reference's operation order
+  and is built without FMA contraction);
+* CGSolveQ / CGSolve converged: relative 2-norm difference <= 1e-10 and the
+  reference's iteration counter within +-1 (the device sums the dot products
+  in a different order, so iterates differ by rounding);
+* a fixed number of PCG iterations: relative 2-norm difference <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from suntans_amd import cases
+
+import _impls
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = [0, 1, 2, 3]
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _pair(make):
+    a = make()
+    return a, a.clone()
+
+
+def _lock():
+    return cases.lock_exchange(seed=7)
+
+
+def _makers():
+    out = [(f"tiny{s}", (lambda s=s: cases.tiny(s))) for s in SEEDS]
+    out.append(("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)))
+    out.append(("lockexchange", _lock))
+    return out
+
+
+MAKERS = _makers()
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_operator_q_bitwise(gpu_available, name, make):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=a.get("stmp").shape)
+    for st in (a, b):
+        o.qcoef(st)               # coefficients into wtmp (same bits for both)
+        st.set("stmp", x)
+        st.set("stmp2", 0.0)
+    o.operatorq(a)
+    d.operatorq(b)
+    m = a.wet_mask()
+    ya, yb = a.get("stmp2")[m], b.get("stmp2")[m]
+    assert np.array_equal(ya, yb), f"max diff {np.abs(ya - yb).max()}"
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_preconditioner_bitwise(gpu_available, name, make):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=a.get("stmp").shape)
+    for st in (a, b):
+        o.qcoef(st)
+        st.set("stmp", x)
+        st.set("uold", 0.0)
+    o.preconditioner(a)
+    d.preconditioner(b)
+    m = a.wet_mask()
+    ua, ub = a.get("uold")[m], b.get("uold")[m]
+    assert np.array_equal(ua, ub), f"max diff {np.abs(ua - ub).max()}"
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_operator_h_bitwise(gpu_available, name, make):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=a.Nc)
+    for st in (a, b):
+        st.set("D", cases.depth_integrated_D(st))
+        o.hcoef(st)
+        st.set("htmp2", x)
+        st.set("htmp3", 0.0)
+    o.operatorh(a)
+    d.operatorh(b)
+    assert np.array_equal(a.get("htmp3"), b.get("htmp3"))
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+@pytest.mark.parametrize("qprecond", [2, 0])
+def test_cgsolveq_converged(gpu_available, name, make, qprecond):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    for st in (a, b):
+        st.set_prop(qprecond=qprecond, qepsilon=1e-10, qmaxiters=2000, resnorm=0)
+    o.cgsolveq(a)
+    d.cgsolveq(b)
+    no, _ = o.stats("q")
+    nd, resid = d.stats("q")
+    m = a.wet_mask()
+    err = relerr(b.get("qc")[m], a.get("qc")[m])
+    print(f"{name} qprecond={qprecond}: iters oracle={no} device={nd} rel err={err:.3e}")
+    assert abs(no - nd) <= 1
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("name,make", MAKERS[-2:], ids=[m[0] for m in MAKERS[-2:]])
+def test_cgsolveq_fixed_iterations(gpu_available, name, make):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    for st in (a, b):
+        st.set_prop(qprecond=2, qepsilon=0.0, qmaxiters=6)
+    o.cgsolveq(a)
+    d.cgsolveq(b)
+    m = a.wet_mask()
+    err = relerr(b.get("qc")[m], a.get("qc")[m])
+    assert o.stats("q")[0] == d.stats("q")[0] == 6
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+@pytest.mark.parametrize("hprecond", [1, 0])
+def test_cgsolve_converged(gpu_available, name, make, hprecond):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    for st in (a, b):
+        st.set("D", cases.depth_integrated_D(st))
+        st.set_prop(hprecond=hprecond, epsilon=1e-10, maxiters=1000)
+    o.cgsolve(a)
+    d.cgsolve(b)
+    no, _ = o.stats("h")
+    nd, _ = d.stats("h")
+    err = relerr(b.get("h"), a.get("h"))
+    print(f"{name} hprecond={hprecond}: iters oracle={no} device={nd} rel err={err:.3e}")
+    assert abs(no - nd) <= 1
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+@pytest.mark.parametrize("tvd", [1, 2, 3, 4])
+def test_face_scalars_bitwise(gpu_available, name, make, tvd):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    o.face_scalars(a, tvd)
+    d.face_scalars(b, tvd)
+    em = a.edge_wet_mask()
+    for f in ("SfHp", "SfHm"):
+        assert np.array_equal(a.get(f)[em], b.get(f)[em]), f
+    m = a.wet_mask()
+    for f in ("gradSx", "gradSy"):
+        assert np.array_equal(a.get(f)[m], b.get(f)[m]), f
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+@pytest.mark.parametrize("tvd", [0, 1, 3, 4])
+def test_update_scalars_bitwise(gpu_available, name, make, tvd):
+    o, d = _impls.oracle(), _impls.dropin()
+    a, b = _pair(make)
+    for st in (a, b):
+        st.set_prop(TVDsalt=tvd, n=2)
+    o.update_salinity(a)
+    d.update_salinity(b)
+    sa, sb = a.get("s"), b.get("s")
+    assert np.array_equal(sa, sb), f"max diff {np.abs(sa - sb).max()}"
+    assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))
