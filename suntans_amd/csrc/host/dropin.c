@@ -9,6 +9,7 @@
  * (ctop, ctopold, Nke, etop) and all field data are refreshed on every call,
  * since the caller may have changed them between calls.
  */
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -20,6 +21,7 @@
 
 typedef struct {
   const gridT *key;
+  uint64_t fp;
   int Nc, Ne;
   sunhip_grid *dev;
   double *flat;      /* host packing buffer */
@@ -40,11 +42,56 @@ static void check(int rc, const char *where) {
   if (rc) fatal(where, sunhip_error());
 }
 
+/* FNV-1a over the static topology and metrics: a gridT freed and another
+ * allocated at the same address must not reuse the old device copy. */
+static uint64_t fnv(uint64_t h, const void *p, size_t n) {
+  const unsigned char *b = (const unsigned char *)p;
+  for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+static uint64_t fingerprint(const gridT *g) {
+  uint64_t h = 1469598103934665603ull;
+  const size_t nc = (size_t)g->Nc, ne = (size_t)g->Ne, nf = (size_t)g->maxfaces * nc;
+  h = fnv(h, &g->Nc, sizeof(int));
+  h = fnv(h, &g->Ne, sizeof(int));
+  h = fnv(h, &g->maxfaces, sizeof(int));
+  h = fnv(h, g->Nk, nc * sizeof(int));
+  h = fnv(h, g->Nkc, ne * sizeof(int));
+  h = fnv(h, g->nfaces, nc * sizeof(int));
+  h = fnv(h, g->neigh, nf * sizeof(int));
+  h = fnv(h, g->face, nf * sizeof(int));
+  h = fnv(h, g->grad, 2 * ne * sizeof(int));
+  h = fnv(h, g->cellp, nc * sizeof(int));
+  h = fnv(h, g->celldist, 3 * sizeof(int));
+  h = fnv(h, g->edgedist, 6 * sizeof(int));
+  h = fnv(h, g->Ac, nc * sizeof(double));
+  h = fnv(h, g->df, ne * sizeof(double));
+  h = fnv(h, g->dg, ne * sizeof(double));
+  return h;
+}
+
 static ctx_t *get_ctx(gridT *g, const char *where) {
+  const uint64_t fp = fingerprint(g);
   for (int i = 0; i < g_nctx; i++)
-    if (g_ctx[i].key == g && g_ctx[i].Nc == g->Nc && g_ctx[i].Ne == g->Ne) return &g_ctx[i];
-  if (g_nctx == MAXCTX) fatal(where, "too many grids; call sunhip_dropin_reset()");
+    if (g_ctx[i].key == g && g_ctx[i].fp == fp) return &g_ctx[i];
+  /* same address, different grid: drop the stale copy */
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i].key == g) {
+      sunhip_grid_destroy(g_ctx[i].dev);
+      free(g_ctx[i].flat);
+      g_ctx[i] = g_ctx[g_nctx - 1];
+      g_nctx--;
+      break;
+    }
+  if (g_nctx == MAXCTX) {
+    /* evict the oldest */
+    sunhip_grid_destroy(g_ctx[0].dev);
+    free(g_ctx[0].flat);
+    memmove(g_ctx, g_ctx + 1, sizeof(ctx_t) * (MAXCTX - 1));
+    g_nctx--;
+  }
   ctx_t *c = &g_ctx[g_nctx];
+  c->fp = fp;
   sunhip_topology T;
   T.Nc = g->Nc; T.Ne = g->Ne; T.maxfaces = g->maxfaces; T.Nkmax = g->Nkmax;
   T.nfaces = g->nfaces; T.neigh = g->neigh; T.face = g->face; T.normal = g->normal;

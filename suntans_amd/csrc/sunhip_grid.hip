@@ -554,4 +554,82 @@ int sunhip_sync(sunhip_grid *g) {
   return 0;
 }
 
+// Resolve recorded event pairs into per-kernel totals.
+static void prof_flush(sunhip_grid *g) {
+  if (g->prof_used == 0) return;
+  hipStreamSynchronize(g->stream);
+  for (int i = 0; i < g->prof_used; i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, g->prof_pool[2 * i], g->prof_pool[2 * i + 1]) == hipSuccess) {
+      g->prof_ms[g->prof_kind[i]] += ms;
+      g->prof_n[g->prof_kind[i]] += 1;
+    }
+  }
+  g->prof_used = 0;
+}
+
+int sunhip_profile(sunhip_grid *g, int on) {
+  HIPCHK(hipSetDevice(g->device));
+  prof_flush(g);
+  g->prof = on;
+  for (int i = 0; i < 8; i++) { g->prof_ms[i] = 0; g->prof_n[i] = 0; }
+  if (on && g->prof_pool.empty()) {
+    g->prof_pool.resize(2 * 4096);
+    g->prof_kind.resize(4096);
+    for (auto &e : g->prof_pool) HIPCHK(hipEventCreate(&e));
+  }
+  return 0;
+}
+
+// Totals per kernel id since sunhip_profile(g, 1); returns the number of ids.
+int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n) {
+  hipSetDevice(g->device);
+  prof_flush(g);
+  for (int i = 0; i < 8; i++) { ms[i] = g->prof_ms[i]; n[i] = g->prof_n[i]; }
+  return 8;
+}
+
 }  // extern "C"
+
+namespace sunhip {
+int prof_begin(sunhip_grid *g, int kind) {
+  if (!g->prof) return -1;
+  if (g->prof_used * 2 + 2 > (int)g->prof_pool.size()) prof_flush(g);
+  const int slot = g->prof_used++;
+  g->prof_kind[slot] = kind;
+  hipEventRecord(g->prof_pool[2 * slot], g->stream);
+  return slot;
+}
+void prof_end(sunhip_grid *g, int slot) {
+  if (slot >= 0) hipEventRecord(g->prof_pool[2 * slot + 1], g->stream);
+}
+}  // namespace sunhip
+
+// Plain streaming copy, the calibration kernel for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b,
+                                                    int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+extern "C" int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms) {
+  HIPCHK(hipSetDevice(g->device));
+  const int64_t n = nbytes / (int64_t)sizeof(double4);
+  if (n <= 0 || reps <= 0) { set_error("sunhip_copy_probe: bad size"); return -1; }
+  double4 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n * sizeof(double4)));
+  HIPCHK(hipMalloc(&b, n * sizeof(double4)));
+  HIPCHK(hipMemsetAsync(a, 0, n * sizeof(double4), g->stream));
+  const int blocks = 256 * 16;
+  k_copy_probe<<<blocks, 256, 0, g->stream>>>(a, b, n);  // warm
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  for (int r = 0; r < reps; r++) k_copy_probe<<<blocks, 256, 0, g->stream>>>(a, b, n);
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  *avg_ms = ms / reps;
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
+  return 0;
+}

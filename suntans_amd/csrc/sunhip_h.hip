@@ -138,7 +138,6 @@ struct HBufs {
   double *fcoef;
 };
 
-double *g_hfcoef_owner = nullptr;
 
 int hbufs(sunhip_grid *g, HBufs &B) {
   int rc = field(g, "h", &B.h) | field(g, "htmp", &B.src) | field(g, "D", &B.D) | field(g, "hcoef", &B.coef) |
@@ -177,8 +176,12 @@ int h_iteration(sunhip_grid *g, HBufs &B, int it) {
   V.pnew = (it & 1) ? B.p0->d : B.p1->d;
   V.x = B.h->d; V.out = B.z->d;
   const int nb = (g->NS + 255) / 256;
+  int ps = prof_begin(g, PK_HAPPLY);
   k_happly<HM_FUSED><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part, g->d_count + 8);
+  prof_end(g, ps);
+  ps = prof_begin(g, PK_HPREC);
   k_hprec<<<nb, 256, 0, g->stream>>>(G, B.coef->d, B.z->d, B.r->d, B.rt->d, g->d_hs, g->d_part, g->d_count + 9, 2, 1);
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -116,8 +116,26 @@ struct sunhip_grid {
   int64_t bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 
+  // per-kernel timing (sunhip_profile): an event pair around every launch of
+  // the solver kernels, resolved by sunhip_kernel_times
+  int prof = 0;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<int> prof_kind;   // kernel id per recorded pair
+  int prof_used = 0;
+  double prof_ms[8] = {0};
+  int prof_n[8] = {0};
+
   sunhip::Geo geo() const;
 };
+
+namespace sunhip {
+// Bracket one launch with events when profiling is on.  Returns the pair
+// index or -1.
+int prof_begin(sunhip_grid *g, int kind);
+void prof_end(sunhip_grid *g, int slot);
+enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, PK_MPREC = 5, PK_RESID = 6,
+       PK_OTHER = 7 };
+}  // namespace sunhip
 
 namespace sunhip {
 int field(sunhip_grid *g, const char *name, Field **out);  // create on demand

