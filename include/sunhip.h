@@ -111,12 +111,37 @@ int sunhip_q_precondition(sunhip_grid *g, const char *x, const char *xc);
  * iteration count). */
 int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sunhip_cg_stats *st);
 
+/* No reference counterpart: the mixed-precision form.  FP64 true residual
+ * r = src - L q and its norm on the device; while sqrt(r.r/r0.r0) >=
+ * prm->epsilon, solve L e = r with an FP32 vertical-line PCG (dots
+ * accumulated in FP64) to inner_tol relative reduction and add e to q in FP64.
+ * st->niters counts refinement steps; *inner_iters the FP32 iterations. */
+int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inner_tol, int max_outer,
+                         sunhip_cg_stats *st, int *inner_iters);
+/* QCoefficients from the device field "dzz" into "coef". */
+int sunhip_q_coefficients(sunhip_grid *g);
+/* Kernel tuning: "unroll" (levels per load group: 1, 2, 4) and "reduce"
+ * (0: last block finishes the dot products, 1: a finisher kernel). */
+int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
+/* Per-kernel device time: sunhip_profile(g, 1) starts recording an event
+ * pair around every solver kernel launch; sunhip_kernel_times returns the
+ * totals (ms) and launch counts of the 8 kernel ids (q apply, q precond,
+ * h apply, h precond, mixed apply, mixed precond, mixed residual, other). */
+int sunhip_profile(sunhip_grid *g, int on);
+int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n);
+/* A plain device copy of nbytes, reps times (HBM byte-counter calibration);
+ * *avg_ms is the mean duration. */
+int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
+
 /* ---- free surface (main/phys.c:3086-3217) ------------------------------ */
 /* Fields: "h" (in: boundary values; out: solution), "htmp" (RHS, clobbered),
  * "D".  grav, theta, dt enter through HCoefficients. */
 int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
                    sunhip_cg_stats *st);
 int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y);
+/* OperatorH with caller-supplied coefficients: "hcoef" (2-D cell field) and
+ * the face coefficients flat as [Nc*maxfaces] in the reference's indexing. */
+int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char *x, const char *y);
 
 /* ---- TVD scalar transport (main/tvd.c:36-120, main/scalars.c:35-325) ---- */
 typedef struct {

@@ -68,7 +68,8 @@ def bumps_depth(rng, Lx, Ly, dmin, dmax, nb=8, mean=None):
     return depth
 
 
-def fill_velocities(st: State, rng, amp_u=0.1, amp_w=1e-3, zero_w_bottom=True):
+def fill_velocities(st: State, rng, amp_u=0.1, amp_w=1e-3, zero_w_bottom=True,
+                    edge_names=("u", "utmp2"), cell_names=("w", "wtmp2", "wnew")):
     """u, utmp2 on wet edge layers (zero on closed boundaries), w, wtmp2, wnew."""
     Lx, Ly = st.Lx, st.Ly
     H = float(np.max(st.get("dv"))) + 1e-9
@@ -79,13 +80,13 @@ def fill_velocities(st: State, rng, amp_u=0.1, amp_w=1e-3, zero_w_bottom=True):
     zmid = 0.5 * (zc[:-1] + zc[1:])
     mark = st.ints("mark")
     wet = st.edge_wet_mask() & (mark[edge] == 0)
-    for name in ("u", "utmp2"):
+    for name in edge_names:
         v = smooth(rng, xe[edge], ye[edge], zmid[np.minimum(k, len(zmid) - 1)], Lx, Ly, H, amp_u)
         st.set(name, np.where(wet, v, 0.0))
     cell, kw = st.cell_layers(extra=1)
     xv, yv = st.get("xv"), st.get("yv")
     nk = st.ints("Nk")
-    for name in ("w", "wtmp2", "wnew"):
+    for name in cell_names:
         v = smooth(rng, xv[cell], yv[cell], zc[np.minimum(kw, len(zc) - 1)], Lx, Ly, H, amp_w)
         if zero_w_bottom:
             v = np.where(kw == nk[cell], 0.0, v)
@@ -171,9 +172,7 @@ def large_solve(seed=0, nx=1000, ny=500, Nkmax=100, dz=2.0, mean_depth=None, dt=
     st = State(nx, ny, 100e3 * nx / 1000, 50e3 * ny / 500, Nkmax, np.full(Nkmax, dz), depth)
     st.update_dz(-1)
     st.update_dz(1)
-    fill_velocities(st, rng, amp_u=0.1, amp_w=1e-3)
-    st.set("utmp2", 0.0)
-    st.set("wtmp2", 0.0)
+    fill_velocities(st, rng, amp_u=0.1, amp_w=1e-3, edge_names=("u",), cell_names=("w",))
     fill_scalar(st, rng)
     st.set_prop(dt=dt)
     st.set_flux_height()

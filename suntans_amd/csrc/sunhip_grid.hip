@@ -382,6 +382,8 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= dev_alloc(g, (void **)&g->d_count, sizeof(unsigned) * 64);
   rc |= dev_alloc(g, (void **)&g->d_qs, sizeof(CGScal));
   rc |= dev_alloc(g, (void **)&g->d_hs, sizeof(CGScal));
+  rc |= dev_alloc(g, (void **)&g->d_ms, sizeof(CGScal));
+  rc |= dev_alloc(g, (void **)&g->d_mo, sizeof(CGScal));
   if (rc) { sunhip_grid_destroy(g); return rc; }
   HIPCHK(hipMemset(g->d_count, 0, sizeof(unsigned) * 64));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CGScal), hipHostMallocDefault));
@@ -397,11 +399,13 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
   for (auto &kv : g->fields) hipFree(kv.second.d);
+  for (auto &kv : g->fields32) hipFree(kv.second);
   void *ptrs[] = {g->d_toff, g->d_tdep, g->d_eoff, g->d_edep, g->d_nk, g->d_ctop, g->d_ctopold, g->d_comp,
                   g->d_nf, g->d_nbb, g->d_nbs, g->d_eb, g->d_es, g->d_nrm, g->d_kr, g->d_Ac, g->d_df,
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
-                  g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs};
+                  g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
+                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);

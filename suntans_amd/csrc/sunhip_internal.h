@@ -111,10 +111,18 @@ struct sunhip_grid {
   double *d_part = nullptr;         // block partials
   unsigned *d_count = nullptr;      // last-block tickets (one per reduction site)
   sunhip::CGScal *d_qs = nullptr, *d_hs = nullptr;
+  sunhip::CGScal *d_ms = nullptr, *d_mo = nullptr;  // mixed form: inner FP32 CG, outer residual
   sunhip::CGScal *h_sc = nullptr;   // pinned mirror
   std::map<std::string, sunhip::Field> fields;
+  std::map<std::string, float *> fields32;  // FP32 copies for the mixed form (cell layout)
+  float *d_D32 = nullptr;
   int64_t bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+
+  int *d_class_tiles = nullptr;  // reserved
+  // tuning: levels per load group of the column kernels (1, 2, 4) and
+  // reduction finish (0 last block, 1 finisher kernel); -1 = default
+  int tune_unroll = -1, tune_reduce = -1;
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of
   // the solver kernels, resolved by sunhip_kernel_times
@@ -139,6 +147,7 @@ enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, 
 
 namespace sunhip {
 int field(sunhip_grid *g, const char *name, Field **out);  // create on demand
+int field32(sunhip_grid *g, const char *name, float **out);
 Kind kind_of(const char *name);
 int dev_alloc(sunhip_grid *g, void **p, size_t bytes);
 }  // namespace sunhip

@@ -8,18 +8,31 @@
 //   apply:   p = rtmp + mu*p_old ; x += nu_prev*p_old ; z = L p ; sum p.z
 //   precond: r -= nu*z ; rtmp = M^-1 r (Thomas down and up the column) ;
 //            sum r.rtmp, sum r.r ; mu, alpha, eps and the convergence test
-// The scalars live on the device; the last block of each kernel finishes the
-// reduction and updates them, so the host only polls a flag every few
-// iterations.  Element-wise arithmetic keeps the reference's operation order
-// and is built without FMA contraction, so OperatorQ and the preconditioner
-// agree with the reference bit for bit; only the summation order of the dot
-// products differs.
+// The scalars live on the device.  Each kernel's block partial sums are
+// finished either by the last block to arrive or by a one-block finisher
+// kernel (a tuning choice); both add the partials in a fixed order, so a
+// solve is reproducible run to run.  Element-wise arithmetic keeps the
+// reference's operation order and is built without FMA contraction, so
+// OperatorQ and the preconditioner agree with the reference bit for bit;
+// only the summation order of the dot products differs.
+//
+// The kernels are templated on the storage type T of vectors and
+// coefficients.  T=double is the drop-in; T=float is the inner solve of the
+// mixed-precision form (sunhip_q_solve_mixed), whose dot products still
+// accumulate in double and whose outer residual, convergence test and
+// solution update stay in FP64.
+//
+// Both kernels walk a column U levels at a time (a tuning choice, 1, 2 or 4)
+// and issue every load of a group before its arithmetic, with addresses
+// clamped into the column (a tile stores one level more than its deepest
+// column), so no load waits behind a branch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "sunhip_common.h"
 
@@ -28,16 +41,117 @@ using namespace sunhip;
 #define QAPPLY_THREADS 256
 #define TILES_PER_APPLY (QAPPLY_THREADS / SUNHIP_TILE)
 
-enum { QM_PLAIN = 0, QM_RESID = 1, QM_FUSED = 2 };
+enum { QM_PLAIN = 0, QM_RESID = 1, QM_FUSED = 2, QM_RESID_DOT = 3 };
+// finish operations on the CG scalars
+enum { OP_NONE = 0, OP_APPLY = 1, OP_PREC_SETUP = 2, OP_PREC_ITER = 3, OP_NOPREC_SETUP = 4, OP_NOPREC_ITER = 5,
+       OP_STORE = 6 };
 
+template <typename T, typename TO = T>
 struct QVec {
-  const double *coef, *dzf, *D;
-  const double *vin;            // PLAIN / RESID: operand
-  const double *src;            // RESID: right-hand side
-  const double *rtmp, *pold;    // FUSED: p = rtmp + mu*pold
-  double *pnew, *x;             // FUSED
-  double *out;                  // L p, or src - L x
+  const T *coef, *dzf, *D;
+  const T *vin;            // PLAIN / RESID: operand
+  const T *src;            // RESID: right-hand side
+  const T *rtmp, *pold;    // FUSED: p = rtmp + mu*pold
+  T *pnew, *x;             // FUSED
+  TO *out;                 // L p, or src - L x
 };
+
+// Where a launch's block sums go: the last block finishes them (cnt set) or
+// they are left at part[pbase + block] for a finisher kernel.
+struct Red {
+  double *part;
+  int pbase;
+  unsigned *cnt;
+  int op;
+};
+
+__device__ void q_op(CGScal *sc, int op, const double *tot) {
+  switch (op) {
+    case OP_APPLY:  // mu = 1/alpha ; nu = alpha/(p.z) (main/phys.c:2484-2485)
+      sc->pz = tot[0];
+      sc->muinv = 1 / sc->alpha;
+      sc->nu = sc->alpha / tot[0];
+      break;
+    case OP_PREC_SETUP:  // main/phys.c:2452-2475 with qprecond=2
+    case OP_NOPREC_SETUP:
+      sc->alpha = sc->alpha0 = op == OP_PREC_SETUP ? tot[0] : tot[1];
+      if (!sc->resnorm) sc->alpha0 = 1;
+      sc->eps = sc->eps0 = op == OP_PREC_SETUP ? tot[1] : sc->alpha0;
+      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
+      sc->done = (sc->eps == 0 || sc->maxit <= 0);
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      break;
+    case OP_PREC_ITER:  // main/phys.c:2496-2524
+    case OP_NOPREC_ITER: {
+      const double alpha = op == OP_PREC_ITER ? tot[0] : tot[1];
+      sc->mu = sc->muinv * alpha;
+      sc->alpha = alpha;
+      sc->eps = op == OP_PREC_ITER ? tot[1] : alpha;
+      sc->iter += 1;
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
+      else if (sc->iter >= sc->maxit || sc->eps == 0) { sc->done = 1; sc->nloop = sc->iter; }
+      break;
+    }
+    case OP_STORE:  // keep the sum (outer residual norm of the mixed form)
+      sc->pz = tot[0];
+      break;
+  }
+}
+
+// Block sums of NV values, then either finish (last block) or publish.
+template <int NV, int NT>
+__device__ void reduce_out(const double (&v)[NV], const Red &R, CGScal *sc) {
+  if (R.op == OP_NONE) return;
+  if (R.cnt) {
+    double tot[NV];
+    if (grid_reduce<NV, NT>(v, R.part, R.cnt, tot) && threadIdx.x == 0) q_op(sc, R.op, tot);
+    return;
+  }
+  constexpr int NW = NT / 64;
+  __shared__ double sh[NV][NW];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double s = wave_sum(v[i]);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0;
+      for (int j = 0; j < NW; j++) s += sh[i][j];
+      R.part[(size_t)(R.pbase + blockIdx.x) * NV + i] = s;
+    }
+  }
+}
+
+// One block adds nparts partials in a fixed order and applies op.
+template <int NV>
+__global__ __launch_bounds__(1024) void k_qfinalize(const double *__restrict__ part, int nparts, CGScal *sc, int op) {
+  if ((op == OP_APPLY || op == OP_PREC_ITER || op == OP_NOPREC_ITER) && sc->done) return;
+  __shared__ double sh[NV][16];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double tot[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nparts; b += 1024) s += part[(size_t)b * NV + i];
+    s = wave_sum(s);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0;
+      for (int j = 0; j < 16; j++) s += sh[i][j];
+      tot[i] = s;
+    }
+    q_op(sc, op, tot);
+  }
+}
 
 // QCoefficients, unpreconditioned branch (main/phys.c:3456-3469), every cell.
 __global__ __launch_bounds__(256) void k_qcoef(Geo G, const double *__restrict__ dzz, double *__restrict__ coef) {
@@ -57,21 +171,48 @@ __global__ __launch_bounds__(256) void k_qcoef(Geo G, const double *__restrict__
   }
 }
 
+// The modified diagonal b' of the Thomas elimination of every column
+// (main/util.c:193-196 applied to the tridiagonal of main/phys.c:3584-3601).
+// It depends only on coef, so it is built once per solve; the back
+// substitution of every application reads it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_qfactor(Geo G, const T *__restrict__ coef, T *__restrict__ bq) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
+  if (n == 0 || G.comp[s] != 1 || kt >= n - 1) return;
+  const int64_t base = G.toff[t] + l;
+  T cprev = 0, bprev = 1;
+  T ck = coef[base + (int64_t)kt * 64];
+  for (int k = kt; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    const T ck1 = k < n - 1 ? coef[idx + 64] : T(0);
+    T a = 0, b, c = 0;
+    if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+    else if (k == n - 1) { a = ck; b = -ck; }
+    else { a = ck; b = -ck - ck1; c = ck1; }
+    if (k > kt) b -= a * cprev / bprev;
+    bq[idx] = b;
+    cprev = c; bprev = b; ck = ck1;
+  }
+}
+
 // y = L v over the computational columns (main/phys.c:3486-3536).
-template <int MODE>
-__global__ __launch_bounds__(QAPPLY_THREADS) void k_qapply(Geo G, QVec V, CGScal *sc, double *part, unsigned *cnt) {
+// Contributions outside a face's coupling range are dropped with a select,
+// which leaves the sum bit-identical to the reference's loop.
+template <int MODE, int NF, int U, typename T, typename TO>
+__global__ __launch_bounds__(QAPPLY_THREADS) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
   if (MODE == QM_FUSED && sc->done) return;
-  const int nblk = gridDim.x;
-  const int t = xcd_block(blockIdx.x, nblk) * TILES_PER_APPLY + (threadIdx.x >> 6);
+  const int t = xcd_block(blockIdx.x, gridDim.x) * TILES_PER_APPLY + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   double dot = 0;
-  const double mu = MODE == QM_FUSED ? sc->mu : 0.0;
-  const double nu = MODE == QM_FUSED ? sc->nu : 0.0;
-  const double *__restrict__ coef = V.coef;
-  const double *__restrict__ dzf = V.dzf;
-  const double *__restrict__ rt = V.rtmp;
-  const double *__restrict__ po = V.pold;
-  const double *__restrict__ vin = V.vin;
+  const T mu = MODE == QM_FUSED ? (T)sc->mu : T(0);
+  const T nu = MODE == QM_FUSED ? (T)sc->nu : T(0);
+  const T *__restrict__ coef = V.coef;
+  const T *__restrict__ dzf = V.dzf;
+  const T *__restrict__ rt = V.rtmp;
+  const T *__restrict__ po = V.pold;
+  const T *__restrict__ vin = V.vin;
   if (t < G.Nt) {
     const int s = t * 64 + l;
     const int n = G.nk[s];
@@ -79,233 +220,220 @@ __global__ __launch_bounds__(QAPPLY_THREADS) void k_qapply(Geo G, QVec V, CGScal
       const int kt = G.ctop[s];
       const int NS = G.NS;
       const int nf = G.nf[s];
-      int nb[SUNHIP_MAXF], eb[SUNHIP_MAXF], kmn[SUNHIP_MAXF], kmx[SUNHIP_MAXF];
-      double Df[SUNHIP_MAXF];
+      const int64_t base = G.toff[t] + l;
+      int64_t nbb[NF], ebb[NF];
+      int kmn[NF], kmx[NF], lim[NF];
+      T Df[NF];
 #pragma unroll
-      for (int f = 0; f < SUNHIP_MAXF; f++) {
-        nb[f] = -1; eb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
-        if (f < nf) {
-          nb[f] = G.nbb[f * NS + s];
-          if (nb[f] >= 0) {
-            eb[f] = G.eb[f * NS + s];
-            const int kr = G.kr[f * NS + s];
-            kmn[f] = kr & 0xffff;
-            kmx[f] = kr >> 16;
-            Df[f] = V.D[G.es[f * NS + s]];
+      for (int f = 0; f < NF; f++) {
+        const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
+        if (nbf >= 0) {
+          nbb[f] = nbf;
+          ebb[f] = G.eb[f * NS + s];
+          const int kr = G.kr[f * NS + s];
+          kmn[f] = kr & 0xffff;
+          kmx[f] = kr >> 16;
+          Df[f] = V.D[G.es[f * NS + s]];
+        } else {
+          nbb[f] = base; ebb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
+        }
+        lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+      }
+      T vprev = 0;
+      for (int kb = kt; kb < n; kb += U) {
+        T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+#pragma unroll
+        for (int j = 0; j <= U; j++) {
+          const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
+          if (MODE == QM_FUSED) { pv[j] = po[idx]; v[j] = rt[idx]; }
+          else { pv[j] = 0; v[j] = vin[idx]; }
+          c[j] = coef[idx];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const int k = kb + j;
+          xv[j] = MODE == QM_FUSED ? V.x[base + (int64_t)min(k, n) * 64] : T(0);
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            const int kk = min(k, lim[f]);
+            const int64_t ni = nbb[f] + (int64_t)kk * 64;
+            nv[j][f] = MODE == QM_FUSED ? rt[ni] + mu * po[ni] : vin[ni];
+            dz[j][f] = dzf[ebb[f] + (int64_t)kk * 64];
+          }
+        }
+        if (MODE == QM_FUSED) {
+#pragma unroll
+          for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const int k = kb + j;
+          if (k < n) {
+            const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
+            const T c0 = c[j], cp1 = c[j + 1];
+            T y = 0;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+              const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+              y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+            }
+            if (kt == n - 1) y -= T(2.0) * c0 * v0;
+            else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
+            else if (k == n - 1) y += c0 * vm1 - c0 * v0;
+            else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
+            const int64_t idx = base + (int64_t)k * 64;
+            if (MODE == QM_FUSED) {
+              V.pnew[idx] = v0;
+              V.x[idx] = xv[j] + nu * pv[j];
+              dot += (double)v0 * (double)y;
+              V.out[idx] = y;
+            } else if (MODE == QM_RESID) {
+              V.out[idx] = V.src[idx] - y;
+            } else if (MODE == QM_RESID_DOT) {
+              const T rk = V.src[idx] - y;
+              V.out[idx] = (TO)rk;
+              dot += (double)rk * (double)rk;
+            } else {
+              V.out[idx] = y;
+            }
+          }
+        }
+        vprev = v[U - 1];
+      }
+    }
+  }
+  if (MODE == QM_FUSED || MODE == QM_RESID_DOT) {
+    double dv[1] = {dot};
+    reduce_out<1, QAPPLY_THREADS>(dv, R, sc);
+  }
+}
+
+// Thomas solve of one column (main/phys.c:3577-3619 with main/util.c:185-203)
+// in the reference's operation order; the single-layer column is
+// xc = -0.5 x / coef.  UPDATE first applies r -= nu*z (main/phys.c:2490-2493)
+// and writes r back.  The forward sweep recomputes b' on the fly and leaves d'
+// in out; the back substitution reads d', the precomputed b' (bq), coef (for
+// c) and r (for r.rtmp).
+template <bool UPDATE, int U, typename T>
+__global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
+                                               T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
+                                               CGScal *sc, Red R) {
+  if (UPDATE && sc->done) return;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  double rz = 0, rr = 0;
+  if (t < G.Nt) {
+    const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
+    if (n > 0 && G.comp[s] == 1) {
+      const T nu = UPDATE ? (T)sc->nu : T(0);
+      const int64_t base = G.toff[t] + l;
+      if (kt == n - 1) {
+        const int64_t idx = base + (int64_t)kt * 64;
+        T rk = r[idx];
+        if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
+        const T u = T(-0.5) * rk / coef[idx];
+        out[idx] = u;
+        rz += (double)rk * (double)u;
+        rr += (double)rk * (double)rk;
+      } else {
+        T cprev = 0, bprev = 1, dprev = 0;
+        for (int kb = kt; kb < n; kb += U) {
+          T rv[U], zv[U], cv[U + 1];
+#pragma unroll
+          for (int j = 0; j <= U; j++) {
+            const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
+            cv[j] = coef[idx];
+            if (j < U) {
+              rv[j] = r[idx];
+              zv[j] = UPDATE ? z[idx] : T(0);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < U; j++) {
+            const int k = kb + j;
+            if (k < n) {
+              const int64_t idx = base + (int64_t)k * 64;
+              T rk = rv[j];
+              if (UPDATE) { rk = rk - nu * zv[j]; r[idx] = rk; }
+              rr += (double)rk * (double)rk;
+              const T ck = cv[j], ck1 = cv[j + 1];
+              T a = 0, b, c = 0;
+              if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+              else if (k == n - 1) { a = ck; b = -ck; }
+              else { a = ck; b = -ck - ck1; c = ck1; }
+              T d = rk;
+              if (k > kt) {
+                b -= a * cprev / bprev;
+                d -= a * dprev / bprev;
+              }
+              out[idx] = d;
+              cprev = c; bprev = b; dprev = d;
+            }
+          }
+        }
+        T unext = 0, cnext = 0;
+        for (int ke = n - 1; ke >= kt; ke -= U) {
+          T bv[U], dv[U], rv[U], cv[U];
+#pragma unroll
+          for (int j = 0; j < U; j++) {
+            const int64_t idx = base + (int64_t)max(ke - j, kt) * 64;
+            bv[j] = bq[idx];
+            dv[j] = out[idx];
+            rv[j] = r[idx];
+            cv[j] = coef[idx];
+          }
+#pragma unroll
+          for (int j = 0; j < U; j++) {
+            const int k = ke - j;
+            if (k >= kt) {
+              const T b = bv[j], d = dv[j];
+              const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
+              out[base + (int64_t)k * 64] = u;
+              rz += (double)rv[j] * (double)u;
+              unext = u;
+              cnext = cv[j];  // c at level k-1 is coef[k]
+            }
           }
         }
       }
-      const int64_t base = G.toff[t] + l;
-      auto val = [&](int64_t idx) -> double {
-        if (MODE == QM_FUSED) return rt[idx] + mu * po[idx];
-        return vin[idx];
-      };
-      double vm1 = 0.0, v0, vp1 = 0.0, c0, cp1 = 0.0, po0 = 0.0, po1 = 0.0;
-      {
-        const int64_t i0 = base + (int64_t)kt * 64;
-        if (MODE == QM_FUSED) { po0 = po[i0]; v0 = rt[i0] + mu * po0; } else v0 = vin[i0];
-        c0 = coef[i0];
-      }
-      for (int k = kt; k < n; k++) {
-        const int64_t idx = base + (int64_t)k * 64;
-        if (k + 1 < n) {
-          if (MODE == QM_FUSED) { po1 = po[idx + 64]; vp1 = rt[idx + 64] + mu * po1; }
-          else vp1 = vin[idx + 64];
-          cp1 = coef[idx + 64];
-        }
-        double y = 0;
-#pragma unroll
-        for (int f = 0; f < SUNHIP_MAXF; f++)
-          if (nb[f] >= 0 && k >= kmn[f] && k < kmx[f])
-            y += (val(nb[f] + (int64_t)k * 64) - v0) * dzf[eb[f] + (int64_t)k * 64] * Df[f];
-        if (kt == n - 1) y -= 2.0 * c0 * v0;
-        else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
-        else if (k == n - 1) y += c0 * vm1 - c0 * v0;
-        else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
-        if (MODE == QM_FUSED) {
-          V.pnew[idx] = v0;
-          V.x[idx] = V.x[idx] + nu * po0;
-          dot += v0 * y;
-          V.out[idx] = y;
-        } else if (MODE == QM_RESID) {
-          V.out[idx] = V.src[idx] - y;
-        } else {
-          V.out[idx] = y;
-        }
-        vm1 = v0; v0 = vp1; c0 = cp1; po0 = po1;
-      }
     }
   }
-  if (MODE == QM_FUSED) {
-    double v[1] = {dot}, tot[1];
-    if (grid_reduce<1, QAPPLY_THREADS>(v, part, cnt, tot) && threadIdx.x == 0) {
-      sc->pz = tot[0];
-      sc->muinv = 1 / sc->alpha;          // mu = 1/alpha  (main/phys.c:2484)
-      sc->nu = sc->alpha / tot[0];        // nu = alpha/(p.z) (:2485)
-    }
-  }
-}
-
-// Thomas solve of one column's vertical operator (main/phys.c:3577-3619 with
-// main/util.c:185-203), keeping the reference's operation order.  The forward
-// sweep's modified diagonal b' and right-hand side d' of all levels stay in
-// registers (the loops are fully unrolled over DC levels); the back
-// substitution re-reads coef (for c) and r (for r.rtmp).
-//   UPDATE: r -= nu*z first (main/phys.c:2490-2493), written back.
-template <int DC, bool UPDATE>
-__global__ __launch_bounds__(64) void k_qprec(Geo G, const double *__restrict__ coef,
-                                              const double *__restrict__ z, double *__restrict__ r,
-                                              double *__restrict__ out, const int *__restrict__ tiles,
-                                              int ntiles, CGScal *sc, double *part, unsigned *cnt, int finish) {
-  if (UPDATE && sc->done) return;
-  const int bt = xcd_block(blockIdx.x, gridDim.x);
-  const int t = tiles ? tiles[bt] : bt;
-  const int l = threadIdx.x;
-  const int s = t * 64 + l;
-  const int n = G.nk[s];
-  const int kt = G.ctop[s];
-  const bool act = n > 0 && G.comp[s] == 1;
-  const double nu = UPDATE ? sc->nu : 0.0;
-  const int64_t base = G.toff[t] + l;
-  double rz = 0, rr = 0;
-  double yv[DC], bv[DC];
-  // the tile's wet levels
-  const int k0 = 0, kend = G.tdep[t] - 1;
-  (void)ntiles;
-  double cprev = 0, bprev = 1, dprev = 0;
-  if (act && kt == n - 1) {
-    const int64_t idx = base + (int64_t)kt * 64;
-    double rk = r[idx];
-    if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
-    const double u = -0.5 * rk / coef[idx];
-    out[idx] = u;
-    rz += rk * u;
-    rr += rk * rk;
-  }
-  const bool multi = act && kt < n - 1;
-#pragma unroll
-  for (int j = 0; j < DC; j++) {
-    const int k = k0 + j;
-    if (k < kend && multi && k >= kt && k < n) {
-      const int64_t idx = base + (int64_t)k * 64;
-      double rk = r[idx];
-      if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
-      rr += rk * rk;
-      const double ck = coef[idx];
-      double a = 0, b, c = 0;
-      if (k == kt) {
-        const double ck1 = coef[idx + 64];
-        b = -2 * ck - ck1;
-        c = ck1;
-      } else if (k == n - 1) {
-        a = ck;
-        b = -ck;
-      } else {
-        const double ck1 = coef[idx + 64];
-        a = ck;
-        b = -ck - ck1;
-        c = ck1;
-      }
-      double d = rk;
-      if (k > kt) {
-        b -= a * cprev / bprev;
-        d -= a * dprev / bprev;
-      }
-      yv[j] = d;
-      bv[j] = b;
-      cprev = c; bprev = b; dprev = d;
-    }
-  }
-  double unext = 0;
-#pragma unroll
-  for (int j = DC - 1; j >= 0; j--) {
-    const int k = k0 + j;
-    if (k < kend && multi && k >= kt && k < n) {
-      const int64_t idx = base + (int64_t)k * 64;
-      double u;
-      if (k == n - 1) u = yv[j] / bv[j];
-      else u = yv[j] / bv[j] - coef[idx + 64] * unext / bv[j];
-      out[idx] = u;
-      rz += r[idx] * u;
-      unext = u;
-    }
-  }
-  if (finish) {
-    double v[2] = {rz, rr}, tot[2];
-    if (grid_reduce<2, 64>(v, part, cnt, tot) && threadIdx.x == 0) {
-      if (finish == 1) {
-        // initial residual (main/phys.c:2452-2475)
-        sc->alpha = sc->alpha0 = tot[0];
-        if (!sc->resnorm) sc->alpha0 = 1;
-        sc->eps = sc->eps0 = tot[1];
-        sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
-        sc->done = (sc->eps == 0 || sc->maxit <= 0);
-        sc->resid = sqrt(sc->eps / sc->eps0);
-      } else {
-        // end of one iteration (main/phys.c:2496-2524)
-        const double alpha = tot[0];
-        sc->mu = sc->muinv * alpha;
-        sc->alpha = alpha;
-        sc->eps = tot[1];
-        sc->iter += 1;
-        sc->resid = sqrt(sc->eps / sc->eps0);
-        if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
-        else if (sc->iter >= sc->maxit || sc->eps == 0) { sc->done = 1; sc->nloop = sc->iter; }
-      }
-    }
-  }
+  double v[2] = {rz, rr};
+  reduce_out<2, 256>(v, R, sc);
 }
 
 // qprecond=0: r -= nu*z, rtmp = r, alpha = eps = r.r (main/phys.c:2505-2519).
-__global__ __launch_bounds__(256) void k_qnoprec(Geo G, const double *__restrict__ z, double *__restrict__ r,
-                                                 double *__restrict__ out, CGScal *sc, double *part,
-                                                 unsigned *cnt, int finish, int update) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_qnoprec(Geo G, const T *__restrict__ z, T *__restrict__ r,
+                                                 T *__restrict__ out, CGScal *sc, Red R, int update) {
   if (update && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
   double rr = 0;
   if (t < G.Nt) {
     const int s = t * 64 + l, n = G.nk[s];
     if (n > 0 && G.comp[s] == 1) {
-      const double nu = update ? sc->nu : 0.0;
+      const T nu = update ? (T)sc->nu : T(0);
       const int64_t base = G.toff[t] + l;
       for (int k = G.ctop[s]; k < n; k++) {
         const int64_t idx = base + (int64_t)k * 64;
-        double rk = r[idx];
+        T rk = r[idx];
         if (update) { rk = rk - nu * z[idx]; r[idx] = rk; }
         out[idx] = rk;
-        rr += rk * rk;
+        rr += (double)rk * (double)rk;
       }
     }
   }
-  double v[1] = {rr}, tot[1];
-  if (grid_reduce<1, 256>(v, part, cnt, tot) && threadIdx.x == 0) {
-    if (finish == 1) {
-      sc->alpha = sc->alpha0 = tot[0];
-      if (!sc->resnorm) sc->alpha0 = 1;
-      sc->eps = sc->eps0 = sc->alpha0;
-      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
-      sc->done = (sc->eps == 0 || sc->maxit <= 0);
-      sc->resid = sqrt(sc->eps / sc->eps0);
-    } else {
-      const double alpha = tot[0];
-      sc->mu = sc->muinv * alpha;
-      sc->alpha = alpha;
-      sc->eps = alpha;
-      sc->iter += 1;
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
-      else if (sc->iter >= sc->maxit || sc->eps == 0) { sc->done = 1; sc->nloop = sc->iter; }
-    }
-  }
+  double v[2] = {rr, rr};
+  reduce_out<2, 256>(v, R, sc);
 }
 
 // x += nu*p for the last iteration (its x update is otherwise folded into the
 // next apply).  p is the buffer the last apply wrote.
-__global__ __launch_bounds__(256) void k_qfinish(Geo G, double *__restrict__ x, const double *__restrict__ p0,
-                                                 const double *__restrict__ p1, const CGScal *sc) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_qfinish(Geo G, T *__restrict__ x, const T *__restrict__ p0,
+                                                 const T *__restrict__ p1, const CGScal *sc) {
   if (sc->iter == 0) return;
-  const double *__restrict__ p = (sc->iter & 1) ? p1 : p0;
-  const double nu = sc->nu;
+  const T *__restrict__ p = (sc->iter & 1) ? p1 : p0;
+  const T nu = (T)sc->nu;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s];
@@ -317,9 +445,29 @@ __global__ __launch_bounds__(256) void k_qfinish(Geo G, double *__restrict__ x, 
   }
 }
 
-__global__ void k_zero(double *p, int64_t n) {
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+  if (i < n) p[i] = v;
+}
+
+template <typename TI, typename TO>
+__global__ void k_cast(const TI *__restrict__ a, TO *__restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (TO)a[i];
+}
+
+// x64 += e32 on the computational columns (mixed form, FP64 accumulation).
+__global__ __launch_bounds__(256) void k_mix_update(Geo G, double *__restrict__ x, const float *__restrict__ e) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  for (int k = G.ctop[s]; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    x[idx] = x[idx] + (double)e[idx];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -327,133 +475,200 @@ __global__ void k_zero(double *p, int64_t n) {
 
 namespace {
 
-struct QBufs {
-  Field *q, *src, *dzz, *dzf, *D, *coef, *r, *z, *rtmp, *p0, *p1;
+// One PCG instance: storage type, its scalars and its buffers.
+template <typename T>
+struct QSolver {
+  CGScal *sc;
+  const T *coef, *dzf, *D;
+  T *x, *r, *z, *rtmp, *p0, *p1, *bq;
 };
 
-int qbufs(sunhip_grid *g, QBufs &B) {
-  int rc = 0;
-  rc |= field(g, "q", &B.q);
-  rc |= field(g, "src", &B.src);
-  rc |= field(g, "dzz", &B.dzz);
-  rc |= field(g, "dzf", &B.dzf);
-  rc |= field(g, "D", &B.D);
-  rc |= field(g, "coef", &B.coef);
-  rc |= field(g, "r", &B.r);
-  rc |= field(g, "z", &B.z);
-  rc |= field(g, "rtmp", &B.rtmp);
-  rc |= field(g, "p0", &B.p0);
-  rc |= field(g, "p1", &B.p1);
-  return rc;
-}
-
-// depth class of the register-resident Thomas kernel
-int depth_class(int d) {
-  static const int cls[] = {8, 16, 24, 32, 48, 64, 80, 96, 112, 128};
-  for (int c : cls) if (d <= c) return c;
-  return -1;
-}
-
-template <bool UPDATE>
-int launch_prec(sunhip_grid *g, int dc, const double *coef, const double *z, double *r, double *out,
-                CGScal *sc, int finish) {
-  const Geo G = g->geo();
-  const int nb = g->Nt;
-  unsigned *cnt = g->d_count + 1;
-#define PREC_CASE(D)                                                                                    \
-  case D:                                                                                               \
-    k_qprec<D, UPDATE><<<nb, 64, 0, g->stream>>>(G, coef, z, r, out, nullptr, nb, sc, g->d_part, cnt, finish); \
-    break;
-  switch (dc) {
-    PREC_CASE(8) PREC_CASE(16) PREC_CASE(24) PREC_CASE(32) PREC_CASE(48) PREC_CASE(64) PREC_CASE(80)
-    PREC_CASE(96) PREC_CASE(112) PREC_CASE(128)
-    default:
-      set_error("columns deeper than 128 layers are not supported by the preconditioner");
-      return -1;
+void read_tuning(sunhip_grid *g) {
+  if (g->tune_unroll < 0) {
+    const char *e = getenv("SUNHIP_QUNROLL");
+    g->tune_unroll = e ? atoi(e) : 4;
+    if (g->tune_unroll != 1 && g->tune_unroll != 2 && g->tune_unroll != 4) g->tune_unroll = 4;
   }
-#undef PREC_CASE
+  if (g->tune_reduce < 0) {
+    const char *e = getenv("SUNHIP_REDUCE");
+    g->tune_reduce = (e && !strcmp(e, "last")) ? 0 : 1;
+  }
+}
+
+int nb4(const sunhip_grid *g) { return (g->Nt + 3) / 4; }
+
+template <typename T>
+int launch_factor(sunhip_grid *g, const T *coef, T *bq) {
+  k_qfactor<T><<<nb4(g), 256, 0, g->stream>>>(g->geo(), coef, bq);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, QBufs &B, int &dc) {
-  if (!g->have_extents) {
-    sunhip::set_error("sunhip_q_*: call sunhip_set_extents first");
-    return -1;
+// Preconditioner over all columns; op decides what happens to the sums
+// (OP_NONE for the plain drop-in application).
+template <bool UPDATE, typename T>
+int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *out, const T *bq, int op,
+                unsigned *cnt) {
+  const Geo G = g->geo();
+  const int nb = nb4(g);
+  Red R{g->d_part, 0, (g->tune_reduce == 0 && op != OP_NONE) ? cnt : nullptr, op};
+  switch (g->tune_unroll) {
+    case 1: k_qprec<UPDATE, 1, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
+    case 2: k_qprec<UPDATE, 2, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
+    default: k_qprec<UPDATE, 4, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
   }
-  if (prm->precond != 0 && prm->precond != 2) {
-    sunhip::set_error("qprecond=%d is not supported on the device (0 or 2)", prm->precond);
-    return -1;
+  HIPCHK(hipGetLastError());
+  if (op != OP_NONE && g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, sc, op);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <int MODE, typename T, typename TO>
+int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R) {
+  const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
+  const Geo G = g->geo();
+#define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, sc, R)
+  if (g->mf <= 3) {
+    if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2); else APPLY(3, 4);
+  } else {
+    if (g->tune_unroll == 1) APPLY(4, 1); else if (g->tune_unroll == 2) APPLY(4, 2); else APPLY(4, 4);
   }
-  int rc = qbufs(g, B);
-  if (rc) return rc;
-  dc = depth_class(g->maxdepth);
-  if (dc < 0) { set_error("column depth %d exceeds 128", g->maxdepth); return -1; }
+#undef APPLY
+  HIPCHK(hipGetLastError());
+  if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nba, sc, R.op);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_noprec(sunhip_grid *g, const QSolver<T> &S, int update, int op) {
+  const int nb = nb4(g);
+  Red R{g->d_part, 0, g->tune_reduce == 0 ? g->d_count + 2 : nullptr, op};
+  k_qnoprec<T><<<nb, 256, 0, g->stream>>>(g->geo(), update ? S.z : nullptr, S.r, S.rtmp, S.sc, R, update);
+  HIPCHK(hipGetLastError());
+  if (g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, op);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int upload_scal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) {
   CGScal h{};
   h.tol = prm->epsilon;
   h.maxit = prm->maxiters;
   h.precond = prm->precond;
   h.resnorm = prm->resnorm;
   *g->h_sc = h;
-  HIPCHK(hipMemcpyAsync(g->d_qs, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
-  const Geo G = g->geo();
-  const int nb4 = (g->Nt + 3) / 4;
-  k_qcoef<<<nb4, 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
-  // r = src - L x  (main/phys.c:2444-2451)
-  QVec V{};
-  V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = B.r->d;
-  const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
-  k_qapply<QM_RESID><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, g->d_qs, g->d_part, g->d_count);
-  HIPCHK(hipGetLastError());
-  // rtmp = M^-1 r, alpha0, eps0
-  if (prm->precond == 2) {
-    rc = launch_prec<false>(g, dc, B.coef->d, nullptr, B.r->d, B.rtmp->d, g->d_qs, 1);
-  } else {
-    k_qnoprec<<<nb4, 256, 0, g->stream>>>(G, nullptr, B.r->d, B.rtmp->d, g->d_qs, g->d_part, g->d_count + 2, 1, 0);
-    HIPCHK(hipGetLastError());
-  }
+  HIPCHK(hipMemcpyAsync(d, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+// Initial residual r (already in S.r), rtmp = M^-1 r, alpha0/eps0, p_old = 0.
+template <typename T>
+int cg_start(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S) {
+  int rc;
+  if (prm->precond == 2)
+    rc = launch_prec<false, T>(g, S.sc, S.coef, nullptr, S.r, S.rtmp, S.bq, OP_PREC_SETUP, g->d_count + 1);
+  else
+    rc = launch_noprec<T>(g, S, 0, OP_NOPREC_SETUP);
   if (rc) return rc;
   // p_old = 0 so the first apply reads p = rtmp exactly
   const int64_t n = g->NT;
-  k_zero<<<(unsigned)((n + 255) / 256), 256, 0, g->stream>>>(B.p0->d, n);
-  k_zero<<<(unsigned)((n + 255) / 256), 256, 0, g->stream>>>(B.p1->d, n);
+  k_fill<T><<<(unsigned)((n + 255) / 256), 256, 0, g->stream>>>(S.p0, n, T(0));
+  k_fill<T><<<(unsigned)((n + 255) / 256), 256, 0, g->stream>>>(S.p1, n, T(0));
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 // One PCG iteration; it = iteration index (selects the p buffers).
-int q_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QBufs &B, int dc, int it) {
-  const Geo G = g->geo();
-  QVec V{};
-  V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d;
-  V.rtmp = B.rtmp->d;
-  V.pold = (it & 1) ? B.p1->d : B.p0->d;
-  V.pnew = (it & 1) ? B.p0->d : B.p1->d;
-  V.x = B.q->d;
-  V.out = B.z->d;
-  const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
-  int ps = prof_begin(g, PK_QAPPLY);
-  k_qapply<QM_FUSED><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, g->d_qs, g->d_part, g->d_count);
+template <typename T>
+int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int it, int pk_apply, int pk_prec) {
+  QVec<T, T> V{};
+  V.coef = S.coef; V.dzf = S.dzf; V.D = S.D;
+  V.rtmp = S.rtmp;
+  V.pold = (it & 1) ? S.p1 : S.p0;
+  V.pnew = (it & 1) ? S.p0 : S.p1;
+  V.x = S.x;
+  V.out = S.z;
+  int ps = prof_begin(g, pk_apply);
+  int rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, Red{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY});
   prof_end(g, ps);
-  HIPCHK(hipGetLastError());
-  ps = prof_begin(g, PK_QPREC);
-  int rc = 0;
-  if (prm->precond == 2) {
-    rc = launch_prec<true>(g, dc, B.coef->d, B.z->d, B.r->d, B.rtmp->d, g->d_qs, 2);
-  } else {
-    k_qnoprec<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, B.z->d, B.r->d, B.rtmp->d, g->d_qs, g->d_part,
-                                                       g->d_count + 2, 2, 1);
-  }
+  if (rc) return rc;
+  ps = prof_begin(g, pk_prec);
+  if (prm->precond == 2)
+    rc = launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1);
+  else
+    rc = launch_noprec<T>(g, S, 1, OP_NOPREC_ITER);
   prof_end(g, ps);
-  HIPCHK(hipGetLastError());
   return rc;
 }
 
-int q_finish(sunhip_grid *g, QBufs &B, sunhip_cg_stats *st) {
-  const Geo G = g->geo();
-  k_qfinish<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, B.q->d, B.p0->d, B.p1->d, g->d_qs);
+// Iterate until the device flag says done; the host polls every few
+// iterations and kernels of a batch that start after convergence return at
+// once.
+template <typename T>
+int cg_loop(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int pk_apply, int pk_prec) {
+  int every = prm->check_every > 0 ? prm->check_every : (g->NT > (1 << 22) ? 4 : 16);
+  if (every & 1) every++;
+  int it = 0, rc;
+  HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  while (!g->h_sc->done && it < prm->maxiters) {
+    for (int b = 0; b < every && it < prm->maxiters; b++, it++)
+      if ((rc = cg_iteration<T>(g, prm, S, it, pk_apply, pk_prec))) return rc;
+    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+  }
+  k_qfinish<T><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(g->h_sc, g->d_qs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+  return 0;
+}
+
+int check_q(sunhip_grid *g, const sunhip_cg_params *prm) {
+  if (!g->have_extents) {
+    set_error("sunhip_q_*: call sunhip_set_extents first");
+    return -1;
+  }
+  if (prm->precond != 0 && prm->precond != 2) {
+    set_error("qprecond=%d is not supported on the device (0 or 2)", prm->precond);
+    return -1;
+  }
+  if (g->maxdepth > 128) { set_error("column depth %d exceeds 128", g->maxdepth); return -1; }
+  read_tuning(g);
+  return 0;
+}
+
+struct F64 {
+  Field *q, *src, *dzz, *dzf, *D, *coef, *r, *z, *rtmp, *p0, *p1, *bq;
+};
+
+int f64bufs(sunhip_grid *g, F64 &B) {
+  return field(g, "q", &B.q) | field(g, "src", &B.src) | field(g, "dzz", &B.dzz) | field(g, "dzf", &B.dzf) |
+         field(g, "D", &B.D) | field(g, "coef", &B.coef) | field(g, "r", &B.r) | field(g, "z", &B.z) |
+         field(g, "rtmp", &B.rtmp) | field(g, "p0", &B.p0) | field(g, "p1", &B.p1) | field(g, "qbp", &B.bq);
+}
+
+QSolver<double> solver64(sunhip_grid *g, F64 &B) {
+  return QSolver<double>{g->d_qs, B.coef->d, B.dzf->d, B.D->d, B.q->d, B.r->d, B.z->d, B.rtmp->d,
+                         B.p0->d, B.p1->d, B.bq->d};
+}
+
+// coefficients, b', r = src - L x, then cg_start (main/phys.c:2441-2475)
+int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double> &S) {
+  int rc = check_q(g, prm);
+  if (rc) return rc;
+  if ((rc = f64bufs(g, B))) return rc;
+  S = solver64(g, B);
+  if ((rc = upload_scal(g, g->d_qs, prm))) return rc;
+  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(g->geo(), B.dzz->d, B.coef->d);
+  if (prm->precond == 2 && (rc = launch_factor<double>(g, B.coef->d, B.bq->d))) return rc;
+  QVec<double, double> V{};
+  V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = B.r->d;
+  if ((rc = launch_apply<QM_RESID, double, double>(g, g->d_qs, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
+  return cg_start<double>(g, prm, S);
+}
+
+int q_stats(sunhip_grid *g, CGScal *d, sunhip_cg_stats *st) {
+  HIPCHK(hipMemcpyAsync(g->h_sc, d, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipEventRecord(g->ev2, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
   if (st) {
@@ -472,29 +687,34 @@ int q_finish(sunhip_grid *g, QBufs &B, sunhip_cg_stats *st) {
 
 }  // namespace
 
+namespace sunhip {
+int field32(sunhip_grid *g, const char *name, float **out) {
+  auto it = g->fields32.find(name);
+  if (it == g->fields32.end()) {
+    float *p = nullptr;
+    const size_t n = (size_t)std::max(g->NT, g->NE);
+    int rc = dev_alloc(g, (void **)&p, n * sizeof(float));
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(p, 0, n * sizeof(float), g->stream));
+    it = g->fields32.emplace(name, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+}  // namespace sunhip
+
 extern "C" {
 
 int sunhip_q_solve(sunhip_grid *g, const sunhip_cg_params *prm, sunhip_cg_stats *st) {
   HIPCHK(hipSetDevice(g->device));
-  QBufs B;
-  int dc, rc;
+  F64 B;
+  QSolver<double> S;
+  int rc;
   HIPCHK(hipEventRecord(g->ev0, g->stream));
-  if ((rc = q_setup(g, prm, B, dc))) return rc;
+  if ((rc = q_setup(g, prm, B, S))) return rc;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
-  int every = prm->check_every > 0 ? prm->check_every : (g->NT > (1 << 22) ? 2 : 16);
-  if (every & 1) every++;
-  int it = 0;
-  // the host checks the device flag after every batch; kernels of a batch
-  // that start after convergence return at once
-  HIPCHK(hipMemcpyAsync(g->h_sc, g->d_qs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  while (!g->h_sc->done && it < prm->maxiters) {
-    for (int b = 0; b < every && it < prm->maxiters; b++, it++)
-      if ((rc = q_iteration(g, prm, B, dc, it))) return rc;
-    HIPCHK(hipMemcpyAsync(g->h_sc, g->d_qs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
-  }
-  return q_finish(g, B, st);
+  if ((rc = cg_loop<double>(g, prm, S, PK_QAPPLY, PK_QPREC))) return rc;
+  return q_stats(g, g->d_qs, st);
 }
 
 int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sunhip_cg_stats *st) {
@@ -502,14 +722,99 @@ int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sun
   sunhip_cg_params p = *prm;
   p.epsilon = 0;       // never satisfied
   p.maxiters = n;
-  QBufs B;
-  int dc, rc;
+  F64 B;
+  QSolver<double> S;
+  int rc;
   HIPCHK(hipEventRecord(g->ev0, g->stream));
-  if ((rc = q_setup(g, &p, B, dc))) return rc;
+  if ((rc = q_setup(g, &p, B, S))) return rc;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
   for (int it = 0; it < n; it++)
-    if ((rc = q_iteration(g, &p, B, dc, it))) return rc;
-  return q_finish(g, B, st);
+    if ((rc = cg_iteration<double>(g, &p, S, it, PK_QAPPLY, PK_QPREC))) return rc;
+  k_qfinish<double><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  HIPCHK(hipGetLastError());
+  return q_stats(g, g->d_qs, st);
+}
+
+// Mixed precision (FP32 inner PCG inside FP64 iterative refinement):
+//   r = src - L x in FP64 (true residual), eps = r.r in FP64; stop when
+//   sqrt(eps/eps0) < epsilon (eps0 the first true residual);
+//   otherwise solve L e = r in FP32 (vertical-line PCG, dots accumulated in
+//   FP64) to inner_tol relative, then x += e in FP64.
+int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inner_tol, int max_outer,
+                         sunhip_cg_stats *st, int *inner_iters) {
+  HIPCHK(hipSetDevice(g->device));
+  int rc = check_q(g, prm);
+  if (rc) return rc;
+  if (prm->precond != 2) { set_error("the mixed form uses the vertical-line preconditioner (qprecond=2)"); return -1; }
+  F64 B;
+  if ((rc = f64bufs(g, B))) return rc;
+  float *c32, *f32, *x32, *r32, *z32, *t32, *pa, *pb, *b32;
+  rc = field32(g, "coef", &c32) | field32(g, "dzf", &f32) | field32(g, "r", &r32) | field32(g, "z", &z32) |
+       field32(g, "rtmp", &t32) | field32(g, "p0", &pa) | field32(g, "p1", &pb) | field32(g, "bq", &b32) |
+       field32(g, "e", &x32);
+  if (rc) return rc;
+  if (!g->d_D32 && (rc = dev_alloc(g, (void **)&g->d_D32, sizeof(float) * (size_t)g->NES))) return rc;
+  float *D32 = g->d_D32;
+  const Geo G = g->geo();
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
+  k_cast<double, float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(B.coef->d, c32, g->NT);
+  k_cast<double, float><<<(unsigned)((g->NE + 255) / 256), 256, 0, g->stream>>>(B.dzf->d, f32, g->NE);
+  k_cast<double, float><<<(unsigned)((g->NES + 255) / 256), 256, 0, g->stream>>>(B.D->d, D32, g->NES);
+  HIPCHK(hipGetLastError());
+  if ((rc = launch_factor<float>(g, c32, b32))) return rc;
+  CGScal *so = g->d_mo;
+  sunhip_cg_params inner = *prm;
+  inner.epsilon = inner_tol;
+  inner.resnorm = 0;
+  QSolver<float> S{g->d_ms, c32, f32, D32, x32, r32, z32, t32, pa, pb, b32};
+  QVec<double, float> R{};
+  R.coef = B.coef->d; R.dzf = B.dzf->d; R.D = B.D->d; R.vin = B.q->d; R.src = B.src->d; R.out = r32;
+  double eps0 = -1, eps = 0;
+  int outer = 0, total_inner = 0, converged = 0;
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  for (;;) {
+    // FP64 true residual, cast to FP32 for the inner solve
+    int ps = prof_begin(g, PK_RESID);
+    rc = launch_apply<QM_RESID_DOT, double, float>(g, so, R, Red{g->d_part, 0, nullptr, OP_STORE});
+    prof_end(g, ps);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(g->h_sc, so, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    eps = g->h_sc->pz;
+    if (eps0 < 0) eps0 = eps;
+    const double ratio = eps == 0 ? 0.0 : sqrt(eps / eps0);
+    if (eps == 0 || ratio < prm->epsilon) { converged = 1; break; }
+    if (outer >= max_outer) break;
+    // inner FP32 PCG from e = 0, asked for no more reduction than the outer
+    // target still needs (with a margin for the FP32/FP64 residual gap)
+    inner.epsilon = std::max(inner_tol, 0.25 * prm->epsilon / ratio);
+    if ((rc = upload_scal(g, S.sc, &inner))) return rc;
+    k_fill<float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(x32, g->NT, 0.0f);
+    if ((rc = cg_start<float>(g, &inner, S))) return rc;
+    if ((rc = cg_loop<float>(g, &inner, S, PK_MAPPLY, PK_MPREC))) return rc;
+    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    total_inner += g->h_sc->iter;
+    k_mix_update<<<nb4(g), 256, 0, g->stream>>>(G, B.q->d, x32);
+    HIPCHK(hipGetLastError());
+    outer++;
+  }
+  HIPCHK(hipEventRecord(g->ev2, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (st) {
+    st->niters = outer;
+    st->converged = converged;
+    st->resid = eps0 > 0 ? sqrt(eps / eps0) : 0;
+    st->eps0 = eps0;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev1, g->ev2);
+    st->device_ms = ms;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    st->setup_ms = ms;
+  }
+  if (inner_iters) *inner_iters = total_inner;
+  return 0;
 }
 
 int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y) {
@@ -519,11 +824,10 @@ int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y) {
   int rc = field(g, x, &fx) | field(g, y, &fy) | field(g, "coef", &coef) | field(g, "dzf", &dzf) |
            field(g, "D", &D);
   if (rc) return rc;
-  QVec V{};
+  QVec<double, double> V{};
   V.coef = coef->d; V.dzf = dzf->d; V.D = D->d; V.vin = fx->d; V.out = fy->d;
-  const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
-  k_qapply<QM_PLAIN><<<nba, QAPPLY_THREADS, 0, g->stream>>>(g->geo(), V, g->d_qs, g->d_part, g->d_count);
-  HIPCHK(hipGetLastError());
+  read_tuning(g);
+  if ((rc = launch_apply<QM_PLAIN, double, double>(g, g->d_qs, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -534,7 +838,7 @@ int sunhip_q_coefficients(sunhip_grid *g) {
   Field *dzz, *coef;
   int rc = field(g, "dzz", &dzz) | field(g, "coef", &coef);
   if (rc) return rc;
-  k_qcoef<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), dzz->d, coef->d);
+  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(g->geo(), dzz->d, coef->d);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -542,13 +846,24 @@ int sunhip_q_coefficients(sunhip_grid *g) {
 int sunhip_q_precondition(sunhip_grid *g, const char *x, const char *xc) {
   HIPCHK(hipSetDevice(g->device));
   if (!g->have_extents) { set_error("sunhip_q_precondition: call sunhip_set_extents first"); return -1; }
-  Field *fx, *fxc, *coef;
-  int rc = field(g, x, &fx) | field(g, xc, &fxc) | field(g, "coef", &coef);
+  read_tuning(g);
+  Field *fx, *fxc, *coef, *bq;
+  int rc = field(g, x, &fx) | field(g, xc, &fxc) | field(g, "coef", &coef) | field(g, "qbp", &bq);
   if (rc) return rc;
-  const int dc = depth_class(g->maxdepth);
-  if (dc < 0) { set_error("column depth %d exceeds 128", g->maxdepth); return -1; }
-  if ((rc = launch_prec<false>(g, dc, coef->d, nullptr, fx->d, fxc->d, g->d_qs, 0))) return rc;
+  if ((rc = launch_factor<double>(g, coef->d, bq->d))) return rc;
+  if ((rc = launch_prec<false, double>(g, g->d_qs, coef->d, nullptr, fx->d, fxc->d, bq->d, OP_NONE, nullptr)))
+    return rc;
   HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+// Tuning knobs: "unroll" levels per load group (1, 2, 4); "reduce" 0 last
+// block / 1 finisher kernel.  -1 restores the default (environment or
+// built-in).
+int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
+  if (!strcmp(key, "unroll")) g->tune_unroll = value;
+  else if (!strcmp(key, "reduce")) g->tune_reduce = value;
+  else { set_error("unknown tuning key %s", key); return -1; }
   return 0;
 }
 

@@ -78,6 +78,9 @@ def lib() -> C.CDLL:
         L.sunhip_kernel_times.argtypes = [vp, _dp, _ip]
         L.sunhip_kernel_times.restype = C.c_int
         L.sunhip_copy_probe.argtypes = [vp, C.c_int64, C.c_int, _dp]
+        L.sunhip_q_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_int,
+                                           C.POINTER(CGStats), _ip]
+        L.sunhip_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -159,6 +162,21 @@ class DeviceGrid:
         else:
             check(lib().sunhip_q_solve_fixed(self.h, C.byref(P), int(fixed), C.byref(S)), "sunhip_q_solve_fixed")
         return S.as_dict()
+
+    def q_solve_mixed(self, epsilon=1e-10, inner_tol=1e-5, max_outer=20, maxiters=2000, check_every=0):
+        """FP32 inner PCG in FP64 iterative refinement; returns stats with
+        niters = refinement steps and inner_iters = FP32 PCG iterations."""
+        P = CGParams(maxiters, 2, 0, epsilon, check_every)
+        S = CGStats()
+        n = C.c_int()
+        check(lib().sunhip_q_solve_mixed(self.h, C.byref(P), inner_tol, max_outer, C.byref(S), C.byref(n)),
+              "sunhip_q_solve_mixed")
+        d = S.as_dict()
+        d["inner_iters"] = n.value
+        return d
+
+    def tune(self, key: str, value: int) -> None:
+        check(lib().sunhip_set_tuning(self.h, key.encode(), int(value)), f"tune {key}")
 
     def h_solve(self, grav, theta, dt, maxiters=1000, precond=1, resnorm=0, epsilon=1e-10, check_every=0):
         P = CGParams(maxiters, precond, resnorm, epsilon, check_every)
