@@ -1,0 +1,179 @@
+"""Checks of the oracle that do not need the reference: dense brute force on
+small grids, closed forms and library routines (so that a dropped term, a
+wrong sign or index, or a transposed operand fails here)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from suntans_amd import cases
+
+import _impls
+
+
+def dense_operator(st, op, n, xname, yname):
+    """Matrix of y = op(x) over the wet cell-layers, column by column."""
+    m = st.wet_mask()
+    idx = np.flatnonzero(m)
+    A = np.zeros((len(idx), len(idx)))
+    base = np.zeros(n)
+    for j, col in enumerate(idx):
+        x = base.copy()
+        x[col] = 1.0
+        st.set(xname, x)
+        st.set(yname, 0.0)
+        op(st)
+        A[:, j] = st.get(yname)[idx]
+    return A, idx
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    return cases.tiny(3)
+
+
+def test_operator_q_symmetric_negative_definite(tiny):
+    o = _impls.oracle()
+    st = tiny.clone()
+    o.qcoef(st)
+    A, _ = dense_operator(st, lambda s: o.operatorq(s), len(st.get("stmp")), "stmp", "stmp2")
+    assert np.allclose(A, A.T, rtol=1e-13, atol=1e-12 * np.abs(A).max())
+    ev = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert ev.max() < 0
+
+
+def test_operator_q_of_constant_is_top_dirichlet_term(tiny):
+    """L(1): horizontal differences vanish, interior and bottom rows vanish,
+    the top row keeps -2 coef[ctop] = -2 Ac/dzz[ctop]."""
+    o = _impls.oracle()
+    st = tiny.clone()
+    o.qcoef(st)
+    st.set("stmp", 1.0)
+    o.operatorq(st)
+    cell, k = st.cell_layers()
+    ctop = st.ints("ctop")
+    y = st.get("stmp2")
+    Ac, dzz = st.get("Ac"), st.get("dzz")
+    top = k == ctop[cell]
+    expect = -2 * Ac[cell[top]] / dzz[top]
+    assert np.allclose(y[top], expect, rtol=1e-14)
+    interior = (k > ctop[cell])
+    assert np.all(np.abs(y[interior]) <= 1e-12 * np.abs(expect).max())
+
+
+def test_preconditioner_inverts_vertical_operator(tiny):
+    """With D = 0 the operator is the vertical part the preconditioner
+    inverts: L_v(M^-1 x) = x."""
+    o = _impls.oracle()
+    st = tiny.clone()
+    o.qcoef(st)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=st.get("stmp").shape)
+    st.set("stmp", x)
+    o.preconditioner(st, x="stmp", xc="uold")
+    st.set("D", 0.0)
+    o.operatorq(st, x="uold", y="stmp2")
+    m = st.wet_mask()
+    assert np.allclose(st.get("stmp2")[m], x[m], rtol=1e-10, atol=1e-10 * np.abs(x).max())
+
+
+def test_trisolve_matches_banded_solver():
+    o = _impls.oracle()
+    import ctypes as C
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 3, 7, 50):
+        a = rng.uniform(0.1, 1, n)
+        c = rng.uniform(0.1, 1, n)
+        b = -(a + c) - rng.uniform(0.1, 1, n)
+        d = rng.normal(size=n)
+        ab = np.zeros((3, n))
+        ab[0, 1:] = c[:-1]
+        ab[1] = b
+        ab[2, :-1] = a[1:]
+        want = scipy.linalg.solve_banded((1, 1), ab, d)
+        u = np.zeros(n)
+        bb, dd = b.copy(), d.copy()
+        o.fn("TriSolve")(a.ctypes.data_as(C.c_void_p), bb.ctypes.data_as(C.c_void_p),
+                         c.ctypes.data_as(C.c_void_p), dd.ctypes.data_as(C.c_void_p),
+                         u.ctypes.data_as(C.c_void_p), n)
+        assert np.allclose(u, want, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("qprecond", [2, 0])
+def test_cgsolveq_matches_dense_solve(tiny, qprecond):
+    o = _impls.oracle()
+    st = tiny.clone()
+    o.qcoef(st)
+    A, idx = dense_operator(st, lambda s: o.operatorq(s), len(st.get("stmp")), "stmp", "stmp2")
+    st = tiny.clone()
+    src = st.get("stmp")
+    q0 = st.get("qc")
+    want = np.linalg.solve(A, src[idx])
+    st.set_prop(qprecond=qprecond, qepsilon=1e-14, qmaxiters=5000)
+    o.cgsolveq(st)
+    got = st.get("qc")[idx]
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+    # the warm start only changes the path, not the answer
+    assert not np.allclose(q0[idx], want)
+
+
+def test_operator_h_of_constant_is_area(tiny):
+    o = _impls.oracle()
+    st = tiny.clone()
+    st.set("D", cases.depth_integrated_D(st))
+    o.hcoef(st)
+    st.set("htmp2", 2.5)
+    o.operatorh(st)
+    assert np.allclose(st.get("htmp3"), 2.5 * st.get("Ac"), rtol=1e-13)
+
+
+@pytest.mark.parametrize("hprecond", [1, 0])
+def test_cgsolve_matches_dense_solve(tiny, hprecond):
+    o = _impls.oracle()
+    st = tiny.clone()
+    st.set("D", cases.depth_integrated_D(st))
+    o.hcoef(st)
+    Nc = st.Nc
+    A = np.zeros((Nc, Nc))
+    for j in range(Nc):
+        e = np.zeros(Nc)
+        e[j] = 1
+        st.set("htmp2", e)
+        o.operatorh(st)
+        A[:, j] = st.get("htmp3")
+    assert np.allclose(A, A.T, rtol=1e-13)
+    assert np.linalg.eigvalsh(A).min() > 0
+    st = tiny.clone()
+    st.set("D", cases.depth_integrated_D(st))
+    b = st.get("htmp")
+    st.set_prop(hprecond=hprecond, epsilon=1e-14, maxiters=5000)
+    o.cgsolve(st)
+    want = np.linalg.solve(A, b)
+    assert np.linalg.norm(st.get("h") - want) <= 1e-11 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("r,superbee,minmod", [(-1, 0, 0), (0, 0, 0), (0.5, 1, 0.5), (1, 1, 1), (2, 2, 1),
+                                               (3, 2, 1), (1.5, 1.5, 1)])
+def test_psi_closed_forms(r, superbee, minmod):
+    o = _impls.oracle()
+    import ctypes as C
+    f = o.fn("Psi")
+    f.argtypes = [C.c_double, C.c_int]
+    f.restype = C.c_double
+    assert f(r, 1) == 0
+    assert f(r, 2) == 1
+    assert f(r, 3) == superbee
+    assert f(r, 4) == minmod
+
+
+@pytest.mark.parametrize("tvd", [0, 1, 3, 4])
+def test_still_water_keeps_any_scalar(tvd):
+    """No flow, no diffusion, fixed layers: UpdateScalars returns s."""
+    o = _impls.oracle()
+    st = cases.tiny(5, move_surface=False)
+    for f in ("u", "utmp2", "w", "wtmp2", "wnew"):
+        st.set(f, 0.0)
+    st.set("dzzold", st.get("dzz"))
+    st.set_prop(TVDsalt=tvd, kappa_s=0.0, n=2)
+    s0 = st.get("s")
+    o.update_salinity(st)
+    assert np.allclose(st.get("s"), s0, rtol=1e-15, atol=0)
