@@ -139,6 +139,13 @@ int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
 int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
                    sunhip_cg_stats *st);
 int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y);
+/* No reference counterpart: the mixed-precision free surface.  FP64 true
+ * residual r = htmp - L h (h = 0 on computational cells first, which lifts
+ * the boundary values like the reference), FP32 PCG on L e = r (dots in
+ * FP64) to inner_tol relative reduction, h += e in FP64, until
+ * sqrt(r.r/r0.r0) < prm->epsilon.  st->niters counts refinement steps. */
+int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                         double inner_tol, int max_outer, sunhip_cg_stats *st, int *inner_iters);
 /* OperatorH with caller-supplied coefficients: "hcoef" (2-D cell field) and
  * the face coefficients flat as [Nc*maxfaces] in the reference's indexing. */
 int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char *x, const char *y);

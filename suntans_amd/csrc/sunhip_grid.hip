@@ -405,7 +405,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
-                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32};
+                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);

@@ -2,23 +2,105 @@
 //
 // Reference: CGSolve main/phys.c:3086-3217, HCoefficients :3252-3269,
 // OperatorH :3335-3349, HPreconditioner :3227-3235, InnerProduct :3280-3294.
-// Same two-kernel iteration as the pressure solve, one thread per cell.
+// Same two-kernel iteration as the pressure solve, one thread per cell, with
+// the scalars on the device and the block sums finished by a one-block
+// kernel in a fixed order.  Templated on the storage type like the pressure
+// solve: T=double is the drop-in, T=float the inner solve of the
+// mixed-precision form (sunhip_h_solve_mixed).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "sunhip_common.h"
 
 using namespace sunhip;
 
-enum { HM_PLAIN = 0, HM_RESID = 1, HM_FUSED = 2 };
+enum { HM_PLAIN = 0, HM_RESID = 1, HM_FUSED = 2, HM_RESID_DOT = 3 };
+enum { HOP_APPLY = 1, HOP_SETUP = 2, HOP_ITER = 3, HOP_STORE = 4 };
 
+template <typename T, typename TO = T>
 struct HVec {
-  const double *coef, *fcoef;   // hcoef [NS], hfcoef [MAXF*NS]
-  const double *vin, *src;
-  const double *rtmp, *pold;
-  double *pnew, *x, *out;
+  const T *coef, *fcoef;   // hcoef [NS], hfcoef [MAXF*NS]
+  const T *vin, *src;
+  const T *rtmp, *pold;
+  T *pnew, *x;
+  TO *out;
 };
+
+__device__ void h_op(CGScal *sc, int op, const double *tot) {
+  const int pc = sc->precond;
+  switch (op) {
+    case HOP_APPLY:
+      sc->pz = tot[0];
+      sc->muinv = 1 / sc->alpha;
+      sc->nu = sc->alpha / tot[0];
+      break;
+    case HOP_SETUP:  // main/phys.c:3137-3158
+      sc->alpha = sc->alpha0 = pc == 1 ? tot[0] : tot[1];
+      if (!sc->resnorm) sc->alpha0 = 1;
+      sc->eps = sc->eps0 = pc == 1 ? tot[1] : sc->alpha0;
+      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
+      sc->done = (sc->eps == 0 || sc->alpha == 0 || sc->maxit <= 0);
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      break;
+    case HOP_ITER: {  // main/phys.c:3175-3203
+      const double alpha = pc == 1 ? tot[0] : tot[1];
+      sc->mu = sc->muinv * alpha;
+      sc->alpha = alpha;
+      sc->eps = pc == 1 ? tot[1] : alpha;
+      sc->iter += 1;
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
+      else if (sc->iter >= sc->maxit || sc->eps == 0 || sc->alpha == 0) { sc->done = 1; sc->nloop = sc->iter; }
+      break;
+    }
+    case HOP_STORE:
+      sc->pz = tot[0];
+      break;
+  }
+}
+
+template <int NV>
+__device__ void h_partials(const double (&v)[NV], double *part) {
+  __shared__ double sh[NV][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double s = wave_sum(v[i]);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) part[(size_t)blockIdx.x * NV + i] = ((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(1024) void k_hfinalize(const double *__restrict__ part, int nparts, CGScal *sc, int op) {
+  if ((op == HOP_APPLY || op == HOP_ITER) && sc->done) return;
+  __shared__ double sh[NV][16];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nparts; b += 1024) s += part[(size_t)b * NV + i];
+    s = wave_sum(s);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0;
+      for (int j = 0; j < 16; j++) s += sh[i][j];
+      tot[i] = s;
+    }
+    h_op(sc, op, tot);
+  }
+}
 
 // HCoefficients: fcoef = g(theta dt)^2 D df/dg on faces with a neighbour,
 // coef = Ac + sum fcoef, added in face order.
@@ -37,115 +119,119 @@ __global__ __launch_bounds__(256) void k_hcoef(Geo G, const double *__restrict__
   coef[s] = c;
 }
 
-__global__ void k_hzero_comp(Geo G, double *__restrict__ x) {
+template <typename T>
+__global__ void k_hzero_comp(Geo G, T *__restrict__ x) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) x[s] = 0;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_happly(Geo G, HVec V, CGScal *sc, double *part, unsigned *cnt) {
+template <int MODE, typename T, typename TO>
+__global__ __launch_bounds__(256) void k_happly(Geo G, HVec<T, TO> V, CGScal *sc, double *part) {
   if (MODE == HM_FUSED && sc->done) return;
   const int s = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   double dot = 0;
   if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) {
-    const double mu = MODE == HM_FUSED ? sc->mu : 0.0, nu = MODE == HM_FUSED ? sc->nu : 0.0;
-    double po0 = 0, v0;
+    const T mu = MODE == HM_FUSED ? (T)sc->mu : T(0), nu = MODE == HM_FUSED ? (T)sc->nu : T(0);
+    T po0 = 0, v0;
     if (MODE == HM_FUSED) { po0 = V.pold[s]; v0 = V.rtmp[s] + mu * po0; } else v0 = V.vin[s];
-    double y = V.coef[s] * v0;
+    T y = V.coef[s] * v0;
     for (int f = 0; f < G.nf[s]; f++) {
       const int nb = G.nbs[f * G.NS + s];
       if (nb < 0) continue;
-      const double vn = MODE == HM_FUSED ? V.rtmp[nb] + mu * V.pold[nb] : V.vin[nb];
+      const T vn = MODE == HM_FUSED ? V.rtmp[nb] + mu * V.pold[nb] : V.vin[nb];
       y -= V.fcoef[f * G.NS + s] * vn;
     }
     if (MODE == HM_FUSED) {
       V.pnew[s] = v0;
       V.x[s] = V.x[s] + nu * po0;
       V.out[s] = y;
-      dot = v0 * y;
+      dot = (double)v0 * (double)y;
     } else if (MODE == HM_RESID) {
       // p = p - z ; r = p (main/phys.c:3122-3128)
       V.out[s] = V.src[s] - y;
+    } else if (MODE == HM_RESID_DOT) {
+      const T rk = V.src[s] - y;
+      V.out[s] = (TO)rk;
+      dot = (double)rk * (double)rk;
     } else {
       V.out[s] = y;
     }
   }
-  if (MODE == HM_FUSED) {
-    double v[1] = {dot}, tot[1];
-    if (grid_reduce<1, 256>(v, part, cnt, tot) && threadIdx.x == 0) {
-      sc->pz = tot[0];
-      sc->muinv = 1 / sc->alpha;
-      sc->nu = sc->alpha / tot[0];
-    }
+  if (MODE == HM_FUSED || MODE == HM_RESID_DOT) {
+    double v[1] = {dot};
+    h_partials<1>(v, part);
   }
 }
 
-// r -= nu*z ; rtmp = r/hcoef (hprecond=1) or r ; dots; scalars.
-__global__ __launch_bounds__(256) void k_hprec(Geo G, const double *__restrict__ coef, const double *__restrict__ z,
-                                               double *__restrict__ r, double *__restrict__ rt, CGScal *sc,
-                                               double *part, unsigned *cnt, int finish, int update) {
+// r -= nu*z ; rtmp = r/hcoef (hprecond=1) or r ; dots.
+template <typename T>
+__global__ __launch_bounds__(256) void k_hprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
+                                               T *__restrict__ r, T *__restrict__ rt, CGScal *sc, double *part,
+                                               int update) {
   if (update && sc->done) return;
   const int s = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   double rz = 0, rr = 0;
   const int pc = sc->precond;
   if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) {
-    double rk = r[s];
-    if (update) { rk = rk - sc->nu * z[s]; r[s] = rk; }
-    const double u = pc == 1 ? rk / coef[s] : rk;
+    T rk = r[s];
+    if (update) { rk = rk - (T)sc->nu * z[s]; r[s] = rk; }
+    const T u = pc == 1 ? rk / coef[s] : rk;
     rt[s] = u;
-    rz = rk * u;
-    rr = rk * rk;
+    rz = (double)rk * (double)u;
+    rr = (double)rk * (double)rk;
   }
-  double v[2] = {rz, rr}, tot[2];
-  if (grid_reduce<2, 256>(v, part, cnt, tot) && threadIdx.x == 0) {
-    if (finish == 1) {
-      sc->alpha = sc->alpha0 = pc == 1 ? tot[0] : tot[1];
-      if (!sc->resnorm) sc->alpha0 = 1;
-      sc->eps = sc->eps0 = pc == 1 ? tot[1] : sc->alpha0;
-      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
-      sc->done = (sc->eps == 0 || sc->alpha == 0 || sc->maxit <= 0);
-      sc->resid = sqrt(sc->eps / sc->eps0);
-    } else {
-      const double alpha = pc == 1 ? tot[0] : tot[1];
-      sc->mu = sc->muinv * alpha;
-      sc->alpha = alpha;
-      sc->eps = pc == 1 ? tot[1] : alpha;
-      sc->iter += 1;
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
-      else if (sc->iter >= sc->maxit || sc->eps == 0 || sc->alpha == 0) { sc->done = 1; sc->nloop = sc->iter; }
-    }
-  }
+  double v[2] = {rz, rr};
+  h_partials<2>(v, part);
 }
 
-__global__ void k_hfinish(Geo G, double *__restrict__ x, const double *__restrict__ p0,
-                          const double *__restrict__ p1, const CGScal *sc) {
+template <typename T>
+__global__ void k_hfinish(Geo G, T *__restrict__ x, const T *__restrict__ p0, const T *__restrict__ p1,
+                          const CGScal *sc) {
   if (sc->iter == 0) return;
-  const double *__restrict__ p = (sc->iter & 1) ? p1 : p0;
+  const T *__restrict__ p = (sc->iter & 1) ? p1 : p0;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) x[s] = x[s] + sc->nu * p[s];
+  if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) x[s] = x[s] + (T)sc->nu * p[s];
 }
 
-__global__ void k_hzero(double *p, int n) {
+template <typename T>
+__global__ void k_hfill(T *p, int n, T v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+  if (i < n) p[i] = v;
+}
+
+template <typename TI, typename TO>
+__global__ void k_hcast(const TI *__restrict__ a, TO *__restrict__ b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (TO)a[i];
+}
+
+__global__ void k_hmix_update(Geo G, double *__restrict__ x, const float *__restrict__ e) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) x[s] = x[s] + (double)e[s];
 }
 
 namespace {
+
+template <typename T>
+struct HSolver {
+  CGScal *sc;
+  const T *coef, *fcoef;
+  T *x, *r, *z, *rt, *p0, *p1;
+};
 
 struct HBufs {
   Field *h, *src, *D, *coef, *r, *z, *rt, *p0, *p1;
   double *fcoef;
 };
 
+int nbh(const sunhip_grid *g) { return (g->NS + 255) / 256; }
 
 int hbufs(sunhip_grid *g, HBufs &B) {
   int rc = field(g, "h", &B.h) | field(g, "htmp", &B.src) | field(g, "D", &B.D) | field(g, "hcoef", &B.coef) |
            field(g, "hr", &B.r) | field(g, "hz", &B.z) | field(g, "hrt", &B.rt) | field(g, "hp0", &B.p0) |
            field(g, "hp1", &B.p1);
   if (rc) return rc;
-  Field *fc;
-  // face coefficients: MAXF rows of NS, kept as a cell field of its own size
+  // face coefficients: MAXF rows of NS
   auto it = g->fields.find("#hfcoef");
   if (it == g->fields.end()) {
     Field f;
@@ -155,34 +241,87 @@ int hbufs(sunhip_grid *g, HBufs &B) {
     HIPCHK(hipMemsetAsync(f.d, 0, f.n * sizeof(double), g->stream));
     it = g->fields.emplace("#hfcoef", f).first;
   }
-  fc = &it->second;
-  B.fcoef = fc->d;
+  B.fcoef = it->second.d;
   return 0;
 }
 
 int h_coefficients(sunhip_grid *g, HBufs &B, double grav, double theta, double dt) {
   const double td = theta * dt;
   const double tmp = grav * (td * td);  // grav*pow(theta*dt,2)
-  k_hcoef<<<(g->NS + 255) / 256, 256, 0, g->stream>>>(g->geo(), B.D->d, tmp, B.coef->d, B.fcoef);
+  k_hcoef<<<nbh(g), 256, 0, g->stream>>>(g->geo(), B.D->d, tmp, B.coef->d, B.fcoef);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int h_iteration(sunhip_grid *g, HBufs &B, int it) {
+template <typename T>
+int h_start(sunhip_grid *g, HSolver<T> &S) {
+  const int nb = nbh(g);
+  k_hprec<T><<<nb, 256, 0, g->stream>>>(g->geo(), S.coef, nullptr, S.r, S.rt, S.sc, g->d_part, 0);
+  k_hfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_SETUP);
+  k_hfill<T><<<nb, 256, 0, g->stream>>>(S.p0, g->NS, T(0));
+  k_hfill<T><<<nb, 256, 0, g->stream>>>(S.p1, g->NS, T(0));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
   const Geo G = g->geo();
-  HVec V{};
-  V.coef = B.coef->d; V.fcoef = B.fcoef; V.rtmp = B.rt->d;
-  V.pold = (it & 1) ? B.p1->d : B.p0->d;
-  V.pnew = (it & 1) ? B.p0->d : B.p1->d;
-  V.x = B.h->d; V.out = B.z->d;
-  const int nb = (g->NS + 255) / 256;
-  int ps = prof_begin(g, PK_HAPPLY);
-  k_happly<HM_FUSED><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part, g->d_count + 8);
+  HVec<T, T> V{};
+  V.coef = S.coef; V.fcoef = S.fcoef; V.rtmp = S.rt;
+  V.pold = (it & 1) ? S.p1 : S.p0;
+  V.pnew = (it & 1) ? S.p0 : S.p1;
+  V.x = S.x; V.out = S.z;
+  const int nb = nbh(g);
+  int ps = prof_begin(g, pk_a);
+  k_happly<HM_FUSED, T, T><<<nb, 256, 0, g->stream>>>(G, V, S.sc, g->d_part);
+  k_hfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_APPLY);
   prof_end(g, ps);
-  ps = prof_begin(g, PK_HPREC);
-  k_hprec<<<nb, 256, 0, g->stream>>>(G, B.coef->d, B.z->d, B.r->d, B.rt->d, g->d_hs, g->d_part, g->d_count + 9, 2, 1);
+  ps = prof_begin(g, pk_p);
+  k_hprec<T><<<nb, 256, 0, g->stream>>>(G, S.coef, S.z, S.r, S.rt, S.sc, g->d_part, 1);
+  k_hfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_ITER);
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a, int pk_p) {
+  int every = prm->check_every > 0 ? prm->check_every : 32;
+  if (every & 1) every++;
+  HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  int it = 0, rc;
+  while (!g->h_sc->done && it < prm->maxiters) {
+    for (int b = 0; b < every && it < prm->maxiters; b++, it++)
+      if ((rc = h_iteration<T>(g, S, it, pk_a, pk_p))) return rc;
+    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+  }
+  k_hfinish<T><<<nbh(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int upload_hscal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) {
+  CGScal h{};
+  h.tol = prm->epsilon; h.maxit = prm->maxiters; h.precond = prm->precond; h.resnorm = prm->resnorm;
+  *g->h_sc = h;
+  HIPCHK(hipMemcpyAsync(d, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+int fill_stats(sunhip_grid *g, const CGScal &c, sunhip_cg_stats *st) {
+  if (!st) return 0;
+  st->niters = c.nloop;
+  st->converged = c.converged;
+  st->resid = c.resid;
+  st->eps0 = c.eps0;
+  float ms = 0;
+  hipEventElapsedTime(&ms, g->ev1, g->ev2);
+  st->device_ms = ms;
+  hipEventElapsedTime(&ms, g->ev0, g->ev1);
+  st->setup_ms = ms;
   return 0;
 }
 
@@ -201,51 +340,88 @@ int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, dou
   int rc = hbufs(g, B);
   if (rc) return rc;
   const Geo G = g->geo();
-  const int nb = (g->NS + 255) / 256;
-  CGScal h{};
-  h.tol = prm->epsilon; h.maxit = prm->maxiters; h.precond = prm->precond; h.resnorm = prm->resnorm;
-  *g->h_sc = h;
+  const int nb = nbh(g);
   HIPCHK(hipEventRecord(g->ev0, g->stream));
-  HIPCHK(hipMemcpyAsync(g->d_hs, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
+  if ((rc = upload_hscal(g, g->d_hs, prm))) return rc;
   if ((rc = h_coefficients(g, B, grav, theta, dt))) return rc;
   // x = 0 on computational cells; r = b - L x lifts type-3 boundary values
-  k_hzero_comp<<<nb, 256, 0, g->stream>>>(G, B.h->d);
-  HVec V{};
+  k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, B.h->d);
+  HVec<double, double> V{};
   V.coef = B.coef->d; V.fcoef = B.fcoef; V.vin = B.h->d; V.src = B.src->d; V.out = B.r->d;
-  k_happly<HM_RESID><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part, g->d_count + 8);
-  k_hprec<<<nb, 256, 0, g->stream>>>(G, B.coef->d, nullptr, B.r->d, B.rt->d, g->d_hs, g->d_part, g->d_count + 9, 1, 0);
-  k_hzero<<<nb, 256, 0, g->stream>>>(B.p0->d, g->NS);
-  k_hzero<<<nb, 256, 0, g->stream>>>(B.p1->d, g->NS);
+  k_happly<HM_RESID, double, double><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part);
   HIPCHK(hipGetLastError());
+  HSolver<double> S{g->d_hs, B.coef->d, B.fcoef, B.h->d, B.r->d, B.z->d, B.rt->d, B.p0->d, B.p1->d};
+  if ((rc = h_start<double>(g, S))) return rc;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
-  int every = prm->check_every > 0 ? prm->check_every : 32;
-  if (every & 1) every++;
-  HIPCHK(hipMemcpyAsync(g->h_sc, g->d_hs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  int it = 0;
-  while (!g->h_sc->done && it < prm->maxiters) {
-    for (int b = 0; b < every && it < prm->maxiters; b++, it++)
-      if ((rc = h_iteration(g, B, it))) return rc;
-    HIPCHK(hipMemcpyAsync(g->h_sc, g->d_hs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
-  }
-  k_hfinish<<<nb, 256, 0, g->stream>>>(G, B.h->d, B.p0->d, B.p1->d, g->d_hs);
-  HIPCHK(hipGetLastError());
+  if ((rc = h_loop<double>(g, prm, S, PK_HAPPLY, PK_HPREC))) return rc;
   HIPCHK(hipMemcpyAsync(g->h_sc, g->d_hs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipEventRecord(g->ev2, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
-  if (st) {
-    st->niters = g->h_sc->nloop;
-    st->converged = g->h_sc->converged;
-    st->resid = g->h_sc->resid;
-    st->eps0 = g->h_sc->eps0;
-    float ms = 0;
-    hipEventElapsedTime(&ms, g->ev1, g->ev2);
-    st->device_ms = ms;
-    hipEventElapsedTime(&ms, g->ev0, g->ev1);
-    st->setup_ms = ms;
+  return fill_stats(g, *g->h_sc, st);
+}
+
+// Mixed precision free surface: FP64 true residual r = b - L h (after the
+// reference's boundary lift with h = 0 inside), FP32 inner Jacobi-PCG on
+// L e = r to inner_tol (adaptive toward the target), h += e in FP64, until
+// sqrt(r.r / r0.r0) < epsilon.
+int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                         double inner_tol, int max_outer, sunhip_cg_stats *st, int *inner_iters) {
+  HIPCHK(hipSetDevice(g->device));
+  HBufs B;
+  int rc = hbufs(g, B);
+  if (rc) return rc;
+  float *c32, *f32, *x32, *r32, *z32, *t32, *pa, *pb;
+  if (!g->d_h32) {
+    if ((rc = dev_alloc(g, (void **)&g->d_h32, sizeof(float) * (size_t)g->NS * (7 + SUNHIP_MAXF)))) return rc;
   }
-  return 0;
+  c32 = g->d_h32; x32 = c32 + g->NS; r32 = x32 + g->NS; z32 = r32 + g->NS; t32 = z32 + g->NS;
+  pa = t32 + g->NS; pb = pa + g->NS; f32 = pb + g->NS;
+  const Geo G = g->geo();
+  const int nb = nbh(g);
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  if ((rc = h_coefficients(g, B, grav, theta, dt))) return rc;
+  k_hcast<double, float><<<nb, 256, 0, g->stream>>>(B.coef->d, c32, g->NS);
+  k_hcast<double, float><<<(SUNHIP_MAXF * g->NS + 255) / 256, 256, 0, g->stream>>>(B.fcoef, f32, SUNHIP_MAXF * g->NS);
+  k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, B.h->d);
+  HIPCHK(hipGetLastError());
+  sunhip_cg_params inner = *prm;
+  inner.resnorm = 0;
+  inner.precond = prm->precond;
+  HSolver<float> S{g->d_ms, c32, f32, x32, r32, z32, t32, pa, pb};
+  HVec<double, float> R{};
+  R.coef = B.coef->d; R.fcoef = B.fcoef; R.vin = B.h->d; R.src = B.src->d; R.out = r32;
+  double eps0 = -1, eps = 0;
+  int outer = 0, total = 0, converged = 0;
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  for (;;) {
+    k_happly<HM_RESID_DOT, double, float><<<nb, 256, 0, g->stream>>>(G, R, g->d_mo, g->d_part);
+    k_hfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, g->d_mo, HOP_STORE);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(g->h_sc, g->d_mo, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    eps = g->h_sc->pz;
+    if (eps0 < 0) eps0 = eps;
+    const double ratio = eps == 0 ? 0.0 : sqrt(eps / eps0);
+    if (eps == 0 || ratio < prm->epsilon) { converged = 1; break; }
+    if (outer >= max_outer) break;
+    inner.epsilon = std::max(inner_tol, 0.25 * prm->epsilon / ratio);
+    if ((rc = upload_hscal(g, S.sc, &inner))) return rc;
+    k_hfill<float><<<nb, 256, 0, g->stream>>>(x32, g->NS, 0.0f);
+    if ((rc = h_start<float>(g, S))) return rc;
+    if ((rc = h_loop<float>(g, &inner, S, PK_OTHER, PK_OTHER))) return rc;
+    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    total += g->h_sc->iter;
+    k_hmix_update<<<nb, 256, 0, g->stream>>>(G, B.h->d, x32);
+    HIPCHK(hipGetLastError());
+    outer++;
+  }
+  HIPCHK(hipEventRecord(g->ev2, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  CGScal c{};
+  c.nloop = outer; c.converged = converged; c.resid = eps0 > 0 ? sqrt(eps / eps0) : 0; c.eps0 = eps0;
+  if (inner_iters) *inner_iters = total;
+  return fill_stats(g, c, st);
 }
 
 int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y) {
@@ -256,9 +432,9 @@ int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, cons
   Field *fx, *fy;
   if ((rc = field(g, x, &fx) | field(g, y, &fy))) return rc;
   if ((rc = h_coefficients(g, B, grav, theta, dt))) return rc;
-  HVec V{};
+  HVec<double, double> V{};
   V.coef = B.coef->d; V.fcoef = B.fcoef; V.vin = fx->d; V.out = fy->d;
-  k_happly<HM_PLAIN><<<(g->NS + 255) / 256, 256, 0, g->stream>>>(g->geo(), V, g->d_hs, g->d_part, g->d_count + 8);
+  k_happly<HM_PLAIN, double, double><<<nbh(g), 256, 0, g->stream>>>(g->geo(), V, g->d_hs, g->d_part);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;
@@ -279,9 +455,9 @@ int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char
     for (int f = 0; f < g->nfaces[c]; f++) fc[(size_t)f * g->NS + s] = hfcoef_flat[(size_t)c * g->mf + f];
   }
   HIPCHK(hipMemcpyAsync(B.fcoef, fc.data(), fc.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
-  HVec V{};
+  HVec<double, double> V{};
   V.coef = B.coef->d; V.fcoef = B.fcoef; V.vin = fx->d; V.out = fy->d;
-  k_happly<HM_PLAIN><<<(g->NS + 255) / 256, 256, 0, g->stream>>>(g->geo(), V, g->d_hs, g->d_part, g->d_count + 8);
+  k_happly<HM_PLAIN, double, double><<<nbh(g), 256, 0, g->stream>>>(g->geo(), V, g->d_hs, g->d_part);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;

@@ -116,6 +116,7 @@ struct sunhip_grid {
   std::map<std::string, sunhip::Field> fields;
   std::map<std::string, float *> fields32;  // FP32 copies for the mixed form (cell layout)
   float *d_D32 = nullptr;
+  float *d_h32 = nullptr;  // mixed free surface: FP32 coefficients and CG vectors
   int64_t bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 

@@ -80,6 +80,8 @@ def lib() -> C.CDLL:
         L.sunhip_copy_probe.argtypes = [vp, C.c_int64, C.c_int, _dp]
         L.sunhip_q_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_int,
                                            C.POINTER(CGStats), _ip]
+        L.sunhip_h_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_int, C.POINTER(CGStats), _ip]
         L.sunhip_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
         _lib = L
     return _lib
@@ -183,6 +185,18 @@ class DeviceGrid:
         S = CGStats()
         check(lib().sunhip_h_solve(self.h, C.byref(P), grav, theta, dt, C.byref(S)), "sunhip_h_solve")
         return S.as_dict()
+
+    def h_solve_mixed(self, grav, theta, dt, epsilon=1e-10, inner_tol=1e-5, max_outer=20, maxiters=1000,
+                      precond=1, check_every=0):
+        """FP32 inner free-surface PCG in FP64 iterative refinement."""
+        P = CGParams(maxiters, precond, 0, epsilon, check_every)
+        S = CGStats()
+        n = C.c_int()
+        check(lib().sunhip_h_solve_mixed(self.h, C.byref(P), grav, theta, dt, inner_tol, max_outer, C.byref(S),
+                                         C.byref(n)), "sunhip_h_solve_mixed")
+        d = S.as_dict()
+        d["inner_iters"] = n.value
+        return d
 
     def update_scalars(self, theta, dt, kappa, TVD, step=2, scal="s"):
         S = ScalarParams(theta, dt, kappa, TVD, step, 0, 0)

@@ -1,0 +1,110 @@
+"""Mixed-precision forms (FP32 inner PCG inside FP64 iterative refinement),
+called through the C ABI of libsunhip.so.  They have no counterpart in the
+reference, so they are checked against what they promise:
+
+* the FP64 true residual of the returned solution, recomputed on the CPU with
+  the oracle's OperatorQ / OperatorH, satisfies ||b - L x|| / ||b - L x0||
+  < epsilon (the stopping rule), to within 1e-3 relative for the different
+  summation order of the norms;
+* the solution is within 1e-8 relative 2-norm of a tight (epsilon=1e-14) FP64
+  oracle solve, the same distance the FP64 drop-in reaches at epsilon=1e-10
+  to within a factor of 100.
+"""
+import numpy as np
+import pytest
+
+from suntans_amd import cases, device
+
+import _impls
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-10
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+MAKERS = [(f"tiny{s}", (lambda s=s: cases.tiny(s))) for s in (0, 1, 2)]
+MAKERS.append(("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)))
+MAKERS.append(("lockexchange", lambda: cases.lock_exchange(seed=7)))
+
+
+def q_residual(o, st, q):
+    s = st.clone()
+    o.qcoef(s)
+    s.set("uold", q)
+    o.operatorq(s, x="uold", y="stmp2")
+    m = st.wet_mask()
+    return (st.get("stmp") - s.get("stmp2"))[m]
+
+
+@pytest.mark.parametrize("inner_tol", [1e-3, 1e-5])
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_q_mixed(gpu_available, name, make, inner_tol):
+    o, d = _impls.oracle(), _impls.dropin()
+    st = make()
+    st.set_prop(qprecond=2, resnorm=0)
+    q0 = st.get("qc")
+    dg = device.DeviceGrid(st)
+    for n, f in (("q", "qc"), ("src", "stmp"), ("dzz", "dzz"), ("dzf", "dzf"), ("D", "D")):
+        dg.upload(n, st.get(f))
+    r = dg.q_solve_mixed(epsilon=EPS, inner_tol=inner_tol, max_outer=30)
+    q = dg.download("q", len(q0))
+    dg.close()
+    assert r["converged"] == 1, r
+    res = np.linalg.norm(q_residual(o, st, q)) / np.linalg.norm(q_residual(o, st, q0))
+    tight = st.clone()
+    tight.set_prop(qepsilon=1e-14, qmaxiters=20000)
+    o.cgsolveq(tight)
+    f64 = st.clone()
+    f64.set_prop(qepsilon=EPS, qmaxiters=2000)
+    d.cgsolveq(f64)
+    m = st.wet_mask()
+    e_mix = relerr(q[m], tight.get("qc")[m])
+    e_64 = relerr(f64.get("qc")[m], tight.get("qc")[m])
+    print(f"{name} inner_tol={inner_tol}: outer={r['niters']} inner={r['inner_iters']} "
+          f"true resid={res:.3e} err vs tight: mixed={e_mix:.3e} fp64={e_64:.3e}")
+    assert res < EPS * (1 + 1e-3)
+    assert abs(res - r["resid"]) <= 1e-3 * r["resid"] + 1e-20
+    assert e_mix <= 1e-8 and e_mix <= max(100 * e_64, 1e-12)
+
+
+def h_residual(o, st, h):
+    s = st.clone()
+    o.hcoef(s)
+    s.set("htmp2", h)
+    o.operatorh(s)
+    return st.get("htmp") - s.get("htmp3")
+
+
+@pytest.mark.parametrize("hprecond", [1, 0])
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_h_mixed(gpu_available, name, make, hprecond):
+    o = _impls.oracle()
+    st = make()
+    st.set("D", cases.depth_integrated_D(st))
+    st.set_prop(hprecond=hprecond)
+    comp = np.zeros(st.Nc, bool)
+    cd = st.ints("celldist")
+    comp[cd[0]:cd[1]] = True
+    h0 = st.get("h")
+    h0[comp] = 0
+    dg = device.DeviceGrid(st)
+    for n in ("h", "htmp", "D"):
+        dg.upload(n, st.get(n))
+    r = dg.h_solve_mixed(st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt"), epsilon=EPS,
+                         inner_tol=1e-4, precond=hprecond, max_outer=30)
+    h = dg.download("h", st.Nc)
+    dg.close()
+    assert r["converged"] == 1, r
+    res = np.linalg.norm(h_residual(o, st, h)[comp]) / np.linalg.norm(h_residual(o, st, h0)[comp])
+    tight = st.clone()
+    tight.set_prop(epsilon=1e-14, maxiters=20000)
+    o.cgsolve(tight)
+    e_mix = relerr(h[comp], tight.get("h")[comp])
+    print(f"{name} hprecond={hprecond}: outer={r['niters']} inner={r['inner_iters']} true resid={res:.3e} "
+          f"err vs tight={e_mix:.3e}")
+    assert res < EPS * (1 + 1e-3)
+    assert e_mix <= 1e-8
