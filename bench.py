@@ -8,9 +8,11 @@ on 100 levels of 2 m (ragged columns, ~5.0e7 pressure unknowns), source
 CGSolveQ with the vertical-line preconditioner (qprecond=2) from q=0 to
 sqrt(r.r/r0.r0) < 1e-10, all fields resident in HBM.
 
-Prints one JSON line (see the keys below).  Extra fields report the
-free-surface solve, the mixed-precision form and the roofline of the
-dominant kernel.
+Beside the headline FP64 figure the JSON line carries the mixed-precision
+form on the same grid and source (FP32 inner PCG in FP64 refinement: speed-up
+and relative error against the FP64 solve and against a tight 1e-14 FP64
+solve), the free-surface solve (FP64 and mixed) on the same grid, per-solve
+latency, the roofline of the dominant kernel and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -28,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_large_solve.txt")
+
+KERNEL_NAMES = {"qapply": "k_qapply<2,3,4,double,double>", "qprec": "k_qprec<true,4,double>"}
 
 
 def log(msg: str) -> None:
@@ -50,9 +55,14 @@ def q_bytes(N: int, Ek: int) -> dict:
         "iteration_ref": 8 * (14 * N + Ek),
         # apply kernel: read rtmp, p_old, x, coef, dzf(E_k); write p_new, z, x
         "qapply": 8 * (7 * N + Ek),
-        # precondition kernel: read r, z, coef; write r, rtmp
+        # precondition kernel: read r, z, coef; write r, rtmp (the stored
+        # Thomas pivots b' are this build's choice and are not counted)
         "qprec": 8 * (5 * N),
     }
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def cpu_baseline(st, gpu_iters: int, budget_s: float):
@@ -102,6 +112,17 @@ def cpu_baseline(st, gpu_iters: int, budget_s: float):
     }
 
 
+def traffic_for(kernel_key: str):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (copy-calibrated FETCH_SIZE + WRITE_SIZE), None if absent."""
+    from suntans_amd import profiling
+    t = profiling.load_traffic(PMC_SUMMARY)
+    name = KERNEL_NAMES[kernel_key]
+    if name not in t:
+        return None
+    return t[name][0] + t[name][1]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,14 +131,18 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nx", type=int, default=1000)
     ap.add_argument("--ny", type=int, default=500)
+    ap.add_argument("--mixed-steps", type=int, default=3)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--skip-cpu", action="store_true", help="leave out the CPU baseline leg")
+    ap.add_argument("--skip-extra", action="store_true",
+                    help="only the FP64 pressure solves (for short profiled runs)")
     ap.add_argument("--cpu-budget", type=float, default=40.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by whole columns across GPUs, not built here)", file=sys.stderr)
+              "(the path would shard by whole columns across GPUs, which is not built here)", file=sys.stderr)
         sys.exit(2)
 
     from suntans_amd import cases, device
@@ -129,13 +154,11 @@ def main():
         sys.exit(3)
     t = time.time()
     dg = device.DeviceGrid(st)
-    dg.upload("q", st.get("qc"))
-    dg.upload("src", st.get("stmp"))
-    dg.upload("dzz", st.get("dzz"))
-    dg.upload("dzf", st.get("dzf"))
-    dg.upload("D", st.get("D"))
+    for name, f in (("q", "qc"), ("src", "stmp"), ("dzz", "dzz"), ("dzf", "dzf"), ("D", "D")):
+        dg.upload(name, st.get(f))
     dg.copy("q0", "q")
     dg.sync()
+    nflat = dg.sizes()["Nflat"]
     log(f"uploaded in {time.time() - t:.1f}s, device bytes {dg.bytes() / 2**30:.2f} GiB")
 
     prm = dict(maxiters=2000, precond=2, resnorm=0, epsilon=1e-10)
@@ -144,6 +167,7 @@ def main():
         s = dg.q_solve(**prm)
         log(f"warmup {w}: {s['niters']} iterations, resid {s['resid']:.2e}, {s['device_ms']:.1f} ms")
 
+    # ---- timed FP64 solves ------------------------------------------------
     dg.profile(True)
     dg.sync()
     stats = []
@@ -162,26 +186,11 @@ def main():
     its = float(np.mean(iters)) + 1  # the loop ran n+1 times when it broke at n
     B = q_bytes(N, Ek)
     iter_ms = float(np.mean([s["device_ms"] for s in stats])) / its
-    # dominant kernel
     dom = max(("qapply", "qprec"), key=lambda k: kt.get(k, (0, 0))[0])
     dom_ms = kt[dom][0] / max(kt[dom][1], 1)
     achieved = B[dom] / (dom_ms * 1e-3) / 1e9
-    sol = dg.download("q", dg.sizes()["Nflat"])
-
-    # free-surface solve on the same grid (D = depth-integrated flux heights)
-    dg.upload("D", cases.depth_integrated_D(st))
-    dg.upload("htmp", st.get("htmp"))
-    hs = []
-    for k in range(3):
-        dg.upload("h", np.zeros(st.Nc))
-        hs.append(dg.h_solve(st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt"), maxiters=1000,
-                             precond=1, epsilon=1e-10))
-    h_ms = float(np.median([s["setup_ms"] + s["device_ms"] for s in hs]))
-    dg.upload("D", st.get("D"))
-
-    if args.copy_probe:
-        cp_ms = dg.copy_probe(1 << 30, 20)
-        log(f"copy probe: 1 GiB in {cp_ms:.3f} ms = {2 * (1 << 30) / cp_ms / 1e6:.0f} GB/s")
+    x64 = dg.download("q", nflat)
+    traffic = traffic_for(dom)
 
     out = {
         "metric": "Poisson-CG solves/s at 1e-10 rel resid (FP64 CGSolveQ, vertical-line PCG)",
@@ -209,20 +218,85 @@ def main():
         "latency_ms_p99": float(np.percentile(lat, 99)),
         "iteration_ms": iter_ms,
         "iteration_GBps_ref_bytes": B["iteration_ref"] / (iter_ms * 1e-3) / 1e9,
+        "iteration_roofline_frac_ref_bytes": B["iteration_ref"] / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in kt.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_qapply" if dom == "qapply" else "k_qprec",
+            "kernel": KERNEL_NAMES[dom],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": B[dom],
+            "launch_ms": dom_ms,
         },
-        "free_surface": {"Nc": st.Nc, "iterations": [s["niters"] for s in hs], "ms_p50": h_ms},
-        "solution_norm": float(np.linalg.norm(sol)),
     }
+
+    if not args.skip_extra:
+        # ---- tight FP64 reference solution --------------------------------
+        dg.copy("q", "q0")
+        tight = dg.q_solve(maxiters=5000, precond=2, resnorm=0, epsilon=1e-14)
+        xt = dg.download("q", nflat)
+        # ---- mixed precision on the same grid and source ------------------
+        dg.copy("q", "q0")
+        dg.q_solve_mixed(epsilon=1e-10, inner_tol=args.inner_tol)  # warm-up
+        dg.profile(True)
+        mix = []
+        t0 = time.perf_counter()
+        for k in range(args.mixed_steps):
+            dg.copy("q", "q0")
+            mix.append(dg.q_solve_mixed(epsilon=1e-10, inner_tol=args.inner_tol))
+        dg.sync()
+        mwall = time.perf_counter() - t0
+        mkt = dg.kernel_times()
+        dg.profile(False)
+        xm = dg.download("q", nflat)
+        mlat = np.array([s["setup_ms"] + s["device_ms"] for s in mix])
+        out["mixed"] = {
+            "value": args.mixed_steps / mwall, "unit": "solves/s",
+            "speedup_vs_fp64": (args.mixed_steps / mwall) / out["value"],
+            "refinement_steps": mix[0]["niters"], "inner_fp32_iterations": mix[0]["inner_iters"],
+            "true_resid": mix[0]["resid"], "inner_tol": args.inner_tol,
+            "rel_err_q_vs_fp64": relerr(xm, x64), "rel_err_q_vs_tight": relerr(xm, xt),
+            "rel_err_fp64_vs_tight": relerr(x64, xt), "tight_iterations": tight["niters"],
+            "latency_ms_p50": float(np.percentile(mlat, 50)), "latency_ms_p99": float(np.percentile(mlat, 99)),
+            "kernels_ms_per_launch": {k: v[0] / max(v[1], 1) for k, v in mkt.items()},
+        }
+        log(f"mixed: {out['mixed']}")
+        dg.copy("q", "q0")
+
+        # ---- free surface on the same grid (D = depth-integrated) ---------
+        dg.upload("D", cases.depth_integrated_D(st))
+        dg.upload("htmp", st.get("htmp"))
+        g, th, dt = st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt")
+        zero = np.zeros(st.Nc)
+
+        def hrun(fn, n=3, **kw):
+            res = []
+            for _ in range(n):
+                dg.upload("h", zero)
+                res.append(fn(g, th, dt, **kw))
+            return res, dg.download("h", st.Nc)
+
+        h64, xh = hrun(dg.h_solve, maxiters=1000, precond=1, epsilon=1e-10)
+        ht, xht = hrun(dg.h_solve, n=1, maxiters=10000, precond=1, epsilon=1e-14)
+        hm, xhm = hrun(dg.h_solve_mixed, epsilon=1e-10, inner_tol=args.inner_tol, precond=1)
+        ms = lambda rs: float(np.median([s["setup_ms"] + s["device_ms"] for s in rs]))
+        out["free_surface"] = {
+            "Nc": st.Nc, "iterations": h64[0]["niters"], "fp64_ms_p50": ms(h64),
+            "mixed_ms_p50": ms(hm), "mixed_refinement_steps": hm[0]["niters"],
+            "mixed_inner_iterations": hm[0]["inner_iters"],
+            "rel_err_h_mixed_vs_fp64": relerr(xhm, xh), "rel_err_h_mixed_vs_tight": relerr(xhm, xht),
+            "rel_err_h_fp64_vs_tight": relerr(xh, xht),
+        }
+        dg.upload("D", st.get("D"))
+
+    if args.copy_probe:
+        cp_ms = dg.copy_probe(1 << 30, 20)
+        log(f"copy probe: 1 GiB in {cp_ms:.3f} ms = {2 * (1 << 30) / cp_ms / 1e6:.0f} GB/s")
+        out["copy_probe_GBps"] = 2 * (1 << 30) / cp_ms / 1e6
+
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(st, int(round(its)), args.cpu_budget)
     else:

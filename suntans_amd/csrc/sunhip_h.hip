@@ -275,12 +275,12 @@ int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
   const int nb = nbh(g);
   int ps = prof_begin(g, pk_a);
   k_happly<HM_FUSED, T, T><<<nb, 256, 0, g->stream>>>(G, V, S.sc, g->d_part);
-  k_hfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_APPLY);
   prof_end(g, ps);
+  k_hfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_APPLY);
   ps = prof_begin(g, pk_p);
   k_hprec<T><<<nb, 256, 0, g->stream>>>(G, S.coef, S.z, S.r, S.rt, S.sc, g->d_part, 1);
-  k_hfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_ITER);
   prof_end(g, ps);
+  k_hfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_ITER);
   HIPCHK(hipGetLastError());
   return 0;
 }

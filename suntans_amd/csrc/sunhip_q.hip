@@ -508,15 +508,18 @@ int launch_factor(sunhip_grid *g, const T *coef, T *bq) {
 // (OP_NONE for the plain drop-in application).
 template <bool UPDATE, typename T>
 int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *out, const T *bq, int op,
-                unsigned *cnt) {
+                unsigned *cnt, int pk = -1) {
   const Geo G = g->geo();
   const int nb = nb4(g);
   Red R{g->d_part, 0, (g->tune_reduce == 0 && op != OP_NONE) ? cnt : nullptr, op};
+  // the timing pair brackets the column kernel alone, not the finisher
+  const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
   switch (g->tune_unroll) {
     case 1: k_qprec<UPDATE, 1, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
     case 2: k_qprec<UPDATE, 2, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
     default: k_qprec<UPDATE, 4, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
   }
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   if (op != OP_NONE && g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, sc, op);
   HIPCHK(hipGetLastError());
@@ -524,9 +527,10 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
 }
 
 template <int MODE, typename T, typename TO>
-int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R) {
+int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R, int pk = -1) {
   const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
   const Geo G = g->geo();
+  const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
 #define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, sc, R)
   if (g->mf <= 3) {
     if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2); else APPLY(3, 4);
@@ -534,6 +538,7 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R)
     if (g->tune_unroll == 1) APPLY(4, 1); else if (g->tune_unroll == 2) APPLY(4, 2); else APPLY(4, 4);
   }
 #undef APPLY
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nba, sc, R.op);
   HIPCHK(hipGetLastError());
@@ -589,15 +594,13 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
   V.pnew = (it & 1) ? S.p0 : S.p1;
   V.x = S.x;
   V.out = S.z;
-  int ps = prof_begin(g, pk_apply);
-  int rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, Red{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY});
-  prof_end(g, ps);
+  int rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, Red{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY},
+                                        pk_apply);
   if (rc) return rc;
-  ps = prof_begin(g, pk_prec);
   if (prm->precond == 2)
-    rc = launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1);
-  else
-    rc = launch_noprec<T>(g, S, 1, OP_NOPREC_ITER);
+    return launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1, pk_prec);
+  const int ps = prof_begin(g, pk_prec);
+  rc = launch_noprec<T>(g, S, 1, OP_NOPREC_ITER);
   prof_end(g, ps);
   return rc;
 }
@@ -775,9 +778,7 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
   HIPCHK(hipEventRecord(g->ev1, g->stream));
   for (;;) {
     // FP64 true residual, cast to FP32 for the inner solve
-    int ps = prof_begin(g, PK_RESID);
-    rc = launch_apply<QM_RESID_DOT, double, float>(g, so, R, Red{g->d_part, 0, nullptr, OP_STORE});
-    prof_end(g, ps);
+    rc = launch_apply<QM_RESID_DOT, double, float>(g, so, R, Red{g->d_part, 0, nullptr, OP_STORE}, PK_RESID);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(g->h_sc, so, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
