@@ -54,6 +54,13 @@ void sunhip_dropin_reset(void);
 /* No counterpart in the reference: select the HIP device used by the
  * drop-ins (default 0). */
 void sunhip_dropin_set_mode(int device);
+/* No counterpart in the reference: mixed = 1 makes CGSolveQ and CGSolve run
+ * the mixed-precision form (FP32 inner PCG inside FP64 iterative refinement,
+ * FP64 true-residual test against the same qepsilon/epsilon); inner_tol is
+ * the inner relative tolerance (<= 0 keeps the current one, default 1e-4).
+ * sunhip_last_solve then reports the FP32 iterations.  mixed = 0 (default)
+ * is the FP64 form whose iterates follow the reference's. */
+void sunhip_dropin_set_precision(int mixed, double inner_tol);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ typedef struct {
 static ctx_t g_ctx[MAXCTX];
 static int g_nctx = 0;
 static int g_device = 0;
+static int g_mixed = 0;           /* 1: CGSolveQ/CGSolve use the mixed-precision form */
+static double g_inner_tol = 1e-4;  /* its inner FP32 tolerance */
 static struct { int valid; sunhip_cg_stats st; } g_last[2];
 
 static void fatal(const char *where, const char *msg) {
@@ -174,7 +176,14 @@ void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *p
   upload_edges(C, grid, "dzf", grid->dzf, W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
   sunhip_cg_params P = {prop->qmaxiters, prop->qprecond, (int)prop->resnorm, prop->qepsilon, 0};
-  check(sunhip_q_solve(C->dev, &P, &g_last[0].st), W);
+  if (g_mixed) {
+    if (prop->qprecond != 2) fatal(W, "the mixed-precision form needs qprecond=2");
+    int inner = 0;
+    check(sunhip_q_solve_mixed(C->dev, &P, g_inner_tol, 50, &g_last[0].st, &inner), W);
+    g_last[0].st.niters = inner;
+  } else {
+    check(sunhip_q_solve(C->dev, &P, &g_last[0].st), W);
+  }
   g_last[0].valid = 1;
   check(sunhip_download(C->dev, "q", C->flat), W);
   unpack_cells(C, grid, q, 0, 1);
@@ -191,7 +200,14 @@ void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MP
   check(sunhip_upload(C->dev, "htmp", phys->htmp), W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
   sunhip_cg_params P = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0};
-  check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
+  if (g_mixed) {
+    int inner = 0;
+    check(sunhip_h_solve_mixed(C->dev, &P, prop->grav, prop->theta, prop->dt, g_inner_tol, 50, &g_last[1].st,
+                               &inner), W);
+    g_last[1].st.niters = inner;
+  } else {
+    check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
+  }
   g_last[1].valid = 1;
   check(sunhip_download(C->dev, "h", C->flat), W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
@@ -354,6 +370,11 @@ void sunhip_dropin_reset(void) {
   }
   memset(g_ctx, 0, sizeof g_ctx);
   g_nctx = 0;
+}
+
+void sunhip_dropin_set_precision(int mixed, double inner_tol) {
+  g_mixed = mixed ? 1 : 0;
+  if (inner_tol > 0) g_inner_tol = inner_tol;
 }
 
 void sunhip_dropin_set_mode(int device) {

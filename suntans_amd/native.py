@@ -106,5 +106,6 @@ def dropin_lib() -> C.CDLL:
         L.sunhip_last_solve.restype = C.c_int
         L.sunhip_dropin_reset.argtypes = []
         L.sunhip_dropin_set_mode.argtypes = [C.c_int]
+        L.sunhip_dropin_set_precision.argtypes = [C.c_int, C.c_double]
         _drop = L
     return _drop

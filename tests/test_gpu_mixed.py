@@ -108,3 +108,39 @@ def test_h_mixed(gpu_available, name, make, hprecond):
           f"err vs tight={e_mix:.3e}")
     assert res < EPS * (1 + 1e-3)
     assert e_mix <= 1e-8
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
+def test_dropin_mixed_switch(gpu_available, name, make):
+    """The reference signatures in mixed mode (sunhip_dropin_set_precision):
+    CGSolveQ and CGSolve reach the same accuracy bound against a tight
+    oracle solve, and mode 0 restores the FP64 form."""
+    o, d = _impls.oracle(), _impls.dropin()
+    a = make()
+    tight = a.clone()
+    tight.set_prop(qepsilon=1e-14, qmaxiters=20000)
+    o.cgsolveq(tight)
+    b = a.clone()
+    d.L.sunhip_dropin_set_precision(1, 1e-4)
+    try:
+        d.cgsolveq(b)
+        nq, _ = d.stats("q")
+        hb = a.clone()
+        hb.set("D", cases.depth_integrated_D(hb))
+        ht = hb.clone()
+        ht.set_prop(epsilon=1e-14, maxiters=20000)
+        o.cgsolve(ht)
+        d.cgsolve(hb)
+    finally:
+        d.L.sunhip_dropin_set_precision(0, 0.0)
+    m = a.wet_mask()
+    eq = relerr(b.get("qc")[m], tight.get("qc")[m])
+    eh = relerr(hb.get("h"), ht.get("h"))
+    print(f"{name}: mixed drop-in q err {eq:.3e} ({nq} FP32 iterations), h err {eh:.3e}")
+    assert eq <= 1e-8 and eh <= 1e-8
+    c = a.clone()
+    d.cgsolveq(c)
+    f = a.clone()
+    o.cgsolveq(f)
+    assert relerr(c.get("qc")[m], f.get("qc")[m]) <= 1e-10
+    assert abs(d.stats("q")[0] - o.stats("q")[0]) <= 1

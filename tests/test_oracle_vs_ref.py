@@ -20,8 +20,10 @@ IDS = [m[0] for m in MAKERS]
 
 
 @pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
-@pytest.mark.parametrize("qprecond", [2, 0])
+@pytest.mark.parametrize("qprecond", [2, 0, 1])
 def test_cgsolveq(name, make, qprecond):
+    """qprecond=1 runs ConditionQ/OperatorQC with the reference's aliasing of
+    the scaling vector c and the residual r, which the oracle reproduces."""
     o = _impls.oracle()
     a = make()
     b = a.clone()
