@@ -340,6 +340,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
 
   int rc = 0;
   HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  g->own_stream = g->stream;
   rc |= upload_vec(g, &g->d_toff, g->toff);
   rc |= upload_vec(g, &g->d_tdep, g->tdep);
   rc |= upload_vec(g, &g->d_eoff, g->eoff);
@@ -398,6 +399,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   if (!g) return 0;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
+  if (g->batch) sunhip_batch_detach(g->batch, g);
   for (auto &kv : g->fields) hipFree(kv.second.d);
   for (auto &kv : g->fields32) hipFree(kv.second);
   void *ptrs[] = {g->d_toff, g->d_tdep, g->d_eoff, g->d_edep, g->d_nk, g->d_ctop, g->d_ctopold, g->d_comp,
@@ -412,7 +414,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
   if (g->ev2) hipEventDestroy(g->ev2);
-  if (g->stream) hipStreamDestroy(g->stream);
+  if (g->own_stream) hipStreamDestroy(g->own_stream);
   delete g;
   return 0;
 }

@@ -12,54 +12,9 @@
 #include <algorithm>
 #include <cmath>
 
-#include "sunhip_common.h"
+#include "sunhip_solvers.h"
 
 using namespace sunhip;
-
-enum { HM_PLAIN = 0, HM_RESID = 1, HM_FUSED = 2, HM_RESID_DOT = 3 };
-enum { HOP_APPLY = 1, HOP_SETUP = 2, HOP_ITER = 3, HOP_STORE = 4 };
-
-template <typename T, typename TO = T>
-struct HVec {
-  const T *coef, *fcoef;   // hcoef [NS], hfcoef [MAXF*NS]
-  const T *vin, *src;
-  const T *rtmp, *pold;
-  T *pnew, *x;
-  TO *out;
-};
-
-__device__ void h_op(CGScal *sc, int op, const double *tot) {
-  const int pc = sc->precond;
-  switch (op) {
-    case HOP_APPLY:
-      sc->pz = tot[0];
-      sc->muinv = 1 / sc->alpha;
-      sc->nu = sc->alpha / tot[0];
-      break;
-    case HOP_SETUP:  // main/phys.c:3137-3158
-      sc->alpha = sc->alpha0 = pc == 1 ? tot[0] : tot[1];
-      if (!sc->resnorm) sc->alpha0 = 1;
-      sc->eps = sc->eps0 = pc == 1 ? tot[1] : sc->alpha0;
-      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
-      sc->done = (sc->eps == 0 || sc->alpha == 0 || sc->maxit <= 0);
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      break;
-    case HOP_ITER: {  // main/phys.c:3175-3203
-      const double alpha = pc == 1 ? tot[0] : tot[1];
-      sc->mu = sc->muinv * alpha;
-      sc->alpha = alpha;
-      sc->eps = pc == 1 ? tot[1] : alpha;
-      sc->iter += 1;
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
-      else if (sc->iter >= sc->maxit || sc->eps == 0 || sc->alpha == 0) { sc->done = 1; sc->nloop = sc->iter; }
-      break;
-    }
-    case HOP_STORE:
-      sc->pz = tot[0];
-      break;
-  }
-}
 
 template <int NV>
 __device__ void h_partials(const double (&v)[NV], double *part) {
@@ -129,34 +84,8 @@ template <int MODE, typename T, typename TO>
 __global__ __launch_bounds__(256) void k_happly(Geo G, HVec<T, TO> V, CGScal *sc, double *part) {
   if (MODE == HM_FUSED && sc->done) return;
   const int s = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  double dot = 0;
-  if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) {
-    const T mu = MODE == HM_FUSED ? (T)sc->mu : T(0), nu = MODE == HM_FUSED ? (T)sc->nu : T(0);
-    T po0 = 0, v0;
-    if (MODE == HM_FUSED) { po0 = V.pold[s]; v0 = V.rtmp[s] + mu * po0; } else v0 = V.vin[s];
-    T y = V.coef[s] * v0;
-    for (int f = 0; f < G.nf[s]; f++) {
-      const int nb = G.nbs[f * G.NS + s];
-      if (nb < 0) continue;
-      const T vn = MODE == HM_FUSED ? V.rtmp[nb] + mu * V.pold[nb] : V.vin[nb];
-      y -= V.fcoef[f * G.NS + s] * vn;
-    }
-    if (MODE == HM_FUSED) {
-      V.pnew[s] = v0;
-      V.x[s] = V.x[s] + nu * po0;
-      V.out[s] = y;
-      dot = (double)v0 * (double)y;
-    } else if (MODE == HM_RESID) {
-      // p = p - z ; r = p (main/phys.c:3122-3128)
-      V.out[s] = V.src[s] - y;
-    } else if (MODE == HM_RESID_DOT) {
-      const T rk = V.src[s] - y;
-      V.out[s] = (TO)rk;
-      dot = (double)rk * (double)rk;
-    } else {
-      V.out[s] = y;
-    }
-  }
+  const T mu = MODE == HM_FUSED ? (T)sc->mu : T(0), nu = MODE == HM_FUSED ? (T)sc->nu : T(0);
+  const double dot = happly_cell<MODE, T, TO>(G, V, mu, nu, s);
   if (MODE == HM_FUSED || MODE == HM_RESID_DOT) {
     double v[1] = {dot};
     h_partials<1>(v, part);
@@ -171,15 +100,7 @@ __global__ __launch_bounds__(256) void k_hprec(Geo G, const T *__restrict__ coef
   if (update && sc->done) return;
   const int s = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   double rz = 0, rr = 0;
-  const int pc = sc->precond;
-  if (s < G.NS && G.nk[s] > 0 && G.comp[s] == 1) {
-    T rk = r[s];
-    if (update) { rk = rk - (T)sc->nu * z[s]; r[s] = rk; }
-    const T u = pc == 1 ? rk / coef[s] : rk;
-    rt[s] = u;
-    rz = (double)rk * (double)u;
-    rr = (double)rk * (double)rk;
-  }
+  hprec_cell<T>(G, coef, z, r, rt, update ? (T)sc->nu : T(0), sc->precond, update != 0, s, rz, rr);
   double v[2] = {rz, rr};
   h_partials<2>(v, part);
 }
@@ -211,13 +132,6 @@ __global__ void k_hmix_update(Geo G, double *__restrict__ x, const float *__rest
 }
 
 namespace {
-
-template <typename T>
-struct HSolver {
-  CGScal *sc;
-  const T *coef, *fcoef;
-  T *x, *r, *z, *rt, *p0, *p1;
-};
 
 struct HBufs {
   Field *h, *src, *D, *coef, *r, *z, *rt, *p0, *p1;
@@ -303,13 +217,7 @@ int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a,
   return 0;
 }
 
-int upload_hscal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) {
-  CGScal h{};
-  h.tol = prm->epsilon; h.maxit = prm->maxiters; h.precond = prm->precond; h.resnorm = prm->resnorm;
-  *g->h_sc = h;
-  HIPCHK(hipMemcpyAsync(d, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
-  return 0;
-}
+int upload_hscal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) { return upload_scal(g, d, prm); }
 
 int fill_stats(sunhip_grid *g, const CGScal &c, sunhip_cg_stats *st) {
   if (!st) return 0;
@@ -327,10 +235,9 @@ int fill_stats(sunhip_grid *g, const CGScal &c, sunhip_cg_stats *st) {
 
 }  // namespace
 
-extern "C" {
-
-int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
-                   sunhip_cg_stats *st) {
+namespace sunhip {
+int h_begin64(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+              HSolver<double> *S, CGScal *stage) {
   HIPCHK(hipSetDevice(g->device));
   if (prm->precond != 0 && prm->precond != 1) {
     set_error("hprecond=%d not supported (0 or 1)", prm->precond);
@@ -341,8 +248,7 @@ int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, dou
   if (rc) return rc;
   const Geo G = g->geo();
   const int nb = nbh(g);
-  HIPCHK(hipEventRecord(g->ev0, g->stream));
-  if ((rc = upload_hscal(g, g->d_hs, prm))) return rc;
+  if ((rc = upload_scal(g, g->d_hs, prm, stage))) return rc;
   if ((rc = h_coefficients(g, B, grav, theta, dt))) return rc;
   // x = 0 on computational cells; r = b - L x lifts type-3 boundary values
   k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, B.h->d);
@@ -350,8 +256,26 @@ int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, dou
   V.coef = B.coef->d; V.fcoef = B.fcoef; V.vin = B.h->d; V.src = B.src->d; V.out = B.r->d;
   k_happly<HM_RESID, double, double><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part);
   HIPCHK(hipGetLastError());
-  HSolver<double> S{g->d_hs, B.coef->d, B.fcoef, B.h->d, B.r->d, B.z->d, B.rt->d, B.p0->d, B.p1->d};
-  if ((rc = h_start<double>(g, S))) return rc;
+  *S = HSolver<double>{g->d_hs, B.coef->d, B.fcoef, B.h->d, B.r->d, B.z->d, B.rt->d, B.p0->d, B.p1->d};
+  return h_start<double>(g, *S);
+}
+
+int h_end64(sunhip_grid *g, const HSolver<double> &S) {
+  k_hfinish<double><<<nbh(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace sunhip
+
+extern "C" {
+
+int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                   sunhip_cg_stats *st) {
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  HSolver<double> S;
+  int rc = h_begin64(g, prm, grav, theta, dt, &S, nullptr);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
   if ((rc = h_loop<double>(g, prm, S, PK_HAPPLY, PK_HPREC))) return rc;
   HIPCHK(hipMemcpyAsync(g->h_sc, g->d_hs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));

@@ -80,9 +80,13 @@ struct Geo {
 
 }  // namespace sunhip
 
+struct sunhip_batch;
+
 struct sunhip_grid {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where this grid's work is enqueued
+  hipStream_t own_stream = nullptr;  // the grid's own stream (stream differs while attached to a batch)
+  sunhip_batch *batch = nullptr;     // batch this grid is attached to, if any
   int Nc = 0, Ne = 0, mf = 0, Nkmax = 0;
   int Nt = 0, Net = 0, NS = 0, NES = 0;
   int64_t NT = 0, NE = 0;           // padded element counts (cells, edges)

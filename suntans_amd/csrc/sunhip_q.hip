@@ -34,27 +34,12 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "sunhip_common.h"
+#include "sunhip_solvers.h"
 
 using namespace sunhip;
 
 #define QAPPLY_THREADS 256
 #define TILES_PER_APPLY (QAPPLY_THREADS / SUNHIP_TILE)
-
-enum { QM_PLAIN = 0, QM_RESID = 1, QM_FUSED = 2, QM_RESID_DOT = 3 };
-// finish operations on the CG scalars
-enum { OP_NONE = 0, OP_APPLY = 1, OP_PREC_SETUP = 2, OP_PREC_ITER = 3, OP_NOPREC_SETUP = 4, OP_NOPREC_ITER = 5,
-       OP_STORE = 6 };
-
-template <typename T, typename TO = T>
-struct QVec {
-  const T *coef, *dzf, *D;
-  const T *vin;            // PLAIN / RESID: operand
-  const T *src;            // RESID: right-hand side
-  const T *rtmp, *pold;    // FUSED: p = rtmp + mu*pold
-  T *pnew, *x;             // FUSED
-  TO *out;                 // L p, or src - L x
-};
 
 // Where a launch's block sums go: the last block finishes them (cnt set) or
 // they are left at part[pbase + block] for a finisher kernel.
@@ -64,40 +49,6 @@ struct Red {
   unsigned *cnt;
   int op;
 };
-
-__device__ void q_op(CGScal *sc, int op, const double *tot) {
-  switch (op) {
-    case OP_APPLY:  // mu = 1/alpha ; nu = alpha/(p.z) (main/phys.c:2484-2485)
-      sc->pz = tot[0];
-      sc->muinv = 1 / sc->alpha;
-      sc->nu = sc->alpha / tot[0];
-      break;
-    case OP_PREC_SETUP:  // main/phys.c:2452-2475 with qprecond=2
-    case OP_NOPREC_SETUP:
-      sc->alpha = sc->alpha0 = op == OP_PREC_SETUP ? tot[0] : tot[1];
-      if (!sc->resnorm) sc->alpha0 = 1;
-      sc->eps = sc->eps0 = op == OP_PREC_SETUP ? tot[1] : sc->alpha0;
-      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
-      sc->done = (sc->eps == 0 || sc->maxit <= 0);
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      break;
-    case OP_PREC_ITER:  // main/phys.c:2496-2524
-    case OP_NOPREC_ITER: {
-      const double alpha = op == OP_PREC_ITER ? tot[0] : tot[1];
-      sc->mu = sc->muinv * alpha;
-      sc->alpha = alpha;
-      sc->eps = op == OP_PREC_ITER ? tot[1] : alpha;
-      sc->iter += 1;
-      sc->resid = sqrt(sc->eps / sc->eps0);
-      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
-      else if (sc->iter >= sc->maxit || sc->eps == 0) { sc->done = 1; sc->nloop = sc->iter; }
-      break;
-    }
-    case OP_STORE:  // keep the sum (outer residual norm of the mixed form)
-      sc->pz = tot[0];
-      break;
-  }
-}
 
 // Block sums of NV values, then either finish (last block) or publish.
 template <int NV, int NT>
@@ -153,250 +104,38 @@ __global__ __launch_bounds__(1024) void k_qfinalize(const double *__restrict__ p
   }
 }
 
-// QCoefficients, unpreconditioned branch (main/phys.c:3456-3469), every cell.
 __global__ __launch_bounds__(256) void k_qcoef(Geo G, const double *__restrict__ dzz, double *__restrict__ coef) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, n = G.nk[s];
-  if (n == 0) return;
-  const int kt = G.ctop[s];
-  const double Ac = G.Ac[s];
-  const int64_t base = G.toff[t] + l;
-  double dprev = dzz[base + (int64_t)kt * 64];
-  coef[base + (int64_t)kt * 64] = Ac / dprev;
-  for (int k = kt + 1; k < n; k++) {
-    const double dk = dzz[base + (int64_t)k * 64];
-    coef[base + (int64_t)k * 64] = 2 * Ac / (dk + dprev);
-    dprev = dk;
-  }
+  qcoef_tile(G, dzz, coef, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-// The modified diagonal b' of the Thomas elimination of every column
-// (main/util.c:193-196 applied to the tridiagonal of main/phys.c:3584-3601).
-// It depends only on coef, so it is built once per solve; the back
-// substitution of every application reads it.
 template <typename T>
 __global__ __launch_bounds__(256) void k_qfactor(Geo G, const T *__restrict__ coef, T *__restrict__ bq) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
-  if (n == 0 || G.comp[s] != 1 || kt >= n - 1) return;
-  const int64_t base = G.toff[t] + l;
-  T cprev = 0, bprev = 1;
-  T ck = coef[base + (int64_t)kt * 64];
-  for (int k = kt; k < n; k++) {
-    const int64_t idx = base + (int64_t)k * 64;
-    const T ck1 = k < n - 1 ? coef[idx + 64] : T(0);
-    T a = 0, b, c = 0;
-    if (k == kt) { b = -2 * ck - ck1; c = ck1; }
-    else if (k == n - 1) { a = ck; b = -ck; }
-    else { a = ck; b = -ck - ck1; c = ck1; }
-    if (k > kt) b -= a * cprev / bprev;
-    bq[idx] = b;
-    cprev = c; bprev = b; ck = ck1;
-  }
+  qfactor_tile<T>(G, coef, bq, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-// y = L v over the computational columns (main/phys.c:3486-3536).
-// Contributions outside a face's coupling range are dropped with a select,
-// which leaves the sum bit-identical to the reference's loop.
+// y = L v (qapply_tile's modes) over all tiles, TILES_PER_APPLY per block.
 template <int MODE, int NF, int U, typename T, typename TO>
 __global__ __launch_bounds__(QAPPLY_THREADS) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
   if (MODE == QM_FUSED && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * TILES_PER_APPLY + (threadIdx.x >> 6);
-  const int l = threadIdx.x & 63;
-  double dot = 0;
   const T mu = MODE == QM_FUSED ? (T)sc->mu : T(0);
   const T nu = MODE == QM_FUSED ? (T)sc->nu : T(0);
-  const T *__restrict__ coef = V.coef;
-  const T *__restrict__ dzf = V.dzf;
-  const T *__restrict__ rt = V.rtmp;
-  const T *__restrict__ po = V.pold;
-  const T *__restrict__ vin = V.vin;
-  if (t < G.Nt) {
-    const int s = t * 64 + l;
-    const int n = G.nk[s];
-    if (n > 0 && G.comp[s] == 1) {
-      const int kt = G.ctop[s];
-      const int NS = G.NS;
-      const int nf = G.nf[s];
-      const int64_t base = G.toff[t] + l;
-      int64_t nbb[NF], ebb[NF];
-      int kmn[NF], kmx[NF], lim[NF];
-      T Df[NF];
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
-        if (nbf >= 0) {
-          nbb[f] = nbf;
-          ebb[f] = G.eb[f * NS + s];
-          const int kr = G.kr[f * NS + s];
-          kmn[f] = kr & 0xffff;
-          kmx[f] = kr >> 16;
-          Df[f] = V.D[G.es[f * NS + s]];
-        } else {
-          nbb[f] = base; ebb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
-        }
-        lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
-      }
-      T vprev = 0;
-      for (int kb = kt; kb < n; kb += U) {
-        T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
-#pragma unroll
-        for (int j = 0; j <= U; j++) {
-          const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
-          if (MODE == QM_FUSED) { pv[j] = po[idx]; v[j] = rt[idx]; }
-          else { pv[j] = 0; v[j] = vin[idx]; }
-          c[j] = coef[idx];
-        }
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-          const int k = kb + j;
-          xv[j] = MODE == QM_FUSED ? V.x[base + (int64_t)min(k, n) * 64] : T(0);
-#pragma unroll
-          for (int f = 0; f < NF; f++) {
-            const int kk = min(k, lim[f]);
-            const int64_t ni = nbb[f] + (int64_t)kk * 64;
-            nv[j][f] = MODE == QM_FUSED ? rt[ni] + mu * po[ni] : vin[ni];
-            dz[j][f] = dzf[ebb[f] + (int64_t)kk * 64];
-          }
-        }
-        if (MODE == QM_FUSED) {
-#pragma unroll
-          for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
-        }
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-          const int k = kb + j;
-          if (k < n) {
-            const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
-            const T c0 = c[j], cp1 = c[j + 1];
-            T y = 0;
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-              const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
-              y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
-            }
-            if (kt == n - 1) y -= T(2.0) * c0 * v0;
-            else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
-            else if (k == n - 1) y += c0 * vm1 - c0 * v0;
-            else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
-            const int64_t idx = base + (int64_t)k * 64;
-            if (MODE == QM_FUSED) {
-              V.pnew[idx] = v0;
-              V.x[idx] = xv[j] + nu * pv[j];
-              dot += (double)v0 * (double)y;
-              V.out[idx] = y;
-            } else if (MODE == QM_RESID) {
-              V.out[idx] = V.src[idx] - y;
-            } else if (MODE == QM_RESID_DOT) {
-              const T rk = V.src[idx] - y;
-              V.out[idx] = (TO)rk;
-              dot += (double)rk * (double)rk;
-            } else {
-              V.out[idx] = y;
-            }
-          }
-        }
-        vprev = v[U - 1];
-      }
-    }
-  }
+  const double dot = qapply_tile<MODE, NF, U, T, TO>(G, V, mu, nu, t, threadIdx.x & 63);
   if (MODE == QM_FUSED || MODE == QM_RESID_DOT) {
     double dv[1] = {dot};
     reduce_out<1, QAPPLY_THREADS>(dv, R, sc);
   }
 }
 
-// Thomas solve of one column (main/phys.c:3577-3619 with main/util.c:185-203)
-// in the reference's operation order; the single-layer column is
-// xc = -0.5 x / coef.  UPDATE first applies r -= nu*z (main/phys.c:2490-2493)
-// and writes r back.  The forward sweep recomputes b' on the fly and leaves d'
-// in out; the back substitution reads d', the precomputed b' (bq), coef (for
-// c) and r (for r.rtmp).
+// Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
 template <bool UPDATE, int U, typename T>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
                                                CGScal *sc, Red R) {
   if (UPDATE && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
-  if (t < G.Nt) {
-    const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
-    if (n > 0 && G.comp[s] == 1) {
-      const T nu = UPDATE ? (T)sc->nu : T(0);
-      const int64_t base = G.toff[t] + l;
-      if (kt == n - 1) {
-        const int64_t idx = base + (int64_t)kt * 64;
-        T rk = r[idx];
-        if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
-        const T u = T(-0.5) * rk / coef[idx];
-        out[idx] = u;
-        rz += (double)rk * (double)u;
-        rr += (double)rk * (double)rk;
-      } else {
-        T cprev = 0, bprev = 1, dprev = 0;
-        for (int kb = kt; kb < n; kb += U) {
-          T rv[U], zv[U], cv[U + 1];
-#pragma unroll
-          for (int j = 0; j <= U; j++) {
-            const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
-            cv[j] = coef[idx];
-            if (j < U) {
-              rv[j] = r[idx];
-              zv[j] = UPDATE ? z[idx] : T(0);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < U; j++) {
-            const int k = kb + j;
-            if (k < n) {
-              const int64_t idx = base + (int64_t)k * 64;
-              T rk = rv[j];
-              if (UPDATE) { rk = rk - nu * zv[j]; r[idx] = rk; }
-              rr += (double)rk * (double)rk;
-              const T ck = cv[j], ck1 = cv[j + 1];
-              T a = 0, b, c = 0;
-              if (k == kt) { b = -2 * ck - ck1; c = ck1; }
-              else if (k == n - 1) { a = ck; b = -ck; }
-              else { a = ck; b = -ck - ck1; c = ck1; }
-              T d = rk;
-              if (k > kt) {
-                b -= a * cprev / bprev;
-                d -= a * dprev / bprev;
-              }
-              out[idx] = d;
-              cprev = c; bprev = b; dprev = d;
-            }
-          }
-        }
-        T unext = 0, cnext = 0;
-        for (int ke = n - 1; ke >= kt; ke -= U) {
-          T bv[U], dv[U], rv[U], cv[U];
-#pragma unroll
-          for (int j = 0; j < U; j++) {
-            const int64_t idx = base + (int64_t)max(ke - j, kt) * 64;
-            bv[j] = bq[idx];
-            dv[j] = out[idx];
-            rv[j] = r[idx];
-            cv[j] = coef[idx];
-          }
-#pragma unroll
-          for (int j = 0; j < U; j++) {
-            const int k = ke - j;
-            if (k >= kt) {
-              const T b = bv[j], d = dv[j];
-              const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
-              out[base + (int64_t)k * 64] = u;
-              rz += (double)rv[j] * (double)u;
-              unext = u;
-              cnext = cv[j];  // c at level k-1 is coef[k]
-            }
-          }
-        }
-      }
-    }
-  }
+  qprec_tile<UPDATE, U, T>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz, rr);
   double v[2] = {rz, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -406,22 +145,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_qnoprec(Geo G, const T *__restrict__ z, T *__restrict__ r,
                                                  T *__restrict__ out, CGScal *sc, Red R, int update) {
   if (update && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rr = 0;
-  if (t < G.Nt) {
-    const int s = t * 64 + l, n = G.nk[s];
-    if (n > 0 && G.comp[s] == 1) {
-      const T nu = update ? (T)sc->nu : T(0);
-      const int64_t base = G.toff[t] + l;
-      for (int k = G.ctop[s]; k < n; k++) {
-        const int64_t idx = base + (int64_t)k * 64;
-        T rk = r[idx];
-        if (update) { rk = rk - nu * z[idx]; r[idx] = rk; }
-        out[idx] = rk;
-        rr += (double)rk * (double)rk;
-      }
-    }
-  }
+  qnoprec_tile<T>(G, z, r, out, update ? (T)sc->nu : T(0), update != 0, t, threadIdx.x & 63, rr);
   double v[2] = {rr, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -432,17 +158,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_qfinish(Geo G, T *__restrict__ x, const T *__restrict__ p0,
                                                  const T *__restrict__ p1, const CGScal *sc) {
   if (sc->iter == 0) return;
-  const T *__restrict__ p = (sc->iter & 1) ? p1 : p0;
-  const T nu = (T)sc->nu;
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, n = G.nk[s];
-  if (n == 0 || G.comp[s] != 1) return;
-  const int64_t base = G.toff[t] + l;
-  for (int k = G.ctop[s]; k < n; k++) {
-    const int64_t idx = base + (int64_t)k * 64;
-    x[idx] = x[idx] + nu * p[idx];
-  }
+  qfinish_tile<T>(G, x, (sc->iter & 1) ? p1 : p0, (T)sc->nu, blockIdx.x * 4 + (threadIdx.x >> 6),
+                  threadIdx.x & 63);
 }
 
 template <typename T>
@@ -459,29 +176,13 @@ __global__ void k_cast(const TI *__restrict__ a, TO *__restrict__ b, int64_t n) 
 
 // x64 += e32 on the computational columns (mixed form, FP64 accumulation).
 __global__ __launch_bounds__(256) void k_mix_update(Geo G, double *__restrict__ x, const float *__restrict__ e) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, n = G.nk[s];
-  if (n == 0 || G.comp[s] != 1) return;
-  const int64_t base = G.toff[t] + l;
-  for (int k = G.ctop[s]; k < n; k++) {
-    const int64_t idx = base + (int64_t)k * 64;
-    x[idx] = x[idx] + (double)e[idx];
-  }
+  mix_update_tile(G, x, e, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
 // host side
 
 namespace {
-
-// One PCG instance: storage type, its scalars and its buffers.
-template <typename T>
-struct QSolver {
-  CGScal *sc;
-  const T *coef, *dzf, *D;
-  T *x, *r, *z, *rtmp, *p0, *p1, *bq;
-};
 
 void read_tuning(sunhip_grid *g) {
   if (g->tune_unroll < 0) {
@@ -553,17 +254,6 @@ int launch_noprec(sunhip_grid *g, const QSolver<T> &S, int update, int op) {
   HIPCHK(hipGetLastError());
   if (g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, op);
   HIPCHK(hipGetLastError());
-  return 0;
-}
-
-int upload_scal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) {
-  CGScal h{};
-  h.tol = prm->epsilon;
-  h.maxit = prm->maxiters;
-  h.precond = prm->precond;
-  h.resnorm = prm->resnorm;
-  *g->h_sc = h;
-  HIPCHK(hipMemcpyAsync(d, g->h_sc, sizeof h, hipMemcpyHostToDevice, g->stream));
   return 0;
 }
 
@@ -656,17 +346,19 @@ QSolver<double> solver64(sunhip_grid *g, F64 &B) {
 }
 
 // coefficients, b', r = src - L x, then cg_start (main/phys.c:2441-2475)
-int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double> &S) {
+int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double> &S, CGScal *stage = nullptr,
+            CGScal *sc = nullptr) {
   int rc = check_q(g, prm);
   if (rc) return rc;
   if ((rc = f64bufs(g, B))) return rc;
   S = solver64(g, B);
-  if ((rc = upload_scal(g, g->d_qs, prm))) return rc;
+  if (sc) S.sc = sc;
+  if ((rc = upload_scal(g, S.sc, prm, stage))) return rc;
   k_qcoef<<<nb4(g), 256, 0, g->stream>>>(g->geo(), B.dzz->d, B.coef->d);
   if (prm->precond == 2 && (rc = launch_factor<double>(g, B.coef->d, B.bq->d))) return rc;
   QVec<double, double> V{};
   V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = B.r->d;
-  if ((rc = launch_apply<QM_RESID, double, double>(g, g->d_qs, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
+  if ((rc = launch_apply<QM_RESID, double, double>(g, S.sc, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
   return cg_start<double>(g, prm, S);
 }
 
@@ -702,6 +394,77 @@ int field32(sunhip_grid *g, const char *name, float **out) {
     it = g->fields32.emplace(name, p).first;
   }
   *out = it->second;
+  return 0;
+}
+
+int upload_scal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm, CGScal *stage) {
+  CGScal h{};
+  h.tol = prm->epsilon;
+  h.maxit = prm->maxiters;
+  h.precond = prm->precond;
+  h.resnorm = prm->resnorm;
+  if (!stage) stage = g->h_sc;
+  *stage = h;
+  HIPCHK(hipMemcpyAsync(d, stage, sizeof h, hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+int q_begin64(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<double> *S, CGScal *stage, CGScal *sc) {
+  HIPCHK(hipSetDevice(g->device));
+  F64 B;
+  return q_setup(g, prm, B, *S, stage, sc);
+}
+
+int q_end64(sunhip_grid *g, const QSolver<double> &S) {
+  k_qfinish<double><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int q_end32(sunhip_grid *g, const QSolver<float> &S) {
+  k_qfinish<float><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int q_mixed_prepare(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<float> *S32, QVec<double, float> *R) {
+  HIPCHK(hipSetDevice(g->device));
+  int rc = check_q(g, prm);
+  if (rc) return rc;
+  if (prm->precond != 2) { set_error("the mixed form uses the vertical-line preconditioner (qprecond=2)"); return -1; }
+  F64 B;
+  if ((rc = f64bufs(g, B))) return rc;
+  float *c32, *f32, *x32, *r32, *z32, *t32, *pa, *pb, *b32;
+  rc = field32(g, "coef", &c32) | field32(g, "dzf", &f32) | field32(g, "r", &r32) | field32(g, "z", &z32) |
+       field32(g, "rtmp", &t32) | field32(g, "p0", &pa) | field32(g, "p1", &pb) | field32(g, "bq", &b32) |
+       field32(g, "e", &x32);
+  if (rc) return rc;
+  if (!g->d_D32 && (rc = dev_alloc(g, (void **)&g->d_D32, sizeof(float) * (size_t)g->NES))) return rc;
+  const Geo G = g->geo();
+  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
+  k_cast<double, float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(B.coef->d, c32, g->NT);
+  k_cast<double, float><<<(unsigned)((g->NE + 255) / 256), 256, 0, g->stream>>>(B.dzf->d, f32, g->NE);
+  k_cast<double, float><<<(unsigned)((g->NES + 255) / 256), 256, 0, g->stream>>>(B.D->d, g->d_D32, g->NES);
+  HIPCHK(hipGetLastError());
+  if ((rc = launch_factor<float>(g, c32, b32))) return rc;
+  *S32 = QSolver<float>{g->d_ms, c32, f32, g->d_D32, x32, r32, z32, t32, pa, pb, b32};
+  QVec<double, float> V{};
+  V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = r32;
+  *R = V;
+  return 0;
+}
+
+int q_inner_begin(sunhip_grid *g, const sunhip_cg_params *inner, QSolver<float> &S32, CGScal *stage) {
+  int rc;
+  if ((rc = upload_scal(g, S32.sc, inner, stage))) return rc;
+  k_fill<float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(S32.x, g->NT, 0.0f);
+  HIPCHK(hipGetLastError());
+  return cg_start<float>(g, inner, S32);
+}
+
+int q_mix_update(sunhip_grid *g, double *x, const float *e) {
+  k_mix_update<<<nb4(g), 256, 0, g->stream>>>(g->geo(), x, e);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 }  // namespace sunhip
@@ -746,33 +509,15 @@ int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sun
 int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inner_tol, int max_outer,
                          sunhip_cg_stats *st, int *inner_iters) {
   HIPCHK(hipSetDevice(g->device));
-  int rc = check_q(g, prm);
-  if (rc) return rc;
-  if (prm->precond != 2) { set_error("the mixed form uses the vertical-line preconditioner (qprecond=2)"); return -1; }
-  F64 B;
-  if ((rc = f64bufs(g, B))) return rc;
-  float *c32, *f32, *x32, *r32, *z32, *t32, *pa, *pb, *b32;
-  rc = field32(g, "coef", &c32) | field32(g, "dzf", &f32) | field32(g, "r", &r32) | field32(g, "z", &z32) |
-       field32(g, "rtmp", &t32) | field32(g, "p0", &pa) | field32(g, "p1", &pb) | field32(g, "bq", &b32) |
-       field32(g, "e", &x32);
-  if (rc) return rc;
-  if (!g->d_D32 && (rc = dev_alloc(g, (void **)&g->d_D32, sizeof(float) * (size_t)g->NES))) return rc;
-  float *D32 = g->d_D32;
-  const Geo G = g->geo();
   HIPCHK(hipEventRecord(g->ev0, g->stream));
-  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
-  k_cast<double, float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(B.coef->d, c32, g->NT);
-  k_cast<double, float><<<(unsigned)((g->NE + 255) / 256), 256, 0, g->stream>>>(B.dzf->d, f32, g->NE);
-  k_cast<double, float><<<(unsigned)((g->NES + 255) / 256), 256, 0, g->stream>>>(B.D->d, D32, g->NES);
-  HIPCHK(hipGetLastError());
-  if ((rc = launch_factor<float>(g, c32, b32))) return rc;
+  QSolver<float> S;
+  QVec<double, float> R;
+  int rc = q_mixed_prepare(g, prm, &S, &R);
+  if (rc) return rc;
   CGScal *so = g->d_mo;
   sunhip_cg_params inner = *prm;
   inner.epsilon = inner_tol;
   inner.resnorm = 0;
-  QSolver<float> S{g->d_ms, c32, f32, D32, x32, r32, z32, t32, pa, pb, b32};
-  QVec<double, float> R{};
-  R.coef = B.coef->d; R.dzf = B.dzf->d; R.D = B.D->d; R.vin = B.q->d; R.src = B.src->d; R.out = r32;
   double eps0 = -1, eps = 0;
   int outer = 0, total_inner = 0, converged = 0;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
@@ -790,15 +535,12 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
     // inner FP32 PCG from e = 0, asked for no more reduction than the outer
     // target still needs (with a margin for the FP32/FP64 residual gap)
     inner.epsilon = std::max(inner_tol, 0.25 * prm->epsilon / ratio);
-    if ((rc = upload_scal(g, S.sc, &inner))) return rc;
-    k_fill<float><<<(unsigned)((g->NT + 255) / 256), 256, 0, g->stream>>>(x32, g->NT, 0.0f);
-    if ((rc = cg_start<float>(g, &inner, S))) return rc;
+    if ((rc = q_inner_begin(g, &inner, S, nullptr))) return rc;
     if ((rc = cg_loop<float>(g, &inner, S, PK_MAPPLY, PK_MPREC))) return rc;
     HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
     total_inner += g->h_sc->iter;
-    k_mix_update<<<nb4(g), 256, 0, g->stream>>>(G, B.q->d, x32);
-    HIPCHK(hipGetLastError());
+    if ((rc = q_mix_update(g, const_cast<double *>(R.vin), S.x))) return rc;
     outer++;
   }
   HIPCHK(hipEventRecord(g->ev2, g->stream));

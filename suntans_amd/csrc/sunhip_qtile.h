@@ -1,0 +1,361 @@
+// sunhip_qtile.h -- the column arithmetic of the pressure solve, one tile
+// (64 columns, one wavefront) at a time.
+//
+// Reference: QCoefficients main/phys.c:3420-3471, OperatorQ :3486-3536,
+// Preconditioner :3577-3619 with TriSolve main/util.c:185-203, the CG
+// scalar updates of CGSolveQ main/phys.c:2452-2524.
+//
+// These device functions are shared by the single-grid kernels
+// (sunhip_q.hip) and the batched kernels that walk a list of (member, tile)
+// work items over many grids (sunhip_batch.hip), so both run the same
+// operations in the same order.  Element-wise arithmetic keeps the
+// reference's operation order (built without FMA contraction), so the
+// operator and the preconditioner agree with the reference bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "sunhip_common.h"
+
+namespace sunhip {
+
+enum { QM_PLAIN = 0, QM_RESID = 1, QM_FUSED = 2, QM_RESID_DOT = 3 };
+// finish operations on the CG scalars
+enum { OP_NONE = 0, OP_APPLY = 1, OP_PREC_SETUP = 2, OP_PREC_ITER = 3, OP_NOPREC_SETUP = 4, OP_NOPREC_ITER = 5,
+       OP_STORE = 6 };
+
+template <typename T, typename TO = T>
+struct QVec {
+  const T *coef, *dzf, *D;
+  const T *vin;            // PLAIN / RESID: operand
+  const T *src;            // RESID: right-hand side
+  const T *rtmp, *pold;    // FUSED: p = rtmp + mu*pold
+  T *pnew, *x;             // FUSED
+  TO *out;                 // L p, or src - L x
+};
+
+__device__ inline void q_op(CGScal *sc, int op, const double *tot) {
+  switch (op) {
+    case OP_APPLY:  // mu = 1/alpha ; nu = alpha/(p.z) (main/phys.c:2484-2485)
+      sc->pz = tot[0];
+      sc->muinv = 1 / sc->alpha;
+      sc->nu = sc->alpha / tot[0];
+      break;
+    case OP_PREC_SETUP:  // main/phys.c:2452-2475 with qprecond=2
+    case OP_NOPREC_SETUP:
+      sc->alpha = sc->alpha0 = op == OP_PREC_SETUP ? tot[0] : tot[1];
+      if (!sc->resnorm) sc->alpha0 = 1;
+      sc->eps = sc->eps0 = op == OP_PREC_SETUP ? tot[1] : sc->alpha0;
+      sc->mu = 0; sc->nu = 0; sc->iter = 0; sc->nloop = 0; sc->converged = 0;
+      sc->done = (sc->eps == 0 || sc->maxit <= 0);
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      break;
+    case OP_PREC_ITER:  // main/phys.c:2496-2524
+    case OP_NOPREC_ITER: {
+      const double alpha = op == OP_PREC_ITER ? tot[0] : tot[1];
+      sc->mu = sc->muinv * alpha;
+      sc->alpha = alpha;
+      sc->eps = op == OP_PREC_ITER ? tot[1] : alpha;
+      sc->iter += 1;
+      sc->resid = sqrt(sc->eps / sc->eps0);
+      if (sc->resid < sc->tol) { sc->done = 1; sc->converged = 1; sc->nloop = sc->iter - 1; }
+      else if (sc->iter >= sc->maxit || sc->eps == 0) { sc->done = 1; sc->nloop = sc->iter; }
+      break;
+    }
+    case OP_STORE:  // keep the sum (outer residual norm of the mixed form)
+      sc->pz = tot[0];
+      break;
+  }
+}
+
+// QCoefficients, unpreconditioned branch (main/phys.c:3456-3469), every cell.
+__device__ __forceinline__ void qcoef_tile(const Geo &G, const double *__restrict__ dzz, double *__restrict__ coef,
+                                           int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0) return;
+  const int kt = G.ctop[s];
+  const double Ac = G.Ac[s];
+  const int64_t base = G.toff[t] + l;
+  double dprev = dzz[base + (int64_t)kt * 64];
+  coef[base + (int64_t)kt * 64] = Ac / dprev;
+  for (int k = kt + 1; k < n; k++) {
+    const double dk = dzz[base + (int64_t)k * 64];
+    coef[base + (int64_t)k * 64] = 2 * Ac / (dk + dprev);
+    dprev = dk;
+  }
+}
+
+// The modified diagonal b' of the Thomas elimination of every column
+// (main/util.c:193-196 applied to the tridiagonal of main/phys.c:3584-3601).
+// It depends only on coef, so it is built once per solve; the back
+// substitution of every application reads it.
+template <typename T>
+__device__ __forceinline__ void qfactor_tile(const Geo &G, const T *__restrict__ coef, T *__restrict__ bq, int t,
+                                             int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
+  if (n == 0 || G.comp[s] != 1 || kt >= n - 1) return;
+  const int64_t base = G.toff[t] + l;
+  T cprev = 0, bprev = 1;
+  T ck = coef[base + (int64_t)kt * 64];
+  for (int k = kt; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    const T ck1 = k < n - 1 ? coef[idx + 64] : T(0);
+    T a = 0, b, c = 0;
+    if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+    else if (k == n - 1) { a = ck; b = -ck; }
+    else { a = ck; b = -ck - ck1; c = ck1; }
+    if (k > kt) b -= a * cprev / bprev;
+    bq[idx] = b;
+    cprev = c; bprev = b; ck = ck1;
+  }
+}
+
+// y = L v over the computational columns of tile t (main/phys.c:3486-3536),
+// in one of four modes:
+//   PLAIN     out = L vin
+//   RESID     out = src - L vin
+//   FUSED     p = rtmp + mu*pold -> pnew ; x += nu*pold ; out = L p ; returns p.(L p)
+//   RESID_DOT out = (TO)(src - L vin) ; returns the residual's r.r
+// NF faces per cell at most; U levels per load group.  Returns the lane's
+// share of the dot product.
+template <int MODE, int NF, int U, typename T, typename TO>
+__device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V, T mu, T nu, int t, int l) {
+  double dot = 0;
+  if (t >= G.Nt) return 0;
+  const T *__restrict__ coef = V.coef;
+  const T *__restrict__ dzf = V.dzf;
+  const T *__restrict__ rt = V.rtmp;
+  const T *__restrict__ po = V.pold;
+  const T *__restrict__ vin = V.vin;
+  const int s = t * 64 + l;
+  const int n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return 0;
+  const int kt = G.ctop[s];
+  const int NS = G.NS;
+  const int nf = G.nf[s];
+  const int64_t base = G.toff[t] + l;
+  int64_t nbb[NF], ebb[NF];
+  int kmn[NF], kmx[NF], lim[NF];
+  T Df[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
+    if (nbf >= 0) {
+      nbb[f] = nbf;
+      ebb[f] = G.eb[f * NS + s];
+      const int kr = G.kr[f * NS + s];
+      kmn[f] = kr & 0xffff;
+      kmx[f] = kr >> 16;
+      Df[f] = V.D[G.es[f * NS + s]];
+    } else {
+      nbb[f] = base; ebb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
+    }
+    lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+  }
+  T vprev = 0;
+  for (int kb = kt; kb < n; kb += U) {
+    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+#pragma unroll
+    for (int j = 0; j <= U; j++) {
+      const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
+      if (MODE == QM_FUSED) { pv[j] = po[idx]; v[j] = rt[idx]; }
+      else { pv[j] = 0; v[j] = vin[idx]; }
+      c[j] = coef[idx];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+      xv[j] = MODE == QM_FUSED ? V.x[base + (int64_t)min(k, n) * 64] : T(0);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const int kk = min(k, lim[f]);
+        const int64_t ni = nbb[f] + (int64_t)kk * 64;
+        nv[j][f] = MODE == QM_FUSED ? rt[ni] + mu * po[ni] : vin[ni];
+        dz[j][f] = dzf[ebb[f] + (int64_t)kk * 64];
+      }
+    }
+    if (MODE == QM_FUSED) {
+#pragma unroll
+      for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+      if (k < n) {
+        const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
+        const T c0 = c[j], cp1 = c[j + 1];
+        T y = 0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+          y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+        }
+        if (kt == n - 1) y -= T(2.0) * c0 * v0;
+        else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
+        else if (k == n - 1) y += c0 * vm1 - c0 * v0;
+        else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
+        const int64_t idx = base + (int64_t)k * 64;
+        if (MODE == QM_FUSED) {
+          V.pnew[idx] = v0;
+          V.x[idx] = xv[j] + nu * pv[j];
+          dot += (double)v0 * (double)y;
+          V.out[idx] = y;
+        } else if (MODE == QM_RESID) {
+          V.out[idx] = V.src[idx] - y;
+        } else if (MODE == QM_RESID_DOT) {
+          const T rk = V.src[idx] - y;
+          V.out[idx] = (TO)rk;
+          dot += (double)rk * (double)rk;
+        } else {
+          V.out[idx] = y;
+        }
+      }
+    }
+    vprev = v[U - 1];
+  }
+  return dot;
+}
+
+// Vertical-line preconditioner of tile t: (UPDATE) r -= nu*z, then
+// out = M^-1 r by Thomas down the column (b' recomputed, d' stored in out)
+// and back up it (b' from bq); rz and rr collect the lane's r.out and r.r.
+template <bool UPDATE, int U, typename T>
+__device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ coef, const T *__restrict__ z,
+                                           T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq, T nu,
+                                           int t, int l, double &rz, double &rr) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  if (kt == n - 1) {
+    const int64_t idx = base + (int64_t)kt * 64;
+    T rk = r[idx];
+    if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
+    const T u = T(-0.5) * rk / coef[idx];
+    out[idx] = u;
+    rz += (double)rk * (double)u;
+    rr += (double)rk * (double)rk;
+    return;
+  }
+  T cprev = 0, bprev = 1, dprev = 0;
+  for (int kb = kt; kb < n; kb += U) {
+    T rv[U], zv[U], cv[U + 1];
+#pragma unroll
+    for (int j = 0; j <= U; j++) {
+      const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
+      cv[j] = coef[idx];
+      if (j < U) {
+        rv[j] = r[idx];
+        zv[j] = UPDATE ? z[idx] : T(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+      if (k < n) {
+        const int64_t idx = base + (int64_t)k * 64;
+        T rk = rv[j];
+        if (UPDATE) { rk = rk - nu * zv[j]; r[idx] = rk; }
+        rr += (double)rk * (double)rk;
+        const T ck = cv[j], ck1 = cv[j + 1];
+        T a = 0, b, c = 0;
+        if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+        else if (k == n - 1) { a = ck; b = -ck; }
+        else { a = ck; b = -ck - ck1; c = ck1; }
+        T d = rk;
+        if (k > kt) {
+          b -= a * cprev / bprev;
+          d -= a * dprev / bprev;
+        }
+        out[idx] = d;
+        cprev = c; bprev = b; dprev = d;
+      }
+    }
+  }
+  T unext = 0, cnext = 0;
+  for (int ke = n - 1; ke >= kt; ke -= U) {
+    T bv[U], dv[U], rv[U], cv[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t idx = base + (int64_t)max(ke - j, kt) * 64;
+      bv[j] = bq[idx];
+      dv[j] = out[idx];
+      rv[j] = r[idx];
+      cv[j] = coef[idx];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = ke - j;
+      if (k >= kt) {
+        const T b = bv[j], d = dv[j];
+        const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
+        out[base + (int64_t)k * 64] = u;
+        rz += (double)rv[j] * (double)u;
+        unext = u;
+        cnext = cv[j];  // c at level k-1 is coef[k]
+      }
+    }
+  }
+}
+
+// qprecond=0: r -= nu*z, out = r; rr collects r.r (main/phys.c:2505-2519).
+template <typename T>
+__device__ __forceinline__ void qnoprec_tile(const Geo &G, const T *__restrict__ z, T *__restrict__ r,
+                                             T *__restrict__ out, T nu, bool update, int t, int l, double &rr) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  for (int k = G.ctop[s]; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    T rk = r[idx];
+    if (update) { rk = rk - nu * z[idx]; r[idx] = rk; }
+    out[idx] = rk;
+    rr += (double)rk * (double)rk;
+  }
+}
+
+// x += nu*p over tile t (the last iteration's x update).
+template <typename T>
+__device__ __forceinline__ void qfinish_tile(const Geo &G, T *__restrict__ x, const T *__restrict__ p, T nu, int t,
+                                             int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  for (int k = G.ctop[s]; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    x[idx] = x[idx] + nu * p[idx];
+  }
+}
+
+// x64 += e32 over tile t (mixed form, FP64 accumulation).
+__device__ __forceinline__ void mix_update_tile(const Geo &G, double *__restrict__ x, const float *__restrict__ e,
+                                                int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  for (int k = G.ctop[s]; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    x[idx] = x[idx] + (double)e[idx];
+  }
+}
+
+// Fill / cast over tile t's stored levels (padding included).
+template <typename T>
+__device__ __forceinline__ void fill_tile(const Geo &G, T *__restrict__ p, T v, int t, int l) {
+  if (t >= G.Nt) return;
+  for (int64_t i = G.toff[t] + l; i < G.toff[t + 1]; i += 64) p[i] = v;
+}
+
+template <typename TI, typename TO>
+__device__ __forceinline__ void cast_tile(const Geo &G, const TI *__restrict__ a, TO *__restrict__ b, int t, int l) {
+  if (t >= G.Nt) return;
+  for (int64_t i = G.toff[t] + l; i < G.toff[t + 1]; i += 64) b[i] = (TO)a[i];
+}
+
+}  // namespace sunhip
