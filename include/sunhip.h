@@ -166,6 +166,61 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
 /* Synchronise the context's stream. */
 int sunhip_sync(sunhip_grid *g);
 
+/* ---- batched solves of many grids ----------------------------------------
+ * No reference counterpart (the reference solves one grid per process).  A
+ * batch owns one stream; grids attached to it enqueue all their work there.
+ * The PCG iterations of every active solve of every attached grid run in the
+ * same launches (a member table and a (member, tile) work list walked by
+ * persistent kernels, replayed from a hipGraph), each member with its own
+ * scalars and a fixed-order reduction over its own tiles, so a member's
+ * result is the same bit for bit alone or in any mix.  Solves join and leave
+ * between iterations (continuous batching).  Pressure solves use
+ * qprecond=2. */
+typedef struct sunhip_batch sunhip_batch;
+typedef struct { double grav, theta, dt; } sunhip_h_phys;
+
+int sunhip_batch_create(sunhip_batch **b, int device);
+int sunhip_batch_destroy(sunhip_batch *b);
+/* Route a grid's work to the batch's stream (implicit in the calls below);
+ * detach restores the grid's own stream.  Destroying an attached grid
+ * detaches it first. */
+int sunhip_batch_attach(sunhip_batch *b, sunhip_grid *g);
+int sunhip_batch_detach(sunhip_batch *b, sunhip_grid *g);
+/* Start a FP64 pressure solve of g (fields as for sunhip_q_solve: "q" is the
+ * warm start) or a free-surface solve (as sunhip_h_solve); tag is returned by
+ * sunhip_batch_poll when it completes. */
+int sunhip_batch_begin_q(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, int64_t tag);
+int sunhip_batch_begin_h(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, const sunhip_h_phys *ph,
+                         int64_t tag);
+/* Enqueue at least iters iterations of every active solve (whole graph
+ * chunks); returns without waiting. */
+int sunhip_batch_advance(sunhip_batch *b, int iters);
+/* Wait for the enqueued work, retire up to max converged (or exhausted)
+ * solves: finish their x, return their tags, kinds (0 pressure FP64,
+ * 1 free surface, 2 pressure FP32 inner) and stats (device_ms = time since
+ * begin).  Returns the number retired. */
+int sunhip_batch_poll(sunhip_batch *b, int64_t *tags, int *kinds, sunhip_cg_stats *st, int max);
+int sunhip_batch_active(const sunhip_batch *b);
+int sunhip_batch_sync(sunhip_batch *b);
+/* Blocking forms over n grids (the batch must be idle); st[i] per grid. */
+int sunhip_batch_q_solve(sunhip_batch *b, sunhip_grid **g, int n, const sunhip_cg_params *prm, sunhip_cg_stats *st);
+int sunhip_batch_h_solve(sunhip_batch *b, sunhip_grid **g, int n, const sunhip_cg_params *prm,
+                         const sunhip_h_phys *ph, sunhip_cg_stats *st);
+/* Mixed precision over n grids: one FP64 residual launch per refinement step
+ * for all unconverged members, their FP32 inner solves batched, FP64
+ * updates; per member the same stopping rule as sunhip_q_solve_mixed. */
+int sunhip_batch_q_solve_mixed(sunhip_batch *b, sunhip_grid **g, int n, const sunhip_cg_params *prm,
+                               double inner_tol, int max_outer, sunhip_cg_stats *st, int *inner_iters);
+/* One scalar step (as sunhip_update_scalars) of each of n grids, sp[i]. */
+int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **g, int n, const sunhip_scalar_params *sp,
+                                const char *scal);
+/* Launch configuration: unroll (1, 2, 4; 0 = autotune), blocks per CU of the
+ * persistent launch (0 = one work item per wavefront), iterations per graph
+ * chunk (0 = automatic).  sunhip_batch_tuning reports the configuration in
+ * use for the current batch shape and returns the number of tuned shapes. */
+int sunhip_batch_set_tuning(sunhip_batch *b, int unroll, int blocks_per_cu, int chunk);
+int sunhip_batch_tuning(const sunhip_batch *b, int *unroll, int *blocks_per_cu, int *chunk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,18 @@ def smooth(rng, x, y, z, Lx, Ly, H, amp=1.0, n=5):
     return out
 
 
+def smooth_columns(rng, xc, yc, ic, zl, il, Lx, Ly, H, amp=1.0, n=5):
+    """smooth() at (xc[ic], yc[ic], zl[il]) with the trigonometry evaluated
+    once per column and once per level; the products are formed in the same
+    order, so the values are identical to smooth() on the expanded points."""
+    out = np.zeros(len(ic))
+    for a, kx, ky, kz, ph in _modes(rng, n, amp):
+        f = a * np.sin(kx * np.pi * xc / Lx + ph) * np.cos(ky * np.pi * yc / Ly + 0.5 * ph)
+        g = np.cos(kz * np.pi * zl / H + 0.3 * ph)
+        out += f[ic] * g[il]
+    return out
+
+
 def bumps_depth(rng, Lx, Ly, dmin, dmax, nb=8, mean=None):
     """Depth function: clip(base + sum of nb Gaussian bumps, dmin, dmax)."""
     cx = rng.uniform(0, Lx, nb)
@@ -80,14 +92,16 @@ def fill_velocities(st: State, rng, amp_u=0.1, amp_w=1e-3, zero_w_bottom=True,
     zmid = 0.5 * (zc[:-1] + zc[1:])
     mark = st.ints("mark")
     wet = st.edge_wet_mask() & (mark[edge] == 0)
+    kz = np.minimum(k, len(zmid) - 1)
     for name in edge_names:
-        v = smooth(rng, xe[edge], ye[edge], zmid[np.minimum(k, len(zmid) - 1)], Lx, Ly, H, amp_u)
+        v = smooth_columns(rng, xe, ye, edge, zmid, kz, Lx, Ly, H, amp_u)
         st.set(name, np.where(wet, v, 0.0))
     cell, kw = st.cell_layers(extra=1)
     xv, yv = st.get("xv"), st.get("yv")
     nk = st.ints("Nk")
+    kwz = np.minimum(kw, len(zc) - 1)
     for name in cell_names:
-        v = smooth(rng, xv[cell], yv[cell], zc[np.minimum(kw, len(zc) - 1)], Lx, Ly, H, amp_w)
+        v = smooth_columns(rng, xv, yv, cell, zc, kwz, Lx, Ly, H, amp_w)
         if zero_w_bottom:
             v = np.where(kw == nk[cell], 0.0, v)
         st.set(name, v)

@@ -399,7 +399,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   if (!g) return 0;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
-  if (g->batch) sunhip_batch_detach(g->batch, g);
+  if (g->batch) batch_forget_grid(g->batch, g);
   for (auto &kv : g->fields) hipFree(kv.second.d);
   for (auto &kv : g->fields32) hipFree(kv.second);
   void *ptrs[] = {g->d_toff, g->d_tdep, g->d_eoff, g->d_edep, g->d_nk, g->d_ctop, g->d_ctopold, g->d_comp,

@@ -155,4 +155,5 @@ int field(sunhip_grid *g, const char *name, Field **out);  // create on demand
 int field32(sunhip_grid *g, const char *name, float **out);
 Kind kind_of(const char *name);
 int dev_alloc(sunhip_grid *g, void **p, size_t bytes);
+void batch_forget_grid(sunhip_batch *b, sunhip_grid *g);
 }  // namespace sunhip

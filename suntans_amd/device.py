@@ -47,6 +47,10 @@ class ScalarParams(C.Structure):
                 ("step", C.c_int), ("alpha_top", C.c_int), ("alpha_bot", C.c_int)]
 
 
+class HPhys(C.Structure):
+    _fields_ = [("grav", C.c_double), ("theta", C.c_double), ("dt", C.c_double)]
+
+
 _lib = None
 
 
@@ -83,6 +87,26 @@ def lib() -> C.CDLL:
         L.sunhip_h_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_int, C.POINTER(CGStats), _ip]
         L.sunhip_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
+        # batched solves
+        pp = C.POINTER(vp)
+        L.sunhip_batch_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.sunhip_batch_destroy.argtypes = [vp]
+        L.sunhip_batch_attach.argtypes = [vp, vp]
+        L.sunhip_batch_detach.argtypes = [vp, vp]
+        L.sunhip_batch_begin_q.argtypes = [vp, vp, C.POINTER(CGParams), C.c_int64]
+        L.sunhip_batch_begin_h.argtypes = [vp, vp, C.POINTER(CGParams), C.POINTER(HPhys), C.c_int64]
+        L.sunhip_batch_advance.argtypes = [vp, C.c_int]
+        L.sunhip_batch_poll.argtypes = [vp, C.POINTER(C.c_int64), _ip, C.POINTER(CGStats), C.c_int]
+        L.sunhip_batch_active.argtypes = [vp]
+        L.sunhip_batch_sync.argtypes = [vp]
+        L.sunhip_batch_q_solve.argtypes = [vp, pp, C.c_int, C.POINTER(CGParams), C.POINTER(CGStats)]
+        L.sunhip_batch_h_solve.argtypes = [vp, pp, C.c_int, C.POINTER(CGParams), C.POINTER(HPhys),
+                                           C.POINTER(CGStats)]
+        L.sunhip_batch_q_solve_mixed.argtypes = [vp, pp, C.c_int, C.POINTER(CGParams), C.c_double, C.c_int,
+                                                 C.POINTER(CGStats), _ip]
+        L.sunhip_batch_update_scalars.argtypes = [vp, pp, C.c_int, C.POINTER(ScalarParams), C.c_char_p]
+        L.sunhip_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.sunhip_batch_tuning.argtypes = [vp, _ip, _ip, _ip]
         _lib = L
     return _lib
 
@@ -218,3 +242,120 @@ class DeviceGrid:
         out = C.c_double()
         check(lib().sunhip_copy_probe(self.h, int(nbytes), int(reps), C.byref(out)), "copy_probe")
         return out.value
+
+
+class Batch:
+    """Many grids solved together (sunhip_batch_*): attached grids enqueue on
+    the batch's stream and the PCG iterations of all active solves share the
+    same launches."""
+
+    KIND = {0: "q", 1: "h", 2: "q32"}
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().sunhip_batch_create(C.byref(self.h), device), "sunhip_batch_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sunhip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _grids(dgs):
+        arr = (C.c_void_p * len(dgs))(*[d.h.value for d in dgs])
+        return arr
+
+    def attach(self, dg):
+        check(lib().sunhip_batch_attach(self.h, dg.h), "sunhip_batch_attach")
+
+    def detach(self, dg):
+        check(lib().sunhip_batch_detach(self.h, dg.h), "sunhip_batch_detach")
+
+    def begin_q(self, dg, tag, epsilon=1e-10, maxiters=2000, resnorm=0):
+        P = CGParams(maxiters, 2, resnorm, epsilon, 0)
+        check(lib().sunhip_batch_begin_q(self.h, dg.h, C.byref(P), tag), "sunhip_batch_begin_q")
+
+    def begin_h(self, dg, tag, grav, theta, dt, epsilon=1e-10, maxiters=1000, precond=1, resnorm=0):
+        P = CGParams(maxiters, precond, resnorm, epsilon, 0)
+        ph = HPhys(grav, theta, dt)
+        check(lib().sunhip_batch_begin_h(self.h, dg.h, C.byref(P), C.byref(ph), tag), "sunhip_batch_begin_h")
+
+    def advance(self, iters: int):
+        check(lib().sunhip_batch_advance(self.h, iters), "sunhip_batch_advance")
+
+    def poll(self, maxn: int = 4096):
+        tags = (C.c_int64 * maxn)()
+        kinds = (C.c_int * maxn)()
+        st = (CGStats * maxn)()
+        n = lib().sunhip_batch_poll(self.h, tags, kinds, st, maxn)
+        if n < 0:
+            check(n, "sunhip_batch_poll")
+        return [(int(tags[i]), self.KIND[int(kinds[i])], st[i].as_dict()) for i in range(n)]
+
+    def active(self) -> int:
+        return int(lib().sunhip_batch_active(self.h))
+
+    def sync(self):
+        check(lib().sunhip_batch_sync(self.h), "sunhip_batch_sync")
+
+    def q_solve(self, dgs, epsilon=1e-10, maxiters=2000, resnorm=0):
+        P = CGParams(maxiters, 2, resnorm, epsilon, 0)
+        st = (CGStats * len(dgs))()
+        check(lib().sunhip_batch_q_solve(self.h, self._grids(dgs), len(dgs), C.byref(P), st),
+              "sunhip_batch_q_solve")
+        return [s.as_dict() for s in st]
+
+    def h_solve(self, dgs, phys, epsilon=1e-10, maxiters=1000, precond=1, resnorm=0):
+        P = CGParams(maxiters, precond, resnorm, epsilon, 0)
+        ph = (HPhys * len(dgs))(*[HPhys(*p) for p in phys])
+        st = (CGStats * len(dgs))()
+        check(lib().sunhip_batch_h_solve(self.h, self._grids(dgs), len(dgs), C.byref(P), ph, st),
+              "sunhip_batch_h_solve")
+        return [s.as_dict() for s in st]
+
+    def q_solve_mixed(self, dgs, epsilon=1e-10, inner_tol=1e-4, max_outer=20, maxiters=2000):
+        P = CGParams(maxiters, 2, 0, epsilon, 0)
+        st = (CGStats * len(dgs))()
+        inner = (C.c_int * len(dgs))()
+        check(lib().sunhip_batch_q_solve_mixed(self.h, self._grids(dgs), len(dgs), C.byref(P), inner_tol,
+                                               max_outer, st, inner), "sunhip_batch_q_solve_mixed")
+        out = [s.as_dict() for s in st]
+        for d, n in zip(out, inner):
+            d["inner_iters"] = int(n)
+        return out
+
+    def update_scalars(self, dgs, params, scal="s"):
+        """params: one (theta, dt, kappa, TVD, step) per grid."""
+        sp = (ScalarParams * len(dgs))(*[ScalarParams(th, dt, ka, tvd, step, 0, 0)
+                                         for th, dt, ka, tvd, step in params])
+        check(lib().sunhip_batch_update_scalars(self.h, self._grids(dgs), len(dgs), sp, scal.encode()),
+              "sunhip_batch_update_scalars")
+
+    def set_tuning(self, unroll=0, blocks_per_cu=0, chunk=0):
+        check(lib().sunhip_batch_set_tuning(self.h, unroll, blocks_per_cu, chunk), "sunhip_batch_set_tuning")
+
+    def tuning(self):
+        u, b, c = C.c_int(), C.c_int(), C.c_int()
+        n = lib().sunhip_batch_tuning(self.h, C.byref(u), C.byref(b), C.byref(c))
+        return {"unroll": u.value, "blocks_per_cu": b.value, "chunk": c.value, "tuned_shapes": n}
+
+
+def member_upload(dg, st, which=("q", "src", "dzz", "dzf", "D")):
+    """Upload a State's pressure-solve inputs to its DeviceGrid."""
+    names = {"q": "qc", "src": "stmp", "dzz": "dzz", "dzf": "dzf", "D": "D", "h": "h", "htmp": "htmp"}
+    for n in which:
+        dg.upload(n, st.get(names[n]))
+
+
+def member_upload_tvd(dg, st):
+    """Upload a State's scalar-transport inputs (UpdateScalars) to its
+    DeviceGrid; every cell active, no open-boundary scalar."""
+    for n in ("s", "u", "utmp2", "dzf", "wnew", "wtmp2", "dzz", "dzzold", "kappa_tv"):
+        dg.upload(n, st.get(n))
+    dg.upload("active", np.ones(st.Nc))
