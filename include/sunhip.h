@@ -87,16 +87,22 @@ int64_t sunhip_grid_bytes(const sunhip_grid *g);
  * padded cell-layer slots and tiles. */
 int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
 
-/* Named device fields.  Cell fields: "q","src","dzz","dzzold","s","wnew",
- * "wtmp2","kappa_tv","Cn","gradSx","gradSy","x","r","z","p","rtmp",
+/* Named device fields; the name fixes the layout.  Cell fields (any other
+ * name): "q","src","dzz","dzzold","s","kappa_tv","Cn","gradSx","gradSy",
  * "stmp"...; w-type fields take Nk+1 values per column (names starting
- * with 'w'); edge fields: "dzf","u","utmp2","SfHp","SfHm"; 2-D cell
- * fields: "h","htmp","hcoef"...; 2-D edge fields: "D".  Upload/download
- * move between a flat host array and the device layout. */
+ * with 'w'); edge fields: "dzf","u","utmp2","SfHp","SfHm","bscal" (the
+ * open-boundary scalar, Nke values per type-2 edge, zero elsewhere); 2-D
+ * cell fields: "h","htmp","htmp0","hcoef","active"...; 2-D edge fields:
+ * "D","Dq","Dh".  Upload/download move between a flat host array and the
+ * device layout. */
 int sunhip_upload(sunhip_grid *g, const char *name, const double *flat);
 int sunhip_download(sunhip_grid *g, const char *name, double *flat);
 /* Device-side copy of one field into another of the same kind. */
 int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src);
+/* dst = a*src element-wise on the device (fields of the same kind);
+ * enqueued, no host wait.  No reference counterpart: builds a time step's
+ * forcing on the device from a resident base field. */
+int sunhip_field_scale(sunhip_grid *g, const char *dst, const char *src, double a);
 
 /* ---- nonhydrostatic pressure (main/phys.c:2411-2550) ------------------- */
 /* Solve on the device-resident fields: "q" (warm start in, solution out),

@@ -34,9 +34,10 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 }
 
 Kind kind_of(const char *name) {
-  static const char *edge[] = {"dzf", "u", "utmp2", "SfHp", "SfHm", "ef0", "ef1"};
-  static const char *edge2[] = {"D", "df", "dg"};
-  static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1", "hx", "hy"};
+  static const char *edge[] = {"dzf", "u", "utmp2", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
+  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh"};
+  static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
+                                "hx", "hy", "active", "htmp0"};
   for (auto n : edge) if (!strcmp(name, n)) return EDGE;
   for (auto n : edge2) if (!strcmp(name, n)) return EDGE2;
   for (auto n : cell2) if (!strcmp(name, n)) return CELL2;
@@ -540,6 +541,15 @@ int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
   return 0;
 }
 
+}  // extern "C"
+
+__global__ void k_scale(double *__restrict__ d, const double *__restrict__ x, double a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = a * x[i];
+}
+
+extern "C" {
+
 int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src) {
   HIPCHK(hipSetDevice(g->device));
   Field *a, *b;
@@ -551,6 +561,21 @@ int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src) {
     return -1;
   }
   HIPCHK(hipMemcpyAsync(a->d, b->d, a->n * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+  return 0;
+}
+
+int sunhip_field_scale(sunhip_grid *g, const char *dst, const char *src, double a) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *d, *x;
+  int rc = field(g, dst, &d);
+  if (rc) return rc;
+  if ((rc = field(g, src, &x))) return rc;
+  if (d->n != x->n) {
+    set_error("sunhip_field_scale: %s and %s differ in size", dst, src);
+    return -1;
+  }
+  k_scale<<<(unsigned)((d->n + 255) / 256), 256, 0, g->stream>>>(d->d, x->d, a, (int64_t)d->n);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

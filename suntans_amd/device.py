@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
         L.sunhip_upload.argtypes = [vp, C.c_char_p, _dp]
         L.sunhip_download.argtypes = [vp, C.c_char_p, _dp]
         L.sunhip_copy_field.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.sunhip_field_scale.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double]
         L.sunhip_q_solve.argtypes = [vp, C.POINTER(CGParams), C.POINTER(CGStats)]
         L.sunhip_q_solve_fixed.argtypes = [vp, C.POINTER(CGParams), C.c_int, C.POINTER(CGStats)]
         L.sunhip_q_operator.argtypes = [vp, C.c_char_p, C.c_char_p]
@@ -176,6 +177,9 @@ class DeviceGrid:
 
     def copy(self, dst: str, src: str) -> None:
         check(lib().sunhip_copy_field(self.h, dst.encode(), src.encode()), f"copy {src}->{dst}")
+
+    def scale(self, dst: str, src: str, a: float) -> None:
+        check(lib().sunhip_field_scale(self.h, dst.encode(), src.encode(), a), f"scale {src}->{dst}")
 
     def sync(self):
         check(lib().sunhip_sync(self.h), "sync")
