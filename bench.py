@@ -123,6 +123,120 @@ def traffic_for(kernel_key: str):
     return t[name][0] + t[name][1]
 
 
+def ensemble_specs(n: int, seed: int):
+    """Member sizes of the ensemble: cells U[5e3, 2e5], aspect U[1, 4],
+    levels U{20..100} (seeded)."""
+    rng = np.random.default_rng(seed)
+    return [(seed * 1000 + i, int(rng.uniform(5e3, 2e5)), float(rng.uniform(1, 4)), int(rng.integers(20, 101)))
+            for i in range(n)]
+
+
+def build_members(specs, threads: int):
+    """Build the members' host states on a thread pool and their device grids
+    (everything a time step reads) on the calling thread, freeing each host
+    state once uploaded.  Returns serve.Sim objects."""
+    from concurrent.futures import ThreadPoolExecutor
+    from suntans_amd import cases, serve
+    sims = []
+    with ThreadPoolExecutor(threads) as ex:
+        for i, st in enumerate(ex.map(lambda sp: cases.ensemble_member(*sp), specs)):
+            sims.append(serve.sim_from_state(i, st, keep_host=False))
+            del st
+    return sims
+
+
+def ensemble_leg(args, sims):
+    """64 independent members solved together (FP32 inner PCG in FP64
+    refinement, one persistent-kernel batch) followed by one TVD superbee
+    step each; compared with the FP64 batch and with the same mixed solves
+    one grid at a time."""
+    from suntans_amd import device
+    dgs = [s.dg for s in sims]
+    N = sum(s.N for s in sims)
+    b = device.Batch()
+
+    def reset():
+        for dg in dgs:
+            dg.copy("src", "src0")
+            dg.copy("D", "Dq")
+            dg.scale("q", "src0", 0.0)
+
+    def timed(fn):
+        reset()
+        for dg in dgs:
+            dg.sync()
+        b.sync()
+        t = time.perf_counter()
+        r = fn()
+        b.sync()
+        for dg in dgs:
+            dg.sync()
+        return r, time.perf_counter() - t
+
+    timed(lambda: b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol))  # warm-up, autotuning
+    mixed, t_mixed = timed(lambda: b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol))
+    tuning = b.tuning()
+    fp64, t_fp64 = timed(lambda: b.q_solve(dgs, epsilon=1e-10))
+    params = [(s.phys["theta"], s.phys["dt"], s.phys["kappa_s"], 3, 2) for s in sims]
+    t = time.perf_counter()
+    b.update_scalars(dgs, params)
+    b.sync()
+    t_tvd = time.perf_counter() - t
+    for dg in dgs:
+        b.detach(dg)
+    # the same mixed solves one grid at a time (each on its own stream)
+    reset()
+    for dg in dgs:
+        dg.sync()
+    t = time.perf_counter()
+    seq = [dg.q_solve_mixed(epsilon=1e-10, inner_tol=args.inner_tol) for dg in dgs]
+    t_seq = time.perf_counter() - t
+    b.close()
+    return {
+        "workload": (f"{len(sims)} members, cells U[5e3,2e5] (aspect U[1,4]), levels U{{20..100}}, "
+                     "Gaussian-bump depth; N total = %d" % N),
+        "members": len(sims), "N_total": N,
+        "Nc_range": [min(s.Nc for s in sims), max(s.Nc for s in sims)],
+        "mixed_batched_ms": t_mixed * 1e3, "mixed_solves_per_s": len(sims) / t_mixed,
+        "mixed_all_converged": all(m["converged"] for m in mixed),
+        "mixed_max_true_resid": max(m["resid"] for m in mixed),
+        "mixed_inner_iterations_total": int(sum(m["inner_iters"] for m in mixed)),
+        "fp64_batched_ms": t_fp64 * 1e3, "fp64_solves_per_s": len(sims) / t_fp64,
+        "fp64_iterations_max": max(f["niters"] for f in fp64),
+        "mixed_one_at_a_time_ms": t_seq * 1e3,
+        "batching_speedup": t_seq / t_mixed,
+        "tvd_superbee_step_all_members_ms": t_tvd * 1e3,
+        "mixed_plus_tvd_members_per_s": len(sims) / (t_mixed + t_tvd),
+        "autotuned": tuning,
+        "unknowns_per_s_mixed": N / t_mixed,
+    }
+
+
+def serving_leg(args, sims, large_sim):
+    """Poisson arrivals of time-step requests (free-surface solve +
+    warm-started pressure solve + TVD step) over the ensemble members, 10% of
+    them for the large grid; admission by a 250 GB HBM budget."""
+    from suntans_amd import serve
+    allsims = sims + [large_sim]
+    sids = [s.sid for s in allsims]
+    weights = [0.9 / len(sims)] * len(sims) + [0.1]
+    res = []
+    for rate in args.serve_rates:
+        arr = serve.poisson_arrivals(rate, args.serve_requests, sids, weights, seed=int(rate))
+        srv = serve.Server(allsims, hbm_budget=250e9, max_slots=128)
+        done = srv.run(arr)
+        srv.close()
+        row = {"rate_per_s": rate, **serve.summarize(done)}
+        log(f"serving {row}")
+        res.append(row)
+    return {
+        "workload": (f"{len(sims)} ensemble members + 1 grid of the large solve (10% of requests); each request = "
+                     "CGSolve + CGSolveQ (q warm start on device) + TVD superbee step, 1e-10, FP64"),
+        "hbm_budget_GB": 250, "max_slots": 128, "requests_per_rate": args.serve_requests,
+        "rates": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +251,12 @@ def main():
     ap.add_argument("--skip-extra", action="store_true",
                     help="only the FP64 pressure solves (for short profiled runs)")
     ap.add_argument("--cpu-budget", type=float, default=40.0)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--skip-ensemble", action="store_true", help="leave out the ensemble and serving legs")
+    ap.add_argument("--skip-serve", action="store_true", help="leave out the serving leg")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10, 20, 50, 100, 200, 500, 1000, 2000])
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--build-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for HBM byte counters)")
     args = ap.parse_args()
@@ -296,6 +416,20 @@ def main():
         cp_ms = dg.copy_probe(1 << 30, 20)
         log(f"copy probe: 1 GiB in {cp_ms:.3f} ms = {2 * (1 << 30) / cp_ms / 1e6:.0f} GB/s")
         out["copy_probe_GBps"] = 2 * (1 << 30) / cp_ms / 1e6
+
+    if not args.skip_ensemble:
+        from suntans_amd import serve
+        t = time.time()
+        sims = build_members(ensemble_specs(args.members, 7), args.build_threads)
+        log(f"built {len(sims)} members (N total {sum(x.N for x in sims)}) in {time.time() - t:.1f}s")
+        out["ensemble"] = ensemble_leg(args, sims)
+        log(f"ensemble: {out['ensemble']}")
+        if not args.skip_serve:
+            large = serve.sim_from_state(len(sims), st, keep_host=False)
+            out["serving"] = serving_leg(args, sims, large)
+            large.dg.close()
+        for x in sims:
+            x.dg.close()
 
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(st, int(round(its)), args.cpu_budget)

@@ -641,6 +641,7 @@ int sunhip_batch_advance(sunhip_batch *b, int iters) {
   HIPCHK(hipSetDevice(b->device));
   if (b->tasks.empty()) return 0;
   int rc;
+  const bool fresh = b->dirty;
   if (b->dirty && (rc = rebuild(b))) return rc;
   const int chunk = chunk_len(b);
   // choose the configuration: fixed, tuned for this shape, or the next
@@ -648,23 +649,35 @@ int sunhip_batch_advance(sunhip_batch *b, int iters) {
   Config c = b->cfg;
   bool time_it = false;
   if (!b->fixed) {
-    const long key = shape_key(b);
-    auto it = b->tuned.find(key);
+    // tuned shapes keep their winner; a batch is tuned once, on the first
+    // shape it runs, and every other shape uses that winner (the candidates
+    // are timed per work item, so members leaving during tuning do not bias
+    // the comparison much)
+    auto it = b->tuned.find(shape_key(b));
     if (it != b->tuned.end()) {
       c = it->second;
-      b->cand = -1;
-    } else {
-      if (b->cand < 0 || b->cand_key != key) { b->cand = 0; b->cand_key = key; }
+    } else if (b->tuned.empty()) {
+      if (b->cand < 0) { b->cand = 0; b->cand_key = shape_key(b); }
       c = kCandidates[b->cand];
-      time_it = true;
+      time_it = !b->timing;
     }
   }
-  if (!b->gexec || b->gchunk != chunk || b->gcfg.unroll != c.unroll || b->gcfg.bpc != c.bpc)
+  // A set of members that has just changed is launched directly: it may
+  // change again at the next poll, and capturing a graph for one replay
+  // costs more than it saves.  A set that survived a chunk is captured once
+  // and replayed.
+  const bool graph = !fresh && !time_it;
+  if (graph && (!b->gexec || b->gchunk != chunk || b->gcfg.unroll != c.unroll || b->gcfg.bpc != c.bpc))
     if ((rc = capture(b, c, chunk))) return rc;
   for (int done = 0; done < iters; done += chunk) {
     const bool t = time_it && done == 0;
     if (t) HIPCHK(hipEventRecord(b->ev0, b->stream));
-    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    if (graph) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+      for (int i = 0; i < chunk; i++)
+        if ((rc = enqueue_iteration(b, c))) return rc;
+    }
     if (t) {
       HIPCHK(hipEventRecord(b->ev1, b->stream));
       b->timing = true;
