@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_large_solve.txt")
 
-KERNEL_NAMES = {"qapply": "k_qapply<2,3,4,double,double>", "qprec": "k_qprec<true,4,double>"}
+KERNEL_NAMES = {"qapply": "k_qapply<2,3,4,double,double,4,true,0>", "qprec": "k_qprec<true,4,double,true>"}
 
 
 def log(msg: str) -> None:
@@ -121,6 +121,41 @@ def traffic_for(kernel_key: str):
     if name not in t:
         return None
     return t[name][0] + t[name][1]
+
+
+def tvd_leg(st, dg, N, Ek, steps=5):
+    """One TVD superbee scalar step (HorizontalFaceScalars + UpdateScalars)
+    on the large grid, all fields resident; per-kernel times against their
+    algorithmic bytes (8 B each): face sums 3E_k + 3N, face values
+    4E_k + 3N, column update 9N + 5E_k."""
+    u, w = st.get("u"), st.get("w")
+    dg.upload("s", st.get("s"))
+    dg.upload("u", u)
+    dg.upload("utmp2", u)
+    dg.upload("wnew", w)
+    dg.upload("wtmp2", w)
+    dg.upload("dzzold", st.get("dzz"))
+    dg.upload("kappa_tv", st.get("kappa_tv"))
+    dg.upload("active", np.ones(st.Nc))
+    dg.copy("s0", "s")
+    th, dt, ka = st.get_prop("theta"), st.get_prop("dt"), st.get_prop("kappa_s")
+    dg.update_scalars(th, dt, ka, 3)
+    dg.profile(True)
+    t = time.perf_counter()
+    for _ in range(steps):
+        dg.copy("s", "s0")
+        dg.update_scalars(th, dt, ka, 3)
+    wall = (time.perf_counter() - t) / steps
+    kt = dg.kernel_times()
+    dg.profile(False)
+    nb = {"tvd_sums": 8 * (3 * Ek + 3 * N), "tvd_faces": 8 * (4 * Ek + 3 * N), "tvd_column": 8 * (9 * N + 5 * Ek)}
+    res = {"step_ms": wall * 1e3, "steps_per_s": 1 / wall}
+    for k, b in nb.items():
+        if k in kt:
+            ms = kt[k][0] / kt[k][1]
+            res[k] = {"ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"tvd: {res}")
+    return res
 
 
 def ensemble_specs(n: int, seed: int):
@@ -411,6 +446,7 @@ def main():
             "rel_err_h_fp64_vs_tight": relerr(xh, xht),
         }
         dg.upload("D", st.get("D"))
+        out["tvd"] = tvd_leg(st, dg, N, Ek)
 
     if args.copy_probe:
         cp_ms = dg.copy_probe(1 << 30, 20)

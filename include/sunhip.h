@@ -126,13 +126,21 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
                          sunhip_cg_stats *st, int *inner_iters);
 /* QCoefficients from the device field "dzz" into "coef". */
 int sunhip_q_coefficients(sunhip_grid *g);
-/* Kernel tuning: "unroll" (levels per load group: 1, 2, 4) and "reduce"
- * (0: last block finishes the dot products, 1: a finisher kernel). */
+/* Kernel tuning: "unroll" (levels per load group: 1, 2, 4), "reduce"
+ * (0: last block finishes the dot products, 1: a finisher kernel), "tiles"
+ * (tiles of 64 columns per block of the iteration's operator kernel: 4, 8,
+ * 16), "ntstore" (1: its results stored with the non-temporal hint), "rzid"
+ * (1: the preconditioned residual dot r.M^-1 r taken as sum d'^2/b' from
+ * the forward Thomas sweep, so the backward sweep does not re-read r;
+ * M^-1 r itself is unchanged). */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event
- * pair around every solver kernel launch; sunhip_kernel_times returns the
- * totals (ms) and launch counts of the 8 kernel ids (q apply, q precond,
- * h apply, h precond, mixed apply, mixed precond, mixed residual, other). */
+ * pair around every solver kernel launch; sunhip_kernel_times fills ms[] and
+ * n[] (SUNHIP_NPROF entries each) with the totals (ms) and launch counts per
+ * kernel id and returns the number of ids: q apply, q precond, h apply,
+ * h precond, mixed apply, mixed precond, mixed residual, other, TVD face
+ * sums, TVD face values, scalar column update. */
+#define SUNHIP_NPROF 16
 int sunhip_profile(sunhip_grid *g, int on);
 int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n);
 /* A plain device copy of nbytes, reps times (HBM byte-counter calibration);

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void kb_qapply(const BQMember<T, TO> *__restri
     } else {
       V.vin = B.vin; V.src = B.src; V.out = B.out;
     }
-    const double d = wave_sum(qapply_tile<MODE, NF, U, T, TO>(B.G, V, mu, nu, t, l));
+    const double d = wave_sum(qapply_tile<MODE, NF, U, T, TO, MODE == QM_FUSED>(B.G, V, mu, nu, t, l));
     if (l == 0) part[w] = d;
   }
 }
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void kb_qprec(const BQMember<T, T> *__restrict
     const BQMember<T, T> &B = M[m];
     if (B.sc->done) continue;
     double rz = 0, rr = 0;
-    qprec_tile<true, U, T>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq, (T)B.sc->nu, t, l, rz, rr);
+    qprec_tile<true, U, T, true>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq, (T)B.sc->nu, t, l, rz, rr);
     rz = wave_sum(rz);
     rr = wave_sum(rr);
     if (l == 0) { part[2 * (size_t)w] = rz; part[2 * (size_t)w + 1] = rr; }

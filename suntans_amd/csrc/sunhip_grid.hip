@@ -242,7 +242,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   }
   int64_t acc = 0;
   for (int t = 0; t < g->Nt; t++) { g->toff[t] = (int)acc; acc += (int64_t)g->tdep[t] * SUNHIP_TILE; }
-  if (acc >= (int64_t)1 << 31) {
+  if (acc >= (int64_t)1 << 29) {  // fields are addressed by 32-bit byte offsets
     set_error("sunhip_grid_create: %lld cell-layer slots exceed the 32-bit layout", (long long)acc);
     delete g;
     return -1;
@@ -278,8 +278,8 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
     g->eoff[t] = (int)acc;
     acc += (int64_t)d * SUNHIP_TILE;
   }
-  if (acc >= (int64_t)1 << 31) {
-    set_error("sunhip_grid_create: edge layout exceeds 32 bits");
+  if (acc >= (int64_t)1 << 29) {  // fields are addressed by 32-bit byte offsets
+    set_error("sunhip_grid_create: edge layout exceeds 2^29 values (32-bit byte offsets)");
     delete g;
     return -1;
   }
@@ -603,7 +603,7 @@ int sunhip_profile(sunhip_grid *g, int on) {
   HIPCHK(hipSetDevice(g->device));
   prof_flush(g);
   g->prof = on;
-  for (int i = 0; i < 8; i++) { g->prof_ms[i] = 0; g->prof_n[i] = 0; }
+  for (int i = 0; i < SUNHIP_NPROF; i++) { g->prof_ms[i] = 0; g->prof_n[i] = 0; }
   if (on && g->prof_pool.empty()) {
     g->prof_pool.resize(2 * 4096);
     g->prof_kind.resize(4096);
@@ -616,8 +616,8 @@ int sunhip_profile(sunhip_grid *g, int on) {
 int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n) {
   hipSetDevice(g->device);
   prof_flush(g);
-  for (int i = 0; i < 8; i++) { ms[i] = g->prof_ms[i]; n[i] = g->prof_n[i]; }
-  return 8;
+  for (int i = 0; i < PK_COUNT; i++) { ms[i] = g->prof_ms[i]; n[i] = g->prof_n[i]; }
+  return PK_COUNT;
 }
 
 }  // extern "C"
@@ -636,22 +636,23 @@ void prof_end(sunhip_grid *g, int slot) {
 }
 }  // namespace sunhip
 
-// Plain streaming copy, the calibration kernel for HBM byte counters.
-__global__ __launch_bounds__(256) void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b,
+// Plain streaming copy, one 16-byte element per thread: the calibration
+// kernel for HBM byte counters and the measure of attainable copy bandwidth.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b,
                                                     int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 extern "C" int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms) {
   HIPCHK(hipSetDevice(g->device));
-  const int64_t n = nbytes / (int64_t)sizeof(double4);
-  if (n <= 0 || reps <= 0) { set_error("sunhip_copy_probe: bad size"); return -1; }
-  double4 *a = nullptr, *b = nullptr;
-  HIPCHK(hipMalloc(&a, n * sizeof(double4)));
-  HIPCHK(hipMalloc(&b, n * sizeof(double4)));
-  HIPCHK(hipMemsetAsync(a, 0, n * sizeof(double4), g->stream));
-  const int blocks = 256 * 16;
+  const int64_t n = nbytes / (int64_t)sizeof(double2);
+  if (n <= 0 || reps <= 0 || n > ((int64_t)1 << 40)) { set_error("sunhip_copy_probe: bad size"); return -1; }
+  double2 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n * sizeof(double2)));
+  HIPCHK(hipMalloc(&b, n * sizeof(double2)));
+  HIPCHK(hipMemsetAsync(a, 0, n * sizeof(double2), g->stream));
+  const unsigned blocks = (unsigned)((n + 255) / 256);
   k_copy_probe<<<blocks, 256, 0, g->stream>>>(a, b, n);  // warm
   HIPCHK(hipEventRecord(g->ev0, g->stream));
   for (int r = 0; r < reps; r++) k_copy_probe<<<blocks, 256, 0, g->stream>>>(a, b, n);

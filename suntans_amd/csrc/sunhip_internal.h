@@ -128,6 +128,10 @@ struct sunhip_grid {
   // tuning: levels per load group of the column kernels (1, 2, 4) and
   // reduction finish (0 last block, 1 finisher kernel); -1 = default
   int tune_unroll = -1, tune_reduce = -1;
+  // tiles per apply block (4, 8, 16), non-temporal apply stores, rz from the
+  // forward sweep (see qprec_tile)
+  int tune_tiles = 4, tune_nts = 1, tune_rzid = 1;
+  int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of
   // the solver kernels, resolved by sunhip_kernel_times
@@ -135,8 +139,8 @@ struct sunhip_grid {
   std::vector<hipEvent_t> prof_pool;
   std::vector<int> prof_kind;   // kernel id per recorded pair
   int prof_used = 0;
-  double prof_ms[8] = {0};
-  int prof_n[8] = {0};
+  double prof_ms[SUNHIP_NPROF] = {0};
+  int prof_n[SUNHIP_NPROF] = {0};
 
   sunhip::Geo geo() const;
 };
@@ -147,7 +151,7 @@ namespace sunhip {
 int prof_begin(sunhip_grid *g, int kind);
 void prof_end(sunhip_grid *g, int slot);
 enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, PK_MPREC = 5, PK_RESID = 6,
-       PK_OTHER = 7 };
+       PK_OTHER = 7, PK_TSUMS = 8, PK_TFACES = 9, PK_TCOLUMN = 10, PK_COUNT = 11 };
 }  // namespace sunhip
 
 namespace sunhip {

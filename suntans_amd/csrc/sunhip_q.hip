@@ -38,8 +38,6 @@
 
 using namespace sunhip;
 
-#define QAPPLY_THREADS 256
-#define TILES_PER_APPLY (QAPPLY_THREADS / SUNHIP_TILE)
 
 // Where a launch's block sums go: the last block finishes them (cnt set) or
 // they are left at part[pbase + block] for a finisher kernel.
@@ -113,29 +111,31 @@ __global__ __launch_bounds__(256) void k_qfactor(Geo G, const T *__restrict__ co
   qfactor_tile<T>(G, coef, bq, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-// y = L v (qapply_tile's modes) over all tiles, TILES_PER_APPLY per block.
-template <int MODE, int NF, int U, typename T, typename TO>
-__global__ __launch_bounds__(QAPPLY_THREADS) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
+// y = L v (qapply_tile's modes) over all tiles, TPB tiles per block.
+// MINW: minimum waves per SIMD the register allocation must allow (0: the
+// compiler's choice).
+template <int MODE, int NF, int U, typename T, typename TO, int TPB = 4, bool NTS = false, int MINW = 0>
+__global__ __launch_bounds__(TPB * 64, MINW > 0 ? MINW : 1) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
   if (MODE == QM_FUSED && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * TILES_PER_APPLY + (threadIdx.x >> 6);
+  const int t = xcd_block(blockIdx.x, gridDim.x) * TPB + (threadIdx.x >> 6);
   const T mu = MODE == QM_FUSED ? (T)sc->mu : T(0);
   const T nu = MODE == QM_FUSED ? (T)sc->nu : T(0);
-  const double dot = qapply_tile<MODE, NF, U, T, TO>(G, V, mu, nu, t, threadIdx.x & 63);
+  const double dot = qapply_tile<MODE, NF, U, T, TO, NTS>(G, V, mu, nu, t, threadIdx.x & 63);
   if (MODE == QM_FUSED || MODE == QM_RESID_DOT) {
     double dv[1] = {dot};
-    reduce_out<1, QAPPLY_THREADS>(dv, R, sc);
+    reduce_out<1, TPB * 64>(dv, R, sc);
   }
 }
 
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
-template <bool UPDATE, int U, typename T>
+template <bool UPDATE, int U, typename T, bool RZID = false>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
                                                CGScal *sc, Red R) {
   if (UPDATE && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
-  qprec_tile<UPDATE, U, T>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz, rr);
+  qprec_tile<UPDATE, U, T, RZID>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz, rr);
   double v[2] = {rz, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -218,7 +218,12 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   switch (g->tune_unroll) {
     case 1: k_qprec<UPDATE, 1, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
     case 2: k_qprec<UPDATE, 2, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
-    default: k_qprec<UPDATE, 4, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
+    default:
+      if (g->tune_rzid && op != OP_NONE)
+        k_qprec<UPDATE, 4, T, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R);
+      else
+        k_qprec<UPDATE, 4, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R);
+      break;
   }
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
@@ -229,19 +234,36 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
 
 template <int MODE, typename T, typename TO>
 int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R, int pk = -1) {
-  const int nba = (g->Nt + TILES_PER_APPLY - 1) / TILES_PER_APPLY;
+  // launch-shape variants (tiles per block, streaming stores) exist for the
+  // fused iteration kernel on triangles at the default unroll
+  const bool variants = MODE == QM_FUSED && g->mf <= 3 && g->tune_unroll == 4;
+  const int tpb = variants ? g->tune_tiles : 4;
+  const int nba = (g->Nt + tpb - 1) / tpb;
   const Geo G = g->geo();
   const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
-#define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, QAPPLY_THREADS, 0, g->stream>>>(G, V, sc, R)
-  if (g->mf <= 3) {
+#define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
+#define APPLYV(TPB, NTS) k_qapply<MODE, 3, 4, T, TO, TPB, NTS><<<nba, TPB * 64, 0, g->stream>>>(G, V, sc, R)
+#define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
+  if (variants && g->tune_minw == 3 && tpb == 4) {
+    if (g->tune_nts) APPLYW(true, 3); else APPLYW(false, 3);
+  } else if (variants && g->tune_minw == 4 && tpb == 4) {
+    if (g->tune_nts) APPLYW(true, 4); else APPLYW(false, 4);
+  } else if (variants && (tpb != 4 || g->tune_nts)) {
+    if (tpb == 8) { if (g->tune_nts) APPLYV(8, true); else APPLYV(8, false); }
+    else if (tpb == 16) { if (g->tune_nts) APPLYV(16, true); else APPLYV(16, false); }
+    else APPLYV(4, true);
+  } else if (g->mf <= 3) {
     if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2); else APPLY(3, 4);
   } else {
     if (g->tune_unroll == 1) APPLY(4, 1); else if (g->tune_unroll == 2) APPLY(4, 2); else APPLY(4, 4);
   }
 #undef APPLY
+#undef APPLYV
+#undef APPLYW
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
   if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nba, sc, R.op);
+
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -606,6 +628,12 @@ int sunhip_q_precondition(sunhip_grid *g, const char *x, const char *xc) {
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   if (!strcmp(key, "unroll")) g->tune_unroll = value;
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
+  else if (!strcmp(key, "tiles")) {
+    if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }
+    g->tune_tiles = value;
+  } else if (!strcmp(key, "ntstore")) g->tune_nts = value != 0;
+  else if (!strcmp(key, "rzid")) g->tune_rzid = value != 0;
+  else if (!strcmp(key, "minwaves")) g->tune_minw = value;
   else { set_error("unknown tuning key %s", key); return -1; }
   return 0;
 }

@@ -113,15 +113,36 @@ __device__ __forceinline__ void qfactor_tile(const Geo &G, const T *__restrict__
   }
 }
 
+// Element i of a field through a 32-bit byte offset (every field is smaller
+// than 4 GiB, which the grid context checks), so an access is a base
+// address in scalar registers plus one 32-bit vector register.
+template <typename T>
+__device__ __forceinline__ const T &at32(const T *p, uint32_t i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T &at32(T *p, uint32_t i) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + i * (uint32_t)sizeof(T));
+}
+
+// Store of a streamed result (NTS: with the non-temporal hint).
+template <bool NTS, typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  if constexpr (NTS) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // y = L v over the computational columns of tile t (main/phys.c:3486-3536),
 // in one of four modes:
 //   PLAIN     out = L vin
 //   RESID     out = src - L vin
 //   FUSED     p = rtmp + mu*pold -> pnew ; x += nu*pold ; out = L p ; returns p.(L p)
 //   RESID_DOT out = (TO)(src - L vin) ; returns the residual's r.r
-// NF faces per cell at most; U levels per load group.  Returns the lane's
-// share of the dot product.
-template <int MODE, int NF, int U, typename T, typename TO>
+// NF faces per cell at most; U levels per load group; NTS stores the FUSED
+// outputs with the non-temporal hint.  Returns the lane's share of the dot
+// product.
+
+template <int MODE, int NF, int U, typename T, typename TO, bool NTS = false>
 __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V, T mu, T nu, int t, int l) {
   double dot = 0;
   if (t >= G.Nt) return 0;
@@ -136,16 +157,16 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
   const int kt = G.ctop[s];
   const int NS = G.NS;
   const int nf = G.nf[s];
-  const int64_t base = G.toff[t] + l;
-  int64_t nbb[NF], ebb[NF];
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  uint32_t nbb[NF], ebb[NF];
   int kmn[NF], kmx[NF], lim[NF];
   T Df[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
     if (nbf >= 0) {
-      nbb[f] = nbf;
-      ebb[f] = G.eb[f * NS + s];
+      nbb[f] = (uint32_t)nbf;
+      ebb[f] = (uint32_t)G.eb[f * NS + s];
       const int kr = G.kr[f * NS + s];
       kmn[f] = kr & 0xffff;
       kmx[f] = kr >> 16;
@@ -160,21 +181,21 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
     T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
 #pragma unroll
     for (int j = 0; j <= U; j++) {
-      const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
-      if (MODE == QM_FUSED) { pv[j] = po[idx]; v[j] = rt[idx]; }
-      else { pv[j] = 0; v[j] = vin[idx]; }
-      c[j] = coef[idx];
+      const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
+      if (MODE == QM_FUSED) { pv[j] = at32(po, idx); v[j] = at32(rt, idx); }
+      else { pv[j] = 0; v[j] = at32(vin, idx); }
+      c[j] = at32(coef, idx);
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int k = kb + j;
-      xv[j] = MODE == QM_FUSED ? V.x[base + (int64_t)min(k, n) * 64] : T(0);
+      xv[j] = MODE == QM_FUSED ? at32((const T *)V.x, base + (uint32_t)min(k, n) * 64u) : T(0);
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        const int kk = min(k, lim[f]);
-        const int64_t ni = nbb[f] + (int64_t)kk * 64;
-        nv[j][f] = MODE == QM_FUSED ? rt[ni] + mu * po[ni] : vin[ni];
-        dz[j][f] = dzf[ebb[f] + (int64_t)kk * 64];
+        const uint32_t kk = (uint32_t)min(k, lim[f]) * 64u;
+        const uint32_t ni = nbb[f] + kk;
+        nv[j][f] = MODE == QM_FUSED ? at32(rt, ni) + mu * at32(po, ni) : at32(vin, ni);
+        dz[j][f] = at32(dzf, ebb[f] + kk);
       }
     }
     if (MODE == QM_FUSED) {
@@ -197,20 +218,20 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
         else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
         else if (k == n - 1) y += c0 * vm1 - c0 * v0;
         else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
-        const int64_t idx = base + (int64_t)k * 64;
+        const uint32_t idx = base + (uint32_t)k * 64u;
         if (MODE == QM_FUSED) {
-          V.pnew[idx] = v0;
-          V.x[idx] = xv[j] + nu * pv[j];
+          st_stream<NTS>(&at32(V.pnew, idx), v0);
+          st_stream<NTS>(&at32(V.x, idx), xv[j] + nu * pv[j]);
           dot += (double)v0 * (double)y;
-          V.out[idx] = y;
+          st_stream<NTS, TO>(&at32(V.out, idx), (TO)y);
         } else if (MODE == QM_RESID) {
-          V.out[idx] = V.src[idx] - y;
+          at32(V.out, idx) = at32(V.src, idx) - y;
         } else if (MODE == QM_RESID_DOT) {
-          const T rk = V.src[idx] - y;
-          V.out[idx] = (TO)rk;
+          const T rk = at32(V.src, idx) - y;
+          at32(V.out, idx) = (TO)rk;
           dot += (double)rk * (double)rk;
         } else {
-          V.out[idx] = y;
+          at32(V.out, idx) = y;
         }
       }
     }
@@ -222,20 +243,23 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
 // Vertical-line preconditioner of tile t: (UPDATE) r -= nu*z, then
 // out = M^-1 r by Thomas down the column (b' recomputed, d' stored in out)
 // and back up it (b' from bq); rz and rr collect the lane's r.out and r.r.
-template <bool UPDATE, int U, typename T>
+// RZID: rz from the forward sweep alone, r.M^-1 r = sum d'^2/b' (M = L D L^T
+// with D = diag(b') and d' = L^-1 r), so the backward sweep does not read r
+// again; out is unchanged, only the rounding of rz differs.
+template <bool UPDATE, int U, typename T, bool RZID = false>
 __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ coef, const T *__restrict__ z,
                                            T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq, T nu,
                                            int t, int l, double &rz, double &rr) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
   if (n == 0 || G.comp[s] != 1) return;
-  const int64_t base = G.toff[t] + l;
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
   if (kt == n - 1) {
-    const int64_t idx = base + (int64_t)kt * 64;
-    T rk = r[idx];
-    if (UPDATE) { rk = rk - nu * z[idx]; r[idx] = rk; }
-    const T u = T(-0.5) * rk / coef[idx];
-    out[idx] = u;
+    const uint32_t idx = base + (uint32_t)kt * 64u;
+    T rk = at32(r, idx);
+    if (UPDATE) { rk = rk - nu * at32(z, idx); at32(r, idx) = rk; }
+    const T u = T(-0.5) * rk / at32(coef, idx);
+    at32(out, idx) = u;
     rz += (double)rk * (double)u;
     rr += (double)rk * (double)rk;
     return;
@@ -245,20 +269,20 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
     T rv[U], zv[U], cv[U + 1];
 #pragma unroll
     for (int j = 0; j <= U; j++) {
-      const int64_t idx = base + (int64_t)min(kb + j, n) * 64;
-      cv[j] = coef[idx];
+      const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
+      cv[j] = at32(coef, idx);
       if (j < U) {
-        rv[j] = r[idx];
-        zv[j] = UPDATE ? z[idx] : T(0);
+        rv[j] = at32((const T *)r, idx);
+        zv[j] = UPDATE ? at32(z, idx) : T(0);
       }
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int k = kb + j;
       if (k < n) {
-        const int64_t idx = base + (int64_t)k * 64;
+        const uint32_t idx = base + (uint32_t)k * 64u;
         T rk = rv[j];
-        if (UPDATE) { rk = rk - nu * zv[j]; r[idx] = rk; }
+        if (UPDATE) { rk = rk - nu * zv[j]; at32(r, idx) = rk; }
         rr += (double)rk * (double)rk;
         const T ck = cv[j], ck1 = cv[j + 1];
         T a = 0, b, c = 0;
@@ -270,7 +294,8 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
           b -= a * cprev / bprev;
           d -= a * dprev / bprev;
         }
-        out[idx] = d;
+        at32(out, idx) = d;
+        if constexpr (RZID) rz += (double)d * (double)d / (double)b;
         cprev = c; bprev = b; dprev = d;
       }
     }
@@ -280,11 +305,11 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
     T bv[U], dv[U], rv[U], cv[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      const int64_t idx = base + (int64_t)max(ke - j, kt) * 64;
-      bv[j] = bq[idx];
-      dv[j] = out[idx];
-      rv[j] = r[idx];
-      cv[j] = coef[idx];
+      const uint32_t idx = base + (uint32_t)max(ke - j, kt) * 64u;
+      bv[j] = at32(bq, idx);
+      dv[j] = at32((const T *)out, idx);
+      rv[j] = RZID ? T(0) : at32((const T *)r, idx);
+      cv[j] = at32(coef, idx);
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -292,8 +317,8 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       if (k >= kt) {
         const T b = bv[j], d = dv[j];
         const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
-        out[base + (int64_t)k * 64] = u;
-        rz += (double)rv[j] * (double)u;
+        at32(out, base + (uint32_t)k * 64u) = u;
+        if constexpr (!RZID) rz += (double)rv[j] * (double)u;
         unext = u;
         cnext = cv[j];  // c at level k-1 is coef[k]
       }

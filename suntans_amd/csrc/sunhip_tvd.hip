@@ -346,8 +346,12 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
   A.oQ = sq->d; A.oQC = sqc->d; A.SfHp = hp->d; A.SfHm = hm->d; A.bscal = bs ? bs->d : nullptr;
   A.theta = sp->theta; A.TVD = sp->TVD;
   const Geo G = g->geo();
+  int ps = prof_begin(g, PK_TSUMS);
   k_tvd_sums<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
+  prof_end(g, ps);
+  ps = prof_begin(g, PK_TFACES);
   k_tvd_faces<<<(g->Net + 3) / 4, 256, 0, g->stream>>>(G, A);
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -373,7 +377,9 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
   A.active = act->d; A.Cn = cn->d; A.tb = tb->d; A.tc = tc->d; A.td = td->d;
   A.theta = sp->theta; A.dt = sp->dt; A.kappa = sp->kappa; A.TVD = sp->TVD; A.step = sp->step;
   A.alpha_top = sp->alpha_top; A.alpha_bot = sp->alpha_bot;
+  const int ps = prof_begin(g, PK_TCOLUMN);
   k_update_column<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), A);
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;

@@ -235,11 +235,12 @@ class DeviceGrid:
 
     def kernel_times(self):
         """{kernel: (total ms, launches)} accumulated since profiling was enabled."""
-        ms = (C.c_double * 8)()
-        n = (C.c_int * 8)()
+        ms = (C.c_double * 16)()
+        n = (C.c_int * 16)()
         k = lib().sunhip_kernel_times(self.h, ms, n)
-        names = ("qapply", "qprec", "happly", "hprec", "mixed_apply", "mixed_prec", "resid64", "other")
-        return {names[i]: (ms[i], n[i]) for i in range(min(k, 8)) if n[i] > 0}
+        names = ("qapply", "qprec", "happly", "hprec", "mixed_apply", "mixed_prec", "resid64", "other",
+                 "tvd_sums", "tvd_faces", "tvd_column")
+        return {names[i]: (ms[i], n[i]) for i in range(min(k, len(names))) if n[i] > 0}
 
     def copy_probe(self, nbytes: int, reps: int):
         """Device copy kernel of nbytes, reps times; returns average ms."""
