@@ -17,7 +17,7 @@
 
 #include <cmath>
 
-#include "sunhip_common.h"
+#include "sunhip_qtile.h"
 
 using namespace sunhip;
 
@@ -44,30 +44,49 @@ struct TvdArgs {
   int TVD;
 };
 
-// pass 1 (main/tvd.c:46-65), cells of celldist[0..2)
+// pass 1 (main/tvd.c:46-65), cells of celldist[0..2).  The faces' level-
+// invariant data (edge base, df, normal, neighbour base and depth) are read
+// once per column.
 __global__ __launch_bounds__(256) void k_tvd_sums(Geo G, TvdArgs A) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s];
   if (n == 0 || G.comp[s] == 0) return;
   const int NS = G.NS, nf = G.nf[s];
-  const int64_t base = G.toff[t] + l;
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
   const double th = A.theta;
-  for (int k = G.ctop[s]; k < n; k++) {
-    const int64_t idx = base + (int64_t)k * 64;
-    const double si = A.s[idx];
-    double sq = 0, sqc = 0;
-    for (int f = 0; f < nf; f++) {
-      const int64_t ei = G.eb[f * NS + s] + (int64_t)k * 64;
-      const int es = G.es[f * NS + s], nbs = G.nbs[f * NS + s];
-      const int nrm = G.nrm[f * NS + s];
-      const double un = nrm * (th * A.u[ei] + (1 - th) * A.utmp2[ei]);
-      const double Qm = 0.5 * A.dzf[ei] * G.df[es] * fabs(un - fabs(un));
-      if (nbs >= 0 && k < G.nk[nbs]) sqc += Qm * (si - A.s[G.nbb[f * NS + s] + (int64_t)k * 64]);
-      sq += Qm;
+  uint32_t eb[SUNHIP_MAXF], nbb[SUNHIP_MAXF];
+  double dfe[SUNHIP_MAXF];
+  int nrm[SUNHIP_MAXF], nkn[SUNHIP_MAXF];
+#pragma unroll
+  for (int f = 0; f < SUNHIP_MAXF; f++) {
+    if (f < nf) {
+      const int nbs = G.nbs[f * NS + s];
+      eb[f] = (uint32_t)G.eb[f * NS + s];
+      dfe[f] = G.df[G.es[f * NS + s]];
+      nrm[f] = G.nrm[f * NS + s];
+      nkn[f] = nbs >= 0 ? G.nk[nbs] : 0;  // no neighbour: never below its bottom
+      nbb[f] = nbs >= 0 ? (uint32_t)G.nbb[f * NS + s] : base;
+    } else {
+      eb[f] = 0; dfe[f] = 0; nrm[f] = 0; nkn[f] = 0; nbb[f] = base;
     }
-    A.oQ[idx] = sq;
-    A.oQC[idx] = sqc;
+  }
+  for (int k = G.ctop[s]; k < n; k++) {
+    const uint32_t k64 = (uint32_t)k * 64u, idx = base + k64;
+    const double si = at32(A.s, idx);
+    double sq = 0, sqc = 0;
+#pragma unroll
+    for (int f = 0; f < SUNHIP_MAXF; f++) {
+      if (f < nf) {
+        const uint32_t ei = eb[f] + k64;
+        const double un = nrm[f] * (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei));
+        const double Qm = 0.5 * at32(A.dzf, ei) * dfe[f] * fabs(un - fabs(un));
+        if (k < nkn[f]) sqc += Qm * (si - at32(A.s, nbb[f] + k64));
+        sq += Qm;
+      }
+    }
+    at32(A.oQ, idx) = sq;
+    at32(A.oQC, idx) = sqc;
   }
 }
 
@@ -174,26 +193,53 @@ __device__ __forceinline__ double upwind_ref(double u, double a, double b) {
   return dmax_ref(a, b);
 }
 
-// horizontal TVD flux divergence at level k (main/scalars.c:251-290)
-__device__ double hflux(const Geo &G, const ColArgs &A, int s, int k, double Ac) {
-  const int NS = G.NS;
-  double acc = 0;
-  for (int f = 0; f < G.nf[s]; f++) {
-    const int es = G.es[f * NS + s];
-    if (k >= G.e_nke[es]) continue;
-    const int64_t ei = G.eb[f * NS + s] + (int64_t)k * 64;
-    const double ut2 = A.utmp2[ei];
-    double temp;
-    if (A.TVD) {
-      temp = ut2 > 0 ? A.SfHp[ei] : A.SfHm[ei];
-    } else {
+// A face's level-invariant part of the horizontal flux of main/scalars.c:
+// 251-290: the edge's layer count and base, the upwind cell bases, and
+// cf = dt*df*normal/Ac, the leading factors of the flux in the reference's
+// order (so the product below rounds as the reference's does).
+struct FaceC {
+  uint32_t eb;
+  int nke;
+  int64_t c1, c2;
+  double cf;
+};
+
+__device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int s, double Ac, FaceC (&fc)[SUNHIP_MAXF]) {
+  const int NS = G.NS, nf = G.nf[s];
+#pragma unroll
+  for (int f = 0; f < SUNHIP_MAXF; f++) {
+    if (f < nf) {
+      const int es = G.es[f * NS + s];
+      fc[f].eb = (uint32_t)G.eb[f * NS + s];
+      fc[f].nke = G.e_nke[es];
       int64_t c1 = G.e_c1[es], c2 = G.e_c2[es];
       if (c1 < 0) c1 = c2;
       if (c2 < 0) c2 = c1;
-      temp = upwind_ref(ut2, A.s_old[c1 + (int64_t)k * 64], A.s_old[c2 + (int64_t)k * 64]);
+      fc[f].c1 = c1;
+      fc[f].c2 = c2;
+      fc[f].cf = A.dt * G.df[es] * G.nrm[f * NS + s] / Ac;
+    } else {
+      fc[f].eb = 0; fc[f].nke = 0; fc[f].c1 = 0; fc[f].c2 = 0; fc[f].cf = 0;
     }
-    const int nrm = G.nrm[f * NS + s];
-    acc += A.dt * G.df[es] * nrm / Ac * (A.theta * A.u[ei] + (1 - A.theta) * ut2) * temp * A.dzf[ei];
+  }
+}
+
+// horizontal TVD flux divergence at level k (main/scalars.c:251-290)
+__device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[SUNHIP_MAXF], int nf, int k) {
+  double acc = 0;
+  const uint32_t k64 = (uint32_t)k * 64u;
+#pragma unroll
+  for (int f = 0; f < SUNHIP_MAXF; f++) {
+    if (f >= nf || k >= fc[f].nke) continue;
+    const uint32_t ei = fc[f].eb + k64;
+    const double ut2 = at32(A.utmp2, ei);
+    double temp;
+    if (A.TVD) {
+      temp = ut2 > 0 ? at32(A.SfHp, ei) : at32(A.SfHm, ei);
+    } else {
+      temp = upwind_ref(ut2, A.s_old[fc[f].c1 + (int64_t)k * 64], A.s_old[fc[f].c2 + (int64_t)k * 64]);
+    }
+    acc += fc[f].cf * (A.theta * at32(A.u, ei) + (1 - A.theta) * ut2) * temp * at32(A.dzf, ei);
   }
   return acc;
 }
@@ -207,6 +253,9 @@ __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
   const int64_t base = G.toff[t] + l;
   const double Ac = G.Ac[s], theta = A.theta, dt = A.dt;
   const Col dzz{A.dzz, base, nk}, dzo{A.dzzold, base, nk}, so{A.s_old, base, nk};
+  FaceC fc[SUNHIP_MAXF];
+  face_consts(G, A, s, Ac, fc);
+  const int nfs = G.nf[s];
   int ktop;
   double dznew;
   if (ct >= cto) {
@@ -234,7 +283,7 @@ __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
   const double td = theta * dt, otd = (1 - theta) * dt;
   // horizontal source of the top row: levels 0..ktop fold into row 0
   double cn0 = 0;
-  for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux(G, A, s, k, Ac);
+  for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux(A, fc, nfs, k);
 
   // forward: assemble row m (level k = ktop+m) and eliminate
   double bprev = 1, cprev = 0, dprev = 0;
@@ -294,7 +343,7 @@ __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
              otd * (bd_k * so.at(k - 1) - (bd_k + bd_k1) * sk + bd_k1 * so.at(k + 1));
       }
       // Cn[m] = 0 - ap[k]; d[m] += fab*Cn[m] with fab = 1
-      const double cnk = -hflux(G, A, s, k, Ac);
+      const double cnk = -hflux(A, fc, nfs, k);
       d += cnk;
       A.Cn[base + (int64_t)k * 64] = cnk;
     }
