@@ -73,6 +73,8 @@ typedef struct {
   double eps0;
   double device_ms;    /* device time of the iteration loop (HIP events) */
   double setup_ms;     /* device time of coefficients + initial residual */
+  int outer;           /* mixed forms: refinement steps (0 otherwise) */
+  int inner;           /* mixed forms: FP32 PCG iterations (0 otherwise) */
 } sunhip_cg_stats;
 
 const char *sunhip_error(void);
@@ -126,13 +128,19 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
                          sunhip_cg_stats *st, int *inner_iters);
 /* QCoefficients from the device field "dzz" into "coef". */
 int sunhip_q_coefficients(sunhip_grid *g);
-/* Kernel tuning: "unroll" (levels per load group: 1, 2, 4), "reduce"
+/* Kernel tuning: "unroll" (levels per load group of the operator kernel:
+ * 1, 2, 4, 8), "prec_unroll" (the same for the preconditioner), "reduce"
  * (0: last block finishes the dot products, 1: a finisher kernel), "tiles"
  * (tiles of 64 columns per block of the iteration's operator kernel: 4, 8,
  * 16), "ntstore" (1: its results stored with the non-temporal hint), "rzid"
  * (1: the preconditioned residual dot r.M^-1 r taken as sum d'^2/b' from
  * the forward Thomas sweep, so the backward sweep does not re-read r;
- * M^-1 r itself is unchanged). */
+ * M^-1 r itself is unchanged), "minwaves" (register budget of the
+ * operator kernel: 3 or 4 waves per SIMD; 0 the compiler's choice),
+ * "blocklds" (1: the operator's neighbour values staged in LDS per block of
+ * four tiles walking the levels together).  Only "tiles", "reduce" and
+ * "rzid" change the rounding of a dot product; every other knob gives
+ * bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event
  * pair around every solver kernel launch; sunhip_kernel_times fills ms[] and
@@ -206,13 +214,19 @@ int sunhip_batch_detach(sunhip_batch *b, sunhip_grid *g);
 int sunhip_batch_begin_q(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, int64_t tag);
 int sunhip_batch_begin_h(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, const sunhip_h_phys *ph,
                          int64_t tag);
+/* Start a mixed-precision pressure solve of g (as sunhip_q_solve_mixed): its
+ * refinement steps advance with every poll, its FP64 residuals and FP32
+ * inner iterations sharing launches with the other members'. */
+int sunhip_batch_begin_q_mixed(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, double inner_tol,
+                               int max_outer, int64_t tag);
 /* Enqueue at least iters iterations of every active solve (whole graph
  * chunks); returns without waiting. */
 int sunhip_batch_advance(sunhip_batch *b, int iters);
 /* Wait for the enqueued work, retire up to max converged (or exhausted)
  * solves: finish their x, return their tags, kinds (0 pressure FP64,
- * 1 free surface, 2 pressure FP32 inner) and stats (device_ms = time since
- * begin).  Returns the number retired. */
+ * 1 free surface, 2 pressure FP32 inner, 3 pressure mixed precision: niters
+ * and inner = FP32 iterations, outer = refinement steps) and stats
+ * (device_ms = time since begin).  Returns the number retired. */
 int sunhip_batch_poll(sunhip_batch *b, int64_t *tags, int *kinds, sunhip_cg_stats *st, int max);
 int sunhip_batch_active(const sunhip_batch *b);
 int sunhip_batch_sync(sunhip_batch *b);

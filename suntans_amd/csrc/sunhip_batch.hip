@@ -187,7 +187,13 @@ __global__ void kb_gather(CGScal *const *__restrict__ src, CGScal *__restrict__ 
 // ---------------------------------------------------------------------------
 // host side
 
-enum TaskKind { T_Q64 = 0, T_H64 = 1, T_Q32 = 2, T_NKIND = 3 };
+// Kinds of solve; the first three are also the kinds of iteration table.  A
+// mixed-precision pressure solve (T_QMIX) walks through its refinement
+// steps on its own: residual to compute (M_RESID), residual computed and
+// waiting to be read (M_READ), FP32 inner solve running in the FP32 table
+// (M_INNER).
+enum TaskKind { T_Q64 = 0, T_H64 = 1, T_Q32 = 2, T_NKIND = 3, T_QMIX = 3 };
+enum MixState { M_RESID = 1, M_READ = 2, M_INNER = 3 };
 
 struct Task {
   sunhip_grid *g;
@@ -198,6 +204,13 @@ struct Task {
   HSolver<double> h64;
   int stage;  // pinned staging slot of its scalar upload
   std::chrono::steady_clock::time_point t0;
+  // mixed precision (T_QMIX)
+  int mstate = 0, outer = 0, inner_total = 0, max_outer = 0;
+  double eps0 = -1, eps = 0, inner_tol = 0;
+  sunhip_cg_params prm{};
+  QVec<double, float> R{};
+  // the iteration table a task is in (-1 none)
+  int table() const { return kind == T_QMIX ? (mstate == M_INNER ? T_Q32 : -1) : kind; }
 };
 
 template <typename T>
@@ -273,10 +286,12 @@ struct sunhip_batch {
   DevBuf<BRange> rres;
   DevBuf<int2> wres;
   DevBuf<double> pres;
+  int nres = 0, nwres = 0;  // members and items of the residual table
   // polling
   DevBuf<CGScal *> d_scp;
   DevBuf<CGScal> d_scv;
   HostBuf<CGScal> h_scv;
+  HostBuf<CGScal *> h_scp;
   // graph of one chunk over the current tables
   hipGraphExec_t gexec = nullptr;
   int gchunk = 0;
@@ -338,13 +353,28 @@ int rebuild(sunhip_batch *b) {
   std::vector<BQMember<double, double>> q64;
   std::vector<BQMember<float, float>> q32;
   std::vector<BHMember> h64;
-  std::vector<BRange> rg[T_NKIND];
-  std::vector<int2> wk[T_NKIND];
+  std::vector<BQMember<double, float>> res;
+  std::vector<BRange> rg[T_NKIND], rgr;
+  std::vector<int2> wk[T_NKIND], wkr;
   int nf = 3;
   for (const Task &t : b->tasks) {
     sunhip_grid *g = t.g;
     nf = std::max(nf, g->mf);
-    const int k = t.kind;
+    if (t.kind == T_QMIX && t.mstate == M_RESID) {
+      // FP64 true residual of a mixed solve
+      const int row = (int)rgr.size();
+      BRange r{g->d_mo, (int)wkr.size(), 0};
+      for (int i = 0; i < g->Nt; i++) wkr.push_back(make_int2(row, i));
+      r.whi = (int)wkr.size();
+      BQMember<double, float> m{};
+      m.G = g->geo(); m.coef = t.R.coef; m.dzf = t.R.dzf; m.D = t.R.D; m.vin = t.R.vin; m.src = t.R.src;
+      m.out = t.R.out; m.sc = g->d_mo;
+      res.push_back(m);
+      rgr.push_back(r);
+      continue;
+    }
+    const int k = t.table();
+    if (k < 0) continue;
     const int row = (int)rg[k].size();
     const int ntiles = k == T_H64 ? (g->NS + 63) / 64 : g->Nt;
     BRange r{nullptr, (int)wk[k].size(), 0};
@@ -375,18 +405,28 @@ int rebuild(sunhip_batch *b) {
     rg[k].push_back(r);
   }
   int rc = 0;
-  size_t bytes = q64.size() * sizeof(q64[0]) + q32.size() * sizeof(q32[0]) + h64.size() * sizeof(h64[0]) + 8192;
+  size_t bytes = q64.size() * sizeof(q64[0]) + q32.size() * sizeof(q32[0]) + h64.size() * sizeof(h64[0]) +
+                 res.size() * sizeof(res[0]) + rgr.size() * sizeof(BRange) + wkr.size() * sizeof(int2) + 16384;
   for (int k = 0; k < T_NKIND; k++) bytes += rg[k].size() * sizeof(BRange) + wk[k].size() * sizeof(int2) + 512;
   // the staging area may still feed the previous upload
   HIPCHK(hipStreamSynchronize(b->stream));
   if ((rc = b->h_tab.ensure(bytes))) return rc;
   size_t off = 0;
   if ((rc = b->tq64.ensure(q64.size() + 1)) || (rc = b->tq32.ensure(q32.size() + 1)) ||
-      (rc = b->th64.ensure(h64.size() + 1)))
+      (rc = b->th64.ensure(h64.size() + 1)) || (rc = b->tres.ensure(res.size() + 1)) ||
+      (rc = b->rres.ensure(rgr.size() + 1)) || (rc = b->wres.ensure(wkr.size() + 1)) ||
+      (rc = b->pres.ensure(wkr.size() + 1)))
     return rc;
   if (!q64.empty()) rc |= table_upload(b, b->tq64.d, q64.data(), q64.size() * sizeof(q64[0]), off);
   if (!q32.empty()) rc |= table_upload(b, b->tq32.d, q32.data(), q32.size() * sizeof(q32[0]), off);
   if (!h64.empty()) rc |= table_upload(b, b->th64.d, h64.data(), h64.size() * sizeof(h64[0]), off);
+  if (!res.empty()) {
+    rc |= table_upload(b, b->tres.d, res.data(), res.size() * sizeof(res[0]), off);
+    rc |= table_upload(b, b->rres.d, rgr.data(), rgr.size() * sizeof(BRange), off);
+    rc |= table_upload(b, b->wres.d, wkr.data(), wkr.size() * sizeof(int2), off);
+  }
+  b->nres = (int)rgr.size();
+  b->nwres = (int)wkr.size();
   for (int k = 0; k < T_NKIND; k++) {
     b->nmem[k] = (int)rg[k].size();
     b->nwork[k] = (int)wk[k].size();
@@ -399,17 +439,6 @@ int rebuild(sunhip_batch *b) {
     }
   }
   if (rc) return rc;
-  // scalar pointers for polling, in task order
-  std::vector<CGScal *> scp;
-  for (const Task &t : b->tasks)
-    scp.push_back(t.kind == T_Q64 ? t.q64.sc : t.kind == T_Q32 ? t.q32.sc : t.h64.sc);
-  if ((rc = b->d_scp.ensure(scp.size() + 1)) || (rc = b->d_scv.ensure(scp.size() + 1)) ||
-      (rc = b->h_scv.ensure(scp.size() + 1)))
-    return rc;
-  if (!scp.empty()) {
-    if ((rc = b->h_tab.ensure(off + scp.size() * sizeof(CGScal *) + 256))) return rc;
-    rc = table_upload(b, b->d_scp.d, scp.data(), scp.size() * sizeof(CGScal *), off);
-  }
   b->nf = nf;
   b->dirty = false;
   if (b->gexec) {
@@ -417,6 +446,23 @@ int rebuild(sunhip_batch *b) {
     b->gexec = nullptr;
   }
   return rc;
+}
+
+// FP64 true residuals of the mixed solves waiting for one (one launch for
+// all of them, then their sums into each member's outer scalars).
+int launch_residuals(sunhip_batch *b) {
+  if (!b->nres) return 0;
+  const int nw = b->nwres, nb = (nw + BW - 1) / BW;
+  if (b->nf <= 3)
+    kb_qapply<QM_RESID_DOT, 3, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
+  else
+    kb_qapply<QM_RESID_DOT, 4, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
+  kb_finalize<1, 0><<<b->nres, 256, 0, b->stream>>>(b->rres.d, b->pres.d, OP_STORE, 0);
+  HIPCHK(hipGetLastError());
+  for (Task &t : b->tasks)
+    if (t.kind == T_QMIX && t.mstate == M_RESID) t.mstate = M_READ;
+  b->dirty = true;  // the residual table is spent
+  return 0;
 }
 
 int grid_for(const sunhip_batch *b, int nwork, const Config &c) {
@@ -643,7 +689,9 @@ int sunhip_batch_advance(sunhip_batch *b, int iters) {
   int rc;
   const bool fresh = b->dirty;
   if (b->dirty && (rc = rebuild(b))) return rc;
+  if ((rc = launch_residuals(b))) return rc;
   const int chunk = chunk_len(b);
+  if (!b->nwork[T_Q64] && !b->nwork[T_Q32] && !b->nwork[T_H64]) return 0;  // only residuals this time
   // choose the configuration: fixed, tuned for this shape, or the next
   // candidate to time
   Config c = b->cfg;
@@ -692,7 +740,18 @@ int sunhip_batch_poll(sunhip_batch *b, int64_t *tags, int *kinds, sunhip_cg_stat
   const int n = (int)b->tasks.size();
   if (n == 0) return 0;
   int rc;
-  if (b->dirty && (rc = rebuild(b))) return rc;
+  // the scalars each solve waits on (a mixed solve: its outer residual or
+  // its inner solve's scalars)
+  if ((rc = b->h_scp.ensure(n)) || (rc = b->d_scp.ensure(n)) || (rc = b->d_scv.ensure(n)) ||
+      (rc = b->h_scv.ensure(n)))
+    return rc;
+  for (int i = 0; i < n; i++) {
+    const Task &t = b->tasks[i];
+    CGScal *p = t.kind == T_Q64 ? t.q64.sc : t.kind == T_H64 ? t.h64.sc : t.kind == T_Q32 ? t.q32.sc
+              : (t.mstate == M_INNER ? t.q32.sc : t.g->d_mo);
+    b->h_scp.h[i] = p;
+  }
+  HIPCHK(hipMemcpyAsync(b->d_scp.d, b->h_scp.h, n * sizeof(CGScal *), hipMemcpyHostToDevice, b->stream));
   kb_gather<<<(n + 255) / 256, 256, 0, b->stream>>>(b->d_scp.d, b->d_scv.d, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->h_scv.h, b->d_scv.d, n * sizeof(CGScal), hipMemcpyDeviceToHost, b->stream));
@@ -704,6 +763,49 @@ int sunhip_batch_poll(sunhip_batch *b, int64_t *tags, int *kinds, sunhip_cg_stat
   for (int i = 0; i < n; i++) {
     const CGScal &c = b->h_scv.h[i];
     Task &t = b->tasks[i];
+    if (t.kind == T_QMIX) {
+      if (t.mstate == M_READ) {
+        // one refinement step decided (the rule of sunhip_q_solve_mixed)
+        t.eps = c.pz;
+        if (t.eps0 < 0) t.eps0 = t.eps;
+        const double ratio = t.eps == 0 ? 0.0 : sqrt(t.eps / t.eps0);
+        const bool conv = t.eps == 0 || ratio < t.prm.epsilon;
+        if (conv || t.outer >= t.max_outer) {
+          if (out >= max) { keep.push_back(t); continue; }
+          if (tags) tags[out] = t.tag;
+          if (kinds) kinds[out] = T_QMIX;
+          if (st) {
+            st[out].niters = t.inner_total;
+            st[out].converged = conv;
+            st[out].resid = ratio;
+            st[out].eps0 = t.eps0;
+            st[out].device_ms = std::chrono::duration<double, std::milli>(now - t.t0).count();
+            st[out].setup_ms = 0;
+            st[out].outer = t.outer;
+            st[out].inner = t.inner_total;
+          }
+          if (t.stage >= 0) b->free_stage.push_back(t.stage);
+          out++;
+          b->dirty = true;
+          continue;
+        }
+        sunhip_cg_params inner = t.prm;
+        inner.resnorm = 0;
+        inner.epsilon = std::max(t.inner_tol, 0.25 * t.prm.epsilon / ratio);
+        if ((rc = q_inner_begin(t.g, &inner, t.q32, b->h_stage.h + t.stage))) return rc;
+        t.mstate = M_INNER;
+        b->dirty = true;
+      } else if (t.mstate == M_INNER && c.done) {
+        t.inner_total += c.iter;
+        if ((rc = q_end32(t.g, t.q32))) return rc;
+        if ((rc = q_mix_update(t.g, const_cast<double *>(t.R.vin), t.q32.x))) return rc;
+        t.outer++;
+        t.mstate = M_RESID;
+        b->dirty = true;
+      }
+      keep.push_back(t);
+      continue;
+    }
     if (!c.done || out >= max) {
       keep.push_back(t);
       continue;
@@ -721,15 +823,33 @@ int sunhip_batch_poll(sunhip_batch *b, int64_t *tags, int *kinds, sunhip_cg_stat
       st[out].eps0 = c.eps0;
       st[out].device_ms = std::chrono::duration<double, std::milli>(now - t.t0).count();
       st[out].setup_ms = 0;
+      st[out].outer = 0;
+      st[out].inner = 0;
     }
     if (t.stage >= 0) b->free_stage.push_back(t.stage);
     out++;
-  }
-  if (out) {
-    b->tasks.swap(keep);
     b->dirty = true;
   }
+  b->tasks.swap(keep);
   return out;
+}
+
+int sunhip_batch_begin_q_mixed(sunhip_batch *b, sunhip_grid *g, const sunhip_cg_params *prm, double inner_tol,
+                               int max_outer, int64_t tag) {
+  int rc = sunhip_batch_attach(b, g);
+  if (rc) return rc;
+  Task t{};
+  t.g = g; t.kind = T_QMIX; t.tag = tag;
+  if ((t.stage = take_stage(b)) < 0) return -3;
+  if ((rc = q_mixed_prepare(g, prm, &t.q32, &t.R))) return rc;
+  t.mstate = M_RESID;
+  t.prm = *prm;
+  t.inner_tol = inner_tol;
+  t.max_outer = max_outer;
+  t.t0 = std::chrono::steady_clock::now();
+  b->tasks.push_back(t);
+  b->dirty = true;
+  return 0;
 }
 
 int sunhip_batch_sync(sunhip_batch *b) {
@@ -805,118 +925,25 @@ int sunhip_batch_h_solve(sunhip_batch *b, sunhip_grid **gs, int n, const sunhip_
   return 0;
 }
 
-// Mixed precision over many grids: every refinement step computes the FP64
-// true residuals of all unconverged members in one launch, then runs their
-// FP32 inner solves together, then adds the corrections in FP64.
+// Mixed precision over many grids: each member refines on its own (its FP64
+// residuals of a round computed in one launch with the others', its FP32
+// inner solves iterating in the same launches as the others'), so no member
+// waits for another's inner solve.
 int sunhip_batch_q_solve_mixed(sunhip_batch *b, sunhip_grid **gs, int n, const sunhip_cg_params *prm,
                                double inner_tol, int max_outer, sunhip_cg_stats *st, int *inner_iters) {
   if (!b->tasks.empty()) { set_error("sunhip_batch_q_solve_mixed needs an idle batch"); return -1; }
   HIPCHK(hipSetDevice(b->device));
   int rc;
-  std::vector<QSolver<float>> S(n);
-  std::vector<QVec<double, float>> R(n);
-  std::vector<double> eps0(n, -1.0), eps(n, 0.0);
-  std::vector<int> state(n, 1), outer(n, 0), inner_tot(n, 0), stage(n, -1);
-  const auto t0 = std::chrono::steady_clock::now();
-  for (int i = 0; i < n; i++) {
-    if ((rc = sunhip_batch_attach(b, gs[i]))) return rc;
-    if ((rc = q_mixed_prepare(gs[i], prm, &S[i], &R[i]))) return rc;
-    if ((stage[i] = take_stage(b)) < 0) return -3;
-  }
-  int nf = 3;
-  for (int i = 0; i < n; i++) nf = std::max(nf, gs[i]->mf);
-  for (;;) {
-    // FP64 true residuals of the members still refining
-    std::vector<int> act;
-    for (int i = 0; i < n; i++) if (state[i] == 1) act.push_back(i);
-    if (act.empty()) break;
-    std::vector<BQMember<double, float>> tab;
-    std::vector<BRange> rg;
-    std::vector<int2> wk;
-    for (int j = 0; j < (int)act.size(); j++) {
-      const int i = act[j];
-      sunhip_grid *g = gs[i];
-      BQMember<double, float> m{};
-      m.G = g->geo(); m.coef = R[i].coef; m.dzf = R[i].dzf; m.D = R[i].D; m.vin = R[i].vin; m.src = R[i].src;
-      m.out = R[i].out; m.sc = g->d_mo;
-      tab.push_back(m);
-      BRange r{g->d_mo, (int)wk.size(), 0};
-      for (int t = 0; t < g->Nt; t++) wk.push_back(make_int2(j, t));
-      r.whi = (int)wk.size();
-      rg.push_back(r);
-    }
-    HIPCHK(hipStreamSynchronize(b->stream));
-    size_t bytes = tab.size() * sizeof(tab[0]) + rg.size() * sizeof(BRange) + wk.size() * sizeof(int2) + 1024;
-    if ((rc = b->h_tab.ensure(bytes)) || (rc = b->tres.ensure(tab.size())) || (rc = b->rres.ensure(rg.size())) ||
-        (rc = b->wres.ensure(wk.size())) || (rc = b->pres.ensure(wk.size())))
-      return rc;
-    size_t off = 0;
-    rc = table_upload(b, b->tres.d, tab.data(), tab.size() * sizeof(tab[0]), off) |
-         table_upload(b, b->rres.d, rg.data(), rg.size() * sizeof(BRange), off) |
-         table_upload(b, b->wres.d, wk.data(), wk.size() * sizeof(int2), off);
-    if (rc) return rc;
-    const int nw = (int)wk.size(), nb = (nw + BW - 1) / BW;
-    if (nf <= 3) kb_qapply<QM_RESID_DOT, 3, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
-    else kb_qapply<QM_RESID_DOT, 4, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
-    kb_finalize<1, 0><<<(int)act.size(), 256, 0, b->stream>>>(b->rres.d, b->pres.d, OP_STORE, 0);
-    HIPCHK(hipGetLastError());
-    // the residual norms
-    std::vector<CGScal *> ptrs;
-    for (int i : act) ptrs.push_back(gs[i]->d_mo);
-    if ((rc = b->d_scp.ensure(ptrs.size())) || (rc = b->d_scv.ensure(ptrs.size())) ||
-        (rc = b->h_scv.ensure(ptrs.size())))
-      return rc;
-    HIPCHK(hipStreamSynchronize(b->stream));
-    off = 0;
-    if ((rc = table_upload(b, b->d_scp.d, ptrs.data(), ptrs.size() * sizeof(CGScal *), off))) return rc;
-    kb_gather<<<((int)ptrs.size() + 255) / 256, 256, 0, b->stream>>>(b->d_scp.d, b->d_scv.d, (int)ptrs.size());
-    HIPCHK(hipMemcpyAsync(b->h_scv.h, b->d_scv.d, ptrs.size() * sizeof(CGScal), hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    b->dirty = true;  // the polling pointers were overwritten
-    std::vector<int> inner_set;
-    for (int j = 0; j < (int)act.size(); j++) {
-      const int i = act[j];
-      eps[i] = b->h_scv.h[j].pz;
-      if (eps0[i] < 0) eps0[i] = eps[i];
-      const double ratio = eps[i] == 0 ? 0.0 : sqrt(eps[i] / eps0[i]);
-      if (eps[i] == 0 || ratio < prm->epsilon) { state[i] = 2; continue; }
-      if (outer[i] >= max_outer) { state[i] = 0; continue; }
-      sunhip_cg_params inner = *prm;
-      inner.resnorm = 0;
-      inner.epsilon = std::max(inner_tol, 0.25 * prm->epsilon / ratio);
-      if ((rc = q_inner_begin(gs[i], &inner, S[i], b->h_stage.h + stage[i]))) return rc;
-      Task t{};
-      t.g = gs[i]; t.kind = T_Q32; t.tag = i; t.q32 = S[i]; t.stage = -1;
-      t.t0 = std::chrono::steady_clock::now();
-      b->tasks.push_back(t);
-      inner_set.push_back(i);
-    }
-    if (inner_set.empty()) continue;
-    b->dirty = true;
-    std::vector<int64_t> tags;
-    std::vector<sunhip_cg_stats> sts;
-    // (the inner tasks carry no stage slot of their own: the mixed solve owns them)
-    if ((rc = drain(b, tags, sts))) return rc;
-    for (size_t k = 0; k < tags.size(); k++) inner_tot[tags[k]] += sts[k].niters + (sts[k].converged ? 1 : 0);
-    for (int i : inner_set) {
-      if ((rc = q_mix_update(gs[i], const_cast<double *>(R[i].vin), S[i].x))) return rc;
-      outer[i]++;
-    }
+  for (int i = 0; i < n; i++)
+    if ((rc = sunhip_batch_begin_q_mixed(b, gs[i], prm, inner_tol, max_outer, i))) return rc;
+  std::vector<int64_t> tags;
+  std::vector<sunhip_cg_stats> sts;
+  if ((rc = drain(b, tags, sts))) return rc;
+  for (size_t k = 0; k < tags.size(); k++) {
+    if (st) { st[tags[k]] = sts[k]; st[tags[k]].niters = sts[k].outer; }
+    if (inner_iters) inner_iters[tags[k]] = sts[k].inner;
   }
   HIPCHK(hipStreamSynchronize(b->stream));
-  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  for (int i = 0; i < n; i++) {
-    b->free_stage.push_back(stage[i]);
-    if (st) {
-      st[i].niters = outer[i];
-      st[i].converged = state[i] == 2;
-      st[i].resid = eps0[i] > 0 ? sqrt(eps[i] / eps0[i]) : 0;
-      st[i].eps0 = eps0[i];
-      st[i].device_ms = ms;
-      st[i].setup_ms = 0;
-    }
-    if (inner_iters) inner_iters[i] = inner_tot[i];
-  }
   return 0;
 }
 

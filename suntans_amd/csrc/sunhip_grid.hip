@@ -389,6 +389,9 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   if (rc) { sunhip_grid_destroy(g); return rc; }
   HIPCHK(hipMemset(g->d_count, 0, sizeof(unsigned) * 64));
   HIPCHK(hipHostMalloc((void **)&g->h_sc, sizeof(CGScal), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&g->h_poll, 2 * sizeof(CGScal), hipHostMallocDefault));
+  HIPCHK(hipEventCreateWithFlags(&g->evp[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g->evp[1], hipEventDisableTiming));
   HIPCHK(hipEventCreate(&g->ev0));
   HIPCHK(hipEventCreate(&g->ev1));
   HIPCHK(hipEventCreate(&g->ev2));
@@ -412,6 +415,8 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);
+  if (g->h_poll) hipHostFree(g->h_poll);
+  for (hipEvent_t e : g->evp) if (e) hipEventDestroy(e);
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
   if (g->ev2) hipEventDestroy(g->ev2);
@@ -623,6 +628,19 @@ int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n) {
 }  // extern "C"
 
 namespace sunhip {
+int poll_chunk(sunhip_grid *g, const CGScal *d, int &slot, bool &pending, bool *done) {
+  HIPCHK(hipMemcpyAsync(&g->h_poll[slot], d, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipEventRecord(g->evp[slot], g->stream));
+  *done = false;
+  if (pending) {
+    HIPCHK(hipEventSynchronize(g->evp[slot ^ 1]));
+    *done = g->h_poll[slot ^ 1].done != 0;
+  }
+  pending = true;
+  slot ^= 1;
+  return 0;
+}
+
 int prof_begin(sunhip_grid *g, int kind) {
   if (!g->prof) return -1;
   if (g->prof_used * 2 + 2 > (int)g->prof_pool.size()) prof_flush(g);

@@ -201,16 +201,15 @@ int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
 
 template <typename T>
 int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a, int pk_p) {
-  int every = prm->check_every > 0 ? prm->check_every : 32;
+  int every = prm->check_every > 0 ? prm->check_every : 8;
   if (every & 1) every++;
-  HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  int it = 0, rc;
-  while (!g->h_sc->done && it < prm->maxiters) {
+  int it = 0, rc, slot = 0;
+  bool pending = false, done = false;
+  // iterations enqueued after convergence return at once (device flag)
+  while (!done && it < prm->maxiters) {
     for (int b = 0; b < every && it < prm->maxiters; b++, it++)
       if ((rc = h_iteration<T>(g, S, it, pk_a, pk_p))) return rc;
-    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
+    if ((rc = poll_chunk(g, S.sc, slot, pending, &done))) return rc;
   }
   k_hfinish<T><<<nbh(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
@@ -219,8 +218,10 @@ int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a,
 
 int upload_hscal(sunhip_grid *g, CGScal *d, const sunhip_cg_params *prm) { return upload_scal(g, d, prm); }
 
-int fill_stats(sunhip_grid *g, const CGScal &c, sunhip_cg_stats *st) {
+int fill_stats(sunhip_grid *g, const CGScal &c, sunhip_cg_stats *st, int outer = 0, int inner = 0) {
   if (!st) return 0;
+  st->outer = outer;
+  st->inner = inner;
   st->niters = c.nloop;
   st->converged = c.converged;
   st->resid = c.resid;
@@ -345,7 +346,7 @@ int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double gra
   CGScal c{};
   c.nloop = outer; c.converged = converged; c.resid = eps0 > 0 ? sqrt(eps / eps0) : 0; c.eps0 = eps0;
   if (inner_iters) *inner_iters = total;
-  return fill_stats(g, c, st);
+  return fill_stats(g, c, st, outer, total);
 }
 
 int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y) {

@@ -117,6 +117,8 @@ struct sunhip_grid {
   sunhip::CGScal *d_qs = nullptr, *d_hs = nullptr;
   sunhip::CGScal *d_ms = nullptr, *d_mo = nullptr;  // mixed form: inner FP32 CG, outer residual
   sunhip::CGScal *h_sc = nullptr;   // pinned mirror
+  sunhip::CGScal *h_poll = nullptr; // two pinned slots of pipelined convergence polling
+  hipEvent_t evp[2] = {nullptr, nullptr};
   std::map<std::string, sunhip::Field> fields;
   std::map<std::string, float *> fields32;  // FP32 copies for the mixed form (cell layout)
   float *d_D32 = nullptr;
@@ -132,6 +134,8 @@ struct sunhip_grid {
   // forward sweep (see qprec_tile)
   int tune_tiles = 4, tune_nts = 1, tune_rzid = 1;
   int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
+  int tune_blk = 0;   // operator kernel with block-staged neighbour values
+  int tune_punroll = 8;  // levels per load group of the preconditioner (1, 2, 4, 8)
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of
   // the solver kernels, resolved by sunhip_kernel_times
@@ -149,6 +153,12 @@ namespace sunhip {
 // Bracket one launch with events when profiling is on.  Returns the pair
 // index or -1.
 int prof_begin(sunhip_grid *g, int kind);
+// Pipelined convergence polling of a CG loop: after each chunk of iterations
+// the scalars at d are copied to one of two pinned slots; the host then
+// waits for the previous chunk's copy while the chunk just enqueued runs, so
+// the GPU does not idle during the poll.  Returns 0 with *done set, or an
+// error code.
+int poll_chunk(sunhip_grid *g, const CGScal *d, int &slot, bool &pending, bool *done);
 void prof_end(sunhip_grid *g, int slot);
 enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, PK_MPREC = 5, PK_RESID = 6,
        PK_OTHER = 7, PK_TSUMS = 8, PK_TFACES = 9, PK_TCOLUMN = 10, PK_COUNT = 11 };

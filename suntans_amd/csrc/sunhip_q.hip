@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "sunhip_solvers.h"
 
@@ -127,6 +128,16 @@ __global__ __launch_bounds__(TPB * 64, MINW > 0 ? MINW : 1) void k_qapply(Geo G,
   }
 }
 
+// The fused operator with block-staged neighbour values (qapply_block).
+template <int NF, int U, typename T, bool NTS>
+__global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
+  if (sc->done) return;
+  const int t0 = xcd_block(blockIdx.x, gridDim.x) * 4;
+  const double dot = qapply_block<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t0);
+  double dv[1] = {dot};
+  reduce_out<1, 256>(dv, R, sc);
+}
+
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
 template <bool UPDATE, int U, typename T, bool RZID = false>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
@@ -188,7 +199,7 @@ void read_tuning(sunhip_grid *g) {
   if (g->tune_unroll < 0) {
     const char *e = getenv("SUNHIP_QUNROLL");
     g->tune_unroll = e ? atoi(e) : 4;
-    if (g->tune_unroll != 1 && g->tune_unroll != 2 && g->tune_unroll != 4) g->tune_unroll = 4;
+    if (g->tune_unroll != 1 && g->tune_unroll != 2 && g->tune_unroll != 4 && g->tune_unroll != 8) g->tune_unroll = 4;
   }
   if (g->tune_reduce < 0) {
     const char *e = getenv("SUNHIP_REDUCE");
@@ -215,16 +226,19 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   Red R{g->d_part, 0, (g->tune_reduce == 0 && op != OP_NONE) ? cnt : nullptr, op};
   // the timing pair brackets the column kernel alone, not the finisher
   const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
-  switch (g->tune_unroll) {
-    case 1: k_qprec<UPDATE, 1, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
-    case 2: k_qprec<UPDATE, 2, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); break;
-    default:
-      if (g->tune_rzid && op != OP_NONE)
-        k_qprec<UPDATE, 4, T, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R);
-      else
-        k_qprec<UPDATE, 4, T><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R);
-      break;
+  const bool rzid = g->tune_rzid && op != OP_NONE;
+#define PREC(U) \
+  do { \
+    if (rzid) k_qprec<UPDATE, U, T, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
+    else k_qprec<UPDATE, U, T, false><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
+  } while (0)
+  switch (g->tune_punroll) {
+    case 1: PREC(1); break;
+    case 2: PREC(2); break;
+    case 4: PREC(4); break;
+    default: PREC(8); break;
   }
+#undef PREC
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
   if (op != OP_NONE && g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, sc, op);
@@ -244,6 +258,17 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
 #define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
 #define APPLYV(TPB, NTS) k_qapply<MODE, 3, 4, T, TO, TPB, NTS><<<nba, TPB * 64, 0, g->stream>>>(G, V, sc, R)
 #define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
+  if constexpr (MODE == QM_FUSED && std::is_same<T, TO>::value) {
+    if (variants && g->tune_blk && tpb == 4) {
+      if (g->tune_nts) k_qapply_blk<3, 4, T, true><<<nba, 256, 0, g->stream>>>(G, V, sc, R);
+      else k_qapply_blk<3, 4, T, false><<<nba, 256, 0, g->stream>>>(G, V, sc, R);
+      prof_end(g, ps);
+      HIPCHK(hipGetLastError());
+      if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nba, sc, R.op);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+  }
   if (variants && g->tune_minw == 3 && tpb == 4) {
     if (g->tune_nts) APPLYW(true, 3); else APPLYW(false, 3);
   } else if (variants && g->tune_minw == 4 && tpb == 4) {
@@ -253,7 +278,8 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
     else if (tpb == 16) { if (g->tune_nts) APPLYV(16, true); else APPLYV(16, false); }
     else APPLYV(4, true);
   } else if (g->mf <= 3) {
-    if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2); else APPLY(3, 4);
+    if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2);
+    else if (g->tune_unroll == 8) APPLY(3, 8); else APPLY(3, 4);
   } else {
     if (g->tune_unroll == 1) APPLY(4, 1); else if (g->tune_unroll == 2) APPLY(4, 2); else APPLY(4, 4);
   }
@@ -322,16 +348,15 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
 // once.
 template <typename T>
 int cg_loop(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int pk_apply, int pk_prec) {
-  int every = prm->check_every > 0 ? prm->check_every : (g->NT > (1 << 22) ? 4 : 16);
+  int every = prm->check_every > 0 ? prm->check_every : (g->NT > (1 << 22) ? 4 : 8);
   if (every & 1) every++;
-  int it = 0, rc;
-  HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
-  while (!g->h_sc->done && it < prm->maxiters) {
+  int it = 0, rc, slot = 0;
+  bool pending = false, done = false;
+  // iterations enqueued after convergence return at once (device flag)
+  while (!done && it < prm->maxiters) {
     for (int b = 0; b < every && it < prm->maxiters; b++, it++)
       if ((rc = cg_iteration<T>(g, prm, S, it, pk_apply, pk_prec))) return rc;
-    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
+    if ((rc = poll_chunk(g, S.sc, slot, pending, &done))) return rc;
   }
   k_qfinish<T><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
@@ -398,6 +423,8 @@ int q_stats(sunhip_grid *g, CGScal *d, sunhip_cg_stats *st) {
     st->device_ms = ms;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
     st->setup_ms = ms;
+    st->outer = 0;
+    st->inner = 0;
   }
   return 0;
 }
@@ -569,6 +596,8 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
   HIPCHK(hipStreamSynchronize(g->stream));
   if (st) {
     st->niters = outer;
+    st->outer = outer;
+    st->inner = total_inner;
     st->converged = converged;
     st->resid = eps0 > 0 ? sqrt(eps / eps0) : 0;
     st->eps0 = eps0;
@@ -627,6 +656,10 @@ int sunhip_q_precondition(sunhip_grid *g, const char *x, const char *xc) {
 // built-in).
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   if (!strcmp(key, "unroll")) g->tune_unroll = value;
+  else if (!strcmp(key, "prec_unroll")) {
+    if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("prec_unroll must be 1, 2, 4 or 8"); return -1; }
+    g->tune_punroll = value;
+  }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {
     if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }
@@ -634,6 +667,7 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   } else if (!strcmp(key, "ntstore")) g->tune_nts = value != 0;
   else if (!strcmp(key, "rzid")) g->tune_rzid = value != 0;
   else if (!strcmp(key, "minwaves")) g->tune_minw = value;
+  else if (!strcmp(key, "blocklds")) g->tune_blk = value != 0;
   else { set_error("unknown tuning key %s", key); return -1; }
   return 0;
 }

@@ -240,6 +240,148 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
   return dot;
 }
 
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// The FUSED mode of qapply_tile for the four consecutive tiles t0..t0+3 of
+// one 256-thread block, walking the levels together.  Each column forms
+// p = rtmp + mu*p_old once per level and stages it in LDS (double-buffered,
+// one barrier per group of U levels); a face whose neighbour column lies in
+// the block reads its p there, and only neighbours outside the block are
+// gathered from memory.  Every value is formed by the same operations in the
+// same order as in qapply_tile, so the results are bit-identical.  Must be
+// called by all 256 threads of the block.
+template <int NF, int U, typename T, bool NTS>
+__device__ __forceinline__ double qapply_block(const Geo &G, const QVec<T, T> &V, T mu, T nu, int t0) {
+  constexpr int BT = 4;
+  __shared__ T pl[2][U][BT * 64];
+  __shared__ int bnd[2][BT];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int t = t0 + w;
+  const int s = t * 64 + l;
+  int n = 0, kt = 0;
+  bool act = false;
+  if (t < G.Nt) {
+    n = G.nk[s];
+    act = n > 0 && G.comp[s] == 1;
+    kt = act ? G.ctop[s] : 0;
+  }
+  {
+    const int lo = wave_min_i(act ? kt : 0x7fffffff), hi = wave_max_i(act ? n : 0);
+    if (l == 0) { bnd[0][w] = lo; bnd[1][w] = hi; }
+  }
+  __syncthreads();
+  const int kb0 = min(min(bnd[0][0], bnd[0][1]), min(bnd[0][2], bnd[0][3]));
+  const int kb1 = max(max(bnd[1][0], bnd[1][1]), max(bnd[1][2], bnd[1][3]));
+  if (kb1 <= kb0) return 0;  // no computational column in the block (uniform)
+  const T *__restrict__ coef = V.coef;
+  const T *__restrict__ dzf = V.dzf;
+  const T *__restrict__ rt = V.rtmp;
+  const T *__restrict__ po = V.pold;
+  const int NS = G.NS;
+  const uint32_t base = act ? (uint32_t)(G.toff[t] + l) : 0u;
+  uint32_t nbb[NF], ebb[NF];
+  int kmn[NF], kmx[NF], lim[NF], loc[NF];
+  T Df[NF];
+  const int nf = act ? G.nf[s] : 0;
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
+    if (nbf >= 0) {
+      nbb[f] = (uint32_t)nbf;
+      ebb[f] = (uint32_t)G.eb[f * NS + s];
+      const int kr = G.kr[f * NS + s];
+      kmn[f] = kr & 0xffff;
+      kmx[f] = kr >> 16;
+      Df[f] = V.D[G.es[f * NS + s]];
+      const int rel = G.nbs[f * NS + s] - t0 * 64;
+      loc[f] = (rel >= 0 && rel < BT * 64) ? rel : -1;
+    } else {
+      nbb[f] = base; ebb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0; loc[f] = -1;
+    }
+    lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+  }
+  double dot = 0;
+  T vprev = 0;
+  int buf = 0;
+  for (int kb = kb0; kb < kb1; kb += U, buf ^= 1) {
+    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+    if (act) {
+#pragma unroll
+      for (int j = 0; j <= U; j++) {
+        const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
+        pv[j] = at32(po, idx);
+        v[j] = at32(rt, idx);
+        c[j] = at32(coef, idx);
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) xv[j] = at32((const T *)V.x, base + (uint32_t)min(kb + j, n) * 64u);
+#pragma unroll
+      for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j <= U; j++) { pv[j] = 0; v[j] = 0; c[j] = 0; }
+#pragma unroll
+      for (int j = 0; j < U; j++) xv[j] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) pl[buf][j][threadIdx.x] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const uint32_t kk = (uint32_t)min(k, lim[f]) * 64u;
+        if (loc[f] >= 0) {
+          nv[j][f] = pl[buf][j][loc[f]];
+        } else if (f < nf) {
+          const uint32_t ni = nbb[f] + kk;
+          nv[j][f] = at32(rt, ni) + mu * at32(po, ni);
+        } else {
+          nv[j][f] = 0;
+        }
+        dz[j][f] = f < nf ? at32(dzf, ebb[f] + kk) : T(0);
+      }
+    }
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = kb + j;
+        if (k >= kt && k < n) {
+          const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
+          const T c0 = c[j], cp1 = c[j + 1];
+          T y = 0;
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+            y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+          }
+          if (kt == n - 1) y -= T(2.0) * c0 * v0;
+          else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
+          else if (k == n - 1) y += c0 * vm1 - c0 * v0;
+          else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          st_stream<NTS>(&at32(V.pnew, idx), v0);
+          st_stream<NTS>(&at32(V.x, idx), xv[j] + nu * pv[j]);
+          dot += (double)v0 * (double)y;
+          st_stream<NTS>(&at32(V.out, idx), y);
+        }
+      }
+    }
+    vprev = v[U - 1];
+  }
+  return dot;
+}
+
 // Vertical-line preconditioner of tile t: (UPDATE) r -= nu*z, then
 // out = M^-1 r by Thomas down the column (b' recomputed, d' stored in out)
 // and back up it (b' from bq); rz and rr collect the lane's r.out and r.r.

@@ -36,7 +36,7 @@ class CGParams(C.Structure):
 
 class CGStats(C.Structure):
     _fields_ = [("niters", C.c_int), ("converged", C.c_int), ("resid", C.c_double), ("eps0", C.c_double),
-                ("device_ms", C.c_double), ("setup_ms", C.c_double)]
+                ("device_ms", C.c_double), ("setup_ms", C.c_double), ("outer", C.c_int), ("inner", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
         L.sunhip_batch_detach.argtypes = [vp, vp]
         L.sunhip_batch_begin_q.argtypes = [vp, vp, C.POINTER(CGParams), C.c_int64]
         L.sunhip_batch_begin_h.argtypes = [vp, vp, C.POINTER(CGParams), C.POINTER(HPhys), C.c_int64]
+        L.sunhip_batch_begin_q_mixed.argtypes = [vp, vp, C.POINTER(CGParams), C.c_double, C.c_int, C.c_int64]
         L.sunhip_batch_advance.argtypes = [vp, C.c_int]
         L.sunhip_batch_poll.argtypes = [vp, C.POINTER(C.c_int64), _ip, C.POINTER(CGStats), C.c_int]
         L.sunhip_batch_active.argtypes = [vp]
@@ -254,7 +255,7 @@ class Batch:
     the batch's stream and the PCG iterations of all active solves share the
     same launches."""
 
-    KIND = {0: "q", 1: "h", 2: "q32"}
+    KIND = {0: "q", 1: "h", 2: "q32", 3: "qmix"}
 
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
@@ -285,6 +286,11 @@ class Batch:
     def begin_q(self, dg, tag, epsilon=1e-10, maxiters=2000, resnorm=0):
         P = CGParams(maxiters, 2, resnorm, epsilon, 0)
         check(lib().sunhip_batch_begin_q(self.h, dg.h, C.byref(P), tag), "sunhip_batch_begin_q")
+
+    def begin_q_mixed(self, dg, tag, epsilon=1e-10, inner_tol=1e-4, max_outer=20, maxiters=2000):
+        P = CGParams(maxiters, 2, 0, epsilon, 0)
+        check(lib().sunhip_batch_begin_q_mixed(self.h, dg.h, C.byref(P), inner_tol, max_outer, tag),
+              "sunhip_batch_begin_q_mixed")
 
     def begin_h(self, dg, tag, grav, theta, dt, epsilon=1e-10, maxiters=1000, precond=1, resnorm=0):
         P = CGParams(maxiters, precond, resnorm, epsilon, 0)

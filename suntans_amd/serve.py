@@ -4,7 +4,8 @@ Each request asks one simulation (an ensemble member, or a large grid) for
 one time step of the path this package accelerates:
 
     free-surface PCG (CGSolve)  ->  nonhydrostatic pressure PCG (CGSolveQ,
-    warm-started from the simulation's previous q, kept on the device)
+    warm-started from the simulation's previous q, kept on the device; FP64,
+    or the mixed-precision form to the same 1e-10 FP64 residual)
     ->  one TVD scalar step (UpdateScalars)
 
 A step's forcing is built on the device from the simulation's resident base
@@ -111,7 +112,9 @@ class Server:
     """Continuous-batching loop over a set of simulations on one GPU."""
 
     def __init__(self, sims, hbm_budget: float = 250e9, max_slots: int = 128, device_id: int = 0,
-                 epsilon: float = 1e-10, tvd: int = 3):
+                 epsilon: float = 1e-10, tvd: int = 3, mixed: bool = False, inner_tol: float = 1e-4):
+        self.mixed = mixed
+        self.inner_tol = inner_tol
         self.sims = {s.sid: s for s in sims}
         self.budget = hbm_budget
         self.max_slots = max_slots
@@ -172,7 +175,10 @@ class Server:
         dg = s.dg
         dg.copy("D", "Dq")
         dg.scale("src", "src0", forcing(s.sid, r.step))
-        self.batch.begin_q(dg, tag, epsilon=self.eps)
+        if self.mixed:
+            self.batch.begin_q_mixed(dg, tag, epsilon=self.eps, inner_tol=self.inner_tol)
+        else:
+            self.batch.begin_q(dg, tag, epsilon=self.eps)
 
     def _scalar_step(self, r: Request):
         s = self.sims[r.sid]

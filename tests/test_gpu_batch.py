@@ -196,3 +196,52 @@ def test_batch_update_scalars_bitwise(gpu_available, states):
         assert np.array_equal(s[m], a.get("s")[m])
         dg.close()
     b.close()
+
+
+def _solve_mixed(states, idx):
+    b = device.Batch()
+    dgs = grids_for([states[i] for i in idx])
+    stats = b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=1e-4)
+    out = [(dg.download("q", dg.sizes()["Nflat"]), s["inner_iters"], s["niters"]) for dg, s in zip(dgs, stats)]
+    for dg in dgs:
+        dg.close()
+    b.close()
+    return out
+
+
+def test_batch_mixed_alone_equals_mix(gpu_available, states):
+    """Each member refines on its own schedule, so its result does not depend
+    on the others: bit-identical alone, in a mix and in reverse order."""
+    alone = [_solve_mixed(states, [i])[0] for i in range(len(states))]
+    mix = _solve_mixed(states, list(range(len(states))))
+    rev = _solve_mixed(states, list(range(len(states)))[::-1])[::-1]
+    for a, m, r in zip(alone, mix, rev):
+        assert a[1:] == m[1:] == r[1:]
+        assert np.array_equal(a[0], m[0]) and np.array_equal(a[0], r[0])
+
+
+def test_batch_mixed_joins_fp64_batch(gpu_available, states):
+    """A mixed-precision solve admitted into a batch that is running FP64 and
+    free-surface solves ends bit-identical to the same mixed solve alone."""
+    alone = _solve_mixed(states, [6])[0]
+    b = device.Batch()
+    dg_f64 = grids_for([states[5]])[0]
+    sth = states[4].clone()
+    sth.set("D", cases.depth_integrated_D(sth))
+    dg_h = grids_for([sth], ("h", "htmp", "D"))[0]
+    dg_m = grids_for([states[6]])[0]
+    b.begin_q(dg_f64, 0)
+    b.begin_h(dg_h, 1, sth.get_prop("grav"), sth.get_prop("theta"), sth.get_prop("dt"))
+    b.advance(4)
+    done = {t: s for t, k, s in b.poll()}
+    b.begin_q_mixed(dg_m, 2)
+    while b.active():
+        b.advance(4)
+        for t, k, s in b.poll():
+            done[t] = s
+    q = dg_m.download("q", dg_m.sizes()["Nflat"])
+    assert done[2]["inner"] == alone[1] and done[2]["outer"] == alone[2]
+    assert np.array_equal(q, alone[0])
+    for dg in (dg_f64, dg_h, dg_m):
+        dg.close()
+    b.close()

@@ -102,6 +102,18 @@ def test_alone_equals_mix(gpu_available):
             assert np.array_equal(x, y)
 
 
+def test_alone_equals_mix_mixed_precision(gpu_available):
+    sts = states()
+    order = [i % len(sts) for i in range(2 * len(sts))]
+    arrivals = [(0.0, sid) for sid in order]
+    alone, it_a, _ = run(sts, arrivals, max_slots=1, mixed=True)
+    mix, it_m, info = run(sts, arrivals, max_slots=64, mixed=True)
+    assert it_a == it_m
+    for sid in alone:
+        for x, y in zip(alone[sid], mix[sid]):
+            assert np.array_equal(x, y)
+
+
 def test_admission_by_hbm_budget(gpu_available):
     sts = states()
     arrivals = serve.poisson_arrivals(400.0, 24, list(range(len(sts))), [1] * len(sts), seed=3)

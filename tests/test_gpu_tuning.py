@@ -1,0 +1,52 @@
+"""Every kernel tuning variant (sunhip_set_tuning) solves the same problem.
+Unroll, store-hint, register-budget and block-staging variants are
+bit-identical to the default configuration (same operations in the same
+order, same reduction order).  Variants that regroup a dot product only
+change its rounding: tiles per block (block partial sums of 8 or 16 tiles)
+and rzid (r.M^-1 r as sum d'^2/b'); for those the iteration count is within
++-1 and q within 1e-12."""
+import numpy as np
+import pytest
+
+from suntans_amd import cases, device
+
+pytestmark = pytest.mark.gpu
+
+BITWISE = [("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1), ("prec_unroll", 4), ("ntstore", 0),
+           ("minwaves", 3), ("minwaves", 4), ("blocklds", 1)]
+ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0)]
+
+
+def solve(st, tuning):
+    dg = device.DeviceGrid(st)
+    device.member_upload(dg, st)
+    for k, v in tuning:
+        dg.tune(k, v)
+    s = dg.q_solve(epsilon=1e-10)
+    q = dg.download("q", dg.sizes()["Nflat"])
+    dg.close()
+    return q, s["niters"]
+
+
+@pytest.fixture(scope="module")
+def grids():
+    return [cases.tiny(11, nx=23, ny=9, Nkmax=13), cases.lock_exchange(seed=7),
+            cases.ensemble_member(6, 9000, 1.2, 40)]
+
+
+@pytest.mark.parametrize("key,value", BITWISE, ids=[f"{k}={v}" for k, v in BITWISE])
+def test_variant_bitwise(gpu_available, grids, key, value):
+    for st in grids:
+        q0, n0 = solve(st, [])
+        q1, n1 = solve(st, [(key, value)])
+        assert n0 == n1
+        assert np.array_equal(q0, q1)
+
+
+@pytest.mark.parametrize("key,value", ROUNDING, ids=[f"{k}={v}" for k, v in ROUNDING])
+def test_variant_rounding(gpu_available, grids, key, value):
+    for st in grids:
+        q0, n0 = solve(st, [])
+        q1, n1 = solve(st, [(key, value)])
+        assert abs(n0 - n1) <= 1
+        assert np.linalg.norm(q1 - q0) <= 1e-12 * np.linalg.norm(q0)
