@@ -250,26 +250,30 @@ def ensemble_leg(args, sims):
 def serving_leg(args, sims, large_sim):
     """Poisson arrivals of time-step requests (free-surface solve +
     warm-started pressure solve + TVD step) over the ensemble members, 10% of
-    them for the large grid; admission by a 250 GB HBM budget."""
+    them for the large grid; admission by a 250 GB HBM budget.  Swept once
+    with FP64 pressure solves and once with the mixed-precision form (same
+    1e-10 FP64 residual)."""
     from suntans_amd import serve
     allsims = sims + [large_sim]
     sids = [s.sid for s in allsims]
     weights = [0.9 / len(sims)] * len(sims) + [0.1]
-    res = []
-    for rate in args.serve_rates:
-        arr = serve.poisson_arrivals(rate, args.serve_requests, sids, weights, seed=int(rate))
-        srv = serve.Server(allsims, hbm_budget=250e9, max_slots=128)
-        done = srv.run(arr)
-        srv.close()
-        row = {"rate_per_s": rate, **serve.summarize(done)}
-        log(f"serving {row}")
-        res.append(row)
-    return {
+    out = {
         "workload": (f"{len(sims)} ensemble members + 1 grid of the large solve (10% of requests); each request = "
-                     "CGSolve + CGSolveQ (q warm start on device) + TVD superbee step, 1e-10, FP64"),
+                     "CGSolve + CGSolveQ (q warm start on device) + TVD superbee step, 1e-10"),
         "hbm_budget_GB": 250, "max_slots": 128, "requests_per_rate": args.serve_requests,
-        "rates": res,
     }
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        res = []
+        for rate in args.serve_rates:
+            arr = serve.poisson_arrivals(rate, args.serve_requests, sids, weights, seed=int(rate))
+            srv = serve.Server(allsims, hbm_budget=250e9, max_slots=128, mixed=mixed)
+            done = srv.run(arr)
+            srv.close()
+            row = {"rate_per_s": rate, **serve.summarize(done)}
+            log(f"serving {name} {row}")
+            res.append(row)
+        out["rates_" + name] = res
+    return out
 
 
 def main():

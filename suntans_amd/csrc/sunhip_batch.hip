@@ -184,6 +184,42 @@ __global__ void kb_gather(CGScal *const *__restrict__ src, CGScal *__restrict__ 
   if (i < n) dst[i] = *src[i];
 }
 
+// Scalar steps of many grids: the three kernels of sunhip_update_scalars,
+// each over a work list of (member, cell tile) or (member, edge tile).
+struct BTMember {
+  Geo G;
+  TvdArgs T;
+  ColArgs C;
+};
+
+__global__ __launch_bounds__(256) void kb_tvd_sums(const BTMember *__restrict__ M, const int2 *__restrict__ W, int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    if (B.T.TVD) tvd_sums_tile(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_tvd_faces(const BTMember *__restrict__ M, const int2 *__restrict__ W, int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    if (B.T.TVD) tvd_faces_tile(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restrict__ M, const int2 *__restrict__ W,
+                                                        int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    update_column_tile(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -288,6 +324,8 @@ struct sunhip_batch {
   DevBuf<double> pres;
   int nres = 0, nwres = 0;  // members and items of the residual table
   // polling
+  DevBuf<BTMember> ttvd;
+  DevBuf<int2> wtc, wte;
   DevBuf<CGScal *> d_scp;
   DevBuf<CGScal> d_scv;
   HostBuf<CGScal> h_scv;
@@ -598,6 +636,7 @@ int sunhip_batch_destroy(sunhip_batch *b) {
   b->tq64.release(); b->tq32.release(); b->th64.release();
   for (int k = 0; k < T_NKIND; k++) { b->rng[k].release(); b->work[k].release(); b->part[k].release(); }
   b->tres.release(); b->rres.release(); b->wres.release(); b->pres.release();
+  b->ttvd.release(); b->wtc.release(); b->wte.release();
   b->d_scp.release(); b->d_scv.release(); b->h_scv.release(); b->h_stage.release(); b->h_tab.release();
   hipEventDestroy(b->ev0);
   hipEventDestroy(b->ev1);
@@ -949,11 +988,42 @@ int sunhip_batch_q_solve_mixed(sunhip_batch *b, sunhip_grid **gs, int n, const s
 
 int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **gs, int n, const sunhip_scalar_params *sp,
                                 const char *scal) {
+  HIPCHK(hipSetDevice(b->device));
   int rc;
+  std::vector<BTMember> tab(n);
+  std::vector<int2> wc, we;
   for (int i = 0; i < n; i++) {
-    if ((rc = sunhip_batch_attach(b, gs[i]))) return rc;
-    if ((rc = sunhip_update_scalars(gs[i], sp + i, scal))) return rc;
+    sunhip_grid *g = gs[i];
+    if ((rc = sunhip_batch_attach(b, g))) return rc;
+    tab[i].G = g->geo();
+    if ((rc = tvd_args(g, sp + i, scal, &tab[i].T, &tab[i].C))) return rc;
+    for (int t = 0; t < g->Nt; t++) wc.push_back(make_int2(i, t));
+    for (int t = 0; t < g->Net; t++) we.push_back(make_int2(i, t));
   }
+  if (n == 0) return 0;
+  // table staging is reused: wait for its previous uploads
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const size_t bytes = tab.size() * sizeof(BTMember) + (wc.size() + we.size()) * sizeof(int2) + 1024;
+  if ((rc = b->h_tab.ensure(bytes)) || (rc = b->ttvd.ensure(tab.size())) || (rc = b->wtc.ensure(wc.size() + 1)) ||
+      (rc = b->wte.ensure(we.size() + 1)))
+    return rc;
+  size_t off = 0;
+  rc = table_upload(b, b->ttvd.d, tab.data(), tab.size() * sizeof(BTMember), off) |
+       table_upload(b, b->wtc.d, wc.data(), wc.size() * sizeof(int2), off) |
+       table_upload(b, b->wte.d, we.data(), we.size() * sizeof(int2), off);
+  if (rc) return rc;
+  const int nc = (int)wc.size(), ne = (int)we.size();
+  // face values from the current scalar, then the column update from its
+  // copy (the order of sunhip_update_scalars)
+  kb_tvd_sums<<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+  if (ne) kb_tvd_faces<<<(ne + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wte.d, ne);
+  HIPCHK(hipGetLastError());
+  for (int i = 0; i < n; i++)
+    HIPCHK(hipMemcpyAsync(const_cast<double *>(tab[i].C.s_old), tab[i].C.s, gs[i]->fields.at(scal).n * sizeof(double),
+                          hipMemcpyDeviceToDevice, b->stream));
+  kb_update_column<<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include "sunhip_qtile.h"
 #include "sunhip_htile.h"
+#include "sunhip_tvdtile.h"
 
 namespace sunhip {
 
@@ -50,5 +51,8 @@ int q_mix_update(sunhip_grid *g, double *x, const float *e);
 int h_begin64(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
               HSolver<double> *S, CGScal *stage);
 int h_end64(sunhip_grid *g, const HSolver<double> &S);
+
+// Arguments of one grid's scalar step (fields created on demand); no launch.
+int tvd_args(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, TvdArgs *T, ColArgs *C);
 
 }  // namespace sunhip

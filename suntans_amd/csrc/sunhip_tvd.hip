@@ -17,383 +17,61 @@
 
 #include <cmath>
 
-#include "sunhip_qtile.h"
+#include "sunhip_tvdtile.h"
+#include "sunhip_solvers.h"
 
 using namespace sunhip;
 
-#define TVD_EPS 1e-12
 
-__device__ __forceinline__ double dmax_ref(double x1, double x2) { return x2 > x1 ? x2 : x1; }
-__device__ __forceinline__ double dmin_ref(double x, double y) { return x < y ? x : y; }
-
-__device__ __forceinline__ double psi(double r, int TVD) {
-  if (TVD == 2) return 1;
-  if (TVD == 3) return dmax_ref(0, dmax_ref(dmin_ref(2 * (r), 1), dmin_ref(r, 2)));
-  if (TVD == 4) return r > 0 ? dmin_ref(r, 1) : 0;
-  return 0;
-}
-
-struct TvdArgs {
-  const double *s;       // scalar (cell field), old values
-  const double *u, *utmp2, *dzf;   // edge fields
-  const double *sumQ, *sumQC;      // cell fields (pass 1 output)
-  double *oQ, *oQC;
-  double *SfHp, *SfHm;             // edge fields
-  const double *bscal;             // edge field, type-2 edges (may be null)
-  double theta;
-  int TVD;
-};
-
-// pass 1 (main/tvd.c:46-65), cells of celldist[0..2).  The faces' level-
-// invariant data (edge base, df, normal, neighbour base and depth) are read
-// once per column.
 __global__ __launch_bounds__(256) void k_tvd_sums(Geo G, TvdArgs A) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, n = G.nk[s];
-  if (n == 0 || G.comp[s] == 0) return;
-  const int NS = G.NS, nf = G.nf[s];
-  const uint32_t base = (uint32_t)(G.toff[t] + l);
-  const double th = A.theta;
-  uint32_t eb[SUNHIP_MAXF], nbb[SUNHIP_MAXF];
-  double dfe[SUNHIP_MAXF];
-  int nrm[SUNHIP_MAXF], nkn[SUNHIP_MAXF];
-#pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
-    if (f < nf) {
-      const int nbs = G.nbs[f * NS + s];
-      eb[f] = (uint32_t)G.eb[f * NS + s];
-      dfe[f] = G.df[G.es[f * NS + s]];
-      nrm[f] = G.nrm[f * NS + s];
-      nkn[f] = nbs >= 0 ? G.nk[nbs] : 0;  // no neighbour: never below its bottom
-      nbb[f] = nbs >= 0 ? (uint32_t)G.nbb[f * NS + s] : base;
-    } else {
-      eb[f] = 0; dfe[f] = 0; nrm[f] = 0; nkn[f] = 0; nbb[f] = base;
-    }
-  }
-  for (int k = G.ctop[s]; k < n; k++) {
-    const uint32_t k64 = (uint32_t)k * 64u, idx = base + k64;
-    const double si = at32(A.s, idx);
-    double sq = 0, sqc = 0;
-#pragma unroll
-    for (int f = 0; f < SUNHIP_MAXF; f++) {
-      if (f < nf) {
-        const uint32_t ei = eb[f] + k64;
-        const double un = nrm[f] * (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei));
-        const double Qm = 0.5 * at32(A.dzf, ei) * dfe[f] * fabs(un - fabs(un));
-        if (k < nkn[f]) sqc += Qm * (si - at32(A.s, nbb[f] + k64));
-        sq += Qm;
-      }
-    }
-    at32(A.oQ, idx) = sq;
-    at32(A.oQC, idx) = sqc;
-  }
+  tvd_sums_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-// pass 2 (main/tvd.c:69-119), one lane per edge
 __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Net) return;
-  const int s = t * 64 + l, ec = G.e_comp[s];
-  if (ec == 0) return;
-  const int64_t base = G.eoff[t] + l;
-  const int et = G.e_etop[s], nke = G.e_nke[s];
-  for (int k = 0; k < et; k++) A.SfHp[base + (int64_t)k * 64] = A.SfHm[base + (int64_t)k * 64] = 0;
-  const double th = A.theta;
-  if (ec == 1) {
-    const int64_t c1 = G.e_c1[s], c2 = G.e_c2[s];
-    for (int k = et; k < nke; k++) {
-      const int64_t ei = base + (int64_t)k * 64;
-      const double un = th * A.u[ei] + (1 - th) * A.utmp2[ei];
-      const double s1 = A.s[c1 + (int64_t)k * 64], s2 = A.s[c2 + (int64_t)k * 64];
-      const int64_t up = (un > 0 ? c2 : c1) + (int64_t)k * 64;
-      const double si = un > 0 ? s2 : s1, sm = un > 0 ? s1 : s2;
-      double r = 0;
-      const double q = A.sumQ[up];
-      if (q != 0 && sm != si) r = A.sumQC[up] / (q * (sm - si));
-      A.SfHp[ei] = s2 + 0.5 * psi(r, A.TVD) * (s1 - s2);
-      A.SfHm[ei] = s1 - 0.5 * psi(r, A.TVD) * (s1 - s2);
-    }
-  } else {
-    const int64_t ib = G.e_c1[s];
-    for (int k = et; k < nke; k++) {
-      const int64_t ei = base + (int64_t)k * 64;
-      A.SfHp[ei] = A.bscal ? A.bscal[ei] : 0.0;
-      A.SfHm[ei] = A.s[ib + (int64_t)k * 64];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// UpdateScalars column kernel
-
-struct ColArgs {
-  const double *s_old;                   // stmp
-  double *s;                             // scal (out)
-  const double *wnew, *wtmp2;            // w-type
-  const double *dzz, *dzzold, *kappa_tv;
-  const double *u, *utmp2, *dzf, *SfHp, *SfHm;
-  const double *active;                  // 2-D cell field (0/1)
-  double *Cn;
-  double *tb, *tc, *td;                  // Thomas scratch (cell fields)
-  double theta, dt, kappa;
-  int TVD, step, alpha_top, alpha_bot;
-};
-
-// One column's view: levels are absolute k; reads outside [0,Nk) give 0, as
-// in the oracle (the reference reads outside the row there).
-struct Col {
-  const double *p;
-  int64_t base;
-  int nk;
-  __device__ double at(int k) const { return (k >= 0 && k < nk) ? p[base + (int64_t)k * 64] : 0.0; }
-};
-
-// ap[k], am[k] of GetApAm (main/tvd.c:161-204) for one k in [ktop+1, Nk]:
-// the final values of rp/rm after the reference's sequence of assignments.
-__device__ void apam(const double *w, const Col &dz, const Col &sc, int64_t base, int nk, int ktop, int k,
-                     double dt, int TVD, double &ap, double &am) {
-  const double wk = w[base + (int64_t)k * 64];
-  const double wp = 0.5 * (wk + fabs(wk)), wm = 0.5 * (wk - fabs(wk));
-  double Cp, Cm;
-  if (k < nk) {
-    const double den = dz.at(k) + dz.at(k - 1);
-    Cp = 2 * wp * dt / den;
-    Cm = 2 * wm * dt / den;
-  } else {
-    Cp = wp * dt / dz.at(nk - 1);
-    Cm = wm * dt / dz.at(nk - 1);
-  }
-  double rp, rm;
-  if (k == nk) {
-    rp = 1;
-    rm = (sc.at(nk - 2) - sc.at(nk - 1) + TVD_EPS) / TVD_EPS;
-  } else if (k == nk - 1) {
-    rp = TVD_EPS / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-    rm = (sc.at(k - 2) - sc.at(k - 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-  } else if (k == ktop + 1) {
-    rp = (sc.at(ktop + 1) - sc.at(ktop + 2) + TVD_EPS) / (sc.at(ktop) - sc.at(ktop + 1) + TVD_EPS);
-    rm = TVD_EPS / (sc.at(ktop) - sc.at(ktop + 1) + TVD_EPS);
-  } else {
-    rp = (sc.at(k) - sc.at(k + 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-    rm = (sc.at(k - 2) - sc.at(k - 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-  }
-  am = 0.5 * wp * psi(rp, TVD) * (1 - Cp) + wm * (1 - 0.5 * psi(rm, TVD) * (1 + Cm));
-  ap = wp * (1 - 0.5 * psi(rp, TVD) * (1 - Cp)) + 0.5 * wm * psi(rm, TVD) * (1 + Cm);
-}
-
-__device__ __forceinline__ void apam_plain(const double *w, int64_t base, int k, double &ap, double &am) {
-  const double wk = w[base + (int64_t)k * 64];
-  ap = 0.5 * (wk + fabs(wk));
-  am = 0.5 * (wk - fabs(wk));
-}
-
-__device__ __forceinline__ double upwind_ref(double u, double a, double b) {
-  if (fabs(u) > 0) return u > 0 ? b : a;
-  return dmax_ref(a, b);
-}
-
-// A face's level-invariant part of the horizontal flux of main/scalars.c:
-// 251-290: the edge's layer count and base, the upwind cell bases, and
-// cf = dt*df*normal/Ac, the leading factors of the flux in the reference's
-// order (so the product below rounds as the reference's does).
-struct FaceC {
-  uint32_t eb;
-  int nke;
-  int64_t c1, c2;
-  double cf;
-};
-
-__device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int s, double Ac, FaceC (&fc)[SUNHIP_MAXF]) {
-  const int NS = G.NS, nf = G.nf[s];
-#pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
-    if (f < nf) {
-      const int es = G.es[f * NS + s];
-      fc[f].eb = (uint32_t)G.eb[f * NS + s];
-      fc[f].nke = G.e_nke[es];
-      int64_t c1 = G.e_c1[es], c2 = G.e_c2[es];
-      if (c1 < 0) c1 = c2;
-      if (c2 < 0) c2 = c1;
-      fc[f].c1 = c1;
-      fc[f].c2 = c2;
-      fc[f].cf = A.dt * G.df[es] * G.nrm[f * NS + s] / Ac;
-    } else {
-      fc[f].eb = 0; fc[f].nke = 0; fc[f].c1 = 0; fc[f].c2 = 0; fc[f].cf = 0;
-    }
-  }
-}
-
-// horizontal TVD flux divergence at level k (main/scalars.c:251-290)
-__device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[SUNHIP_MAXF], int nf, int k) {
-  double acc = 0;
-  const uint32_t k64 = (uint32_t)k * 64u;
-#pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
-    if (f >= nf || k >= fc[f].nke) continue;
-    const uint32_t ei = fc[f].eb + k64;
-    const double ut2 = at32(A.utmp2, ei);
-    double temp;
-    if (A.TVD) {
-      temp = ut2 > 0 ? at32(A.SfHp, ei) : at32(A.SfHm, ei);
-    } else {
-      temp = upwind_ref(ut2, A.s_old[fc[f].c1 + (int64_t)k * 64], A.s_old[fc[f].c2 + (int64_t)k * 64]);
-    }
-    acc += fc[f].cf * (A.theta * at32(A.u, ei) + (1 - A.theta) * ut2) * temp * at32(A.dzf, ei);
-  }
-  return acc;
+  tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (t >= G.Nt) return;
-  const int s = t * 64 + l, nk = G.nk[s];
-  if (nk == 0 || G.comp[s] != 1) return;
-  const int ct = G.ctop[s], cto = G.ctopold[s];
-  const int64_t base = G.toff[t] + l;
-  const double Ac = G.Ac[s], theta = A.theta, dt = A.dt;
-  const Col dzz{A.dzz, base, nk}, dzo{A.dzzold, base, nk}, so{A.s_old, base, nk};
-  FaceC fc[SUNHIP_MAXF];
-  face_consts(G, A, s, Ac, fc);
-  const int nfs = G.nf[s];
-  int ktop;
-  double dznew;
-  if (ct >= cto) {
-    ktop = ct;
-    dznew = dzz.at(ktop);
-  } else {
-    ktop = cto;
-    dznew = 0;
-    for (int k = ct; k <= cto; k++) dznew += dzz.at(k);
-  }
-  const int nb = nk - 1 - ktop;
-  const bool tvdv = A.TVD != 0;
-  auto ap_am_new = [&](int k, double &ap, double &am) {
-    if (tvdv) apam(A.wnew, dzz, so, base, nk, ktop, k, dt, A.TVD, ap, am);
-    else apam_plain(A.wnew, base, k, ap, am);
-  };
-  auto ap_am_old = [&](int k, double &ap, double &am) {
-    if (tvdv) apam(A.wtmp2, dzo, so, base, nk, ktop, k, dt, A.TVD, ap, am);
-    else apam_plain(A.wtmp2, base, k, ap, am);
-  };
-  auto bd = [&](int k) {
-    return (2.0 * A.kappa + A.kappa_tv[base + (int64_t)(k - 1) * 64] + A.kappa_tv[base + (int64_t)k * 64]) /
-           (dzz.at(k - 1) + dzz.at(k));
-  };
-  const double td = theta * dt, otd = (1 - theta) * dt;
-  // horizontal source of the top row: levels 0..ktop fold into row 0
-  double cn0 = 0;
-  for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux(A, fc, nfs, k);
-
-  // forward: assemble row m (level k = ktop+m) and eliminate
-  double bprev = 1, cprev = 0, dprev = 0;
-  // new-time advection at the top face of the next level, carried
-  double apN, amN, apO, amO;  // at k+1 (new and old velocities)
-  ap_am_new(ktop + 1, apN, amN);
-  ap_am_old(ktop + 1, apO, amO);
-  double bd_k1 = (ktop + 1 < nk) ? bd(ktop + 1) : 0.0;  // bd at k+1
-  double apN_k = 0, amN_k = 0, apO_k = 0, amO_k = 0, bd_k = 0;
-  for (int m = 0; m <= nb; m++) {
-    const int k = ktop + m;
-    // shift the level-(k+1) quantities to level k, compute level k+1
-    if (m > 0) {
-      apN_k = apN; amN_k = amN; apO_k = apO; amO_k = amO; bd_k = bd_k1;
-      ap_am_new(k + 1, apN, amN);
-      ap_am_old(k + 1, apO, amO);
-      bd_k1 = (k + 1 < nk) ? bd(k + 1) : 0.0;
-    }
-    double a, b, c, d;
-    const double sk = so.at(k);
-    if (m == 0) {
-      a = 0;
-      b = dznew - td * amN;
-      c = -td * apN;
-      if (nb == 0) b += c;
-      if (ktop < nk - 1) {
-        b += td * (bd_k1 + 2 * A.alpha_top * bd_k1);
-        c -= td * bd_k1;
-      }
-      if (cto <= ct) {
-        d = 0;
-        for (int kk = cto; kk <= ct; kk++) d += dzo.at(kk) * so.at(kk);
-      } else {
-        d = dzo.at(ktop) * so.at(ktop);
-      }
-      if (ktop < nk - 1) {
-        d = d - otd * (-amO * sk - apO * so.at(k + 1)) +
-            otd * (-(2 * A.alpha_top * bd_k1 + bd_k1) * sk + bd_k1 * so.at(k + 1));
-      }
-      d += cn0;
-    } else {
-      a = td * amN_k;
-      b = dzz.at(k) + td * (apN_k - amN);
-      c = -td * apN;
-      d = dzo.at(k) * sk;
-      if (m == nb) {
-        b += c;
-        a -= td * bd_k;
-        b += td * (bd_k + 2 * A.alpha_bot * bd_k);
-        d -= otd * (amO_k * so.at(k - 1) + apO_k * sk) -
-             otd * (bd_k * so.at(k - 1) - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
-      } else {
-        a -= td * bd_k;
-        b += td * (bd_k + bd_k1);
-        c -= td * bd_k1;
-        d -= otd * (amO_k * so.at(k - 1) + (apO_k - amO) * sk - apO * so.at(k + 1)) -
-             otd * (bd_k * so.at(k - 1) - (bd_k + bd_k1) * sk + bd_k1 * so.at(k + 1));
-      }
-      // Cn[m] = 0 - ap[k]; d[m] += fab*Cn[m] with fab = 1
-      const double cnk = -hflux(A, fc, nfs, k);
-      d += cnk;
-      A.Cn[base + (int64_t)k * 64] = cnk;
-    }
-    if (m > 0) {
-      b -= a * cprev / bprev;
-      d -= a * dprev / bprev;
-    }
-    const int64_t idx = base + (int64_t)k * 64;
-    A.tb[idx] = b;
-    A.tc[idx] = c;
-    A.td[idx] = d;
-    bprev = b; cprev = c; dprev = d;
-  }
-  // Cn: zero above and at ktop, the folded top value on [ctop, ktop]
-  for (int k = 0; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = 0;
-  for (int k = ct; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = cn0 / (1 + abs(ct - ktop));
-  // back substitution into scal[ktop..Nk-1]
-  if (nb > 0) {
-    double un = A.td[base + (int64_t)(nk - 1) * 64] / A.tb[base + (int64_t)(nk - 1) * 64];
-    A.s[base + (int64_t)(nk - 1) * 64] = un;
-    for (int k = nk - 2; k >= ktop; k--) {
-      const int64_t idx = base + (int64_t)k * 64;
-      const double bk = A.tb[idx];
-      un = A.td[idx] / bk - A.tc[idx] * un / bk;
-      A.s[idx] = un;
-    }
-  } else if (A.step > 1) {
-    const double b0 = A.tb[base + (int64_t)ktop * 64], d0 = A.td[base + (int64_t)ktop * 64];
-    A.s[base + (int64_t)ktop * 64] = (b0 > 0 && A.active[s] != 0) ? d0 / b0 : 0;
-  }
-  for (int k = 0; k < ct; k++) A.s[base + (int64_t)k * 64] = 0;
-  const double stop = A.s[base + (int64_t)ktop * 64];
-  for (int k = ct; k < cto; k++) A.s[base + (int64_t)k * 64] = stop;
+  update_column_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-extern "C" {
-
-int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
-  HIPCHK(hipSetDevice(g->device));
-  if (!g->have_extents) { set_error("sunhip_face_scalars: call sunhip_set_extents first"); return -1; }
-  Field *fs, *u, *ut, *dzf, *sq, *sqc, *hp, *hm, *bs = nullptr;
-  int rc = field(g, scal, &fs) | field(g, "u", &u) | field(g, "utmp2", &ut) | field(g, "dzf", &dzf) |
-           field(g, "gradSx", &sq) | field(g, "gradSy", &sqc) | field(g, "SfHp", &hp) | field(g, "SfHm", &hm);
+namespace sunhip {
+int tvd_args(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, TvdArgs *T, ColArgs *Cc) {
+  if (!g->have_extents) { set_error("scalar step: call sunhip_set_extents first"); return -1; }
+  Field *fs, *so, *wn, *w2, *dzz, *dzo, *ktv, *u, *ut, *dzf, *hp, *hm, *act, *cn, *tb, *tc, *td, *sq, *sqc;
+  int rc = field(g, scal, &fs) | field(g, "stmp", &so) | field(g, "wnew", &wn) | field(g, "wtmp2", &w2) |
+           field(g, "dzz", &dzz) | field(g, "dzzold", &dzo) | field(g, "kappa_tv", &ktv) | field(g, "u", &u) |
+           field(g, "utmp2", &ut) | field(g, "dzf", &dzf) | field(g, "SfHp", &hp) | field(g, "SfHm", &hm) |
+           field(g, "active", &act) | field(g, "Cn", &cn) | field(g, "us_b", &tb) | field(g, "us_c", &tc) |
+           field(g, "us_d", &td) | field(g, "gradSx", &sq) | field(g, "gradSy", &sqc);
   if (rc) return rc;
+  Field *bs = nullptr;
   auto it = g->fields.find("bscal");
   if (it != g->fields.end()) bs = &it->second;
   TvdArgs A{};
   A.s = fs->d; A.u = u->d; A.utmp2 = ut->d; A.dzf = dzf->d; A.sumQ = sq->d; A.sumQC = sqc->d;
   A.oQ = sq->d; A.oQC = sqc->d; A.SfHp = hp->d; A.SfHm = hm->d; A.bscal = bs ? bs->d : nullptr;
   A.theta = sp->theta; A.TVD = sp->TVD;
+  *T = A;
+  ColArgs C{};
+  C.s_old = so->d; C.s = fs->d; C.wnew = wn->d; C.wtmp2 = w2->d; C.dzz = dzz->d; C.dzzold = dzo->d;
+  C.kappa_tv = ktv->d; C.u = u->d; C.utmp2 = ut->d; C.dzf = dzf->d; C.SfHp = hp->d; C.SfHm = hm->d;
+  C.active = act->d; C.Cn = cn->d; C.tb = tb->d; C.tc = tc->d; C.td = td->d;
+  C.theta = sp->theta; C.dt = sp->dt; C.kappa = sp->kappa; C.TVD = sp->TVD; C.step = sp->step;
+  C.alpha_top = sp->alpha_top; C.alpha_bot = sp->alpha_bot;
+  *Cc = C;
+  return 0;
+}
+}  // namespace sunhip
+
+extern "C" {
+
+int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
+  HIPCHK(hipSetDevice(g->device));
+  TvdArgs A;
+  ColArgs C;
+  int rc = tvd_args(g, sp, scal, &A, &C);
+  if (rc) return rc;
   const Geo G = g->geo();
   int ps = prof_begin(g, PK_TSUMS);
   k_tvd_sums<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
@@ -407,25 +85,14 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
 
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
   HIPCHK(hipSetDevice(g->device));
-  if (!g->have_extents) { set_error("sunhip_update_scalars: call sunhip_set_extents first"); return -1; }
-  int rc = 0;
-  if (sp->TVD) {
-    if ((rc = sunhip_face_scalars(g, sp, scal))) return rc;
-  }
-  Field *fs, *so, *wn, *w2, *dzz, *dzo, *ktv, *u, *ut, *dzf, *hp, *hm, *act, *cn, *tb, *tc, *td;
-  rc = field(g, scal, &fs) | field(g, "stmp", &so) | field(g, "wnew", &wn) | field(g, "wtmp2", &w2) |
-       field(g, "dzz", &dzz) | field(g, "dzzold", &dzo) | field(g, "kappa_tv", &ktv) | field(g, "u", &u) |
-       field(g, "utmp2", &ut) | field(g, "dzf", &dzf) | field(g, "SfHp", &hp) | field(g, "SfHm", &hm) |
-       field(g, "active", &act) | field(g, "Cn", &cn) | field(g, "us_b", &tb) | field(g, "us_c", &tc) |
-       field(g, "us_d", &td);
+  TvdArgs T;
+  ColArgs A;
+  int rc = tvd_args(g, sp, scal, &T, &A);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(so->d, fs->d, fs->n * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
-  ColArgs A{};
-  A.s_old = so->d; A.s = fs->d; A.wnew = wn->d; A.wtmp2 = w2->d; A.dzz = dzz->d; A.dzzold = dzo->d;
-  A.kappa_tv = ktv->d; A.u = u->d; A.utmp2 = ut->d; A.dzf = dzf->d; A.SfHp = hp->d; A.SfHm = hm->d;
-  A.active = act->d; A.Cn = cn->d; A.tb = tb->d; A.tc = tc->d; A.td = td->d;
-  A.theta = sp->theta; A.dt = sp->dt; A.kappa = sp->kappa; A.TVD = sp->TVD; A.step = sp->step;
-  A.alpha_top = sp->alpha_top; A.alpha_bot = sp->alpha_bot;
+  if (sp->TVD && (rc = sunhip_face_scalars(g, sp, scal))) return rc;
+  // the column update reads the old values from stmp and writes scal
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(A.s_old), A.s, g->fields.at(scal).n * sizeof(double),
+                        hipMemcpyDeviceToDevice, g->stream));
   const int ps = prof_begin(g, PK_TCOLUMN);
   k_update_column<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), A);
   prof_end(g, ps);

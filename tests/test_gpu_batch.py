@@ -177,17 +177,22 @@ def test_batch_mixed(gpu_available, states):
 
 
 def test_batch_update_scalars_bitwise(gpu_available, states):
+    """One batched scalar step over members with different TVD schemes
+    (batched face sums, face values and column kernels) equals the oracle's
+    UpdateScalars of each member bit for bit."""
     o = _impls.oracle()
-    sts = [st.clone() for st in states[:5]]
-    for st in sts:
-        st.set_prop(TVDsalt=3, n=2)
+    sts = [st.clone() for st in states]
+    schemes = [3, 0, 4, 1, 3, 4, 3][:len(sts)]
+    for st, tvd in zip(sts, schemes):
+        st.set_prop(TVDsalt=tvd, n=2)
     dgs = []
     for st in sts:
         dg = device.DeviceGrid(st)
         device.member_upload_tvd(dg, st)
         dgs.append(dg)
     b = device.Batch()
-    b.update_scalars(dgs, [(st.get_prop("theta"), st.get_prop("dt"), st.get_prop("kappa_s"), 3, 2) for st in sts])
+    b.update_scalars(dgs, [(st.get_prop("theta"), st.get_prop("dt"), st.get_prop("kappa_s"), tvd, 2)
+                           for st, tvd in zip(sts, schemes)])
     for st, dg in zip(sts, dgs):
         a = st.clone()
         o.update_salinity(a)
