@@ -380,7 +380,8 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= upload_vec(g, &g->d_e_etop, ezeros);
   // reductions: one partial pair per block of the widest launch
   const int nblk = std::max(g->Nt, g->Net) + 64;
-  rc |= dev_alloc(g, (void **)&g->d_part, sizeof(double) * 4 * (size_t)nblk);
+  g->part_n = 4 * (int64_t)nblk;
+  rc |= dev_alloc(g, (void **)&g->d_part, sizeof(double) * (size_t)g->part_n);
   rc |= dev_alloc(g, (void **)&g->d_count, sizeof(unsigned) * 64);
   rc |= dev_alloc(g, (void **)&g->d_qs, sizeof(CGScal));
   rc |= dev_alloc(g, (void **)&g->d_hs, sizeof(CGScal));

@@ -113,6 +113,7 @@ struct sunhip_grid {
   bool have_extents = false;
   // reductions
   double *d_part = nullptr;         // block partials
+  int64_t part_n = 0;               // its capacity in doubles
   unsigned *d_count = nullptr;      // last-block tickets (one per reduction site)
   sunhip::CGScal *d_qs = nullptr, *d_hs = nullptr;
   sunhip::CGScal *d_ms = nullptr, *d_mo = nullptr;  // mixed form: inner FP32 CG, outer residual
@@ -136,6 +137,7 @@ struct sunhip_grid {
   int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
   int tune_blk = 0;   // operator kernel with block-staged neighbour values
   int tune_punroll = 8;  // levels per load group of the preconditioner (1, 2, 4, 8)
+  int tune_band = 0;     // operator kernel over level bands of this many levels (0: whole columns)
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of
   // the solver kernels, resolved by sunhip_kernel_times

@@ -128,6 +128,22 @@ __global__ __launch_bounds__(TPB * 64, MINW > 0 ? MINW : 1) void k_qapply(Geo G,
   }
 }
 
+// The fused operator over level bands of H levels: block b covers the tiles
+// of group b % ngroups in band b / ngroups (band-major, so blocks running
+// together on an XCD are neighbouring tiles at the same depths and share
+// their rows in L2).
+template <int NF, int U, typename T, bool NTS>
+__global__ __launch_bounds__(256) void k_qapply_band(Geo G, QVec<T, T> V, CGScal *sc, Red R, int H, int ngroups) {
+  if (sc->done) return;
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int band = vb / ngroups, grp = vb - band * ngroups;
+  const int t = grp * 4 + (threadIdx.x >> 6);
+  const double dot = qapply_tile<QM_FUSED, NF, U, T, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63,
+                                                            band * H, band * H + H);
+  double dv[1] = {dot};
+  reduce_out<1, 256>(dv, R, sc);
+}
+
 // The fused operator with block-staged neighbour values (qapply_block).
 template <int NF, int U, typename T, bool NTS>
 __global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
@@ -259,6 +275,21 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
 #define APPLYV(TPB, NTS) k_qapply<MODE, 3, 4, T, TO, TPB, NTS><<<nba, TPB * 64, 0, g->stream>>>(G, V, sc, R)
 #define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
   if constexpr (MODE == QM_FUSED && std::is_same<T, TO>::value) {
+    if (variants && g->tune_band > 0 && tpb == 4) {
+      const int H = g->tune_band, ngroups = (g->Nt + 3) / 4, nbands = (g->maxdepth + H - 1) / H;
+      const int nblk = ngroups * nbands;
+      if ((int64_t)nblk > g->part_n) {
+        set_error("band=%d needs %d block partials, more than the %lld held", H, nblk, (long long)g->part_n);
+        return -1;
+      }
+      if (g->tune_nts) k_qapply_band<3, 4, T, true><<<nblk, 256, 0, g->stream>>>(G, V, sc, R, H, ngroups);
+      else k_qapply_band<3, 4, T, false><<<nblk, 256, 0, g->stream>>>(G, V, sc, R, H, ngroups);
+      prof_end(g, ps);
+      HIPCHK(hipGetLastError());
+      if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nblk, sc, R.op);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
     if (variants && g->tune_blk && tpb == 4) {
       if (g->tune_nts) k_qapply_blk<3, 4, T, true><<<nba, 256, 0, g->stream>>>(G, V, sc, R);
       else k_qapply_blk<3, 4, T, false><<<nba, 256, 0, g->stream>>>(G, V, sc, R);
@@ -668,6 +699,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   else if (!strcmp(key, "rzid")) g->tune_rzid = value != 0;
   else if (!strcmp(key, "minwaves")) g->tune_minw = value;
   else if (!strcmp(key, "blocklds")) g->tune_blk = value != 0;
+  else if (!strcmp(key, "band")) {
+    if (value < 0 || (value > 0 && value < 8)) { set_error("band must be 0 or at least 8 levels"); return -1; }
+    g->tune_band = value;
+  }
   else { set_error("unknown tuning key %s", key); return -1; }
   return 0;
 }

@@ -143,7 +143,8 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 // product.
 
 template <int MODE, int NF, int U, typename T, typename TO, bool NTS = false>
-__device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V, T mu, T nu, int t, int l) {
+__device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V, T mu, T nu, int t, int l,
+                                              int k0 = 0, int k1 = 0x7fffffff) {
   double dot = 0;
   if (t >= G.Nt) return 0;
   const T *__restrict__ coef = V.coef;
@@ -176,8 +177,14 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
     }
     lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
   }
+  // levels [kbeg, kend) of the column: all of it, or one band [k0, k1)
+  const int kbeg = max(kt, k0), kend = min(n, k1);
   T vprev = 0;
-  for (int kb = kt; kb < n; kb += U) {
+  if (kbeg > kt && kbeg < kend) {  // a band below the top: the level above it
+    const uint32_t idx = base + (uint32_t)(kbeg - 1) * 64u;
+    vprev = MODE == QM_FUSED ? at32(rt, idx) + mu * at32(po, idx) : at32(vin, idx);
+  }
+  for (int kb = kbeg; kb < kend; kb += U) {
     T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
 #pragma unroll
     for (int j = 0; j <= U; j++) {
@@ -205,7 +212,7 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int k = kb + j;
-      if (k < n) {
+      if (k < kend) {
         const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
         const T c0 = c[j], cp1 = c[j + 1];
         T y = 0;

@@ -3,8 +3,8 @@ Unroll, store-hint, register-budget and block-staging variants are
 bit-identical to the default configuration (same operations in the same
 order, same reduction order).  Variants that regroup a dot product only
 change its rounding: tiles per block (block partial sums of 8 or 16 tiles)
-and rzid (r.M^-1 r as sum d'^2/b'); for those the iteration count is within
-+-1 and q within 1e-12."""
+rzid (r.M^-1 r as sum d'^2/b') and band (partial sums per level band); for
+those the iteration count is within +-1 and q within 1e-12."""
 import numpy as np
 import pytest
 
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 BITWISE = [("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1), ("prec_unroll", 4), ("ntstore", 0),
            ("minwaves", 3), ("minwaves", 4), ("blocklds", 1)]
-ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0)]
+ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32)]
 
 
 def solve(st, tuning):
