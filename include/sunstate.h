@@ -1,9 +1,10 @@
 /*
  * sunstate.h -- host-side construction of SUNTANS grid/physics/property
- * state for the drop-in solvers: a triangulated-rectangle grid builder with
- * the reference's geometry conventions, allocation of the jagged per-cell and
- * per-edge rows in the reference's shapes, the vertical-grid and flux-height
- * updates the solvers consume, and flat (column-compact) access to fields.
+ * state for the drop-in solvers: planform builders (a triangulated
+ * rectangle, and any polygon list) with the reference's geometry
+ * conventions, allocation of the jagged per-cell and per-edge rows in the
+ * reference's shapes, the vertical-grid and flux-height updates the solvers
+ * consume, and flat (column-compact) access to fields.
  *
  * These are set-up utilities that produce the solvers' inputs; they are not
  * part of the accelerated path.  Every function here is plain C.
@@ -25,6 +26,19 @@ extern "C" {
  * :992-1007 (normal), :3493-3533 (n1, n2, dg), :1628-1685 (cellp/edgep,
  * celldist/edgedist). */
 gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly);
+
+/* General unstructured planform: Np points, Nc polygons of nfaces[i] corners
+ * (cells[i*maxfaces + v], either orientation, 3 <= nfaces[i] <= maxfaces)
+ * with Voronoi points (xv, yv).  Edges are the polygon sides, numbered in
+ * the order they first appear in the cell list; an edge's end points and
+ * grad[2e] come from that first cell, grad[2e+1] is the cell across or -1
+ * (boundary, mark 1).  Face nf of cell i joins corners nf and nf+1; unused
+ * face slots hold -1.  Ac is the polygon area.  Metrics, normals and the
+ * computational ordering follow the same conventions as sun_grid_tri.
+ * Returns NULL for a malformed cell list (a corner out of range, a
+ * degenerate side, or a side shared by more than two cells). */
+gridT *sun_grid_polygons(int Np, const double *xp, const double *yp, int Nc, int maxfaces, const int *nfaces,
+                         const int *cells, const double *xv, const double *yv);
 
 /* Vertical grid: dz[Nkmax] reference layer thicknesses and dv[Nc] depths.
  * Sets Nk (main/grid.c:1466-1498), Nke/Nkc (:1530-1563), ctop/etop = 0, and

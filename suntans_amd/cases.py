@@ -139,8 +139,10 @@ def depth_integrated_D(st: State) -> np.ndarray:
     return np.bincount(edge, weights=dzf * st.edge_wet_mask(), minlength=st.Ne)
 
 
-def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3) -> State:
-    """Small grid with ragged columns, dry top layers and a moved surface."""
+def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3, planform="tri",
+         split_every=0, maxfaces=None) -> State:
+    """Small grid with ragged columns, dry top layers and a moved surface.
+    planform/split_every/maxfaces choose the cell shapes (State)."""
     rng = np.random.default_rng(seed)
     dz = np.full(Nkmax, 1.0)
     H = Nkmax * 1.0
@@ -150,7 +152,8 @@ def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3) -
         d[rng.integers(len(d))] = H
         return d
 
-    st = State(nx, ny, 400.0 * nx, 300.0 * ny, Nkmax, dz, depth)
+    st = State(nx, ny, 400.0 * nx, 300.0 * ny, Nkmax, dz, depth, planform=planform, split_every=split_every,
+               maxfaces=maxfaces)
     st.update_dz(-1)
     h1 = rng.uniform(-1.6, 0.3, st.Nc) if move_surface else np.zeros(st.Nc)
     st.set("h", h1)

@@ -36,6 +36,95 @@ static size_t rows_total(int nrows, const int *len, int extra) {
 /* ------------------------------------------------------------------------ */
 /* planform                                                                  */
 
+/* Cell, edge and point arrays of a planform with g->Nc, g->Ne, g->Np and
+ * g->maxfaces set (shapes as main/gridio.c:86-150). */
+static void planform_alloc(gridT *g) {
+  const size_t Nc = (size_t)g->Nc, Ne = (size_t)g->Ne, mf = (size_t)g->maxfaces;
+  g->cells = (int *)xcalloc(mf * Nc, sizeof(int));
+  g->face = (int *)xcalloc(mf * Nc, sizeof(int));
+  g->neigh = (int *)xcalloc(mf * Nc, sizeof(int));
+  g->normal = (int *)xcalloc(mf * Nc, sizeof(int));
+  g->nfaces = (int *)xcalloc(Nc, sizeof(int));
+  g->xv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->yv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->Ac = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->def = (REAL *)xcalloc(mf * Nc, sizeof(REAL));
+  g->dv = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->edges = (int *)xcalloc(3 * Ne, sizeof(int));
+  g->grad = (int *)xcalloc(2 * Ne, sizeof(int));
+  g->gradf = (int *)xcalloc(2 * Ne, sizeof(int));
+  g->mark = (int *)xcalloc(Ne, sizeof(int));
+  g->eneigh = (int *)xcalloc(2 * (mf - 1) * Ne, sizeof(int));
+  g->df = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->dg = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->n1 = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->n2 = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->xe = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->ye = (REAL *)xcalloc(Ne, sizeof(REAL));
+}
+
+/* Edge metrics, face normals and the single-process computational ordering
+ * of a planform whose points, cells (nfaces, face, neigh), Voronoi points,
+ * areas and edges (end points, grad, mark) are set. */
+static void planform_finish(gridT *g) {
+  const int Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
+  /* metrics, main/grid.c:3457-3533 conventions */
+  for (int e = 0; e < Ne; e++) {
+    int p1 = g->edges[3 * e], p2 = g->edges[3 * e + 1];
+    double tx = g->xp[p2] - g->xp[p1], ty = g->yp[p2] - g->yp[p1];
+    double tm = sqrt(tx * tx + ty * ty);
+    g->df[e] = tm;
+    tx /= tm; ty /= tm;
+    int c = g->grad[2 * e] >= 0 ? g->grad[2 * e] : g->grad[2 * e + 1];
+    double s = (g->xv[c] - g->xp[p1]) * tx + (g->yv[c] - g->yp[p1]) * ty;
+    g->xe[e] = g->xp[p1] + s * tx; g->ye[e] = g->yp[p1] + s * ty;
+    int a = g->grad[2 * e], b = g->grad[2 * e + 1];
+    double nx_, ny_;
+    if (a >= 0 && b >= 0) {
+      nx_ = g->xv[a] - g->xv[b]; ny_ = g->yv[a] - g->yv[b];
+      g->dg[e] = sqrt(nx_ * nx_ + ny_ * ny_);
+      g->n1[e] = nx_ / g->dg[e]; g->n2[e] = ny_ / g->dg[e];
+    } else {
+      int in = a >= 0 ? a : b;
+      nx_ = a >= 0 ? g->xv[in] - g->xe[e] : g->xe[e] - g->xv[in];
+      ny_ = a >= 0 ? g->yv[in] - g->ye[e] : g->ye[e] - g->yv[in];
+      double d = sqrt(nx_ * nx_ + ny_ * ny_);
+      g->n1[e] = nx_ / d; g->n2[e] = ny_ / d; g->dg[e] = 2.0 * d;
+    }
+  }
+  for (int i = 0; i < Nc; i++)
+    for (int nf = 0; nf < g->nfaces[i]; nf++) {
+      int e = g->face[mf * i + nf];
+      g->normal[mf * i + nf] = (i == g->grad[2 * e]) ? -1 : 1;
+      if (g->grad[2 * e] == i) g->gradf[2 * e] = nf; else g->gradf[2 * e + 1] = nf;
+      int p1 = g->edges[3 * e];
+      g->def[mf * i + nf] = -((g->xv[i] - g->xp[p1]) * g->n1[e] + (g->yv[i] - g->yp[p1]) * g->n2[e]) *
+                           g->normal[mf * i + nf];
+    }
+
+  /* computational ordering (single process): every cell is computational;
+   * edges sorted by mark 0,1,2,3,4 */
+  g->cellp = (int *)xcalloc(Nc, sizeof(int));
+  g->celldist = (int *)xcalloc(3, sizeof(int));
+  for (int i = 0; i < Nc; i++) g->cellp[i] = i;
+  g->celldist[0] = 0; g->celldist[1] = Nc; g->celldist[2] = Nc;
+  g->edgep = (int *)xcalloc(Ne, sizeof(int));
+  g->edgedist = (int *)xcalloc(6, sizeof(int));
+  int k = 0;
+  for (int e = 0; e < Ne; e++) if (g->mark[e] == 0 || g->mark[e] == 5) g->edgep[k++] = e;
+  g->edgedist[1] = k;
+  for (int m = 1; m < 5; m++) {
+    for (int e = 0; e < Ne; e++) if (g->mark[e] == m) g->edgep[k++] = e;
+    g->edgedist[m + 1] = k;
+  }
+  g->mnptr = (int *)xcalloc(Nc, sizeof(int));
+  g->eptr = (int *)xcalloc(Ne, sizeof(int));
+  for (int i = 0; i < Nc; i++) g->mnptr[i] = i;
+  for (int e = 0; e < Ne; e++) g->eptr[e] = e;
+  g->Nneighs = 0;
+  g->myneighs = (int *)xcalloc(1, sizeof(int));
+}
+
 gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly) {
   gridT *g = (gridT *)xcalloc(1, sizeof(gridT));
   const int Nc = 2 * nx * ny, Np = (nx + 1) * (ny + 1);
@@ -57,27 +146,7 @@ gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly) {
   for (int iy = 0; iy <= ny; iy++)
     for (int ix = 0; ix <= nx; ix++) { g->xp[PT(ix, iy)] = ix * dx; g->yp[PT(ix, iy)] = iy * dy; }
 
-  g->cells = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
-  g->face = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
-  g->neigh = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
-  g->normal = (int *)xcalloc(3 * (size_t)Nc, sizeof(int));
-  g->nfaces = (int *)xcalloc(Nc, sizeof(int));
-  g->xv = (REAL *)xcalloc(Nc, sizeof(REAL));
-  g->yv = (REAL *)xcalloc(Nc, sizeof(REAL));
-  g->Ac = (REAL *)xcalloc(Nc, sizeof(REAL));
-  g->def = (REAL *)xcalloc(3 * (size_t)Nc, sizeof(REAL));
-  g->dv = (REAL *)xcalloc(Nc, sizeof(REAL));
-  g->edges = (int *)xcalloc(3 * (size_t)Ne, sizeof(int));
-  g->grad = (int *)xcalloc(2 * (size_t)Ne, sizeof(int));
-  g->gradf = (int *)xcalloc(2 * (size_t)Ne, sizeof(int));
-  g->mark = (int *)xcalloc(Ne, sizeof(int));
-  g->eneigh = (int *)xcalloc(2 * (size_t)(3 - 1) * Ne, sizeof(int));
-  g->df = (REAL *)xcalloc(Ne, sizeof(REAL));
-  g->dg = (REAL *)xcalloc(Ne, sizeof(REAL));
-  g->n1 = (REAL *)xcalloc(Ne, sizeof(REAL));
-  g->n2 = (REAL *)xcalloc(Ne, sizeof(REAL));
-  g->xe = (REAL *)xcalloc(Ne, sizeof(REAL));
-  g->ye = (REAL *)xcalloc(Ne, sizeof(REAL));
+  planform_alloc(g);
 
   for (int iy = 0; iy < ny; iy++)
     for (int ix = 0; ix < nx; ix++) {
@@ -126,61 +195,7 @@ gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly) {
     }
   for (int e = 0; e < Ne; e++) g->edges[3 * e + 2] = g->mark[e];
 
-  /* metrics, main/grid.c:3457-3533 conventions */
-  for (int e = 0; e < Ne; e++) {
-    int p1 = g->edges[3 * e], p2 = g->edges[3 * e + 1];
-    double tx = g->xp[p2] - g->xp[p1], ty = g->yp[p2] - g->yp[p1];
-    double tm = sqrt(tx * tx + ty * ty);
-    g->df[e] = tm;
-    tx /= tm; ty /= tm;
-    int c = g->grad[2 * e] >= 0 ? g->grad[2 * e] : g->grad[2 * e + 1];
-    double s = (g->xv[c] - g->xp[p1]) * tx + (g->yv[c] - g->yp[p1]) * ty;
-    g->xe[e] = g->xp[p1] + s * tx; g->ye[e] = g->yp[p1] + s * ty;
-    int a = g->grad[2 * e], b = g->grad[2 * e + 1];
-    double nx_, ny_;
-    if (a >= 0 && b >= 0) {
-      nx_ = g->xv[a] - g->xv[b]; ny_ = g->yv[a] - g->yv[b];
-      g->dg[e] = sqrt(nx_ * nx_ + ny_ * ny_);
-      g->n1[e] = nx_ / g->dg[e]; g->n2[e] = ny_ / g->dg[e];
-    } else {
-      int in = a >= 0 ? a : b;
-      nx_ = a >= 0 ? g->xv[in] - g->xe[e] : g->xe[e] - g->xv[in];
-      ny_ = a >= 0 ? g->yv[in] - g->ye[e] : g->ye[e] - g->yv[in];
-      double d = sqrt(nx_ * nx_ + ny_ * ny_);
-      g->n1[e] = nx_ / d; g->n2[e] = ny_ / d; g->dg[e] = 2.0 * d;
-    }
-  }
-  for (int i = 0; i < Nc; i++)
-    for (int nf = 0; nf < 3; nf++) {
-      int e = g->face[3 * i + nf];
-      g->normal[3 * i + nf] = (i == g->grad[2 * e]) ? -1 : 1;
-      if (g->grad[2 * e] == i) g->gradf[2 * e] = nf; else g->gradf[2 * e + 1] = nf;
-      int p1 = g->edges[3 * e];
-      g->def[3 * i + nf] = -((g->xv[i] - g->xp[p1]) * g->n1[e] + (g->yv[i] - g->yp[p1]) * g->n2[e]) *
-                           g->normal[3 * i + nf];
-    }
-
-  /* computational ordering (single process): every cell is computational;
-   * edges sorted by mark 0,1,2,3,4 */
-  g->cellp = (int *)xcalloc(Nc, sizeof(int));
-  g->celldist = (int *)xcalloc(3, sizeof(int));
-  for (int i = 0; i < Nc; i++) g->cellp[i] = i;
-  g->celldist[0] = 0; g->celldist[1] = Nc; g->celldist[2] = Nc;
-  g->edgep = (int *)xcalloc(Ne, sizeof(int));
-  g->edgedist = (int *)xcalloc(6, sizeof(int));
-  int k = 0;
-  for (int e = 0; e < Ne; e++) if (g->mark[e] == 0 || g->mark[e] == 5) g->edgep[k++] = e;
-  g->edgedist[1] = k;
-  for (int m = 1; m < 5; m++) {
-    for (int e = 0; e < Ne; e++) if (g->mark[e] == m) g->edgep[k++] = e;
-    g->edgedist[m + 1] = k;
-  }
-  g->mnptr = (int *)xcalloc(Nc, sizeof(int));
-  g->eptr = (int *)xcalloc(Ne, sizeof(int));
-  for (int i = 0; i < Nc; i++) g->mnptr[i] = i;
-  for (int e = 0; e < Ne; e++) g->eptr[e] = e;
-  g->Nneighs = 0;
-  g->myneighs = (int *)xcalloc(1, sizeof(int));
+  planform_finish(g);
   return g;
 #undef PT
 #undef T0
@@ -188,6 +203,110 @@ gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly) {
 #undef EH
 #undef EV
 #undef ED
+}
+
+/* One side of a cell: its end points (lower index first) and where it is. */
+struct side { int lo, hi, cell, nf; };
+static int side_by_points(const void *a, const void *b) {
+  const struct side *x = (const struct side *)a, *y = (const struct side *)b;
+  if (x->lo != y->lo) return x->lo < y->lo ? -1 : 1;
+  if (x->hi != y->hi) return x->hi < y->hi ? -1 : 1;
+  if (x->cell != y->cell) return x->cell < y->cell ? -1 : 1;
+  return x->nf < y->nf ? -1 : (x->nf > y->nf);
+}
+
+gridT *sun_grid_polygons(int Np, const double *xp, const double *yp, int Nc, int maxfaces, const int *nfaces,
+                         const int *cells, const double *xv, const double *yv) {
+  if (Np < 3 || Nc < 1 || maxfaces < 3) return NULL;
+  for (int i = 0; i < Nc; i++) {
+    if (nfaces[i] < 3 || nfaces[i] > maxfaces) return NULL;
+    for (int v = 0; v < nfaces[i]; v++) {
+      const int pt = cells[(size_t)i * maxfaces + v];
+      if (pt < 0 || pt >= Np) return NULL;
+    }
+  }
+  /* sides grouped by their end points; each group is one edge */
+  size_t ns = 0;
+  for (int i = 0; i < Nc; i++) ns += (size_t)nfaces[i];
+  struct side *sd = (struct side *)xcalloc(ns, sizeof(struct side));
+  size_t k = 0;
+  for (int i = 0; i < Nc; i++)
+    for (int nf = 0; nf < nfaces[i]; nf++) {
+      const int a = cells[(size_t)i * maxfaces + nf], b = cells[(size_t)i * maxfaces + (nf + 1) % nfaces[i]];
+      if (a == b) { free(sd); return NULL; }
+      sd[k].lo = a < b ? a : b; sd[k].hi = a < b ? b : a; sd[k].cell = i; sd[k].nf = nf; k++;
+    }
+  qsort(sd, ns, sizeof(struct side), side_by_points);
+  /* edges are numbered in the order their first side appears in the cell
+   * list: rank the groups by (cell, nf) of their first member */
+  int ne = 0;
+  int *first = (int *)xcalloc(ns, sizeof(int)); /* group -> index of its first side in sd */
+  for (size_t j = 0; j < ns; j++) {
+    if (j > 0 && sd[j].lo == sd[j - 1].lo && sd[j].hi == sd[j - 1].hi) {
+      if (j > 1 && sd[j].lo == sd[j - 2].lo && sd[j].hi == sd[j - 2].hi) { free(sd); free(first); return NULL; }
+      continue;
+    }
+    first[ne++] = (int)j;
+  }
+  int *order = (int *)xcalloc((size_t)Nc * maxfaces, sizeof(int)); /* (cell, nf) -> group, -1 */
+  for (size_t j = 0; j < (size_t)Nc * maxfaces; j++) order[j] = -1;
+  for (int e = 0; e < ne; e++) {
+    const struct side *f = &sd[first[e]];
+    order[(size_t)f->cell * maxfaces + f->nf] = e;
+  }
+  int *edge_of_group = (int *)xcalloc((size_t)ne, sizeof(int));
+  int next = 0;
+  for (size_t j = 0; j < (size_t)Nc * maxfaces; j++)
+    if (order[j] >= 0) edge_of_group[order[j]] = next++;
+
+  gridT *g = (gridT *)xcalloc(1, sizeof(gridT));
+  g->Nc = g->Nc_comp = Nc; g->Ne = g->Ne_comp = ne; g->Np = Np;
+  g->maxfaces = maxfaces; g->Nge = 0;
+  g->xp = (REAL *)xcalloc((size_t)Np, sizeof(REAL));
+  g->yp = (REAL *)xcalloc((size_t)Np, sizeof(REAL));
+  memcpy(g->xp, xp, sizeof(REAL) * (size_t)Np);
+  memcpy(g->yp, yp, sizeof(REAL) * (size_t)Np);
+  planform_alloc(g);
+  for (size_t j = 0; j < (size_t)Nc * maxfaces; j++) {
+    g->cells[j] = cells[j];
+    g->face[j] = -1;
+    g->neigh[j] = -1;
+  }
+  for (int i = 0; i < Nc; i++) {
+    g->nfaces[i] = nfaces[i];
+    g->xv[i] = xv[i];
+    g->yv[i] = yv[i];
+    /* shoelace area of the (counterclockwise or clockwise) polygon */
+    double A = 0;
+    for (int v = 0; v < nfaces[i]; v++) {
+      const int a = cells[(size_t)i * maxfaces + v], b = cells[(size_t)i * maxfaces + (v + 1) % nfaces[i]];
+      A += xp[a] * yp[b] - xp[b] * yp[a];
+    }
+    g->Ac[i] = 0.5 * fabs(A);
+  }
+  for (int grp = 0; grp < ne; grp++) {
+    const int e = edge_of_group[grp];
+    const struct side *f = &sd[first[grp]];
+    const int two = first[grp] + 1 < (int)ns && sd[first[grp] + 1].lo == f->lo && sd[first[grp] + 1].hi == f->hi;
+    /* first side in cell order gives the end-point order and grad[2e] */
+    const struct side *s0 = f, *s1 = two ? f + 1 : NULL;
+    if (s1 && (s1->cell < s0->cell || (s1->cell == s0->cell && s1->nf < s0->nf))) { s0 = f + 1; s1 = f; }
+    g->edges[3 * e] = cells[(size_t)s0->cell * maxfaces + s0->nf];
+    g->edges[3 * e + 1] = cells[(size_t)s0->cell * maxfaces + (s0->nf + 1) % nfaces[s0->cell]];
+    g->grad[2 * e] = s0->cell;
+    g->grad[2 * e + 1] = s1 ? s1->cell : -1;
+    g->mark[e] = s1 ? 0 : 1;
+    g->edges[3 * e + 2] = g->mark[e];
+    g->face[(size_t)s0->cell * maxfaces + s0->nf] = e;
+    g->neigh[(size_t)s0->cell * maxfaces + s0->nf] = s1 ? s1->cell : -1;
+    if (s1) {
+      g->face[(size_t)s1->cell * maxfaces + s1->nf] = e;
+      g->neigh[(size_t)s1->cell * maxfaces + s1->nf] = s0->cell;
+    }
+  }
+  free(sd); free(first); free(order); free(edge_of_group);
+  planform_finish(g);
+  return g;
 }
 
 int sun_grid_vertical(gridT *g, int Nkmax, const double *dz, const double *dv, int stairstep) {

@@ -192,12 +192,13 @@ struct BTMember {
   ColArgs C;
 };
 
+template <int NF>
 __global__ __launch_bounds__(256) void kb_tvd_sums(const BTMember *__restrict__ M, const int2 *__restrict__ W, int n) {
   const int l = threadIdx.x & 63, stride = gridDim.x * BW;
   for (int w = first_item(); w < n; w += stride) {
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
-    if (B.T.TVD) tvd_sums_tile(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
+    if (B.T.TVD) tvd_sums_tile<NF>(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
 
@@ -210,13 +211,14 @@ __global__ __launch_bounds__(256) void kb_tvd_faces(const BTMember *__restrict__
   }
 }
 
+template <int NF>
 __global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restrict__ M, const int2 *__restrict__ W,
                                                         int n) {
   const int l = threadIdx.x & 63, stride = gridDim.x * BW;
   for (int w = first_item(); w < n; w += stride) {
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
-    update_column_tile(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+    update_column_tile<NF>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
 
@@ -374,7 +376,7 @@ long shape_key(const sunhip_batch *b) {
     while ((1 << lg) < b->nwork[i]) lg++;
     k = k * 64 + (b->nwork[i] ? lg + 1 : 0);
   }
-  return k * 8 + (b->nf > 3);
+  return k * 16 + nf_class(b->nf);
 }
 
 int table_upload(sunhip_batch *b, void *dst, const void *src, size_t bytes, size_t &off) {
@@ -491,10 +493,12 @@ int rebuild(sunhip_batch *b) {
 int launch_residuals(sunhip_batch *b) {
   if (!b->nres) return 0;
   const int nw = b->nwres, nb = (nw + BW - 1) / BW;
-  if (b->nf <= 3)
-    kb_qapply<QM_RESID_DOT, 3, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
-  else
-    kb_qapply<QM_RESID_DOT, 4, 4, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw, b->pres.d);
+  with_nf(b->nf, [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    kb_qapply<QM_RESID_DOT, NF, NF <= 4 ? 4 : 2, double, float><<<nb, 256, 0, b->stream>>>(b->tres.d, b->wres.d, nw,
+                                                                                          b->pres.d);
+    return 0;
+  });
   kb_finalize<1, 0><<<b->nres, 256, 0, b->stream>>>(b->rres.d, b->pres.d, OP_STORE, 0);
   HIPCHK(hipGetLastError());
   for (Task &t : b->tasks)
@@ -518,7 +522,9 @@ int enqueue_iteration(sunhip_batch *b, const Config &c) {
     double *P = b->part[T_Q64].d;
 #define QA(NF, U) kb_qapply<QM_FUSED, NF, U, double, double><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q64].d, n, P)
     if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
-    else { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
+    else if (b->nf == 4) { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
+    else if (b->nf <= 6) { if (c.unroll == 1) QA(6, 1); else QA(6, 2); }
+    else { if (c.unroll == 1) QA(8, 1); else QA(8, 2); }
 #undef QA
     kb_finalize<1, 0><<<b->nmem[T_Q64], 256, 0, b->stream>>>(b->rng[T_Q64].d, P, OP_APPLY, 1);
     if (c.unroll == 1) kb_qprec<1, double><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q64].d, n, P);
@@ -533,7 +539,9 @@ int enqueue_iteration(sunhip_batch *b, const Config &c) {
     double *P = b->part[T_Q32].d;
 #define QA(NF, U) kb_qapply<QM_FUSED, NF, U, float, float><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q32].d, n, P)
     if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
-    else { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
+    else if (b->nf == 4) { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
+    else if (b->nf <= 6) { if (c.unroll == 1) QA(6, 1); else QA(6, 2); }
+    else { if (c.unroll == 1) QA(8, 1); else QA(8, 2); }
 #undef QA
     kb_finalize<1, 0><<<b->nmem[T_Q32], 256, 0, b->stream>>>(b->rng[T_Q32].d, P, OP_APPLY, 1);
     if (c.unroll == 1) kb_qprec<1, float><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q32].d, n, P);
@@ -1015,13 +1023,21 @@ int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **gs, int n, const 
   const int nc = (int)wc.size(), ne = (int)we.size();
   // face values from the current scalar, then the column update from its
   // copy (the order of sunhip_update_scalars)
-  kb_tvd_sums<<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+  int mf = 3;
+  for (int i = 0; i < n; i++) mf = std::max(mf, gs[i]->mf);
+  with_nf(mf, [&](auto nf) {
+    kb_tvd_sums<decltype(nf)::value><<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+    return 0;
+  });
   if (ne) kb_tvd_faces<<<(ne + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wte.d, ne);
   HIPCHK(hipGetLastError());
   for (int i = 0; i < n; i++)
     HIPCHK(hipMemcpyAsync(const_cast<double *>(tab[i].C.s_old), tab[i].C.s, gs[i]->fields.at(scal).n * sizeof(double),
                           hipMemcpyDeviceToDevice, b->stream));
-  kb_update_column<<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+  with_nf(mf, [&](auto nf) {
+    kb_update_column<decltype(nf)::value><<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
+    return 0;
+  });
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;

@@ -145,12 +145,12 @@ int hbufs(sunhip_grid *g, HBufs &B) {
            field(g, "hr", &B.r) | field(g, "hz", &B.z) | field(g, "hrt", &B.rt) | field(g, "hp0", &B.p0) |
            field(g, "hp1", &B.p1);
   if (rc) return rc;
-  // face coefficients: MAXF rows of NS
+  // face coefficients: maxfaces rows of NS
   auto it = g->fields.find("#hfcoef");
   if (it == g->fields.end()) {
     Field f;
     f.kind = CELL2;
-    f.n = (size_t)SUNHIP_MAXF * g->NS;
+    f.n = (size_t)g->mf * g->NS;
     if ((rc = dev_alloc(g, (void **)&f.d, f.n * sizeof(double)))) return rc;
     HIPCHK(hipMemsetAsync(f.d, 0, f.n * sizeof(double), g->stream));
     it = g->fields.emplace("#hfcoef", f).first;
@@ -297,7 +297,7 @@ int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double gra
   if (rc) return rc;
   float *c32, *f32, *x32, *r32, *z32, *t32, *pa, *pb;
   if (!g->d_h32) {
-    if ((rc = dev_alloc(g, (void **)&g->d_h32, sizeof(float) * (size_t)g->NS * (7 + SUNHIP_MAXF)))) return rc;
+    if ((rc = dev_alloc(g, (void **)&g->d_h32, sizeof(float) * (size_t)g->NS * (7 + g->mf)))) return rc;
   }
   c32 = g->d_h32; x32 = c32 + g->NS; r32 = x32 + g->NS; z32 = r32 + g->NS; t32 = z32 + g->NS;
   pa = t32 + g->NS; pb = pa + g->NS; f32 = pb + g->NS;
@@ -306,7 +306,7 @@ int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double gra
   HIPCHK(hipEventRecord(g->ev0, g->stream));
   if ((rc = h_coefficients(g, B, grav, theta, dt))) return rc;
   k_hcast<double, float><<<nb, 256, 0, g->stream>>>(B.coef->d, c32, g->NS);
-  k_hcast<double, float><<<(SUNHIP_MAXF * g->NS + 255) / 256, 256, 0, g->stream>>>(B.fcoef, f32, SUNHIP_MAXF * g->NS);
+  k_hcast<double, float><<<(g->mf * g->NS + 255) / 256, 256, 0, g->stream>>>(B.fcoef, f32, g->mf * g->NS);
   k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, B.h->d);
   HIPCHK(hipGetLastError());
   sunhip_cg_params inner = *prm;
@@ -374,7 +374,7 @@ int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char
   if (rc) return rc;
   Field *fx, *fy;
   if ((rc = field(g, x, &fx) | field(g, y, &fy))) return rc;
-  std::vector<double> fc((size_t)SUNHIP_MAXF * g->NS, 0.0);
+  std::vector<double> fc((size_t)g->mf * g->NS, 0.0);
   for (int c = 0; c < g->Nc; c++) {
     const int s = g->slot_of_cell[c];
     for (int f = 0; f < g->nfaces[c]; f++) fc[(size_t)f * g->NS + s] = hfcoef_flat[(size_t)c * g->mf + f];

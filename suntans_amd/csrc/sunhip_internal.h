@@ -14,16 +14,31 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sunhip.h"
 
-#define SUNHIP_MAXF 4   // faces per cell supported (triangles 3, quadrilaterals 4)
+#define SUNHIP_MAXF 8   // faces per cell supported (triangles to octagons)
 #define SUNHIP_TILE 64
 
 namespace sunhip {
 
 void set_error(const char *fmt, ...);
+
+// Face-count classes the column kernels are instantiated for: a grid of
+// maxfaces mf runs the smallest class >= mf (face slots past a cell's own
+// count are skipped).  with_nf calls fn(std::integral_constant<int, NF>).
+inline int nf_class(int mf) { return mf <= 3 ? 3 : mf <= 4 ? 4 : mf <= 6 ? 6 : 8; }
+template <typename F>
+inline auto with_nf(int mf, F &&fn) {
+  switch (nf_class(mf)) {
+    case 3: return fn(std::integral_constant<int, 3>{});
+    case 4: return fn(std::integral_constant<int, 4>{});
+    case 6: return fn(std::integral_constant<int, 6>{});
+    default: return fn(std::integral_constant<int, 8>{});
+  }
+}
 
 #define HIPCHK(expr)                                                              \
   do {                                                                            \

@@ -311,8 +311,12 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
   } else if (g->mf <= 3) {
     if (g->tune_unroll == 1) APPLY(3, 1); else if (g->tune_unroll == 2) APPLY(3, 2);
     else if (g->tune_unroll == 8) APPLY(3, 8); else APPLY(3, 4);
-  } else {
+  } else if (g->mf == 4) {
     if (g->tune_unroll == 1) APPLY(4, 1); else if (g->tune_unroll == 2) APPLY(4, 2); else APPLY(4, 4);
+  } else if (g->mf <= 6) {  // 2 levels per group at most: 6 or 8 gathers per level
+    if (g->tune_unroll == 1) APPLY(6, 1); else APPLY(6, 2);
+  } else {
+    if (g->tune_unroll == 1) APPLY(8, 1); else APPLY(8, 2);
   }
 #undef APPLY
 #undef APPLYV

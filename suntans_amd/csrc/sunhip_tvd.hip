@@ -23,16 +23,18 @@
 using namespace sunhip;
 
 
+template <int NF>
 __global__ __launch_bounds__(256) void k_tvd_sums(Geo G, TvdArgs A) {
-  tvd_sums_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+  tvd_sums_tile<NF>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
   tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
+template <int NF>
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  update_column_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+  update_column_tile<NF>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 namespace sunhip {
@@ -74,7 +76,10 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
   if (rc) return rc;
   const Geo G = g->geo();
   int ps = prof_begin(g, PK_TSUMS);
-  k_tvd_sums<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
+  with_nf(g->mf, [&](auto nf) {
+    k_tvd_sums<decltype(nf)::value><<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
+    return 0;
+  });
   prof_end(g, ps);
   ps = prof_begin(g, PK_TFACES);
   k_tvd_faces<<<(g->Net + 3) / 4, 256, 0, g->stream>>>(G, A);
@@ -94,7 +99,10 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
   HIPCHK(hipMemcpyAsync(const_cast<double *>(A.s_old), A.s, g->fields.at(scal).n * sizeof(double),
                         hipMemcpyDeviceToDevice, g->stream));
   const int ps = prof_begin(g, PK_TCOLUMN);
-  k_update_column<<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), A);
+  with_nf(g->mf, [&](auto nf) {
+    k_update_column<decltype(nf)::value><<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), A);
+    return 0;
+  });
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g->stream));

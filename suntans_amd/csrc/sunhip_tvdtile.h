@@ -41,7 +41,8 @@ struct TvdArgs {
 
 // pass 1 (main/tvd.c:46-65), cells of celldist[0..2).  The faces' level-
 // invariant data (edge base, df, normal, neighbour base and depth) are read
-// once per column.
+// once per column.  NF: faces per cell at most.
+template <int NF>
 __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s];
@@ -49,11 +50,11 @@ __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, in
   const int NS = G.NS, nf = G.nf[s];
   const uint32_t base = (uint32_t)(G.toff[t] + l);
   const double th = A.theta;
-  uint32_t eb[SUNHIP_MAXF], nbb[SUNHIP_MAXF];
-  double dfe[SUNHIP_MAXF];
-  int nrm[SUNHIP_MAXF], nkn[SUNHIP_MAXF];
+  uint32_t eb[NF], nbb[NF];
+  double dfe[NF];
+  int nrm[NF], nkn[NF];
 #pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
+  for (int f = 0; f < NF; f++) {
     if (f < nf) {
       const int nbs = G.nbs[f * NS + s];
       eb[f] = (uint32_t)G.eb[f * NS + s];
@@ -70,7 +71,7 @@ __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, in
     const double si = at32(A.s, idx);
     double sq = 0, sqc = 0;
 #pragma unroll
-    for (int f = 0; f < SUNHIP_MAXF; f++) {
+    for (int f = 0; f < NF; f++) {
       if (f < nf) {
         const uint32_t ei = eb[f] + k64;
         const double un = nrm[f] * (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei));
@@ -197,10 +198,11 @@ struct FaceC {
   double cf;
 };
 
-__device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int s, double Ac, FaceC (&fc)[SUNHIP_MAXF]) {
+template <int NF>
+__device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int s, double Ac, FaceC (&fc)[NF]) {
   const int NS = G.NS, nf = G.nf[s];
 #pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
+  for (int f = 0; f < NF; f++) {
     if (f < nf) {
       const int es = G.es[f * NS + s];
       fc[f].eb = (uint32_t)G.eb[f * NS + s];
@@ -218,11 +220,12 @@ __device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int 
 }
 
 // horizontal TVD flux divergence at level k (main/scalars.c:251-290)
-__device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[SUNHIP_MAXF], int nf, int k) {
+template <int NF>
+__device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF], int nf, int k) {
   double acc = 0;
   const uint32_t k64 = (uint32_t)k * 64u;
 #pragma unroll
-  for (int f = 0; f < SUNHIP_MAXF; f++) {
+  for (int f = 0; f < NF; f++) {
     if (f >= nf || k >= fc[f].nke) continue;
     const uint32_t ei = fc[f].eb + k64;
     const double ut2 = at32(A.utmp2, ei);
@@ -237,6 +240,7 @@ __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[SUNH
   return acc;
 }
 
+template <int NF>
 __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, nk = G.nk[s];
@@ -245,7 +249,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   const int64_t base = G.toff[t] + l;
   const double Ac = G.Ac[s], theta = A.theta, dt = A.dt;
   const Col dzz{A.dzz, base, nk}, dzo{A.dzzold, base, nk}, so{A.s_old, base, nk};
-  FaceC fc[SUNHIP_MAXF];
+  FaceC fc[NF];
   face_consts(G, A, s, Ac, fc);
   const int nfs = G.nf[s];
   int ktop;

@@ -55,6 +55,8 @@ def state_lib() -> C.CDLL:
         L = _load("libsunstate.so")
         L.sun_grid_tri.restype = _vp
         L.sun_grid_tri.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        L.sun_grid_polygons.restype = _vp
+        L.sun_grid_polygons.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, _ip, _ip, _dp, _dp]
         L.sun_grid_vertical.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int]
         L.sun_phys_alloc.restype = _vp
         L.sun_phys_alloc.argtypes = [_vp]

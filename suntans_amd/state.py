@@ -29,19 +29,31 @@ class State:
     """One grid with its physical state and run properties."""
 
     def __init__(self, nx: int, ny: int, Lx: float, Ly: float, Nkmax: int, dz, depth,
-                 stairstep: int = 0):
+                 stairstep: int = 0, planform: str = "tri", split_every: int = 0, maxfaces=None):
+        """planform "tri": nx*ny rectangles split into two triangles
+        (sun_grid_tri); "quad" / "hex": rectangles or hexagons, some split
+        when split_every > 0, built by sun_grid_polygons (suntans_amd.planform)."""
         L = native.state_lib()
         _bind(L)
         self._L = L
-        self.g = L.sun_grid_tri(int(nx), int(ny), float(Lx), float(Ly))
+        if planform == "tri":
+            self.g = L.sun_grid_tri(int(nx), int(ny), float(Lx), float(Ly))
+        else:
+            from . import planform as pf
+            P = pf.make(planform, nx, ny, Lx, Ly, split_every, maxfaces)
+            cells = np.ascontiguousarray(P.cells.ravel())
+            self.g = L.sun_grid_polygons(len(P.xp), native.dptr(P.xp), native.dptr(P.yp), P.Nc, P.maxfaces,
+                                         native.iptr(P.nfaces), native.iptr(cells), native.dptr(P.xv),
+                                         native.dptr(P.yv))
         if not self.g:
-            raise RuntimeError("sun_grid_tri failed")
+            raise RuntimeError(f"building the {planform} planform failed")
+        self.planform = planform
         self.nx, self.ny, self.Lx, self.Ly = nx, ny, Lx, Ly
         self.p = None
         self.prop = None
         self._owns_grid = True
         dz = np.ascontiguousarray(np.broadcast_to(np.asarray(dz, np.float64), (Nkmax,)))
-        Nc = 2 * nx * ny
+        Nc = int(L.sun_grid_scalar(self.g, b"Nc"))
         if callable(depth):
             depth = depth(self.get("xv"), self.get("yv"))
         dv = np.ascontiguousarray(np.broadcast_to(np.asarray(depth, np.float64), (Nc,)))
@@ -58,6 +70,7 @@ class State:
         other._L = self._L
         other.g = self.g
         other.nx, other.ny, other.Lx, other.Ly = self.nx, self.ny, self.Lx, self.Ly
+        other.planform = self.planform
         other.p = self._L.sun_phys_clone(self.g, self.p)
         other.prop = self._L.sun_prop_default()
         self._L.sun_prop_copy(self.prop, other.prop)

@@ -2,7 +2,8 @@
 
 Run where the reference was compiled from its own sources (oracle/_ref):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py hex_tiny   # the named ones
 
 Every fixture is data only: the grid parameters and every input field after
 set-up, and the outputs the real reference computed from them (CGSolveQ with
@@ -81,13 +82,20 @@ def main():
     specs = [(f"tiny{s}", dict(kind="tiny", seed=s)) for s in range(4)]
     specs.append(("tiny_wide", dict(kind="tiny", seed=11, nx=23, ny=9, Nkmax=13)))
     specs.append(("lockexchange", dict(kind="lock_exchange", seed=7)))
+    # other planforms (sun_grid_polygons)
+    specs.append(("tiny_quadtri", dict(kind="tiny", seed=22, nx=7, ny=5, planform="quad", split_every=3)))
+    specs.append(("tiny_hexquad7", dict(kind="tiny", seed=24, nx=7, ny=6, planform="hex", split_every=4,
+                                        maxfaces=7)))
+    only = set(sys.argv[1:])
     for name, spec in specs:
+        if only and name not in only:
+            continue
         kw = {k: v for k, v in spec.items() if k != "kind"}
         make = (lambda kw=kw, spec=spec: getattr(cases, spec["kind"])(**kw))
         st = make()
         outs = outputs(make, ref)
         if spec["kind"] == "tiny":
-            data = capture(st)
+            data = capture(st, kw.get("split_every", 0), kw.get("maxfaces") or 0)
             data.update(outs)
         else:
             # large case: inputs are regenerated from the seed (their digest is

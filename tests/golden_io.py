@@ -11,9 +11,15 @@ PROPS = ["dt", "theta", "epsilon", "qepsilon", "resnorm", "grav", "kappa_s", "qm
          "qprecond", "hprecond", "TVDsalt", "n"]
 
 
-def capture(st: State) -> dict:
+PLANFORMS = ["tri", "quad", "hex"]
+
+
+def capture(st: State, split_every: int = 0, maxfaces: int = 0) -> dict:
+    """split_every/maxfaces: the planform arguments the state was built with."""
     d = {"grid": np.array([st.nx, st.ny, st.Lx, st.Ly, st.scalar("Nkmax")], np.float64),
          "dz": st.get("dz"), "dv": st.get("dv")}
+    if st.planform != "tri":
+        d["planform"] = np.array([PLANFORMS.index(st.planform), split_every, maxfaces], np.int32)
     for f in INPUT_FIELDS:
         d["in_" + f] = st.get(f)
     for f in INT_FIELDS:
@@ -35,7 +41,11 @@ def inputs_digest(st: State) -> bytes:
 
 def restore(data) -> State:
     nx, ny, Lx, Ly, Nkmax = data["grid"]
-    st = State(int(nx), int(ny), float(Lx), float(Ly), int(Nkmax), data["dz"], data["dv"])
+    kw = {}
+    if "planform" in data:
+        kind, split, mf = (int(v) for v in data["planform"])
+        kw = dict(planform=PLANFORMS[kind], split_every=split, maxfaces=mf or None)
+    st = State(int(nx), int(ny), float(Lx), float(Ly), int(Nkmax), data["dz"], data["dv"], **kw)
     for f in INT_FIELDS:
         st.set_ints(f, data["int_" + f])
     for f in INPUT_FIELDS:

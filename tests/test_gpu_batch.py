@@ -30,6 +30,11 @@ def members():
     out.append(cases.lock_exchange(seed=7))
     out.append(cases.ensemble_member(5, 6000, 2.5, 24))
     out.append(cases.ensemble_member(6, 9000, 1.2, 40))
+    # other planforms: a batch runs the face-count class of its largest
+    # maxfaces, so triangles also run the 8-face kernels when mixed with these
+    out.append(cases.tiny(22, nx=12, ny=10, Nkmax=9, planform="quad", split_every=3))
+    out.append(cases.tiny(23, nx=12, ny=10, Nkmax=9, planform="hex"))
+    out.append(cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7))
     return out
 
 
@@ -91,9 +96,10 @@ def test_batch_alone_equals_mix(gpu_available, states):
 
 
 @pytest.mark.parametrize("tuning", [(1, 0, 2), (2, 4, 4), (4, 8, 8), (4, 0, 16)])
-def test_batch_independent_of_launch_configuration(gpu_available, states, tuning):
-    ref = _solve_q(states, [0, 3, 5], (4, 0, 0))
-    got = _solve_q(states, [0, 3, 5], tuning)
+@pytest.mark.parametrize("idx", [(0, 3, 5), (0, 7, 9)], ids=["tri", "polygons"])
+def test_batch_independent_of_launch_configuration(gpu_available, states, tuning, idx):
+    ref = _solve_q(states, list(idx), (4, 0, 0))
+    got = _solve_q(states, list(idx), tuning)
     for a, g in zip(ref, got):
         assert a[1] == g[1]
         assert np.array_equal(a[0], g[0])
@@ -182,7 +188,7 @@ def test_batch_update_scalars_bitwise(gpu_available, states):
     UpdateScalars of each member bit for bit."""
     o = _impls.oracle()
     sts = [st.clone() for st in states]
-    schemes = [3, 0, 4, 1, 3, 4, 3][:len(sts)]
+    schemes = [3, 0, 4, 1, 3, 4, 3, 4, 1, 3][:len(sts)]
     for st, tvd in zip(sts, schemes):
         st.set_prop(TVDsalt=tvd, n=2)
     dgs = []

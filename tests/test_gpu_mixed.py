@@ -29,6 +29,8 @@ def relerr(a, b):
 MAKERS = [(f"tiny{s}", (lambda s=s: cases.tiny(s))) for s in (0, 1, 2)]
 MAKERS.append(("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)))
 MAKERS.append(("lockexchange", lambda: cases.lock_exchange(seed=7)))
+MAKERS.append(("quadtri", lambda: cases.tiny(22, nx=12, ny=10, Nkmax=9, planform="quad", split_every=3)))
+MAKERS.append(("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7)))
 
 
 def q_residual(o, st, q):

@@ -39,7 +39,18 @@ def _makers():
     out = [(f"tiny{s}", (lambda s=s: cases.tiny(s))) for s in SEEDS]
     out.append(("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)))
     out.append(("lockexchange", _lock))
+    out += POLYGONS
     return out
+
+
+# quadrilateral, hybrid (quads + triangles, hexagons + quads) and hexagonal
+# planforms: the 4-, 6- and 8-face kernel instantiations
+POLYGONS = [
+    ("quad", lambda: cases.tiny(21, nx=13, ny=11, Nkmax=9, planform="quad")),
+    ("quadtri", lambda: cases.tiny(22, nx=12, ny=10, Nkmax=9, planform="quad", split_every=3)),
+    ("hex", lambda: cases.tiny(23, nx=12, ny=10, Nkmax=9, planform="hex")),
+    ("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7)),
+]
 
 
 MAKERS = _makers()
@@ -119,7 +130,10 @@ def test_cgsolveq_converged(gpu_available, name, make, qprecond):
     assert err <= 1e-10
 
 
-@pytest.mark.parametrize("name,make", MAKERS[-2:], ids=[m[0] for m in MAKERS[-2:]])
+FIXED = [m for m in MAKERS if m[0] in ("tiny_wide", "lockexchange", "quadtri", "hexquad7")]
+
+
+@pytest.mark.parametrize("name,make", FIXED, ids=[m[0] for m in FIXED])
 def test_cgsolveq_fixed_iterations(gpu_available, name, make):
     o, d = _impls.oracle(), _impls.dropin()
     a, b = _pair(make)

@@ -29,6 +29,7 @@ def states():
     out.append(cases.lock_exchange(seed=7))
     out.append(cases.ensemble_member(5, 6000, 2.5, 24))
     out.append(cases.ensemble_member(6, 9000, 1.2, 40))
+    out.append(cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7))
     return out
 
 

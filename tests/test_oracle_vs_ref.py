@@ -16,6 +16,11 @@ pytestmark = pytest.mark.skipif(ref is None, reason="reference harness not built
 MAKERS = [(f"tiny{s}", (lambda s=s: cases.tiny(s))) for s in range(6)]
 MAKERS.append(("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)))
 MAKERS.append(("lockexchange", lambda: cases.lock_exchange(seed=7)))
+# quadrilateral, hybrid and hexagonal planforms (sun_grid_polygons)
+MAKERS.append(("quad", lambda: cases.tiny(21, nx=6, ny=5, planform="quad")))
+MAKERS.append(("quadtri", lambda: cases.tiny(22, nx=7, ny=5, planform="quad", split_every=3)))
+MAKERS.append(("hex", lambda: cases.tiny(23, nx=6, ny=5, planform="hex")))
+MAKERS.append(("hexquad7", lambda: cases.tiny(24, nx=7, ny=6, planform="hex", split_every=4, maxfaces=7)))
 IDS = [m[0] for m in MAKERS]
 
 
