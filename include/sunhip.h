@@ -147,7 +147,8 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
  * n[] (SUNHIP_NPROF entries each) with the totals (ms) and launch counts per
  * kernel id and returns the number of ids: q apply, q precond, h apply,
  * h precond, mixed apply, mixed precond, mixed residual, other, TVD face
- * sums, TVD face values, scalar column update. */
+ * sums, TVD face values, scalar column update, Poisson source, Corrector,
+ * Continuity. */
 #define SUNHIP_NPROF 16
 int sunhip_profile(sunhip_grid *g, int on);
 int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n);
@@ -184,6 +185,20 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
 /* One UpdateScalars step of field scal with "wnew","wtmp2","dzz","dzzold",
  * "kappa_tv","u","utmp2","dzf"; writes scal and "Cn". */
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
+
+/* ---- the stencils either side of the pressure solve ------------------------
+ * Solve's nonhydrostatic sequence is ComputeQSource, CGSolveQ, Corrector,
+ * Continuity (main/phys.c:1270-1300).  Bit-identical to the reference. */
+/* ComputeQSource (main/phys.c:2345-2395): the Poisson source into cell field
+ * src from "u", "utmp2", "w", "wtmp2" and "dzf"; "D" <- df/dg on every edge. */
+int sunhip_compute_qsource(sunhip_grid *g, double theta, double dt, const char *src);
+/* Corrector (main/phys.c:2301-2332): with the pressure correction in cell
+ * field qc, "u" and "w" are projected (reading "D" and "dzz") and cell field
+ * q, the nonhydrostatic pressure, accumulates qc. */
+int sunhip_corrector(sunhip_grid *g, double dt, const char *qc, const char *q);
+/* Continuity (main/phys.c:3814-3844): w-type field w from "u", "utmp2",
+ * "wtmp2" and "dzf". */
+int sunhip_continuity(sunhip_grid *g, double theta, const char *w);
 
 /* Synchronise the context's stream. */
 int sunhip_sync(sunhip_grid *g);
@@ -242,6 +257,13 @@ int sunhip_batch_q_solve_mixed(sunhip_batch *b, sunhip_grid **g, int n, const su
 /* One scalar step (as sunhip_update_scalars) of each of n grids, sp[i]. */
 int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **g, int n, const sunhip_scalar_params *sp,
                                 const char *scal);
+/* The stencils of each of n grids in one launch per stencil (theta[i],
+ * dt[i]): the Poisson source (as sunhip_compute_qsource), or Corrector then
+ * Continuity (as sunhip_corrector followed by sunhip_continuity). */
+int sunhip_batch_compute_qsource(sunhip_batch *b, sunhip_grid **g, int n, const double *theta, const double *dt,
+                                 const char *src);
+int sunhip_batch_correct(sunhip_batch *b, sunhip_grid **g, int n, const double *theta, const double *dt,
+                         const char *qc, const char *q, const char *w);
 /* Launch configuration: unroll (1, 2, 4; 0 = autotune), blocks per CU of the
  * persistent launch (0 = one work item per wavefront), iterations per graph
  * chunk (0 = automatic).  sunhip_batch_tuning reports the configuration in

@@ -6,7 +6,9 @@
  *
  * Semantics follow the reference function each declaration replaces.  The
  * caller owns every array; results are written to the same rows the
- * reference writes (computational cells / edges).  Scratch rows the
+ * reference writes (computational cells / edges).  ComputeQSource,
+ * Corrector and Continuity, the stencils either side of CGSolveQ, run on the
+ * device too.  Scratch rows the
  * reference clobbers as a side effect (src, stmp2, stmp3, uold, wtmp, qtmp,
  * a..d, hold, htmp*, hcoef, hfcoef) are left untouched: the reference reads
  * none of them afterwards.  Unsupported configurations (numprocs>1,
@@ -34,6 +36,12 @@ void OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *grid, physT *ph
 void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, propT *prop);
 /* main/phys.c:3335 (static OperatorH; prototype main/phys.c:65-66) */
 void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *phys, propT *prop);
+/* main/phys.c:2345 (static ComputeQSource; prototype main/phys.c:43-44) */
+void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs);
+/* main/phys.c:2301 (static Corrector; prototype main/phys.c:41-42) */
+void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
+/* main/phys.c:3814 (Continuity; prototypes main/phys.c:73-74) */
+void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop);
 /* main/tvd.h:27-28 (definition main/tvd.c:36) */
 void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal,
                            int TVD, MPI_Comm comm, int myproc);

@@ -374,6 +374,54 @@ void oracle_ComputeQSource(REAL **src, gridT *g, physT *p, propT *prop) {
   for (int j = 0; j < g->Ne; j++) p->D[j] = g->df[j] / g->dg[j];
 }
 
+/* main/phys.c:2301-2332.  The horizontal velocity of computational interior
+ * edges with D != 0 and more than one layer takes the gradient of the
+ * pressure correction; the vertical velocity its vertical gradient (with the
+ * surface ghost value -qc[ctop]); the pressure of multi-layer columns
+ * accumulates the correction. */
+void oracle_Corrector(REAL **qc, gridT *g, physT *p, propT *prop) {
+  const double dt = prop->dt;
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp];
+    if (p->D[j] == 0 || g->etop[j] >= g->Nke[j] - 1) continue;
+    const int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) p->u[j][k] -= dt / g->dg[j] * (qc[a][k] - qc[b][k]);
+  }
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], kt = g->ctop[i];
+    for (int k = kt + 1; k < g->Nk[i]; k++)
+      p->w[i][k] -= 2.0 * dt / (g->dzz[i][k - 1] + g->dzz[i][k]) * (qc[i][k - 1] - qc[i][k]);
+    p->w[i][kt] += 2.0 * dt / g->dzz[i][kt] * qc[i][kt];
+  }
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    if (g->ctop[i] >= g->Nk[i] - 1) continue;
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) p->q[i][k] += qc[i][k];
+  }
+}
+
+/* main/phys.c:3814-3844.  Per computational column, from the bottom (w = 0
+ * at Nk) up to ctop: the theta-weighted discrete continuity equation, with
+ * the old vertical velocity wtmp2 and the face fluxes of u (new) and utmp2
+ * (old); every other level of the column is zero. */
+void oracle_Continuity(REAL **w, gridT *g, physT *p, propT *prop) {
+  const double th = prop->theta;
+  const int mf = g->maxfaces;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = 0; k <= g->Nk[i]; k++) w[i][k] = 0;
+    for (int k = g->Nk[i] - 1; k >= g->ctop[i]; k--) {
+      w[i][k] = w[i][k + 1] - (1 - th) / th * (p->wtmp2[i][k] - p->wtmp2[i][k + 1]);
+      for (int f = 0; f < g->nfaces[i]; f++) {
+        const int ne = g->face[i * mf + f];
+        if (k < g->Nke[ne])
+          w[i][k] -= (th * p->u[ne][k] + (1 - th) * p->utmp2[ne][k]) * g->df[ne] * g->normal[i * mf + f] /
+                     g->Ac[i] / th * g->dzf[ne][k];
+      }
+    }
+  }
+}
+
 /* main/tvd.c:36-120.  Pass 1 builds, per cell-layer, the inflow volume flux
  * sumQ and the inflow-weighted scalar jump sumQC (in gradSx/gradSy); pass 2
  * limits the face values of each wet interior edge-layer by Psi of the

@@ -41,6 +41,8 @@ void oracle_OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *g, physT
 void oracle_HPreconditioner(REAL *x, REAL *y, gridT *g, physT *p, propT *prop);
 void oracle_CGSolve(gridT *g, physT *p, propT *prop);
 void oracle_ComputeQSource(REAL **src, gridT *g, physT *p, propT *prop);
+void oracle_Corrector(REAL **qc, gridT *g, physT *p, propT *prop);
+void oracle_Continuity(REAL **w, gridT *g, physT *p, propT *prop);
 void oracle_HorizontalFaceScalars(gridT *g, physT *p, propT *prop, REAL **scal, REAL **boundary_scal,
                                   int TVD);
 void oracle_GetApAm(REAL *ap, REAL *am, REAL *wp, REAL *wm, REAL *Cp, REAL *Cm, REAL *rp, REAL *rm,

@@ -355,6 +355,87 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   if (TVDscheme) download_face_values(C, grid, phys, W);
 }
 
+/* write back rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
+static void unpack_cells_w(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Nc + 1));
+  start[0] = 0;
+  for (int i = 0; i < g->Nc; i++) start[i + 1] = start[i] + (size_t)g->Nk[i] + 1;
+  for (int ip = g->celldist[lo]; ip < g->celldist[hi]; ip++) {
+    const int i = g->cellp[ip];
+    memcpy(rows[i], c->flat + start[i], ((size_t)g->Nk[i] + 1) * sizeof(double));
+  }
+  free(start);
+}
+/* write back levels [ctop, Nk) of every cell */
+static void unpack_cells_wet(ctx_t *c, const gridT *g, REAL **rows) {
+  size_t o = 0;
+  for (int i = 0; i < g->Nc; i++) {
+    const int kt = g->ctop[i];
+    if (kt < g->Nk[i]) memcpy(rows[i] + kt, c->flat + o + kt, (size_t)(g->Nk[i] - kt) * sizeof(double));
+    o += (size_t)g->Nk[i];
+  }
+}
+/* write back every edge row */
+static void unpack_edges(ctx_t *c, const gridT *g, REAL **rows) {
+  size_t o = 0;
+  for (int j = 0; j < g->Ne; j++) {
+    memcpy(rows[j], c->flat + o, (size_t)g->Nkc[j] * sizeof(double));
+    o += (size_t)g->Nkc[j];
+  }
+}
+
+void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs) {
+  static const char *W = "ComputeQSource";
+  (void)myproc;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_edges(C, grid, "utmp2", phys->utmp2, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  upload_cells(C, grid, "w", phys->w, 1, W);
+  upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
+  check(sunhip_compute_qsource(C->dev, prop->theta, prop->dt, "src"), W);
+  check(sunhip_download(C->dev, "src", C->flat), W);
+  unpack_cells_wet(C, grid, src);
+  check(sunhip_download(C->dev, "D", C->flat), W);
+  memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
+}
+
+void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
+  static const char *W = "Corrector";
+  (void)myproc; (void)comm;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "q", qc, 0, W);
+  upload_cells(C, grid, "qp", phys->q, 0, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_cells(C, grid, "w", phys->w, 1, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  check(sunhip_upload(C->dev, "D", phys->D), W);
+  check(sunhip_corrector(C->dev, prop->dt, "q", "qp"), W);
+  check(sunhip_download(C->dev, "u", C->flat), W);
+  unpack_edges(C, grid, phys->u);
+  check(sunhip_download(C->dev, "w", C->flat), W);
+  unpack_cells_w(C, grid, phys->w, 0, 1);
+  check(sunhip_download(C->dev, "qp", C->flat), W);
+  unpack_cells(C, grid, phys->q, 0, 1);
+}
+
+void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
+  static const char *W = "Continuity";
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_edges(C, grid, "utmp2", phys->utmp2, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
+  check(sunhip_continuity(C->dev, prop->theta, "w"), W);
+  check(sunhip_download(C->dev, "w", C->flat), W);
+  unpack_cells_w(C, grid, w, 0, 1);
+}
+
 int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) {
   if (which < 0 || which > 1 || !g_last[which].valid) return -1;
   if (niters) *niters = g_last[which].st.niters;

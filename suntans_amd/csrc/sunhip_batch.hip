@@ -222,6 +222,44 @@ __global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restri
   }
 }
 
+// The stencils either side of the pressure solve over a work list of
+// (member, tile), tiles covering max(Nt, Net) of each member.
+struct BSMember {
+  Geo G;
+  StepArgs A;
+};
+
+template <int NF>
+__global__ __launch_bounds__(256) void kb_qsource(const BSMember *__restrict__ M, const int2 *__restrict__ W, int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BSMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    qsource_tile<NF>(B.G, B.A, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
+__global__ __launch_bounds__(256) void kb_corrector(const BSMember *__restrict__ M, const int2 *__restrict__ W,
+                                                    int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BSMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    corrector_tile(B.G, B.A, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
+template <int NF>
+__global__ __launch_bounds__(256) void kb_continuity(const BSMember *__restrict__ M, const int2 *__restrict__ W,
+                                                     int n) {
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < n; w += stride) {
+    const int2 it = W[w];
+    const BSMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
+    continuity_tile<NF>(B.G, B.A, __builtin_amdgcn_readfirstlane(it.y), l);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -328,6 +366,8 @@ struct sunhip_batch {
   // polling
   DevBuf<BTMember> ttvd;
   DevBuf<int2> wtc, wte;
+  DevBuf<BSMember> tstep;  // stencils either side of the pressure solve
+  DevBuf<int2> wstep;
   DevBuf<CGScal *> d_scp;
   DevBuf<CGScal> d_scv;
   HostBuf<CGScal> h_scv;
@@ -644,7 +684,7 @@ int sunhip_batch_destroy(sunhip_batch *b) {
   b->tq64.release(); b->tq32.release(); b->th64.release();
   for (int k = 0; k < T_NKIND; k++) { b->rng[k].release(); b->work[k].release(); b->part[k].release(); }
   b->tres.release(); b->rres.release(); b->wres.release(); b->pres.release();
-  b->ttvd.release(); b->wtc.release(); b->wte.release();
+  b->ttvd.release(); b->wtc.release(); b->wte.release(); b->tstep.release(); b->wstep.release();
   b->d_scp.release(); b->d_scv.release(); b->h_scv.release(); b->h_stage.release(); b->h_tab.release();
   hipEventDestroy(b->ev0);
   hipEventDestroy(b->ev1);
@@ -1040,6 +1080,63 @@ int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **gs, int n, const 
   });
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// Member table and work list of a stencil launch; returns the number of work
+// items (0: nothing to launch) or a negative error.
+static int step_tables(sunhip_batch *b, sunhip_grid **gs, int n, const double *theta, const double *dt,
+                       const char *qc, const char *q, const char *w, const char *src, int *mf) {
+  int rc;
+  std::vector<BSMember> tab(n);
+  std::vector<int2> wk;
+  *mf = 3;
+  for (int i = 0; i < n; i++) {
+    sunhip_grid *g = gs[i];
+    if ((rc = sunhip_batch_attach(b, g))) return rc;
+    tab[i].G = g->geo();
+    if ((rc = step_args(g, theta[i], dt[i], qc, q, w, src, &tab[i].A))) return rc;
+    for (int t = 0; t < std::max(g->Nt, g->Net); t++) wk.push_back(make_int2(i, t));
+    *mf = std::max(*mf, g->mf);
+  }
+  if (n == 0) return 0;
+  HIPCHK(hipStreamSynchronize(b->stream));  // the staging area is reused
+  const size_t bytes = tab.size() * sizeof(BSMember) + wk.size() * sizeof(int2) + 1024;
+  if ((rc = b->h_tab.ensure(bytes)) || (rc = b->tstep.ensure(tab.size())) || (rc = b->wstep.ensure(wk.size() + 1)))
+    return rc;
+  size_t off = 0;
+  rc = table_upload(b, b->tstep.d, tab.data(), tab.size() * sizeof(BSMember), off) |
+       table_upload(b, b->wstep.d, wk.data(), wk.size() * sizeof(int2), off);
+  if (rc) return rc;
+  return (int)wk.size();
+}
+
+int sunhip_batch_compute_qsource(sunhip_batch *b, sunhip_grid **gs, int n, const double *theta, const double *dt,
+                                 const char *src) {
+  HIPCHK(hipSetDevice(b->device));
+  int mf;
+  const int nw = step_tables(b, gs, n, theta, dt, "q", "qp", "w", src, &mf);
+  if (nw <= 0) return nw;
+  with_nf(mf, [&](auto nf) {
+    kb_qsource<decltype(nf)::value><<<(nw + BW - 1) / BW, 256, 0, b->stream>>>(b->tstep.d, b->wstep.d, nw);
+    return 0;
+  });
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int sunhip_batch_correct(sunhip_batch *b, sunhip_grid **gs, int n, const double *theta, const double *dt,
+                         const char *qc, const char *q, const char *w) {
+  HIPCHK(hipSetDevice(b->device));
+  int mf;
+  const int nw = step_tables(b, gs, n, theta, dt, qc, q, w, "src", &mf);
+  if (nw <= 0) return nw;
+  kb_corrector<<<(nw + BW - 1) / BW, 256, 0, b->stream>>>(b->tstep.d, b->wstep.d, nw);
+  with_nf(mf, [&](auto nf) {
+    kb_continuity<decltype(nf)::value><<<(nw + BW - 1) / BW, 256, 0, b->stream>>>(b->tstep.d, b->wstep.d, nw);
+    return 0;
+  });
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -178,7 +178,8 @@ int prof_begin(sunhip_grid *g, int kind);
 int poll_chunk(sunhip_grid *g, const CGScal *d, int &slot, bool &pending, bool *done);
 void prof_end(sunhip_grid *g, int slot);
 enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, PK_MPREC = 5, PK_RESID = 6,
-       PK_OTHER = 7, PK_TSUMS = 8, PK_TFACES = 9, PK_TCOLUMN = 10, PK_COUNT = 11 };
+       PK_OTHER = 7, PK_TSUMS = 8, PK_TFACES = 9, PK_TCOLUMN = 10, PK_QSOURCE = 11, PK_CORRECT = 12,
+       PK_CONTINUITY = 13, PK_COUNT = 14 };
 }  // namespace sunhip
 
 namespace sunhip {

@@ -6,6 +6,7 @@
 #include "sunhip_qtile.h"
 #include "sunhip_htile.h"
 #include "sunhip_tvdtile.h"
+#include "sunhip_steptile.h"
 
 namespace sunhip {
 
@@ -54,5 +55,11 @@ int h_end64(sunhip_grid *g, const HSolver<double> &S);
 
 // Arguments of one grid's scalar step (fields created on demand); no launch.
 int tvd_args(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, TvdArgs *T, ColArgs *C);
+
+// Arguments of one grid's source / Corrector / Continuity stencils: qc, q and
+// w name the correction, pressure and vertical velocity fields, src the
+// source (fields created on demand); no launch.
+int step_args(sunhip_grid *g, double theta, double dt, const char *qc, const char *q, const char *w,
+              const char *src, StepArgs *out);
 
 }  // namespace sunhip

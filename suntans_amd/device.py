@@ -107,6 +107,11 @@ def lib() -> C.CDLL:
         L.sunhip_batch_q_solve_mixed.argtypes = [vp, pp, C.c_int, C.POINTER(CGParams), C.c_double, C.c_int,
                                                  C.POINTER(CGStats), _ip]
         L.sunhip_batch_update_scalars.argtypes = [vp, pp, C.c_int, C.POINTER(ScalarParams), C.c_char_p]
+        L.sunhip_compute_qsource.argtypes = [vp, C.c_double, C.c_double, C.c_char_p]
+        L.sunhip_corrector.argtypes = [vp, C.c_double, C.c_char_p, C.c_char_p]
+        L.sunhip_continuity.argtypes = [vp, C.c_double, C.c_char_p]
+        L.sunhip_batch_compute_qsource.argtypes = [vp, pp, C.c_int, _dp, _dp, C.c_char_p]
+        L.sunhip_batch_correct.argtypes = [vp, pp, C.c_int, _dp, _dp, C.c_char_p, C.c_char_p, C.c_char_p]
         L.sunhip_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.sunhip_batch_tuning.argtypes = [vp, _ip, _ip, _ip]
         _lib = L
@@ -231,6 +236,18 @@ class DeviceGrid:
         S = ScalarParams(theta, dt, kappa, TVD, step, 0, 0)
         check(lib().sunhip_update_scalars(self.h, C.byref(S), scal.encode()), "sunhip_update_scalars")
 
+    def compute_qsource(self, theta, dt, src="src"):
+        """ComputeQSource: src from u, utmp2, w, wtmp2, dzf; D <- df/dg."""
+        check(lib().sunhip_compute_qsource(self.h, theta, dt, src.encode()), "sunhip_compute_qsource")
+
+    def corrector(self, dt, qc="q", q="qp"):
+        """Corrector: u, w projected with the correction qc; q += qc."""
+        check(lib().sunhip_corrector(self.h, dt, qc.encode(), q.encode()), "sunhip_corrector")
+
+    def continuity(self, theta, w="w"):
+        """Continuity: w from u, utmp2, wtmp2, dzf."""
+        check(lib().sunhip_continuity(self.h, theta, w.encode()), "sunhip_continuity")
+
     def profile(self, on: bool):
         lib().sunhip_profile(self.h, int(on))
 
@@ -240,7 +257,7 @@ class DeviceGrid:
         n = (C.c_int * 16)()
         k = lib().sunhip_kernel_times(self.h, ms, n)
         names = ("qapply", "qprec", "happly", "hprec", "mixed_apply", "mixed_prec", "resid64", "other",
-                 "tvd_sums", "tvd_faces", "tvd_column")
+                 "tvd_sums", "tvd_faces", "tvd_column", "qsource", "corrector", "continuity")
         return {names[i]: (ms[i], n[i]) for i in range(min(k, len(names))) if n[i] > 0}
 
     def copy_probe(self, nbytes: int, reps: int):
@@ -348,6 +365,22 @@ class Batch:
         check(lib().sunhip_batch_update_scalars(self.h, self._grids(dgs), len(dgs), sp, scal.encode()),
               "sunhip_batch_update_scalars")
 
+    def compute_qsource(self, dgs, theta, dt, src="src"):
+        """ComputeQSource of every grid in one launch; theta/dt per grid or scalars."""
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float64), (len(dgs),)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(dgs),)))
+        check(lib().sunhip_batch_compute_qsource(self.h, self._grids(dgs), len(dgs), th.ctypes.data_as(_dp),
+                                                 d.ctypes.data_as(_dp), src.encode()),
+              "sunhip_batch_compute_qsource")
+
+    def correct(self, dgs, theta, dt, qc="q", q="qp", w="w"):
+        """Corrector then Continuity of every grid, one launch each."""
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float64), (len(dgs),)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(dgs),)))
+        check(lib().sunhip_batch_correct(self.h, self._grids(dgs), len(dgs), th.ctypes.data_as(_dp),
+                                         d.ctypes.data_as(_dp), qc.encode(), q.encode(), w.encode()),
+              "sunhip_batch_correct")
+
     def set_tuning(self, unroll=0, blocks_per_cu=0, chunk=0):
         check(lib().sunhip_batch_set_tuning(self.h, unroll, blocks_per_cu, chunk), "sunhip_batch_set_tuning")
 
@@ -362,6 +395,15 @@ def member_upload(dg, st, which=("q", "src", "dzz", "dzf", "D")):
     names = {"q": "qc", "src": "stmp", "dzz": "dzz", "dzf": "dzf", "D": "D", "h": "h", "htmp": "htmp"}
     for n in which:
         dg.upload(n, st.get(names[n]))
+
+
+def member_upload_step(dg, st):
+    """Upload a State's inputs of the stencils either side of the pressure
+    solve (ComputeQSource, Corrector, Continuity): velocities, flux heights,
+    layer thicknesses, D and the nonhydrostatic pressure (as "qp")."""
+    for n in ("u", "utmp2", "w", "wtmp2", "dzf", "dzz", "D"):
+        dg.upload(n, st.get(n))
+    dg.upload("qp", st.get("q"))
 
 
 def member_upload_tvd(dg, st):

@@ -34,7 +34,7 @@ class _Cpu:
             "CGSolveQ": [_vp] * 6, "CGSolve": [_vp] * 3, "QCoefficients": [_vp] * 6,
             "OperatorQ": [_vp] * 7, "Preconditioner": [_vp] * 6, "HCoefficients": [_vp] * 5,
             "OperatorH": [_vp] * 7, "ComputeQSource": [_vp] * 4, "ConditionQ": [_vp] * 4,
-            "OperatorQC": [_vp] * 8,
+            "OperatorQC": [_vp] * 8, "Corrector": [_vp] * 4, "Continuity": [_vp] * 4,
             "HorizontalFaceScalars": [_vp] * 5 + [C.c_int],
             "TriSolve": [_vp] * 5 + [C.c_int],
         }.items():
@@ -68,6 +68,15 @@ class _Cpu:
 
     def operatorh(self, st, x="htmp2", y="htmp3"):
         self.fn("OperatorH")(st.ptr(x), st.ptr(y), st.ptr("hcoef"), st.ptr("hfcoef"), st.g, st.p, st.prop)
+
+    def compute_qsource(self, st, src="stmp"):
+        self.fn("ComputeQSource")(st.ptr(src), st.g, st.p, st.prop)
+
+    def corrector(self, st, qc="qc"):
+        self.fn("Corrector")(st.ptr(qc), st.g, st.p, st.prop)
+
+    def continuity(self, st, w="w"):
+        self.fn("Continuity")(st.ptr(w), st.g, st.p, st.prop)
 
     def face_scalars(self, st, tvd):
         self.fn("HorizontalFaceScalars")(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd))
@@ -158,6 +167,15 @@ class DropinImpl:
 
     def operatorh(self, st, x="htmp2", y="htmp3"):
         self.L.OperatorH(st.ptr(x), st.ptr(y), st.ptr("hcoef"), st.ptr("hfcoef"), st.g, st.p, st.prop)
+
+    def compute_qsource(self, st, src="stmp"):
+        self.L.ComputeQSource(st.ptr(src), st.g, st.p, st.prop, 0, 1)
+
+    def corrector(self, st, qc="qc"):
+        self.L.Corrector(st.ptr(qc), st.g, st.p, st.prop, 0, 1, 0)
+
+    def continuity(self, st, w="w"):
+        self.L.Continuity(st.ptr(w), st.g, st.p, st.prop)
 
     def face_scalars(self, st, tvd):
         self.L.HorizontalFaceScalars(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd), 0, 0)

@@ -177,3 +177,53 @@ def test_still_water_keeps_any_scalar(tvd):
     s0 = st.get("s")
     o.update_salinity(st)
     assert np.allclose(st.get("s"), s0, rtol=1e-15, atol=0)
+
+
+STEP_CASES = [
+    ("tiny3", lambda **kw: cases.tiny(3, **kw)),
+    ("quadtri", lambda **kw: cases.tiny(22, nx=7, ny=5, planform="quad", split_every=3, **kw)),
+    ("hexquad7", lambda **kw: cases.tiny(24, nx=7, ny=6, planform="hex", split_every=4, maxfaces=7, **kw)),
+]
+
+
+@pytest.mark.parametrize("name,make", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_continuity_zeroes_the_source(name, make):
+    """w from Continuity makes the discrete divergence that ComputeQSource
+    measures vanish: the two are the same theta-weighted flux balance."""
+    o = _impls.oracle()
+    st = make()
+    m = st.wet_mask()
+    o.compute_qsource(st)
+    before = np.abs(st.get("stmp")[m]).max()
+    o.continuity(st)
+    o.compute_qsource(st)
+    after = np.abs(st.get("stmp")[m]).max()
+    assert before > 1.0
+    assert after <= 1e-14 * before, (before, after)
+
+
+@pytest.mark.parametrize("name,make", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_projection_zeroes_the_source(name, make):
+    """Corrector with the converged solution of L qc = src removes the
+    divergence: ComputeQSource(u', w') = src - L(qc).  On a flat surface
+    (etop = max ctop) the corrected edge layers are exactly the operator's,
+    so the identity holds to rounding; a wrong sign, factor or surface ghost
+    term in Corrector leaves an O(1) source."""
+    o = _impls.oracle()
+    st = make(move_surface=False)
+    m = st.wet_mask()
+    o.compute_qsource(st)
+    before = np.abs(st.get("stmp")[m]).max()
+    st.set("qc", 0.0)
+    st.set_prop(qepsilon=1e-14, qmaxiters=20000, qprecond=2)
+    o.cgsolveq(st)
+    q0 = st.get("q")
+    o.compute_qsource(st)  # CGSolveQ clobbers its source
+    o.corrector(st)
+    o.compute_qsource(st)
+    after = np.abs(st.get("stmp")[m]).max()
+    assert after <= 1e-12 * before, (before, after)
+    # the pressure accumulates the correction on multi-layer columns
+    cell, k = st.cell_layers()
+    multi = (st.ints("Nk") - st.ints("ctop") > 1)[cell]
+    assert np.array_equal(st.get("q")[m & multi], (q0 + st.get("qc"))[m & multi])

@@ -110,3 +110,47 @@ def test_compute_qsource(name, make):
     ref.fn("ComputeQSource")(b.ptr("stmp"), b.g, b.p, b.prop)
     assert np.array_equal(a.get("stmp"), b.get("stmp"))
     assert np.array_equal(a.get("D"), b.get("D"))
+
+
+def _step_inputs(st, seed):
+    """Random correction and pressure, D zero on a few edges (the branch
+    Corrector skips)."""
+    rng = np.random.default_rng(seed)
+    st.set("qc", rng.normal(size=st.get("qc").shape))
+    st.set("q", rng.normal(size=st.get("q").shape))
+    D = st.get("D")
+    D[rng.choice(len(D), size=max(1, len(D) // 10), replace=False)] = 0.0
+    st.set("D", D)
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
+def test_corrector(name, make):
+    o = _impls.oracle()
+    a = make()
+    _step_inputs(a, 3)
+    b = a.clone()
+    o.corrector(a)
+    ref.corrector(b)
+    for f in ("u", "w", "q"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
+def test_continuity(name, make):
+    o = _impls.oracle()
+    a = make()
+    b = a.clone()
+    o.continuity(a)
+    ref.continuity(b)
+    assert np.array_equal(a.get("w"), b.get("w"))
+
+
+@pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
+def test_compute_qsource_all_planforms(name, make):
+    o = _impls.oracle()
+    a = make()
+    b = a.clone()
+    o.compute_qsource(a)
+    ref.compute_qsource(b)
+    assert np.array_equal(a.get("stmp"), b.get("stmp"))
+    assert np.array_equal(a.get("D"), b.get("D"))
