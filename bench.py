@@ -181,20 +181,31 @@ def build_members(specs, threads: int):
 
 
 def ensemble_leg(args, sims):
-    """64 independent members solved together (FP32 inner PCG in FP64
-    refinement, one persistent-kernel batch) followed by one TVD superbee
-    step each; compared with the FP64 batch and with the same mixed solves
-    one grid at a time."""
+    """64 independent members, each taking the nonhydrostatic part of a time
+    step on the device: the Poisson source of its predicted velocities
+    (ComputeQSource), a mixed-precision pressure solve (FP32 inner PCG in
+    FP64 refinement), Corrector + Continuity (divergence-free u and w), then
+    one TVD superbee scalar step transported by them.  Every stage runs all
+    members in shared launches (one persistent-kernel batch).  Also timed:
+    the solves alone (mixed and FP64 batches, mixed one grid at a time)."""
     from suntans_amd import device
     dgs = [s.dg for s in sims]
     N = sum(s.N for s in sims)
     b = device.Batch()
+    theta = [s.phys["theta"] for s in sims]
+    dt = [s.phys["dt"] for s in sims]
+    for dg in dgs:  # base velocities, restored before every timed run
+        dg.copy("u0", "u")
+        dg.copy("w0", "w")
 
     def reset():
         for dg in dgs:
             dg.copy("src", "src0")
             dg.copy("D", "Dq")
+            dg.copy("u", "u0")
+            dg.copy("w", "w0")
             dg.scale("q", "src0", 0.0)
+            dg.scale("qp", "src0", 0.0)
 
     def timed(fn):
         reset()
@@ -208,15 +219,38 @@ def ensemble_leg(args, sims):
             dg.sync()
         return r, time.perf_counter() - t
 
+    tvd = [(s.phys["theta"], s.phys["dt"], s.phys["kappa_s"], 3, 2) for s in sims]
+
+    def scalar_step():  # transported by the corrected velocities
+        for dg in dgs:
+            dg.copy("wnew", "w")
+        b.update_scalars(dgs, tvd)
+
+    def member_step():
+        b.compute_qsource(dgs, theta, dt)
+        st = b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol)
+        b.correct(dgs, theta, dt)
+        scalar_step()
+        return st
+
     timed(lambda: b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol))  # warm-up, autotuning
     mixed, t_mixed = timed(lambda: b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol))
     tuning = b.tuning()
     fp64, t_fp64 = timed(lambda: b.q_solve(dgs, epsilon=1e-10))
-    params = [(s.phys["theta"], s.phys["dt"], s.phys["kappa_s"], 3, 2) for s in sims]
-    t = time.perf_counter()
-    b.update_scalars(dgs, params)
+    timed(member_step)  # warm-up of the stencil and scalar launches
+    step, t_step = timed(member_step)
+    # stage times of one member step (each stage synchronised on its own)
+    reset()
     b.sync()
-    t_tvd = time.perf_counter() - t
+    stages = {}
+    for name, fn in (("source", lambda: b.compute_qsource(dgs, theta, dt)),
+                     ("mixed_solve", lambda: b.q_solve_mixed(dgs, epsilon=1e-10, inner_tol=args.inner_tol)),
+                     ("corrector_continuity", lambda: b.correct(dgs, theta, dt)),
+                     ("tvd_superbee", scalar_step)):
+        t = time.perf_counter()
+        fn()
+        b.sync()
+        stages[name + "_ms"] = (time.perf_counter() - t) * 1e3
     for dg in dgs:
         b.detach(dg)
     # the same mixed solves one grid at a time (each on its own stream)
@@ -224,14 +258,18 @@ def ensemble_leg(args, sims):
     for dg in dgs:
         dg.sync()
     t = time.perf_counter()
-    seq = [dg.q_solve_mixed(epsilon=1e-10, inner_tol=args.inner_tol) for dg in dgs]
+    [dg.q_solve_mixed(epsilon=1e-10, inner_tol=args.inner_tol) for dg in dgs]
     t_seq = time.perf_counter() - t
     b.close()
     return {
         "workload": (f"{len(sims)} members, cells U[5e3,2e5] (aspect U[1,4]), levels U{{20..100}}, "
-                     "Gaussian-bump depth; N total = %d" % N),
+                     "Gaussian-bump depth; N total = %d; member step = ComputeQSource + mixed CGSolveQ + "
+                     "Corrector + Continuity + TVD superbee UpdateScalars" % N),
         "members": len(sims), "N_total": N,
         "Nc_range": [min(s.Nc for s in sims), max(s.Nc for s in sims)],
+        "member_step_batched_ms": t_step * 1e3, "member_steps_per_s": len(sims) / t_step,
+        "member_step_stages": stages,
+        "member_step_all_converged": all(m["converged"] for m in step),
         "mixed_batched_ms": t_mixed * 1e3, "mixed_solves_per_s": len(sims) / t_mixed,
         "mixed_all_converged": all(m["converged"] for m in mixed),
         "mixed_max_true_resid": max(m["resid"] for m in mixed),
@@ -240,8 +278,6 @@ def ensemble_leg(args, sims):
         "fp64_iterations_max": max(f["niters"] for f in fp64),
         "mixed_one_at_a_time_ms": t_seq * 1e3,
         "batching_speedup": t_seq / t_mixed,
-        "tvd_superbee_step_all_members_ms": t_tvd * 1e3,
-        "mixed_plus_tvd_members_per_s": len(sims) / (t_mixed + t_tvd),
         "autotuned": tuning,
         "unknowns_per_s_mixed": N / t_mixed,
     }

@@ -92,7 +92,7 @@ int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
 /* Named device fields; the name fixes the layout.  Cell fields (any other
  * name): "q","src","dzz","dzzold","s","kappa_tv","Cn","gradSx","gradSy",
  * "stmp"...; w-type fields take Nk+1 values per column (names starting
- * with 'w'); edge fields: "dzf","u","utmp2","SfHp","SfHm","bscal" (the
+ * with 'w'); edge fields: "dzf","u","utmp2","u0","SfHp","SfHm","bscal" (the
  * open-boundary scalar, Nke values per type-2 edge, zero elsewhere); 2-D
  * cell fields: "h","htmp","htmp0","hcoef","active"...; 2-D edge fields:
  * "D","Dq","Dh".  Upload/download move between a flat host array and the

@@ -34,7 +34,7 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 }
 
 Kind kind_of(const char *name) {
-  static const char *edge[] = {"dzf", "u", "utmp2", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
+  static const char *edge[] = {"dzf", "u", "utmp2", "u0", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
   static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
                                 "hx", "hy", "active", "htmp0"};

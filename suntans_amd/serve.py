@@ -46,16 +46,20 @@ def forcing(sid: int, step: int) -> float:
 
 
 def estimate_bytes(st) -> int:
-    """HBM estimate of a resident simulation before it is built (about 31
-    FP64 values per cell-layer for the solver, scalar and face fields)."""
-    return int(8 * 31 * st.N * 1.1)
+    """HBM estimate of a resident simulation before it is built (about 34
+    FP64 values per cell-layer for the solver, scalar, velocity and face
+    fields)."""
+    return int(8 * 34 * st.N * 1.1)
 
 
 def make_device_grid(st, device_id: int = 0) -> device.DeviceGrid:
     """A DeviceGrid holding everything a time step reads: solver inputs, the
     two coefficient fields D (free surface: depth-integrated; pressure:
-    df/dg), base forcings and the scalar-transport inputs."""
+    df/dg), base forcings, the scalar-transport inputs and the predicted
+    vertical velocity w (the velocities the pressure source and Corrector
+    read)."""
     dg = device.DeviceGrid(st, device_id)
+    dg.upload("w", st.get("w"))
     dg.upload("q", st.get("qc"))
     dg.upload("src0", st.get("stmp"))
     dg.upload("dzf", st.get("dzf"))
