@@ -138,9 +138,12 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * M^-1 r itself is unchanged), "minwaves" (register budget of the
  * operator kernel: 3 or 4 waves per SIMD; 0 the compiler's choice),
  * "blocklds" (1: the operator's neighbour values staged in LDS per block of
- * four tiles walking the levels together).  Only "tiles", "reduce" and
- * "rzid" change the rounding of a dot product; every other knob gives
- * bit-identical results. */
+ * four tiles walking the levels together), "band" (levels per workgroup of
+ * the operator kernel, 8 or more; 0 whole columns), "prec_head" (top levels
+ * of each column whose Thomas pivots and forward values the preconditioner
+ * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
+ * at most 16).  Only "tiles", "reduce", "rzid" and "band" change the
+ * rounding of a dot product; every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event
  * pair around every solver kernel launch; sunhip_kernel_times fills ms[] and

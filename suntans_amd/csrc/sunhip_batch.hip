@@ -108,7 +108,10 @@ __global__ __launch_bounds__(256) void kb_qprec(const BQMember<T, T> *__restrict
     const BQMember<T, T> &B = M[m];
     if (B.sc->done) continue;
     double rz = 0, rr = 0;
-    qprec_tile<true, U, T, true>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq, (T)B.sc->nu, t, l, rz, rr);
+    // the top 24 (FP64) or 16 (FP32) levels of each column stay in registers
+    // between the Thomas sweeps, as in the single-grid kernel
+    qprec_tile<true, U, T, true, sizeof(T) == 8 ? 24 : 16>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq, (T)B.sc->nu, t, l,
+                                                          rz, rr);
     rz = wave_sum(rz);
     rr = wave_sum(rr);
     if (l == 0) { part[2 * (size_t)w] = rz; part[2 * (size_t)w + 1] = rr; }

@@ -152,6 +152,8 @@ struct sunhip_grid {
   int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
   int tune_blk = 0;   // operator kernel with block-staged neighbour values
   int tune_punroll = 8;  // levels per load group of the preconditioner (1, 2, 4, 8)
+  int tune_phead = 24;   // top levels per column the preconditioner keeps in registers (0, 8, 16, 24, 32;
+                         // FP32 solves use at most 16)
   int tune_band = 0;     // operator kernel over level bands of this many levels (0: whole columns)
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of

@@ -155,14 +155,15 @@ __global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal 
 }
 
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
-template <bool UPDATE, int U, typename T, bool RZID = false>
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
                                                CGScal *sc, Red R) {
   if (UPDATE && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
-  qprec_tile<UPDATE, U, T, RZID>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz, rr);
+  qprec_tile<UPDATE, U, T, RZID, KR>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz,
+                                     rr);
   double v[2] = {rz, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -243,16 +244,28 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   // the timing pair brackets the column kernel alone, not the finisher
   const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
   const bool rzid = g->tune_rzid && op != OP_NONE;
-#define PREC(U) \
+#define PREC(U, KR) \
   do { \
-    if (rzid) k_qprec<UPDATE, U, T, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
-    else k_qprec<UPDATE, U, T, false><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
+    if (rzid) k_qprec<UPDATE, U, T, true, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
+    else k_qprec<UPDATE, U, T, false, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
   } while (0)
-  switch (g->tune_punroll) {
-    case 1: PREC(1); break;
-    case 2: PREC(2); break;
-    case 4: PREC(4); break;
-    default: PREC(8); break;
+  // FP32 columns keep at most 16 levels: beyond that the register head costs
+  // more occupancy than it saves traffic
+  const int head = std::is_same<T, float>::value ? std::min(g->tune_phead, 16) : g->tune_phead;
+  if (head == 0) {
+    switch (g->tune_punroll) {
+      case 1: PREC(1, 0); break;
+      case 2: PREC(2, 0); break;
+      case 4: PREC(4, 0); break;
+      default: PREC(8, 0); break;
+    }
+  } else {  // register head: the memory part at 4 levels per group
+    switch (head) {
+      case 8: PREC(4, 8); break;
+      case 16: PREC(4, 16); break;
+      case 24: PREC(4, 24); break;
+      default: PREC(4, 32); break;
+    }
   }
 #undef PREC
   prof_end(g, ps);
@@ -694,6 +707,13 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   else if (!strcmp(key, "prec_unroll")) {
     if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("prec_unroll must be 1, 2, 4 or 8"); return -1; }
     g->tune_punroll = value;
+  }
+  else if (!strcmp(key, "prec_head")) {
+    if (value != 0 && value != 8 && value != 16 && value != 24 && value != 32) {
+      set_error("prec_head must be 0, 8, 16, 24 or 32");
+      return -1;
+    }
+    g->tune_phead = value;
   }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {

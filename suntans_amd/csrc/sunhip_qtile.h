@@ -395,7 +395,11 @@ __device__ __forceinline__ double qapply_block(const Geo &G, const QVec<T, T> &V
 // RZID: rz from the forward sweep alone, r.M^-1 r = sum d'^2/b' (M = L D L^T
 // with D = diag(b') and d' = L^-1 r), so the backward sweep does not read r
 // again; out is unchanged, only the rounding of rz differs.
-template <bool UPDATE, int U, typename T, bool RZID = false>
+// KR > 0: the top KR levels of the column keep b', d' and c in registers
+// between the sweeps (their backward step reads nothing and their d' never
+// goes to memory); the levels below go through out and bq as above.  The
+// arithmetic and its order are the same for every KR.
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0>
 __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ coef, const T *__restrict__ z,
                                            T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq, T nu,
                                            int t, int l, double &rz, double &rr) {
@@ -413,8 +417,37 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
     rr += (double)rk * (double)rk;
     return;
   }
+  constexpr int KH = KR > 0 ? KR : 1;
+  T hb[KH], hd[KH], hc[KH], hr[KH];
   T cprev = 0, bprev = 1, dprev = 0;
-  for (int kb = kt; kb < n; kb += U) {
+  if constexpr (KR > 0) {
+#pragma unroll
+    for (int j = 0; j < KR; j++) {
+      const int k = kt + j;
+      if (k < n) {
+        const uint32_t idx = base + (uint32_t)k * 64u;
+        const T ck = at32(coef, idx), ck1 = at32(coef, idx + 64u);
+        T rk = at32((const T *)r, idx);
+        if (UPDATE) { rk = rk - nu * at32(z, idx); at32(r, idx) = rk; }
+        rr += (double)rk * (double)rk;
+        T a = 0, b, c = 0;
+        if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+        else if (k == n - 1) { a = ck; b = -ck; }
+        else { a = ck; b = -ck - ck1; c = ck1; }
+        T d = rk;
+        if (k > kt) {
+          b -= a * cprev / bprev;
+          d -= a * dprev / bprev;
+        }
+        if constexpr (RZID) rz += (double)d * (double)d / (double)b;
+        hb[j] = b; hd[j] = d; hc[j] = c;
+        if constexpr (!RZID) hr[j] = rk;
+        cprev = c; bprev = b; dprev = d;
+      }
+    }
+  }
+  const int klo = kt + KR;  // first level that goes through memory
+  for (int kb = klo; kb < n; kb += U) {
     T rv[U], zv[U], cv[U + 1];
 #pragma unroll
     for (int j = 0; j <= U; j++) {
@@ -450,11 +483,11 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
     }
   }
   T unext = 0, cnext = 0;
-  for (int ke = n - 1; ke >= kt; ke -= U) {
+  for (int ke = n - 1; ke >= klo; ke -= U) {
     T bv[U], dv[U], rv[U], cv[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      const uint32_t idx = base + (uint32_t)max(ke - j, kt) * 64u;
+      const uint32_t idx = base + (uint32_t)max(ke - j, klo) * 64u;
       bv[j] = at32(bq, idx);
       dv[j] = at32((const T *)out, idx);
       rv[j] = RZID ? T(0) : at32((const T *)r, idx);
@@ -463,13 +496,26 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int k = ke - j;
-      if (k >= kt) {
+      if (k >= klo) {
         const T b = bv[j], d = dv[j];
         const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
         at32(out, base + (uint32_t)k * 64u) = u;
         if constexpr (!RZID) rz += (double)rv[j] * (double)u;
         unext = u;
         cnext = cv[j];  // c at level k-1 is coef[k]
+      }
+    }
+  }
+  if constexpr (KR > 0) {
+#pragma unroll
+    for (int j = KR - 1; j >= 0; j--) {
+      const int k = kt + j;
+      if (k < n) {
+        const T b = hb[j], d = hd[j];
+        const T u = (k == n - 1) ? d / b : d / b - hc[j] * unext / b;
+        at32(out, base + (uint32_t)k * 64u) = u;
+        if constexpr (!RZID) rz += (double)hr[j] * (double)u;
+        unext = u;
       }
     }
   }

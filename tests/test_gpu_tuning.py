@@ -1,5 +1,5 @@
 """Every kernel tuning variant (sunhip_set_tuning) solves the same problem.
-Unroll, store-hint, register-budget and block-staging variants are
+Unroll, store-hint, register-budget, register-head and block-staging variants are
 bit-identical to the default configuration (same operations in the same
 order, same reduction order).  Variants that regroup a dot product only
 change its rounding: tiles per block (block partial sums of 8 or 16 tiles)
@@ -13,7 +13,8 @@ from suntans_amd import cases, device
 pytestmark = pytest.mark.gpu
 
 BITWISE = [("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1), ("prec_unroll", 4), ("ntstore", 0),
-           ("minwaves", 3), ("minwaves", 4), ("blocklds", 1)]
+           ("minwaves", 3), ("minwaves", 4), ("blocklds", 1), ("prec_head", 0), ("prec_head", 8),
+           ("prec_head", 16), ("prec_head", 32)]
 ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32)]
 
 
@@ -31,7 +32,8 @@ def solve(st, tuning):
 @pytest.fixture(scope="module")
 def grids():
     return [cases.tiny(11, nx=23, ny=9, Nkmax=13), cases.lock_exchange(seed=7),
-            cases.ensemble_member(6, 9000, 1.2, 40)]
+            cases.ensemble_member(6, 9000, 1.2, 40),
+            cases.tiny(24, nx=14, ny=9, Nkmax=40, planform="hex", split_every=4, maxfaces=7)]
 
 
 @pytest.mark.parametrize("key,value", BITWISE, ids=[f"{k}={v}" for k, v in BITWISE])
