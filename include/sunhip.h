@@ -142,7 +142,10 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * the operator kernel, 8 or more; 0 whole columns), "prec_head" (top levels
  * of each column whose Thomas pivots and forward values the preconditioner
  * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
- * at most 16).  Only "tiles", "reduce", "rzid" and "band" change the
+ * at most 16), "xchg" (the iteration's operator kernel exchanges neighbour
+ * values inside each wave through LDS, gathering only cross-tile neighbours
+ * from memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD
+ * (default), 3 on with 2 levels per load group).  Only "tiles", "reduce", "rzid" and "band" change the
  * rounding of a dot product; every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event

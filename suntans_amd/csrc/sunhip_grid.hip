@@ -78,6 +78,7 @@ sunhip::Geo sunhip_grid::geo() const {
   G.e_c1 = d_e_c1; G.e_c2 = d_e_c2; G.e_s1 = d_e_s1; G.e_s2 = d_e_s2;
   G.e_nke = d_e_nke; G.e_nkc = d_e_nkc; G.e_etop = d_e_etop; G.e_mark = d_e_mark; G.e_comp = d_e_comp;
   G.df = d_df; G.dg = d_dg;
+  G.xcode = d_xcode; G.pair = d_pair; G.npair = d_npair;
   return G;
 }
 
@@ -351,6 +352,32 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= upload_vec(g, &g->d_comp, comp);
   rc |= upload_vec(g, &g->d_nbb, nbb);
   rc |= upload_vec(g, &g->d_nbs, nbs);
+  {
+    // neighbour exchange: same-tile neighbours by lane, the others as a
+    // compact list of cross-tile (face, slot) pairs per tile
+    std::vector<int> xcode(mf * NS, -1), pair((size_t)g->Nt * SUNHIP_XPAIRS, -1), npair(g->Nt, 0);
+    for (int t = 0; t < g->Nt; t++) {
+      int m = 0;
+      for (int l = 0; l < SUNHIP_TILE; l++) {
+        const int s = t * SUNHIP_TILE + l;
+        for (int f = 0; f < nf[s]; f++) {
+          const int nb = nbs[f * NS + s];
+          if (nb < 0) continue;
+          if (nb / SUNHIP_TILE == t) {
+            xcode[f * NS + s] = nb % SUNHIP_TILE;
+          } else {
+            if (m < SUNHIP_XPAIRS) { xcode[f * NS + s] = SUNHIP_TILE + m; pair[(size_t)t * SUNHIP_XPAIRS + m] = f * NS + s; }
+            m++;
+          }
+        }
+      }
+      npair[t] = std::min(m, SUNHIP_XPAIRS);
+      g->max_pairs = std::max(g->max_pairs, m);
+    }
+    rc |= upload_vec(g, &g->d_xcode, xcode);
+    rc |= upload_vec(g, &g->d_pair, pair);
+    rc |= upload_vec(g, &g->d_npair, npair);
+  }
   rc |= upload_vec(g, &g->d_eb, eb);
   rc |= upload_vec(g, &g->d_es, es);
   rc |= upload_vec(g, &g->d_nrm, nrm);
@@ -409,6 +436,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   for (auto &kv : g->fields32) hipFree(kv.second);
   void *ptrs[] = {g->d_toff, g->d_tdep, g->d_eoff, g->d_edep, g->d_nk, g->d_ctop, g->d_ctopold, g->d_comp,
                   g->d_nf, g->d_nbb, g->d_nbs, g->d_eb, g->d_es, g->d_nrm, g->d_kr, g->d_Ac, g->d_df,
+                  g->d_xcode, g->d_pair, g->d_npair,
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,

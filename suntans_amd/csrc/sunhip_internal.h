@@ -21,6 +21,7 @@
 
 #define SUNHIP_MAXF 8   // faces per cell supported (triangles to octagons)
 #define SUNHIP_TILE 64
+#define SUNHIP_XPAIRS 128  // cross-tile faces per tile the exchange kernel handles
 
 namespace sunhip {
 
@@ -91,6 +92,11 @@ struct Geo {
   const int *e_s1, *e_s2;  // slots of grad[0], grad[1] (-1 none)
   const int *e_nke, *e_nkc, *e_etop, *e_mark, *e_comp;  // e_comp: 1 for edgedist[0..1)
   const double *df, *dg;
+  // neighbour exchange of the operator kernel: per (face, slot) where the
+  // neighbour's value is found in the tile's exchange row (0..63: the lane of
+  // a neighbour in the same tile; 64 + i: cross-tile pair i; -1 none), and
+  // per tile the pairs' (face, slot) indices and count
+  const int *xcode, *pair, *npair;
 };
 
 }  // namespace sunhip
@@ -116,6 +122,8 @@ struct sunhip_grid {
   int *d_toff = nullptr, *d_tdep = nullptr, *d_eoff = nullptr, *d_edep = nullptr;
   int *d_nk = nullptr, *d_ctop = nullptr, *d_ctopold = nullptr, *d_comp = nullptr, *d_nf = nullptr;
   int *d_nbb = nullptr, *d_nbs = nullptr, *d_eb = nullptr, *d_es = nullptr, *d_nrm = nullptr, *d_kr = nullptr;
+  int *d_xcode = nullptr, *d_pair = nullptr, *d_npair = nullptr;  // neighbour exchange (Geo)
+  int max_pairs = 0;                 // most cross-tile faces of any tile
   double *d_Ac = nullptr, *d_df = nullptr, *d_dg = nullptr;
   int *d_e_c1 = nullptr, *d_e_c2 = nullptr, *d_e_s1 = nullptr, *d_e_s2 = nullptr;
   int *d_e_nke = nullptr, *d_e_nkc = nullptr, *d_e_etop = nullptr, *d_e_mark = nullptr, *d_e_comp = nullptr;
@@ -152,6 +160,8 @@ struct sunhip_grid {
   int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
   int tune_blk = 0;   // operator kernel with block-staged neighbour values
   int tune_punroll = 8;  // levels per load group of the preconditioner (1, 2, 4, 8)
+  int tune_xchg = 2;     // operator kernel with the neighbour exchange inside each wave: 0 off, 1 on,
+                         // 2 on with a 4-waves-per-SIMD register budget, 3 on with 2 levels per group
   int tune_phead = 24;   // top levels per column the preconditioner keeps in registers (0, 8, 16, 24, 32;
                          // FP32 solves use at most 16)
   int tune_band = 0;     // operator kernel over level bands of this many levels (0: whole columns)

@@ -154,6 +154,19 @@ __global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal 
   reduce_out<1, 256>(dv, R, sc);
 }
 
+// The iteration's operator kernel with the neighbour exchange inside each
+// wave (qapply_xchg); 4 tiles per block, one exchange row per wave.
+template <int NF, int U, typename T, bool NTS, int MINW = 0>
+__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_x(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
+  if (sc->done) return;
+  __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
+  const int w = threadIdx.x >> 6;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+  const double dot = qapply_xchg<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w]);
+  double v[1] = {dot};
+  reduce_out<1, 256>(v, R, sc);
+}
+
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
 template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
@@ -288,6 +301,31 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
 #define APPLYV(TPB, NTS) k_qapply<MODE, 3, 4, T, TO, TPB, NTS><<<nba, TPB * 64, 0, g->stream>>>(G, V, sc, R)
 #define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
   if constexpr (MODE == QM_FUSED && std::is_same<T, TO>::value) {
+    // the exchange kernel, unless another launch shape was asked for
+    const bool xchg = g->tune_xchg && g->max_pairs <= SUNHIP_XPAIRS && g->tune_tiles == 4 && !g->tune_blk &&
+                      !g->tune_band && !g->tune_minw && g->tune_unroll == 4;
+    if (xchg) {
+      const int nbx = (g->Nt + 3) / 4;
+      with_nf(g->mf, [&](auto nfc) {
+        constexpr int NF = decltype(nfc)::value, UX = NF <= 4 ? 4 : 2;
+        if (g->tune_xchg == 2) {  // register budget of 4 waves per SIMD
+          if (g->tune_nts) k_qapply_x<NF, UX, T, true, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_x<NF, UX, T, false, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        } else if (g->tune_xchg == 3) {  // 2 levels per group
+          if (g->tune_nts) k_qapply_x<NF, 2, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_x<NF, 2, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        } else {
+          if (g->tune_nts) k_qapply_x<NF, UX, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_x<NF, UX, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        }
+        return 0;
+      });
+      prof_end(g, ps);
+      HIPCHK(hipGetLastError());
+      if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nbx, sc, R.op);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
     if (variants && g->tune_band > 0 && tpb == 4) {
       const int H = g->tune_band, ngroups = (g->Nt + 3) / 4, nbands = (g->maxdepth + H - 1) / H;
       const int nblk = ngroups * nbands;
@@ -714,6 +752,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
       return -1;
     }
     g->tune_phead = value;
+  }
+  else if (!strcmp(key, "xchg")) {
+    if (value < 0 || value > 3) { set_error("xchg must be 0..3"); return -1; }
+    g->tune_xchg = value;
   }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {

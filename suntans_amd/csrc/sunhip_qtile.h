@@ -389,6 +389,144 @@ __device__ __forceinline__ double qapply_block(const Geo &G, const QVec<T, T> &V
   return dot;
 }
 
+// The FUSED mode of qapply_tile with the neighbour values exchanged inside
+// the wave.  The lanes of tile t walk the levels together (the tile's range of
+// computational levels); per group of U levels each lane writes its own
+// p = rtmp + mu*p_old to the wave's exchange row in LDS, the first lanes
+// gather the tile's cross-tile neighbours (the compact pair list, one load per
+// 64 pairs and level instead of one per face), and every face reads its
+// neighbour's p from the row.  The row is private to the wave and rewritten
+// only after the wave's own reads (LDS operations of a wave complete in
+// order), so no barrier is needed.  Columns that are not computational but
+// have layers still publish their p.  Values are formed by the same
+// operations in the same order as in qapply_tile: bit-identical results.
+template <int NF, int U, typename T, bool NTS>
+__device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V, T mu, T nu, int t, int l,
+                                              T (*row)[SUNHIP_TILE + SUNHIP_XPAIRS]) {
+  if (t >= G.Nt) return 0;
+  const T *__restrict__ coef = V.coef;
+  const T *__restrict__ dzf = V.dzf;
+  const T *__restrict__ rt = V.rtmp;
+  const T *__restrict__ po = V.pold;
+  const int s = t * 64 + l, NS = G.NS;
+  const int n = G.nk[s];
+  const bool own = n > 0 && G.comp[s] == 1;
+  const int kt = n > 0 ? G.ctop[s] : 0;
+  const int kb0 = wave_min_i(own ? kt : 0x7fffffff), kb1 = wave_max_i(own ? n : 0);
+  if (kb1 <= kb0) return 0;  // no computational column in the tile (uniform)
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  const int nf = n > 0 ? G.nf[s] : 0;
+  int code[NF], kmn[NF], kmx[NF], lim[NF];
+  uint32_t ebb[NF];
+  T Df[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    code[f] = f < nf ? G.xcode[f * NS + s] : -1;
+    if (f < nf) {
+      const int kr = G.kr[f * NS + s];
+      kmn[f] = kr & 0xffff;
+      kmx[f] = kr >> 16;
+      ebb[f] = (uint32_t)G.eb[f * NS + s];
+      Df[f] = G.nbb[f * NS + s] >= 0 ? V.D[G.es[f * NS + s]] : T(0);
+    } else {
+      kmn[f] = 0; kmx[f] = 0; ebb[f] = 0; Df[f] = 0;
+    }
+    if (code[f] < 0) { kmn[f] = 0; kmx[f] = 0; }
+    lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+  }
+  // cross-tile pairs this lane gathers: pair l and pair 64 + l
+  const int np = G.npair[t];
+  uint32_t pnb[2];
+  int plim[2];
+  bool pact[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int pi = r * 64 + l;
+    pact[r] = pi < np;
+    if (pact[r]) {
+      const int fs = G.pair[(size_t)t * SUNHIP_XPAIRS + pi];
+      pnb[r] = (uint32_t)G.nbb[fs];
+      const int mx = G.kr[fs] >> 16;
+      plim[r] = mx > 0 ? mx - 1 : 0;
+    } else {
+      pnb[r] = base; plim[r] = 0;
+    }
+  }
+  const bool two = np > 64;
+  const int nn = n > 0 ? n : 0;
+  double dot = 0;
+  T vprev = 0;
+  for (int kb = kb0; kb < kb1; kb += U) {
+    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+#pragma unroll
+    for (int j = 0; j <= U; j++) {
+      const uint32_t idx = base + (uint32_t)min(kb + j, nn) * 64u;
+      pv[j] = at32(po, idx);
+      v[j] = at32(rt, idx);
+      c[j] = at32(coef, idx);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) xv[j] = at32((const T *)V.x, base + (uint32_t)min(kb + j, nn) * 64u);
+    T g0[U], g1[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+      g0[j] = 0; g1[j] = 0;
+      if (pact[0]) {
+        const uint32_t ni = pnb[0] + (uint32_t)min(k, plim[0]) * 64u;
+        g0[j] = at32(rt, ni) + mu * at32(po, ni);
+      }
+      if (two && pact[1]) {
+        const uint32_t ni = pnb[1] + (uint32_t)min(k, plim[1]) * 64u;
+        g1[j] = at32(rt, ni) + mu * at32(po, ni);
+      }
+#pragma unroll
+      for (int f = 0; f < NF; f++) dz[j][f] = f < nf ? at32(dzf, ebb[f] + (uint32_t)min(k, lim[f]) * 64u) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      row[j][l] = v[j];
+      row[j][64 + l] = g0[j];
+      if (two) row[j][128 + l] = g1[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < U; j++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) nv[j][f] = code[f] >= 0 ? row[j][code[f]] : T(0);
+    __builtin_amdgcn_wave_barrier();
+    if (own) {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = kb + j;
+        if (k >= kt && k < n) {
+          const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
+          const T c0 = c[j], cp1 = c[j + 1];
+          T y = 0;
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+            y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+          }
+          if (kt == n - 1) y -= T(2.0) * c0 * v0;
+          else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
+          else if (k == n - 1) y += c0 * vm1 - c0 * v0;
+          else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          st_stream<NTS>(&at32(V.pnew, idx), v0);
+          st_stream<NTS>(&at32(V.x, idx), xv[j] + nu * pv[j]);
+          dot += (double)v0 * (double)y;
+          st_stream<NTS>(&at32(V.out, idx), y);
+        }
+      }
+    }
+    vprev = v[U - 1];
+  }
+  return dot;
+}
+
 // Vertical-line preconditioner of tile t: (UPDATE) r -= nu*z, then
 // out = M^-1 r by Thomas down the column (b' recomputed, d' stored in out)
 // and back up it (b' from bq); rz and rr collect the lane's r.out and r.r.
