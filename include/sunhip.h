@@ -142,10 +142,12 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * the operator kernel, 8 or more; 0 whole columns), "prec_head" (top levels
  * of each column whose Thomas pivots and forward values the preconditioner
  * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
- * at most 16), "xchg" (the iteration's operator kernel exchanges neighbour
- * values inside each wave through LDS, gathering only cross-tile neighbours
- * from memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD
- * (default), 3 on with 2 levels per load group).  Only "tiles", "reduce", "rzid" and "band" change the
+ * at most 16; -1, the default: 24 from 1024 tiles of 64 columns up in FP64,
+ * 16 from 2048 tiles up in FP32, else 0), "xchg" (the iteration's operator
+ * kernel exchanges neighbour values inside each wave through LDS, gathering
+ * only cross-tile neighbours from memory: 0 off, 1 on, 2 on with a register
+ * budget of 4 waves per SIMD, 3 on with 2 levels per load group; -1, the
+ * default: 2 from 4096 tiles up, else 0).  Only "tiles", "reduce", "rzid" and "band" change the
  * rounding of a dot product; every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event

@@ -98,6 +98,33 @@ __global__ __launch_bounds__(256) void kb_qapply(const BQMember<T, TO> *__restri
   }
 }
 
+// kb_qapply in the FUSED mode with the neighbour exchange inside each wave
+// (qapply_xchg): the same values and the same per-item sums.
+template <int NF, int U, typename T>
+__global__ __launch_bounds__(256) void kb_qapply_x(const BQMember<T, T> *__restrict__ M,
+                                                      const int2 *__restrict__ W, int nwork,
+                                                      double *__restrict__ part) {
+  __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
+  const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  for (int w = first_item(); w < nwork; w += stride) {
+    const int2 it = W[w];
+    const int m = __builtin_amdgcn_readfirstlane(it.x), t = __builtin_amdgcn_readfirstlane(it.y);
+    const BQMember<T, T> &B = M[m];
+    CGScal *sc = B.sc;
+    if (sc->done) continue;
+    QVec<T, T> V{};
+    V.coef = B.coef; V.dzf = B.dzf; V.D = B.D;
+    const bool odd = sc->iter & 1;
+    V.rtmp = B.rtmp;
+    V.pold = odd ? B.p1 : B.p0;
+    V.pnew = odd ? B.p0 : B.p1;
+    V.x = B.x;
+    V.out = B.z;
+    const double d = wave_sum(qapply_xchg<NF, U, T, true>(B.G, V, (T)sc->mu, (T)sc->nu, t, l, rows[threadIdx.x >> 6]));
+    if (l == 0) part[w] = d;
+  }
+}
+
 template <int U, typename T>
 __global__ __launch_bounds__(256) void kb_qprec(const BQMember<T, T> *__restrict__ M, const int2 *__restrict__ W,
                                                 int nwork, double *__restrict__ part) {
@@ -352,6 +379,7 @@ struct sunhip_batch {
   // tables of the current set
   bool dirty = true;
   int nf = 3;
+  bool xok = true;  // every member's cross-tile faces fit the exchange kernel
   DevBuf<BQMember<double, double>> tq64;
   DevBuf<BQMember<float, float>> tq32;
   DevBuf<BHMember> th64;
@@ -440,9 +468,11 @@ int rebuild(sunhip_batch *b) {
   std::vector<BRange> rg[T_NKIND], rgr;
   std::vector<int2> wk[T_NKIND], wkr;
   int nf = 3;
+  bool xok = true;
   for (const Task &t : b->tasks) {
     sunhip_grid *g = t.g;
     nf = std::max(nf, g->mf);
+    xok = xok && g->max_pairs <= SUNHIP_XPAIRS;
     if (t.kind == T_QMIX && t.mstate == M_RESID) {
       // FP64 true residual of a mixed solve
       const int row = (int)rgr.size();
@@ -523,6 +553,7 @@ int rebuild(sunhip_batch *b) {
   }
   if (rc) return rc;
   b->nf = nf;
+  b->xok = xok;
   b->dirty = false;
   if (b->gexec) {
     hipGraphExecDestroy(b->gexec);
@@ -564,7 +595,13 @@ int enqueue_iteration(sunhip_batch *b, const Config &c) {
     auto *M = b->tq64.d;
     double *P = b->part[T_Q64].d;
 #define QA(NF, U) kb_qapply<QM_FUSED, NF, U, double, double><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q64].d, n, P)
-    if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
+    if (b->xok && c.unroll == 4)  // neighbour exchange inside each wave
+      with_nf(b->nf, [&](auto nfc) {
+        constexpr int NF = decltype(nfc)::value;
+        kb_qapply_x<NF, NF <= 4 ? 4 : 2, double><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q64].d, n, P);
+        return 0;
+      });
+    else if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
     else if (b->nf == 4) { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
     else if (b->nf <= 6) { if (c.unroll == 1) QA(6, 1); else QA(6, 2); }
     else { if (c.unroll == 1) QA(8, 1); else QA(8, 2); }
@@ -581,7 +618,13 @@ int enqueue_iteration(sunhip_batch *b, const Config &c) {
     auto *M = b->tq32.d;
     double *P = b->part[T_Q32].d;
 #define QA(NF, U) kb_qapply<QM_FUSED, NF, U, float, float><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q32].d, n, P)
-    if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
+    if (b->xok && c.unroll == 4)  // neighbour exchange inside each wave
+      with_nf(b->nf, [&](auto nfc) {
+        constexpr int NF = decltype(nfc)::value;
+        kb_qapply_x<NF, NF <= 4 ? 4 : 2, float><<<nb, 256, 0, b->stream>>>(M, b->work[T_Q32].d, n, P);
+        return 0;
+      });
+    else if (b->nf <= 3) { if (c.unroll == 1) QA(3, 1); else if (c.unroll == 2) QA(3, 2); else QA(3, 4); }
     else if (b->nf == 4) { if (c.unroll == 1) QA(4, 1); else if (c.unroll == 2) QA(4, 2); else QA(4, 4); }
     else if (b->nf <= 6) { if (c.unroll == 1) QA(6, 1); else QA(6, 2); }
     else { if (c.unroll == 1) QA(8, 1); else QA(8, 2); }

@@ -160,10 +160,12 @@ struct sunhip_grid {
   int tune_minw = 0;  // register budget of the operator kernel: waves per SIMD (0, 3, 4)
   int tune_blk = 0;   // operator kernel with block-staged neighbour values
   int tune_punroll = 8;  // levels per load group of the preconditioner (1, 2, 4, 8)
-  int tune_xchg = 2;     // operator kernel with the neighbour exchange inside each wave: 0 off, 1 on,
-                         // 2 on with a 4-waves-per-SIMD register budget, 3 on with 2 levels per group
-  int tune_phead = 24;   // top levels per column the preconditioner keeps in registers (0, 8, 16, 24, 32;
-                         // FP32 solves use at most 16)
+  int tune_xchg = -1;    // operator kernel with the neighbour exchange inside each wave: 0 off, 1 on,
+                         // 2 on with a 4-waves-per-SIMD register budget, 3 on with 2 levels per group,
+                         // -1 (default) 2 from 4096 tiles up, off below (short walks favour the plain kernel)
+  int tune_phead = -1;   // top levels per column the preconditioner keeps in registers (0, 8, 16, 24, 32;
+                         // FP32 solves use at most 16); -1 (default) 24 from 1024 tiles up in FP64, 16
+                         // from 2048 tiles up in FP32, 0 below
   int tune_band = 0;     // operator kernel over level bands of this many levels (0: whole columns)
 
   // per-kernel timing (sunhip_profile): an event pair around every launch of

@@ -264,7 +264,10 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   } while (0)
   // FP32 columns keep at most 16 levels: beyond that the register head costs
   // more occupancy than it saves traffic
-  const int head = std::is_same<T, float>::value ? std::min(g->tune_phead, 16) : g->tune_phead;
+  const bool fp32 = std::is_same<T, float>::value;
+  int head = g->tune_phead;
+  if (head < 0) head = fp32 ? (g->Nt >= 2048 ? 16 : 0) : (g->Nt >= 1024 ? 24 : 0);
+  if (fp32) head = std::min(head, 16);
   if (head == 0) {
     switch (g->tune_punroll) {
       case 1: PREC(1, 0); break;
@@ -302,16 +305,17 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
 #define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
   if constexpr (MODE == QM_FUSED && std::is_same<T, TO>::value) {
     // the exchange kernel, unless another launch shape was asked for
-    const bool xchg = g->tune_xchg && g->max_pairs <= SUNHIP_XPAIRS && g->tune_tiles == 4 && !g->tune_blk &&
+    const int xmode = g->tune_xchg >= 0 ? g->tune_xchg : (g->Nt >= 4096 ? 2 : 0);
+    const bool xchg = xmode && g->max_pairs <= SUNHIP_XPAIRS && g->tune_tiles == 4 && !g->tune_blk &&
                       !g->tune_band && !g->tune_minw && g->tune_unroll == 4;
     if (xchg) {
       const int nbx = (g->Nt + 3) / 4;
       with_nf(g->mf, [&](auto nfc) {
         constexpr int NF = decltype(nfc)::value, UX = NF <= 4 ? 4 : 2;
-        if (g->tune_xchg == 2) {  // register budget of 4 waves per SIMD
+        if (xmode == 2) {  // register budget of 4 waves per SIMD
           if (g->tune_nts) k_qapply_x<NF, UX, T, true, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
           else k_qapply_x<NF, UX, T, false, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
-        } else if (g->tune_xchg == 3) {  // 2 levels per group
+        } else if (xmode == 3) {  // 2 levels per group
           if (g->tune_nts) k_qapply_x<NF, 2, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
           else k_qapply_x<NF, 2, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
         } else {
@@ -747,14 +751,14 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     g->tune_punroll = value;
   }
   else if (!strcmp(key, "prec_head")) {
-    if (value != 0 && value != 8 && value != 16 && value != 24 && value != 32) {
-      set_error("prec_head must be 0, 8, 16, 24 or 32");
+    if (value != -1 && value != 0 && value != 8 && value != 16 && value != 24 && value != 32) {
+      set_error("prec_head must be -1, 0, 8, 16, 24 or 32");
       return -1;
     }
     g->tune_phead = value;
   }
   else if (!strcmp(key, "xchg")) {
-    if (value < 0 || value > 3) { set_error("xchg must be 0..3"); return -1; }
+    if (value < -1 || value > 3) { set_error("xchg must be -1..3"); return -1; }
     g->tune_xchg = value;
   }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;

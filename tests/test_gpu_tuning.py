@@ -14,8 +14,7 @@ pytestmark = pytest.mark.gpu
 
 BITWISE = [("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1), ("prec_unroll", 4), ("ntstore", 0),
            ("minwaves", 3), ("minwaves", 4), ("blocklds", 1), ("prec_head", 0), ("prec_head", 8),
-           ("prec_head", 16), ("prec_head", 32), ("xchg", 0), ("xchg", 1),
-           ("xchg", 3)]
+           ("prec_head", 16), ("prec_head", 24), ("prec_head", 32), ("xchg", 1), ("xchg", 2), ("xchg", 3)]
 ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32)]
 
 
