@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_large_solve.txt")
 
-KERNEL_NAMES = {"qapply": "k_qapply<2,3,4,double,double,4,true,0>", "qprec": "k_qprec<true,4,double,true>"}
+KERNEL_NAMES = {"qapply": "k_qapply_x<3,4,double,true,4>", "qprec": "k_qprec<true,4,double,true,24>"}
 
 
 def log(msg: str) -> None:
