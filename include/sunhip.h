@@ -194,6 +194,35 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
  * "kappa_tv","u","utmp2","dzf"; writes scal and "Cn". */
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
 
+/* ---- the horizontal-momentum predictor (UPredictor, main/phys.c:2579-3018) --
+ * In two parts either side of the reference's open-boundary hooks
+ * (BoundaryVelocities, OpenBoundaryFluxes, user code of each case).  Fields:
+ * edge "u", "utmp" (in: the horizontal source; out: U***), "utmp2", "ue"
+ * (E = A^-1 1, the reference's phys->ut), "dzf"; edge 2-D "D", "tau_T",
+ * "CdT", "CdB"; cell "dzz", "nu_tv", "nu_lax", "s"; w-type "w", "wtmp2";
+ * cell 2-D "h", "htmp", "active", "hcorr", "dv" (depth); scratch "up_*".
+ * Bit-identical to the reference.  On a one-layer edge with no-slip drag
+ * (CdB or CdT = -1) the reference reads a coefficient left from the
+ * previous edge; this build uses the edge's own, 0. */
+typedef struct {
+  double dt, theta, grav;
+  double nu;             /* prop->nu */
+  double lax_vertical;   /* prop->laxWendroff_Vertical */
+  double thetaM;         /* implicitness of vertical momentum advection; < 0 none */
+  int nonlinear;         /* prop->nonlinear (0: no momentum advection) */
+} sunhip_upred_params;
+/* main/phys.c:2596-2942: wtmp2 = w, utmp2 = u, D = 0; type-2 edges utmp = u;
+ * computational edges: explicit free-surface gradient, wind stress, the
+ * vertical system of viscosity, drag (implicit) and vertical advection
+ * solved for U*** and E, D = dzf^T E.  Where the reference exits it returns
+ * -5 (an edge level whose two cells have zero height) or -6 (U*** = nan). */
+int sunhip_u_predict(sunhip_grid *g, const sunhip_upred_params *p);
+/* main/phys.c:2951-3018: htmp = Ac h - dt sum of theta-weighted face fluxes,
+ * the free-surface PCG (as sunhip_h_solve, from h = 0), cells drying below
+ * 1e-10 above the bed clamped (h, active = 0, bottom "s" = 0), and
+ * u = U*** - g theta dt E (h1 - h2)/dg on computational edges. */
+int sunhip_u_solve(sunhip_grid *g, const sunhip_upred_params *p, const sunhip_cg_params *prm, sunhip_cg_stats *st);
+
 /* ---- the stencils either side of the pressure solve ------------------------
  * Solve's nonhydrostatic sequence is ComputeQSource, CGSolveQ, Corrector,
  * Continuity (main/phys.c:1270-1300).  Bit-identical to the reference. */

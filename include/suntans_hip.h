@@ -42,6 +42,15 @@ void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int mypro
 void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
 /* main/phys.c:3814 (Continuity; prototypes main/phys.c:73-74) */
 void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop);
+/* UPredictor (static, main/phys.c:2579-3074) in the two parts either side
+ * of its open-boundary hooks; the caller runs BoundaryVelocities and
+ * OpenBoundaryFluxes between them and UpdateDZ plus the boundary-flux tail
+ * (main/phys.c:3023-3073) after, as the reference's UPredictor does.
+ * _begin: main/phys.c:2596-2942 (U*** in utmp, E in ut, D; wtmp2, utmp2);
+ * _solve: main/phys.c:2951-3018 (htmp, CGSolve, drying, u).  No counterpart
+ * by these names: the split is where a device call cannot reach the hooks. */
+void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
+void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
 /* main/tvd.h:27-28 (definition main/tvd.c:36) */
 void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal,
                            int TVD, MPI_Comm comm, int myproc);

@@ -422,6 +422,213 @@ void oracle_Continuity(REAL **w, gridT *g, physT *p, propT *prop) {
   }
 }
 
+/* main/phys.c:2596-2942, the part of UPredictor before the open-boundary
+ * hooks: wtmp2 = w, utmp2 = u, D = 0; on type-2 edges utmp = u; on
+ * computational edges the explicit free-surface gradient, the wind stress,
+ * and per edge the vertical system of viscosity, drag (implicit, theta = 1)
+ * and vertical momentum advection (thetaM), solved for U*** (into utmp) and
+ * E = A^-1 1 (into ut), with D = dzf^T E.  Returns 1 where the reference
+ * exits: a level of an edge whose two cells both have zero height
+ * (main/phys.c:2900-2904) or U*** = nan (:2945-2950).
+ * On a single-layer edge with no-slip drag (CdB or CdT = -1) the reference
+ * reads a viscosity coefficient left over from the previous edge
+ * (main/phys.c:2803-2815, :2880-2889, index k after its loops); the
+ * restatement uses this edge's own value, 0. */
+int oracle_UPredictor_begin(gridT *g, physT *p, propT *prop) {
+  const double dt = prop->dt, theta = prop->theta, th = 1.0, alpha = 1.0;
+  const double nu = prop->nu, lw = prop->laxWendroff_Vertical, tM = prop->thetaM;
+  const int n1 = g->Nkmax + 1;
+  double *a = (double *)calloc(n1, sizeof(double)), *b = (double *)calloc(n1, sizeof(double));
+  double *c = (double *)calloc(n1, sizeof(double)), *d = (double *)calloc(n1, sizeof(double));
+  double *a0 = (double *)calloc(n1, sizeof(double)), *b0 = (double *)calloc(n1, sizeof(double));
+  double *c0 = (double *)calloc(n1, sizeof(double)), *dw = (double *)calloc(n1, sizeof(double));
+  double *bp = (double *)calloc(n1, sizeof(double)), *dp = (double *)calloc(n1, sizeof(double));
+  double *ep = (double *)calloc(n1, sizeof(double));
+  int bad = 0;
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = 0; k <= g->Nk[i]; k++) p->wtmp2[i][k] = p->w[i][k];
+  for (int j = 0; j < g->Ne; j++) {
+    p->D[j] = 0;
+    for (int k = 0; k < g->Nke[j]; k++) p->utmp2[j][k] = p->u[j][k];
+  }
+  for (int jp = g->edgedist[2]; jp < g->edgedist[3]; jp++) {
+    const int j = g->edgep[jp];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) p->utmp[j][k] = p->u[j][k];
+  }
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp];
+    const int i1 = g->grad[2 * j], i2 = g->grad[2 * j + 1];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++)
+      p->utmp[j][k] -= prop->grav * (1 - theta) * dt * (p->h[i1] - p->h[i2]) / g->dg[j];
+  }
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1] && !bad; jp++) {
+    const int j = g->edgep[jp];
+    int i1 = g->grad[2 * j], i2 = g->grad[2 * j + 1];
+    if (i1 == -1) i1 = i2;
+    if (i2 == -1) i2 = i1;
+    const int et = g->etop[j], ne = g->Nke[j];
+    double *u = p->u[j], *ut = p->utmp[j];
+    REAL *z1 = g->dzz[i1], *z2 = g->dzz[i2];
+    ut[et] += 2.0 * dt * p->tau_T[j] / (z1[et] + z2[et]);
+    if (z1[et] == 0 && z2[et] == 0) continue;
+    for (int k = et; k < ne; k++) a[k] = b[k] = c[k] = d[k] = 0;
+    for (int k = et + 1; k < ne; k++)
+      c[k] = 0.25 * (p->nu_tv[i1][k - 1] + p->nu_tv[i2][k - 1] + p->nu_tv[i1][k] + p->nu_tv[i2][k] +
+                     lw * (p->nu_lax[i1][k - 1] + p->nu_lax[i2][k - 1] + p->nu_lax[i1][k] + p->nu_lax[i2][k]));
+    for (int k = et + 1; k < ne; k++)
+      a[k] = 2.0 * (nu + c[k]) / (0.25 * (z1[k] + z2[k]) * (z1[k - 1] + z2[k - 1] + z1[k] + z2[k]));
+    for (int k = et; k < ne - 1; k++)
+      b[k] = 2.0 * (nu + c[k + 1]) / (0.25 * (z1[k] + z2[k]) * (z1[k] + z2[k] + z1[k + 1] + z2[k + 1]));
+    const int multi = ne - et > 1, adv = prop->nonlinear && tM >= 0 && multi;
+    if (adv) {
+      const int lo = g->ctop[i1] > g->ctop[i2] ? i2 : i1, hi = g->ctop[i1] > g->ctop[i2] ? i1 : i2;
+      for (int k = 0; k < g->ctop[hi]; k++) dw[k] = 0;
+      for (int k = g->ctop[hi]; k < ne; k++) dw[k] = 0.5 * (p->w[lo][k] + p->w[hi][k]);
+      dw[ne] = 0;
+      for (int k = et; k < ne; k++) {
+        const double hz = 0.5 * (z1[k] + z2[k]);
+        a0[k] = (alpha * 0.5 * (dw[k] - fabs(dw[k])) + 0.5 * (1 - alpha) * dw[k]) / hz;
+        b0[k] = (alpha * 0.5 * (dw[k] + fabs(dw[k]) - dw[k + 1] + fabs(dw[k + 1])) +
+                 0.5 * (1 - alpha) * (dw[k] - dw[k + 1])) / hz;
+        c0[k] = -(alpha * 0.5 * (dw[k + 1] + fabs(dw[k + 1])) + 0.5 * (1 - alpha) * dw[k + 1]) / hz;
+      }
+    }
+    const double CdT = p->CdT[j], CdB = p->CdB[j];
+    if (multi) {
+      for (int k = et + 1; k < ne - 1; k++)
+        ut[k] += dt * (1 - th) * (a[k] * u[k - 1] - (a[k] + b[k]) * u[k] + b[k] * u[k + 1]);
+      if (CdT == -1)
+        ut[et] += dt * (1 - th) * (a[et] * -u[et] - (a[et] + b[et]) * u[et] + b[et] * u[et + 1]);
+      else
+        ut[et] += dt * (1 - th) * (-(b[et] + 2.0 * CdT * fabs(u[et]) / (z1[et] + z2[et])) * u[et] + b[et] * u[et + 1]);
+      const int kb = ne - 1;
+      if (CdB == -1)
+        ut[kb] += dt * (1 - th) * (a[kb] * u[kb - 1] - (a[kb] + b[kb]) * u[kb] + b[kb] * -u[kb]);
+      else
+        ut[kb] += dt * (1 - th) * (a[kb] * u[kb - 1] - (a[kb] + 2.0 * CdB * fabs(u[kb]) / (z1[kb] + z2[kb])) * u[kb]);
+    } else {
+      const double s = z1[et] + z2[et];
+      if (CdB == -1) ut[et] -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + c[et])) * u[et] / (s * s));
+      else ut[et] -= 2.0 * dt * (1 - th) * (CdB) / s * fabs(u[et]) * u[et];
+      if (CdT == -1) ut[et] -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + c[et])) * u[et] / (s * s));
+      else ut[et] -= 2.0 * dt * (1 - th) * (CdT) / s * fabs(u[et]) * u[et];
+    }
+    if (adv) {
+      for (int k = et + 1; k < ne - 1; k++)
+        ut[k] -= dt * (1 - tM) * (a0[k] * u[k - 1] + b0[k] * u[k] + c0[k] * u[k + 1]);
+      ut[et] -= dt * (1 - tM) * ((a0[et] + b0[et]) * u[et] + c0[et] * u[et + 1]);
+      const int kb = ne - 1;
+      ut[kb] -= dt * (1 - tM) * (a0[kb] * u[kb - 1] + (b0[kb] + c0[kb]) * u[kb]);
+    }
+    if (multi) {
+      const int kb = ne - 1;
+      c[et] = -th * dt * b[et];
+      if (CdT == -1) b[et] = 1.0 + th * dt * (a[et] + a[et + 1] + b[et]);
+      else b[et] = 1.0 + th * dt * (b[et] + 2.0 * CdT * fabs(u[et]) / (z1[et] + z2[et]));
+      a[et] = 0;
+      c[kb] = 0;
+      if (CdB == -1) b[kb] = 1.0 + th * dt * (a[kb] + b[kb] + b[kb - 1]);
+      else b[kb] = 1.0 + th * dt * (a[kb] + 2.0 * CdB * fabs(u[kb]) / (z1[kb] + z2[kb]));
+      a[kb] = -th * dt * a[kb];
+      for (int k = et + 1; k < kb; k++) {
+        c[k] = -th * dt * b[k];
+        b[k] = 1.0 + th * dt * (a[k] + b[k]);
+        a[k] = -th * dt * a[k];
+      }
+    } else {
+      const double s = z1[et] + z2[et];
+      b[et] = 1.0;
+      if (CdB == -1) b[et] += 4.0 * th * dt * 2.0 * (nu + c[et]) / (s * s);
+      else b[et] += 2.0 * th * dt * fabs(ut[et]) / s * (CdB);
+      if (CdT == -1) b[et] += 4.0 * th * dt * 2.0 * (nu + c[et]) / (s * s);
+      else b[et] += 2.0 * th * dt * fabs(ut[et]) / s * (CdT);
+    }
+    if (adv) {
+      const int kb = ne - 1;
+      for (int k = et + 1; k < kb; k++) {
+        a[k] += dt * tM * a0[k];
+        b[k] += dt * tM * b0[k];
+        c[k] += dt * tM * c0[k];
+      }
+      b[et] += dt * tM * (a0[et] + b0[et]);
+      c[et] += dt * tM * c0[et];
+      a[kb] += dt * tM * a0[kb];
+      b[kb] += dt * tM * (b0[kb] + c0[kb]);
+    }
+    for (int k = et; k < ne; k++)
+      if (z1[k] == 0 && z2[k] == 0) bad = 1;
+    if (bad) break;
+    double *E = p->ut[j];
+    if (multi) {
+      /* two Thomas solves with the same matrix (main/util.c:185-203):
+       * U*** from U** and E from ones; the pivots are the same for both */
+      bp[et] = b[et]; dp[et] = ut[et]; ep[et] = 1.0;
+      for (int k = et + 1; k < ne; k++) {
+        bp[k] = b[k] - a[k] * c[k - 1] / bp[k - 1];
+        dp[k] = ut[k] - a[k] * dp[k - 1] / bp[k - 1];
+        ep[k] = 1.0 - a[k] * ep[k - 1] / bp[k - 1];
+      }
+      ut[ne - 1] = dp[ne - 1] / bp[ne - 1];
+      E[ne - 1] = ep[ne - 1] / bp[ne - 1];
+      for (int k = ne - 2; k >= et; k--) {
+        ut[k] = dp[k] / bp[k] - c[k] * ut[k + 1] / bp[k];
+        E[k] = ep[k] / bp[k] - c[k] * E[k + 1] / bp[k];
+      }
+    } else {
+      ut[et] /= b[et];
+      E[et] = 1.0 / b[et];
+    }
+    p->D[j] = 0;
+    for (int k = et; k < ne; k++) p->D[j] += E[k] * g->dzf[j][k];
+  }
+  for (int j = 0; j < g->Ne && !bad; j++)
+    for (int k = g->etop[j]; k < g->Nke[j]; k++)
+      if (p->utmp[j][k] != p->utmp[j][k]) bad = 1;
+  free(a); free(b); free(c); free(d); free(a0); free(b0); free(c0); free(dw); free(bp); free(dp); free(ep);
+  return bad;
+}
+
+/* main/phys.c:2951-3018, the part of UPredictor after the open-boundary
+ * hooks: the free-surface source htmp = Ac h - dt sum of the theta-weighted
+ * face fluxes, CGSolve (cgsolver = 1), cells that dry below DRYCELLHEIGHT
+ * clamped (h, active, bottom scalar), and u = U*** - g theta dt E grad h on
+ * computational edges (zero on a dry one-layer edge).  Returns 1 for
+ * cgsolver != 1 (GSSolve is not restated). */
+int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
+  const double dt = prop->dt, theta = prop->theta;
+  const int mf = g->maxfaces;
+  if (prop->cgsolver != 1) return 1;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    double sum = 0;
+    for (int f = 0; f < g->nfaces[i]; f++) {
+      const int ne = g->face[i * mf + f], nrm = g->normal[i * mf + f];
+      for (int k = g->etop[ne]; k < g->Nke[ne]; k++)
+        sum += ((1 - theta) * p->u[ne][k] + theta * p->utmp[ne][k]) * g->df[ne] * nrm * g->dzf[ne][k];
+    }
+    p->htmp[i] = g->Ac[i] * p->h[i] - dt * sum;
+  }
+  oracle_CGSolve(g, p, prop);
+  for (int i = 0; i < g->Nc; i++)
+    if (p->h[i] <= -g->dv[i] + 1e-10) {
+      p->h[i] = -g->dv[i] + 1e-10;
+      p->active[i] = 0;
+      p->s[i][g->Nk[i] - 1] = 0;
+    } else {
+      p->hcorr[i] = 0;
+      p->active[i] = 1;
+    }
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp];
+    const int i1 = g->grad[2 * j], i2 = g->grad[2 * j + 1];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++)
+      p->u[j][k] = p->utmp[j][k] - prop->grav * theta * dt * p->ut[j][k] * (p->h[i1] - p->h[i2]) / g->dg[j];
+    if (g->etop[j] == g->Nke[j] - 1 && g->dzz[i1][g->etop[j]] == 0 && g->dzz[i2][g->etop[j]] == 0)
+      p->u[j][g->etop[j]] = 0;
+  }
+  return 0;
+}
+
 /* main/tvd.c:36-120.  Pass 1 builds, per cell-layer, the inflow volume flux
  * sumQ and the inflow-weighted scalar jump sumQC (in gradSx/gradSy); pass 2
  * limits the face values of each wet interior edge-layer by Psi of the

@@ -218,3 +218,31 @@ def ensemble_member(seed: int, Nc_target: int, aspect: float, Nkmax: int, dt=10.
     fill_scalar(st, rng)
     _finish(st, rng, dt, warm_q=False)
     return st
+
+
+def upredictor_inputs(st: State, rng, noslip: bool = True, nonlinear: int = 1, thetaM: float = 0.5,
+                      lax: float = 1.0) -> State:
+    """Inputs of UPredictor on a closed grid: the horizontal momentum source in
+    utmp, wind stress, drag coefficients (with no-slip, -1, on a third of the
+    edges of more than one wet layer when noslip), eddy viscosities, and the
+    properties it reads.  Single-layer edges never get no-slip (there the
+    reference reads a coefficient left from the previous edge)."""
+    edge, k = st.edge_layers()
+    st.set("utmp", rng.normal(0, 0.05, len(edge)) * st.edge_wet_mask())
+    ne = st.Ne
+    etop, nke = st.ints("etop"), st.ints("Nke")
+    multi = (nke - etop) > 1
+    cdb = rng.uniform(1e-3, 5e-3, ne)
+    cdt = rng.uniform(0.0, 2e-3, ne)
+    if noslip:
+        cdb[multi & (rng.random(ne) < 0.33)] = -1.0
+        cdt[multi & (rng.random(ne) < 0.33)] = -1.0
+    st.set("CdB", cdb)
+    st.set("CdT", cdt)
+    st.set("tau_T", rng.normal(0, 1e-4, ne))
+    n = len(st.get("nu_tv"))
+    st.set("nu_tv", rng.uniform(0, 1e-3, n))
+    st.set("nu_lax", rng.uniform(0, 1e-4, n))
+    st.set_prop(nu=1e-4, laxWendroff_Vertical=lax, nonlinear=nonlinear, thetaM=thetaM, cgsolver=1,
+                epsilon=1e-10, maxiters=1000, hprecond=1, grav=9.81)
+    return st

@@ -436,6 +436,96 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
   unpack_cells_w(C, grid, w, 0, 1);
 }
 
+/* write back levels [0, Nke) of every edge row */
+static void unpack_edges_wet(ctx_t *c, const gridT *g, REAL **rows) {
+  size_t o = 0;
+  for (int j = 0; j < g->Ne; j++) {
+    memcpy(rows[j], c->flat + o, (size_t)g->Nke[j] * sizeof(double));
+    o += (size_t)g->Nkc[j];
+  }
+}
+
+static sunhip_upred_params upred_params(const propT *prop) {
+  sunhip_upred_params P = {prop->dt, prop->theta, prop->grav, prop->nu, prop->laxWendroff_Vertical, prop->thetaM,
+                           prop->nonlinear};
+  return P;
+}
+
+void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
+  static const char *W = "UPredictor";
+  (void)myproc; (void)comm;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  for (int jp = grid->edgedist[0]; jp < grid->edgedist[1]; jp++) {
+    const int j = grid->edgep[jp];
+    if (grid->Nke[j] - grid->etop[j] == 1 && (phys->CdB[j] == -1 || phys->CdT[j] == -1))
+      fatal(W, "no-slip drag (CdB or CdT = -1) on a one-layer edge reads a coefficient left from the previous "
+               "edge in the reference (main/phys.c:2803-2815, 2880-2889); not built for the device");
+  }
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_edges(C, grid, "utmp", phys->utmp, W);
+  upload_edges(C, grid, "ue", phys->ut, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  upload_cells(C, grid, "w", phys->w, 1, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_cells(C, grid, "nu_tv", phys->nu_tv, 0, W);
+  upload_cells(C, grid, "nu_lax", phys->nu_lax, 0, W);
+  check(sunhip_upload(C->dev, "h", phys->h), W);
+  check(sunhip_upload(C->dev, "tau_T", phys->tau_T), W);
+  check(sunhip_upload(C->dev, "CdT", phys->CdT), W);
+  check(sunhip_upload(C->dev, "CdB", phys->CdB), W);
+  sunhip_upred_params P = upred_params(prop);
+  const int rc = sunhip_u_predict(C->dev, &P);
+  if (rc == -5) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(0); }  /* main/phys.c:2902 exits with 0 */
+  if (rc == -6) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(EXIT_FAILURE); }
+  check(rc, W);
+  check(sunhip_download(C->dev, "utmp", C->flat), W);
+  unpack_edges(C, grid, phys->utmp);
+  check(sunhip_download(C->dev, "ue", C->flat), W);
+  unpack_edges(C, grid, phys->ut);
+  check(sunhip_download(C->dev, "utmp2", C->flat), W);
+  unpack_edges_wet(C, grid, phys->utmp2);
+  check(sunhip_download(C->dev, "wtmp2", C->flat), W);
+  unpack_cells_w(C, grid, phys->wtmp2, 0, 2);
+  check(sunhip_download(C->dev, "D", C->flat), W);
+  memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
+}
+
+void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
+  static const char *W = "UPredictor";
+  (void)myproc; (void)comm;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  if (prop->cgsolver != 1) fatal(W, "cgsolver must be 1 (GSSolve is not built for the device)");
+  if (prop->hprecond != 0 && prop->hprecond != 1) fatal(W, "hprecond must be 0 or 1");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_edges(C, grid, "utmp", phys->utmp, W);
+  upload_edges(C, grid, "ue", phys->ut, W);
+  upload_edges(C, grid, "dzf", grid->dzf, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_cells(C, grid, "s", phys->s, 0, W);
+  check(sunhip_upload(C->dev, "h", phys->h), W);
+  check(sunhip_upload(C->dev, "dv", grid->dv), W);
+  check(sunhip_upload(C->dev, "hcorr", phys->hcorr), W);
+  check(sunhip_upload(C->dev, "D", phys->D), W);
+  sunhip_upred_params P = upred_params(prop);
+  sunhip_cg_params G = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0};
+  check(sunhip_u_solve(C->dev, &P, &G, &g_last[1].st), W);
+  g_last[1].valid = 1;
+  check(sunhip_download(C->dev, "h", C->flat), W);
+  memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
+  check(sunhip_download(C->dev, "hcorr", C->flat), W);
+  memcpy(phys->hcorr, C->flat, (size_t)grid->Nc * sizeof(double));
+  check(sunhip_download(C->dev, "active", C->flat), W);
+  for (int i = 0; i < grid->Nc; i++) phys->active[i] = C->flat[i] != 0;
+  check(sunhip_download(C->dev, "u", C->flat), W);
+  unpack_edges(C, grid, phys->u);
+  check(sunhip_download(C->dev, "s", C->flat), W);
+  unpack_cells(C, grid, phys->s, 0, 2);
+}
+
 int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) {
   if (which < 0 || which > 1 || !g_last[which].valid) return -1;
   if (niters) *niters = g_last[which].st.niters;

@@ -823,11 +823,11 @@ static double prop_access(propT *prop, const char *name, double value, int set, 
   *ok = 1;
   PROP_D(dt) PROP_D(theta) PROP_D(thetaS) PROP_D(thetaB) PROP_D(epsilon) PROP_D(qepsilon)
   PROP_D(resnorm) PROP_D(grav) PROP_D(beta) PROP_D(kappa_s) PROP_D(kappa_sH) PROP_D(gamma)
-  PROP_D(nu) PROP_D(nu_H) PROP_D(CdB)
+  PROP_D(nu) PROP_D(nu_H) PROP_D(CdB) PROP_D(thetaM) PROP_D(laxWendroff_Vertical) PROP_D(rtime)
   PROP_I(n) PROP_I(nstart) PROP_I(nsteps) PROP_I(maxiters) PROP_I(qmaxiters) PROP_I(hprecond)
   PROP_I(qprecond) PROP_I(TVD) PROP_I(horiTVD) PROP_I(vertTVD) PROP_I(TVDsalt) PROP_I(TVDtemp)
   PROP_I(nonhydrostatic) PROP_I(cgsolver) PROP_I(linearFS) PROP_I(stairstep) PROP_I(wetdry)
-  PROP_I(nonlinear)
+  PROP_I(nonlinear) PROP_I(thetaramptime) PROP_I(netcdfBdy)
   *ok = 0;
   return 0;
 }

@@ -34,10 +34,11 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 }
 
 Kind kind_of(const char *name) {
-  static const char *edge[] = {"dzf", "u", "utmp2", "u0", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
-  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh"};
+  static const char *edge[] = {"dzf", "u", "utmp", "utmp2", "u0", "ue", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
+  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
-                                "hx", "hy", "active", "htmp0"};
+                                "hx", "hy", "active", "htmp0", "dv", "hcorr"};
+  if (!strncmp(name, "up_", 3)) return EDGE;  // UPredictor's per-edge scratch
   for (auto n : edge) if (!strcmp(name, n)) return EDGE;
   for (auto n : edge2) if (!strcmp(name, n)) return EDGE2;
   for (auto n : cell2) if (!strcmp(name, n)) return CELL2;
@@ -440,7 +441,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
-                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32};
+                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);

@@ -123,6 +123,7 @@ struct sunhip_grid {
   int *d_nk = nullptr, *d_ctop = nullptr, *d_ctopold = nullptr, *d_comp = nullptr, *d_nf = nullptr;
   int *d_nbb = nullptr, *d_nbs = nullptr, *d_eb = nullptr, *d_es = nullptr, *d_nrm = nullptr, *d_kr = nullptr;
   int *d_xcode = nullptr, *d_pair = nullptr, *d_npair = nullptr;  // neighbour exchange (Geo)
+  int *d_uerr = nullptr;             // UPredictor's exit conditions
   int max_pairs = 0;                 // most cross-tile faces of any tile
   double *d_Ac = nullptr, *d_df = nullptr, *d_dg = nullptr;
   int *d_e_c1 = nullptr, *d_e_c2 = nullptr, *d_e_s1 = nullptr, *d_e_s2 = nullptr;

@@ -178,6 +178,16 @@ class State:
         k = np.arange(int(nkc.sum())) - np.repeat(start, nkc)
         return edge, k
 
+    def active(self) -> np.ndarray:
+        """phys->active (1 wet, 0 dried) as uint8."""
+        ptr = self._L.sun_phys_ptr(self.p, b"active")
+        return np.frombuffer(C.string_at(C.c_void_p(ptr), self.Nc), np.uint8).copy()
+
+    def set_active(self, values) -> None:
+        ptr = self._L.sun_phys_ptr(self.p, b"active")
+        a = np.ascontiguousarray(values, np.uint8)
+        C.memmove(C.c_void_p(ptr), a.ctypes.data, self.Nc)
+
     def wet_mask(self) -> np.ndarray:
         """Flat cell-field mask of the wet layers [ctop, Nk)."""
         cell, k = self.cell_layers()

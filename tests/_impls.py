@@ -81,6 +81,21 @@ class _Cpu:
     def face_scalars(self, st, tvd):
         self.fn("HorizontalFaceScalars")(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd))
 
+    def upredictor_begin(self, st):
+        return self.L.oracle_UPredictor_begin(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
+    def upredictor_solve(self, st):
+        return self.L.oracle_UPredictor_solve(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
+    def upredictor(self, st):
+        """The whole of UPredictor (reference), or its two device-covered
+        parts in order (oracle); returns nonzero where the reference exits."""
+        if self.pre == "ref_":
+            self.L.ref_UPredictor(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+            return 0
+        rc = self.L.oracle_UPredictor_begin(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+        return rc or self.L.oracle_UPredictor_solve(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
     def update_salinity(self, st):
         getattr(self.L, self.pre + "update_salinity")(C.c_void_p(st.g), C.c_void_p(st.p),
                                                       C.c_void_p(st.prop))
@@ -179,6 +194,12 @@ class DropinImpl:
 
     def face_scalars(self, st, tvd):
         self.L.HorizontalFaceScalars(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd), 0, 0)
+
+    def upredictor_begin(self, st):
+        self.L.sunhip_UPredictor_begin(st.g, st.p, st.prop, 0, 1, 0)
+
+    def upredictor_solve(self, st):
+        self.L.sunhip_UPredictor_solve(st.g, st.p, st.prop, 0, 1, 0)
 
     def update_salinity(self, st):
         kappa = st.get_prop("kappa_s")

@@ -227,3 +227,40 @@ def test_projection_zeroes_the_source(name, make):
     cell, k = st.cell_layers()
     multi = (st.ints("Nk") - st.ints("ctop") > 1)[cell]
     assert np.array_equal(st.get("q")[m & multi], (q0 + st.get("qc"))[m & multi])
+
+
+def test_upredictor_inviscid_closed_form():
+    """No viscosity, drag, wind or momentum advection: every vertical system
+    is the identity, so E = 1, D = sum of dzf over the edge's wet layers
+    (the depth-integrated coefficient, main/phys.c:2936-2939), U*** = U**
+    (the source minus the explicit free-surface gradient), and after the
+    free-surface solve u = U*** - g theta dt (h1 - h2)/dg."""
+    o = _impls.oracle()
+    st = cases.tiny(11, nx=23, ny=9, Nkmax=13)
+    cases.upredictor_inputs(st, np.random.default_rng(3), noslip=False, nonlinear=0)
+    st.set("nu_tv", 0.0)
+    st.set("nu_lax", 0.0)
+    st.set("CdB", 0.0)
+    st.set("CdT", 0.0)
+    st.set("tau_T", 0.0)
+    st.set_prop(nu=0.0)
+    g, th, dt = st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt")
+    edge, k = st.edge_layers()
+    grad = st.ints("grad").reshape(-1, 2)
+    etop, nke, mark = st.ints("etop"), st.ints("Nke"), st.ints("mark")
+    comp = (mark[edge] == 0) & (k >= etop[edge]) & (k < nke[edge])
+    h0, dg = st.get("h"), st.get("dg")
+    grad_h = (h0[grad[edge, 0]] - h0[grad[edge, 1]]) / dg[edge]
+    expect = st.get("utmp") - g * (1 - th) * dt * grad_h
+    assert o.upredictor_begin(st) == 0
+    assert np.array_equal(st.get("ut")[comp], np.ones(comp.sum()))
+    assert np.allclose(st.get("utmp")[comp], expect[comp], rtol=1e-15, atol=1e-17)
+    D = st.get("D")
+    wet = comp & (st.get("dzf") > 0)
+    depth = np.bincount(edge[wet], weights=st.get("dzf")[wet], minlength=st.Ne)
+    assert np.allclose(D[mark == 0], depth[mark == 0], rtol=1e-14)
+    ustar = st.get("utmp").copy()
+    assert o.upredictor_solve(st) == 0
+    h1 = st.get("h")
+    grad_h1 = (h1[grad[edge, 0]] - h1[grad[edge, 1]]) / dg[edge]
+    assert np.allclose(st.get("u")[comp], (ustar - g * th * dt * grad_h1)[comp], rtol=1e-14, atol=1e-16)

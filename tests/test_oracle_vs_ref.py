@@ -154,3 +154,36 @@ def test_compute_qsource_all_planforms(name, make):
     ref.compute_qsource(b)
     assert np.array_equal(a.get("stmp"), b.get("stmp"))
     assert np.array_equal(a.get("D"), b.get("D"))
+
+
+UPRED = [("tiny0", lambda: cases.tiny(0)), ("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)),
+         ("lockexchange", lambda: cases.lock_exchange(seed=7)),
+         # shallow: one-layer edges (the single-layer drag branch)
+         ("shallow31", lambda: cases.tiny(31, nx=12, ny=8, Nkmax=6, min_layers=-1.2)),
+         ("shallow36", lambda: cases.tiny(36, nx=12, ny=8, Nkmax=6, min_layers=-1.2)),
+         ("quadtri", lambda: cases.tiny(22, nx=7, ny=5, planform="quad", split_every=3)),
+         ("hexquad7", lambda: cases.tiny(24, nx=7, ny=6, planform="hex", split_every=4, maxfaces=7))]
+
+
+@pytest.mark.parametrize("name,make", UPRED, ids=[u[0] for u in UPRED])
+@pytest.mark.parametrize("variant", ["advection", "linear", "no-advection-implicit", "drying"])
+def test_upredictor(name, make, variant):
+    """The oracle's two parts of UPredictor (before and after the
+    open-boundary hooks, which do nothing on a closed grid) against the
+    reference's whole UPredictor: every field they write, bit for bit."""
+    o = _impls.oracle()
+    a = make()
+    kw = {"advection": {}, "linear": dict(nonlinear=0), "no-advection-implicit": dict(thetaM=-1.0, lax=0.0),
+          "drying": {}}
+    cases.upredictor_inputs(a, np.random.default_rng(5), **kw[variant])
+    if variant == "drying":  # zero depth under a fifth of the cells: those with h <= 1e-10 dry
+        dv = a.get("dv")
+        dv[np.random.default_rng(6).random(a.Nc) < 0.2] = 0.0
+        a.set("dv", dv)
+    b = a.clone()
+    assert o.upredictor(a) == 0
+    ref.upredictor(b)
+    assert np.isfinite(a.get("h")).all()
+    for f in ("utmp", "ut", "u", "D", "h", "htmp", "wtmp2", "utmp2", "hcorr", "s"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert np.array_equal(a.active(), b.active())
