@@ -158,6 +158,48 @@ def tvd_leg(st, dg, N, Ek, steps=5):
     return res
 
 
+def upred_leg(st, dg, N, Ek, steps=3):
+    """UPredictor on the large grid, all fields resident: the per-edge
+    vertical systems (viscosity, drag, vertical advection at thetaM = 0.5)
+    solved for U*** and E, then the free-surface source, PCG, drying and
+    velocity update.  Seeded inputs as cases.upredictor_inputs draws them."""
+    from suntans_amd import device
+    rng = np.random.default_rng(11)
+    u = st.get("u")
+    ne, nflat = st.Ne, len(st.get("dzz"))
+    dg.upload("u", u)
+    dg.upload("u0", u)
+    dg.upload("w", st.get("w"))
+    dg.upload("utmp", rng.normal(0, 0.05, len(u)) * st.edge_wet_mask())
+    dg.copy("up_src", "utmp")
+    dg.upload("nu_tv", rng.uniform(0, 1e-3, nflat))
+    dg.upload("nu_lax", rng.uniform(0, 1e-4, nflat))
+    dg.upload("CdB", rng.uniform(1e-3, 5e-3, ne))
+    dg.upload("CdT", rng.uniform(0.0, 2e-3, ne))
+    dg.upload("tau_T", rng.normal(0, 1e-4, ne))
+    dg.upload("dv", st.get("dv"))
+    dg.upload("h", np.zeros(st.Nc))
+    P = device.UPredParams(st.get_prop("dt"), st.get_prop("theta"), st.get_prop("grav"), 1e-4, 1.0, 0.5, 1)
+    tp, ts, it = [], [], []
+    for _ in range(steps + 1):
+        dg.copy("utmp", "up_src")
+        dg.copy("u", "u0")
+        dg.scale("h", "dv", 0.0)
+        dg.sync()
+        t = time.perf_counter()
+        dg.u_predict(P)
+        t1 = time.perf_counter()
+        st_h = dg.u_solve(P, maxiters=1000, precond=1, epsilon=1e-10)
+        t2 = time.perf_counter()
+        tp.append(t1 - t)
+        ts.append(t2 - t1)
+        it.append(st_h["niters"])
+    res = {"predict_ms": float(np.median(tp[1:])) * 1e3, "solve_ms": float(np.median(ts[1:])) * 1e3,
+           "h_iterations": it[-1], "edges": ne, "edge_layers": int(len(u))}
+    log(f"upredictor: {res}")
+    return res
+
+
 def ensemble_specs(n: int, seed: int):
     """Member sizes of the ensemble: cells U[5e3, 2e5], aspect U[1, 4],
     levels U{20..100} (seeded)."""
@@ -487,6 +529,7 @@ def main():
         }
         dg.upload("D", st.get("D"))
         out["tvd"] = tvd_leg(st, dg, N, Ek)
+        out["upredictor"] = upred_leg(st, dg, N, Ek)
 
     if args.copy_probe:
         cp_ms = dg.copy_probe(1 << 30, 20)

@@ -47,6 +47,13 @@ class ScalarParams(C.Structure):
                 ("step", C.c_int), ("alpha_top", C.c_int), ("alpha_bot", C.c_int)]
 
 
+class UPredParams(C.Structure):
+    """sunhip_upred_params: dt, theta, grav, nu, laxWendroff_Vertical,
+    thetaM (< 0: no vertical momentum advection), nonlinear."""
+    _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("grav", C.c_double), ("nu", C.c_double),
+                ("lax_vertical", C.c_double), ("thetaM", C.c_double), ("nonlinear", C.c_int)]
+
+
 class HPhys(C.Structure):
     _fields_ = [("grav", C.c_double), ("theta", C.c_double), ("dt", C.c_double)]
 
@@ -107,6 +114,8 @@ def lib() -> C.CDLL:
         L.sunhip_batch_q_solve_mixed.argtypes = [vp, pp, C.c_int, C.POINTER(CGParams), C.c_double, C.c_int,
                                                  C.POINTER(CGStats), _ip]
         L.sunhip_batch_update_scalars.argtypes = [vp, pp, C.c_int, C.POINTER(ScalarParams), C.c_char_p]
+        L.sunhip_u_predict.argtypes = [vp, C.POINTER(UPredParams)]
+        L.sunhip_u_solve.argtypes = [vp, C.POINTER(UPredParams), C.POINTER(CGParams), C.POINTER(CGStats)]
         L.sunhip_compute_qsource.argtypes = [vp, C.c_double, C.c_double, C.c_char_p]
         L.sunhip_corrector.argtypes = [vp, C.c_double, C.c_char_p, C.c_char_p]
         L.sunhip_continuity.argtypes = [vp, C.c_double, C.c_char_p]
@@ -235,6 +244,17 @@ class DeviceGrid:
     def update_scalars(self, theta, dt, kappa, TVD, step=2, scal="s"):
         S = ScalarParams(theta, dt, kappa, TVD, step, 0, 0)
         check(lib().sunhip_update_scalars(self.h, C.byref(S), scal.encode()), "sunhip_update_scalars")
+
+    def u_predict(self, P: "UPredParams"):
+        """UPredictor before the open-boundary hooks: U*** in utmp, E in ue, D."""
+        check(lib().sunhip_u_predict(self.h, C.byref(P)), "sunhip_u_predict")
+
+    def u_solve(self, P: "UPredParams", maxiters=1000, precond=1, resnorm=0, epsilon=1e-10):
+        """UPredictor after the hooks: htmp, free-surface PCG, drying, u."""
+        prm = CGParams(maxiters, precond, resnorm, epsilon, 0)
+        S = CGStats()
+        check(lib().sunhip_u_solve(self.h, C.byref(P), C.byref(prm), C.byref(S)), "sunhip_u_solve")
+        return S.as_dict()
 
     def compute_qsource(self, theta, dt, src="src"):
         """ComputeQSource: src from u, utmp2, w, wtmp2, dzf; D <- df/dg."""
