@@ -194,6 +194,25 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
  * "kappa_tv","u","utmp2","dzf"; writes scal and "Cn". */
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
 
+/* ---- the vertical grid that moves with the free surface -------------------
+ * UpdateDZ (main/phys.c:816-968) and SetFluxHeight (:4338-4435) on the
+ * device; the per-face level ranges are rebuilt from the new extents there,
+ * so a resident simulation needs no sunhip_set_extents between steps.
+ * Fields: cell "dzz", "dzzold"; cell 2-D "h", "dv", "dzbot"; edge "u",
+ * "dzf"; edge 2-D "dzfB".  Bit-identical to the reference. */
+typedef struct {
+  const double *dz;        /* [Nkmax] reference layer thicknesses (grid->dz), host memory */
+  double dzsmall, smoothbot;
+  int stairstep, fixdzz, linearFS;
+} sunhip_vgrid_params;
+/* option 0 (each step), -1 (initialisation from h = 0) or 1 (after it),
+ * as the reference's UpdateDZ; linearFS makes it a no-op. */
+int sunhip_update_dz(sunhip_grid *g, const sunhip_vgrid_params *vp, int option);
+int sunhip_set_flux_height(sunhip_grid *g, const sunhip_vgrid_params *vp);
+/* The device's current extents in the reference's cell/edge order (any
+ * pointer may be NULL). */
+int sunhip_get_extents(sunhip_grid *g, int *ctop, int *ctopold, int *etop, int *etopold);
+
 /* ---- the horizontal-momentum predictor (UPredictor, main/phys.c:2579-3018) --
  * In two parts either side of the reference's open-boundary hooks
  * (BoundaryVelocities, OpenBoundaryFluxes, user code of each case).  Fields:

@@ -78,6 +78,9 @@ long sun_grid_set_int(gridT *g, const char *name, const int *in);
 /* Scalars: Nc, Ne, Np, Nkmax, maxfaces, N (= sum Nk), Ek (edge-layers with two
  * wet neighbours). */
 long sun_grid_scalar(const gridT *g, const char *name);
+/* gridT's vertical-grid parameters smoothbot, dzsmall, fixdzz, stairstep. */
+int sun_grid_set_param(gridT *g, const char *name, double value);
+double sun_grid_param(const gridT *g, const char *name);
 /* The member itself (a REAL** or REAL* row table) of physT / gridT by name,
  * for passing to functions with the reference's signatures. */
 void *sun_phys_ptr(const physT *p, const char *name);

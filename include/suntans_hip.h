@@ -51,6 +51,10 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop);
  * by these names: the split is where a device call cannot reach the hooks. */
 void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
 void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
+/* main/phys.c:816 (UpdateDZ; prototype main/phys.h:220) */
+void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option);
+/* main/phys.c:4338 (SetFluxHeight; prototype main/phys.c:103) */
+void SetFluxHeight(gridT *grid, physT *phys, propT *prop);
 /* main/tvd.h:27-28 (definition main/tvd.c:36) */
 void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal,
                            int TVD, MPI_Comm comm, int myproc);

@@ -629,6 +629,129 @@ int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
   return 0;
 }
 
+static double ref_max(double x1, double x2) { return x2 > x1 ? x2 : x1; }  /* main/util.c:212-217 */
+static double ref_min(double x, double y) { return x < y ? x : y; }      /* main/util.c:241-246 */
+static double ref_upwind(double u, double a, double b) {                 /* main/util.c:219-232 */
+  if (fabs(u) > 0) return u > 0 ? b : a;
+  return ref_max(a, b);
+}
+
+/* main/phys.c:816-968.  The layer heights dzz and the top cell ctop of every
+ * column from the free surface h (a column's cells are full below the
+ * surface cell, the bottom cell reaches the bed dv, cells above the surface
+ * are empty), and etop = the smaller ctop of an edge's cells.  option 0
+ * first keeps the old extents and heights (etopold, ctopold, dzzold);
+ * option -1 (initialisation) starts from h = 0 and records the bottom cell
+ * height (dzbot), deepening dv where fixdzz asks; option 1 afterwards copies
+ * the new state into the old one and sets the bottom flux height dzfB. */
+void oracle_UpdateDZ(gridT *g, physT *p, propT *prop, int option) {
+  const int Nc = g->Nc, Ne = g->Ne;
+  if (prop->linearFS) return;
+  if (option == 0) {
+    for (int j = 0; j < Ne; j++) g->etopold[j] = g->etop[j];
+    for (int i = 0; i < Nc; i++) {
+      g->ctopold[i] = g->ctop[i];
+      for (int k = 0; k < g->ctop[i]; k++) g->dzzold[i][k] = 0;
+      for (int k = g->ctop[i]; k < g->Nk[i]; k++) g->dzzold[i][k] = g->dzz[i][k];
+    }
+  }
+  if (option == -1)
+    for (int i = 0; i < Nc; i++) p->h[i] = 0.0;
+  for (int i = 0; i < Nc; i++) {
+    double z = 0;
+    for (int k = 0; k < g->Nk[i]; k++) z -= g->dz[k];
+    g->dzz[i][g->Nk[i] - 1] = g->dz[g->Nk[i] - 1] + g->dv[i] + z;
+  }
+  if (g->Nkmax > 1) {
+    for (int i = 0; i < Nc; i++) {
+      const int n = g->Nk[i];
+      double z = 0;
+      int found = 0;
+      for (int k = 0; k < n; k++) {
+        z -= g->dz[k];
+        if (p->h[i] >= z) {
+          if (!found) {
+            if (k == n - 1) { g->dzz[i][k] = p->h[i] + g->dv[i]; g->ctop[i] = k; }
+            else if (p->h[i] == z) { g->dzz[i][k] = 0; g->ctop[i] = k + 1; }
+            else { g->dzz[i][k] = p->h[i] - z; g->ctop[i] = k; }
+            found = 1;
+          } else if (k == n - 1) {
+            g->dzz[i][k] = g->dz[k] + g->dv[i] + z;
+          } else {
+            g->dzz[i][k] = z < -g->dv[i] ? 0 : g->dz[k];
+          }
+        } else {
+          g->dzz[i][k] = 0;
+        }
+      }
+    }
+  } else {
+    for (int i = 0; i < Nc; i++) g->dzz[i][0] = g->dv[i] + p->h[i];
+  }
+  for (int j = 0; j < Ne; j++) {
+    const int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+    if (a == -1) g->etop[j] = g->ctop[b];
+    else if (b == -1) g->etop[j] = g->ctop[a];
+    else g->etop[j] = g->ctop[a] < g->ctop[b] ? g->ctop[a] : g->ctop[b];
+  }
+  if (option == -1)
+    for (int i = 0; i < Nc; i++) {
+      const int k = g->Nk[i] - 1;
+      g->dzbot[i] = g->dzz[i][k];
+      if (!g->stairstep && g->fixdzz && g->dzz[i][k] < g->dz[k] * g->dzsmall)
+        g->dv[i] += (g->dz[k] * g->dzsmall - g->dzz[i][k]);
+    }
+  if (option == 1) {
+    for (int j = 0; j < Ne; j++) g->etopold[j] = g->etop[j];
+    for (int i = 0; i < Nc; i++) {
+      g->ctopold[i] = g->ctop[i];
+      for (int k = 0; k < g->Nk[i]; k++) g->dzzold[i][k] = g->dzz[i][k];
+    }
+    for (int j = 0; j < Ne; j++) {
+      int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+      if (a == -1) a = b;
+      if (b == -1) b = a;
+      double z1 = g->dzz[a][g->Nk[a] - 1], z2 = g->dzz[b][g->Nk[b] - 1], z = 0;
+      for (int k = 0; k < g->Nke[j] - 1; k++) z -= g->dz[k];
+      if (p->h[a] < z) z1 = z1 - z + p->h[a];
+      if (p->h[b] < z) z2 = z2 - z + p->h[b];
+      g->dzfB[j] = ref_min(z1, z2);
+    }
+  }
+}
+
+/* main/phys.c:4338-4435.  Flux-face heights: zero everywhere, then on the
+ * wet layers of each edge the upwind cell height by the sign of u (the
+ * larger for u = 0); the bottom wet layer takes the smaller cell height, or
+ * for a one-layer edge the upwind free surface over the shallower bed
+ * (not below 0); heights at or below DRYCELLHEIGHT are zeroed.  smoothbot
+ * bounds the bottom cell heights from below before and after. */
+void oracle_SetFluxHeight(gridT *g, physT *p, propT *prop) {
+  (void)prop;
+  for (int j = 0; j < g->Ne; j++)
+    for (int k = 0; k < g->Nkc[j]; k++) g->dzf[j][k] = 0;
+  if (g->smoothbot)
+    for (int i = 0; i < g->Nc; i++)
+      g->dzz[i][g->Nk[i] - 1] = ref_max(g->dzbot[i], g->smoothbot * g->dz[g->Nk[i] - 1]);
+  for (int j = 0; j < g->Ne; j++) {
+    int a = g->grad[2 * j], b = g->grad[2 * j + 1];
+    if (a == -1) a = b;
+    if (b == -1) b = a;
+    for (int k = 0; k < g->etop[j]; k++) g->dzf[j][k] = 0;
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) g->dzf[j][k] = ref_upwind(p->u[j][k], g->dzz[a][k], g->dzz[b][k]);
+    const int k = g->Nke[j] - 1;
+    if (g->etop[j] == k)
+      g->dzf[j][k] = ref_max(0, ref_upwind(p->u[j][k], p->h[a], p->h[b]) + ref_min(g->dv[a], g->dv[b]));
+    else
+      g->dzf[j][k] = ref_min(g->dzz[a][k], g->dzz[b][k]);
+    for (int kk = g->etop[j]; kk < g->Nke[j]; kk++)
+      if (g->dzf[j][kk] <= 1e-10) g->dzf[j][kk] = 0;
+  }
+  if (g->smoothbot)
+    for (int i = 0; i < g->Nc; i++)
+      g->dzz[i][g->Nk[i] - 1] = ref_max(g->dzz[i][g->Nk[i] - 1], g->dzsmall * g->dz[g->Nk[i] - 1]);
+}
+
 /* main/tvd.c:36-120.  Pass 1 builds, per cell-layer, the inflow volume flux
  * sumQ and the inflow-weighted scalar jump sumQC (in gradSx/gradSy); pass 2
  * limits the face values of each wet interior edge-layer by Psi of the

@@ -44,6 +44,8 @@ void oracle_ComputeQSource(REAL **src, gridT *g, physT *p, propT *prop);
 void oracle_Corrector(REAL **qc, gridT *g, physT *p, propT *prop);
 void oracle_Continuity(REAL **w, gridT *g, physT *p, propT *prop);
 int oracle_UPredictor_begin(gridT *g, physT *p, propT *prop);
+void oracle_UpdateDZ(gridT *g, physT *p, propT *prop, int option);
+void oracle_SetFluxHeight(gridT *g, physT *p, propT *prop);
 int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop);
 void oracle_HorizontalFaceScalars(gridT *g, physT *p, propT *prop, REAL **scal, REAL **boundary_scal,
                                   int TVD);

@@ -47,6 +47,8 @@ EXPORT void ref_ComputeQSource(REAL **src, gridT *g, physT *p, propT *pr) {
 }
 EXPORT void ref_Corrector(REAL **qc, gridT *g, physT *p, propT *pr) { Corrector(qc, g, p, pr, 0, 1, 0); }
 EXPORT void ref_Continuity(REAL **w, gridT *g, physT *p, propT *pr) { Continuity(w, g, p, pr); }
+EXPORT void ref_UpdateDZ(gridT *g, physT *p, propT *pr, int option) { UpdateDZ(g, p, pr, option); }
+EXPORT void ref_SetFluxHeight(gridT *g, physT *p, propT *pr) { SetFluxHeight(g, p, pr); }
 EXPORT void ref_UPredictor(gridT *g, physT *p, propT *pr) { UPredictor(g, p, pr, 0, 1, 0); }
 EXPORT void ref_HorizontalFaceScalars(gridT *g, physT *p, propT *pr, REAL **scal, REAL **bscal, int TVD) {
   HorizontalFaceScalars(g, p, pr, scal, bscal, TVD, 0, 0);

@@ -526,6 +526,62 @@ void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, 
   unpack_cells(C, grid, phys->s, 0, 2);
 }
 
+static sunhip_vgrid_params vgrid_params(const gridT *grid, const propT *prop) {
+  sunhip_vgrid_params V = {grid->dz, grid->dzsmall, grid->smoothbot, grid->stairstep, grid->fixdzz, prop->linearFS};
+  return V;
+}
+
+void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option) {
+  static const char *W = "UpdateDZ";
+  if (prop->linearFS) return;  /* main/phys.c:820-822 */
+  if (option != 0 && option != 1 && option != -1) fatal(W, "option must be 0, 1 or -1");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  check(sunhip_upload(C->dev, "h", phys->h), W);
+  check(sunhip_upload(C->dev, "dv", grid->dv), W);
+  sunhip_vgrid_params V = vgrid_params(grid, prop);
+  check(sunhip_update_dz(C->dev, &V, option), W);
+  check(sunhip_download(C->dev, "dzz", C->flat), W);
+  unpack_cells(C, grid, grid->dzz, 0, 2);
+  check(sunhip_get_extents(C->dev, grid->ctop, option != -1 ? grid->ctopold : NULL, grid->etop,
+                           option != -1 ? grid->etopold : NULL), W);
+  if (option != -1) {
+    check(sunhip_download(C->dev, "dzzold", C->flat), W);
+    unpack_cells(C, grid, grid->dzzold, 0, 2);
+  } else {
+    check(sunhip_download(C->dev, "h", C->flat), W);
+    memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
+    check(sunhip_download(C->dev, "dv", C->flat), W);
+    memcpy(grid->dv, C->flat, (size_t)grid->Nc * sizeof(double));
+    check(sunhip_download(C->dev, "dzbot", C->flat), W);
+    memcpy(grid->dzbot, C->flat, (size_t)grid->Nc * sizeof(double));
+  }
+  if (option == 1) {
+    check(sunhip_download(C->dev, "dzfB", C->flat), W);
+    memcpy(grid->dzfB, C->flat, (size_t)grid->Ne * sizeof(double));
+  }
+}
+
+void SetFluxHeight(gridT *grid, physT *phys, propT *prop) {
+  static const char *W = "SetFluxHeight";
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_edges(C, grid, "u", phys->u, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  check(sunhip_upload(C->dev, "h", phys->h), W);
+  check(sunhip_upload(C->dev, "dv", grid->dv), W);
+  if (grid->smoothbot) check(sunhip_upload(C->dev, "dzbot", grid->dzbot), W);
+  sunhip_vgrid_params V = vgrid_params(grid, prop);
+  check(sunhip_set_flux_height(C->dev, &V), W);
+  check(sunhip_download(C->dev, "dzf", C->flat), W);
+  unpack_edges(C, grid, grid->dzf);
+  if (grid->smoothbot) {
+    check(sunhip_download(C->dev, "dzz", C->flat), W);
+    unpack_cells(C, grid, grid->dzz, 0, 2);
+  }
+}
+
 int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) {
   if (which < 0 || which > 1 || !g_last[which].valid) return -1;
   if (niters) *niters = g_last[which].st.niters;

@@ -684,6 +684,7 @@ static int grid_real(const gridT *g, const char *name, int *kind, REAL ***rows, 
   else if (!strcmp(name, "xe")) { *kind = 5; *vec = g->xe; }
   else if (!strcmp(name, "ye")) { *kind = 5; *vec = g->ye; }
   else if (!strcmp(name, "dzfB")) { *kind = 5; *vec = g->dzfB; }
+  else if (!strcmp(name, "dzbot")) { *kind = 4; *vec = g->dzbot; }
   else if (!strcmp(name, "dz")) { *kind = 7; *vec = g->dz; }
   else return 0;
   return 1;
@@ -792,6 +793,23 @@ void *sun_grid_ptr(const gridT *g, const char *name) {
 
 size_t sun_prop_size(void) { return sizeof(propT); }
 void sun_prop_copy(const propT *from, propT *to) { memcpy(to, from, sizeof(propT)); }
+
+int sun_grid_set_param(gridT *g, const char *name, double value) {
+  if (!strcmp(name, "smoothbot")) g->smoothbot = value;
+  else if (!strcmp(name, "dzsmall")) g->dzsmall = value;
+  else if (!strcmp(name, "fixdzz")) g->fixdzz = (int)value;
+  else if (!strcmp(name, "stairstep")) g->stairstep = (int)value;
+  else return -1;
+  return 0;
+}
+
+double sun_grid_param(const gridT *g, const char *name) {
+  if (!strcmp(name, "smoothbot")) return g->smoothbot;
+  if (!strcmp(name, "dzsmall")) return g->dzsmall;
+  if (!strcmp(name, "fixdzz")) return g->fixdzz;
+  if (!strcmp(name, "stairstep")) return g->stairstep;
+  return 0;
+}
 
 long sun_grid_scalar(const gridT *g, const char *name) {
   if (!strcmp(name, "Nc")) return g->Nc;

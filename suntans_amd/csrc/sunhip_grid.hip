@@ -35,9 +35,9 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 
 Kind kind_of(const char *name) {
   static const char *edge[] = {"dzf", "u", "utmp", "utmp2", "u0", "ue", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
-  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB"};
+  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB", "dzfB"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
-                                "hx", "hy", "active", "htmp0", "dv", "hcorr"};
+                                "hx", "hy", "active", "htmp0", "dv", "hcorr", "dzbot"};
   if (!strncmp(name, "up_", 3)) return EDGE;  // UPredictor's per-edge scratch
   for (auto n : edge) if (!strcmp(name, n)) return EDGE;
   for (auto n : edge2) if (!strcmp(name, n)) return EDGE2;
@@ -148,6 +148,15 @@ static uint32_t spread16(uint32_t v) {
   v = (v | (v << 1)) & 0x55555555;
   return v;
 }
+
+namespace sunhip {
+int refresh_face_ranges(sunhip_grid *g) {
+  k_face_ranges<<<(g->NS + 255) / 256, 256, 0, g->stream>>>(g->NS, g->mf, g->d_nbs, g->d_es, g->d_ctop, g->d_e_nke,
+                                                            g->d_kr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace sunhip
 
 template <class T>
 static int upload_vec(sunhip_grid *g, T **d, const std::vector<T> &h) {
@@ -406,6 +415,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= upload_vec(g, &g->d_kr, kr0);
   rc |= upload_vec(g, &g->d_e_nke, ezeros);
   rc |= upload_vec(g, &g->d_e_etop, ezeros);
+  rc |= upload_vec(g, &g->d_e_etopold, ezeros);
   // reductions: one partial pair per block of the widest launch
   const int nblk = std::max(g->Nt, g->Net) + 64;
   g->part_n = 4 * (int64_t)nblk;
@@ -441,7 +451,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
-                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr};
+                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);
@@ -459,6 +469,28 @@ int64_t sunhip_grid_bytes(const sunhip_grid *g) { return g ? g->bytes : 0; }
 
 int sunhip_grid_sizes(const sunhip_grid *g, int64_t *o) {
   o[0] = g->Nflat; o[1] = g->NflatW; o[2] = g->NflatE; o[3] = g->NT; o[4] = g->Nt; o[5] = g->NE;
+  return 0;
+}
+
+int sunhip_get_extents(sunhip_grid *g, int *ctop, int *ctopold, int *etop, int *etopold) {
+  HIPCHK(hipSetDevice(g->device));
+  const int NS = g->NS, NES = g->NES;
+  std::vector<int> ct(NS), cto(NS), et(NES), eto(NES);
+  HIPCHK(hipMemcpyAsync(ct.data(), g->d_ctop, NS * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(cto.data(), g->d_ctopold, NS * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(et.data(), g->d_e_etop, NES * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(eto.data(), g->d_e_etopold, NES * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (int c = 0; c < g->Nc; c++) {
+    const int s = g->slot_of_cell[c];
+    if (ctop) ctop[c] = ct[s];
+    if (ctopold) ctopold[c] = cto[s];
+  }
+  for (int e = 0; e < g->Ne; e++) {
+    const int s = g->slot_of_edge[e];
+    if (etop) etop[e] = et[s];
+    if (etopold) etopold[e] = eto[s];
+  }
   return 0;
 }
 
@@ -488,9 +520,8 @@ int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
   HIPCHK(hipMemcpyAsync(g->d_ctopold, cto.data(), NS * sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->d_e_nke, nke.data(), NES * sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->d_e_etop, et.data(), NES * sizeof(int), hipMemcpyHostToDevice, g->stream));
-  k_face_ranges<<<(NS + 255) / 256, 256, 0, g->stream>>>(NS, g->mf, g->d_nbs, g->d_es, g->d_ctop, g->d_e_nke,
-                                                          g->d_kr);
-  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g->d_e_etopold, et.data(), NES * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  if (int rc = refresh_face_ranges(g)) return rc;
   HIPCHK(hipStreamSynchronize(g->stream));  // the host vectors go out of scope
   g->have_extents = true;
   return 0;

@@ -124,6 +124,8 @@ struct sunhip_grid {
   int *d_nbb = nullptr, *d_nbs = nullptr, *d_eb = nullptr, *d_es = nullptr, *d_nrm = nullptr, *d_kr = nullptr;
   int *d_xcode = nullptr, *d_pair = nullptr, *d_npair = nullptr;  // neighbour exchange (Geo)
   int *d_uerr = nullptr;             // UPredictor's exit conditions
+  double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)
+  int *d_e_etopold = nullptr;        // [NES] etop before the last UpdateDZ
   int max_pairs = 0;                 // most cross-tile faces of any tile
   double *d_Ac = nullptr, *d_df = nullptr, *d_dg = nullptr;
   int *d_e_c1 = nullptr, *d_e_c2 = nullptr, *d_e_s1 = nullptr, *d_e_s2 = nullptr;
@@ -202,5 +204,7 @@ int field(sunhip_grid *g, const char *name, Field **out);  // create on demand
 int field32(sunhip_grid *g, const char *name, float **out);
 Kind kind_of(const char *name);
 int dev_alloc(sunhip_grid *g, void **p, size_t bytes);
+// Per-face level ranges (kmin | kmax << 16) from the device extents.
+int refresh_face_ranges(sunhip_grid *g);
 void batch_forget_grid(sunhip_batch *b, sunhip_grid *g);
 }  // namespace sunhip

@@ -47,6 +47,11 @@ class ScalarParams(C.Structure):
                 ("step", C.c_int), ("alpha_top", C.c_int), ("alpha_bot", C.c_int)]
 
 
+class VGridParams(C.Structure):
+    _fields_ = [("dz", _dp), ("dzsmall", C.c_double), ("smoothbot", C.c_double), ("stairstep", C.c_int),
+                ("fixdzz", C.c_int), ("linearFS", C.c_int)]
+
+
 class UPredParams(C.Structure):
     """sunhip_upred_params: dt, theta, grav, nu, laxWendroff_Vertical,
     thetaM (< 0: no vertical momentum advection), nonlinear."""
@@ -115,6 +120,9 @@ def lib() -> C.CDLL:
                                                  C.POINTER(CGStats), _ip]
         L.sunhip_batch_update_scalars.argtypes = [vp, pp, C.c_int, C.POINTER(ScalarParams), C.c_char_p]
         L.sunhip_u_predict.argtypes = [vp, C.POINTER(UPredParams)]
+        L.sunhip_update_dz.argtypes = [vp, C.POINTER(VGridParams), C.c_int]
+        L.sunhip_set_flux_height.argtypes = [vp, C.POINTER(VGridParams)]
+        L.sunhip_get_extents.argtypes = [vp, _ip, _ip, _ip, _ip]
         L.sunhip_u_solve.argtypes = [vp, C.POINTER(UPredParams), C.POINTER(CGParams), C.POINTER(CGStats)]
         L.sunhip_compute_qsource.argtypes = [vp, C.c_double, C.c_double, C.c_char_p]
         L.sunhip_corrector.argtypes = [vp, C.c_double, C.c_char_p, C.c_char_p]
@@ -153,6 +161,7 @@ class DeviceGrid:
             setattr(T, n, native.dptr(a))
         self.h = C.c_void_p()
         check(L.sunhip_grid_create(C.byref(self.h), C.byref(T), device), "sunhip_grid_create")
+        self.Nc, self.Ne = st.Nc, st.Ne
         self.st = st
         self.set_extents(st)
 
@@ -244,6 +253,31 @@ class DeviceGrid:
     def update_scalars(self, theta, dt, kappa, TVD, step=2, scal="s"):
         S = ScalarParams(theta, dt, kappa, TVD, step, 0, 0)
         check(lib().sunhip_update_scalars(self.h, C.byref(S), scal.encode()), "sunhip_update_scalars")
+
+    def _vgrid(self, st):
+        self._dz = np.ascontiguousarray(st.get("dz"), np.float64)
+        return VGridParams(self._dz.ctypes.data_as(_dp), st.grid_param("dzsmall"), st.grid_param("smoothbot"),
+                           int(st.grid_param("stairstep")), int(st.grid_param("fixdzz")),
+                           int(st.get_prop("linearFS")))
+
+    def update_dz(self, st, option: int = 0):
+        """UpdateDZ on the device from the resident h (vertical-grid
+        parameters and dz from the State)."""
+        V = self._vgrid(st)
+        check(lib().sunhip_update_dz(self.h, C.byref(V), int(option)), "sunhip_update_dz")
+
+    def set_flux_height(self, st):
+        V = self._vgrid(st)
+        check(lib().sunhip_set_flux_height(self.h, C.byref(V)), "sunhip_set_flux_height")
+
+    def extents(self):
+        """The device's ctop, ctopold, etop, etopold in reference order."""
+        nc, ne = self.Nc, self.Ne
+        out = {k: np.zeros(n, np.int32) for k, n in (("ctop", nc), ("ctopold", nc), ("etop", ne), ("etopold", ne))}
+        check(lib().sunhip_get_extents(self.h, *[out[k].ctypes.data_as(_ip) for k in ("ctop", "ctopold", "etop",
+                                                                                        "etopold")]),
+              "sunhip_get_extents")
+        return out
 
     def u_predict(self, P: "UPredParams"):
         """UPredictor before the open-boundary hooks: U*** in utmp, E in ue, D."""

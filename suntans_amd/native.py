@@ -55,6 +55,9 @@ def state_lib() -> C.CDLL:
         L = _load("libsunstate.so")
         L.sun_grid_tri.restype = _vp
         L.sun_grid_tri.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+        L.sun_grid_set_param.argtypes = [_vp, C.c_char_p, C.c_double]
+        L.sun_grid_param.restype = C.c_double
+        L.sun_grid_param.argtypes = [_vp, C.c_char_p]
         L.sun_grid_polygons.restype = _vp
         L.sun_grid_polygons.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, _ip, _ip, _dp, _dp]
         L.sun_grid_vertical.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int]
@@ -105,6 +108,8 @@ def dropin_lib() -> C.CDLL:
         L.ComputeQSource.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int]
         L.Corrector.argtypes = [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.Continuity.argtypes = [_vp] * 4
+        L.UpdateDZ.argtypes = [_vp, _vp, _vp, C.c_int]
+        L.SetFluxHeight.argtypes = [_vp, _vp, _vp]
         L.sunhip_UPredictor_begin.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.sunhip_UPredictor_solve.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.UpdateScalars.argtypes = ([_vp] * 7 + [C.c_double, C.c_double, _vp, C.c_double]

@@ -178,6 +178,15 @@ class State:
         k = np.arange(int(nkc.sum())) - np.repeat(start, nkc)
         return edge, k
 
+    def set_grid_param(self, **kw) -> None:
+        """gridT's smoothbot, dzsmall, fixdzz, stairstep."""
+        for k, v in kw.items():
+            if self._L.sun_grid_set_param(self.g, k.encode(), float(v)) != 0:
+                raise KeyError(k)
+
+    def grid_param(self, name: str) -> float:
+        return self._L.sun_grid_param(self.g, name.encode())
+
     def active(self) -> np.ndarray:
         """phys->active (1 wet, 0 dried) as uint8."""
         ptr = self._L.sun_phys_ptr(self.p, b"active")

@@ -81,6 +81,13 @@ class _Cpu:
     def face_scalars(self, st, tvd):
         self.fn("HorizontalFaceScalars")(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd))
 
+    def update_dz(self, st, option):
+        getattr(self.L, self.pre + "UpdateDZ")(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop),
+                                               C.c_int(option))
+
+    def set_flux_height(self, st):
+        getattr(self.L, self.pre + "SetFluxHeight")(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
     def upredictor_begin(self, st):
         return self.L.oracle_UPredictor_begin(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
 
@@ -194,6 +201,12 @@ class DropinImpl:
 
     def face_scalars(self, st, tvd):
         self.L.HorizontalFaceScalars(st.g, st.p, st.prop, st.ptr("s"), st.ptr("boundary_s"), int(tvd), 0, 0)
+
+    def update_dz(self, st, option):
+        self.L.UpdateDZ(st.g, st.p, st.prop, int(option))
+
+    def set_flux_height(self, st):
+        self.L.SetFluxHeight(st.g, st.p, st.prop)
 
     def upredictor_begin(self, st):
         self.L.sunhip_UPredictor_begin(st.g, st.p, st.prop, 0, 1, 0)

@@ -264,3 +264,27 @@ def test_upredictor_inviscid_closed_form():
     h1 = st.get("h")
     grad_h1 = (h1[grad[edge, 0]] - h1[grad[edge, 1]]) / dg[edge]
     assert np.allclose(st.get("u")[comp], (ustar - g * th * dt * grad_h1)[comp], rtol=1e-14, atol=1e-16)
+
+
+def test_update_dz_column_heights():
+    """After UpdateDZ each wet column's layer heights add up to h + dv, the
+    cells above ctop are empty and the ctop cell is not, and etop is the
+    shallower ctop of an edge's two cells."""
+    o = _impls.oracle()
+    st = cases.tiny(7, nx=12, ny=8, Nkmax=10)
+    rng = np.random.default_rng(4)
+    st.set("h", st.get("h") + rng.uniform(-2.5, 1.5, st.Nc))
+    o.update_dz(st, 0)
+    cell, k = st.cell_layers()
+    dzz, h, dv = st.get("dzz"), st.get("h"), st.get("dv")
+    ctop, nk = st.ints("ctop"), st.ints("Nk")
+    total = np.bincount(cell, weights=dzz, minlength=st.Nc)
+    wet = h > -dv
+    assert np.allclose(total[wet], (h + dv)[wet], rtol=0, atol=1e-12 * dv.max())
+    assert np.all(dzz[k < ctop[cell]] == 0)
+    top = (k == ctop[cell]) & wet[cell] & (ctop[cell] < nk[cell])
+    assert np.all(dzz[top] >= 0)
+    grad = st.ints("grad").reshape(-1, 2)
+    etop = st.ints("etop")
+    both = (grad[:, 0] >= 0) & (grad[:, 1] >= 0)
+    assert np.array_equal(etop[both], np.minimum(ctop[grad[both, 0]], ctop[grad[both, 1]]))

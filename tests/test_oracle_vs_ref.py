@@ -187,3 +187,48 @@ def test_upredictor(name, make, variant):
     for f in ("utmp", "ut", "u", "D", "h", "htmp", "wtmp2", "utmp2", "hcorr", "s"):
         assert np.array_equal(a.get(f), b.get(f)), f
     assert np.array_equal(a.active(), b.active())
+
+
+VGRID = UPRED + [("shallow36", lambda: cases.tiny(36, nx=12, ny=8, Nkmax=6, min_layers=-1.2))]
+
+
+def _grid_state(st):
+    return {f: st.get(f) for f in ("dzz", "dzzold", "h", "dv", "dzfB", "dzbot")} | \
+        {f: st.ints(f) for f in ("ctop", "ctopold", "etop", "etopold")}
+
+
+@pytest.mark.parametrize("name,make", VGRID, ids=[v[0] for v in VGRID])
+@pytest.mark.parametrize("option", [0, 1, -1])
+@pytest.mark.parametrize("fixdzz", [0, 1])
+def test_update_dz(name, make, option, fixdzz):
+    """UpdateDZ after a seeded change of the free surface (cells wetting
+    and drying at the top), all three options."""
+    o = _impls.oracle()
+    a = make()
+    rng = np.random.default_rng(9)
+    a.set("h", a.get("h") + rng.uniform(-1.5, 1.0, a.Nc))
+    a.set_grid_param(fixdzz=fixdzz, dzsmall=0.3, stairstep=0)
+    b = a.clone()
+    o.update_dz(a, option)
+    ref.update_dz(b, option)
+    sa, sb = _grid_state(a), _grid_state(b)
+    for f in sa:
+        assert np.array_equal(sa[f], sb[f]), f
+
+
+@pytest.mark.parametrize("name,make", VGRID, ids=[v[0] for v in VGRID])
+@pytest.mark.parametrize("smoothbot", [0.0, 0.5])
+def test_set_flux_height(name, make, smoothbot):
+    o = _impls.oracle()
+    a = make()
+    rng = np.random.default_rng(10)
+    u = a.get("u")
+    u[rng.random(len(u)) < 0.1] = 0.0  # u = 0 takes the larger height
+    a.set("u", u)
+    a.set_grid_param(smoothbot=smoothbot, dzsmall=0.3)
+    a.set("dzbot", rng.uniform(0.1, 1.0, a.Nc))
+    b = a.clone()
+    o.set_flux_height(a)
+    ref.set_flux_height(b)
+    assert np.array_equal(a.get("dzf"), b.get("dzf"))
+    assert np.array_equal(a.get("dzz"), b.get("dzz"))
