@@ -40,6 +40,14 @@ gridT *sun_grid_tri(int nx, int ny, double Lx, double Ly);
 gridT *sun_grid_polygons(int Np, const double *xp, const double *yp, int Nc, int maxfaces, const int *nfaces,
                          const int *cells, const double *xv, const double *yv);
 
+/* Open boundaries: mark[Ne] gives every edge its mark; only boundary edges
+ * (grad[2e+1] == -1) may change, to 1..4 (2 velocity, 3 specified-value
+ * cells behind it, 4 no-slip).  Cells with a mark-3 face become type-3
+ * cells; cellp/celldist and edgep/edgedist are rebuilt.  Call before
+ * sun_phys_alloc (the open-boundary rows follow edgedist).  Returns -1 and
+ * changes nothing for a mark the rule does not allow. */
+int sun_grid_open(gridT *g, const int *mark);
+
 /* Vertical grid: dz[Nkmax] reference layer thicknesses and dv[Nc] depths.
  * Sets Nk (main/grid.c:1466-1498), Nke/Nkc (:1530-1563), ctop/etop = 0, and
  * allocates dzz/dzzold (Nk) and dzf (Nkc) rows as main/phys.c:765-799. */

@@ -1022,3 +1022,221 @@ void oracle_update_salinity(gridT *g, physT *p, propT *prop) {
   oracle_UpdateScalars(g, p, prop, p->wnew, p->s, p->boundary_s, p->Cn_R, prop->kappa_s, prop->kappa_sH,
                        p->kappa_tv, prop->theta, NULL, NULL, NULL, NULL, 0, 0, 1, prop->TVDsalt);
 }
+
+/* ------------------------------------------------------------------------ */
+/* WPredictor (main/phys.c:2025-2290) and the face values it advects with   */
+
+/* Distance from the centre of the edge's first cell to the edge midpoint,
+ * as main/phys.c:4272-4273 computes it. */
+double oracle_face_def1(gridT *g, int j) {
+  int nc1 = g->grad[2 * j];
+  if (nc1 == -1) nc1 = g->grad[2 * j + 1];
+  return sqrt(pow(g->xv[nc1] - g->xe[j], 2) + pow(g->yv[nc1] - g->ye[j], 2));
+}
+
+/* main/phys.c:4265-4289: distance-weighted interpolation of phi to face j
+ * (method 2), less the Lax-Wendroff correction (method 4); upwind where a
+ * cell centre lies on the face. */
+static double face_interp(gridT *g, int j, int k, REAL **phi, REAL **U, double dt, int method) {
+  int nc1 = g->grad[2 * j], nc2 = g->grad[2 * j + 1];
+  if (nc1 == -1) nc1 = nc2;
+  if (nc2 == -1) nc2 = nc1;
+  const double dg = g->dg[j], d1 = oracle_face_def1(g, j), d2 = dg - d1;
+  double C = 0;
+  if (method == 4) C = U[j][k] * dt / dg;
+  if (d1 == 0 || d2 == 0) return upwind_ref(U[j][k], phi[nc1][k], phi[nc2][k]);
+  return (phi[nc1][k] * d2 + phi[nc2][k] * d1) / (d1 + d2) - C / 2 * (phi[nc1][k] - phi[nc2][k]);
+}
+
+/* main/phys.c:4879-4992: face values uf of the cell field ui advected by the
+ * edge velocity U.  Type-2 edges take the boundary row where U > 0 (inflow)
+ * and the inner cell otherwise; type-4 edges the boundary row; each
+ * computational edge is zero above the deeper of its cells' tops and below
+ * that takes the scheme's value (1 upwind, 2/4 face_interp, 5 the TVD face
+ * values of HorizontalFaceScalars); faces between a type-3 cell and a
+ * neighbour are upwind whatever the scheme.  Other edges keep their values. */
+void oracle_MomentumFaceValues(REAL **uf, REAL **ui, REAL **bui, REAL **U, gridT *g, physT *p, propT *prop) {
+  const int mf = g->maxfaces, b0 = g->edgedist[2], scheme = prop->nonlinear;
+  for (int jp = g->edgedist[2]; jp < g->edgedist[3]; jp++) {
+    const int j = g->edgep[jp], i = g->grad[2 * j];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) uf[j][k] = U[j][k] > 0 ? bui[jp - b0][k] : ui[i][k];
+  }
+  for (int jp = g->edgedist[4]; jp < g->edgedist[5]; jp++) {
+    const int j = g->edgep[jp];
+    for (int k = g->etop[j]; k < g->Nke[j]; k++) uf[j][k] = bui[jp - b0][k];
+  }
+  if (scheme == 5) oracle_HorizontalFaceScalars(g, p, prop, ui, bui, prop->TVDmomentum);
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp], nc1 = g->grad[2 * j], nc2 = g->grad[2 * j + 1];
+    const int kmin = g->ctop[nc1] > g->ctop[nc2] ? g->ctop[nc1] : g->ctop[nc2];
+    for (int k = 0; k < kmin; k++) uf[j][k] = 0;
+    for (int k = kmin; k < g->Nke[j]; k++) {
+      const int up = U[j][k] > 0 ? nc2 : nc1;
+      if (scheme == 2 || scheme == 4) uf[j][k] = face_interp(g, j, k, ui, U, prop->dt, scheme);
+      else if (scheme == 5) uf[j][k] = U[j][k] > 0 ? p->SfHp[j][k] : p->SfHm[j][k];
+      else uf[j][k] = ui[up][k];
+    }
+  }
+  for (int ip = g->celldist[1]; ip < g->celldist[2]; ip++) {
+    const int i = g->cellp[ip];
+    for (int f = 0; f < g->nfaces[i]; f++) {
+      if (g->neigh[i * mf + f] == -1) continue;
+      const int j = g->face[i * mf + f], nc1 = g->grad[2 * j], nc2 = g->grad[2 * j + 1];
+      const int kmin = g->ctop[nc1] > g->ctop[nc2] ? g->ctop[nc1] : g->ctop[nc2];
+      for (int k = 0; k < kmin; k++) uf[j][k] = 0;
+      for (int k = kmin; k < g->Nke[j]; k++) uf[j][k] = ui[U[j][k] > 0 ? nc2 : nc1][k];
+    }
+  }
+}
+
+/* main/phys.c:2025-2290.  Adams-Bashforth weights from the step number and
+ * AB (Cn_W/Cn_W2 cleared on the first step); per computational column
+ * wtmp = w + f2 Cn_W + f3 Cn_W2 less the explicit nonhydrostatic pressure
+ * gradient, Cn_W2 <- Cn_W.  The cell source stmp: horizontal advection of
+ * the cell-centred w (wc) through the face values above (ut), vertical
+ * advection of w^2 for schemes 1, 2 and 5, horizontal diffusion
+ * (nu_H + nu_lax) and sidewall drag (CdW) scattered from computational
+ * edges in edgep order, the no-slip term of type-4 edges.  Cn_W takes
+ * -dt stmp averaged to the faces (and for scheme 4 the vertical
+ * Lax-Wendroff flux), wtmp += f1 Cn_W, and the implicit vertical diffusion
+ * of each column with more than one layer is solved into w (TriSolve),
+ * a one-layer column copies wtmp.
+ * Reads outside rows as the reference makes them: with conserveMomentum the
+ * top-cell fill reads a[k] above ctop left over from the previous column
+ * (the same scratch here, so the result matches); scheme 4 reads dzz one
+ * past the column bottom (read as 0 here); the type-4 term runs to
+ * Nke[cell] of the inner cell's index and reads its boundary row past
+ * Nke+1 (read as 0 here). */
+void oracle_WPredictor(gridT *g, physT *p, propT *prop) {
+  REAL *a = p->a, *b = p->b, *c = p->c;
+  const int mf = g->maxfaces, scheme = prop->nonlinear;
+  const double dt = prop->dt;
+  double f1, f2, f3;
+  if (prop->n == 1) {
+    f1 = 1; f2 = f3 = 0;
+    for (int i = 0; i < g->Nc; i++)
+      for (int k = 0; k < g->Nk[i]; k++) p->Cn_W[i][k] = p->Cn_W2[i][k] = 0;
+  } else if (prop->n == 2 || prop->AB == 2) {
+    f1 = 3.0 / 2.0; f2 = -1.0 / 2.0; f3 = 0;
+  } else {
+    f1 = 23.0 / 12.0; f2 = -4.0 / 3.0; f3 = 5.0 / 12.0;
+  }
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], ct = g->ctop[i];
+    for (int k = ct; k < g->Nk[i]; k++) {
+      p->wtmp[i][k] = p->w[i][k] + f2 * p->Cn_W[i][k] + f3 * p->Cn_W2[i][k];
+      p->Cn_W2[i][k] = p->Cn_W[i][k];
+      p->Cn_W[i][k] = 0;
+    }
+    for (int k = ct + 1; k < g->Nk[i]; k++)
+      p->wtmp[i][k] -= 2.0 * dt / (g->dzz[i][k - 1] + g->dzz[i][k]) * (p->q[i][k - 1] - p->q[i][k]);
+    p->wtmp[i][ct] += 2.0 * dt / g->dzz[i][ct] * p->q[i][ct];
+  }
+  for (int i = 0; i < g->Nc; i++)
+    for (int k = 0; k < g->Nk[i]; k++) p->stmp[i][k] = 0;
+
+  if (scheme) {
+    for (int i = 0; i < g->Nc; i++)
+      for (int k = g->ctop[i]; k < g->Nk[i]; k++) p->wc[i][k] = 0.5 * (p->w[i][k] + p->w[i][k + 1]);
+    oracle_MomentumFaceValues(p->ut, p->wc, p->boundary_w, p->utmp2, g, p, prop);
+    if (prop->conserveMomentum)
+      for (int jp = g->edgedist[0]; jp < g->edgedist[5]; jp++) {
+        const int j = g->edgep[jp];
+        for (int k = g->etop[j]; k < g->Nke[j]; k++) p->ut[j][k] *= g->dzf[j][k];
+      }
+    for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+      const int i = g->cellp[ip], ct = g->ctop[i];
+      for (int k = ct; k < g->Nk[i]; k++) a[k] = prop->conserveMomentum ? g->dzz[i][k] : 1.0;
+      for (int f = 0; f < g->nfaces[i]; f++) {
+        const int ne = g->face[i * mf + f];
+        for (int k = ct; k < g->Nk[i]; k++)
+          p->stmp[i][k] += p->ut[ne][k] * p->utmp2[ne][k] * g->df[ne] * g->normal[i * mf + f] / (a[k] * g->Ac[i]);
+        if (prop->conserveMomentum)
+          for (int k = g->etop[ne]; k < ct; k++)
+            p->stmp[i][ct] += p->ut[ne][k] * p->utmp2[ne][k] * g->df[ne] * g->normal[i * mf + f] / (a[k] * g->Ac[i]);
+      }
+      if (scheme == 1 || scheme == 2 || scheme == 5)
+        for (int k = ct; k < g->Nk[i]; k++)
+          p->stmp[i][k] += (pow(p->w[i][k], 2) - pow(p->w[i][k + 1], 2)) / g->dzz[i][k];
+    }
+  }
+
+  for (int jp = g->edgedist[0]; jp < g->edgedist[1]; jp++) {
+    const int j = g->edgep[jp], nc1 = g->grad[2 * j], nc2 = g->grad[2 * j + 1];
+    const int kmin = g->ctop[nc1] > g->ctop[nc2] ? g->ctop[nc1] : g->ctop[nc2];
+    for (int k = kmin; k < g->Nke[j]; k++) {
+      a[k] = .5 * (prop->nu_H + 0.5 * (p->nu_lax[nc1][k] + p->nu_lax[nc2][k])) *
+             (p->w[nc2][k] - p->w[nc1][k] + p->w[nc2][k + 1] - p->w[nc1][k + 1]) * g->df[j] / g->dg[j];
+      p->stmp[nc1][k] -= a[k] / g->Ac[nc1];
+      p->stmp[nc2][k] += a[k] / g->Ac[nc2];
+    }
+    const int side[2] = {nc1, nc2};
+    for (int s = 0; s < 2; s++) {
+      const int i = side[s];
+      for (int k = g->Nke[j]; k < g->Nk[i]; k++)
+        p->stmp[i][k] += 0.25 * prop->CdW * fabs(p->w[i][k] + p->w[i][k + 1]) * (p->w[i][k] + p->w[i][k + 1]) *
+                         g->df[j] / g->Ac[i];
+    }
+  }
+  for (int jp = g->edgedist[4]; jp < g->edgedist[5]; jp++) {
+    const int j = g->edgep[jp];
+    const int i = g->grad[2 * j] == -1 ? g->grad[2 * j + 1] : g->grad[2 * j];
+    const REAL *bw = p->boundary_w[jp - g->edgedist[2]];
+    for (int k = g->ctop[i]; k < g->Nke[i]; k++) {
+      const double bk = k <= g->Nke[j] ? bw[k] : 0.0;
+      p->stmp[i][k] += -2.0 * prop->nu_H * (bk - 0.5 * (p->w[i][k] + p->w[i][k + 1])) / g->dg[j] * g->df[j] /
+                       g->Ac[i];
+    }
+  }
+
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], ct = g->ctop[i];
+    for (int k = ct + 1; k < g->Nk[i]; k++)
+      p->Cn_W[i][k] -= dt * (g->dzz[i][k - 1] * p->stmp[i][k - 1] + g->dzz[i][k] * p->stmp[i][k]) /
+                       (g->dzz[i][k - 1] + g->dzz[i][k]);
+    p->Cn_W[i][ct] -= dt * p->stmp[i][ct];
+  }
+  if (scheme == 4)
+    for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+      const int i = g->cellp[ip], n = g->Nk[i];
+      for (int k = g->ctop[i] + 1; k < n + 1; k++) {
+        const double wm = 0.5 * (p->w[i][k - 1] + p->w[i][k]);
+        const double Cz = wm * dt / g->dzz[i][k - 1];
+        a[k] = wm * (wm - 0.5 * Cz * (p->w[i][k - 1] - p->w[i][k]));
+      }
+      for (int k = g->ctop[i] + 1; k < n; k++) {
+        const double below = k + 1 < n ? g->dzz[i][k + 1] : 0.0;
+        p->Cn_W[i][k] -= 2.0 * dt * (a[k] - a[k + 1]) / (g->dzz[i][k] + below);
+      }
+    }
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip];
+    for (int k = g->ctop[i]; k < g->Nk[i]; k++) p->wtmp[i][k] += f1 * p->Cn_W[i][k];
+  }
+
+  const double nu = prop->nu, lw = prop->laxWendroff_Vertical, th = prop->theta;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
+    const int i = g->cellp[ip], ct = g->ctop[i], n = g->Nk[i];
+    REAL *z = g->dzz[i], *w = p->w[i], *rhs = p->wtmp[i];
+    if (n - ct <= 1) {
+      for (int k = ct; k < n; k++) w[k] = rhs[k];
+      continue;
+    }
+    for (int k = ct + 1; k < n; k++) {
+      a[k] = 2 * (nu + lw * p->nu_lax[i][k - 1] + p->nu_tv[i][k - 1]) / z[k - 1] / (z[k] + z[k - 1]);
+      b[k] = 2 * (nu + lw * p->nu_lax[i][k] + p->nu_tv[i][k]) / z[k] / (z[k] + z[k - 1]);
+    }
+    b[ct] = (nu + lw * p->nu_lax[i][ct] + p->nu_tv[i][ct]) / pow(z[ct], 2);
+    a[ct] = b[ct];
+    for (int k = ct + 1; k < n; k++)
+      rhs[k] += dt * (1 - th) * (a[k] * w[k - 1] - (a[k] + b[k]) * w[k] + b[k] * w[k + 1]);
+    rhs[ct] += dt * (1 - th) * (-(a[ct] + b[ct]) * w[ct] + (a[ct] + b[ct]) * w[ct + 1]);
+    for (int k = ct; k < n; k++) {
+      c[k] = 1 + dt * th * (a[k] + b[k]);
+      a[k] *= (-dt * th);
+      b[k] *= (-dt * th);
+    }
+    b[ct] += a[ct];
+    oracle_TriSolve(a + ct, c + ct, b + ct, rhs + ct, w + ct, n - ct);
+  }
+}

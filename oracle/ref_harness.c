@@ -50,6 +50,10 @@ EXPORT void ref_Continuity(REAL **w, gridT *g, physT *p, propT *pr) { Continuity
 EXPORT void ref_UpdateDZ(gridT *g, physT *p, propT *pr, int option) { UpdateDZ(g, p, pr, option); }
 EXPORT void ref_SetFluxHeight(gridT *g, physT *p, propT *pr) { SetFluxHeight(g, p, pr); }
 EXPORT void ref_UPredictor(gridT *g, physT *p, propT *pr) { UPredictor(g, p, pr, 0, 1, 0); }
+EXPORT void ref_WPredictor(gridT *g, physT *p, propT *pr) { WPredictor(g, p, pr, 0, 1, 0); }
+EXPORT void ref_GetMomentumFaceValues(gridT *g, physT *p, propT *pr) {
+  GetMomentumFaceValues(p->ut, p->wc, p->boundary_w, p->utmp2, g, p, pr, 0, 0, pr->nonlinear);
+}
 EXPORT void ref_HorizontalFaceScalars(gridT *g, physT *p, propT *pr, REAL **scal, REAL **bscal, int TVD) {
   HorizontalFaceScalars(g, p, pr, scal, bscal, TVD, 0, 0);
 }

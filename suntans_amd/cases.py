@@ -139,10 +139,26 @@ def depth_integrated_D(st: State) -> np.ndarray:
     return np.bincount(edge, weights=dzf * st.edge_wet_mask(), minlength=st.Ne)
 
 
+def open_ends(west: int = 2, east: int = 3, frac: float = 0.1):
+    """open_marks for State: closed boundary edges whose midpoints lie in the
+    westmost (eastmost) frac of the boundary's x-range take mark west (east)."""
+    def marks(xe, ye, mark):
+        out = mark.copy()
+        b = mark == 1
+        lo, hi = xe[b].min(), xe[b].max()
+        span = hi - lo
+        out[b & (xe <= lo + frac * span)] = west
+        out[b & (xe >= hi - frac * span)] = east
+        return out
+    return marks
+
+
 def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3, planform="tri",
-         split_every=0, maxfaces=None) -> State:
+         split_every=0, maxfaces=None, open_marks=None, grid_params=None) -> State:
     """Small grid with ragged columns, dry top layers and a moved surface.
-    planform/split_every/maxfaces choose the cell shapes (State)."""
+    planform/split_every/maxfaces choose the cell shapes, open_marks opens
+    boundary edges (State); grid_params (smoothbot, dzsmall, fixdzz) are set
+    before the first UpdateDZ."""
     rng = np.random.default_rng(seed)
     dz = np.full(Nkmax, 1.0)
     H = Nkmax * 1.0
@@ -153,7 +169,9 @@ def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3, p
         return d
 
     st = State(nx, ny, 400.0 * nx, 300.0 * ny, Nkmax, dz, depth, planform=planform, split_every=split_every,
-               maxfaces=maxfaces)
+               maxfaces=maxfaces, open_marks=open_marks)
+    if grid_params:
+        st.set_grid_param(**grid_params)
     st.update_dz(-1)
     h1 = rng.uniform(-1.6, 0.3, st.Nc) if move_surface else np.zeros(st.Nc)
     st.set("h", h1)
@@ -167,9 +185,9 @@ def tiny(seed=0, nx=4, ny=3, Nkmax=8, move_surface=True, dt=5.0, min_layers=3, p
     return st
 
 
-def lock_exchange(seed=0, nx=100, ny=10, Nkmax=50, depth=20.0, dt=5.0) -> State:
+def lock_exchange(seed=0, nx=100, ny=10, Nkmax=50, depth=20.0, dt=5.0, open_marks=None) -> State:
     rng = np.random.default_rng(seed)
-    st = State(nx, ny, 10e3, 1e3, Nkmax, np.full(Nkmax, depth / Nkmax), depth)
+    st = State(nx, ny, 10e3, 1e3, Nkmax, np.full(Nkmax, depth / Nkmax), depth, open_marks=open_marks)
     st.update_dz(-1)
     st.update_dz(1)
     fill_velocities(st, rng, amp_u=0.05, amp_w=5e-4)
@@ -245,4 +263,31 @@ def upredictor_inputs(st: State, rng, noslip: bool = True, nonlinear: int = 1, t
     st.set("nu_lax", rng.uniform(0, 1e-4, n))
     st.set_prop(nu=1e-4, laxWendroff_Vertical=lax, nonlinear=nonlinear, thetaM=thetaM, cgsolver=1,
                 epsilon=1e-10, maxiters=1000, hprecond=1, grav=9.81)
+    return st
+
+
+def wpredictor_inputs(st: State, rng, nonlinear: int = 2, step: int = 3, AB: int = 3, CdW: float = 5e-3,
+                      nu_H: float = 0.1, conserve: int = 0, tvd: int = 4, stress: bool = False) -> State:
+    """Inputs of WPredictor: the Adams-Bashforth histories Cn_W/Cn_W2, eddy
+    and Lax-Wendroff viscosities, the old edge velocity utmp2 (= u, as
+    UPredictor leaves it), stale face values ut and cell-centred wc (what
+    the reference reads where it does not write them this step), open
+    boundary rows of w, and the properties it reads.  stress: utmp2 random on
+    every edge-layer, dry and closed ones included, so that every product of
+    a face the reference reads counts."""
+    nw = len(st.get("Cn_W"))
+    st.set("Cn_W", rng.normal(0, 1e-5, nw))
+    st.set("Cn_W2", rng.normal(0, 1e-5, nw))
+    n = len(st.get("nu_tv"))
+    st.set("nu_tv", rng.uniform(0, 1e-3, n))
+    st.set("nu_lax", rng.uniform(0, 1e-4, n))
+    st.set("wc", rng.normal(0, 1e-3, n))
+    ne = len(st.get("ut"))
+    st.set("ut", rng.normal(0, 1e-2, ne))
+    st.set("utmp2", rng.normal(0, 0.05, ne) if stress else st.get("u"))
+    nb = len(st.get("boundary_w"))
+    if nb:
+        st.set("boundary_w", rng.normal(0, 1e-3, nb))
+    st.set_prop(nu=1e-4, nu_H=nu_H, CdW=CdW, laxWendroff_Vertical=1.0, nonlinear=nonlinear, n=step, AB=AB,
+                conserveMomentum=conserve, TVDmomentum=tvd)
     return st

@@ -24,6 +24,7 @@ static REAL **rows_alloc(int nrows, const int *len, int extra) {
   REAL *blk = (REAL *)xcalloc(total, sizeof(REAL));
   size_t off = 0;
   for (int i = 0; i < nrows; i++) { r[i] = blk + off; off += (size_t)(len[i] + extra); }
+  if (!nrows) r[0] = blk;  /* the one slot xcalloc gives keeps the block for rows_free */
   return r;
 }
 static void rows_free(REAL **r) { if (r) { free(r[0]); free(r); } }
@@ -66,6 +67,42 @@ static void planform_alloc(gridT *g) {
 /* Edge metrics, face normals and the single-process computational ordering
  * of a planform whose points, cells (nfaces, face, neigh), Voronoi points,
  * areas and edges (end points, grad, mark) are set. */
+/* Computational ordering of a single process (main/grid.c:1652-1679):
+ * computational cells first, then the type-3 cells (celldist[1..2)), each
+ * group in index order; edges sorted by mark 0 (and 5), 1, 2, 3, 4 into
+ * edgedist[0..5].  The builders take a cell with a mark-3 face as type 3
+ * (the reference reads the cell's type from its cells file). */
+static void computational_order(gridT *g) {
+  const int Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
+  int n = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) g->celldist[1] = n;
+    for (int i = 0; i < Nc; i++) {
+      int t3 = 0;
+      for (int nf = 0; nf < g->nfaces[i]; nf++) t3 |= g->mark[g->face[i * mf + nf]] == 3;
+      if (t3 == pass) g->cellp[n++] = i;
+    }
+  }
+  g->celldist[0] = 0; g->celldist[2] = Nc;
+  int k = 0;
+  for (int e = 0; e < Ne; e++) if (g->mark[e] == 0 || g->mark[e] == 5) g->edgep[k++] = e;
+  g->edgedist[0] = 0; g->edgedist[1] = k;
+  for (int m = 1; m < 5; m++) {
+    for (int e = 0; e < Ne; e++) if (g->mark[e] == m) g->edgep[k++] = e;
+    g->edgedist[m + 1] = k;
+  }
+}
+
+int sun_grid_open(gridT *g, const int *mark) {
+  for (int e = 0; e < g->Ne; e++) {
+    if (mark[e] == g->mark[e]) continue;
+    if (g->grad[2 * e + 1] != -1 || mark[e] < 1 || mark[e] > 4) return -1;
+  }
+  for (int e = 0; e < g->Ne; e++) g->mark[e] = g->edges[3 * e + 2] = mark[e];
+  computational_order(g);
+  return 0;
+}
+
 static void planform_finish(gridT *g) {
   const int Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
   /* metrics, main/grid.c:3457-3533 conventions */
@@ -102,21 +139,11 @@ static void planform_finish(gridT *g) {
                            g->normal[mf * i + nf];
     }
 
-  /* computational ordering (single process): every cell is computational;
-   * edges sorted by mark 0,1,2,3,4 */
   g->cellp = (int *)xcalloc(Nc, sizeof(int));
   g->celldist = (int *)xcalloc(3, sizeof(int));
-  for (int i = 0; i < Nc; i++) g->cellp[i] = i;
-  g->celldist[0] = 0; g->celldist[1] = Nc; g->celldist[2] = Nc;
   g->edgep = (int *)xcalloc(Ne, sizeof(int));
   g->edgedist = (int *)xcalloc(6, sizeof(int));
-  int k = 0;
-  for (int e = 0; e < Ne; e++) if (g->mark[e] == 0 || g->mark[e] == 5) g->edgep[k++] = e;
-  g->edgedist[1] = k;
-  for (int m = 1; m < 5; m++) {
-    for (int e = 0; e < Ne; e++) if (g->mark[e] == m) g->edgep[k++] = e;
-    g->edgedist[m + 1] = k;
-  }
+  computational_order(g);
   g->mnptr = (int *)xcalloc(Nc, sizeof(int));
   g->eptr = (int *)xcalloc(Ne, sizeof(int));
   for (int i = 0; i < Nc; i++) g->mnptr[i] = i;
@@ -371,7 +398,9 @@ int sun_grid_vertical(gridT *g, int Nkmax, const double *dz, const double *dv, i
 
 struct fdesc { const char *name; int kind; size_t off; };
 /* kinds: 1 cell rows (Nk), 2 w rows (Nk+1), 3 edge rows (Nkc), 4 cell 2-D,
- * 5 edge 2-D, 6 column scratch (Nkmax+1) */
+ * 5 edge 2-D, 6 column scratch (Nkmax+1), 8 open-boundary rows (Nke+1, one
+ * per edge of edgedist[2..5)), 9 open-boundary rows (Nke) */
+#define ROWS_KIND(k) ((k) <= 3 || (k) >= 8)
 #define PF(f, k) {#f, k, offsetof(physT, f)}
 static const struct fdesc phys_fields[] = {
   PF(uc, 1), PF(vc, 1), PF(wc, 1), PF(uold, 1), PF(vold, 1), PF(q, 1), PF(qc, 1),
@@ -386,6 +415,8 @@ static const struct fdesc phys_fields[] = {
   PF(D, 5), PF(tau_T, 5), PF(tau_B, 5), PF(CdT, 5), PF(CdB, 5),
   PF(ap, 6), PF(am, 6), PF(bp, 6), PF(bm, 6), PF(a, 6), PF(b, 6), PF(c, 6), PF(d, 6),
   PF(Cp, 6), PF(Cm, 6), PF(rp, 6), PF(rm, 6), PF(wp, 6), PF(wm, 6),
+  PF(boundary_w, 8), PF(boundary_tmp, 8), PF(boundary_s, 9), PF(boundary_T, 9), PF(boundary_u, 9),
+  PF(boundary_v, 9), PF(boundary_rho, 9),
 };
 #undef PF
 #define NPF (int)(sizeof(phys_fields) / sizeof(phys_fields[0]))
@@ -442,14 +473,11 @@ void sun_phys_free(const gridT *g, physT *p) {
   if (!p) return;
   for (int f = 0; f < NPF; f++) {
     void *slot = (char *)p + phys_fields[f].off;
-    if (phys_fields[f].kind <= 3) rows_free(*(REAL ***)slot); else free(*(REAL **)slot);
+    if (ROWS_KIND(phys_fields[f].kind)) rows_free(*(REAL ***)slot); else free(*(REAL **)slot);
   }
   free(p->hfcoef);
   rows_free(p->qtmp);
   free(p->active);
-  rows_free(p->boundary_s); rows_free(p->boundary_T); rows_free(p->boundary_u);
-  rows_free(p->boundary_v); rows_free(p->boundary_rho); rows_free(p->boundary_w);
-  rows_free(p->boundary_tmp);
   free(p->boundary_h); free(p->boundary_flag);
   free(p);
 }
@@ -462,6 +490,11 @@ static size_t kind_len(const gridT *g, int kind) {
     case 4: return (size_t)g->Nc;
     case 5: return (size_t)g->Ne;
     case 6: return (size_t)g->Nkmax + 1;
+    case 8: case 9: {
+      size_t n = 0;
+      for (int j = g->edgedist[2]; j < g->edgedist[5]; j++) n += (size_t)(g->Nke[g->edgep[j]] + (kind == 8));
+      return n;
+    }
   }
   return 0;
 }
@@ -469,7 +502,8 @@ static size_t kind_len(const gridT *g, int kind) {
 void sun_phys_copy(const gridT *g, const physT *from, physT *to) {
   for (int f = 0; f < NPF; f++) {
     size_t off = phys_fields[f].off, n = kind_len(g, phys_fields[f].kind);
-    if (phys_fields[f].kind <= 3)
+    if (!n) continue;
+    if (ROWS_KIND(phys_fields[f].kind))
       memcpy((*(REAL ***)((char *)to + off))[0], (*(REAL ***)((char *)from + off))[0], n * sizeof(REAL));
     else
       memcpy(*(REAL **)((char *)to + off), *(REAL **)((char *)from + off), n * sizeof(REAL));
@@ -477,12 +511,6 @@ void sun_phys_copy(const gridT *g, const physT *from, physT *to) {
   memcpy(to->htmp, from->htmp, 10 * (size_t)g->Nc * sizeof(REAL));
   memcpy(to->hfcoef, from->hfcoef, (size_t)g->maxfaces * g->Nc * sizeof(REAL));
   memcpy(to->active, from->active, (size_t)g->Nc);
-  {
-    int nb = g->edgedist[5] - g->edgedist[2];
-    size_t n = 0;
-    for (int j = 0; j < nb; j++) n += (size_t)g->Nke[g->edgep[g->edgedist[2] + j]];
-    if (n) memcpy(to->boundary_s[0], from->boundary_s[0], n * sizeof(REAL));
-  }
   to->mass = from->mass; to->volume = from->volume;
 }
 
@@ -692,7 +720,15 @@ static int grid_real(const gridT *g, const char *name, int *kind, REAL ***rows, 
 
 static long xfer(const gridT *g, int kind, REAL **rows, REAL *vec, double *flat, int to_flat) {
   size_t n = 0;
-  if (kind <= 3) {
+  if (kind >= 8) {
+    for (int j = g->edgedist[2]; j < g->edgedist[5]; j++) {
+      size_t m = (size_t)(g->Nke[g->edgep[j]] + (kind == 8));
+      REAL *row = rows[j - g->edgedist[2]];
+      if (to_flat) memcpy(flat + n, row, m * sizeof(REAL));
+      else memcpy(row, flat + n, m * sizeof(REAL));
+      n += m;
+    }
+  } else if (kind <= 3) {
     const int nr = kind == 3 ? g->Ne : g->Nc;
     const int *len = kind == 3 ? g->Nkc : g->Nk;
     const int extra = kind == 2;
@@ -722,7 +758,8 @@ long sun_field_get(const gridT *g, const physT *p, const char *name, double *out
   const struct fdesc *f = find_phys(name);
   if (f) {
     void *slot = (char *)p + f->off;
-    return xfer(g, f->kind, f->kind <= 3 ? *(REAL ***)slot : NULL, f->kind > 3 ? *(REAL **)slot : NULL, out, 1);
+    return xfer(g, f->kind, ROWS_KIND(f->kind) ? *(REAL ***)slot : NULL, ROWS_KIND(f->kind) ? NULL : *(REAL **)slot,
+                out, 1);
   }
   if (!strcmp(name, "hfcoef")) {
     memcpy(out, p->hfcoef, sizeof(REAL) * (size_t)g->maxfaces * g->Nc);
@@ -737,7 +774,7 @@ long sun_field_set(gridT *g, physT *p, const char *name, const double *in) {
   const struct fdesc *f = find_phys(name);
   if (f) {
     void *slot = (char *)p + f->off;
-    return xfer(g, f->kind, f->kind <= 3 ? *(REAL ***)slot : NULL, f->kind > 3 ? *(REAL **)slot : NULL,
+    return xfer(g, f->kind, ROWS_KIND(f->kind) ? *(REAL ***)slot : NULL, ROWS_KIND(f->kind) ? NULL : *(REAL **)slot,
                 (double *)in, 0);
   }
   int kind; REAL **rows; REAL *vec;
@@ -841,11 +878,12 @@ static double prop_access(propT *prop, const char *name, double value, int set, 
   *ok = 1;
   PROP_D(dt) PROP_D(theta) PROP_D(thetaS) PROP_D(thetaB) PROP_D(epsilon) PROP_D(qepsilon)
   PROP_D(resnorm) PROP_D(grav) PROP_D(beta) PROP_D(kappa_s) PROP_D(kappa_sH) PROP_D(gamma)
-  PROP_D(nu) PROP_D(nu_H) PROP_D(CdB) PROP_D(thetaM) PROP_D(laxWendroff_Vertical) PROP_D(rtime)
+  PROP_D(nu) PROP_D(nu_H) PROP_D(CdB) PROP_D(CdW) PROP_D(thetaM) PROP_D(laxWendroff_Vertical) PROP_D(rtime)
   PROP_I(n) PROP_I(nstart) PROP_I(nsteps) PROP_I(maxiters) PROP_I(qmaxiters) PROP_I(hprecond)
   PROP_I(qprecond) PROP_I(TVD) PROP_I(horiTVD) PROP_I(vertTVD) PROP_I(TVDsalt) PROP_I(TVDtemp)
   PROP_I(nonhydrostatic) PROP_I(cgsolver) PROP_I(linearFS) PROP_I(stairstep) PROP_I(wetdry)
-  PROP_I(nonlinear) PROP_I(thetaramptime) PROP_I(netcdfBdy)
+  PROP_I(nonlinear) PROP_I(thetaramptime) PROP_I(netcdfBdy) PROP_I(AB) PROP_I(conserveMomentum)
+  PROP_I(TVDmomentum)
   *ok = 0;
   return 0;
 }

@@ -29,10 +29,13 @@ class State:
     """One grid with its physical state and run properties."""
 
     def __init__(self, nx: int, ny: int, Lx: float, Ly: float, Nkmax: int, dz, depth,
-                 stairstep: int = 0, planform: str = "tri", split_every: int = 0, maxfaces=None):
+                 stairstep: int = 0, planform: str = "tri", split_every: int = 0, maxfaces=None,
+                 open_marks=None):
         """planform "tri": nx*ny rectangles split into two triangles
         (sun_grid_tri); "quad" / "hex": rectangles or hexagons, some split
-        when split_every > 0, built by sun_grid_polygons (suntans_amd.planform)."""
+        when split_every > 0, built by sun_grid_polygons (suntans_amd.planform).
+        open_marks(xe, ye, mark) -> new marks per edge opens boundary edges
+        (2 velocity, 3 specified-value cells, 4 no-slip; sun_grid_open)."""
         L = native.state_lib()
         _bind(L)
         self._L = L
@@ -59,6 +62,10 @@ class State:
         dv = np.ascontiguousarray(np.broadcast_to(np.asarray(depth, np.float64), (Nc,)))
         if L.sun_grid_vertical(self.g, int(Nkmax), native.dptr(dz), native.dptr(dv), int(stairstep)):
             raise RuntimeError("sun_grid_vertical failed")
+        if open_marks is not None:
+            marks = np.ascontiguousarray(open_marks(self.get("xe"), self.get("ye"), self.ints("mark")), np.int32)
+            if L.sun_grid_open(self.g, native.iptr(marks)):
+                raise ValueError("sun_grid_open: only boundary edges take marks 1..4")
         self.p = L.sun_phys_alloc(self.g)
         self.prop = L.sun_prop_default()
         self._owns_grid = True

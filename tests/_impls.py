@@ -103,6 +103,17 @@ class _Cpu:
         rc = self.L.oracle_UPredictor_begin(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
         return rc or self.L.oracle_UPredictor_solve(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
 
+    def wpredictor(self, st):
+        getattr(self.L, self.pre + "WPredictor")(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
+    def momentum_faces(self, st):
+        """Face values of wc into ut, as WPredictor's call makes them."""
+        if self.pre == "ref_":
+            self.L.ref_GetMomentumFaceValues(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+        else:
+            self.L.oracle_MomentumFaceValues(*(C.c_void_p(st.ptr(f)) for f in ("ut", "wc", "boundary_w", "utmp2")),
+                                             C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
     def update_salinity(self, st):
         getattr(self.L, self.pre + "update_salinity")(C.c_void_p(st.g), C.c_void_p(st.p),
                                                       C.c_void_p(st.prop))

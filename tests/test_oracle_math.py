@@ -288,3 +288,110 @@ def test_update_dz_column_heights():
     etop = st.ints("etop")
     both = (grad[:, 0] >= 0) & (grad[:, 1] >= 0)
     assert np.array_equal(etop[both], np.minimum(ctop[grad[both, 0]], ctop[grad[both, 1]]))
+
+
+def _edge_cells(st):
+    grad = st.ints("grad").reshape(-1, 2)
+    return grad[:, 0], grad[:, 1]
+
+
+def test_central_face_values_of_a_linear_field():
+    """On rectangles (each edge midpoint on the segment joining the two cell
+    centres) the distance-weighted face value of a field linear in x, y and
+    k is the field at the edge midpoint."""
+    o = _impls.oracle()
+    st = cases.tiny(21, nx=6, ny=5, planform="quad")
+    cases.wpredictor_inputs(st, np.random.default_rng(2), nonlinear=2, stress=True)
+    cell, k = st.cell_layers()
+    xv, yv = st.get("xv"), st.get("yv")
+    st.set("wc", 2e-3 * xv[cell] - 3e-3 * yv[cell] + 0.25 * k)
+    o.momentum_faces(st)
+    edge, ke = st.edge_layers()
+    xe, ye = st.get("xe"), st.get("ye")
+    nc1, nc2 = _edge_cells(st)
+    ctop = st.ints("ctop")
+    comp = (st.ints("mark") == 0)[edge]
+    kmin = np.maximum(ctop[nc1[edge].clip(0)], ctop[nc2[edge].clip(0)])
+    m = comp & (ke >= kmin) & (ke < st.ints("Nke")[edge])
+    want = 2e-3 * xe[edge] - 3e-3 * ye[edge] + 0.25 * ke
+    assert m.sum() > 100
+    assert np.allclose(st.get("ut")[m], want[m], rtol=1e-12, atol=1e-12)
+    assert np.all(st.get("ut")[comp & (ke < kmin)] == 0)
+
+
+def test_upwind_face_value_comes_from_the_outflow_cell():
+    """Scheme 1: the face value is wc of the cell the flow leaves, the one
+    whose outward normal velocity normal * U is positive (when U = 0, the
+    edge's first cell, grad[2j])."""
+    o = _impls.oracle()
+    st = cases.tiny(4, nx=9, ny=6)
+    cases.wpredictor_inputs(st, np.random.default_rng(3), nonlinear=1, stress=True)
+    o.momentum_faces(st)
+    edge, ke = st.edge_layers()
+    nc1, nc2 = _edge_cells(st)
+    nke, ctop, mf = st.ints("Nke"), st.ints("ctop"), st.scalar("maxfaces")
+    face, normal = st.ints("face").reshape(-1, mf), st.ints("normal").reshape(-1, mf)
+    # outward normal sign of each edge seen from its first cell
+    n1 = np.zeros(st.Ne, int)
+    for i in range(st.Nc):
+        for f in range(st.ints("nfaces")[i]):
+            if nc1[face[i, f]] == i:
+                n1[face[i, f]] = normal[i, f]
+    wc, ut, U = st.get("wc"), st.get("ut"), st.get("utmp2")
+    cstart = np.concatenate([[0], np.cumsum(st.ints("Nk"))[:-1]])
+    checked = 0
+    for idx in range(len(edge)):
+        j, k = edge[idx], ke[idx]
+        if st.ints("mark")[j] != 0 or k < max(ctop[nc1[j]], ctop[nc2[j]]) or k >= nke[j]:
+            continue
+        src = nc1[j] if n1[j] * U[idx] > 0 or U[idx] == 0 else nc2[j]
+        assert ut[idx] == wc[cstart[src] + k]
+        checked += 1
+    assert checked > 100
+
+
+def test_horizontal_diffusion_conserves_momentum():
+    """Without advection and sidewall drag the horizontal diffusion only
+    moves w between neighbours: per level, sum of Ac * stmp over the cells
+    vanishes (the reference's own conservation check, main/phys.c:2147-2156)."""
+    o = _impls.oracle()
+    st = cases.tiny(6, nx=10, ny=7)
+    cases.wpredictor_inputs(st, np.random.default_rng(4), nonlinear=0, CdW=0.0, nu_H=1.0)
+    o.wpredictor(st)
+    cell, k = st.cell_layers()
+    flux = st.get("Ac")[cell] * st.get("stmp")
+    scale = np.abs(flux).max()
+    assert scale > 0
+    for lev in range(k.max() + 1):
+        assert abs(flux[k == lev].sum()) <= 1e-13 * scale
+
+
+def test_wpredictor_without_sources_or_viscosity():
+    """No advection, diffusion or drag and zero viscosity: w becomes
+    w + f2 Cn_W + f3 Cn_W2 (Adams-Bashforth 3 weights -4/3 and 5/12) minus
+    dt times the vertical gradient of q between layer centres (q = 0 at the
+    free surface, half a layer above the top centre); Cn_W2 takes the old
+    Cn_W and Cn_W becomes 0."""
+    o = _impls.oracle()
+    st = cases.tiny(7, nx=8, ny=6)
+    cases.wpredictor_inputs(st, np.random.default_rng(5), nonlinear=0, CdW=0.0, nu_H=0.0)
+    st.set("nu_tv", 0.0)
+    st.set("nu_lax", 0.0)
+    st.set_prop(nu=0.0)
+    w0, cn, cn2, q, dzz = (st.get(f) for f in ("w", "Cn_W", "Cn_W2", "q", "dzz"))
+    o.wpredictor(st)
+    dt = st.get_prop("dt")
+    nk, ctop = st.ints("Nk"), st.ints("ctop")
+    wstart = np.concatenate([[0], np.cumsum(nk + 1)[:-1]])
+    cstart = np.concatenate([[0], np.cumsum(nk)[:-1]])
+    w = st.get("w")
+    for i in range(st.Nc):
+        for k in range(ctop[i], nk[i]):
+            iw, ic = wstart[i] + k, cstart[i] + k
+            if k == ctop[i]:
+                grad = -q[ic] / (0.5 * dzz[ic])
+            else:
+                grad = (q[ic - 1] - q[ic]) / (0.5 * (dzz[ic - 1] + dzz[ic]))
+            want = w0[iw] - 4.0 / 3.0 * cn[iw] + 5.0 / 12.0 * cn2[iw] - dt * grad
+            assert abs(w[iw] - want) <= 1e-12 * max(1.0, abs(want)), (i, k)
+            assert st.get("Cn_W2")[iw] == cn[iw] and st.get("Cn_W")[iw] == 0
