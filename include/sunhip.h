@@ -242,6 +242,35 @@ int sunhip_u_predict(sunhip_grid *g, const sunhip_upred_params *p);
  * u = U*** - g theta dt E (h1 - h2)/dg on computational edges. */
 int sunhip_u_solve(sunhip_grid *g, const sunhip_upred_params *p, const sunhip_cg_params *prm, sunhip_cg_stats *st);
 
+/* ---- the vertical-momentum predictor (WPredictor, main/phys.c:2025-2290) ----
+ * With the face values of the cell-centred w it advects (GetMomentumFaceValues
+ * main/phys.c:4879-4992, UFaceFlux :4265-4289).  Fields: w-type "w" (in/out),
+ * "wtmp" (out: the right-hand side as the reference's TriSolve leaves it),
+ * "Cn_W", "Cn_W2" (in/out); cell "wc" (out), "stmp" (out: the cell source),
+ * "q", "dzz", "nu_tv", "nu_lax"; edge "ue" (the reference's phys->ut: face
+ * values out on the edges the reference writes), "utmp2", "bw" (type-2
+ * boundary rows of w, Nke values per edge); for scheme 5 also edge "u",
+ * "dzf", "SfHp", "SfHm" and cell "gradSx", "gradSy" (HorizontalFaceScalars
+ * of wc); scratch "wp_b", "wp_c".  Bit-identical to the reference.
+ * Refused (-1): scheme 4 (its vertical Lax-Wendroff term reads dzz one past
+ * the column bottom), conserveMomentum (the top-cell fill reads scratch left
+ * from the previous column), grids with type-4 edges (the type-4 term runs
+ * to Nke of a cell index). */
+typedef struct {
+  double dt, theta;
+  double nu, nu_H, CdW;  /* prop->nu, nu_H, CdW */
+  double lax_vertical;   /* prop->laxWendroff_Vertical */
+  int nonlinear;         /* 0 none, 1 upwind, 2 central, 5 TVD (TVDmomentum) */
+  int step;              /* prop->n (1: Cn_W and Cn_W2 start from zero) */
+  int AB;                /* prop->AB: 2 or 3 (Adams-Bashforth order from the third step) */
+  int TVDmomentum;       /* limiter of scheme 5 */
+  int conserveMomentum;  /* must be 0 */
+} sunhip_wpred_params;
+/* Distances from each edge's first cell centre to the edge midpoint, [Ne]
+ * host (main/phys.c:4272-4273); central differencing (scheme 2) needs them. */
+int sunhip_set_face_distances(sunhip_grid *g, const double *def1);
+int sunhip_w_predict(sunhip_grid *g, const sunhip_wpred_params *p);
+
 /* ---- the stencils either side of the pressure solve ------------------------
  * Solve's nonhydrostatic sequence is ComputeQSource, CGSolveQ, Corrector,
  * Continuity (main/phys.c:1270-1300).  Bit-identical to the reference. */

@@ -51,6 +51,11 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop);
  * by these names: the split is where a device call cannot reach the hooks. */
 void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
 void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
+/* WPredictor (static, main/phys.c:2025-2290; called at main/phys.c:1271)
+ * under this name, same signature.  Exits through the drop-ins' error path
+ * for nonlinear=4, conserveMomentum or type-4 edges (not built for the
+ * device). */
+void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm);
 /* main/phys.c:816 (UpdateDZ; prototype main/phys.h:220) */
 void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option);
 /* main/phys.c:4338 (SetFluxHeight; prototype main/phys.c:103) */

@@ -32,7 +32,7 @@ OBJ = os.path.join(ROOT, "_build", "obj")
 
 HIP_SOURCES = ["sunhip_grid.hip", "sunhip_q.hip", "sunhip_h.hip", "sunhip_tvd.hip", "sunhip_mixed.hip",
                "sunhip_batch.hip", "sunhip_step.hip", "sunhip_upred.hip",
-               "sunhip_vgrid.hip"]
+               "sunhip_vgrid.hip", "sunhip_wpred.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

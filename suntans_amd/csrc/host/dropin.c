@@ -261,8 +261,8 @@ void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *ph
   }
 }
 
-/* open-boundary rows (edges of edgedist[2..5)) into an edge field */
-static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) {
+/* open-boundary rows (edges of edgedist[2..5)) into edge field name */
+static void upload_brows(ctx_t *C, gridT *g, REAL **bscal, const char *name, const char *W) {
   const int lo = g->edgedist[2], hi = g->edgedist[5];
   if (!bscal || hi <= lo) return;
   size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
@@ -274,8 +274,9 @@ static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) {
     memcpy(C->flat + start[j], bscal[jp - lo], (size_t)g->Nke[j] * sizeof(double));
   }
   free(start);
-  check(sunhip_upload(C->dev, "bscal", C->flat), W);
+  check(sunhip_upload(C->dev, name, C->flat), W);
 }
+static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) { upload_brows(C, g, bscal, "bscal", W); }
 
 static void download_face_values(ctx_t *C, gridT *g, physT *p, const char *W) {
   size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
@@ -579,6 +580,83 @@ void SetFluxHeight(gridT *grid, physT *phys, propT *prop) {
   if (grid->smoothbot) {
     check(sunhip_download(C->dev, "dzz", C->flat), W);
     unpack_cells(C, grid, grid->dzz, 0, 2);
+  }
+}
+
+/* WPredictor (main/phys.c:2025-2290), file-static in the reference: the
+ * same signature under this name, for the call at main/phys.c:1271. */
+void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
+  static const char *W = "WPredictor";
+  (void)myproc; (void)comm;
+  if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
+  const int scheme = prop->nonlinear;
+  if (scheme != 0 && scheme != 1 && scheme != 2 && scheme != 5)
+    fatal(W, "nonlinear must be 0, 1, 2 or 5 (scheme 4's vertical Lax-Wendroff term reads dzz one past the "
+             "column bottom, main/phys.c:2230-2233; not built for the device)");
+  if (prop->conserveMomentum)
+    fatal(W, "conserveMomentum reads scratch left from the previous column in the reference "
+             "(main/phys.c:2136-2138); not built for the device");
+  if (grid->edgedist[5] > grid->edgedist[4])
+    fatal(W, "type-4 (no-slip) edges are not built for the device (the reference's type-4 term runs to Nke of "
+             "a cell index, main/phys.c:2199)");
+  ctx_t *C = get_ctx(grid, W);
+  extents(C, grid, W);
+  upload_cells(C, grid, "w", phys->w, 1, W);
+  upload_cells(C, grid, "wtmp", phys->wtmp, 1, W);
+  upload_cells(C, grid, "Cn_W", phys->Cn_W, 1, W);
+  upload_cells(C, grid, "Cn_W2", phys->Cn_W2, 1, W);
+  upload_cells(C, grid, "q", phys->q, 0, W);
+  upload_cells(C, grid, "dzz", grid->dzz, 0, W);
+  upload_cells(C, grid, "nu_tv", phys->nu_tv, 0, W);
+  upload_cells(C, grid, "nu_lax", phys->nu_lax, 0, W);
+  upload_cells(C, grid, "wc", phys->wc, 0, W);
+  upload_cells(C, grid, "stmp", phys->stmp, 0, W);
+  upload_edges(C, grid, "ue", phys->ut, W);
+  upload_edges(C, grid, "utmp2", phys->utmp2, W);
+  upload_brows(C, grid, phys->boundary_w, "bw", W);
+  if (scheme == 2) {
+    /* main/phys.c:4272-4273 on the host, with the reference's own libm */
+    for (int j = 0; j < grid->Ne; j++) {
+      int nc1 = grid->grad[2 * j];
+      if (nc1 == -1) nc1 = grid->grad[2 * j + 1];
+      C->flat[j] = sqrt(pow(grid->xv[nc1] - grid->xe[j], 2) + pow(grid->yv[nc1] - grid->ye[j], 2));
+    }
+    check(sunhip_set_face_distances(C->dev, C->flat), W);
+  }
+  if (scheme == 5) {
+    upload_edges(C, grid, "u", phys->u, W);
+    upload_edges(C, grid, "dzf", grid->dzf, W);
+    upload_edges(C, grid, "SfHp", phys->SfHp, W);
+    upload_edges(C, grid, "SfHm", phys->SfHm, W);
+    upload_cells(C, grid, "gradSx", phys->gradSx, 0, W);
+    upload_cells(C, grid, "gradSy", phys->gradSy, 0, W);
+  }
+  sunhip_wpred_params P = {prop->dt, prop->theta, prop->nu, prop->nu_H, prop->CdW, prop->laxWendroff_Vertical,
+                           scheme, prop->n, prop->AB, prop->TVDmomentum, prop->conserveMomentum};
+  check(sunhip_w_predict(C->dev, &P), W);
+  const char *wnames[] = {"w", "wtmp", "Cn_W", "Cn_W2"};
+  REAL **wrows[] = {phys->w, phys->wtmp, phys->Cn_W, phys->Cn_W2};
+  for (int f = 0; f < 4; f++) {
+    check(sunhip_download(C->dev, wnames[f], C->flat), W);
+    unpack_cells_w(C, grid, wrows[f], 0, 2);
+  }
+  check(sunhip_download(C->dev, "stmp", C->flat), W);
+  unpack_cells(C, grid, phys->stmp, 0, 2);
+  if (scheme) {
+    check(sunhip_download(C->dev, "wc", C->flat), W);
+    unpack_cells(C, grid, phys->wc, 0, 2);
+    check(sunhip_download(C->dev, "ue", C->flat), W);
+    unpack_edges(C, grid, phys->ut);
+  }
+  if (scheme == 5) {
+    check(sunhip_download(C->dev, "SfHp", C->flat), W);
+    unpack_edges(C, grid, phys->SfHp);
+    check(sunhip_download(C->dev, "SfHm", C->flat), W);
+    unpack_edges(C, grid, phys->SfHm);
+    check(sunhip_download(C->dev, "gradSx", C->flat), W);
+    unpack_cells(C, grid, phys->gradSx, 0, 2);
+    check(sunhip_download(C->dev, "gradSy", C->flat), W);
+    unpack_cells(C, grid, phys->gradSy, 0, 2);
   }
 }
 

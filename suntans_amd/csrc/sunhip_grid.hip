@@ -34,11 +34,15 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 }
 
 Kind kind_of(const char *name) {
-  static const char *edge[] = {"dzf", "u", "utmp", "utmp2", "u0", "ue", "SfHp", "SfHm", "ef0", "ef1", "bscal"};
-  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB", "dzfB"};
+  static const char *edge[] = {"dzf", "u", "utmp", "utmp2", "u0", "ue", "SfHp", "SfHm", "ef0", "ef1", "bscal",
+                               "bw"};
+  static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB", "dzfB", "def1"};
+  static const char *cellw[] = {"Cn_W", "Cn_W2"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
                                 "hx", "hy", "active", "htmp0", "dv", "hcorr", "dzbot"};
   if (!strncmp(name, "up_", 3)) return EDGE;  // UPredictor's per-edge scratch
+  if (!strcmp(name, "wc")) return CELL;       // cell-centred w: Nk values per column
+  for (auto n : cellw) if (!strcmp(name, n)) return CELLW;
   for (auto n : edge) if (!strcmp(name, n)) return EDGE;
   for (auto n : edge2) if (!strcmp(name, n)) return EDGE2;
   for (auto n : cell2) if (!strcmp(name, n)) return CELL2;
@@ -349,6 +353,26 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   }
   for (int jp = T->edgedist[0]; jp < T->edgedist[1]; jp++) ecomp[g->slot_of_edge[T->edgep[jp]]] = 1;
   for (int jp = T->edgedist[2]; jp < T->edgedist[3]; jp++) ecomp[g->slot_of_edge[T->edgep[jp]]] = 2;
+  // each column's faces in the order their edges appear in edgep (the order
+  // of the reference's edge loops that scatter onto cells)
+  std::vector<int> ford(mf * NS, -1);
+  {
+    std::vector<int> pos(Ne, 0);
+    for (int jp = 0; jp < Ne; jp++) pos[T->edgep[jp]] = jp;
+    for (int s = 0; s < NS; s++) {
+      const int c = g->cell_of_slot[s];
+      if (c < 0) continue;
+      std::vector<int> fs(T->nfaces[c]);
+      for (int f = 0; f < T->nfaces[c]; f++) fs[f] = f;
+      std::stable_sort(fs.begin(), fs.end(),
+                       [&](int x, int y) { return pos[T->face[c * mf + x]] < pos[T->face[c * mf + y]]; });
+      for (int r = 0; r < T->nfaces[c]; r++) ford[r * NS + s] = fs[r];
+    }
+  }
+  g->n_type4 = T->edgedist[5] - T->edgedist[4];
+  g->type2_ok = 1;
+  for (int jp = T->edgedist[2]; jp < T->edgedist[3]; jp++)
+    if (T->grad[2 * T->edgep[jp]] < 0) g->type2_ok = 0;
 
   int rc = 0;
   HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
@@ -404,6 +428,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= upload_vec(g, &g->d_df, df);
   rc |= upload_vec(g, &g->d_dg, dg);
   rc |= upload_vec(g, &g->d_estart, est);
+  rc |= upload_vec(g, &g->d_ford, ford);
   {
     std::vector<int> cs(g->slot_of_cell), esv(g->slot_of_edge);
     rc |= upload_vec(g, &g->d_cellslot, cs);
@@ -451,7 +476,8 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
-                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold};
+                  g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold,
+                  g->d_ford};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   if (g->h_sc) hipHostFree(g->h_sc);

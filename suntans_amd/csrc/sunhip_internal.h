@@ -126,6 +126,10 @@ struct sunhip_grid {
   int *d_uerr = nullptr;             // UPredictor's exit conditions
   double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)
   int *d_e_etopold = nullptr;        // [NES] etop before the last UpdateDZ
+  int *d_ford = nullptr;             // [MAXF*NS] each column's faces in edgep order of their edges
+  int n_type4 = 0;                   // no-slip (type-4) edges
+  int type2_ok = 1;                  // every type-2 edge has its cell on the grad[2j] side
+  bool have_def1 = false;            // "def1" holds the face distances (sunhip_set_face_distances)
   int max_pairs = 0;                 // most cross-tile faces of any tile
   double *d_Ac = nullptr, *d_df = nullptr, *d_dg = nullptr;
   int *d_e_c1 = nullptr, *d_e_c2 = nullptr, *d_e_s1 = nullptr, *d_e_s2 = nullptr;

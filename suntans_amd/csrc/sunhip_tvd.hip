@@ -74,6 +74,14 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
   ColArgs C;
   int rc = tvd_args(g, sp, scal, &A, &C);
   if (rc) return rc;
+  return face_scalars_launch(g, A);
+}
+
+}  // extern "C"
+
+namespace sunhip {
+
+int face_scalars_launch(sunhip_grid *g, const TvdArgs &A) {
   const Geo G = g->geo();
   int ps = prof_begin(g, PK_TSUMS);
   with_nf(g->mf, [&](auto nf) {
@@ -87,6 +95,10 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
   HIPCHK(hipGetLastError());
   return 0;
 }
+
+}  // namespace sunhip
+
+extern "C" {
 
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
   HIPCHK(hipSetDevice(g->device));

@@ -375,4 +375,8 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   for (int k = ct; k < cto; k++) A.s[base + (int64_t)k * 64] = stop;
 }
 
+// Both passes of HorizontalFaceScalars (sums, then limited face values) for
+// the arguments in A, on the grid's stream.
+int face_scalars_launch(sunhip_grid *g, const TvdArgs &A);
+
 }  // namespace sunhip

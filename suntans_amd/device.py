@@ -59,6 +59,32 @@ class UPredParams(C.Structure):
                 ("lax_vertical", C.c_double), ("thetaM", C.c_double), ("nonlinear", C.c_int)]
 
 
+class WPredParams(C.Structure):
+    """sunhip_wpred_params: dt, theta, nu, nu_H, CdW, laxWendroff_Vertical,
+    nonlinear (0, 1, 2, 5), step (prop->n), AB, TVDmomentum,
+    conserveMomentum (must be 0)."""
+    _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("nu", C.c_double), ("nu_H", C.c_double),
+                ("CdW", C.c_double), ("lax_vertical", C.c_double), ("nonlinear", C.c_int), ("step", C.c_int),
+                ("AB", C.c_int), ("TVDmomentum", C.c_int), ("conserveMomentum", C.c_int)]
+
+    @classmethod
+    def of(cls, st) -> "WPredParams":
+        g = st.get_prop
+        return cls(g("dt"), g("theta"), g("nu"), g("nu_H"), g("CdW"), g("laxWendroff_Vertical"),
+                   int(g("nonlinear")), int(g("n")), int(g("AB")), int(g("TVDmomentum")),
+                   int(g("conserveMomentum")))
+
+
+def face_distances(st) -> np.ndarray:
+    """Distance from each edge's first cell centre to its midpoint
+    (main/phys.c:4272-4273), for sunhip_set_face_distances."""
+    grad = st.ints("grad").reshape(-1, 2)
+    nc1 = np.where(grad[:, 0] == -1, grad[:, 1], grad[:, 0])
+    dx = st.get("xv")[nc1] - st.get("xe")
+    dy = st.get("yv")[nc1] - st.get("ye")
+    return np.sqrt(dx * dx + dy * dy)
+
+
 class HPhys(C.Structure):
     _fields_ = [("grav", C.c_double), ("theta", C.c_double), ("dt", C.c_double)]
 
@@ -120,6 +146,8 @@ def lib() -> C.CDLL:
                                                  C.POINTER(CGStats), _ip]
         L.sunhip_batch_update_scalars.argtypes = [vp, pp, C.c_int, C.POINTER(ScalarParams), C.c_char_p]
         L.sunhip_u_predict.argtypes = [vp, C.POINTER(UPredParams)]
+        L.sunhip_w_predict.argtypes = [vp, C.POINTER(WPredParams)]
+        L.sunhip_set_face_distances.argtypes = [vp, vp]
         L.sunhip_update_dz.argtypes = [vp, C.POINTER(VGridParams), C.c_int]
         L.sunhip_set_flux_height.argtypes = [vp, C.POINTER(VGridParams)]
         L.sunhip_get_extents.argtypes = [vp, _ip, _ip, _ip, _ip]
@@ -289,6 +317,14 @@ class DeviceGrid:
         S = CGStats()
         check(lib().sunhip_u_solve(self.h, C.byref(P), C.byref(prm), C.byref(S)), "sunhip_u_solve")
         return S.as_dict()
+
+    def set_face_distances(self, def1):
+        d = np.ascontiguousarray(def1, np.float64)
+        check(lib().sunhip_set_face_distances(self.h, d.ctypes.data_as(C.c_void_p)), "sunhip_set_face_distances")
+
+    def w_predict(self, P: "WPredParams"):
+        """WPredictor: w, wtmp, wc, stmp, Cn_W, Cn_W2 and the face values in ue."""
+        check(lib().sunhip_w_predict(self.h, C.byref(P)), "sunhip_w_predict")
 
     def compute_qsource(self, theta, dt, src="src"):
         """ComputeQSource: src from u, utmp2, w, wtmp2, dzf; D <- df/dg."""

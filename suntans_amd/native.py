@@ -113,6 +113,7 @@ def dropin_lib() -> C.CDLL:
         L.SetFluxHeight.argtypes = [_vp, _vp, _vp]
         L.sunhip_UPredictor_begin.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.sunhip_UPredictor_solve.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
+        L.sunhip_WPredictor.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int]
         L.UpdateScalars.argtypes = ([_vp] * 7 + [C.c_double, C.c_double, _vp, C.c_double]
                                     + [_vp] * 4 + [C.c_int] * 6)
         L.sunhip_last_solve.argtypes = [C.c_int, _ip, _dp, _dp]

@@ -225,6 +225,9 @@ class DropinImpl:
     def upredictor_solve(self, st):
         self.L.sunhip_UPredictor_solve(st.g, st.p, st.prop, 0, 1, 0)
 
+    def wpredictor(self, st):
+        self.L.sunhip_WPredictor(st.g, st.p, st.prop, 0, 1, 0)
+
     def update_salinity(self, st):
         kappa = st.get_prop("kappa_s")
         kappaH = st.get_prop("kappa_sH")

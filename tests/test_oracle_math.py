@@ -395,3 +395,17 @@ def test_wpredictor_without_sources_or_viscosity():
             want = w0[iw] - 4.0 / 3.0 * cn[iw] + 5.0 / 12.0 * cn2[iw] - dt * grad
             assert abs(w[iw] - want) <= 1e-12 * max(1.0, abs(want)), (i, k)
             assert st.get("Cn_W2")[iw] == cn[iw] and st.get("Cn_W")[iw] == 0
+
+
+@pytest.mark.parametrize("planform", ["tri", "quad", "hex"])
+def test_face_distances_match_the_oracle(planform):
+    """The host-side face distances handed to the device (numpy) equal the
+    oracle's (the reference's expression, main/phys.c:4272-4273) bit for bit."""
+    import ctypes as C
+    from suntans_amd import device
+    o = _impls.oracle()
+    st = cases.tiny(8, nx=7, ny=5, planform=planform, open_marks=cases.open_ends())
+    o.L.oracle_face_def1.restype = C.c_double
+    o.L.oracle_face_def1.argtypes = [C.c_void_p, C.c_int]
+    want = np.array([o.L.oracle_face_def1(st.g, j) for j in range(st.Ne)])
+    assert np.array_equal(device.face_distances(st), want)
