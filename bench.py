@@ -200,6 +200,129 @@ def upred_leg(st, dg, N, Ek, steps=3):
     return res
 
 
+def wpred_leg(st, dg, N, Ek, steps=3):
+    """WPredictor on the large grid, all fields resident: central
+    differencing (nonlinear=2) at an Adams-Bashforth-3 step, horizontal
+    diffusion and sidewall drag, implicit vertical diffusion.  Per-kernel
+    times against algorithmic bytes (8 B each): cell source 2E_k + 4N (face
+    values and velocities once per edge-layer; w, dzz, nu_lax, stmp), column
+    12N (w, Cn_W, Cn_W2, q, dzz, stmp, nu_lax, nu_tv in; w, wtmp, Cn_W, Cn_W2
+    out)."""
+    from suntans_amd import device
+    rng = np.random.default_rng(13)
+    nw, nflat = N + st.Nc, len(st.get("dzz"))
+    u = st.get("u")
+    dg.upload("w", st.get("w"))
+    dg.copy("w0", "w")
+    dg.upload("Cn_W", rng.normal(0, 1e-5, nw))
+    dg.copy("wcn0", "Cn_W")
+    dg.upload("Cn_W2", rng.normal(0, 1e-5, nw))
+    dg.copy("wcn20", "Cn_W2")
+    dg.upload("qp", rng.normal(0, 1e-3, nflat))
+    dg.upload("nu_tv", rng.uniform(0, 1e-3, nflat))
+    dg.upload("nu_lax", rng.uniform(0, 1e-4, nflat))
+    dg.upload("utmp2", u)
+    dg.set_face_distances(device.face_distances(st))
+    P = device.WPredParams(st.get_prop("dt"), st.get_prop("theta"), 1e-4, 0.1, 5e-3, 1.0, 2, 3, 3, 4, 0)
+    walls = []
+    for i in range(steps + 1):
+        dg.copy("w", "w0")
+        dg.copy("Cn_W", "wcn0")
+        dg.copy("Cn_W2", "wcn20")
+        dg.sync()
+        if i == 1:
+            dg.profile(True)
+        t = time.perf_counter()
+        dg.w_predict(P)
+        walls.append(time.perf_counter() - t)
+    kt = dg.kernel_times()
+    dg.profile(False)
+    res = {"ms": float(np.median(walls[1:])) * 1e3}
+    # "other" holds the centres and face-value launches, two per call: time per call
+    per_call = {"wp_source": 1, "wp_column": 1, "other": 2}
+    for k, b in {"wp_source": 8 * (2 * Ek + 4 * N), "wp_column": 8 * 12 * N, "other": 8 * (2 * Ek + 5 * N)}.items():
+        if k in kt:
+            ms = kt[k][0] / kt[k][1] * per_call[k]
+            res[k] = {"ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"wpredictor: {res}")
+    return res
+
+
+def step_leg(st, dg, steps=3):
+    """Consecutive nonhydrostatic time steps on the large grid with every
+    field resident, in the reference's order (Solve, main/phys.c:1145-1306):
+    SetFluxHeight, UPredictor (free-surface PCG inside), UpdateDZ, Continuity
+    into wnew, UpdateScalars (TVD superbee salinity), WPredictor,
+    ComputeQSource, CGSolveQ (qprecond=2), Corrector, Continuity.  The
+    explicit horizontal momentum source (HorizontalSource, host code) is a
+    fixed seeded field, and the open-boundary hooks do nothing on the closed
+    grid.  Stage times are host wall clock around synchronised calls."""
+    from suntans_amd import device
+    rng = np.random.default_rng(17)
+    nw, nflat, ne = st.N + st.Nc, len(st.get("dzz")), st.Ne
+    u = st.get("u")
+    dg.set_extents(st)
+    for f in ("u", "utmp2", "dzf", "dzz", "dzzold", "s", "h", "dv"):
+        dg.upload(f, st.get(f))
+    w = st.get("w")
+    for f in ("w", "wnew", "wtmp2"):
+        dg.upload(f, w)
+    dg.upload("utmp", rng.normal(0, 0.05, len(u)) * st.edge_wet_mask())
+    dg.copy("up_src", "utmp")
+    dg.upload("kappa_tv", np.zeros(nflat))
+    dg.upload("nu_tv", rng.uniform(0, 1e-3, nflat))
+    dg.upload("nu_lax", rng.uniform(0, 1e-4, nflat))
+    dg.upload("CdB", rng.uniform(1e-3, 5e-3, ne))
+    dg.upload("CdT", rng.uniform(0.0, 2e-3, ne))
+    dg.upload("tau_T", rng.normal(0, 1e-4, ne))
+    dg.upload("hcorr", np.zeros(st.Nc))
+    dg.upload("active", np.ones(st.Nc))
+    dg.upload("Cn_W", np.zeros(nw))
+    dg.upload("Cn_W2", np.zeros(nw))
+    dg.upload("qp", np.zeros(nflat))
+    dg.upload("q", np.zeros(nflat))
+    dg.set_face_distances(device.face_distances(st))
+    g = st.get_prop
+    th, dt = g("theta"), g("dt")
+    PU = device.UPredParams(dt, th, g("grav"), 1e-4, 1.0, 0.5, 2)
+    stages = ("flux_height", "u_predict", "h_solve", "update_dz", "continuity_wnew", "scalars", "w_predict",
+              "q_source", "q_solve", "corrector", "continuity")
+    rows, its = [], []
+    for i in range(steps + 1):
+        PW = device.WPredParams(dt, th, 1e-4, 0.1, 5e-3, 1.0, 2, min(i + 1, 3), 3, 4, 0)
+        tm = {}
+
+        def run(name, fn):
+            t = time.perf_counter()
+            r = fn()
+            dg.sync()
+            tm[name] = (time.perf_counter() - t) * 1e3
+            return r
+
+        dg.copy("utmp", "up_src")
+        dg.sync()
+        run("flux_height", lambda: dg.set_flux_height(st))
+        run("u_predict", lambda: dg.u_predict(PU))
+        sh = run("h_solve", lambda: dg.u_solve(PU, maxiters=1000, precond=1, epsilon=1e-10))
+        run("update_dz", lambda: dg.update_dz(st, 0))
+        run("continuity_wnew", lambda: dg.continuity(th, "wnew"))
+        run("scalars", lambda: dg.update_scalars(th, dt, 0.0, 4, step=i + 1))
+        run("w_predict", lambda: dg.w_predict(PW))
+        run("q_source", lambda: dg.compute_qsource(th, dt, "src"))
+        sq = run("q_solve", lambda: dg.q_solve(maxiters=2000, precond=2, epsilon=1e-10))
+        run("corrector", lambda: dg.corrector(dt, "q", "qp"))
+        run("continuity", lambda: dg.continuity(th, "w"))
+        rows.append(tm)
+        its.append((sh["niters"], sq["niters"]))
+    med = {k: float(np.median([r[k] for r in rows[1:]])) for k in stages}
+    total = [sum(r.values()) for r in rows[1:]]
+    res = {"step_ms": float(np.median(total)), "steps_per_s": 1e3 / float(np.median(total)),
+           "stages_ms": med, "h_iterations": [x[0] for x in its], "q_iterations": [x[1] for x in its],
+           "w_finite": bool(np.isfinite(dg.download("w", nw)).all())}
+    log(f"time step: {res}")
+    return res
+
+
 def ensemble_specs(n: int, seed: int):
     """Member sizes of the ensemble: cells U[5e3, 2e5], aspect U[1, 4],
     levels U{20..100} (seeded)."""
@@ -530,6 +653,8 @@ def main():
         dg.upload("D", st.get("D"))
         out["tvd"] = tvd_leg(st, dg, N, Ek)
         out["upredictor"] = upred_leg(st, dg, N, Ek)
+        out["wpredictor"] = wpred_leg(st, dg, N, Ek)
+        out["time_step"] = step_leg(st, dg)
 
     if args.copy_probe:
         cp_ms = dg.copy_probe(1 << 30, 20)

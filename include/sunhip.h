@@ -156,7 +156,8 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
  * kernel id and returns the number of ids: q apply, q precond, h apply,
  * h precond, mixed apply, mixed precond, mixed residual, other, TVD face
  * sums, TVD face values, scalar column update, Poisson source, Corrector,
- * Continuity. */
+ * Continuity, WPredictor cell source, WPredictor column (its face values
+ * count as other). */
 #define SUNHIP_NPROF 16
 int sunhip_profile(sunhip_grid *g, int on);
 int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n);
@@ -247,7 +248,8 @@ int sunhip_u_solve(sunhip_grid *g, const sunhip_upred_params *p, const sunhip_cg
  * main/phys.c:4879-4992, UFaceFlux :4265-4289).  Fields: w-type "w" (in/out),
  * "wtmp" (out: the right-hand side as the reference's TriSolve leaves it),
  * "Cn_W", "Cn_W2" (in/out); cell "wc" (out), "stmp" (out: the cell source),
- * "q", "dzz", "nu_tv", "nu_lax"; edge "ue" (the reference's phys->ut: face
+ * "qp" (the nonhydrostatic pressure phys->q, which sunhip_corrector
+ * accumulates), "dzz", "nu_tv", "nu_lax"; edge "ue" (the reference's phys->ut: face
  * values out on the edges the reference writes), "utmp2", "bw" (type-2
  * boundary rows of w, Nke values per edge); for scheme 5 also edge "u",
  * "dzf", "SfHp", "SfHm" and cell "gradSx", "gradSy" (HorizontalFaceScalars

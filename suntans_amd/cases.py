@@ -291,3 +291,22 @@ def wpredictor_inputs(st: State, rng, nonlinear: int = 2, step: int = 3, AB: int
     st.set_prop(nu=1e-4, nu_H=nu_H, CdW=CdW, laxWendroff_Vertical=1.0, nonlinear=nonlinear, n=step, AB=AB,
                 conserveMomentum=conserve, TVDmomentum=tvd)
     return st
+
+
+def timestep_inputs(st: State, rng, nonlinear: int = 1, tvd: int = 4) -> State:
+    """Inputs of one nonhydrostatic time step after HorizontalSource: those of
+    UPredictor (the horizontal source in utmp, drag, viscosities), the
+    vertical-momentum histories Cn_W/Cn_W2, the nonhydrostatic pressure q,
+    stale face values and the properties of WPredictor, UpdateScalars
+    (salinity, TVD limiter tvd) and CGSolveQ.  u, utmp2 and w stay as built."""
+    upredictor_inputs(st, rng, nonlinear=nonlinear)
+    nw = len(st.get("Cn_W"))
+    st.set("Cn_W", rng.normal(0, 1e-5, nw))
+    st.set("Cn_W2", rng.normal(0, 1e-5, nw))
+    n = len(st.get("wc"))
+    st.set("wc", rng.normal(0, 1e-3, n))
+    st.set("q", rng.normal(0, 1e-3, n) * st.wet_mask())
+    st.set("ut", rng.normal(0, 1e-2, len(st.get("ut"))))
+    st.set_prop(nu_H=0.1, CdW=5e-3, n=3, AB=3, TVDmomentum=tvd, conserveMomentum=0, beta=1.0, TVDsalt=tvd,
+                kappa_s=1e-5, qprecond=2, qepsilon=1e-10, qmaxiters=2000, resnorm=0)
+    return st

@@ -605,7 +605,7 @@ void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int nu
   upload_cells(C, grid, "wtmp", phys->wtmp, 1, W);
   upload_cells(C, grid, "Cn_W", phys->Cn_W, 1, W);
   upload_cells(C, grid, "Cn_W2", phys->Cn_W2, 1, W);
-  upload_cells(C, grid, "q", phys->q, 0, W);
+  upload_cells(C, grid, "qp", phys->q, 0, W);
   upload_cells(C, grid, "dzz", grid->dzz, 0, W);
   upload_cells(C, grid, "nu_tv", phys->nu_tv, 0, W);
   upload_cells(C, grid, "nu_lax", phys->nu_lax, 0, W);

@@ -200,7 +200,7 @@ int poll_chunk(sunhip_grid *g, const CGScal *d, int &slot, bool &pending, bool *
 void prof_end(sunhip_grid *g, int slot);
 enum { PK_QAPPLY = 0, PK_QPREC = 1, PK_HAPPLY = 2, PK_HPREC = 3, PK_MAPPLY = 4, PK_MPREC = 5, PK_RESID = 6,
        PK_OTHER = 7, PK_TSUMS = 8, PK_TFACES = 9, PK_TCOLUMN = 10, PK_QSOURCE = 11, PK_CORRECT = 12,
-       PK_CONTINUITY = 13, PK_COUNT = 14 };
+       PK_CONTINUITY = 13, PK_WSOURCE = 14, PK_WCOLUMN = 15, PK_COUNT = 16 };
 }  // namespace sunhip
 
 namespace sunhip {

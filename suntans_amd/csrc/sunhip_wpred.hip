@@ -292,7 +292,7 @@ static int wpred_args(sunhip_grid *g, const sunhip_wpred_params *p, WPArgs *out)
   }
   struct { const char *name; double **ptr; } f[] = {
       {"w", nullptr}, {"wtmp", nullptr}, {"wc", nullptr}, {"stmp", nullptr}, {"Cn_W", nullptr},
-      {"Cn_W2", nullptr}, {"wp_b", nullptr}, {"wp_c", nullptr}, {"q", nullptr}, {"dzz", nullptr},
+      {"Cn_W2", nullptr}, {"wp_b", nullptr}, {"wp_c", nullptr}, {"qp", nullptr}, {"dzz", nullptr},
       {"nu_tv", nullptr}, {"nu_lax", nullptr}, {"ue", nullptr}, {"utmp2", nullptr}, {"bw", nullptr},
       {"SfHp", nullptr}, {"SfHm", nullptr}, {"def1", nullptr}};
   WPArgs A{};
@@ -345,7 +345,9 @@ int sunhip_w_predict(sunhip_grid *g, const sunhip_wpred_params *p) {
   const Geo G = g->geo();
   const int nc = (g->Nt + 3) / 4, ne = (g->Net + 3) / 4;
   if (A.scheme) {
+    int ps = prof_begin(g, PK_OTHER);
     k_wp_centres<<<nc, 256, 0, g->stream>>>(G, A);
+    prof_end(g, ps);
     HIPCHK(hipGetLastError());
     if (A.scheme == 5) {
       // HorizontalFaceScalars of wc with the boundary rows of w (main/phys.c:4906-4907)
@@ -359,14 +361,20 @@ int sunhip_w_predict(sunhip_grid *g, const sunhip_wpred_params *p) {
       T.bscal = A.bw; T.theta = p->theta; T.TVD = p->TVDmomentum;
       if ((rc = face_scalars_launch(g, T))) return rc;
     }
+    ps = prof_begin(g, PK_OTHER);
     k_wp_faces<<<ne, 256, 0, g->stream>>>(G, A);
+    prof_end(g, ps);
     HIPCHK(hipGetLastError());
   }
+  int ps = prof_begin(g, PK_WSOURCE);
   with_nf(g->mf, [&](auto nf) {
     k_wp_source<decltype(nf)::value><<<nc, 256, 0, g->stream>>>(G, A);
     return 0;
   });
+  prof_end(g, ps);
+  ps = prof_begin(g, PK_WCOLUMN);
   k_wp_column<<<nc, 256, 0, g->stream>>>(G, A);
+  prof_end(g, ps);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;

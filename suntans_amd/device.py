@@ -347,7 +347,8 @@ class DeviceGrid:
         n = (C.c_int * 16)()
         k = lib().sunhip_kernel_times(self.h, ms, n)
         names = ("qapply", "qprec", "happly", "hprec", "mixed_apply", "mixed_prec", "resid64", "other",
-                 "tvd_sums", "tvd_faces", "tvd_column", "qsource", "corrector", "continuity")
+                 "tvd_sums", "tvd_faces", "tvd_column", "qsource", "corrector", "continuity", "wp_source",
+                 "wp_column")
         return {names[i]: (ms[i], n[i]) for i in range(min(k, len(names))) if n[i] > 0}
 
     def copy_probe(self, nbytes: int, reps: int):

@@ -1,0 +1,101 @@
+"""One nonhydrostatic time step with every field resident on the GPU, in
+the reference's order (Solve, main/phys.c:1145-1306): SetFluxHeight,
+UPredictor (its open-boundary hooks do nothing on these closed grids),
+UpdateDZ, Continuity into wnew, UpdateScalars (salinity), WPredictor,
+ComputeQSource, CGSolveQ, Corrector and Continuity, against the same
+sequence of oracle calls.  The free-surface and pressure solves sum their
+dot products in another order, so the fields agree to 1e-9 (relative
+2-norm over the wet layers) with the solves' iteration counts within +-1,
+and the discrete state (ctop, etop, wet and dry cells) is identical."""
+import numpy as np
+import pytest
+
+from suntans_amd import cases, device
+
+import _impls
+
+pytestmark = pytest.mark.gpu
+
+MAKERS = [
+    ("lockexchange", lambda: cases.lock_exchange(seed=7)),
+    ("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)),
+    ("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7)),
+]
+IDS = [m[0] for m in MAKERS]
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def oracle_step(o, a):
+    o.set_flux_height(a)
+    assert o.upredictor(a) == 0
+    o.update_dz(a, 0)
+    o.continuity(a, w="wnew")
+    o.update_salinity(a)
+    o.wpredictor(a)
+    o.compute_qsource(a)
+    o.cgsolveq(a)
+    o.corrector(a, qc="qc")
+    o.continuity(a, w="w")
+
+
+def device_step(dg, a):
+    g = a.get_prop
+    th, dt = g("theta"), g("dt")
+    P = device.UPredParams(dt, th, g("grav"), g("nu"), g("laxWendroff_Vertical"), g("thetaM"), int(g("nonlinear")))
+    dg.set_flux_height(a)
+    dg.u_predict(P)
+    sh = dg.u_solve(P, maxiters=int(g("maxiters")), precond=int(g("hprecond")), epsilon=g("epsilon"))
+    dg.update_dz(a, 0)
+    dg.continuity(th, "wnew")
+    dg.update_scalars(th, dt, g("kappa_s"), int(g("TVDsalt")), step=int(g("n")))
+    dg.w_predict(device.WPredParams.of(a))
+    dg.compute_qsource(th, dt, "src")
+    sq = dg.q_solve(maxiters=int(g("qmaxiters")), precond=2, epsilon=g("qepsilon"))
+    dg.corrector(dt, "q", "qp")
+    dg.continuity(th, "w")
+    return sh, sq
+
+
+def upload_state(dg, a):
+    for f in ("u", "utmp", "utmp2", "dzf", "dzz", "dzzold", "s", "kappa_tv", "nu_tv", "nu_lax", "wc", "w", "wnew",
+              "wtmp2", "wtmp", "Cn_W", "Cn_W2", "h", "dv", "hcorr", "CdB", "CdT", "tau_T", "D"):
+        dg.upload(f, a.get(f))
+    dg.upload("ue", a.get("ut"))
+    dg.upload("q", a.get("qc"))
+    dg.upload("qp", a.get("q"))
+    dg.upload("active", a.active().astype(np.float64))
+    dg.set_face_distances(device.face_distances(a))
+
+
+@pytest.mark.parametrize("nonlinear", [1, 2])
+@pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
+def test_resident_time_step(gpu_available, name, make, nonlinear):
+    o = _impls.oracle()
+    a = make()
+    cases.timestep_inputs(a, np.random.default_rng(29), nonlinear=nonlinear)
+    dg = device.DeviceGrid(a)
+    upload_state(dg, a)
+    sh, sq = device_step(dg, a)
+    oracle_step(o, a)
+    assert abs(sh["niters"] - o.stats("h")[0]) <= 1
+    assert abs(sq["niters"] - o.stats("q")[0]) <= 1
+    ext = dg.extents()
+    for f in ("ctop", "ctopold", "etop", "etopold"):
+        assert np.array_equal(ext[f], a.ints(f)), f
+    assert np.array_equal(dg.download("active", a.Nc) != 0, a.active() != 0)
+    m = a.wet_mask()
+    assert relerr(dg.download("h", a.Nc), a.get("h")) <= 1e-9
+    assert np.array_equal(dg.download("dzz", len(m)), a.get("dzz")) or \
+        relerr(dg.download("dzz", len(m)), a.get("dzz")) <= 1e-9
+    for dev_name, st_name, wet in (("u", "u", a.edge_wet_mask()), ("w", "w", None), ("s", "s", m),
+                                   ("q", "qc", m), ("qp", "q", m), ("Cn_W", "Cn_W", None), ("wnew", "wnew", None)):
+        want = a.get(st_name)
+        got = dg.download(dev_name, len(want))
+        if wet is not None:
+            want, got = want[wet], got[wet]
+        assert np.isfinite(got).all(), dev_name
+        assert relerr(got, want) <= 1e-9, (dev_name, relerr(got, want))
+    dg.close()

@@ -54,8 +54,9 @@ def test_wpredictor_bitwise(gpu_available, name, make, variant):
 
 def _device(st):
     dg = device.DeviceGrid(st)
-    for f in ("w", "wtmp", "Cn_W", "Cn_W2", "q", "dzz", "nu_tv", "nu_lax", "wc", "utmp2"):
+    for f in ("w", "wtmp", "Cn_W", "Cn_W2", "dzz", "nu_tv", "nu_lax", "wc", "utmp2"):
         dg.upload(f, st.get(f))
+    dg.upload("qp", st.get("q"))
     dg.upload("ue", st.get("ut"))
     dg.set_face_distances(device.face_distances(st))
     return dg
