@@ -48,6 +48,18 @@ gridT *sun_grid_polygons(int Np, const double *xp, const double *yp, int Nc, int
  * changes nothing for a mark the rule does not allow. */
 int sun_grid_open(gridT *g, const int *mark);
 
+/* A grid as the reference's run phase reads it for one process
+ * (ReadGrid, main/gridio.c:54-187, then InitializeVerticalGrid,
+ * main/phys.c:765-799): dir holds suntans.dat, whose celldata, edgedata,
+ * nodes, topology and vertspace keys name the files GetGrid wrote (".0"
+ * appended to the first four), maxFaces (default 3), Nkmax, stairstep,
+ * fixdzz, dzsmall and smoothbot.  Numbers are parsed as the reference's
+ * getfield parses them (main/fileio.c:137-174), so every array is the
+ * reference's bit for bit.  The nodes' neighbour lists are skipped.
+ * Returns NULL when a file is missing or written for more than one
+ * process. */
+gridT *sun_grid_load(const char *dir);
+
 /* Vertical grid: dz[Nkmax] reference layer thicknesses and dv[Nc] depths.
  * Sets Nk (main/grid.c:1466-1498), Nke/Nkc (:1530-1563), ctop/etop = 0, and
  * allocates dzz/dzzold (Nk) and dzf (Nkc) rows as main/phys.c:765-799. */

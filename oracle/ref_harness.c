@@ -69,3 +69,22 @@ EXPORT void ref_update_salinity(gridT *g, physT *p, propT *pr) {
 EXPORT void ref_solve_q(gridT *g, physT *p, propT *pr) {
   CGSolveQ(p->qc, p->stmp, p->stmp3, g, p, pr, 0, 1, 0);
 }
+
+/* Grid generation and the run-phase reader, for the grid-file tests: the
+ * data directory dir holds suntans.dat and the planform files it names;
+ * GetGrid (main/grid.c:110) writes the run-phase files there for one
+ * process, ReadGrid (main/gridio.c:54) reads them back. */
+EXPORT void ref_grid_generate(const char *dir) {
+  snprintf(DATADIR, sizeof DATADIR, "%s", dir);
+  snprintf(DATAFILE, sizeof DATAFILE, "%s/suntans.dat", dir);
+  TRIANGULATE = 0;
+  gridT *grid = NULL;
+  GetGrid(&grid, 0, 1, 0);
+}
+EXPORT gridT *ref_grid_read(const char *dir) {
+  snprintf(DATADIR, sizeof DATADIR, "%s", dir);
+  snprintf(DATAFILE, sizeof DATAFILE, "%s/suntans.dat", dir);
+  gridT *grid = NULL;
+  ReadGrid(&grid, 0, 1, 0);
+  return grid;
+}

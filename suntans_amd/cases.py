@@ -310,3 +310,20 @@ def timestep_inputs(st: State, rng, nonlinear: int = 1, tvd: int = 4) -> State:
     st.set_prop(nu_H=0.1, CdW=5e-3, n=3, AB=3, TVDmomentum=tvd, conserveMomentum=0, beta=1.0, TVDsalt=tvd,
                 kappa_s=1e-5, qprecond=2, qepsilon=1e-10, qmaxiters=2000, resnorm=0)
     return st
+
+
+def from_files(datadir: str, seed: int = 0, dt: float = 5.0, h_amp: float = 0.5, amp_u: float = 0.1) -> State:
+    """A grid from run-phase files (State.load) with a moved free surface,
+    seeded velocities and scalar, and the solver inputs of tiny()."""
+    rng = np.random.default_rng(seed)
+    st = State.load(datadir)
+    st.update_dz(-1)
+    h1 = rng.uniform(-h_amp, 0.3 * h_amp, st.Nc)
+    st.set("h", h1)
+    st.update_dz(1)
+    st.set("h", h1 + rng.uniform(-0.5 * h_amp, 0.5 * h_amp, st.Nc))
+    st.update_dz(0)
+    fill_velocities(st, rng, amp_u=amp_u)
+    fill_scalar(st, rng)
+    _finish(st, rng, dt)
+    return st

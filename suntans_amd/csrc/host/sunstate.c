@@ -2,6 +2,7 @@
  * sunstate.c -- host construction of the state the drop-in solvers consume.
  * See include/sunstate.h for the contract of each function.
  */
+#include <ctype.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
@@ -64,9 +65,6 @@ static void planform_alloc(gridT *g) {
   g->ye = (REAL *)xcalloc(Ne, sizeof(REAL));
 }
 
-/* Edge metrics, face normals and the single-process computational ordering
- * of a planform whose points, cells (nfaces, face, neigh), Voronoi points,
- * areas and edges (end points, grad, mark) are set. */
 /* Computational ordering of a single process (main/grid.c:1652-1679):
  * computational cells first, then the type-3 cells (celldist[1..2)), each
  * group in index order; edges sorted by mark 0 (and 5), 1, 2, 3, 4 into
@@ -103,6 +101,9 @@ int sun_grid_open(gridT *g, const int *mark) {
   return 0;
 }
 
+/* Edge metrics, face normals and the single-process computational ordering
+ * of a planform whose points, cells (nfaces, face, neigh), Voronoi points,
+ * areas and edges (end points, grad, mark) are set. */
 static void planform_finish(gridT *g) {
   const int Nc = g->Nc, Ne = g->Ne, mf = g->maxfaces;
   /* metrics, main/grid.c:3457-3533 conventions */
@@ -399,8 +400,9 @@ int sun_grid_vertical(gridT *g, int Nkmax, const double *dz, const double *dv, i
 struct fdesc { const char *name; int kind; size_t off; };
 /* kinds: 1 cell rows (Nk), 2 w rows (Nk+1), 3 edge rows (Nkc), 4 cell 2-D,
  * 5 edge 2-D, 6 column scratch (Nkmax+1), 8 open-boundary rows (Nke+1, one
- * per edge of edgedist[2..5)), 9 open-boundary rows (Nke) */
-#define ROWS_KIND(k) ((k) <= 3 || (k) >= 8)
+ * per edge of edgedist[2..5)), 9 open-boundary rows (Nke); grid-owned
+ * vectors also use 7 (Nkmax), 10 (Nc*maxfaces) and 11 (Np) */
+#define ROWS_KIND(k) ((k) <= 3 || (k) == 8 || (k) == 9)
 #define PF(f, k) {#f, k, offsetof(physT, f)}
 static const struct fdesc phys_fields[] = {
   PF(uc, 1), PF(vc, 1), PF(wc, 1), PF(uold, 1), PF(vold, 1), PF(q, 1), PF(qc, 1),
@@ -490,6 +492,8 @@ static size_t kind_len(const gridT *g, int kind) {
     case 4: return (size_t)g->Nc;
     case 5: return (size_t)g->Ne;
     case 6: return (size_t)g->Nkmax + 1;
+    case 10: return (size_t)g->Nc * (size_t)g->maxfaces;
+    case 11: return (size_t)g->Np;
     case 8: case 9: {
       size_t n = 0;
       for (int j = g->edgedist[2]; j < g->edgedist[5]; j++) n += (size_t)(g->Nke[g->edgep[j]] + (kind == 8));
@@ -542,7 +546,7 @@ void sun_grid_free(gridT *g) {
   int *i1[] = {g->face, g->edges, g->cells, g->neigh, g->eneigh, g->grad, g->gradf, g->mark,
                g->normal, g->celldist, g->edgedist, g->cellp, g->edgep, g->mnptr, g->eptr,
                g->myneighs, g->Nk, g->ctop, g->ctopold, g->etop, g->etopold, g->Nkp, g->Nke,
-               g->Nkc, g->nfaces};
+               g->Nkc, g->nfaces, g->edge_id};
   for (size_t i = 0; i < sizeof(i1) / sizeof(i1[0]); i++) free(i1[i]);
   rows_free(g->dzz); rows_free(g->dzzold); rows_free(g->dzf);
   free(g);
@@ -714,13 +718,16 @@ static int grid_real(const gridT *g, const char *name, int *kind, REAL ***rows, 
   else if (!strcmp(name, "dzfB")) { *kind = 5; *vec = g->dzfB; }
   else if (!strcmp(name, "dzbot")) { *kind = 4; *vec = g->dzbot; }
   else if (!strcmp(name, "dz")) { *kind = 7; *vec = g->dz; }
+  else if (!strcmp(name, "def")) { *kind = 10; *vec = g->def; }
+  else if (!strcmp(name, "xp")) { *kind = 11; *vec = g->xp; }
+  else if (!strcmp(name, "yp")) { *kind = 11; *vec = g->yp; }
   else return 0;
   return 1;
 }
 
 static long xfer(const gridT *g, int kind, REAL **rows, REAL *vec, double *flat, int to_flat) {
   size_t n = 0;
-  if (kind >= 8) {
+  if (kind == 8 || kind == 9) {
     for (int j = g->edgedist[2]; j < g->edgedist[5]; j++) {
       size_t m = (size_t)(g->Nke[g->edgep[j]] + (kind == 8));
       REAL *row = rows[j - g->edgedist[2]];
@@ -791,6 +798,7 @@ static int *grid_ivec(const gridT *g, const char *name, long *n) {
     {"Nkc", g->Nkc, Ne}, {"etop", g->etop, Ne}, {"etopold", g->etopold, Ne}, {"mark", g->mark, Ne},
     {"edgep", g->edgep, Ne}, {"grad", g->grad, 2 * Ne}, {"gradf", g->gradf, 2 * Ne},
     {"edges", g->edges, 3 * Ne}, {"celldist", g->celldist, 3}, {"edgedist", g->edgedist, 6},
+    {"mnptr", g->mnptr, Nc}, {"eptr", g->eptr, Ne}, {"edge_id", g->edge_id, Ne},
   };
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (!strcmp(t[i].nm, name)) { *n = t[i].n; return t[i].p; }
@@ -896,4 +904,214 @@ int sun_prop_set(propT *prop, const char *name, double value) {
 }
 double sun_prop_get(const propT *prop, const char *name) {
   int ok; return prop_access((propT *)prop, name, 0, 0, &ok);
+}
+
+/* ------------------------------------------------------------------------ */
+/* run-phase grid files                                                      */
+
+/* main/grid.h:24-25, main/suntans.h:21 */
+#define GRID_MAXBCTYPES 4
+#define GRID_MAXMARKS 7
+#define GRID_EDGECOLUMNS 3
+
+/* The value of key in a suntans.dat-style file (main/fileio.c:237-267,
+ * 462-487): the first line whose first word is key; the search stops at
+ * the first empty line, and a value that is the line's last character is
+ * not taken.  Returns 1 with the value's first word in val, else 0. */
+static int dat_string(const char *file, const char *key, char *val, size_t n) {
+  FILE *f = fopen(file, "r");
+  if (!f) return 0;
+  char line[4096];
+  int found = 0;
+  while (fgets(line, sizeof line, f)) {
+    size_t len = strcspn(line, "\n");
+    line[len] = 0;
+    if (len == 0) break;
+    size_t kl = 0;
+    while (kl < len && !isspace((unsigned char)line[kl])) kl++;
+    if (kl != strlen(key) || strncmp(line, key, kl)) continue;
+    size_t i = kl;
+    while (i < len && isspace((unsigned char)line[i])) i++;
+    if (i + 1 == len) break;
+    size_t vl = 0;
+    while (i + vl < len && !isspace((unsigned char)line[i + vl]) && vl + 1 < n) vl++;
+    memcpy(val, line + i, vl);
+    val[vl] = 0;
+    found = 1;
+    break;
+  }
+  fclose(f);
+  return found;
+}
+
+/* A file named by key, relative to the data directory unless absolute
+ * (main/mympi.c:94-102), with suffix appended. */
+static int dat_file(const char *dir, const char *datafile, const char *key, const char *suffix, char *out,
+                    size_t n) {
+  char v[1024];
+  if (!dat_string(datafile, key, v, sizeof v)) return 0;
+  if (v[0] == '/') snprintf(out, n, "%s%s", v, suffix);
+  else snprintf(out, n, "%s/%s%s", dir, v, suffix);
+  return 1;
+}
+
+/* Rows of a text file (main/fileio.c:183-196). */
+static long file_lines(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  long n = 0;
+  int c;
+  while ((c = fgetc(f)) != EOF)
+    if (c == '\n') n++;
+  fclose(f);
+  return n;
+}
+
+/* The next number of a grid file (main/fileio.c:137-174): words split at
+ * space, tab, CR and LF; a word with any character other than digits, '.',
+ * '+', '-' and 'e' is skipped; 0 at the end of the file. */
+static double next_field(FILE *f) {
+  char w[256];
+  for (;;) {
+    int c, i = 0;
+    while ((c = fgetc(f)) == ' ' || c == '\t' || c == '\n' || c == '\r') {}
+    if (c == EOF) return 0;
+    while (c != EOF && c != ' ' && c != '\t' && c != '\n' && c != '\r') {
+      if (i < 255) w[i++] = (char)c;
+      c = fgetc(f);
+    }
+    w[i] = 0;
+    int ok = 1;
+    for (int k = 0; k < i; k++)
+      if (!isdigit((unsigned char)w[k]) && w[k] != '.' && w[k] != '-' && w[k] != '+' && w[k] != 'e') { ok = 0; break; }
+    if (ok) return strtod(w, NULL);
+  }
+}
+
+gridT *sun_grid_load(const char *dir) {
+  char datafile[1024], v[256];
+  snprintf(datafile, sizeof datafile, "%s/suntans.dat", dir);
+  if (!dat_string(datafile, "Nkmax", v, sizeof v)) return NULL;
+  gridT *g = (gridT *)xcalloc(1, sizeof(gridT));
+  g->Nkmax = (int)strtod(v, NULL);
+  g->maxfaces = dat_string(datafile, "maxFaces", v, sizeof v) ? (int)strtod(v, NULL) : 3;
+  const char *keys[5] = {"celldata", "edgedata", "nodes", "topology", "vertspace"};
+  char files[5][1100];
+  for (int i = 0; i < 5; i++)
+    if (!dat_file(dir, datafile, keys[i], i < 4 ? ".0" : "", files[i], sizeof files[i])) { free(g); return NULL; }
+  const long Nc = file_lines(files[0]), Ne = file_lines(files[1]), Np = file_lines(files[2]);
+  if (Nc <= 0 || Ne <= 0 || Np <= 0) { free(g); return NULL; }
+  g->Nc = (int)Nc; g->Ne = (int)Ne; g->Np = (int)Np;
+  const int mf = g->maxfaces;
+
+  /* topology (main/gridio.c:365-425): one process, no neighbours */
+  FILE *f = fopen(files[3], "r");
+  if (!f) { free(g); return NULL; }
+  const int nprocs = (int)next_field(f);
+  g->Nneighs = (int)next_field(f);
+  if (nprocs != 1 || g->Nneighs != 0) { fclose(f); free(g); return NULL; }
+  g->myneighs = (int *)xcalloc(1, sizeof(int));
+  g->celldist = (int *)xcalloc(GRID_MAXBCTYPES - 1, sizeof(int));
+  g->edgedist = (int *)xcalloc(GRID_MAXMARKS - 1, sizeof(int));
+  g->cellp = (int *)xcalloc(Nc, sizeof(int));
+  g->edgep = (int *)xcalloc(Ne, sizeof(int));
+  for (int n = 0; n < GRID_MAXBCTYPES - 1; n++) g->celldist[n] = (int)next_field(f);
+  for (int n = 0; n < GRID_MAXMARKS - 1; n++) g->edgedist[n] = (int)next_field(f);
+  for (int n = 0; n < Nc; n++) g->cellp[n] = (int)next_field(f);
+  for (int n = 0; n < Ne; n++) g->edgep[n] = (int)next_field(f);
+  fclose(f);
+
+  /* cell-centred data (main/gridio.c:631-662) */
+  planform_alloc(g);
+  free(g->eneigh); g->eneigh = NULL;
+  g->Nk = (int *)xcalloc(Nc, sizeof(int));
+  g->mnptr = (int *)xcalloc(Nc, sizeof(int));
+  f = fopen(files[0], "r");
+  for (int n = 0; n < Nc; n++) {
+    g->nfaces[n] = (int)next_field(f);
+    if (g->nfaces[n] > mf || g->nfaces[n] < 0) { fclose(f); sun_grid_free(g); return NULL; }
+    g->xv[n] = next_field(f);
+    g->yv[n] = next_field(f);
+    g->Ac[n] = next_field(f);
+    g->dv[n] = next_field(f);
+    g->Nk[n] = (int)next_field(f);
+    for (int nf = 0; nf < g->nfaces[n]; nf++) g->face[mf * n + nf] = (int)next_field(f);
+    for (int nf = 0; nf < g->nfaces[n]; nf++) g->neigh[mf * n + nf] = (int)next_field(f);
+    for (int nf = 0; nf < g->nfaces[n]; nf++) g->normal[mf * n + nf] = (int)next_field(f);
+    for (int nf = 0; nf < g->nfaces[n]; nf++) g->def[mf * n + nf] = next_field(f);
+    for (int nf = 0; nf < g->nfaces[n]; nf++) g->cells[mf * n + nf] = (int)next_field(f);
+    g->mnptr[n] = (int)next_field(f);
+  }
+  fclose(f);
+
+  /* edge-centred data (main/gridio.c:687-711) */
+  g->Nke = (int *)xcalloc(Ne, sizeof(int));
+  g->Nkc = (int *)xcalloc(Ne, sizeof(int));
+  g->edge_id = (int *)xcalloc(Ne, sizeof(int));
+  g->eptr = (int *)xcalloc(Ne, sizeof(int));
+  f = fopen(files[1], "r");
+  for (int n = 0; n < Ne; n++) {
+    g->df[n] = next_field(f);
+    g->dg[n] = next_field(f);
+    g->n1[n] = next_field(f);
+    g->n2[n] = next_field(f);
+    g->xe[n] = next_field(f);
+    g->ye[n] = next_field(f);
+    g->Nke[n] = (int)next_field(f);
+    g->Nkc[n] = (int)next_field(f);
+    g->grad[2 * n] = (int)next_field(f);
+    g->grad[2 * n + 1] = (int)next_field(f);
+    g->gradf[2 * n] = (int)next_field(f);
+    g->gradf[2 * n + 1] = (int)next_field(f);
+    g->mark[n] = (int)next_field(f);
+    g->edges[GRID_EDGECOLUMNS * n] = (int)next_field(f);
+    g->edges[GRID_EDGECOLUMNS * n + 1] = (int)next_field(f);
+    g->edge_id[n] = (int)next_field(f);
+    g->eptr[n] = (int)next_field(f);
+  }
+  fclose(f);
+
+  /* nodes (main/gridio.c:728-783): coordinates; the neighbour lists and
+   * per-layer areas are skipped (nothing on the path reads them) */
+  g->xp = (REAL *)xcalloc(Np, sizeof(REAL));
+  g->yp = (REAL *)xcalloc(Np, sizeof(REAL));
+  g->Nkp = (int *)xcalloc(Np, sizeof(int));
+  f = fopen(files[2], "r");
+  for (int n = 0; n < Np; n++) {
+    next_field(f);
+    g->xp[n] = next_field(f);
+    g->yp[n] = next_field(f);
+    for (int list = 0; list < 3; list++) {
+      const int m = (int)next_field(f);
+      for (int j = 0; j < m; j++) next_field(f);
+    }
+    g->Nkp[n] = (int)next_field(f);
+    for (int j = 0; j < g->Nkp[n]; j++) next_field(f);
+  }
+  fclose(f);
+
+  /* vertical spacing (main/gridio.c:496-504) and the vertical grid of
+   * InitializeVerticalGrid (main/phys.c:765-799) */
+  g->dz = (REAL *)xcalloc(g->Nkmax, sizeof(REAL));
+  f = fopen(files[4], "r");
+  if (!f) { sun_grid_free(g); return NULL; }
+  for (int k = 0; k < g->Nkmax; k++) g->dz[k] = next_field(f);
+  fclose(f);
+  g->stairstep = dat_string(datafile, "stairstep", v, sizeof v) ? (int)strtod(v, NULL) : 0;
+  g->fixdzz = dat_string(datafile, "fixdzz", v, sizeof v) ? (int)strtod(v, NULL) : 0;
+  g->dzsmall = dat_string(datafile, "dzsmall", v, sizeof v) ? strtod(v, NULL) : 0;
+  g->smoothbot = dat_string(datafile, "smoothbot", v, sizeof v) ? strtod(v, NULL) : 0;
+  g->ctop = (int *)xcalloc(Nc, sizeof(int));
+  g->ctopold = (int *)xcalloc(Nc, sizeof(int));
+  g->etop = (int *)xcalloc(Ne, sizeof(int));
+  g->etopold = (int *)xcalloc(Ne, sizeof(int));
+  g->dztop = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->dzbot = (REAL *)xcalloc(Nc, sizeof(REAL));
+  g->dzfB = (REAL *)xcalloc(Ne, sizeof(REAL));
+  g->dzz = rows_alloc((int)Nc, g->Nk, 0);
+  g->dzzold = rows_alloc((int)Nc, g->Nk, 0);
+  for (int i = 0; i < Nc; i++)
+    for (int k = 0; k < g->Nk[i]; k++) g->dzz[i][k] = g->dzzold[i][k] = g->dz[k];
+  g->dzf = rows_alloc((int)Ne, g->Nkc, 0);
+  return g;
 }

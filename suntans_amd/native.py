@@ -62,6 +62,8 @@ def state_lib() -> C.CDLL:
         L.sun_grid_polygons.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, _ip, _ip, _dp, _dp]
         L.sun_grid_vertical.argtypes = [_vp, C.c_int, _dp, _dp, C.c_int]
         L.sun_grid_open.argtypes = [_vp, _ip]
+        L.sun_grid_load.restype = _vp
+        L.sun_grid_load.argtypes = [C.c_char_p]
         L.sun_phys_alloc.restype = _vp
         L.sun_phys_alloc.argtypes = [_vp]
         L.sun_prop_default.restype = _vp

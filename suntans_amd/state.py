@@ -70,6 +70,40 @@ class State:
         self.prop = L.sun_prop_default()
         self._owns_grid = True
 
+    @classmethod
+    def load(cls, datadir: str) -> "State":
+        """A grid from the run-phase files the reference's GetGrid wrote for
+        one process (sun_grid_load: suntans.dat in datadir names them), with
+        zero physical state and default properties."""
+        L = native.state_lib()
+        _bind(L)
+        st = cls.__new__(cls)
+        st._L = L
+        st.p = st.prop = None
+        st._owns_grid = True
+        st.g = L.sun_grid_load(str(datadir).encode())
+        if not st.g:
+            raise RuntimeError(f"sun_grid_load: no one-process run-phase grid in {datadir}")
+        st.p = L.sun_phys_alloc(st.g)
+        st.prop = L.sun_prop_default()
+        st.planform = "file"
+        xp, yp = st.get("xp"), st.get("yp")
+        st.nx = st.ny = 0
+        st.Lx, st.Ly = float(xp.max() - xp.min()), float(yp.max() - yp.min())
+        return st
+
+    @classmethod
+    def view(cls, gptr: int) -> "State":
+        """Read-only access to the grid arrays of a gridT that another library
+        owns (no physical state)."""
+        L = native.state_lib()
+        _bind(L)
+        st = cls.__new__(cls)
+        st._L = L
+        st.g, st.p, st.prop, st._owns_grid = gptr, None, None, False
+        st.planform = "view"
+        return st
+
     # -- lifetime ---------------------------------------------------------
     def clone(self) -> "State":
         """A second physT and propT on the same grid, copies of this one's."""

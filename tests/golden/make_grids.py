@@ -1,0 +1,81 @@
+"""Write the run-phase grid fixtures from the reference's own example grids.
+
+Run where the reference was compiled from its own sources (oracle/_ref):
+
+    python tests/golden/make_grids.py            # every fixture
+    python tests/golden/make_grids.py tides      # the named ones
+
+For each example the data directory is copied to a scratch directory, the
+reference's GetGrid (main/grid.c:110, one process) writes its run-phase
+files there, and this script keeps celldata.dat.0, edgedata.dat.0,
+topology.dat.0 and vertspace.dat as written, nodes.dat.0 with the point
+coordinates only (each point's neighbour lists and per-layer areas, which
+nothing on the path reads, are dropped), and a suntans.dat with the keys a
+run-phase reader needs.  Data only: the files are the reference's output.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsunref.so")
+EXAMPLES = "/root/reference/examples"
+
+# fixture name -> example data directory
+GRIDS = {
+    "tides": "tides/rundata",    # Monterey Bay: 1534 cells, stair-stepped bathymetry, open boundaries
+    "iwaves": "iwaves/rundata",  # 200 cells on 100 stretched levels
+}
+KEYS = ("Nkmax", "maxFaces", "stairstep", "fixdzz", "dzsmall", "smoothbot")
+
+
+def generate(src: str, dst: str) -> None:
+    """The reference's GetGrid into dst, in a child process (the reference
+    exits the process on a fatal grid error)."""
+    for f in os.listdir(src):
+        if os.path.isfile(os.path.join(src, f)):
+            shutil.copy(os.path.join(src, f), dst)
+    code = ("import ctypes as C, sys; L = C.CDLL(sys.argv[1]); L.ref_set_verbose(0); "
+            "L.ref_grid_generate.argtypes = [C.c_char_p]; L.ref_grid_generate(sys.argv[2].encode())")
+    subprocess.run([sys.executable, "-c", code, REF_SO, dst], check=True, cwd=dst, capture_output=True)
+
+
+def dat_lines(path: str) -> dict:
+    out = {}
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) >= 2 and w[0] in KEYS and w[0] not in out:
+                out[w[0]] = w[1]
+    return out
+
+
+def make(name: str) -> None:
+    dst = os.path.join(HERE, "grids", name)
+    os.makedirs(dst, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        generate(os.path.join(EXAMPLES, GRIDS[name]), tmp)
+        for f in ("celldata.dat.0", "edgedata.dat.0", "topology.dat.0", "vertspace.dat"):
+            shutil.copy(os.path.join(tmp, f), dst)
+        with open(os.path.join(tmp, "nodes.dat.0")) as fi, open(os.path.join(dst, "nodes.dat.0"), "w") as fo:
+            for line in fi:
+                w = line.split()
+                fo.write(f"{w[0]} {w[1]} {w[2]} 0 0 0 0\n")
+        keys = dat_lines(os.path.join(tmp, "suntans.dat"))
+    with open(os.path.join(dst, "suntans.dat"), "w") as f:
+        f.write(f"# run-phase grid of examples/{GRIDS[name]}, written by the reference's GetGrid\n")
+        for k in KEYS:
+            if k in keys:
+                f.write(f"{k}\t\t{keys[k]}\t# from the example\n")
+        for k, v in (("celldata", "celldata.dat"), ("edgedata", "edgedata.dat"), ("nodes", "nodes.dat"),
+                     ("topology", "topology.dat"), ("vertspace", "vertspace.dat")):
+            f.write(f"{k}\t\t{v}\t# run-phase file\n")
+    print(f"{name}: {sorted(os.listdir(dst))}")
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:] or list(GRIDS):
+        make(n)

@@ -12,6 +12,7 @@ import pytest
 
 from suntans_amd import cases, device
 
+import _grids
 import _impls
 
 pytestmark = pytest.mark.gpu
@@ -20,7 +21,7 @@ MAKERS = [
     ("lockexchange", lambda: cases.lock_exchange(seed=7)),
     ("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)),
     ("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7)),
-]
+] + _grids.REAL
 IDS = [m[0] for m in MAKERS]
 
 

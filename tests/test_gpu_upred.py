@@ -13,6 +13,7 @@ import pytest
 
 from suntans_amd import cases
 
+import _grids
 import _impls
 
 pytestmark = pytest.mark.gpu
@@ -24,7 +25,7 @@ MAKERS = [
     ("shallow31", lambda: cases.tiny(31, nx=12, ny=8, Nkmax=6, min_layers=-1.2)),
     ("quadtri", lambda: cases.tiny(22, nx=12, ny=10, Nkmax=9, planform="quad", split_every=3)),
     ("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7)),
-]
+] + _grids.REAL
 IDS = [m[0] for m in MAKERS]
 VARIANTS = {"advection": {}, "linear": dict(nonlinear=0), "implicit-off": dict(thetaM=-1.0, lax=0.0),
             "drying": {}}

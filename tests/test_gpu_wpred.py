@@ -9,6 +9,7 @@ import pytest
 
 from suntans_amd import cases, device
 
+import _grids
 import _impls
 
 pytestmark = pytest.mark.gpu
@@ -24,7 +25,7 @@ MAKERS = [
     ("open_tri", lambda: cases.tiny(3, nx=16, ny=9, Nkmax=10, open_marks=OPEN)),
     ("open_hex", lambda: cases.tiny(25, nx=12, ny=9, Nkmax=9, planform="hex", split_every=4, maxfaces=7,
                                     open_marks=OPEN)),
-]
+] + _grids.REAL
 IDS = [m[0] for m in MAKERS]
 VARIANTS = {"central": dict(nonlinear=2), "upwind": dict(nonlinear=1), "none": dict(nonlinear=0),
             "tvd": dict(nonlinear=5, tvd=4), "tvd1": dict(nonlinear=5, tvd=1),

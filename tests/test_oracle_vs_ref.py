@@ -8,6 +8,7 @@ import pytest
 
 from suntans_amd import cases
 
+import _grids
 import _impls
 
 ref = _impls.reference()
@@ -21,6 +22,8 @@ MAKERS.append(("quad", lambda: cases.tiny(21, nx=6, ny=5, planform="quad")))
 MAKERS.append(("quadtri", lambda: cases.tiny(22, nx=7, ny=5, planform="quad", split_every=3)))
 MAKERS.append(("hex", lambda: cases.tiny(23, nx=6, ny=5, planform="hex")))
 MAKERS.append(("hexquad7", lambda: cases.tiny(24, nx=7, ny=6, planform="hex", split_every=4, maxfaces=7)))
+# real grids of the reference's examples, read from the run-phase files its GetGrid wrote
+MAKERS += _grids.REAL
 IDS = [m[0] for m in MAKERS]
 
 
@@ -86,6 +89,9 @@ def test_operators(name, make):
 @pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
 @pytest.mark.parametrize("tvd", [0, 1, 2, 3, 4])
 def test_update_scalars(name, make, tvd):
+    if name == "mbay" and tvd in (3, 4):
+        pytest.skip("two-layer columns: the limiter's ratio reads two levels past the column in the reference "
+                    "(GetApAm, main/tvd.c:161-204), 0 in the oracle")
     o = _impls.oracle()
     a = make()
     b = a.clone()
@@ -247,7 +253,7 @@ WPRED = [("tiny0", lambda: cases.tiny(0)), ("tiny_wide", lambda: cases.tiny(11, 
                                          open_marks=OPEN)),
          # type-4 (no-slip) edges on a flat bed, where the reference's type-4 term stays inside the rows
          ("open4_flat", lambda: cases.lock_exchange(seed=8, nx=20, ny=5, Nkmax=10,
-                                                    open_marks=cases.open_ends(4, 2)))]
+                                                    open_marks=cases.open_ends(4, 2)))] + _grids.REAL
 WVARIANTS = {"central": dict(nonlinear=2), "upwind": dict(nonlinear=1), "none": dict(nonlinear=0),
              "tvd": dict(nonlinear=5, tvd=4), "tvd1": dict(nonlinear=5, tvd=1),
              "stress": dict(nonlinear=2, stress=True), "first": dict(step=1), "second": dict(step=2),
