@@ -60,6 +60,18 @@ int sun_grid_open(gridT *g, const int *mark);
  * process. */
 gridT *sun_grid_load(const char *dir);
 
+/* Restart records in the reference's binary format (OutputData's store
+ * block and ReadPhysicalVariables, main/physio.c:337-394, 407-495): the step
+ * number, h, then Cn_U, Cn_U2 (Nke per edge), Cn_W, Cn_W2, Cn_R, Cn_T (Nk
+ * per cell), with turbmodel >= 1 Cn_q, Cn_l, qT, lT, then nu_tv, kappa_tv,
+ * u (Nke), w (Nk + 1), q, qc, s, T, s0, in native byte order.  The reader
+ * fills only these arrays (the reference's reader also runs UpdateDZ with
+ * options -1 before and 0 after, and recomputes uc, vc and rho); it returns
+ * -1 for a short or a longer file.  Both return -1 when a needed array is
+ * not allocated. */
+int sun_restart_write(const char *path, const gridT *g, const physT *p, int step, int turbmodel);
+int sun_restart_read(const char *path, const gridT *g, physT *p, int *step, int turbmodel);
+
 /* Vertical grid: dz[Nkmax] reference layer thicknesses and dv[Nc] depths.
  * Sets Nk (main/grid.c:1466-1498), Nke/Nkc (:1530-1563), ctop/etop = 0, and
  * allocates dzz/dzzold (Nk) and dzf (Nkc) rows as main/phys.c:765-799. */

@@ -81,6 +81,13 @@ EXPORT void ref_grid_generate(const char *dir) {
   gridT *grid = NULL;
   GetGrid(&grid, 0, 1, 0);
 }
+/* ReadPhysicalVariables (main/physio.c:407-495) from the restart file path. */
+EXPORT int ref_read_restart(gridT *g, physT *p, propT *pr, const char *path) {
+  pr->StartFID = fopen(path, "rb");
+  if (!pr->StartFID) return -1;
+  ReadPhysicalVariables(g, p, pr, 0, 0);
+  return 0;
+}
 EXPORT gridT *ref_grid_read(const char *dir) {
   snprintf(DATADIR, sizeof DATADIR, "%s", dir);
   snprintf(DATAFILE, sizeof DATAFILE, "%s/suntans.dat", dir);

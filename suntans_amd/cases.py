@@ -327,3 +327,26 @@ def from_files(datadir: str, seed: int = 0, dt: float = 5.0, h_amp: float = 0.5,
     fill_scalar(st, rng)
     _finish(st, rng, dt)
     return st
+
+
+def warm_start(datadir: str, restart: str = None, dt: float = 5.0, step: int = None) -> State:
+    """A simulation restarted from the reference's files: the run-phase grid
+    (State.load) and, when given, a restart record (State.load_restart), with
+    the per-step inputs of the path derived from the restored state as a
+    step would find them: flux heights (SetFluxHeight), the pressure source
+    of the restored velocities taken as both old and new (ComputeQSource,
+    which also sets D = df/dg), the free-surface source Ac h, the previous
+    pressure correction as the warm start."""
+    st = State.load(datadir)
+    st.set_prop(dt=dt)
+    n = st.load_restart(restart) if restart else None
+    if not restart:
+        st.update_dz(-1)
+    st.set_prop(n=(step if step is not None else (n if n is not None else 1)))
+    st.set("utmp2", st.get("u"))
+    st.set("wtmp2", st.get("w"))
+    st.set("wnew", st.get("w"))
+    st.set_flux_height()
+    st.compute_qsource()
+    st.set("htmp", st.get("Ac") * st.get("h"))
+    return st

@@ -1115,3 +1115,68 @@ gridT *sun_grid_load(const char *dir) {
   g->dzf = rows_alloc((int)Ne, g->Nkc, 0);
   return g;
 }
+
+/* ------------------------------------------------------------------------ */
+/* restart files                                                             */
+
+/* The rows of one restart record in file order (main/physio.c:345-387),
+ * with their lengths: 'e' Nke per edge, 'c' Nk per cell, 'w' Nk+1. */
+struct restart_part { REAL **rows; char len; };
+static int restart_parts(const physT *p, int turbmodel, struct restart_part *out) {
+  struct restart_part a[] = {
+    {p->Cn_U, 'e'}, {p->Cn_U2, 'e'}, {p->Cn_W, 'c'}, {p->Cn_W2, 'c'}, {p->Cn_R, 'c'}, {p->Cn_T, 'c'},
+    {p->Cn_q, 'c'}, {p->Cn_l, 'c'}, {p->qT, 'c'}, {p->lT, 'c'},
+    {p->nu_tv, 'c'}, {p->kappa_tv, 'c'}, {p->u, 'e'}, {p->w, 'w'}, {p->q, 'c'}, {p->qc, 'c'},
+    {p->s, 'c'}, {p->T, 'c'}, {p->s0, 'c'}};
+  int n = 0;
+  for (int i = 0; i < (int)(sizeof a / sizeof a[0]); i++) {
+    if (i >= 6 && i < 10 && turbmodel < 1) continue;  /* turbulence quantities only with a model */
+    if (!a[i].rows) return -1;
+    out[n++] = a[i];
+  }
+  return n;
+}
+
+static size_t row_len(const gridT *g, char len, int i) {
+  return len == 'e' ? (size_t)g->Nke[i] : (size_t)g->Nk[i] + (len == 'w');
+}
+
+int sun_restart_write(const char *path, const gridT *g, const physT *p, int step, int turbmodel) {
+  struct restart_part parts[19];
+  const int np = restart_parts(p, turbmodel, parts);
+  if (np < 0) return -1;
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  int ok = fwrite(&step, sizeof(int), 1, f) == 1 && fwrite(p->h, sizeof(REAL), g->Nc, f) == (size_t)g->Nc;
+  for (int k = 0; ok && k < np; k++) {
+    const int nr = parts[k].len == 'e' ? g->Ne : g->Nc;
+    for (int i = 0; ok && i < nr; i++) {
+      const size_t m = row_len(g, parts[k].len, i);
+      ok = fwrite(parts[k].rows[i], sizeof(REAL), m, f) == m;
+    }
+  }
+  return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+int sun_restart_read(const char *path, const gridT *g, physT *p, int *step, int turbmodel) {
+  struct restart_part parts[19];
+  const int np = restart_parts(p, turbmodel, parts);
+  if (np < 0) return -1;
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  int n = 0;
+  int ok = fread(&n, sizeof(int), 1, f) == 1 && fread(p->h, sizeof(REAL), g->Nc, f) == (size_t)g->Nc;
+  for (int k = 0; ok && k < np; k++) {
+    const int nr = parts[k].len == 'e' ? g->Ne : g->Nc;
+    for (int i = 0; ok && i < nr; i++) {
+      const size_t m = row_len(g, parts[k].len, i);
+      ok = fread(parts[k].rows[i], sizeof(REAL), m, f) == m;
+    }
+  }
+  /* the record must end here: a longer file was written for another grid or model */
+  if (ok && fgetc(f) != EOF) ok = 0;
+  fclose(f);
+  if (!ok) return -1;
+  if (step) *step = n;
+  return 0;
+}

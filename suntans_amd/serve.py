@@ -112,6 +112,15 @@ def sim_from_state(sid: int, st, keep_host: bool, device_id: int = 0, resident: 
     return s
 
 
+def sim_from_files(sid: int, datadir: str, restart: str = None, keep_host: bool = False, device_id: int = 0,
+                   dt: float = 5.0) -> Sim:
+    """A simulation warm-started from the reference's own files: the
+    run-phase grid its GetGrid wrote in datadir and a restart record (its
+    store file) holding h, u, w, q, qc, s and the Adams-Bashforth histories
+    (cases.warm_start)."""
+    return sim_from_state(sid, cases.warm_start(datadir, restart, dt=dt), keep_host, device_id)
+
+
 class Server:
     """Continuous-batching loop over a set of simulations on one GPU."""
 

@@ -104,6 +104,22 @@ class State:
         st.planform = "view"
         return st
 
+    # -- restart files (the reference's format, sun_restart_write/_read) ------
+    def save_restart(self, path: str, step: int, turbmodel: int = 0) -> None:
+        if self._L.sun_restart_write(str(path).encode(), self.g, self.p, int(step), int(turbmodel)):
+            raise OSError(f"sun_restart_write {path}")
+
+    def load_restart(self, path: str, turbmodel: int = 0) -> int:
+        """The restart record's fields into this state, with the vertical
+        grid rebuilt from its h as the reference's reader does (UpdateDZ -1
+        before, 0 after); returns the step number."""
+        n = np.zeros(1, np.int32)
+        self.update_dz(-1)
+        if self._L.sun_restart_read(str(path).encode(), self.g, self.p, native.iptr(n), int(turbmodel)):
+            raise OSError(f"sun_restart_read {path}: not a restart record of this grid")
+        self.update_dz(0)
+        return int(n[0])
+
     # -- lifetime ---------------------------------------------------------
     def clone(self) -> "State":
         """A second physT and propT on the same grid, copies of this one's."""

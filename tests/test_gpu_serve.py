@@ -14,6 +14,7 @@ import pytest
 
 from suntans_amd import cases, serve
 
+import _grids
 import _impls
 
 pytestmark = pytest.mark.gpu
@@ -58,9 +59,28 @@ def run(sts, arrivals, **kw):
 
 
 def test_one_step_matches_oracle(gpu_available):
-    o = _impls.oracle()
     st = cases.ensemble_member(5, 6000, 2.5, 24)
     sims = make_sims([st])
+    check_one_step(st, sims)
+
+
+def test_warm_start_from_reference_files(gpu_available, tmp_path):
+    """A simulation restarted from the reference's own files (the Monterey
+    Bay run-phase grid its GetGrid wrote, and a restart record in its store
+    format) serves a step that matches the oracle's from the same files."""
+    st0 = cases.from_files(_grids.path("tides"), seed=3)
+    rec = tmp_path / "store.dat.0"
+    st0.save_restart(rec, 12)
+    sims = [serve.sim_from_files(0, _grids.path("tides"), restart=str(rec))]
+    st = cases.warm_start(_grids.path("tides"), str(rec))
+    assert st.get_prop("n") == 12
+    check_one_step(st, sims)
+
+
+def check_one_step(st, sims):
+    """One served request of simulation 0 against the oracle's free-surface
+    solve, pressure solve and TVD scalar step on the same inputs."""
+    o = _impls.oracle()
     srv = serve.Server(sims)
     done = srv.run([(0.0, 0)])
     s = sims[0]
