@@ -248,7 +248,7 @@ def wpred_leg(st, dg, N, Ek, steps=3):
     return res
 
 
-def step_leg(st, dg, steps=3):
+def step_leg(st, dg, steps=3, mixed=False, inner_tol=1e-4):
     """Consecutive nonhydrostatic time steps on the large grid with every
     field resident, in the reference's order (Solve, main/phys.c:1145-1306):
     SetFluxHeight, UPredictor (free-surface PCG inside), UpdateDZ, Continuity
@@ -256,7 +256,9 @@ def step_leg(st, dg, steps=3):
     ComputeQSource, CGSolveQ (qprecond=2), Corrector, Continuity.  The
     explicit horizontal momentum source (HorizontalSource, host code) is a
     fixed seeded field, and the open-boundary hooks do nothing on the closed
-    grid.  Stage times are host wall clock around synchronised calls."""
+    grid.  Stage times are host wall clock around synchronised calls.  mixed:
+    the pressure solve in the mixed-precision form (same 1e-10 FP64
+    residual)."""
     from suntans_amd import device
     rng = np.random.default_rng(17)
     nw, nflat, ne = st.N + st.Nc, len(st.get("dzz")), st.Ne
@@ -309,14 +311,18 @@ def step_leg(st, dg, steps=3):
         run("scalars", lambda: dg.update_scalars(th, dt, 0.0, 4, step=i + 1))
         run("w_predict", lambda: dg.w_predict(PW))
         run("q_source", lambda: dg.compute_qsource(th, dt, "src"))
-        sq = run("q_solve", lambda: dg.q_solve(maxiters=2000, precond=2, epsilon=1e-10))
+        if mixed:
+            sq = run("q_solve", lambda: dg.q_solve_mixed(epsilon=1e-10, inner_tol=inner_tol))
+        else:
+            sq = run("q_solve", lambda: dg.q_solve(maxiters=2000, precond=2, epsilon=1e-10))
         run("corrector", lambda: dg.corrector(dt, "q", "qp"))
         run("continuity", lambda: dg.continuity(th, "w"))
         rows.append(tm)
         its.append((sh["niters"], sq["niters"]))
     med = {k: float(np.median([r[k] for r in rows[1:]])) for k in stages}
     total = [sum(r.values()) for r in rows[1:]]
-    res = {"step_ms": float(np.median(total)), "steps_per_s": 1e3 / float(np.median(total)),
+    res = {"pressure_solve": "mixed" if mixed else "fp64",
+           "step_ms": float(np.median(total)), "steps_per_s": 1e3 / float(np.median(total)),
            "stages_ms": med, "h_iterations": [x[0] for x in its], "q_iterations": [x[1] for x in its],
            "w_finite": bool(np.isfinite(dg.download("w", nw)).all())}
     log(f"time step: {res}")
@@ -655,6 +661,7 @@ def main():
         out["upredictor"] = upred_leg(st, dg, N, Ek)
         out["wpredictor"] = wpred_leg(st, dg, N, Ek)
         out["time_step"] = step_leg(st, dg)
+        out["time_step_mixed"] = step_leg(st, dg, mixed=True, inner_tol=args.inner_tol)
 
     if args.copy_probe:
         cp_ms = dg.copy_probe(1 << 30, 20)

@@ -40,14 +40,12 @@ static int upred_args(sunhip_grid *g, const sunhip_upred_params *p, UPArgs *out)
   if (!(p->dt > 0) || !(p->theta > 0)) { set_error("UPredictor: dt=%g and theta=%g must be positive", p->dt, p->theta); return -1; }
   struct { const char *name; double **ptr; } f[] = {
       {"u", nullptr}, {"utmp", nullptr}, {"utmp2", nullptr}, {"ue", nullptr}, {"dzf", nullptr},
-      {"up_a", nullptr}, {"up_b", nullptr}, {"up_c", nullptr}, {"up_a0", nullptr}, {"up_b0", nullptr},
-      {"up_c0", nullptr}, {"up_bp", nullptr}, {"up_ep", nullptr}, {"D", nullptr}, {"tau_T", nullptr},
+      {"up_c", nullptr}, {"up_bp", nullptr}, {"up_ep", nullptr}, {"D", nullptr}, {"tau_T", nullptr},
       {"CdT", nullptr}, {"CdB", nullptr}, {"dzz", nullptr}, {"nu_tv", nullptr}, {"nu_lax", nullptr},
       {"w", nullptr}, {"wtmp2", nullptr}, {"s", nullptr}, {"h", nullptr}, {"htmp", nullptr},
       {"active", nullptr}, {"hcorr", nullptr}, {"dv", nullptr}};
   UPArgs A{};
-  double **dst[] = {&A.u, &A.utmp, &A.utmp2, &A.E, (double **)&A.dzf, &A.sa, &A.sb, &A.sc, &A.sa0, &A.sb0,
-                    &A.sc0, &A.sbp, &A.sep, &A.D, (double **)&A.tau_T, (double **)&A.CdT, (double **)&A.CdB,
+  double **dst[] = {&A.u, &A.utmp, &A.utmp2, &A.E, (double **)&A.dzf, &A.sc, &A.sbp, &A.sep, &A.D, (double **)&A.tau_T, (double **)&A.CdT, (double **)&A.CdB,
                     (double **)&A.dzz, (double **)&A.nu_tv, (double **)&A.nu_lax, (double **)&A.w, &A.wtmp2, &A.s,
                     &A.h, &A.htmp, &A.active, &A.hcorr, (double **)&A.dv};
   for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) {

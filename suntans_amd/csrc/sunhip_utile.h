@@ -5,11 +5,12 @@
 // the free-surface source htmp, and after the free-surface solve the drying
 // correction and the velocity update.
 //
-// One lane owns one edge (edge tiles) or one column (cell tiles).  The
-// per-edge arrays the reference keeps in phys->a..d, am, bp, bm are edge
-// fields here ("up_*"), walked by the lane level by level in the reference's
-// loop order; every expression keeps the reference's operation order, so the
-// results are the reference's bit for bit.
+// One lane owns one edge (edge tiles) or one column (cell tiles).  Where the
+// reference fills per-level arrays (phys->a..d, am, bp, bm) in separate
+// passes, the lane forms each level's coefficients in registers in one
+// top-down walk and keeps only the eliminated rows in edge fields ("up_*");
+// every expression keeps the reference's operation order, so the results
+// are the reference's bit for bit.
 #pragma once
 #include "sunhip_qtile.h"
 
@@ -19,7 +20,7 @@ struct UPArgs {
   // edge fields
   double *u, *utmp, *utmp2, *E;
   const double *dzf;
-  double *sa, *sb, *sc, *sa0, *sb0, *sc0, *sbp, *sep;  // per-edge scratch (up_*)
+  double *sc, *sbp, *sep;  // per-edge scratch (up_*): upper diagonal, eliminated diagonal and ones
   // edge 2-D (by edge slot)
   double *D;
   const double *tau_T, *CdT, *CdB;
@@ -77,134 +78,161 @@ __device__ __forceinline__ void upred_edge_tile(const Geo &G, const UPArgs &A, i
   auto C2 = [&](const double *p, int k) { return at32(p, (uint32_t)b2 + (uint32_t)k * 64u); };
   E(A.utmp, et) += 2.0 * dt * A.tau_T[s] / (Z1(et) + Z2(et));
   if (Z1(et) == 0 && Z2(et) == 0) { nan_check(); return; }
-  for (int k = et; k < ne; k++) { E(A.sa, k) = 0; E(A.sb, k) = 0; E(A.sc, k) = 0; }
-  for (int k = et + 1; k < ne; k++)
-    E(A.sc, k) = 0.25 * (C1(A.nu_tv, k - 1) + C2(A.nu_tv, k - 1) + C1(A.nu_tv, k) + C2(A.nu_tv, k) +
-                         lw * (C1(A.nu_lax, k - 1) + C2(A.nu_lax, k - 1) + C1(A.nu_lax, k) + C2(A.nu_lax, k)));
-  for (int k = et + 1; k < ne; k++)
-    E(A.sa, k) = 2.0 * (nu + E(A.sc, k)) / (0.25 * (Z1(k) + Z2(k)) * (Z1(k - 1) + Z2(k - 1) + Z1(k) + Z2(k)));
-  for (int k = et; k < ne - 1; k++)
-    E(A.sb, k) = 2.0 * (nu + E(A.sc, k + 1)) / (0.25 * (Z1(k) + Z2(k)) * (Z1(k) + Z2(k) + Z1(k + 1) + Z2(k + 1)));
+  // The reference builds the edge's vertical system in passes over level
+  // arrays (main/phys.c:2684-2920): viscosity coefficients, vertical
+  // advection coefficients, the explicit part into U***, the implicit
+  // matrix (with wall drag at the top and bottom), then TriSolve for U***
+  // and for E = A^-1 1.  Every coefficient of level k depends only on levels
+  // k-1..k+1, so here one top-down walk forms level k's final coefficients
+  // and right-hand sides with the same expressions in the same order and
+  // eliminates it at once; a bottom-up walk substitutes; a last top-down walk
+  // sums D = dzf^T E in the reference's order.
   const bool multi = ne - et > 1, adv = A.nonlinear && tM >= 0 && multi;
-  if (adv) {
-    // w at the faces between U levels, from the cell whose top is lower
-    // (n1) and the other (n0); zero above n1's top and at the bottom
-    const int hi1 = G.ctop[s1] > G.ctop[s2];
-    const int bl = hi1 ? b2 : b1, bh = hi1 ? b1 : b2, kth = hi1 ? G.ctop[s1] : G.ctop[s2];
-    auto DW = [&](int k) -> double {
-      if (k >= ne || k < kth) return 0.0;
-      return 0.5 * (at32(A.w, (uint32_t)bl + (uint32_t)k * 64u) + at32(A.w, (uint32_t)bh + (uint32_t)k * 64u));
-    };
-    double dk = DW(et);
-    for (int k = et; k < ne; k++) {
+  const double CdT = A.CdT[s], CdB = A.CdB[s];
+  const int kb = ne - 1;
+  // w at the faces between U levels, from the cell whose top is lower (n1)
+  // and the other (n0); zero above n1's top and at the bottom
+  const int hi1 = G.ctop[s1] > G.ctop[s2];
+  const int bl = hi1 ? b2 : b1, bh = hi1 ? b1 : b2, kth = hi1 ? G.ctop[s1] : G.ctop[s2];
+  auto DW = [&](int k) -> double {
+    if (k >= ne || k < kth) return 0.0;
+    return 0.5 * (at32(A.w, (uint32_t)bl + (uint32_t)k * 64u) + at32(A.w, (uint32_t)bh + (uint32_t)k * 64u));
+  };
+  // mean eddy and Lax-Wendroff viscosity at the face above level k (k > et)
+  auto VIS = [&](int k) {
+    return 0.25 * (C1(A.nu_tv, k - 1) + C2(A.nu_tv, k - 1) + C1(A.nu_tv, k) + C2(A.nu_tv, k) +
+                   lw * (C1(A.nu_lax, k - 1) + C2(A.nu_lax, k - 1) + C1(A.nu_lax, k) + C2(A.nu_lax, k)));
+  };
+  const double f1 = dt * (1 - th), fm = dt * (1 - tM), ti = th * dt, ta = dt * tM;
+  bool dry = false;
+  // sliding window: the two cells' heights and u at k-1, k, k+1 and the
+  // viscosity at the faces above k and k+1
+  double h1m = 0, h2m = 0, h1k = Z1(et), h2k = Z2(et);
+  double h1p = multi ? Z1(et + 1) : 0.0, h2p = multi ? Z2(et + 1) : 0.0;
+  double um = 0, uk = E(A.u, et), up = multi ? E(A.u, et + 1) : 0.0;
+  double sck = 0, scp = multi ? VIS(et + 1) : 0.0;
+  double dk = adv ? DW(et) : 0.0;
+  double sb_above = 0, btop = 0;  // level k-1's coefficient below it; the top row's diagonal
+  double bprev = 0, dprev = 0, eprev = 0, cprev = 0;
+  for (int k = et; k < ne; k++) {
+    if (h1k == 0 && h2k == 0) dry = true;
+    const double hs = h1k + h2k;
+    // viscosity coefficients above (sa) and below (sb) level k
+    const double sa = k > et ? 2.0 * (nu + sck) / (0.25 * hs * (h1m + h2m + h1k + h2k)) : 0.0;
+    const double sb = k < kb ? 2.0 * (nu + scp) / (0.25 * hs * (h1k + h2k + h1p + h2p)) : 0.0;
+    // vertical advection coefficients (thetaM)
+    double sa0 = 0, sb0 = 0, sc0 = 0;
+    if (adv) {
       const double dk1 = DW(k + 1);
-      const double hz = 0.5 * (Z1(k) + Z2(k));
-      E(A.sa0, k) = (alpha * 0.5 * (dk - fabs(dk)) + 0.5 * (1 - alpha) * dk) / hz;
-      E(A.sb0, k) = (alpha * 0.5 * (dk + fabs(dk) - dk1 + fabs(dk1)) + 0.5 * (1 - alpha) * (dk - dk1)) / hz;
-      E(A.sc0, k) = -(alpha * 0.5 * (dk1 + fabs(dk1)) + 0.5 * (1 - alpha) * dk1) / hz;
+      const double hz = 0.5 * hs;
+      sa0 = (alpha * 0.5 * (dk - fabs(dk)) + 0.5 * (1 - alpha) * dk) / hz;
+      sb0 = (alpha * 0.5 * (dk + fabs(dk) - dk1 + fabs(dk1)) + 0.5 * (1 - alpha) * (dk - dk1)) / hz;
+      sc0 = -(alpha * 0.5 * (dk1 + fabs(dk1)) + 0.5 * (1 - alpha) * dk1) / hz;
       dk = dk1;
     }
+    // explicit viscosity and drag, then explicit vertical advection, into U***
+    double rhs = E(A.utmp, k);
+    if (multi) {
+      if (k == et) {
+        if (CdT == -1) rhs += f1 * (sa * -uk - (sa + sb) * uk + sb * up);
+        else rhs += f1 * (-(sb + 2.0 * CdT * fabs(uk) / hs) * uk + sb * up);
+      } else if (k == kb) {
+        if (CdB == -1) rhs += f1 * (sa * um - (sa + sb) * uk + sb * -uk);
+        else rhs += f1 * (sa * um - (sa + 2.0 * CdB * fabs(uk) / hs) * uk);
+      } else {
+        rhs += f1 * (sa * um - (sa + sb) * uk + sb * up);
+      }
+    } else {
+      // the viscosity coefficient at the top of a one-layer edge is 0
+      const double cv = 0.0;
+      if (CdB == -1) rhs -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + cv)) * uk / (hs * hs));
+      else rhs -= 2.0 * dt * (1 - th) * (CdB) / hs * fabs(uk) * uk;
+      if (CdT == -1) rhs -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + cv)) * uk / (hs * hs));
+      else rhs -= 2.0 * dt * (1 - th) * (CdT) / hs * fabs(uk) * uk;
+    }
+    if (adv) {
+      if (k == et) rhs -= fm * ((sa0 + sb0) * uk + sc0 * up);
+      else if (k == kb) rhs -= fm * (sa0 * um + (sb0 + sc0) * uk);
+      else rhs -= fm * (sa0 * um + sb0 * uk + sc0 * up);
+    }
+    // the implicit row: a (below the diagonal), b (diagonal), c (above)
+    double a = 0, b, c = 0;
+    if (multi) {
+      if (k == et) {
+        c = -th * dt * sb;
+        if (CdT == -1) {
+          // the coefficient above level et + 1
+          const double sa_next = 2.0 * (nu + scp) / (0.25 * (h1p + h2p) * (h1k + h2k + h1p + h2p));
+          b = 1.0 + ti * (sa + sa_next + sb);
+        } else {
+          b = 1.0 + ti * (sb + 2.0 * CdT * fabs(uk) / hs);
+        }
+        btop = b;
+      } else if (k == kb) {
+        // the reference reads level kb-1's coefficient after the top row
+        // was formed, so on a two-layer edge it is the top diagonal
+        const double sbm = kb - 1 == et ? btop : sb_above;
+        if (CdB == -1) b = 1.0 + ti * (sa + sb + sbm);
+        else b = 1.0 + ti * (sa + 2.0 * CdB * fabs(uk) / hs);
+        a = -th * dt * sa;
+      } else {
+        c = -th * dt * sb;
+        b = 1.0 + ti * (sa + sb);
+        a = -th * dt * sa;
+      }
+      if (adv) {
+        if (k == et) {
+          b += ta * (sa0 + sb0);
+          c += ta * sc0;
+        } else if (k == kb) {
+          a += ta * sa0;
+          b += ta * (sb0 + sc0);
+        } else {
+          a += ta * sa0;
+          b += ta * sb0;
+          c += ta * sc0;
+        }
+      }
+    } else {
+      const double cv = 0.0;
+      b = 1.0;
+      if (CdB == -1) b += 4.0 * th * dt * 2.0 * (nu + cv) / (hs * hs);
+      else b += 2.0 * th * dt * fabs(rhs) / hs * (CdB);
+      if (CdT == -1) b += 4.0 * th * dt * 2.0 * (nu + cv) / (hs * hs);
+      else b += 2.0 * th * dt * fabs(rhs) / hs * (CdT);
+    }
+    // elimination; the reference's two TriSolve calls (main/util.c:185-203)
+    // share the matrix, so one sweep carries U*** and the ones of E
+    double bk = b, dd = rhs, ek = 1.0;
+    if (k > et) {
+      bk = b - a * cprev / bprev;
+      dd = rhs - a * dprev / bprev;
+      ek = 1.0 - a * eprev / bprev;
+    }
+    E(A.sbp, k) = bk;
+    E(A.sep, k) = ek;
+    E(A.sc, k) = c;
+    E(A.utmp, k) = dd;
+    bprev = bk; dprev = dd; eprev = ek; cprev = c;
+    sb_above = sb;
+    // slide the window down one level
+    h1m = h1k; h2m = h2k; h1k = h1p; h2k = h2p;
+    um = uk; uk = up;
+    sck = scp;
+    if (k + 2 < ne) { h1p = Z1(k + 2); h2p = Z2(k + 2); up = E(A.u, k + 2); scp = VIS(k + 2); }
+    else { h1p = 0; h2p = 0; up = 0; scp = 0; }
   }
-  const double CdT = A.CdT[s], CdB = A.CdB[s];
-  if (multi) {
-    for (int k = et + 1; k < ne - 1; k++) {
-      const double a = E(A.sa, k), b = E(A.sb, k);
-      E(A.utmp, k) += dt * (1 - th) * (a * E(A.u, k - 1) - (a + b) * E(A.u, k) + b * E(A.u, k + 1));
-    }
-    {
-      const double a = E(A.sa, et), b = E(A.sb, et), u0 = E(A.u, et), u1 = E(A.u, et + 1);
-      if (CdT == -1) E(A.utmp, et) += dt * (1 - th) * (a * -u0 - (a + b) * u0 + b * u1);
-      else E(A.utmp, et) += dt * (1 - th) * (-(b + 2.0 * CdT * fabs(u0) / (Z1(et) + Z2(et))) * u0 + b * u1);
-    }
-    {
-      const int kb = ne - 1;
-      const double a = E(A.sa, kb), b = E(A.sb, kb), um = E(A.u, kb - 1), u0 = E(A.u, kb);
-      if (CdB == -1) E(A.utmp, kb) += dt * (1 - th) * (a * um - (a + b) * u0 + b * -u0);
-      else E(A.utmp, kb) += dt * (1 - th) * (a * um - (a + 2.0 * CdB * fabs(u0) / (Z1(kb) + Z2(kb))) * u0);
-    }
-  } else {
-    // the viscosity coefficient at the top of a one-layer edge is 0
-    const double sz = Z1(et) + Z2(et), u0 = E(A.u, et), cv = E(A.sc, et);
-    if (CdB == -1) E(A.utmp, et) -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + cv)) * u0 / (sz * sz));
-    else E(A.utmp, et) -= 2.0 * dt * (1 - th) * (CdB) / sz * fabs(u0) * u0;
-    if (CdT == -1) E(A.utmp, et) -= 2.0 * dt * (1 - th) * (2.0 * (2.0 * (nu + cv)) * u0 / (sz * sz));
-    else E(A.utmp, et) -= 2.0 * dt * (1 - th) * (CdT) / sz * fabs(u0) * u0;
-  }
-  if (adv) {
-    for (int k = et + 1; k < ne - 1; k++)
-      E(A.utmp, k) -= dt * (1 - tM) *
-                      (E(A.sa0, k) * E(A.u, k - 1) + E(A.sb0, k) * E(A.u, k) + E(A.sc0, k) * E(A.u, k + 1));
-    E(A.utmp, et) -= dt * (1 - tM) * ((E(A.sa0, et) + E(A.sb0, et)) * E(A.u, et) + E(A.sc0, et) * E(A.u, et + 1));
-    const int kb = ne - 1;
-    E(A.utmp, kb) -= dt * (1 - tM) * (E(A.sa0, kb) * E(A.u, kb - 1) + (E(A.sb0, kb) + E(A.sc0, kb)) * E(A.u, kb));
-  }
-  if (multi) {
-    const int kb = ne - 1;
-    E(A.sc, et) = -th * dt * E(A.sb, et);
-    if (CdT == -1) E(A.sb, et) = 1.0 + th * dt * (E(A.sa, et) + E(A.sa, et + 1) + E(A.sb, et));
-    else E(A.sb, et) = 1.0 + th * dt * (E(A.sb, et) + 2.0 * CdT * fabs(E(A.u, et)) / (Z1(et) + Z2(et)));
-    E(A.sa, et) = 0;
-    E(A.sc, kb) = 0;
-    if (CdB == -1) E(A.sb, kb) = 1.0 + th * dt * (E(A.sa, kb) + E(A.sb, kb) + E(A.sb, kb - 1));
-    else E(A.sb, kb) = 1.0 + th * dt * (E(A.sa, kb) + 2.0 * CdB * fabs(E(A.u, kb)) / (Z1(kb) + Z2(kb)));
-    E(A.sa, kb) = -th * dt * E(A.sa, kb);
-    for (int k = et + 1; k < kb; k++) {
-      E(A.sc, k) = -th * dt * E(A.sb, k);
-      E(A.sb, k) = 1.0 + th * dt * (E(A.sa, k) + E(A.sb, k));
-      E(A.sa, k) = -th * dt * E(A.sa, k);
-    }
-  } else {
-    const double sz = Z1(et) + Z2(et), cv = E(A.sc, et);
-    double b = 1.0;
-    if (CdB == -1) b += 4.0 * th * dt * 2.0 * (nu + cv) / (sz * sz);
-    else b += 2.0 * th * dt * fabs(E(A.utmp, et)) / sz * (CdB);
-    if (CdT == -1) b += 4.0 * th * dt * 2.0 * (nu + cv) / (sz * sz);
-    else b += 2.0 * th * dt * fabs(E(A.utmp, et)) / sz * (CdT);
-    E(A.sb, et) = b;
-  }
-  if (adv) {
-    const int kb = ne - 1;
-    for (int k = et + 1; k < kb; k++) {
-      E(A.sa, k) += dt * tM * E(A.sa0, k);
-      E(A.sb, k) += dt * tM * E(A.sb0, k);
-      E(A.sc, k) += dt * tM * E(A.sc0, k);
-    }
-    E(A.sb, et) += dt * tM * (E(A.sa0, et) + E(A.sb0, et));
-    E(A.sc, et) += dt * tM * E(A.sc0, et);
-    E(A.sa, kb) += dt * tM * E(A.sa0, kb);
-    E(A.sb, kb) += dt * tM * (E(A.sb0, kb) + E(A.sc0, kb));
-  }
-  for (int k = et; k < ne; k++)
-    if (Z1(k) == 0 && Z2(k) == 0) { A.err[0] = 1; return; }
-  if (multi) {
-    // the reference's two TriSolve calls (main/util.c:185-203) share the
-    // matrix: one forward sweep carries both right-hand sides
-    double bprev = E(A.sb, et), dprev = E(A.utmp, et), eprev = 1.0, cprev = E(A.sc, et);
-    E(A.sbp, et) = bprev;
-    E(A.sep, et) = eprev;
-    for (int k = et + 1; k < ne; k++) {
-      const double a = E(A.sa, k);
-      const double bk = E(A.sb, k) - a * cprev / bprev;
-      const double dk = E(A.utmp, k) - a * dprev / bprev;
-      const double ek = 1.0 - a * eprev / bprev;
-      E(A.sbp, k) = bk;
-      E(A.utmp, k) = dk;
-      E(A.sep, k) = ek;
-      bprev = bk; dprev = dk; eprev = ek; cprev = E(A.sc, k);
-    }
-    double un = dprev / bprev, en = eprev / bprev;
-    E(A.utmp, ne - 1) = un;
-    E(A.E, ne - 1) = en;
-    for (int k = ne - 2; k >= et; k--) {
-      const double bk = E(A.sbp, k), ck = E(A.sc, k);
-      un = E(A.utmp, k) / bk - ck * un / bk;
-      en = E(A.sep, k) / bk - ck * en / bk;
-      E(A.utmp, k) = un;
-      E(A.E, k) = en;
-    }
-  } else {
-    E(A.utmp, et) /= E(A.sb, et);
-    E(A.E, et) = 1.0 / E(A.sb, et);
+  if (dry) { A.err[0] = 1; return; }
+  // substitution, bottom up
+  double un = dprev / bprev, en = eprev / bprev;
+  E(A.utmp, kb) = un;
+  E(A.E, kb) = en;
+  for (int k = kb - 1; k >= et; k--) {
+    const double bk = E(A.sbp, k), ck = E(A.sc, k);
+    un = E(A.utmp, k) / bk - ck * un / bk;
+    en = E(A.sep, k) / bk - ck * en / bk;
+    E(A.utmp, k) = un;
+    E(A.E, k) = en;
   }
   double Dj = 0;
   for (int k = et; k < ne; k++) Dj += E(A.E, k) * at32(A.dzf, base + (uint32_t)k * 64u);
