@@ -1027,6 +1027,7 @@ gridT *sun_grid_load(const char *dir) {
   g->Nk = (int *)xcalloc(Nc, sizeof(int));
   g->mnptr = (int *)xcalloc(Nc, sizeof(int));
   f = fopen(files[0], "r");
+  if (!f) { sun_grid_free(g); return NULL; }
   for (int n = 0; n < Nc; n++) {
     g->nfaces[n] = (int)next_field(f);
     if (g->nfaces[n] > mf || g->nfaces[n] < 0) { fclose(f); sun_grid_free(g); return NULL; }
@@ -1050,6 +1051,7 @@ gridT *sun_grid_load(const char *dir) {
   g->edge_id = (int *)xcalloc(Ne, sizeof(int));
   g->eptr = (int *)xcalloc(Ne, sizeof(int));
   f = fopen(files[1], "r");
+  if (!f) { sun_grid_free(g); return NULL; }
   for (int n = 0; n < Ne; n++) {
     g->df[n] = next_field(f);
     g->dg[n] = next_field(f);
@@ -1077,6 +1079,7 @@ gridT *sun_grid_load(const char *dir) {
   g->yp = (REAL *)xcalloc(Np, sizeof(REAL));
   g->Nkp = (int *)xcalloc(Np, sizeof(int));
   f = fopen(files[2], "r");
+  if (!f) { sun_grid_free(g); return NULL; }
   for (int n = 0; n < Np; n++) {
     next_field(f);
     g->xp[n] = next_field(f);
