@@ -1104,9 +1104,10 @@ void oracle_MomentumFaceValues(REAL **uf, REAL **ui, REAL **bui, REAL **U, gridT
  * Reads outside rows as the reference makes them: with conserveMomentum the
  * top-cell fill reads a[k] above ctop left over from the previous column
  * (the same scratch here, so the result matches); scheme 4 reads dzz one
- * past the column bottom (read as 0 here); the type-4 term runs to
- * Nke[cell] of the inner cell's index and reads its boundary row past
- * Nke+1 (read as 0 here). */
+ * past the column bottom (read as 0 here: parity unpinned for scheme 4's
+ * vertical Lax-Wendroff term, which the device refuses); the type-4 term
+ * runs to Nke[cell] of the inner cell's index and reads its boundary row
+ * past Nke+1 (read as 0 here; pinned where it stays inside the row). */
 void oracle_WPredictor(gridT *g, physT *p, propT *prop) {
   REAL *a = p->a, *b = p->b, *c = p->c;
   const int mf = g->maxfaces, scheme = prop->nonlinear;
