@@ -100,3 +100,36 @@ def test_resident_time_step(gpu_available, name, make, nonlinear):
         assert np.isfinite(got).all(), dev_name
         assert relerr(got, want) <= 1e-9, (dev_name, relerr(got, want))
     dg.close()
+
+
+@pytest.mark.parametrize("name,make", [MAKERS[0]] + _grids.REAL[:1], ids=[IDS[0], _grids.REAL[0][0]])
+def test_resident_time_steps(gpu_available, name, make):
+    """Three consecutive steps (Adams-Bashforth steps 3, 4, 5) with the state
+    left on the device between them, against three oracle steps: the
+    solves' rounding differences stay within 1e-9 and never flip the
+    discrete state."""
+    o = _impls.oracle()
+    a = make()
+    cases.timestep_inputs(a, np.random.default_rng(31), nonlinear=2)
+    source = a.get("utmp")
+    dg = device.DeviceGrid(a)
+    upload_state(dg, a)
+    for step in (3, 4, 5):
+        a.set_prop(n=step)
+        a.set("utmp", source)
+        dg.upload("utmp", source)
+        sh, sq = device_step(dg, a)
+        oracle_step(o, a)
+        assert abs(sh["niters"] - o.stats("h")[0]) <= 1 and abs(sq["niters"] - o.stats("q")[0]) <= 1, step
+        ext = dg.extents()
+        for f in ("ctop", "etop"):
+            assert np.array_equal(ext[f], a.ints(f)), (step, f)
+    m = a.wet_mask()
+    for dev_name, st_name, wet in (("u", "u", a.edge_wet_mask()), ("w", "w", None), ("s", "s", m),
+                                   ("qp", "q", m), ("h", "h", None)):
+        want = a.get(st_name)
+        got = dg.download(dev_name, len(want))
+        if wet is not None:
+            want, got = want[wet], got[wet]
+        assert relerr(got, want) <= 1e-9, (dev_name, relerr(got, want))
+    dg.close()
