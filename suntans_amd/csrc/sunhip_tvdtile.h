@@ -145,32 +145,33 @@ struct Col {
 
 // ap[k], am[k] of GetApAm (main/tvd.c:161-204) for one k in [ktop+1, Nk]:
 // the final values of rp/rm after the reference's sequence of assignments.
-__device__ void apam(const double *w, const Col &dz, const Col &sc, int64_t base, int nk, int ktop, int k,
-                     double dt, int TVD, double &ap, double &am) {
-  const double wk = w[base + (int64_t)k * 64];
+// Takes the values it reads: w at k, the layer heights at k and k-1, and the
+// scalar at k-2..k+1 (0 outside the column).
+__device__ __forceinline__ void apam(double wk, double dzk, double dzk1, double sm2, double sm1, double s0, double sp1,
+                                     int nk, int ktop, int k, double dt, int TVD, double &ap, double &am) {
   const double wp = 0.5 * (wk + fabs(wk)), wm = 0.5 * (wk - fabs(wk));
   double Cp, Cm;
   if (k < nk) {
-    const double den = dz.at(k) + dz.at(k - 1);
+    const double den = dzk + dzk1;
     Cp = 2 * wp * dt / den;
     Cm = 2 * wm * dt / den;
   } else {
-    Cp = wp * dt / dz.at(nk - 1);
-    Cm = wm * dt / dz.at(nk - 1);
+    Cp = wp * dt / dzk1;
+    Cm = wm * dt / dzk1;
   }
   double rp, rm;
   if (k == nk) {
     rp = 1;
-    rm = (sc.at(nk - 2) - sc.at(nk - 1) + TVD_EPS) / TVD_EPS;
+    rm = (sm2 - sm1 + TVD_EPS) / TVD_EPS;
   } else if (k == nk - 1) {
-    rp = TVD_EPS / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-    rm = (sc.at(k - 2) - sc.at(k - 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
+    rp = TVD_EPS / (sm1 - s0 + TVD_EPS);
+    rm = (sm2 - sm1 + TVD_EPS) / (sm1 - s0 + TVD_EPS);
   } else if (k == ktop + 1) {
-    rp = (sc.at(ktop + 1) - sc.at(ktop + 2) + TVD_EPS) / (sc.at(ktop) - sc.at(ktop + 1) + TVD_EPS);
-    rm = TVD_EPS / (sc.at(ktop) - sc.at(ktop + 1) + TVD_EPS);
+    rp = (s0 - sp1 + TVD_EPS) / (sm1 - s0 + TVD_EPS);
+    rm = TVD_EPS / (sm1 - s0 + TVD_EPS);
   } else {
-    rp = (sc.at(k) - sc.at(k + 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
-    rm = (sc.at(k - 2) - sc.at(k - 1) + TVD_EPS) / (sc.at(k - 1) - sc.at(k) + TVD_EPS);
+    rp = (s0 - sp1 + TVD_EPS) / (sm1 - s0 + TVD_EPS);
+    rm = (sm2 - sm1 + TVD_EPS) / (sm1 - s0 + TVD_EPS);
   }
   am = 0.5 * wp * psi(rp, TVD) * (1 - Cp) + wm * (1 - 0.5 * psi(rm, TVD) * (1 + Cm));
   ap = wp * (1 - 0.5 * psi(rp, TVD) * (1 - Cp)) + 0.5 * wm * psi(rm, TVD) * (1 + Cm);
@@ -264,18 +265,24 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   }
   const int nb = nk - 1 - ktop;
   const bool tvdv = A.TVD != 0;
-  auto ap_am_new = [&](int k, double &ap, double &am) {
-    if (tvdv) apam(A.wnew, dzz, so, base, nk, ktop, k, dt, A.TVD, ap, am);
-    else apam_plain(A.wnew, base, k, ap, am);
+  // level windows, positioned for level k of the walk below: the old scalar
+  // at k-1..k+2, the new and old layer heights and kappa_tv at k and k+1
+  // (0 outside the column), so that every value is loaded once
+  double s0 = so.at(ktop - 1), s1 = so.at(ktop), s2 = so.at(ktop + 1), s3 = so.at(ktop + 2);
+  double z0 = dzz.at(ktop), z1 = dzz.at(ktop + 1), o0 = dzo.at(ktop), o1 = dzo.at(ktop + 1);
+  auto kap = [&](int k) { return k < nk ? A.kappa_tv[base + (int64_t)k * 64] : 0.0; };
+  double q0 = kap(ktop), q1 = kap(ktop + 1);
+  // ap, am at level k+1 (its top face) from the windows at level k
+  auto ap_am_new = [&](int k1, double &ap, double &am) {
+    if (tvdv) apam(A.wnew[base + (int64_t)k1 * 64], z1, z0, s0, s1, s2, s3, nk, ktop, k1, dt, A.TVD, ap, am);
+    else apam_plain(A.wnew, base, k1, ap, am);
   };
-  auto ap_am_old = [&](int k, double &ap, double &am) {
-    if (tvdv) apam(A.wtmp2, dzo, so, base, nk, ktop, k, dt, A.TVD, ap, am);
-    else apam_plain(A.wtmp2, base, k, ap, am);
+  auto ap_am_old = [&](int k1, double &ap, double &am) {
+    if (tvdv) apam(A.wtmp2[base + (int64_t)k1 * 64], o1, o0, s0, s1, s2, s3, nk, ktop, k1, dt, A.TVD, ap, am);
+    else apam_plain(A.wtmp2, base, k1, ap, am);
   };
-  auto bd = [&](int k) {
-    return (2.0 * A.kappa + A.kappa_tv[base + (int64_t)(k - 1) * 64] + A.kappa_tv[base + (int64_t)k * 64]) /
-           (dzz.at(k - 1) + dzz.at(k));
-  };
+  // vertical diffusion coefficient of the face above level k+1
+  auto bd = [&]() { return (2.0 * A.kappa + q0 + q1) / (z0 + z1); };
   const double td = theta * dt, otd = (1 - theta) * dt;
   // horizontal source of the top row: levels 0..ktop fold into row 0
   double cn0 = 0;
@@ -287,7 +294,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   double apN, amN, apO, amO;  // at k+1 (new and old velocities)
   ap_am_new(ktop + 1, apN, amN);
   ap_am_old(ktop + 1, apO, amO);
-  double bd_k1 = (ktop + 1 < nk) ? bd(ktop + 1) : 0.0;  // bd at k+1
+  double bd_k1 = (ktop + 1 < nk) ? bd() : 0.0;  // bd at k+1
   double apN_k = 0, amN_k = 0, apO_k = 0, amO_k = 0, bd_k = 0;
   for (int m = 0; m <= nb; m++) {
     const int k = ktop + m;
@@ -296,10 +303,10 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
       apN_k = apN; amN_k = amN; apO_k = apO; amO_k = amO; bd_k = bd_k1;
       ap_am_new(k + 1, apN, amN);
       ap_am_old(k + 1, apO, amO);
-      bd_k1 = (k + 1 < nk) ? bd(k + 1) : 0.0;
+      bd_k1 = (k + 1 < nk) ? bd() : 0.0;
     }
     double a, b, c, d;
-    const double sk = so.at(k);
+    const double sk = s1, skm = s0, skp = s2;
     if (m == 0) {
       a = 0;
       b = dznew - td * amN;
@@ -313,30 +320,28 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
         d = 0;
         for (int kk = cto; kk <= ct; kk++) d += dzo.at(kk) * so.at(kk);
       } else {
-        d = dzo.at(ktop) * so.at(ktop);
+        d = o0 * sk;
       }
       if (ktop < nk - 1) {
-        d = d - otd * (-amO * sk - apO * so.at(k + 1)) +
-            otd * (-(2 * A.alpha_top * bd_k1 + bd_k1) * sk + bd_k1 * so.at(k + 1));
+        d = d - otd * (-amO * sk - apO * skp) + otd * (-(2 * A.alpha_top * bd_k1 + bd_k1) * sk + bd_k1 * skp);
       }
       d += cn0;
     } else {
       a = td * amN_k;
-      b = dzz.at(k) + td * (apN_k - amN);
+      b = z0 + td * (apN_k - amN);
       c = -td * apN;
-      d = dzo.at(k) * sk;
+      d = o0 * sk;
       if (m == nb) {
         b += c;
         a -= td * bd_k;
         b += td * (bd_k + 2 * A.alpha_bot * bd_k);
-        d -= otd * (amO_k * so.at(k - 1) + apO_k * sk) -
-             otd * (bd_k * so.at(k - 1) - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
+        d -= otd * (amO_k * skm + apO_k * sk) - otd * (bd_k * skm - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
       } else {
         a -= td * bd_k;
         b += td * (bd_k + bd_k1);
         c -= td * bd_k1;
-        d -= otd * (amO_k * so.at(k - 1) + (apO_k - amO) * sk - apO * so.at(k + 1)) -
-             otd * (bd_k * so.at(k - 1) - (bd_k + bd_k1) * sk + bd_k1 * so.at(k + 1));
+        d -= otd * (amO_k * skm + (apO_k - amO) * sk - apO * skp) -
+             otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
       }
       // Cn[m] = 0 - ap[k]; d[m] += fab*Cn[m] with fab = 1
       const double cnk = -hflux(A, fc, nfs, k);
@@ -352,6 +357,11 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
     A.tc[idx] = c;
     A.td[idx] = d;
     bprev = b; cprev = c; dprev = d;
+    // slide the windows down one level
+    s0 = s1; s1 = s2; s2 = s3; s3 = so.at(k + 3);
+    z0 = z1; z1 = dzz.at(k + 2);
+    o0 = o1; o1 = dzo.at(k + 2);
+    q0 = q1; q1 = kap(k + 2);
   }
   // Cn: zero above and at ktop, the folded top value on [ctop, ktop]
   for (int k = 0; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = 0;
