@@ -161,6 +161,12 @@ __global__ __launch_bounds__(256) void k_wp_source(Geo G, WPArgs A) {
     }
   }
   auto Wd = [&](uint32_t b, int k) { return at32(A.w, b + (uint32_t)k * 64u); };
+  // w of this column at k and k+1, and of each face's other column at k and
+  // k+1 over the face's levels, slide down the column in registers
+  double wo0 = Wd(base, 0), wo1 = Wd(base, 1);
+  double wn0[NF], wn1[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) { wn0[f] = 0; wn1[f] = 0; }
   for (int k = 0; k < n; k++) {
     double st = 0;
     const uint32_t kk = (uint32_t)k * 64u;
@@ -168,25 +174,33 @@ __global__ __launch_bounds__(256) void k_wp_source(Geo G, WPArgs A) {
 #pragma unroll
       for (int f = 0; f < NF; f++)
         if (f < nf) st += at32(A.uf, eb[f] + kk) * at32(A.U, eb[f] + kk) * df[f] * dfn[f] / (1.0 * Ac);
-      if (vadv) {
-        const double w0 = Wd(base, k), w1 = Wd(base, k + 1);
-        st += (w0 * w0 - w1 * w1) / at32(A.dzz, base + kk);
-      }
+      if (vadv) st += (wo0 * wo0 - wo1 * wo1) / at32(A.dzz, base + kk);
     }
+    const double nlo = at32(A.nu_lax, base + kk);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       if (f >= nf || !ocomp[f]) continue;
       if (k >= okmin[f] && k < onke[f]) {
-        const double a = .5 * (A.nu_H + 0.5 * (at32(A.nu_lax, oc1[f] + kk) + at32(A.nu_lax, oc2[f] + kk))) *
-                         (Wd(oc2[f], k) - Wd(oc1[f], k) + Wd(oc2[f], k + 1) - Wd(oc1[f], k + 1)) * odf[f] / odg[f];
+        const uint32_t other = ofirst[f] ? oc2[f] : oc1[f];
+        if (k == okmin[f]) wn0[f] = Wd(other, k);
+        else wn0[f] = wn1[f];
+        wn1[f] = Wd(other, k + 1);
+        const double nln = at32(A.nu_lax, other + kk);
+        // the edge's first cell (grad[2j]) is c1
+        const double w1k = ofirst[f] ? wo0 : wn0[f], w1k1 = ofirst[f] ? wo1 : wn1[f];
+        const double w2k = ofirst[f] ? wn0[f] : wo0, w2k1 = ofirst[f] ? wn1[f] : wo1;
+        const double l1 = ofirst[f] ? nlo : nln, l2 = ofirst[f] ? nln : nlo;
+        const double a = .5 * (A.nu_H + 0.5 * (l1 + l2)) * (w2k - w1k + w2k1 - w1k1) * odf[f] / odg[f];
         if (ofirst[f]) st -= a / oAc1[f];
         else st += a / oAc2[f];
       } else if (k >= onke[f]) {
-        const double sw = Wd(base, k) + Wd(base, k + 1);
+        const double sw = wo0 + wo1;
         st += 0.25 * A.CdW * fabs(sw) * sw * odf[f] / Ac;
       }
     }
     at32(A.stmp, base + kk) = st;
+    wo0 = wo1;
+    wo1 = k + 2 <= n ? Wd(base, k + 2) : 0.0;
   }
 }
 
