@@ -60,6 +60,26 @@ int sun_grid_open(gridT *g, const int *mark);
  * process. */
 gridT *sun_grid_load(const char *dir);
 
+/* The reference's grid generation for one process (GetGrid with -g,
+ * main/grid.c:110-212, TRIANGULATE off), in time linear in the grid: reads
+ * the points, edges and cells files suntans.dat names in dir (main/gridio.c:
+ * 515-603), moves obtuse triangles' Voronoi points for CorrectVoronoi < 0
+ * (main/grid.c CorrectAngles), takes the depth from the "<depth>-voro" file
+ * for IntDepth 2 or from depth(x, y) -- the case's ReturnDepth
+ * (main/initialization.c:78) -- otherwise (then writing "<depth>-voro" as
+ * GetDepth does), applies minimum_depth, builds the faces, the single
+ * process's cell and edge order, the point lists, the vertical grid
+ * (rstretch as GetDZ, main/initialization.c:27-56, vertgridcorrect,
+ * stairstep) and the metrics, and writes cells.dat.0, edges.dat.0 and the
+ * run-phase files (celldata, edgedata, nodes, topology with ".0", and
+ * vertspace) as OutputGridData does (main/gridio.c:269-330): the same bytes
+ * as the reference's.  Returns the grid as GetGrid leaves it in memory
+ * (full-precision metrics; sun_grid_load of the files gives the run phase's
+ * rounded copy), or NULL with a message on stderr for a missing file or
+ * key, an inconsistent grid, CorrectVoronoi > 0 or IntDepth 1 (neither is
+ * rebuilt). */
+gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y));
+
 /* Restart records in the reference's binary format (OutputData's store
  * block and ReadPhysicalVariables, main/physio.c:337-394, 407-495): the step
  * number, h, then Cn_U, Cn_U2 (Nke per edge), Cn_W, Cn_W2, Cn_R, Cn_T (Nk

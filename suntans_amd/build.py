@@ -62,7 +62,7 @@ def build_product() -> None:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", os.path.join(LIB, "libsunhip.so")])
     host = os.path.join(CSRC, "host")
     _run([CC, "-O2", "-fPIC", "-shared", "-Wall", f"-I{INC}", os.path.join(host, "sunstate.c"),
-          "-o", os.path.join(LIB, "libsunstate.so"), "-lm"])
+          "-o", os.path.join(LIB, "libsunstate.so"), "-lm", "-pthread"])
     _run([CC, "-O2", "-fPIC", "-shared", "-Wall", f"-I{INC}", os.path.join(host, "dropin.c"),
           f"-L{LIB}", "-lsunhip", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIB, "libsuntans.so")])
 

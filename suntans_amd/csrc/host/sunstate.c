@@ -3,6 +3,10 @@
  * See include/sunstate.h for the contract of each function.
  */
 #include <ctype.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <unistd.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
@@ -798,7 +802,7 @@ static int *grid_ivec(const gridT *g, const char *name, long *n) {
     {"Nkc", g->Nkc, Ne}, {"etop", g->etop, Ne}, {"etopold", g->etopold, Ne}, {"mark", g->mark, Ne},
     {"edgep", g->edgep, Ne}, {"grad", g->grad, 2 * Ne}, {"gradf", g->gradf, 2 * Ne},
     {"edges", g->edges, 3 * Ne}, {"celldist", g->celldist, 3}, {"edgedist", g->edgedist, 6},
-    {"mnptr", g->mnptr, Nc}, {"eptr", g->eptr, Ne}, {"edge_id", g->edge_id, Ne},
+    {"mnptr", g->mnptr, Nc}, {"eptr", g->eptr, Ne}, {"edge_id", g->edge_id, Ne}, {"Nkp", g->Nkp, g->Np},
   };
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (!strcmp(t[i].nm, name)) { *n = t[i].n; return t[i].p; }
@@ -988,6 +992,8 @@ static double next_field(FILE *f) {
   }
 }
 
+static void grid_vertical_rows(gridT *g, const char *datafile);
+
 gridT *sun_grid_load(const char *dir) {
   char datafile[1024], v[256];
   snprintf(datafile, sizeof datafile, "%s/suntans.dat", dir);
@@ -1100,6 +1106,16 @@ gridT *sun_grid_load(const char *dir) {
   if (!f) { sun_grid_free(g); return NULL; }
   for (int k = 0; k < g->Nkmax; k++) g->dz[k] = next_field(f);
   fclose(f);
+  grid_vertical_rows(g, datafile);
+  return g;
+}
+
+/* The vertical state InitializeVerticalGrid starts from (main/phys.c:765-799)
+ * on a grid whose Nk, Nkc and dz are set, with the suntans.dat switches it
+ * reads. */
+static void grid_vertical_rows(gridT *g, const char *datafile) {
+  char v[256];
+  const int Nc = g->Nc, Ne = g->Ne;
   g->stairstep = dat_string(datafile, "stairstep", v, sizeof v) ? (int)strtod(v, NULL) : 0;
   g->fixdzz = dat_string(datafile, "fixdzz", v, sizeof v) ? (int)strtod(v, NULL) : 0;
   g->dzsmall = dat_string(datafile, "dzsmall", v, sizeof v) ? strtod(v, NULL) : 0;
@@ -1116,6 +1132,640 @@ gridT *sun_grid_load(const char *dir) {
   for (int i = 0; i < Nc; i++)
     for (int k = 0; k < g->Nk[i]; k++) g->dzz[i][k] = g->dzzold[i][k] = g->dz[k];
   g->dzf = rows_alloc((int)Ne, g->Nkc, 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid generation                                                           */
+
+/* main/suntans.h:23 (the reference's value of pi, which sets the angle
+ * test of the Voronoi-point correction) */
+#define GRID_PI 3.141592654
+
+/* A number from suntans.dat as MPI_GetValue reads it (main/mympi.c:55-77):
+ * the file's value, else the default of main/fileio.c:277 where one exists
+ * (has_default), else an error. */
+static int gen_value(const char *datafile, const char *key, int has_default, double dflt, double *out) {
+  char v[256];
+  if (dat_string(datafile, key, v, sizeof v)) { *out = strtod(v, NULL); return 1; }
+  if (has_default) { *out = dflt; return 1; }
+  fprintf(stderr, "sun_grid_generate: %s: no value for %s\n", datafile, key);
+  return 0;
+}
+
+/* Columns of a grid file's first line (main/fileio.c:205-228): a blank or
+ * tab after a word starts a column; a blank just before the line's end
+ * takes one back. */
+static int gen_columns(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return 0;
+  int n = 1, c, prev = 0;
+  while ((c = fgetc(f)) != EOF) {
+    if ((c == ' ' || c == '\t') && !(prev == ' ' || prev == '\t' || prev == 0)) n++;
+    if (c == '\n') { if (prev == ' ') n--; break; }
+    prev = c;
+  }
+  fclose(f);
+  return n;
+}
+
+/* The vertical spacing of GetDZ (main/initialization.c:27-56): uniform for
+ * rstretch 1, geometric growing downward for 1 < r <= 1.1 and upward for
+ * -1.1 < r < -1.  Returns 0 for any other r. */
+static int gen_dz(double *dz, double depth, int Nkmax, double r) {
+  if (r == 1) {
+    for (int k = 0; k < Nkmax; k++) dz[k] = depth / Nkmax;
+  } else if (r > 1 && r <= 1.1) {
+    dz[0] = depth * (r - 1) / (pow(r, Nkmax) - 1);
+    for (int k = 1; k < Nkmax; k++) dz[k] = r * dz[k - 1];
+  } else if (r > -1.1 && r < -1) {
+    r = fabs(r);
+    dz[Nkmax - 1] = depth * (r - 1) / (pow(r, Nkmax) - 1);
+    for (int k = Nkmax - 2; k >= 0; k--) dz[k] = r * dz[k + 1];
+  } else {
+    return 0;
+  }
+  return 1;
+}
+
+/* Area of a polygon as GetArea computes it (main/grid.c:3647-3678): a
+ * triangle from its base and height, a polygon as the fan from its first
+ * point. */
+static double gen_tri_area(const double *x, const double *y) {
+  const double b = sqrt(pow(x[1] - x[0], 2) + pow(y[1] - y[0], 2));
+  const double r2 = sqrt(pow(x[2] - x[1], 2) + pow(y[2] - y[1], 2));
+  const double l = ((x[0] - x[1]) * (x[2] - x[1]) + (y[0] - y[1]) * (y[2] - y[1])) / b;
+  const double h = sqrt(r2 * r2 - l * l);
+  return 0.5 * b * h;
+}
+static double gen_area(const double *x, const double *y, int nf) {
+  if (nf < 3) return 0;
+  double a = gen_tri_area(x, y);
+  for (int i = 0; i < nf - 3; i++) {
+    const double tx[3] = {x[i + 2], x[i + 3], x[0]}, ty[3] = {y[i + 2], y[i + 3], y[0]};
+    a += gen_tri_area(tx, ty);
+  }
+  return a;
+}
+
+/* face and gradf of a planform from grad and neigh (main/grid.c:654-713).
+ * First each interior edge takes the slot of its neighbour in both cells,
+ * scanning the first cell's nfaces slots in both (so a slot of the second
+ * cell past the first cell's count is not seen here); then every cell side
+ * still without its edge takes the first free slot, edges in index order. */
+static void gen_faces(int *grad, int *gradf, const int *neigh, int *face, const int *nfaces, int mf, int Nc,
+                      int Ne) {
+  for (int n = 0; n < Nc; n++)
+    for (int nf = 0; nf < nfaces[n]; nf++) face[n * mf + nf] = -1;
+  for (int e = 0; e < 2 * Ne; e++) gradf[e] = -1;
+  for (int e = 0; e < Ne; e++) {
+    const int c1 = grad[2 * e], c2 = grad[2 * e + 1];
+    if (c1 == -1 || c2 == -1) continue;
+    for (int nf = 0; nf < nfaces[c1]; nf++) {
+      if (neigh[c1 * mf + nf] == c2) { face[c1 * mf + nf] = e; gradf[2 * e] = nf; }
+      if (neigh[c2 * mf + nf] == c1) { face[c2 * mf + nf] = e; gradf[2 * e + 1] = nf; }
+    }
+  }
+  for (int e = 0; e < Ne; e++)
+    for (int j = 0; j < 2; j++) {
+      const int c = grad[2 * e + j];
+      if (c == -1) continue;
+      int have = 0;
+      for (int nf = 0; nf < nfaces[c]; nf++) have |= face[c * mf + nf] == e;
+      if (have) continue;
+      for (int nf = 0; nf < nfaces[c]; nf++)
+        if (face[c * mf + nf] == -1) { face[c * mf + nf] = e; gradf[2 * e + j] = nf; break; }
+    }
+}
+
+/* Triangle corners wound with the faces (main/grid.c:754-832): corner nf+1
+ * is the point faces nf and nf+1 share, corner 0 the point of the last and
+ * first faces (the first of the earlier face's two points found in the
+ * other).  Returns the number of sides with no shared point. */
+static int gen_shared(const int *e1, const int *e2) {
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      if (e1[i] == e2[j]) return e1[i];
+  return -1;
+}
+static int gen_wind_triangles(const int *face, const int *edges, int *cells, const int *nfaces, int mf, int Nc) {
+  int bad = 0;
+  for (int c = 0; c < Nc; c++) {
+    const int nff = nfaces[c];
+    for (int nf = 0; nf < nff - 1; nf++) {
+      const int s = gen_shared(edges + 3 * face[c * mf + nf], edges + 3 * face[c * mf + nf + 1]);
+      if (s == -1) bad++;
+      else cells[mf * c + nf + 1] = s;
+    }
+    const int s = gen_shared(edges + 3 * face[c * mf + nff - 1], edges + 3 * face[c * mf]);
+    if (s == -1) bad++;
+    else cells[mf * c] = s;
+  }
+  return bad;
+}
+
+static int gen_fail(const char *what, const char *detail) {
+  fprintf(stderr, "sun_grid_generate: %s%s%s\n", what, detail ? ": " : "", detail ? detail : "");
+  return 0;
+}
+
+static int gen_write(const char *path, FILE **f) {
+  *f = fopen(path, "w");
+  if (!*f) return gen_fail("cannot write", path);
+  return 1;
+}
+
+/* Text output formatted in parallel: lines [lo, hi) of a file go to one
+ * thread's buffer through the C library's own printf, so the bytes are
+ * those of a serial fprintf; the buffers are written in line order. */
+struct gen_buf { char *p; size_t n, cap; };
+static void gb_printf(struct gen_buf *b, const char *fmt, ...) {
+  for (;;) {
+    va_list ap;
+    va_start(ap, fmt);
+    const int m = vsnprintf(b->p + b->n, b->cap - b->n, fmt, ap);
+    va_end(ap);
+    if (m < 0) return;
+    if ((size_t)m < b->cap - b->n) { b->n += (size_t)m; return; }
+    b->cap = 2 * b->cap + (size_t)m + 1;
+    b->p = (char *)realloc(b->p, b->cap);
+    if (!b->p) { fprintf(stderr, "sun_grid_generate: out of memory\n"); abort(); }
+  }
+}
+/* what the per-line writers read */
+struct gen_out {
+  const gridT *g;
+  const int *pc_off, *pc, *pe_off, *pe, *pp, *npp;
+};
+typedef void (*gen_line_fn)(struct gen_buf *b, long i, const struct gen_out *o);
+struct gen_job { gen_line_fn fn; const struct gen_out *o; long lo, hi; struct gen_buf *b; };
+static void *gen_run(void *arg) {
+  struct gen_job *j = (struct gen_job *)arg;
+  j->b->n = 0;
+  for (long i = j->lo; i < j->hi; i++) j->fn(j->b, i, j->o);
+  return NULL;
+}
+static int gen_emit(const char *path, long n, gen_line_fn fn, const struct gen_out *o) {
+  FILE *f = fopen(path, "w");
+  if (!f) return gen_fail("cannot write", path);
+  enum { GEN_TMAX = 16, GEN_CHUNK = 4096 };
+  long nt = sysconf(_SC_NPROCESSORS_ONLN);
+  if (nt < 1) nt = 1;
+  if (nt > GEN_TMAX) nt = GEN_TMAX;
+  struct gen_buf b[GEN_TMAX];
+  for (int t = 0; t < nt; t++) { b[t].cap = 1 << 20; b[t].n = 0; b[t].p = (char *)xcalloc(b[t].cap, 1); }
+  for (long base = 0; base < n; base += nt * GEN_CHUNK) {
+    pthread_t th[GEN_TMAX];
+    struct gen_job jb[GEN_TMAX];
+    int started[GEN_TMAX];
+    for (int t = 0; t < nt; t++) {
+      const long lo = base + t * GEN_CHUNK, hi = lo + GEN_CHUNK < n ? lo + GEN_CHUNK : n;
+      jb[t] = (struct gen_job){fn, o, lo, lo < hi ? hi : lo, &b[t]};
+      started[t] = t > 0 && lo < hi && pthread_create(&th[t], NULL, gen_run, &jb[t]) == 0;
+    }
+    for (int t = 0; t < nt; t++) if (!started[t]) gen_run(&jb[t]);
+    for (int t = 0; t < nt; t++) {
+      if (started[t]) pthread_join(th[t], NULL);
+      fwrite(b[t].p, 1, b[t].n, f);
+    }
+  }
+  for (int t = 0; t < nt; t++) free(b[t].p);
+  const int bad = ferror(f);
+  fclose(f);
+  if (bad) return gen_fail("write failed", path);
+  return 1;
+}
+
+/* one line of each run-phase file (main/gridio.c:608-627, 562-578,
+ * 631-662, 687-711, 728-783) */
+static void line_cells0(struct gen_buf *b, long j, const struct gen_out *o) {
+  const gridT *g = o->g;
+  const int mf = g->maxfaces;
+  gb_printf(b, "%d ", g->nfaces[j]);
+  gb_printf(b, "%e %e ", g->xv[j], g->yv[j]);
+  for (int nf = 0; nf < g->nfaces[j]; nf++) gb_printf(b, "%d ", g->cells[j * mf + nf]);
+  for (int nf = 0; nf < g->nfaces[j]; nf++) gb_printf(b, "%d ", g->neigh[j * mf + nf]);
+  gb_printf(b, "\n");
+}
+static void line_edges0(struct gen_buf *b, long j, const struct gen_out *o) {
+  const gridT *g = o->g;
+  gb_printf(b, "%d %d %d %d %d %d \n", g->edges[3 * j], g->edges[3 * j + 1], g->mark[j], g->grad[2 * j],
+            g->grad[2 * j + 1], g->edge_id[j]);
+}
+static void line_celldata(struct gen_buf *b, long n, const struct gen_out *o) {
+  const gridT *g = o->g;
+  const int mf = g->maxfaces;
+  gb_printf(b, "%d %e %e %e %e %d ", g->nfaces[n], g->xv[n], g->yv[n], g->Ac[n], g->dv[n], g->Nk[n]);
+  for (int nf = 0; nf < g->nfaces[n]; nf++) gb_printf(b, "%d ", g->face[mf * n + nf]);
+  for (int nf = 0; nf < g->nfaces[n]; nf++) gb_printf(b, "%d ", g->neigh[mf * n + nf]);
+  for (int nf = 0; nf < g->nfaces[n]; nf++) gb_printf(b, "%d ", g->normal[mf * n + nf]);
+  for (int nf = 0; nf < g->nfaces[n]; nf++) gb_printf(b, "%e ", g->def[mf * n + nf]);
+  for (int nf = 0; nf < g->nfaces[n]; nf++) gb_printf(b, "%d ", g->cells[mf * n + nf]);
+  gb_printf(b, "%d ", g->mnptr[n]);
+  gb_printf(b, "\n");
+}
+static void line_edgedata(struct gen_buf *b, long n, const struct gen_out *o) {
+  const gridT *g = o->g;
+  gb_printf(b, "%e %e %e %e %e %e %d %d %d %d %d %d %d %d %d %d %d\n", g->df[n], g->dg[n], g->n1[n], g->n2[n],
+            g->xe[n], g->ye[n], g->Nke[n], g->Nkc[n], g->grad[2 * n], g->grad[2 * n + 1], g->gradf[2 * n],
+            g->gradf[2 * n + 1], g->mark[n], g->edges[3 * n], g->edges[3 * n + 1], g->edge_id[n], g->eptr[n]);
+}
+static void line_nodes(struct gen_buf *b, long n, const struct gen_out *o) {
+  const gridT *g = o->g;
+  gb_printf(b, "%d %e %e ", (int)n, g->xp[n], g->yp[n]);
+  gb_printf(b, "%d ", o->npp[n]);
+  for (int i = 0; i < o->npp[n]; i++) gb_printf(b, "%d ", o->pp[o->pe_off[n] + i]);
+  gb_printf(b, "%d ", o->pe_off[n + 1] - o->pe_off[n]);
+  for (int i = o->pe_off[n]; i < o->pe_off[n + 1]; i++) gb_printf(b, "%d ", o->pe[i]);
+  gb_printf(b, "%d ", o->pc_off[n + 1] - o->pc_off[n]);
+  for (int i = o->pc_off[n]; i < o->pc_off[n + 1]; i++) gb_printf(b, "%d ", o->pc[i]);
+  gb_printf(b, "%d ", g->Nkp[n]);
+  /* the summed area of the point's cells in each layer */
+  for (int k = 0; k < g->Nkp[n]; k++) {
+    double a = 0;
+    for (int i = o->pc_off[n]; i < o->pc_off[n + 1]; i++)
+      if (k < g->Nk[o->pc[i]]) a += g->Ac[o->pc[i]];
+    gb_printf(b, "%e ", a);
+  }
+  gb_printf(b, "\n");
+}
+
+gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
+  char datafile[1024];
+  snprintf(datafile, sizeof datafile, "%s/suntans.dat", dir);
+  double Nkmax_d, stair_d, intdepth_d, mindepth, dzsmall, maxfaces_d, correct_d, vgc_d, rstretch;
+  if (!gen_value(datafile, "maxFaces", 1, 3, &maxfaces_d) || !gen_value(datafile, "CorrectVoronoi", 0, 0, &correct_d) ||
+      !gen_value(datafile, "Nkmax", 0, 0, &Nkmax_d) || !gen_value(datafile, "stairstep", 0, 0, &stair_d) ||
+      !gen_value(datafile, "IntDepth", 0, 0, &intdepth_d) || !gen_value(datafile, "minimum_depth", 1, 0, &mindepth) ||
+      !gen_value(datafile, "dzsmall", 0, 0, &dzsmall) || !gen_value(datafile, "vertgridcorrect", 0, 0, &vgc_d) ||
+      !gen_value(datafile, "rstretch", 0, 0, &rstretch))
+    return NULL;
+  const int mf = (int)maxfaces_d, correct = (int)correct_d, intdepth = (int)intdepth_d;
+  int Nkmax = (int)Nkmax_d;
+  const int stairstep = (int)stair_d, vgc = (int)vgc_d;
+  (void)dzsmall;
+  double vratio = 0;
+  if (correct > 0) { gen_fail("CorrectVoronoi > 0 is not rebuilt", NULL); return NULL; }
+  if (correct < 0 && !gen_value(datafile, "VoronoiRatio", 0, 0, &vratio)) return NULL;
+  if (intdepth == 1) { gen_fail("IntDepth 1 (depth interpolation) is not rebuilt", NULL); return NULL; }
+  if (intdepth != 2 && !depth) { gen_fail("IntDepth 0 needs the depth function (ReturnDepth)", NULL); return NULL; }
+  if (mf < 3 || Nkmax < 1) { gen_fail("maxFaces and Nkmax must be at least 3 and 1", NULL); return NULL; }
+
+  char fpts[1100], fedg[1100], fcel[1100], fdep[1100], fcd[1100], fed[1100], fnod[1100], ftop[1100], fvs[1100];
+  if (!dat_file(dir, datafile, "points", "", fpts, sizeof fpts) || !dat_file(dir, datafile, "edges", "", fedg, sizeof fedg) ||
+      !dat_file(dir, datafile, "cells", "", fcel, sizeof fcel) || !dat_file(dir, datafile, "depth", "", fdep, sizeof fdep) ||
+      !dat_file(dir, datafile, "celldata", ".0", fcd, sizeof fcd) ||
+      !dat_file(dir, datafile, "edgedata", ".0", fed, sizeof fed) ||
+      !dat_file(dir, datafile, "nodes", ".0", fnod, sizeof fnod) ||
+      !dat_file(dir, datafile, "topology", ".0", ftop, sizeof ftop) ||
+      !dat_file(dir, datafile, "vertspace", "", fvs, sizeof fvs)) {
+    gen_fail("suntans.dat lacks a grid file name", datafile);
+    return NULL;
+  }
+  const long Np = file_lines(fpts), Ne = file_lines(fedg), Nc = file_lines(fcel);
+  if (Np <= 0 || Ne <= 0 || Nc <= 0) { gen_fail("empty or missing points, edges or cells file", dir); return NULL; }
+
+  /* the planform as read (main/gridio.c:515-603); slots past a cell's
+   * count start at zero */
+  double *xp = (double *)xcalloc(Np, sizeof(double)), *yp = (double *)xcalloc(Np, sizeof(double));
+  int *medges = (int *)xcalloc(3 * Ne, sizeof(int)), *mmark = (int *)xcalloc(Ne, sizeof(int));
+  int *mgrad = (int *)xcalloc(2 * Ne, sizeof(int)), *mgradf = (int *)xcalloc(2 * Ne, sizeof(int));
+  int *medge_id = (int *)xcalloc(Ne, sizeof(int));
+  int *mnf = (int *)xcalloc(Nc, sizeof(int)), *mcells = (int *)xcalloc(mf * Nc, sizeof(int));
+  int *mneigh = (int *)xcalloc(mf * Nc, sizeof(int)), *mface = (int *)xcalloc(mf * Nc, sizeof(int));
+  double *mxv = (double *)xcalloc(Nc, sizeof(double)), *myv = (double *)xcalloc(Nc, sizeof(double));
+  double *mdv = (double *)xcalloc(Nc, sizeof(double));
+  int *mNk = (int *)xcalloc(Nc, sizeof(int)), *lcptr = (int *)xcalloc(Nc, sizeof(int));
+  int *leptr = (int *)xcalloc(Ne, sizeof(int));
+  int *pc_off = NULL, *pc = NULL, *fill = NULL, *pe_off = NULL, *pe = NULL, *pp = NULL, *npp = NULL;
+  int *type3 = NULL;
+  gridT *g = NULL;
+  int ok = 0;
+  FILE *f = NULL;
+  do {
+    if (!(f = fopen(fpts, "r"))) { gen_fail("cannot read", fpts); break; }
+    for (long n = 0; n < Np; n++) { xp[n] = next_field(f); yp[n] = next_field(f); next_field(f); }
+    fclose(f);
+    const int ecols = gen_columns(fedg);
+    if (!(f = fopen(fedg, "r"))) { gen_fail("cannot read", fedg); break; }
+    for (long n = 0; n < Ne; n++) {
+      medges[3 * n] = (int)next_field(f);
+      medges[3 * n + 1] = (int)next_field(f);
+      mmark[n] = (int)next_field(f);
+      mgrad[2 * n] = (int)next_field(f);
+      mgrad[2 * n + 1] = (int)next_field(f);
+      medge_id[n] = ecols > 5 ? (int)next_field(f) : 0;
+    }
+    fclose(f);
+    const int ccols = gen_columns(fcel);
+    if (!(f = fopen(fcel, "r"))) { gen_fail("cannot read", fcel); break; }
+    int bad = 0;
+    for (long n = 0; n < Nc && !bad; n++) {
+      mnf[n] = ccols > 8 ? (int)next_field(f) : 3;
+      if (mnf[n] > mf || mnf[n] < 3) { bad = 1; break; }
+      mxv[n] = next_field(f);
+      myv[n] = next_field(f);
+      for (int nf = 0; nf < mnf[n]; nf++) mcells[n * mf + nf] = (int)next_field(f);
+      for (int nf = 0; nf < mnf[n]; nf++) mneigh[n * mf + nf] = (int)next_field(f);
+    }
+    fclose(f);
+    if (bad) { gen_fail("a cell has more faces than maxFaces (or fewer than 3)", fcel); break; }
+    for (long e = 0; e < Ne; e++)
+      for (int j = 0; j < 2; j++)
+        if (medges[3 * e + j] < 0 || medges[3 * e + j] >= Np || mgrad[2 * e + j] < -1 || mgrad[2 * e + j] >= Nc) bad = 1;
+    for (long n = 0; n < Nc; n++)
+      for (int nf = 0; nf < mnf[n]; nf++)
+        if (mcells[n * mf + nf] < 0 || mcells[n * mf + nf] >= Np || mneigh[n * mf + nf] < -1 ||
+            mneigh[n * mf + nf] >= Nc)
+          bad = 1;
+    if (bad) { gen_fail("an index in the edges or cells file is out of range", dir); break; }
+
+    /* Voronoi points of obtuse triangles moved to the centroid
+     * (CorrectAngles, main/grid.c:3752-3800) */
+    if (correct < 0) {
+      const double cosr = cos(vratio * GRID_PI / 180.0);
+      for (long n = 0; n < Nc; n++) {
+        if (mnf[n] != 3) continue;
+        const int *pc = mcells + n * mf;
+        const double x1 = xp[pc[0]], x2 = xp[pc[1]], x3 = xp[pc[2]];
+        const double y1 = yp[pc[0]], y2 = yp[pc[1]], y3 = yp[pc[2]];
+        const double m1 = sqrt(pow(x2 - x1, 2) + pow(y2 - y1, 2));
+        const double m2 = sqrt(pow(x3 - x1, 2) + pow(y3 - y1, 2));
+        const double m3 = sqrt(pow(x3 - x2, 2) + pow(y3 - y2, 2));
+        const double d1 = (m1 * m1 + m2 * m2 - m3 * m3) / (2 * m1 * m2);
+        const double d2 = (m2 * m2 + m3 * m3 - m1 * m1) / (2 * m2 * m3);
+        const double d3 = (m3 * m3 + m1 * m1 - m2 * m2) / (2 * m3 * m1);
+        if (d1 <= cosr || d2 <= cosr || d3 <= cosr) {
+          const double sx = 0 + (x1 + x2 + x3) / 3, sy = 0 + (y1 + y2 + y3) / 3;
+          mxv[n] = sx / (mnf[n] - 2);
+          myv[n] = sy / (mnf[n] - 2);
+        }
+      }
+    }
+
+    /* depth (GetDepth, main/grid.c:429-535) */
+    if (intdepth == 2) {
+      char fv[1200];
+      snprintf(fv, sizeof fv, "%s-voro", fdep);
+      if (!(f = fopen(fv, "r"))) { gen_fail("cannot read", fv); break; }
+      for (long n = 0; n < Nc; n++) { next_field(f); next_field(f); mdv[n] = next_field(f); }
+      fclose(f);
+    } else {
+      for (long n = 0; n < Nc; n++) mdv[n] = depth(mxv[n], myv[n]);
+      char fv[1200];
+      snprintf(fv, sizeof fv, "%s-voro", fdep);
+      if (!gen_write(fv, &f)) break;
+      for (long n = 0; n < Nc; n++) fprintf(f, "%f %f %f\n", mxv[n], myv[n], mdv[n]);
+      fclose(f);
+    }
+    double maxd = 0;
+    for (long n = 0; n < Nc; n++) if (mdv[n] > maxd) maxd = mdv[n];
+    if (mindepth > 0) {
+      for (long n = 0; n < Nc; n++) if (mdv[n] < mindepth) mdv[n] = mindepth;
+    } else if (mindepth < 0) {
+      double *dz0 = (double *)xcalloc(Nkmax, sizeof(double));
+      if (!gen_dz(dz0, maxd, Nkmax, rstretch)) { free(dz0); gen_fail("rstretch out of range", NULL); break; }
+      for (long n = 0; n < Nc; n++) if (mdv[n] < dz0[0]) mdv[n] = dz0[0];
+      free(dz0);
+    }
+
+    /* the whole grid's faces and triangle corners (Connectivity,
+     * main/grid.c:1026-1048) */
+    gen_faces(mgrad, mgradf, mneigh, mface, mnf, mf, (int)Nc, (int)Ne);
+    for (long n = 0; n < Nc; n++)
+      for (int nf = 0; nf < mnf[n]; nf++) if (mface[n * mf + nf] < 0) bad = 1;
+    if (bad) { gen_fail("a cell side has no edge (grad and neigh disagree)", dir); break; }
+    if (mf == 3 && gen_wind_triangles(mface, medges, mcells, mnf, mf, (int)Nc))
+      fprintf(stderr, "sun_grid_generate: a triangle's faces share no point\n");
+
+    /* cells in the single process's order (TransferData,
+     * main/grid.c:2670-2760): those without a type-3 face, then those with
+     * one, each in index order */
+    type3 = (int *)xcalloc(Nc, sizeof(int));
+    for (long n = 0; n < Nc; n++)
+      for (int nf = 0; nf < mnf[n]; nf++) type3[n] |= mmark[mface[n * mf + nf]] == 3;
+    g = (gridT *)xcalloc(1, sizeof(gridT));
+    g->Nc = (int)Nc; g->Ne = (int)Ne; g->Np = (int)Np; g->maxfaces = mf;
+    planform_alloc(g);
+    g->mnptr = (int *)xcalloc(Nc, sizeof(int));
+    int k = 0;
+    for (int pass = 0; pass < 2; pass++)
+      for (long j = 0; j < Nc; j++)
+        if (type3[j] == pass) {
+          lcptr[j] = k;
+          g->mnptr[k] = (int)j;
+          g->xv[k] = mxv[j]; g->yv[k] = myv[j]; g->dv[k] = mdv[j]; g->nfaces[k] = mnf[j];
+          for (int nf = 0; nf < mnf[j]; nf++) g->cells[k * mf + nf] = mcells[j * mf + nf];
+          k++;
+        }
+    for (long i = 0; i < Nc; i++)
+      for (int nf = 0; nf < g->nfaces[i]; nf++) {
+        const int m = mneigh[g->mnptr[i] * mf + nf];
+        g->neigh[i * mf + nf] = m >= 0 ? lcptr[m] : -1;
+      }
+    /* edges in order of first appearance around the cells (main/grid.c:2799-2831) */
+    g->eptr = (int *)xcalloc(Ne, sizeof(int));
+    g->edge_id = (int *)xcalloc(Ne, sizeof(int));
+    for (long e = 0; e < Ne; e++) leptr[e] = -1;
+    k = 0;
+    for (long i = 0; i < Nc; i++)
+      for (int nf = 0; nf < g->nfaces[i]; nf++) {
+        const int ie = mface[g->mnptr[i] * mf + nf];
+        if (leptr[ie] >= 0) continue;
+        leptr[ie] = k;
+        for (int j = 0; j < 3; j++) g->edges[3 * k + j] = medges[3 * ie + j];
+        g->mark[k] = mmark[ie];
+        g->eptr[k] = ie;
+        g->edge_id[k] = medge_id[ie];
+        k++;
+      }
+    if (k != Ne) { gen_fail("the cells do not reach every edge", dir); break; }
+    for (long e = 0; e < Ne; e++)
+      for (int j = 0; j < 2; j++) {
+        const int c = mgrad[2 * g->eptr[e] + j];
+        g->grad[2 * e + j] = c != -1 ? lcptr[c] : -1;
+      }
+    gen_faces(g->grad, g->gradf, g->neigh, g->face, g->nfaces, mf, (int)Nc, (int)Ne);
+    if (mf == 3 && gen_wind_triangles(g->face, g->edges, g->cells, g->nfaces, mf, (int)Nc))
+      fprintf(stderr, "sun_grid_generate: a triangle's faces share no point\n");
+    for (long n = 0; n < Nc; n++)
+      for (int nf = 0; nf < g->nfaces[n]; nf++)
+        g->normal[n * mf + nf] = (n == g->grad[2 * g->face[mf * n + nf]]) ? -1 : 1;
+
+    /* each point's cells, edges and neighbour points (CreateNodeArray,
+     * main/grid.c:834-990), as offsets into flat lists */
+    pc_off = (int *)xcalloc(Np + 1, sizeof(int));
+    for (long c = 0; c < Nc; c++)
+      for (int nf = 0; nf < g->nfaces[c]; nf++) pc_off[g->cells[c * mf + nf] + 1]++;
+    for (long p = 0; p < Np; p++) pc_off[p + 1] += pc_off[p];
+    pc = (int *)xcalloc(pc_off[Np] + 1, sizeof(int));
+    fill = (int *)xcalloc(Np, sizeof(int));
+    for (long c = 0; c < Nc; c++)
+      for (int nf = 0; nf < g->nfaces[c]; nf++) {
+        const int p = g->cells[c * mf + nf];
+        pc[pc_off[p] + fill[p]++] = (int)c;
+      }
+    pe_off = (int *)xcalloc(Np + 1, sizeof(int));
+    for (long p = 0; p < Np; p++) {
+      int m = 0;
+      for (int i = pc_off[p]; i < pc_off[p + 1]; i++)
+        for (int nf = 0; nf < g->nfaces[pc[i]]; nf++) {
+          const int ie = g->face[mf * pc[i] + nf];
+          m += (g->edges[3 * ie] == p) + (g->edges[3 * ie + 1] == p);
+        }
+      pe_off[p + 1] = pe_off[p] + m;
+    }
+    pe = (int *)xcalloc(pe_off[Np] + 1, sizeof(int));
+    pp = (int *)xcalloc(pe_off[Np] + 1, sizeof(int));
+    npp = (int *)xcalloc(Np, sizeof(int));
+    for (long p = 0; p < Np; p++) {
+      int m = pe_off[p];
+      for (int i = pc_off[p]; i < pc_off[p + 1]; i++)
+        for (int nf = 0; nf < g->nfaces[pc[i]]; nf++) {
+          const int ie = g->face[mf * pc[i] + nf], a = g->edges[3 * ie], b = g->edges[3 * ie + 1];
+          if (a == p) { pe[m] = ie; pp[m++] = b; }
+          if (b == p) { pe[m] = ie; pp[m++] = a; }
+        }
+      /* neighbour points without repeats, first appearances kept */
+      int u = 0;
+      for (int i = pe_off[p]; i < pe_off[p + 1]; i++) {
+        int seen = 0;
+        for (int j = pe_off[p]; j < pe_off[p] + u; j++) seen |= pp[j] == pp[i];
+        if (!seen) pp[pe_off[p] + u++] = pp[i];
+      }
+      npp[p] = u;
+    }
+
+    /* vertical grid (VertGrid, main/grid.c:1387-1585) */
+    double dmax = mdv[0], dmin = mdv[0];
+    for (long i = 0; i < Nc; i++) { if (mdv[i] > dmax) dmax = mdv[i]; if (mdv[i] < dmin) dmin = mdv[i]; }
+    double dz0 = dmax / (double)Nkmax;
+    if (dmin < dz0 && vgc) { dz0 = dmin; Nkmax = 1 + (int)(dmax / dz0); }
+    g->Nkmax = Nkmax;
+    g->dz = (REAL *)xcalloc(Nkmax, sizeof(REAL));
+    if (!gen_dz(g->dz, dmax, Nkmax, rstretch)) { gen_fail("rstretch out of range", NULL); break; }
+    for (long i = 0; i < Nc; i++) {
+      if (Nkmax > 1 && mdv[i] != dmax) {
+        double s = 0;
+        for (int kk = 0; kk < Nkmax; kk++) {
+          s += g->dz[kk];
+          mNk[i] = kk + 1;
+          if (s >= mdv[i]) break;
+        }
+      } else {
+        mNk[i] = Nkmax;
+      }
+    }
+    g->Nk = (int *)xcalloc(Nc, sizeof(int));
+    for (long i = 0; i < Nc; i++) g->Nk[i] = mNk[g->mnptr[i]];
+    if (stairstep)
+      for (long i = 0; i < Nc; i++) {
+        double s = 0;
+        for (int kk = 0; kk < g->Nk[i]; kk++) s += g->dz[kk];
+        g->dv[i] = s;
+      }
+    g->Nke = (int *)xcalloc(Ne, sizeof(int));
+    g->Nkc = (int *)xcalloc(Ne, sizeof(int));
+    for (long j = 0; j < Ne; j++) {
+      const int c1 = mgrad[2 * g->eptr[j]], c2 = mgrad[2 * g->eptr[j] + 1];
+      if (c1 == -1) { g->Nke[j] = g->Nkc[j] = mNk[c2]; }
+      else if (c2 == -1) { g->Nke[j] = g->Nkc[j] = mNk[c1]; }
+      else {
+        g->Nke[j] = mNk[c1] < mNk[c2] ? mNk[c1] : mNk[c2];
+        g->Nkc[j] = mNk[c1] < mNk[c2] ? mNk[c2] : mNk[c1];
+      }
+    }
+    g->Nkp = (int *)xcalloc(Np, sizeof(int));
+    for (long p = 0; p < Np; p++) {
+      int m = 0;
+      for (int i = pc_off[p]; i < pc_off[p + 1]; i++) if (g->Nk[pc[i]] > m) m = g->Nk[pc[i]];
+      g->Nkp[p] = m;
+    }
+
+    /* metrics (Geometry, main/grid.c:3278-3563) */
+    double xt[64], yt[64];
+    if (mf > 64) { gen_fail("maxFaces above 64", NULL); break; }
+    for (long n = 0; n < Nc; n++) {
+      for (int nf = 0; nf < g->nfaces[n]; nf++) { xt[nf] = xp[g->cells[n * mf + nf]]; yt[nf] = yp[g->cells[n * mf + nf]]; }
+      g->Ac[n] = gen_area(xt, yt, g->nfaces[n]);
+    }
+    for (long n = 0; n < Ne; n++) {
+      const int p1 = g->edges[3 * n], p2 = g->edges[3 * n + 1];
+      g->df[n] = sqrt(pow(xp[p1] - xp[p2], 2) + pow(yp[p1] - yp[p2], 2));
+      double tx = xp[p2] - xp[p1], ty = yp[p2] - yp[p1];
+      const double tm = sqrt(tx * tx + ty * ty);
+      tx = tx / tm;
+      ty = ty / tm;
+      const int c = g->grad[2 * n] >= 0 ? g->grad[2 * n] : g->grad[2 * n + 1];
+      const double xd = (g->xv[c] - xp[p1]) * tx + (g->yv[c] - yp[p1]) * ty;
+      g->xe[n] = xp[p1] + xd * tx;
+      g->ye[n] = yp[p1] + xd * ty;
+    }
+    for (long n = 0; n < Ne; n++) {
+      const int c1 = g->grad[2 * n], c2 = g->grad[2 * n + 1];
+      if (c1 != -1 && c2 != -1) {
+        g->n1[n] = g->xv[c1] - g->xv[c2];
+        g->n2[n] = g->yv[c1] - g->yv[c2];
+        g->dg[n] = sqrt(pow(g->n1[n], 2) + pow(g->n2[n], 2));
+        g->n1[n] = g->n1[n] / g->dg[n];
+        g->n2[n] = g->n2[n] / g->dg[n];
+      } else {
+        if (c1 == -1) { g->n1[n] = g->xe[n] - g->xv[c2]; g->n2[n] = g->ye[n] - g->yv[c2]; }
+        else { g->n1[n] = g->xv[c1] - g->xe[n]; g->n2[n] = g->yv[c1] - g->ye[n]; }
+        g->dg[n] = sqrt(pow(g->n1[n], 2) + pow(g->n2[n], 2));
+        g->n1[n] = g->n1[n] / g->dg[n];
+        g->n2[n] = g->n2[n] / g->dg[n];
+        g->dg[n] = 2.0 * g->dg[n];
+      }
+    }
+    for (long n = 0; n < Nc; n++)
+      for (int nf = 0; nf < g->nfaces[n]; nf++) {
+        const int ne = g->face[n * mf + nf], p = g->edges[3 * ne];
+        g->def[n * mf + nf] =
+            -((g->xv[n] - xp[p]) * g->n1[ne] + (g->yv[n] - yp[p]) * g->n2[ne]) * g->normal[n * mf + nf];
+      }
+
+    /* computational order (MakePointers, main/grid.c:1628-1679) */
+    g->celldist = (int *)xcalloc(GRID_MAXBCTYPES - 1, sizeof(int));
+    g->edgedist = (int *)xcalloc(GRID_MAXMARKS - 1, sizeof(int));
+    g->cellp = (int *)xcalloc(Nc, sizeof(int));
+    g->edgep = (int *)xcalloc(Ne, sizeof(int));
+    g->myneighs = (int *)xcalloc(1, sizeof(int));
+    computational_order(g);
+    g->xp = xp; g->yp = yp; xp = yp = NULL;
+
+    /* the run-phase files (OutputGridData, main/gridio.c:269-330) */
+    char fc0[1200], fe0[1200];
+    snprintf(fc0, sizeof fc0, "%s.0", fcel);
+    snprintf(fe0, sizeof fe0, "%s.0", fedg);
+    const struct gen_out o = {g, pc_off, pc, pe_off, pe, pp, npp};
+    if (!gen_emit(fc0, Nc, line_cells0, &o) || !gen_emit(fe0, Ne, line_edges0, &o) ||
+        !gen_emit(fcd, Nc, line_celldata, &o) || !gen_emit(fed, Ne, line_edgedata, &o) ||
+        !gen_emit(fnod, Np, line_nodes, &o))
+      break;
+    if (!gen_write(ftop, &f)) break;
+    fprintf(f, "%d %d\n", 1, 0);
+    fprintf(f, "\n");
+    for (int n = 0; n < GRID_MAXBCTYPES - 1; n++) fprintf(f, "%d ", g->celldist[n]);
+    fprintf(f, "\n");
+    for (int n = 0; n < GRID_MAXMARKS - 1; n++) fprintf(f, "%d ", g->edgedist[n]);
+    fprintf(f, "\n");
+    for (long n = 0; n < Nc; n++) fprintf(f, "%d ", g->cellp[n]);
+    fprintf(f, "\n");
+    for (long n = 0; n < Ne; n++) fprintf(f, "%d ", g->edgep[n]);
+    fprintf(f, "\n");
+    fclose(f);
+    if (!gen_write(fvs, &f)) break;
+    for (int n = 0; n < Nkmax; n++) fprintf(f, "%e\n", g->dz[n]);
+    fclose(f);
+    f = NULL;
+    grid_vertical_rows(g, datafile);
+    ok = 1;
+  } while (0);
+  free(xp); free(yp); free(medges); free(mmark); free(mgrad); free(mgradf); free(medge_id); free(mnf);
+  free(mcells); free(mneigh); free(mface); free(mxv); free(myv); free(mdv); free(mNk); free(lcptr); free(leptr);
+  free(pc_off); free(pc); free(fill); free(pe_off); free(pe); free(pp); free(npp); free(type3);
+  if (f) fclose(f);
+  if (!ok) { sun_grid_free(g); return NULL; }
   return g;
 }
 

@@ -64,6 +64,8 @@ def state_lib() -> C.CDLL:
         L.sun_grid_open.argtypes = [_vp, _ip]
         L.sun_grid_load.restype = _vp
         L.sun_grid_load.argtypes = [C.c_char_p]
+        L.sun_grid_generate.restype = _vp
+        L.sun_grid_generate.argtypes = [C.c_char_p, _vp]
         L.sun_restart_write.argtypes = [C.c_char_p, _vp, _vp, C.c_int, C.c_int]
         L.sun_restart_read.argtypes = [C.c_char_p, _vp, _vp, _ip, C.c_int]
         L.sun_phys_alloc.restype = _vp

@@ -93,6 +93,33 @@ class State:
         return st
 
     @classmethod
+    def generate(cls, datadir: str, depth=None) -> "State":
+        """The reference's one-process grid generation (GetGrid with -g,
+        main/grid.c:110-212) in linear time (sun_grid_generate): reads the
+        points, edges and cells files suntans.dat in datadir names, writes
+        the run-phase files there byte for byte as the reference does, and
+        returns the grid GetGrid leaves in memory, with zero physical state
+        and default properties.  depth(x, y) is the case's ReturnDepth
+        (main/initialization.c:78), used when IntDepth is not 2."""
+        L = native.state_lib()
+        _bind(L)
+        st = cls.__new__(cls)
+        st._L = L
+        st.p = st.prop = None
+        st._owns_grid = True
+        cb = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double)(depth) if depth is not None else None
+        st.g = L.sun_grid_generate(str(datadir).encode(), C.cast(cb, C.c_void_p) if cb is not None else None)
+        if not st.g:
+            raise RuntimeError(f"sun_grid_generate: no grid from {datadir} (the reason is on stderr)")
+        st.p = L.sun_phys_alloc(st.g)
+        st.prop = L.sun_prop_default()
+        st.planform = "file"
+        xp, yp = st.get("xp"), st.get("yp")
+        st.nx = st.ny = 0
+        st.Lx, st.Ly = float(xp.max() - xp.min()), float(yp.max() - yp.min())
+        return st
+
+    @classmethod
     def view(cls, gptr: int) -> "State":
         """Read-only access to the grid arrays of a gridT that another library
         owns (no physical state)."""

@@ -7,11 +7,14 @@ Run where the reference was compiled from its own sources (oracle/_ref):
 
 For each example the data directory is copied to a scratch directory, the
 reference's GetGrid (main/grid.c:110, one process) writes its run-phase
-files there, and this script keeps celldata.dat.0, edgedata.dat.0,
-topology.dat.0 and vertspace.dat as written, nodes.dat.0 with the point
-coordinates only (each point's neighbour lists and per-layer areas, which
-nothing on the path reads, are dropped), and a suntans.dat with the keys a
-run-phase reader needs.  Data only: the files are the reference's output.
+files there, and this script keeps cells.dat.0, edges.dat.0,
+celldata.dat.0, edgedata.dat.0, nodes.dat.0, topology.dat.0 and
+vertspace.dat as written, with a suntans.dat holding the keys a run-phase
+reader needs.  planform/ keeps the example's own input -- points, edges and
+cells files, the "<depth>-voro" file where IntDepth is 2, and a suntans.dat
+with the keys grid generation reads -- so a generator can be checked against
+the files above.  Data only: the inputs are the example's, the rest the
+reference's output.
 """
 import os
 import shutil
@@ -30,6 +33,10 @@ GRIDS = {
     "iwaves": "iwaves/rundata",  # 200 cells on 100 stretched levels
 }
 KEYS = ("Nkmax", "maxFaces", "stairstep", "fixdzz", "dzsmall", "smoothbot")
+# what GetGrid reads (main/grid.c:110-212, GetDepth, VertGrid, GetDZ)
+GEN_KEYS = ("maxFaces", "Nkmax", "stairstep", "rstretch", "CorrectVoronoi", "VoronoiRatio", "vertgridcorrect",
+            "IntDepth", "minimum_depth", "dzsmall", "fixdzz", "smoothbot")
+FILE_KEYS = ("points", "edges", "cells", "depth", "celldata", "edgedata", "nodes", "topology", "vertspace")
 
 
 def generate(src: str, dst: str) -> None:
@@ -43,12 +50,12 @@ def generate(src: str, dst: str) -> None:
     subprocess.run([sys.executable, "-c", code, REF_SO, dst], check=True, cwd=dst, capture_output=True)
 
 
-def dat_lines(path: str) -> dict:
+def dat_lines(path: str, keys=KEYS) -> dict:
     out = {}
     with open(path) as f:
         for line in f:
             w = line.split()
-            if len(w) >= 2 and w[0] in KEYS and w[0] not in out:
+            if len(w) >= 2 and w[0] in keys and w[0] not in out:
                 out[w[0]] = w[1]
     return out
 
@@ -58,13 +65,24 @@ def make(name: str) -> None:
     os.makedirs(dst, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
         generate(os.path.join(EXAMPLES, GRIDS[name]), tmp)
-        for f in ("celldata.dat.0", "edgedata.dat.0", "topology.dat.0", "vertspace.dat"):
+        for f in ("cells.dat.0", "edges.dat.0", "celldata.dat.0", "edgedata.dat.0", "nodes.dat.0",
+                  "topology.dat.0", "vertspace.dat"):
             shutil.copy(os.path.join(tmp, f), dst)
-        with open(os.path.join(tmp, "nodes.dat.0")) as fi, open(os.path.join(dst, "nodes.dat.0"), "w") as fo:
-            for line in fi:
-                w = line.split()
-                fo.write(f"{w[0]} {w[1]} {w[2]} 0 0 0 0\n")
         keys = dat_lines(os.path.join(tmp, "suntans.dat"))
+        gen = dat_lines(os.path.join(tmp, "suntans.dat"), GEN_KEYS + FILE_KEYS)
+        src = os.path.join(EXAMPLES, GRIDS[name])
+        plan = os.path.join(dst, "planform")
+        os.makedirs(plan, exist_ok=True)
+        inputs = [gen["points"], gen["edges"], gen["cells"]]
+        if int(float(gen["IntDepth"])) == 2:
+            inputs.append(gen["depth"] + "-voro")
+        for f in inputs:
+            shutil.copyfile(os.path.join(src, f), os.path.join(plan, f))
+        with open(os.path.join(plan, "suntans.dat"), "w") as f:
+            f.write(f"# grid-generation keys of examples/{GRIDS[name]}\n")
+            for k in GEN_KEYS + FILE_KEYS:
+                if k in gen:
+                    f.write(f"{k}\t\t{gen[k]}\t# from the example\n")
     with open(os.path.join(dst, "suntans.dat"), "w") as f:
         f.write(f"# run-phase grid of examples/{GRIDS[name]}, written by the reference's GetGrid\n")
         for k in KEYS:
