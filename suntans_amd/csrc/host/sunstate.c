@@ -803,6 +803,7 @@ static int *grid_ivec(const gridT *g, const char *name, long *n) {
     {"edgep", g->edgep, Ne}, {"grad", g->grad, 2 * Ne}, {"gradf", g->gradf, 2 * Ne},
     {"edges", g->edges, 3 * Ne}, {"celldist", g->celldist, 3}, {"edgedist", g->edgedist, 6},
     {"mnptr", g->mnptr, Nc}, {"eptr", g->eptr, Ne}, {"edge_id", g->edge_id, Ne}, {"Nkp", g->Nkp, g->Np},
+    {"eneigh", g->eneigh, 2 * (mf - 1) * Ne},
   };
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (!strcmp(t[i].nm, name)) { *n = t[i].n; return t[i].p; }
@@ -1590,6 +1591,19 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
     for (long n = 0; n < Nc; n++)
       for (int nf = 0; nf < g->nfaces[n]; nf++)
         g->normal[n * mf + nf] = (n == g->grad[2 * g->face[mf * n + nf]]) ? -1 : 1;
+
+    /* the other faces of each edge's cells (CreateMomentumCV,
+     * main/grid.c:1071-1092) */
+    for (long n = 0; n < Ne; n++) {
+      int *en = g->eneigh + 2 * (mf - 1) * n, m = 0;
+      for (int j = 0; j < 2 * (mf - 1); j++) en[j] = -1;
+      for (int j = 0; j < 2; j++) {
+        const int c = g->grad[2 * n + j];
+        if (c == -1) continue;
+        for (int nf = 0; nf < g->nfaces[c]; nf++)
+          if (g->face[mf * c + nf] != n && m < 2 * (mf - 1)) en[m++] = g->face[mf * c + nf];
+      }
+    }
 
     /* each point's cells, edges and neighbour points (CreateNodeArray,
      * main/grid.c:834-990), as offsets into flat lists */

@@ -83,6 +83,23 @@ def test_generated_grid_matches_files(name, tmp_path):
         np.testing.assert_allclose(a, b, rtol=6e-7, atol=1e-12 * max(1.0, float(np.abs(b).max())), err_msg=k)
 
 
+def test_generated_momentum_neighbours(tmp_path):
+    """eneigh (CreateMomentumCV, main/grid.c:1071-1092): for each edge the
+    other faces of its first cell in face order, then of its second, then
+    -1."""
+    copy_dir(os.path.join(GOLD, "tides", "planform"), tmp_path)
+    g = generate(str(tmp_path))
+    Ne, mf = g.Ne, g.scalar("maxfaces")
+    en = g.ints("eneigh").reshape(Ne, 2 * (mf - 1))
+    grad = g.ints("grad").reshape(Ne, 2)
+    face = g.ints("face").reshape(-1, mf)
+    nfaces = g.ints("nfaces")
+    for n in range(Ne):
+        want = [e for c in grad[n] if c != -1 for e in face[c, :nfaces[c]] if e != n]
+        want += [-1] * (2 * (mf - 1) - len(want))
+        assert list(en[n]) == want, n
+
+
 def test_generate_refusals(tmp_path):
     """CorrectVoronoi > 0 and IntDepth 1 are not rebuilt; IntDepth 0 needs
     the depth function; a missing planform file is an error."""
