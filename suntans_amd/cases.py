@@ -15,6 +15,8 @@ where a step needs a divergence-free field.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from .state import State
@@ -315,8 +317,30 @@ def timestep_inputs(st: State, rng, nonlinear: int = 1, tvd: int = 4) -> State:
 def from_files(datadir: str, seed: int = 0, dt: float = 5.0, h_amp: float = 0.5, amp_u: float = 0.1) -> State:
     """A grid from run-phase files (State.load) with a moved free surface,
     seeded velocities and scalar, and the solver inputs of tiny()."""
+    return _seeded(State.load(datadir), seed, dt, h_amp, amp_u)
+
+
+def from_planform(planform_dir: str, seed: int = 0, depth=None, dt: float = 5.0, h_amp: float = 0.5,
+                  amp_u: float = 0.1) -> State:
+    """A grid generated from a planform (State.generate on a scratch copy of
+    planform_dir, so the run-phase files it writes land there), seeded as
+    from_files seeds it."""
+    import shutil
+    import tempfile
+    work = tempfile.mkdtemp(prefix="sun_planform_")
+    try:
+        for f in os.listdir(planform_dir):
+            src = os.path.join(planform_dir, f)
+            if os.path.isfile(src):
+                shutil.copyfile(src, os.path.join(work, f))
+        st = State.generate(work, depth=depth)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return _seeded(st, seed, dt, h_amp, amp_u)
+
+
+def _seeded(st: State, seed: int, dt: float, h_amp: float, amp_u: float) -> State:
     rng = np.random.default_rng(seed)
-    st = State.load(datadir)
     st.update_dz(-1)
     h1 = rng.uniform(-h_amp, 0.3 * h_amp, st.Nc)
     st.set("h", h1)

@@ -89,7 +89,7 @@ def test_operators(name, make):
 @pytest.mark.parametrize("name,make", MAKERS, ids=IDS)
 @pytest.mark.parametrize("tvd", [0, 1, 2, 3, 4])
 def test_update_scalars(name, make, tvd):
-    if name == "mbay" and tvd in (3, 4):
+    if name.startswith("mbay") and tvd in (3, 4):
         pytest.skip("two-layer columns: the limiter's ratio reads two levels past the column in the reference "
                     "(GetApAm, main/tvd.c:161-204), 0 in the oracle")
     o = _impls.oracle()
