@@ -63,8 +63,9 @@ gridT *sun_grid_load(const char *dir);
 /* The reference's grid generation for one process (GetGrid with -g,
  * main/grid.c:110-212, TRIANGULATE off), in time linear in the grid: reads
  * the points, edges and cells files suntans.dat names in dir (main/gridio.c:
- * 515-603), moves obtuse triangles' Voronoi points for CorrectVoronoi < 0
- * (main/grid.c CorrectAngles), takes the depth from the "<depth>-voro" file
+ * 515-603), corrects the Voronoi points (CorrectVoronoi > 0: toward the
+ * centroids by VoronoiRatio, < 0: obtuse triangles to the centroid;
+ * main/grid.c:3735-3869), takes the depth from the "<depth>-voro" file
  * for IntDepth 2 or from depth(x, y) -- the case's ReturnDepth
  * (main/initialization.c:78) -- otherwise (then writing "<depth>-voro" as
  * GetDepth does), applies minimum_depth, builds the faces, the single
@@ -76,7 +77,7 @@ gridT *sun_grid_load(const char *dir);
  * as the reference's.  Returns the grid as GetGrid leaves it in memory
  * (full-precision metrics; sun_grid_load of the files gives the run phase's
  * rounded copy), or NULL with a message on stderr for a missing file or
- * key, an inconsistent grid, CorrectVoronoi > 0 or IntDepth 1 (neither is
+ * key, an inconsistent grid, or IntDepth 1 (the depth interpolation, not
  * rebuilt). */
 gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y));
 

@@ -1405,8 +1405,7 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
   const int stairstep = (int)stair_d, vgc = (int)vgc_d;
   (void)dzsmall;
   double vratio = 0;
-  if (correct > 0) { gen_fail("CorrectVoronoi > 0 is not rebuilt", NULL); return NULL; }
-  if (correct < 0 && !gen_value(datafile, "VoronoiRatio", 0, 0, &vratio)) return NULL;
+  if (correct != 0 && !gen_value(datafile, "VoronoiRatio", 0, 0, &vratio)) return NULL;
   if (intdepth == 1) { gen_fail("IntDepth 1 (depth interpolation) is not rebuilt", NULL); return NULL; }
   if (intdepth != 2 && !depth) { gen_fail("IntDepth 0 needs the depth function (ReturnDepth)", NULL); return NULL; }
   if (mf < 3 || Nkmax < 1) { gen_fail("maxFaces and Nkmax must be at least 3 and 1", NULL); return NULL; }
@@ -1480,8 +1479,49 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
           bad = 1;
     if (bad) { gen_fail("an index in the edges or cells file is out of range", dir); break; }
 
+    /* Voronoi points too close to their neighbours' moved toward the
+     * centroids (CorrectVoronoi, main/grid.c:3735-3821): edge by edge in
+     * index order, each test reading the points as earlier edges left them */
+    if (correct > 0) {
+      for (long e = 0; e < Ne; e++) {
+        const int c1 = mgrad[2 * e], c2 = mgrad[2 * e + 1];
+        double cx[2] = {0, 0}, cy[2] = {0, 0};
+        const int cc[2] = {c1, c2};
+        for (int j = 0; j < 2; j++) {
+          if (cc[j] == -1) continue;
+          for (int nf = 0; nf < mnf[cc[j]]; nf++) {
+            cx[j] += xp[mcells[cc[j] * mf + nf]] / mnf[cc[j]];
+            cy[j] += yp[mcells[cc[j] * mf + nf]] / mnf[cc[j]];
+          }
+        }
+        if (c1 != -1 && c2 != -1 && mnf[c1] == 3 && mnf[c2] == 3) {
+          const double xc = 0.5 * (cx[0] + cx[1]), yc = 0.5 * (cy[0] + cy[1]);
+          const double dg0 = sqrt(pow(cx[1] - cx[0], 2) + pow(cy[1] - cy[0], 2));
+          const double dg = sqrt(pow(mxv[c2] - mxv[c1], 2) + pow(myv[c2] - myv[c1], 2));
+          if (dg < vratio * dg0) {
+            mxv[c1] = xc + vratio * (cx[0] - xc);
+            mxv[c2] = xc + vratio * (cx[1] - xc);
+            myv[c1] = yc + vratio * (cy[0] - yc);
+            myv[c2] = yc + vratio * (cy[1] - yc);
+          }
+        } else {
+          /* a boundary edge, or an edge of a non-triangle: the cell on the
+           * first side that exists against the edge's midpoint */
+          const double xc = 0.5 * (xp[medges[3 * e]] + xp[medges[3 * e + 1]]);
+          const double yc = 0.5 * (yp[medges[3 * e]] + yp[medges[3 * e + 1]]);
+          const int j = c1 == -1 ? 1 : 0, c = cc[j];
+          const double dg0 = sqrt(pow(cx[j] - xc, 2) + pow(cy[j] - yc, 2));
+          const double dg = sqrt(pow(mxv[c] - xc, 2) + pow(myv[c] - yc, 2));
+          if (dg < vratio * dg0) {
+            mxv[c] = xc + vratio * (cx[j] - xc);
+            myv[c] = yc + vratio * (cy[j] - yc);
+          }
+        }
+      }
+    }
+
     /* Voronoi points of obtuse triangles moved to the centroid
-     * (CorrectAngles, main/grid.c:3752-3800) */
+     * (CorrectAngles, main/grid.c:3823-3869) */
     if (correct < 0) {
       const double cosr = cos(vratio * GRID_PI / 180.0);
       for (long n = 0; n < Nc; n++) {

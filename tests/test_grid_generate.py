@@ -101,8 +101,8 @@ def test_generated_momentum_neighbours(tmp_path):
 
 
 def test_generate_refusals(tmp_path):
-    """CorrectVoronoi > 0 and IntDepth 1 are not rebuilt; IntDepth 0 needs
-    the depth function; a missing planform file is an error."""
+    """IntDepth 1 is not rebuilt; IntDepth 0 needs the depth function; a
+    missing planform file is an error."""
     src = os.path.join(GOLD, "iwaves", "planform")
 
     def case(sub, edit=None, depth=return_depth):
@@ -117,7 +117,6 @@ def test_generate_refusals(tmp_path):
             State.generate(str(d), depth=depth)
         assert not (d / "celldata.dat.0").exists()
 
-    case("voronoi", lambda l: "CorrectVoronoi\t1\t#" if l.startswith("CorrectVoronoi") else l)
     case("interp", lambda l: "IntDepth\t1\t#" if l.startswith("IntDepth") else l)
     case("nodepth", depth=None)
     case("nopoints", lambda l: "points\tabsent.dat\t#" if l.startswith("points") else l)
@@ -156,16 +155,41 @@ def test_generate_minimum_depth_and_stretch(tmp_path):
         assert abs(dv[i] - dz[:Nk[i]].sum()) < 1e-9 * D
 
 
-@pytest.mark.skipif(not (os.path.exists(REF_SO) and os.path.isdir(EXAMPLES)),
-                    reason="needs the reference compiled from its sources (oracle/_ref) and its examples")
+NEEDS_REF = pytest.mark.skipif(not (os.path.exists(REF_SO) and os.path.isdir(EXAMPLES)),
+                               reason="needs the reference compiled from its sources (oracle/_ref) and its examples")
+
+
+@NEEDS_REF
 @pytest.mark.parametrize("name", sorted(EXAMPLE_GRIDS))
 def test_generate_matches_reference(name, tmp_path):
+    both_ways(name, tmp_path)
+
+
+@NEEDS_REF
+@pytest.mark.parametrize("name,ratio", [("tides", "0.9"), ("cavity", "1.0"), ("cylinderquad", "0.95")])
+def test_generate_correct_voronoi_matches_reference(name, ratio, tmp_path):
+    """CorrectVoronoi > 0 (main/grid.c:3735-3821) on triangles, and on
+    quadrilaterals, whose edges take the midpoint branch."""
+    def edit(line):
+        if line.startswith("CorrectVoronoi"):
+            return "CorrectVoronoi\t1\t# edited"
+        if line.startswith("VoronoiRatio"):
+            return f"VoronoiRatio\t{ratio}\t# edited"
+        return line
+    both_ways(name, tmp_path, edit)
+
+
+def both_ways(name, tmp_path, edit=None):
     src = os.path.join(EXAMPLES, EXAMPLE_GRIDS[name])
     a, b = tmp_path / "ref", tmp_path / "ours"
     a.mkdir()
     b.mkdir()
     copy_dir(src, a)
     copy_dir(src, b)
+    if edit:
+        for d in (a, b):
+            lines = (d / "suntans.dat").read_text().splitlines()
+            (d / "suntans.dat").write_text("\n".join(edit(l) for l in lines) + "\n")
     code = ("import ctypes as C, sys; L = C.CDLL(sys.argv[1]); L.ref_set_verbose(0); "
             "L.ref_grid_generate.argtypes = [C.c_char_p]; L.ref_grid_generate(sys.argv[2].encode())")
     subprocess.run([sys.executable, "-c", code, REF_SO, str(a)], check=True, cwd=a, capture_output=True)
