@@ -66,7 +66,9 @@ gridT *sun_grid_load(const char *dir);
  * 515-603), corrects the Voronoi points (CorrectVoronoi > 0: toward the
  * centroids by VoronoiRatio, < 0: obtuse triangles to the centroid;
  * main/grid.c:3735-3869), takes the depth from the "<depth>-voro" file
- * for IntDepth 2 or from depth(x, y) -- the case's ReturnDepth
+ * for IntDepth 2, interpolates it from the depth file's soundings for
+ * IntDepth 1 (inverse squared distance over the maxFaces + 1 nearest, as
+ * InterpDepth, found through a bucket grid), or takes it from depth(x, y) -- the case's ReturnDepth
  * (main/initialization.c:78) -- otherwise (then writing "<depth>-voro" as
  * GetDepth does), applies minimum_depth, builds the faces, the single
  * process's cell and edge order, the point lists, the vertical grid
@@ -77,8 +79,7 @@ gridT *sun_grid_load(const char *dir);
  * as the reference's.  Returns the grid as GetGrid leaves it in memory
  * (full-precision metrics; sun_grid_load of the files gives the run phase's
  * rounded copy), or NULL with a message on stderr for a missing file or
- * key, an inconsistent grid, or IntDepth 1 (the depth interpolation, not
- * rebuilt). */
+ * key, or an inconsistent grid. */
 gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y));
 
 /* Restart records in the reference's binary format (OutputData's store

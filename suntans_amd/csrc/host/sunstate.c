@@ -1390,6 +1390,110 @@ static void line_nodes(struct gen_buf *b, long n, const struct gen_out *o) {
   gb_printf(b, "\n");
 }
 
+/* Depth at each query point from scattered soundings (InterpDepth,
+ * main/grid.c:3686-3733, with Interp and FindNearest, main/util.c): the
+ * np = maxFaces + 1 soundings nearest to the point, in order of squared
+ * distance and then index (FindNearest's repeated scans), weighted by
+ * inverse squared distance; a point on a sounding takes that sounding's
+ * depth, or the second's where several coincide with it (the scan returns at
+ * the second zero distance it meets).  The reference scans every sounding
+ * for every point; here soundings sit in a bucket grid searched in growing
+ * rings until no unvisited bucket can hold a nearer sounding. */
+struct gen_near { double d; int i; };
+static int gen_near_before(const struct gen_near *a, const struct gen_near *b) {
+  return a->d < b->d || (a->d == b->d && a->i < b->i);
+}
+static int gen_interp_depth(const double *xd, const double *yd, const double *zd, int Nd, const double *xq,
+                            const double *yq, double *zq, long Nq, int np) {
+  if (Nd < np) return 0;
+  double x0 = xd[0], x1 = xd[0], y0 = yd[0], y1 = yd[0];
+  for (int i = 1; i < Nd; i++) {
+    if (xd[i] < x0) x0 = xd[i];
+    if (xd[i] > x1) x1 = xd[i];
+    if (yd[i] < y0) y0 = yd[i];
+    if (yd[i] > y1) y1 = yd[i];
+  }
+  /* about two soundings per bucket */
+  const double w = x1 - x0 > 0 ? x1 - x0 : 1, h = y1 - y0 > 0 ? y1 - y0 : 1;
+  double side = sqrt(2.0 * w * h / Nd);
+  if (!(side > 0)) side = w > h ? w : h;
+  long nbx = (long)(w / side) + 1, nby = (long)(h / side) + 1;
+  while (nbx * nby > 4L * Nd + 16) { side *= 1.5; nbx = (long)(w / side) + 1; nby = (long)(h / side) + 1; }
+  int *start = (int *)xcalloc(nbx * nby + 1, sizeof(int)), *items = (int *)xcalloc(Nd, sizeof(int));
+  int *fillc = (int *)xcalloc(nbx * nby, sizeof(int));
+  long *bk = (long *)xcalloc(Nd, sizeof(long));
+  for (int i = 0; i < Nd; i++) {
+    long bx = (long)((xd[i] - x0) / side), by = (long)((yd[i] - y0) / side);
+    if (bx >= nbx) bx = nbx - 1;
+    if (by >= nby) by = nby - 1;
+    bk[i] = by * nbx + bx;
+    start[bk[i] + 1]++;
+  }
+  for (long b = 0; b < nbx * nby; b++) start[b + 1] += start[b];
+  for (int i = 0; i < Nd; i++) items[start[bk[i]] + fillc[bk[i]]++] = i;
+  struct gen_near best[65];
+  for (long n = 0; n < Nq; n++) {
+    const double qx = xq[n], qy = yq[n];
+    long cx = (long)floor((qx - x0) / side), cy = (long)floor((qy - y0) / side);
+    if (cx < 0) cx = 0;
+    if (cx >= nbx) cx = nbx - 1;
+    if (cy < 0) cy = 0;
+    if (cy >= nby) cy = nby - 1;
+    int m = 0, zeros = 0, zero1 = -1, zero2 = -1;
+    for (long r = 0;; r++) {
+      for (long by = cy - r; by <= cy + r; by++) {
+        if (by < 0 || by >= nby) continue;
+        for (long bx = cx - r; bx <= cx + r; bx++) {
+          if (bx < 0 || bx >= nbx) continue;
+          if (r > 0 && by != cy - r && by != cy + r && bx != cx - r && bx != cx + r) continue;
+          for (int t = start[by * nbx + bx]; t < start[by * nbx + bx + 1]; t++) {
+            const int i = items[t];
+            const struct gen_near c = {(xd[i] - qx) * (xd[i] - qx) + (yd[i] - qy) * (yd[i] - qy), i};
+            if (c.d == 0) {
+              zeros++;
+              if (zero1 < 0 || i < zero1) { zero2 = zero1; zero1 = i; }
+              else if (zero2 < 0 || i < zero2) zero2 = i;
+            }
+            if (m < np || gen_near_before(&c, &best[np - 1])) {
+              int j = m < np ? m++ : np - 1;
+              while (j > 0 && gen_near_before(&c, &best[j - 1])) { best[j] = best[j - 1]; j--; }
+              best[j] = c;
+            }
+          }
+        }
+      }
+      if (cx - r <= 0 && cy - r <= 0 && cx + r >= nbx - 1 && cy + r >= nby - 1) break;
+      if (m == np) {
+        /* the nearest a sounding outside the searched square can be */
+        const double lx = x0 + (cx - r) * side, hx = x0 + (cx + r + 1) * side;
+        const double ly = y0 + (cy - r) * side, hy = y0 + (cy + r + 1) * side;
+        if (qx > lx && qx < hx && qy > ly && qy < hy) {
+          double g = qx - lx;
+          if (hx - qx < g) g = hx - qx;
+          if (qy - ly < g) g = qy - ly;
+          if (hy - qy < g) g = hy - qy;
+          if (g * g > best[np - 1].d * (1 + 1e-9) + 1e-300) break;
+        }
+      }
+    }
+    if (zeros) {
+      zq[n] = zd[zeros >= 2 ? zero2 : zero1];
+      continue;
+    }
+    double z = 0, rs = 0;
+    for (int j = 0; j < np; j++) {
+      const int i = best[j].i;
+      const double dist = pow(xd[i] - qx, 2.0) + pow(yd[i] - qy, 2.0);
+      const double r0 = 1.0 / dist;
+      z += zd[i] * r0;
+      rs += r0;
+    }
+    zq[n] = z / rs;
+  }
+  free(start); free(items); free(fillc); free(bk);
+  return 1;
+}
+
 gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
   char datafile[1024];
   snprintf(datafile, sizeof datafile, "%s/suntans.dat", dir);
@@ -1406,8 +1510,12 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
   (void)dzsmall;
   double vratio = 0;
   if (correct != 0 && !gen_value(datafile, "VoronoiRatio", 0, 0, &vratio)) return NULL;
-  if (intdepth == 1) { gen_fail("IntDepth 1 (depth interpolation) is not rebuilt", NULL); return NULL; }
-  if (intdepth != 2 && !depth) { gen_fail("IntDepth 0 needs the depth function (ReturnDepth)", NULL); return NULL; }
+  double scaledepth = 0, scalefactor = 1;
+  if (intdepth == 1 && (!gen_value(datafile, "scaledepth", 0, 0, &scaledepth) ||
+                        !gen_value(datafile, "scaledepthfactor", 0, 0, &scalefactor)))
+    return NULL;
+  if (mf > 64) { gen_fail("maxFaces above 64", NULL); return NULL; }
+  if (intdepth != 2 && intdepth != 1 && !depth) { gen_fail("IntDepth 0 needs the depth function (ReturnDepth)", NULL); return NULL; }
   if (mf < 3 || Nkmax < 1) { gen_fail("maxFaces and Nkmax must be at least 3 and 1", NULL); return NULL; }
 
   char fpts[1100], fedg[1100], fcel[1100], fdep[1100], fcd[1100], fed[1100], fnod[1100], ftop[1100], fvs[1100];
@@ -1551,7 +1659,22 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
       for (long n = 0; n < Nc; n++) { next_field(f); next_field(f); mdv[n] = next_field(f); }
       fclose(f);
     } else {
-      for (long n = 0; n < Nc; n++) mdv[n] = depth(mxv[n], myv[n]);
+      if (intdepth == 1) {
+        const long Nd = file_lines(fdep);
+        if (Nd <= 0 || !(f = fopen(fdep, "r"))) { gen_fail("cannot read", fdep); break; }
+        double *xd = (double *)xcalloc(Nd, sizeof(double)), *yd = (double *)xcalloc(Nd, sizeof(double));
+        double *zd = (double *)xcalloc(Nd, sizeof(double));
+        for (long n = 0; n < Nd; n++) { xd[n] = next_field(f); yd[n] = next_field(f); zd[n] = fabs(next_field(f)); }
+        fclose(f);
+        f = NULL;
+        const int got = gen_interp_depth(xd, yd, zd, (int)Nd, mxv, myv, mdv, Nc, mf + 1);
+        free(xd); free(yd); free(zd);
+        if (!got) { gen_fail("fewer soundings than maxFaces + 1", fdep); break; }
+        if ((int)scaledepth)
+          for (long n = 0; n < Nc; n++) mdv[n] *= scalefactor;
+      } else {
+        for (long n = 0; n < Nc; n++) mdv[n] = depth(mxv[n], myv[n]);
+      }
       char fv[1200];
       snprintf(fv, sizeof fv, "%s-voro", fdep);
       if (!gen_write(fv, &f)) break;
@@ -1737,7 +1860,6 @@ gridT *sun_grid_generate(const char *dir, double (*depth)(double x, double y)) {
 
     /* metrics (Geometry, main/grid.c:3278-3563) */
     double xt[64], yt[64];
-    if (mf > 64) { gen_fail("maxFaces above 64", NULL); break; }
     for (long n = 0; n < Nc; n++) {
       for (int nf = 0; nf < g->nfaces[n]; nf++) { xt[nf] = xp[g->cells[n * mf + nf]]; yt[nf] = yp[g->cells[n * mf + nf]]; }
       g->Ac[n] = gen_area(xt, yt, g->nfaces[n]);

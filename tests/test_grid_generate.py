@@ -101,8 +101,8 @@ def test_generated_momentum_neighbours(tmp_path):
 
 
 def test_generate_refusals(tmp_path):
-    """IntDepth 1 is not rebuilt; IntDepth 0 needs the depth function; a
-    missing planform file is an error."""
+    """IntDepth 1 needs the scaledepth keys; IntDepth 0 needs the depth
+    function; a missing planform file is an error."""
     src = os.path.join(GOLD, "iwaves", "planform")
 
     def case(sub, edit=None, depth=return_depth):
@@ -179,7 +179,47 @@ def test_generate_correct_voronoi_matches_reference(name, ratio, tmp_path):
     both_ways(name, tmp_path, edit)
 
 
-def both_ways(name, tmp_path, edit=None):
+@NEEDS_REF
+@pytest.mark.parametrize("name", ["tides", "cavity", "cylinderquad"])
+def test_generate_interp_depth_matches_reference(name, tmp_path):
+    """IntDepth 1 (InterpDepth with Interp and FindNearest, main/grid.c:
+    3686-3733, main/util.c): scattered soundings, some exactly on Voronoi
+    points -- one of them twice, with different depths, where the
+    reference takes the second -- and scaledepth."""
+    src = os.path.join(EXAMPLES, EXAMPLE_GRIDS[name])
+    probe = tmp_path / "probe"
+    probe.mkdir()
+    copy_dir(src, probe)
+    xv = generate(str(probe)).get("xv"), generate(str(probe)).get("yv")
+    xs, ys = xv
+    rng = np.random.default_rng(17)
+    n = 3 * len(xs)
+    x = rng.uniform(xs.min() - 0.05 * np.ptp(xs), xs.max() + 0.05 * np.ptp(xs), n)
+    y = rng.uniform(ys.min() - 0.05 * np.ptp(ys), ys.max() + 0.05 * np.ptp(ys), n)
+    z = -rng.uniform(5.0, 50.0, n)
+    hits = rng.choice(len(xs), 20, replace=False)
+    x = np.r_[x, xs[hits], xs[hits[0]]]
+    y = np.r_[y, ys[hits], ys[hits[0]]]
+    z = np.r_[z, -rng.uniform(5.0, 50.0, 21)]
+    lines = "".join(f"{a!r} {b!r} {c!r}\n" for a, b, c in zip(x, y, z))
+
+    def edit(line):
+        for key, val in (("IntDepth", "1"), ("depth", "soundings.dat")):
+            if line.split()[:1] == [key]:
+                return f"{key}\t{val}\t# edited"
+        return line
+
+    def prepare(d):
+        (d / "soundings.dat").write_text(lines)
+        text = (d / "suntans.dat").read_text().splitlines()
+        text = [edit(l) for l in text]
+        text.insert(1, "scaledepth\t1\t# edited")
+        text.insert(1, "scaledepthfactor\t1.5\t# edited")
+        (d / "suntans.dat").write_text("\n".join(text) + "\n")
+    both_ways(name, tmp_path, prepare=prepare)
+
+
+def both_ways(name, tmp_path, edit=None, prepare=None):
     src = os.path.join(EXAMPLES, EXAMPLE_GRIDS[name])
     a, b = tmp_path / "ref", tmp_path / "ours"
     a.mkdir()
@@ -190,6 +230,9 @@ def both_ways(name, tmp_path, edit=None):
         for d in (a, b):
             lines = (d / "suntans.dat").read_text().splitlines()
             (d / "suntans.dat").write_text("\n".join(edit(l) for l in lines) + "\n")
+    if prepare:
+        prepare(a)
+        prepare(b)
     code = ("import ctypes as C, sys; L = C.CDLL(sys.argv[1]); L.ref_set_verbose(0); "
             "L.ref_grid_generate.argtypes = [C.c_char_p]; L.ref_grid_generate(sys.argv[2].encode())")
     subprocess.run([sys.executable, "-c", code, REF_SO, str(a)], check=True, cwd=a, capture_output=True)
