@@ -219,6 +219,21 @@ def test_generate_interp_depth_matches_reference(name, tmp_path):
     both_ways(name, tmp_path, prepare=prepare)
 
 
+@NEEDS_REF
+def test_generate_large_matches_reference(tmp_path):
+    """The 55,870-cell Monterey Bay planform (examples/tides/datasets/
+    big3dgrid with the example's suntans.dat): about 15 s in the
+    reference, whose partition is quadratic."""
+    src = os.path.join(EXAMPLES, "tides", "datasets", "big3dgrid")
+    if not os.path.isdir(src):
+        pytest.skip("big3dgrid is not in this copy of the examples")
+
+    def prepare(d):
+        copy_dir(src, d)
+        shutil.copyfile(os.path.join(EXAMPLES, "tides", "rundata", "suntans.dat"), d / "suntans.dat")
+    both_ways("tides", tmp_path, prepare=prepare)
+
+
 def both_ways(name, tmp_path, edit=None, prepare=None):
     src = os.path.join(EXAMPLES, EXAMPLE_GRIDS[name])
     a, b = tmp_path / "ref", tmp_path / "ours"
