@@ -143,20 +143,39 @@ __device__ __forceinline__ void continuity_tile(const Geo &G, const StepArgs &A,
   for (int k = 0; k < kt; k++) at32(A.w, base + (uint32_t)k * 64u) = 0;
   at32(A.w, base + (uint32_t)n * 64u) = 0;
   double wk1 = 0, w2k1 = at32(A.wtmp2, base + (uint32_t)n * 64u);
-  for (int k = n - 1; k >= kt; k--) {
-    const uint32_t k64 = (uint32_t)k * 64u;
-    const double w2k = at32(A.wtmp2, base + k64);
-    double v = wk1 - tf * (w2k - w2k1);
+  // four levels' gathers and face terms first (independent of the upward
+  // chain), then the chain subtracts them in the reference's order
+  constexpr int GL = 4;
+  for (int kb = n - 1; kb >= kt; kb -= GL) {
+    double w2[GL], tm[GL][NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (f < nf && k < F.nke[f]) {
-        const uint32_t ei = F.eb[f] + k64;
-        v -= (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei)) * F.df[f] * F.nrm[f] / Ac / th * at32(A.dzf, ei);
+    for (int j = 0; j < GL; j++) {
+      const int k = kb - j;
+      if (k < kt) break;
+      const uint32_t k64 = (uint32_t)k * 64u;
+      w2[j] = at32(A.wtmp2, base + k64);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        tm[j][f] = 0;
+        if (f < nf && k < F.nke[f]) {
+          const uint32_t ei = F.eb[f] + k64;
+          tm[j][f] = (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei)) * F.df[f] * F.nrm[f] / Ac / th *
+                     at32(A.dzf, ei);
+        }
       }
     }
-    at32(A.w, base + k64) = v;
-    wk1 = v;
-    w2k1 = w2k;
+#pragma unroll
+    for (int j = 0; j < GL; j++) {
+      const int k = kb - j;
+      if (k < kt) break;
+      double v = wk1 - tf * (w2[j] - w2k1);
+#pragma unroll
+      for (int f = 0; f < NF; f++)
+        if (f < nf && k < F.nke[f]) v -= tm[j][f];
+      at32(A.w, base + (uint32_t)k * 64u) = v;
+      wk1 = v;
+      w2k1 = w2[j];
+    }
   }
 }
 
