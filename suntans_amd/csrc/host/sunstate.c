@@ -966,7 +966,7 @@ static long file_lines(const char *path) {
   if (!f) return -1;
   long n = 0;
   int c;
-  while ((c = fgetc(f)) != EOF)
+  while ((c = getc_unlocked(f)) != EOF)
     if (c == '\n') n++;
   fclose(f);
   return n;
@@ -979,11 +979,11 @@ static double next_field(FILE *f) {
   char w[256];
   for (;;) {
     int c, i = 0;
-    while ((c = fgetc(f)) == ' ' || c == '\t' || c == '\n' || c == '\r') {}
+    while ((c = getc_unlocked(f)) == ' ' || c == '\t' || c == '\n' || c == '\r') {}
     if (c == EOF) return 0;
     while (c != EOF && c != ' ' && c != '\t' && c != '\n' && c != '\r') {
       if (i < 255) w[i++] = (char)c;
-      c = fgetc(f);
+      c = getc_unlocked(f);
     }
     w[i] = 0;
     int ok = 1;
