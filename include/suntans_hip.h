@@ -2,7 +2,7 @@
  * suntans_hip.h -- drop-in replacements for the SUNTANS elliptic solvers and
  * TVD scalar transport, with the reference's own C signatures, running on one
  * MI355X (numprocs=1, the whole grid on one GPU).  Link libsuntans.so in
- * place of the reference's definitions; see INTEGRATION.md.
+ * place of the reference's definitions of these functions.
  *
  * Semantics follow the reference function each declaration replaces.  The
  * caller owns every array; results are written to the same rows the

@@ -1,7 +1,7 @@
 """The oracle against the real reference, compiled from its own sources
 (oracle/_ref/libsunref.so).  Bit-identical results and identical iteration
-counts are required.  Skipped where the reference tree was not available at
-build time (the GPU box); there the golden fixtures pin the oracle instead
+counts are required.  Skipped on a machine where the reference tree was not
+available at build time; there the golden fixtures pin the oracle instead
 (test_oracle_golden.py)."""
 import numpy as np
 import pytest
