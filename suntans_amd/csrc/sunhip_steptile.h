@@ -72,20 +72,39 @@ __device__ __forceinline__ void qsource_tile(const Geo &G, const StepArgs &A, in
   F.load(G, s, nf);
   const double tf = (1 - A.theta) / A.theta, Ac = G.Ac[s], dt = A.dt;
   double wk = at32(A.w, base + (uint32_t)kt * 64u), w2k = at32(A.wtmp2, base + (uint32_t)kt * 64u);
-  for (int k = kt; k < n; k++) {
-    const uint32_t k64 = (uint32_t)k * 64u, idx = base + k64;
-    const double wk1 = at32(A.w, idx + 64u), w2k1 = at32(A.wtmp2, idx + 64u);
-    double v = Ac * (wk - wk1) + tf * Ac * (w2k - w2k1);
+  // four levels' loads and face terms first, then each level's sum in the
+  // reference's order
+  constexpr int GL = 4;
+  for (int kb = kt; kb < n; kb += GL) {
+    double w1[GL], w21[GL], tm[GL][NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (f < nf && k < F.nke[f]) {
-        const uint32_t ei = F.eb[f] + k64;
-        v += (at32(A.u, ei) + tf * at32(A.utmp2, ei)) * at32(A.dzf, ei) * F.nrm[f] * F.df[f];
+    for (int j = 0; j < GL; j++) {
+      const int k = kb + j;
+      if (k >= n) break;
+      const uint32_t k64 = (uint32_t)k * 64u, idx = base + k64;
+      w1[j] = at32(A.w, idx + 64u);
+      w21[j] = at32(A.wtmp2, idx + 64u);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        tm[j][f] = 0;
+        if (f < nf && k < F.nke[f]) {
+          const uint32_t ei = F.eb[f] + k64;
+          tm[j][f] = (at32(A.u, ei) + tf * at32(A.utmp2, ei)) * at32(A.dzf, ei) * F.nrm[f] * F.df[f];
+        }
       }
     }
-    at32(A.src, idx) = v / dt;
-    wk = wk1;
-    w2k = w2k1;
+#pragma unroll
+    for (int j = 0; j < GL; j++) {
+      const int k = kb + j;
+      if (k >= n) break;
+      double v = Ac * (wk - w1[j]) + tf * Ac * (w2k - w21[j]);
+#pragma unroll
+      for (int f = 0; f < NF; f++)
+        if (f < nf && k < F.nke[f]) v += tm[j][f];
+      at32(A.src, base + (uint32_t)k * 64u) = v / dt;
+      wk = w1[j];
+      w2k = w21[j];
+    }
   }
 }
 
