@@ -80,6 +80,14 @@ typedef struct {
 const char *sunhip_error(void);
 int sunhip_device_count(void);
 
+/* Checks a topology, and the extents X unless NULL, without touching a
+ * device: every index the kernels follow in range (neigh and grad in
+ * [-1,Nc), face in [0,Ne), cellp and edgep permutations, celldist and
+ * edgedist nondecreasing splits, 1 <= Nk <= Nkmax, 0 <= Nkc <= Nkmax;
+ * ctop and ctopold in [0,Nk], Nke and etop in [0,Nkc]).  0, or -1 with
+ * sunhip_error() naming the first bad entry.  sunhip_grid_create and
+ * sunhip_set_extents run the same checks first. */
+int sunhip_validate(const sunhip_topology *topo, const sunhip_extents *ext);
 int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *topo, int device);
 int sunhip_grid_destroy(sunhip_grid *g);
 int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *ext);

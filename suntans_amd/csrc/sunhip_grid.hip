@@ -180,8 +180,9 @@ int sunhip_device_count(void) {
   return n;
 }
 
-int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) {
-  *out = nullptr;
+// Host-side checks of everything the kernels index with: a value outside its
+// range would make a kernel read or write outside a device array.
+static int validate_topology(const sunhip_topology *T) {
   if (!T || T->Nc <= 0 || T->Ne <= 0 || T->maxfaces <= 0) {
     set_error("sunhip_grid_create: empty topology");
     return -1;
@@ -190,6 +191,88 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
     set_error("sunhip_grid_create: maxfaces=%d exceeds %d", T->maxfaces, SUNHIP_MAXF);
     return -1;
   }
+  const int Nc = T->Nc, Ne = T->Ne, mf = T->maxfaces;
+  for (int i = 0; i < Nc; i++) {
+    if (T->Nk[i] < 1 || T->Nk[i] > T->Nkmax || T->nfaces[i] < 1 || T->nfaces[i] > mf) {
+      set_error("sunhip_grid_create: cell %d has Nk=%d (Nkmax %d) nfaces=%d (maxfaces %d)", i, T->Nk[i], T->Nkmax,
+                T->nfaces[i], mf);
+      return -1;
+    }
+    for (int f = 0; f < T->nfaces[i]; f++) {
+      const int nb = T->neigh[(size_t)i * mf + f], e = T->face[(size_t)i * mf + f];
+      if (nb < -1 || nb >= Nc || e < 0 || e >= Ne) {
+        set_error("sunhip_grid_create: cell %d face %d has neigh=%d face=%d outside [-1,%d) and [0,%d)", i, f, nb, e,
+                  Nc, Ne);
+        return -1;
+      }
+    }
+  }
+  for (int e = 0; e < Ne; e++) {
+    const int a = T->grad[2 * (size_t)e], b = T->grad[2 * (size_t)e + 1];
+    if (a < -1 || a >= Nc || b < -1 || b >= Nc || (a < 0 && b < 0) || T->Nkc[e] < 0 || T->Nkc[e] > T->Nkmax) {
+      set_error("sunhip_grid_create: edge %d has grad=(%d,%d) Nkc=%d (Nc %d, Nkmax %d)", e, a, b, T->Nkc[e], Nc,
+                T->Nkmax);
+      return -1;
+    }
+  }
+  std::vector<char> seen((size_t)std::max(Nc, Ne), 0);
+  for (int i = 0; i < Nc; i++) {
+    const int c = T->cellp[i];
+    if (c < 0 || c >= Nc || seen[c]) {
+      set_error("sunhip_grid_create: cellp is not a permutation of the cells (entry %d = %d)", i, c);
+      return -1;
+    }
+    seen[c] = 1;
+  }
+  std::fill(seen.begin(), seen.end(), 0);
+  for (int j = 0; j < Ne; j++) {
+    const int e = T->edgep[j];
+    if (e < 0 || e >= Ne || seen[e]) {
+      set_error("sunhip_grid_create: edgep is not a permutation of the edges (entry %d = %d)", j, e);
+      return -1;
+    }
+    seen[e] = 1;
+  }
+  for (int k = 0; k < 3; k++)
+    if (T->celldist[k] < 0 || T->celldist[k] > Nc || (k && T->celldist[k] < T->celldist[k - 1])) {
+      set_error("sunhip_grid_create: celldist is not a nondecreasing split of [0,%d]", Nc);
+      return -1;
+    }
+  for (int k = 0; k < 6; k++)
+    if (T->edgedist[k] < 0 || T->edgedist[k] > Ne || (k && T->edgedist[k] < T->edgedist[k - 1])) {
+      set_error("sunhip_grid_create: edgedist is not a nondecreasing split of [0,%d]", Ne);
+      return -1;
+    }
+  return 0;
+}
+
+// The moving extents against the topology's column sizes.
+static int validate_extents(int Nc, int Ne, const int *Nk, const int *Nkc, const sunhip_extents *X) {
+  for (int c = 0; c < Nc; c++) {
+    const int ct = X->ctop[c], cto = X->ctopold ? X->ctopold[c] : ct;
+    if (ct < 0 || ct > Nk[c] || cto < 0 || cto > Nk[c]) {
+      set_error("sunhip_set_extents: cell %d has ctop=%d ctopold=%d outside [0,Nk=%d]", c, ct, cto, Nk[c]);
+      return -1;
+    }
+  }
+  for (int e = 0; e < Ne; e++) {
+    const int nke = X->Nke[e], et = X->etop ? X->etop[e] : 0;
+    if (nke < 0 || nke > Nkc[e] || et < 0 || et > Nkc[e]) {
+      set_error("sunhip_set_extents: edge %d has Nke=%d etop=%d outside [0,Nkc=%d]", e, nke, et, Nkc[e]);
+      return -1;
+    }
+  }
+  return 0;
+}
+
+int sunhip_validate(const sunhip_topology *T, const sunhip_extents *X) {
+  if (int rc = validate_topology(T)) return rc;
+  return X ? validate_extents(T->Nc, T->Ne, T->Nk, T->Nkc, X) : 0;
+}
+
+int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) {
+  *out = nullptr;
+  if (int rc = validate_topology(T)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     set_error("sunhip_grid_create: no HIP device");
@@ -216,12 +299,6 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   g->edgep.assign(T->edgep, T->edgep + Ne);
   g->celldist.assign(T->celldist, T->celldist + 3);
   g->edgedist.assign(T->edgedist, T->edgedist + 6);
-  for (int i = 0; i < Nc; i++)
-    if (T->Nk[i] < 1 || T->nfaces[i] < 1 || T->nfaces[i] > mf) {
-      set_error("sunhip_grid_create: cell %d has Nk=%d nfaces=%d", i, T->Nk[i], T->nfaces[i]);
-      delete g;
-      return -1;
-    }
 
   // --- cell order: Morton curve of the Voronoi points
   double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
@@ -521,6 +598,7 @@ int sunhip_get_extents(sunhip_grid *g, int *ctop, int *ctopold, int *etop, int *
 }
 
 int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
+  if (int rc = validate_extents(g->Nc, g->Ne, g->Nk.data(), g->Nkc.data(), X)) return rc;
   HIPCHK(hipSetDevice(g->device));
   const int NS = g->NS, NES = g->NES;
   std::vector<int> ct(NS, 0), cto(NS, 0), nke(NES, 0), et(NES, 0);
@@ -528,19 +606,11 @@ int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
     int c = g->cell_of_slot[s];
     ct[s] = X->ctop[c];
     cto[s] = X->ctopold ? X->ctopold[c] : X->ctop[c];
-    if (ct[s] < 0 || ct[s] > g->Nk[c]) {
-      set_error("sunhip_set_extents: cell %d has ctop=%d outside [0,Nk=%d]", c, ct[s], g->Nk[c]);
-      return -1;
-    }
   }
   for (int s = 0; s < g->Ne; s++) {
     int e = g->edge_of_slot[s];
     nke[s] = X->Nke[e];
     et[s] = X->etop ? X->etop[e] : 0;
-    if (nke[s] < 0 || nke[s] > g->Nkc[e]) {
-      set_error("sunhip_set_extents: edge %d has Nke=%d outside [0,Nkc=%d]", e, nke[s], g->Nkc[e]);
-      return -1;
-    }
   }
   HIPCHK(hipMemcpyAsync(g->d_ctop, ct.data(), NS * sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->d_ctopold, cto.data(), NS * sizeof(int), hipMemcpyHostToDevice, g->stream));

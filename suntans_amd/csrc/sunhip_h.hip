@@ -288,7 +288,11 @@ int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, dou
 // Mixed precision free surface: FP64 true residual r = b - L h (after the
 // reference's boundary lift with h = 0 inside), FP32 inner Jacobi-PCG on
 // L e = r to inner_tol (adaptive toward the target), h += e in FP64, until
-// sqrt(r.r / r0.r0) < epsilon.
+// sqrt(r.r / eps0) < epsilon.  eps0 follows the reference's rule
+// (main/phys.c:3150-3156): r0.r0, except that without the preconditioner
+// and without resnorm it is 1, an absolute test on the residual norm.  The
+// inner solves always stop on a reduction relative to their own first
+// residual.
 int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
                          double inner_tol, int max_outer, sunhip_cg_stats *st, int *inner_iters) {
   HIPCHK(hipSetDevice(g->device));
@@ -310,7 +314,7 @@ int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double gra
   k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, B.h->d);
   HIPCHK(hipGetLastError());
   sunhip_cg_params inner = *prm;
-  inner.resnorm = 0;
+  inner.resnorm = 1;  // relative inner test for both preconditioners
   inner.precond = prm->precond;
   HSolver<float> S{g->d_ms, c32, f32, x32, r32, z32, t32, pa, pb};
   HVec<double, float> R{};
@@ -325,7 +329,7 @@ int sunhip_h_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double gra
     HIPCHK(hipMemcpyAsync(g->h_sc, g->d_mo, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
     eps = g->h_sc->pz;
-    if (eps0 < 0) eps0 = eps;
+    if (eps0 < 0) eps0 = (prm->precond == 0 && !prm->resnorm) ? 1.0 : eps;
     const double ratio = eps == 0 ? 0.0 : sqrt(eps / eps0);
     if (eps == 0 || ratio < prm->epsilon) { converged = 1; break; }
     if (outer >= max_outer) break;

@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         L.sunhip_error.restype = C.c_char_p
         L.sunhip_grid_create.argtypes = [C.POINTER(vp), C.POINTER(Topology), C.c_int]
+        L.sunhip_validate.argtypes = [C.POINTER(Topology), C.POINTER(Extents)]
         L.sunhip_grid_destroy.argtypes = [vp]
         L.sunhip_set_extents.argtypes = [vp, C.POINTER(Extents)]
         L.sunhip_grid_bytes.argtypes = [vp]
@@ -172,21 +173,52 @@ def device_count() -> int:
     return int(lib().sunhip_device_count())
 
 
+def topology(st, override=None):
+    """The sunhip_topology of a State, with the arrays it points to (keep
+    them alive while the structure is used).  override replaces named
+    arrays (tests of the input checks)."""
+    ints = {n: np.ascontiguousarray(st.ints(n), np.int32) for n in
+            ("nfaces", "neigh", "face", "normal", "grad", "mark", "Nk", "Nkc", "cellp", "celldist",
+             "edgep", "edgedist")}
+    reals = {n: np.ascontiguousarray(st.get(n)) for n in ("Ac", "df", "dg", "xv", "yv")}
+    for n, a in (override or {}).items():
+        if n in ints:
+            ints[n] = np.ascontiguousarray(a, np.int32)
+        else:
+            reals[n] = np.ascontiguousarray(a, np.float64)
+    T = Topology(st.Nc, st.Ne, st.scalar("maxfaces"), st.scalar("Nkmax"))
+    for n, a in ints.items():
+        setattr(T, n, native.iptr(a))
+    for n, a in reals.items():
+        setattr(T, n, native.dptr(a))
+    return T, (ints, reals)
+
+
+def extents(st, override=None):
+    """The sunhip_extents of a State and the arrays it points to."""
+    a = {n: np.ascontiguousarray(st.ints(n), np.int32) for n in ("ctop", "ctopold", "Nke", "etop")}
+    for n, v in (override or {}).items():
+        a[n] = np.ascontiguousarray(v, np.int32)
+    return Extents(*(native.iptr(a[n]) for n in ("ctop", "ctopold", "Nke", "etop"))), a
+
+
+def validate(st, with_extents=True, override=None):
+    """sunhip_validate on a State's topology (and extents): 0 or raises
+    RuntimeError with the library's message.  Needs no device."""
+    override = override or {}
+    T, keep = topology(st, {k: v for k, v in override.items() if k not in ("ctop", "ctopold", "Nke", "etop")})
+    X, keepx = extents(st, {k: v for k, v in override.items() if k in ("ctop", "ctopold", "Nke", "etop")})
+    check(lib().sunhip_validate(C.byref(T), C.byref(X) if with_extents else None), "sunhip_validate")
+    return 0
+
+
 class DeviceGrid:
     """One grid on one GPU (topology from a :class:`State`)."""
 
     def __init__(self, st, device: int = 0):
         L = lib()
         self._keep = {}
-        ints = {n: np.ascontiguousarray(st.ints(n), np.int32) for n in
-                ("nfaces", "neigh", "face", "normal", "grad", "mark", "Nk", "Nkc", "cellp", "celldist",
-                 "edgep", "edgedist")}
-        reals = {n: np.ascontiguousarray(st.get(n)) for n in ("Ac", "df", "dg", "xv", "yv")}
-        T = Topology(st.Nc, st.Ne, st.scalar("maxfaces"), st.scalar("Nkmax"))
-        for n, a in ints.items():
-            setattr(T, n, native.iptr(a))
-        for n, a in reals.items():
-            setattr(T, n, native.dptr(a))
+        T, keep = topology(st)
         self.h = C.c_void_p()
         check(L.sunhip_grid_create(C.byref(self.h), C.byref(T), device), "sunhip_grid_create")
         self.Nc, self.Ne = st.Nc, st.Ne
@@ -206,8 +238,7 @@ class DeviceGrid:
 
     def set_extents(self, st=None):
         st = st or self.st
-        a = {n: np.ascontiguousarray(st.ints(n), np.int32) for n in ("ctop", "ctopold", "Nke", "etop")}
-        X = Extents(*(native.iptr(a[n]) for n in ("ctop", "ctopold", "Nke", "etop")))
+        X, keep = extents(st)
         check(lib().sunhip_set_extents(self.h, C.byref(X)), "sunhip_set_extents")
 
     def sizes(self):

@@ -152,16 +152,20 @@ class Server:
         self.evictions += 1
 
     def _make_resident(self, s: Sim) -> bool:
+        """Make s resident, evicting idle rebuildable simulations (least
+        recently used first) while the budget is short.  False when it does
+        not fit now; an error when it never can (it cannot be rebuilt)."""
         if s.dg is not None:
             return True
         if s.make_grid is None:
-            return False
+            raise RuntimeError(f"simulation {s.sid} is not resident and has no host state to rebuild it from")
         need = s.est_bytes
         if need > self.budget:
             raise RuntimeError(f"simulation {s.sid} needs {need / 1e9:.1f} GB, over the {self.budget / 1e9:.1f} GB "
                                "budget")
         while self._resident_bytes() + need > self.budget:
-            idle = [x for x in self.sims.values() if x.dg is not None and not x.busy]
+            # only simulations that can be rebuilt may leave the device
+            idle = [x for x in self.sims.values() if x.dg is not None and not x.busy and x.make_grid is not None]
             if not idle:
                 return False
             self._evict(min(idle, key=lambda x: x.last_used))
@@ -216,6 +220,7 @@ class Server:
                 rid += 1
                 i += 1
             # admission between chunks: FIFO, one request per simulation at a time
+            admitted = 0
             for r in list(queue):
                 if len(inflight) >= self.max_slots:
                     break
@@ -229,6 +234,13 @@ class Server:
                 r.t_admit = clock()
                 inflight[2 * r.rid] = r
                 self._start_h(r, 2 * r.rid)
+                admitted += 1
+            if queue and not inflight and not admitted:
+                # every simulation is idle, so nothing can free memory for the
+                # queued requests: they would wait forever
+                raise RuntimeError(f"request for simulation {queue[0].sid} cannot be placed within the "
+                                   f"{self.budget / 1e9:.1f} GB budget (resident simulations that cannot be "
+                                   "rebuilt hold it)")
             if inflight:
                 self.batch.advance(poll_every)
                 for tag, kind, st in self.batch.poll():

@@ -1,0 +1,197 @@
+"""The large-solve workload (1000x500 quads as triangles, 8 seeded Gaussian
+bumps, up to 100 levels of 2 m, source = ComputeQSource of seeded smooth u
+and w) built at reduced planform size with the same bathymetry statistics,
+against the oracle.  The kernel variants the full size selects by default
+(the in-wave neighbour exchange of the operator, "xchg" 2, and the
+register-held top of the preconditioner, "prec_head" 24 in FP64 and 16 in
+FP32) are forced here, since the reduced grid would otherwise pick the small
+defaults; the defaults are run too.
+
+Tolerances (FP64 pressure and free-surface solves, as the reference's CG
+stopping rule fixes them): q and h within 1e-10 relative 2-norm of the
+oracle, iteration counters within +-1; exactly n PCG iterations within
+1e-12.  The mixed form: the FP64 true residual (oracle operator) below
+1e-10 relative to the initial one, and the error against a tight
+(epsilon 1e-14) oracle solve at most 1e-8; the measured errors are printed.
+"""
+import numpy as np
+import pytest
+
+from suntans_amd import cases, device
+
+import _impls
+
+NX, NY = 160, 80
+FULL = (1000, 500, 5.0e7)
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+_CACHE = {}
+
+
+def large(seed=0):
+    """The large-solve case at NX x NY quads, mean depth as at full size."""
+    if seed not in _CACHE:
+        tN = FULL[2] * (NX * NY) / (FULL[0] * FULL[1])
+        _CACHE[seed] = cases.large_solve(seed=seed, nx=NX, ny=NY, target_N=tN)
+    return _CACHE[seed].clone()
+
+
+_SOLVED = {}
+
+
+def oracle_q(eps, seed=0):
+    """The oracle's CGSolveQ (qprecond 2) of the case at tolerance eps:
+    (q, iteration counter)."""
+    if (eps, seed) not in _SOLVED:
+        o = _impls.oracle()
+        a = large(seed)
+        a.set_prop(qprecond=2, qepsilon=eps, qmaxiters=20000, resnorm=0)
+        o.cgsolveq(a)
+        _SOLVED[(eps, seed)] = (a.get("qc"), o.stats("q")[0])
+    return _SOLVED[(eps, seed)]
+
+
+def _dev(st, tuning):
+    dg = device.DeviceGrid(st)
+    for n, f in (("q", "qc"), ("src", "stmp"), ("dzz", "dzz"), ("dzf", "dzf"), ("D", "D")):
+        dg.upload(n, st.get(f))
+    for k, v in tuning.items():
+        dg.tune(k, v)
+    return dg
+
+
+TUNINGS = [("default", {}), ("full_size_kernels", {"xchg": 2, "prec_head": 24}),
+           ("xchg3_head16", {"xchg": 3, "prec_head": 16})]
+
+
+def test_large_case_shape():
+    """The reduced case keeps the full-size statistics: ragged columns
+    between 5 and 100 levels, about 50 levels on average."""
+    st = large()
+    nk = st.ints("Nk")
+    assert st.Nc == 2 * NX * NY
+    assert nk.min() >= 5 and nk.max() <= 100 and 40 <= nk.mean() <= 60
+    assert len(np.unique(nk)) > 50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tname,tuning", TUNINGS, ids=[t[0] for t in TUNINGS])
+def test_large_q_solve_fp64(gpu_available, tname, tuning):
+    a = large()
+    qo, no = oracle_q(1e-10)
+    dg = _dev(a, tuning)
+    r = dg.q_solve(maxiters=2000, precond=2, epsilon=1e-10)
+    q = dg.download("q", len(qo))
+    dg.close()
+    m = a.wet_mask()
+    err = relerr(q[m], qo[m])
+    print(f"large {NX}x{NY} N={a.N} {tname}: iterations oracle={no} device={r['niters']} rel err={err:.3e}")
+    assert r["converged"] == 1
+    assert abs(r["niters"] - no) <= 1
+    assert err <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 3])
+def test_large_q_fixed_iterations(gpu_available, n):
+    o = _impls.oracle()
+    a = large()
+    a.set_prop(qprecond=2, qepsilon=0.0, qmaxiters=n)
+    b = a.clone()
+    o.cgsolveq(a)
+    dg = _dev(b, {"xchg": 2, "prec_head": 24})
+    r = dg.q_solve(maxiters=n, precond=2, epsilon=0.0, fixed=n)
+    q = dg.download("q", len(a.get("qc")))
+    dg.close()
+    m = a.wet_mask()
+    err = relerr(q[m], a.get("qc")[m])
+    print(f"large fixed {n}: rel err {err:.3e}")
+    assert o.stats("q")[0] == r["niters"] == n
+    assert err <= 1e-12
+
+
+@pytest.mark.gpu
+def test_large_q_dropin(gpu_available):
+    """The reference signature (libsuntans.so) on the same case."""
+    d = _impls.dropin()
+    qo, no = oracle_q(1e-10, seed=1)
+    b = large(1)
+    b.set_prop(qprecond=2, qepsilon=1e-10, qmaxiters=2000, resnorm=0)
+    d.cgsolveq(b)
+    m = b.wet_mask()
+    err = relerr(b.get("qc")[m], qo[m])
+    print(f"large drop-in: iterations oracle={no} device={d.stats('q')[0]} rel err={err:.3e}")
+    assert abs(d.stats("q")[0] - no) <= 1
+    assert err <= 1e-10
+
+
+def _q_residual(o, st, q):
+    s = st.clone()
+    o.qcoef(s)
+    s.set("uold", q)
+    o.operatorq(s, x="uold", y="stmp2")
+    return (st.get("stmp") - s.get("stmp2"))[st.wet_mask()]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("inner_tol", [1e-4, 1e-3])
+@pytest.mark.parametrize("head", [16, -1])
+def test_large_q_mixed(gpu_available, inner_tol, head):
+    o = _impls.oracle()
+    st = large()
+    st.set_prop(qprecond=2, resnorm=0)
+    q0 = st.get("qc")
+    dg = _dev(st, {"xchg": 2, "prec_head": head})
+    r = dg.q_solve_mixed(epsilon=1e-10, inner_tol=inner_tol, max_outer=30)
+    q = dg.download("q", len(q0))
+    dg.close()
+    assert r["converged"] == 1, r
+    res = np.linalg.norm(_q_residual(o, st, q)) / np.linalg.norm(_q_residual(o, st, q0))
+    qt, _ = oracle_q(1e-14)
+    q64, _ = oracle_q(1e-10)
+    m = st.wet_mask()
+    e_mix = relerr(q[m], qt[m])
+    e_64 = relerr(q64[m], qt[m])
+    print(f"large mixed inner_tol={inner_tol} head={head}: outer={r['niters']} inner={r['inner_iters']} "
+          f"true resid={res:.3e}; error vs tight oracle: mixed {e_mix:.3e}, oracle FP64 {e_64:.3e}; "
+          f"mixed vs oracle FP64 {relerr(q[m], q64[m]):.3e}")
+    assert res < 1e-10 * (1 + 1e-3)
+    assert e_mix <= 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hprecond", [1, 0])
+def test_large_h_solve(gpu_available, hprecond):
+    """The free-surface solve of the workload: D = depth-integrated face
+    heights (the inviscid closed form), seeded smooth right-hand side; FP64
+    against the oracle, and the mixed form against a tight oracle solve."""
+    o, d = _impls.oracle(), _impls.dropin()
+    a = large()
+    a.set("D", cases.depth_integrated_D(a))
+    a.set_prop(hprecond=hprecond, epsilon=1e-10, maxiters=1000, resnorm=0)
+    b, c = a.clone(), a.clone()
+    o.cgsolve(a)
+    d.cgsolve(b)
+    no, nd = o.stats("h")[0], d.stats("h")[0]
+    err = relerr(b.get("h"), a.get("h"))
+    tight = c.clone()
+    tight.set_prop(epsilon=1e-14, maxiters=20000)
+    o.cgsolve(tight)
+    dg = device.DeviceGrid(c)
+    for n in ("h", "htmp", "D"):
+        dg.upload(n, c.get(n))
+    r = dg.h_solve_mixed(c.get_prop("grav"), c.get_prop("theta"), c.get_prop("dt"), epsilon=1e-10,
+                         inner_tol=1e-4, precond=hprecond, max_outer=30)
+    hm = dg.download("h", c.Nc)
+    dg.close()
+    e_mix = relerr(hm, tight.get("h"))
+    print(f"large h hprecond={hprecond}: iterations oracle={no} device={nd} rel err={err:.3e}; "
+          f"mixed outer={r['niters']} inner={r['inner_iters']} err vs tight {e_mix:.3e}")
+    assert abs(no - nd) <= 1
+    assert err <= 1e-10
+    assert r["converged"] == 1
+    assert e_mix <= 1e-8

@@ -87,7 +87,7 @@ def test_h_mixed(gpu_available, name, make, hprecond):
     o = _impls.oracle()
     st = make()
     st.set("D", cases.depth_integrated_D(st))
-    st.set_prop(hprecond=hprecond)
+    st.set_prop(hprecond=hprecond, resnorm=0)
     comp = np.zeros(st.Nc, bool)
     cd = st.ints("celldist")
     comp[cd[0]:cd[1]] = True
@@ -101,15 +101,21 @@ def test_h_mixed(gpu_available, name, make, hprecond):
     h = dg.download("h", st.Nc)
     dg.close()
     assert r["converged"] == 1, r
-    res = np.linalg.norm(h_residual(o, st, h)[comp]) / np.linalg.norm(h_residual(o, st, h0)[comp])
+    # the reference's stopping rule (main/phys.c:3150-3156): relative to the
+    # first residual with the preconditioner, absolute without it (resnorm 0)
+    r_end = np.linalg.norm(h_residual(o, st, h)[comp])
+    res = r_end if hprecond == 0 else r_end / np.linalg.norm(h_residual(o, st, h0)[comp])
     tight = st.clone()
     tight.set_prop(epsilon=1e-14, maxiters=20000)
     o.cgsolve(tight)
+    f64 = st.clone()
+    o.cgsolve(f64)
     e_mix = relerr(h[comp], tight.get("h")[comp])
+    e_64 = relerr(f64.get("h")[comp], tight.get("h")[comp])
     print(f"{name} hprecond={hprecond}: outer={r['niters']} inner={r['inner_iters']} true resid={res:.3e} "
-          f"err vs tight={e_mix:.3e}")
+          f"err vs tight={e_mix:.3e} (oracle FP64 {e_64:.3e})")
     assert res < EPS * (1 + 1e-3)
-    assert e_mix <= 1e-8
+    assert e_mix <= max(1e-8, 100 * e_64)
 
 
 @pytest.mark.parametrize("name,make", MAKERS, ids=[m[0] for m in MAKERS])
