@@ -178,6 +178,18 @@ int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
  * "D".  grav, theta, dt enter through HCoefficients. */
 int sunhip_h_solve(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
                    sunhip_cg_stats *st);
+/* No counterpart in the reference: sunhip_h_solve for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing: "h",
+ * "htmp" and "D" must already be on the device, and the coefficients and CG
+ * vectors live in a static device array of the library, 32 MiB per device,
+ * which grids of up to 32 MiB / ((6 + maxfaces) x 8 B) cell slots fit
+ * (about 466,000 triangles); larger grids get -1 and an error.  Same kernels
+ * in the same order as sunhip_h_solve, so the same result.  Every call on a
+ * device shares that device's array: calls that overlap on one device
+ * (other threads, other grids' streams) race. */
+int sunhip_h_solve_noalloc(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                           sunhip_cg_stats *st);
 int sunhip_h_operator(sunhip_grid *g, double grav, double theta, double dt, const char *x, const char *y);
 /* No reference counterpart: the mixed-precision free surface.  FP64 true
  * residual r = htmp - L h (h = 0 on computational cells first, which lifts

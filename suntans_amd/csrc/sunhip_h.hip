@@ -11,10 +11,19 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "sunhip_solvers.h"
 
 using namespace sunhip;
+
+// Scratch of the allocation-free free-surface solve (sunhip_h_solve_noalloc):
+// the cell and face coefficients and the five CG vectors of one grid,
+// (6 + maxfaces) doubles per cell slot.  Each device that runs the solve
+// holds its own copy for the life of the process; calls on one device share
+// it.
+#define SUNHIP_NOALLOC_BYTES (32u << 20)
+static __device__ __align__(256) double scratch[SUNHIP_NOALLOC_BYTES / sizeof(double)];
 
 template <int NV>
 __device__ void h_partials(const double (&v)[NV], double *part) {
@@ -267,6 +276,93 @@ int h_end64(sunhip_grid *g, const HSolver<double> &S) {
   return 0;
 }
 }  // namespace sunhip
+
+namespace {
+// The device address of scratch on each device, taken once with that device
+// current (the host-side name of the array is not a device address).
+constexpr int kMaxDev = 64;
+double *g_scratch[kMaxDev];
+std::mutex g_scratch_mu;
+
+int scratch_on(int device, double **out) {
+  if (device < 0 || device >= kMaxDev) {
+    set_error("sunhip_h_solve_noalloc: device %d out of range", device);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("sunhip_h_solve_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return -3;
+    }
+    g_scratch[device] = (double *)p;
+  }
+  *out = g_scratch[device];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+// sunhip_h_solve with every work array in the library's static scratch:
+// no device memory is allocated.  "h", "htmp" and "D" must already be on
+// the device (uploaded); the same kernels run in the same order, so the
+// result is the one sunhip_h_solve gives.
+int sunhip_h_solve_noalloc(sunhip_grid *g, const sunhip_cg_params *prm, double grav, double theta, double dt,
+                           sunhip_cg_stats *st) {
+  HIPCHK(hipSetDevice(g->device));
+  if (prm->precond != 0 && prm->precond != 1) {
+    set_error("hprecond=%d not supported (0 or 1)", prm->precond);
+    return -1;
+  }
+  const size_t NS = (size_t)g->NS, need = (6 + (size_t)g->mf) * NS * sizeof(double);
+  if (need > SUNHIP_NOALLOC_BYTES) {
+    set_error("sunhip_h_solve_noalloc: the grid needs %zu bytes of scratch ((6 + maxfaces) x %zu cell slots x 8), "
+              "more than its %u-byte maximum; use sunhip_h_solve", need, NS, (unsigned)SUNHIP_NOALLOC_BYTES);
+    return -1;
+  }
+  Field *fh, *fsrc, *fD;
+  {
+    auto a = g->fields.find("h"), b = g->fields.find("htmp"), c = g->fields.find("D");
+    if (a == g->fields.end() || b == g->fields.end() || c == g->fields.end()) {
+      set_error("sunhip_h_solve_noalloc: upload \"h\", \"htmp\" and \"D\" first (it allocates nothing)");
+      return -1;
+    }
+    fh = &a->second; fsrc = &b->second; fD = &c->second;
+  }
+  double *ws = nullptr;
+  if (int rc = scratch_on(g->device, &ws)) return rc;
+  double *coef = ws, *fcoef = coef + NS, *r = fcoef + (size_t)g->mf * NS, *z = r + NS, *rt = z + NS, *p0 = rt + NS,
+         *p1 = p0 + NS;
+  const Geo G = g->geo();
+  const int nb = nbh(g);
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  // the last call's contents: clear all of it (slots the kernels skip are read as 0)
+  const int nz = (int)((6 + (size_t)g->mf) * NS);
+  k_hfill<double><<<(nz + 255) / 256, 256, 0, g->stream>>>(ws, nz, 0.0);
+  HIPCHK(hipGetLastError());
+  int rc = upload_scal(g, g->d_hs, prm);
+  if (rc) return rc;
+  const double td = theta * dt;
+  k_hcoef<<<nb, 256, 0, g->stream>>>(G, fD->d, grav * (td * td), coef, fcoef);
+  k_hzero_comp<double><<<nb, 256, 0, g->stream>>>(G, fh->d);
+  HVec<double, double> V{};
+  V.coef = coef; V.fcoef = fcoef; V.vin = fh->d; V.src = fsrc->d; V.out = r;
+  k_happly<HM_RESID, double, double><<<nb, 256, 0, g->stream>>>(G, V, g->d_hs, g->d_part);
+  HIPCHK(hipGetLastError());
+  HSolver<double> S{g->d_hs, coef, fcoef, fh->d, r, z, rt, p0, p1};
+  if ((rc = h_start<double>(g, S))) return rc;
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  if ((rc = h_loop<double>(g, prm, S, PK_HAPPLY, PK_HPREC))) return rc;
+  HIPCHK(hipMemcpyAsync(g->h_sc, g->d_hs, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipEventRecord(g->ev2, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return fill_stats(g, *g->h_sc, st);
+}
+
+}  // extern "C"
 
 extern "C" {
 

@@ -115,6 +115,8 @@ def lib() -> C.CDLL:
         L.sunhip_q_precondition.argtypes = [vp, C.c_char_p, C.c_char_p]
         L.sunhip_h_solve.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
                                      C.POINTER(CGStats)]
+        L.sunhip_h_solve_noalloc.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
+                                             C.POINTER(CGStats)]
         L.sunhip_face_scalars.argtypes = [vp, C.POINTER(ScalarParams), C.c_char_p]
         L.sunhip_update_scalars.argtypes = [vp, C.POINTER(ScalarParams), C.c_char_p]
         L.sunhip_sync.argtypes = [vp]
@@ -295,6 +297,15 @@ class DeviceGrid:
         P = CGParams(maxiters, precond, resnorm, epsilon, check_every)
         S = CGStats()
         check(lib().sunhip_h_solve(self.h, C.byref(P), grav, theta, dt, C.byref(S)), "sunhip_h_solve")
+        return S.as_dict()
+
+    def h_solve_noalloc(self, grav, theta, dt, maxiters=1000, precond=1, resnorm=0, epsilon=1e-10, check_every=0):
+        """sunhip_h_solve with its work arrays in the library's static
+        scratch (allocates no device memory)."""
+        P = CGParams(maxiters, precond, resnorm, epsilon, check_every)
+        S = CGStats()
+        check(lib().sunhip_h_solve_noalloc(self.h, C.byref(P), grav, theta, dt, C.byref(S)),
+              "sunhip_h_solve_noalloc")
         return S.as_dict()
 
     def h_solve_mixed(self, grav, theta, dt, epsilon=1e-10, inner_tol=1e-5, max_outer=20, maxiters=1000,
