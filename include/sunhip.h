@@ -64,6 +64,10 @@ typedef struct {
   int resnorm;
   double epsilon;      /* qepsilon or epsilon */
   int check_every;     /* iterations between host convergence polls (0: auto) */
+  int qc_alias;        /* qprecond=1 only: 1 when the caller's c rows are its residual rows, as in
+                          Solve (main/phys.c:1280 passes phys->stmp3, CGSolveQ's r), so the final
+                          x /= c divides by the final residual (main/phys.c:2539-2546); 0 divides
+                          by the conditioning c */
 } sunhip_cg_params;
 
 typedef struct {
@@ -107,6 +111,13 @@ int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
  * device layout. */
 int sunhip_upload(sunhip_grid *g, const char *name, const double *flat);
 int sunhip_download(sunhip_grid *g, const char *name, double *flat);
+/* Pinned host buffer slot (0 or 1) of the context, at least n doubles,
+ * for transfers at the full host-link rate.  sunhip_upload from a slot
+ * returns once the copy is enqueued (no wait); the next sunhip_pinned of
+ * that slot waits until the device has read it, so a caller can fill one
+ * slot while the other is in flight.  Valid until the next call for the
+ * same slot or sunhip_grid_destroy.  No reference counterpart. */
+int sunhip_pinned(sunhip_grid *g, int slot, int64_t n, double **out);
 /* Device-side copy of one field into another of the same kind. */
 int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src);
 /* dst = a*src element-wise on the device (fields of the same kind);
@@ -117,7 +128,12 @@ int sunhip_field_scale(sunhip_grid *g, const char *dst, const char *src, double 
 /* ---- nonhydrostatic pressure (main/phys.c:2411-2550) ------------------- */
 /* Solve on the device-resident fields: "q" (warm start in, solution out),
  * "src" (RHS, clobbered as the reference clobbers it), "dzz", "dzf", "D".
- * Requires sunhip_set_extents.  theta/dt are not needed by this solve. */
+ * Requires sunhip_set_extents.  theta/dt are not needed by this solve.
+ * qprecond (prm->precond) 0, 1 or 2.  With 1 the conditioning c of
+ * ConditionQ is formed in cell field "qcond" on the computational columns;
+ * its other columns are read as the caller left them (the reference reads a
+ * boundary neighbour's c as it finds it), and prm->qc_alias selects the
+ * final rescaling. */
 int sunhip_q_solve(sunhip_grid *g, const sunhip_cg_params *prm, sunhip_cg_stats *st);
 /* y = OperatorQ(x) on named device fields (QCoefficients from "dzz"). */
 int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y);

@@ -10,11 +10,12 @@
  * Corrector and Continuity, the stencils either side of CGSolveQ, run on the
  * device too.  Scratch rows the
  * reference clobbers as a side effect (src, stmp2, stmp3, uold, wtmp, qtmp,
- * a..d, hold, htmp*, hcoef, hfcoef) are left untouched: the reference reads
- * none of them afterwards.  Unsupported configurations (numprocs>1,
- * qprecond=1, UpdateScalars with src1/src2/Ftop/Fbot) print an error and
- * exit, as the reference does on fatal conditions; nonconvergence is not an
- * error (query it with sunhip_last_solve).
+ * a..d, hold, htmp*, hcoef, hfcoef, and CGSolveQ's c with qprecond=1) are
+ * left untouched: the reference reads none of them afterwards.
+ * Unsupported configurations (numprocs>1, UpdateScalars with
+ * src1/src2/Ftop/Fbot) print an error and exit, as the reference does on
+ * fatal conditions; nonconvergence is not an error (query it with
+ * sunhip_last_solve).
  */
 #ifndef SUNTANS_HIP_H
 #define SUNTANS_HIP_H
@@ -25,7 +26,11 @@
 extern "C" {
 #endif
 
-/* main/phys.c:2411 (static CGSolveQ; prototype main/phys.c:50-51) */
+/* main/phys.c:2411 (static CGSolveQ; prototype main/phys.c:50-51).  qprecond
+ * 0, 1 or 2.  With 1, when c's rows are phys->stmp3 (Solve's call,
+ * main/phys.c:1280) they are also the residual rows, and the result is the
+ * reference's: the scaled solution divided by the final residual
+ * (main/phys.c:2539-2546). */
 void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *prop, int myproc,
               int numprocs, MPI_Comm comm);
 /* main/phys.c:3086 (static CGSolve; prototype main/phys.c:45-46) */

@@ -30,9 +30,8 @@ LIB = os.path.join(HERE, "lib")
 INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(ROOT, "_build", "obj")
 
-HIP_SOURCES = ["sunhip_grid.hip", "sunhip_q.hip", "sunhip_h.hip", "sunhip_tvd.hip", "sunhip_mixed.hip",
-               "sunhip_batch.hip", "sunhip_step.hip", "sunhip_upred.hip",
-               "sunhip_vgrid.hip", "sunhip_wpred.hip"]
+HIP_SOURCES = ["sunhip_grid.hip", "sunhip_q.hip", "sunhip_h.hip", "sunhip_tvd.hip", "sunhip_batch.hip",
+               "sunhip_step.hip", "sunhip_upred.hip", "sunhip_vgrid.hip", "sunhip_wpred.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,7 +63,7 @@ def build_product() -> None:
     _run([CC, "-O2", "-fPIC", "-shared", "-Wall", f"-I{INC}", os.path.join(host, "sunstate.c"),
           "-o", os.path.join(LIB, "libsunstate.so"), "-lm", "-pthread"])
     _run([CC, "-O2", "-fPIC", "-shared", "-Wall", f"-I{INC}", os.path.join(host, "dropin.c"),
-          f"-L{LIB}", "-lsunhip", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIB, "libsuntans.so")])
+          f"-L{LIB}", "-lsunhip", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIB, "libsuntans.so"), "-lm", "-pthread"])
 
 
 def build_checkers() -> None:

@@ -13,19 +13,36 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
+#include <pthread.h>
+#include <unistd.h>
 
 #include "sunhip.h"
 #include "suntans_hip.h"
 
 #define MAXCTX 16
 
+/* What identifies a grid between calls without reading it whole: the
+ * addresses of its topology arrays, its sizes and a sampled hash of the
+ * arrays' contents (taken when the context is built and compared on every
+ * call).  A grid edited in place in a way the samples miss needs
+ * sunhip_dropin_reset. */
+#define NIDENT 17
+typedef struct {
+  const void *p[NIDENT];
+  int Nc, Ne, maxfaces, Nkmax;
+} ident_t;
+
 typedef struct {
   const gridT *key;
+  ident_t id;
   uint64_t fp;
   int Nc, Ne;
   sunhip_grid *dev;
-  double *flat;      /* host packing buffer */
+  double *flat;      /* the pinned slot in use (sunhip_pinned) */
   size_t flat_n;
+  int slot;          /* the slot the next transfer takes */
+  size_t *cst, *wst, *est;  /* flat offsets of cell (Nk), w-type (Nk+1) and edge (Nkc) rows */
 } ctx_t;
 
 static ctx_t g_ctx[MAXCTX];
@@ -44,56 +61,86 @@ static void check(int rc, const char *where) {
   if (rc) fatal(where, sunhip_error());
 }
 
-/* FNV-1a over the static topology and metrics: a gridT freed and another
- * allocated at the same address must not reuse the old device copy. */
-static uint64_t fnv(uint64_t h, const void *p, size_t n) {
+static void ident(const gridT *g, ident_t *id) {
+  const void *p[NIDENT] = {g->Nk, g->Nkc, g->nfaces, g->neigh, g->face, g->normal, g->grad, g->mark, g->cellp,
+                           g->celldist, g->edgep, g->edgedist, g->Ac, g->df, g->dg, g->xv, g->yv};
+  memset(id, 0, sizeof *id);
+  memcpy(id->p, p, sizeof p);
+  id->Nc = g->Nc; id->Ne = g->Ne; id->maxfaces = g->maxfaces; id->Nkmax = g->Nkmax;
+}
+
+/* FNV-1a over at most 64 evenly spaced elements of an array */
+static uint64_t fnv_sample(uint64_t h, const void *p, size_t n, size_t size) {
   const unsigned char *b = (const unsigned char *)p;
-  for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  const size_t m = n < 64 ? n : 64;
+  for (size_t j = 0; j < m; j++) {
+    const size_t i = m > 1 ? j * (n - 1) / (m - 1) : 0;
+    for (size_t k = 0; k < size; k++) h = (h ^ b[i * size + k]) * 1099511628211ull;
+  }
   return h;
 }
 static uint64_t fingerprint(const gridT *g) {
   uint64_t h = 1469598103934665603ull;
   const size_t nc = (size_t)g->Nc, ne = (size_t)g->Ne, nf = (size_t)g->maxfaces * nc;
-  h = fnv(h, &g->Nc, sizeof(int));
-  h = fnv(h, &g->Ne, sizeof(int));
-  h = fnv(h, &g->maxfaces, sizeof(int));
-  h = fnv(h, g->Nk, nc * sizeof(int));
-  h = fnv(h, g->Nkc, ne * sizeof(int));
-  h = fnv(h, g->nfaces, nc * sizeof(int));
-  h = fnv(h, g->neigh, nf * sizeof(int));
-  h = fnv(h, g->face, nf * sizeof(int));
-  h = fnv(h, g->grad, 2 * ne * sizeof(int));
-  h = fnv(h, g->cellp, nc * sizeof(int));
-  h = fnv(h, g->celldist, 3 * sizeof(int));
-  h = fnv(h, g->edgedist, 6 * sizeof(int));
-  h = fnv(h, g->Ac, nc * sizeof(double));
-  h = fnv(h, g->df, ne * sizeof(double));
-  h = fnv(h, g->dg, ne * sizeof(double));
+  h = fnv_sample(h, g->Nk, nc, sizeof(int));
+  h = fnv_sample(h, g->Nkc, ne, sizeof(int));
+  h = fnv_sample(h, g->nfaces, nc, sizeof(int));
+  h = fnv_sample(h, g->neigh, nf, sizeof(int));
+  h = fnv_sample(h, g->face, nf, sizeof(int));
+  h = fnv_sample(h, g->normal, nf, sizeof(int));
+  h = fnv_sample(h, g->grad, 2 * ne, sizeof(int));
+  h = fnv_sample(h, g->mark, ne, sizeof(int));
+  h = fnv_sample(h, g->cellp, nc, sizeof(int));
+  h = fnv_sample(h, g->edgep, ne, sizeof(int));
+  h = fnv_sample(h, g->celldist, 3, sizeof(int));
+  h = fnv_sample(h, g->edgedist, 6, sizeof(int));
+  h = fnv_sample(h, g->Ac, nc, sizeof(double));
+  h = fnv_sample(h, g->df, ne, sizeof(double));
+  h = fnv_sample(h, g->dg, ne, sizeof(double));
+  h = fnv_sample(h, g->xv, nc, sizeof(double));
+  h = fnv_sample(h, g->yv, nc, sizeof(double));
   return h;
 }
 
+static void ctx_free(ctx_t *c) {
+  sunhip_grid_destroy(c->dev);  /* frees the pinned slots too */
+  free(c->cst);
+  free(c->wst);
+  free(c->est);
+}
+
+static size_t *offsets(const int *len, int n, int extra) {
+  size_t *o = (size_t *)malloc(sizeof(size_t) * ((size_t)n + 1));
+  if (!o) return NULL;
+  o[0] = 0;
+  for (int i = 0; i < n; i++) o[i + 1] = o[i] + (size_t)(len[i] + extra);
+  return o;
+}
+
 static ctx_t *get_ctx(gridT *g, const char *where) {
-  const uint64_t fp = fingerprint(g);
+  ident_t id;
+  ident(g, &id);
   for (int i = 0; i < g_nctx; i++)
-    if (g_ctx[i].key == g && g_ctx[i].fp == fp) return &g_ctx[i];
+    if (g_ctx[i].key == g && !memcmp(&g_ctx[i].id, &id, sizeof id) && g_ctx[i].fp == fingerprint(g))
+      return &g_ctx[i];
   /* same address, different grid: drop the stale copy */
   for (int i = 0; i < g_nctx; i++)
     if (g_ctx[i].key == g) {
-      sunhip_grid_destroy(g_ctx[i].dev);
-      free(g_ctx[i].flat);
+      ctx_free(&g_ctx[i]);
       g_ctx[i] = g_ctx[g_nctx - 1];
       g_nctx--;
       break;
     }
   if (g_nctx == MAXCTX) {
     /* evict the oldest */
-    sunhip_grid_destroy(g_ctx[0].dev);
-    free(g_ctx[0].flat);
+    ctx_free(&g_ctx[0]);
     memmove(g_ctx, g_ctx + 1, sizeof(ctx_t) * (MAXCTX - 1));
     g_nctx--;
   }
   ctx_t *c = &g_ctx[g_nctx];
-  c->fp = fp;
+  memset(c, 0, sizeof *c);
+  c->fp = fingerprint(g);
+  c->id = id;
   sunhip_topology T;
   T.Nc = g->Nc; T.Ne = g->Ne; T.maxfaces = g->maxfaces; T.Nkmax = g->Nkmax;
   T.nfaces = g->nfaces; T.neigh = g->neigh; T.face = g->face; T.normal = g->normal;
@@ -106,12 +153,22 @@ static ctx_t *get_ctx(gridT *g, const char *where) {
   size_t n = (size_t)sz[1];
   if ((size_t)sz[2] > n) n = (size_t)sz[2];
   if ((size_t)g->maxfaces * g->Nc > n) n = (size_t)g->maxfaces * g->Nc;
-  c->flat = (double *)malloc(n * sizeof(double));
-  if (!c->flat) fatal(where, "out of host memory");
   c->flat_n = n;
+  c->cst = offsets(g->Nk, g->Nc, 0);
+  c->wst = offsets(g->Nk, g->Nc, 1);
+  c->est = offsets(g->Nkc, g->Ne, 0);
+  if (!c->cst || !c->wst || !c->est) fatal(where, "out of host memory");
   c->key = g; c->Nc = g->Nc; c->Ne = g->Ne;
   g_nctx++;
   return c;
+}
+
+/* The next pinned slot for a transfer (it waits until the device has read
+ * what was last uploaded from it); it becomes c->flat. */
+static double *next_buf(ctx_t *c, const char *where) {
+  check(sunhip_pinned(c->dev, c->slot, (int64_t)c->flat_n, &c->flat), where);
+  c->slot ^= 1;
+  return c->flat;
 }
 
 static void extents(ctx_t *c, gridT *g, const char *where) {
@@ -119,39 +176,90 @@ static void extents(ctx_t *c, gridT *g, const char *where) {
   check(sunhip_set_extents(c->dev, &X), where);
 }
 
+/* ---- jagged rows <-> flat, split over threads -------------------------- */
+enum { ROWS_PACK, ROWS_UNPACK, ROWS_UNPACK_FROM, ROWS_UNPACK_UPTO };
+typedef struct {
+  int mode;
+  double *flat;
+  REAL **rows;
+  const size_t *start;     /* row i at flat + start[i], start[i+1] - start[i] values */
+  const int *order;        /* unpack: row of entry ip (NULL: ip itself) */
+  const int *lvl;          /* FROM: first level written; UPTO: levels written */
+  int lo, hi;
+} rows_job;
+
+static void rows_run(const rows_job *J, int lo, int hi) {
+  for (int ip = lo; ip < hi; ip++) {
+    const int i = J->order ? J->order[ip] : ip;
+    const size_t o = J->start[i], n = J->start[i + 1] - o;
+    switch (J->mode) {
+      case ROWS_PACK: memcpy(J->flat + o, J->rows[i], n * sizeof(double)); break;
+      case ROWS_UNPACK: memcpy(J->rows[i], J->flat + o, n * sizeof(double)); break;
+      case ROWS_UNPACK_FROM: {
+        const size_t k0 = (size_t)J->lvl[i];
+        if (k0 < n) memcpy(J->rows[i] + k0, J->flat + o + k0, (n - k0) * sizeof(double));
+        break;
+      }
+      case ROWS_UNPACK_UPTO: memcpy(J->rows[i], J->flat + o, (size_t)J->lvl[i] * sizeof(double)); break;
+    }
+  }
+}
+
+typedef struct { const rows_job *J; int lo, hi; } rows_part;
+static void *rows_thread(void *a) {
+  const rows_part *P = (const rows_part *)a;
+  rows_run(P->J, P->lo, P->hi);
+  return NULL;
+}
+
+/* Threads for packing: SUNHIP_DROPIN_THREADS, else the online cores, at most 16. */
+static int n_threads(void) {
+  static int n = 0;
+  if (!n) {
+    const char *e = getenv("SUNHIP_DROPIN_THREADS");
+    long v = e ? atol(e) : sysconf(_SC_NPROCESSORS_ONLN);
+    if (v < 1) v = 1;
+    if (!e && v > 16) v = 16;
+    if (v > 64) v = 64;
+    n = (int)v;
+  }
+  return n;
+}
+
+static void rows_par(const rows_job *J, size_t nvalues) {
+  int T = n_threads();
+  if (nvalues < ((size_t)1 << 18) || J->hi - J->lo < 2 * T) T = 1;
+  pthread_t th[64];
+  rows_part P[64];
+  int ok[64];
+  const int n = J->hi - J->lo;
+  for (int t = 0; t < T; t++) {
+    P[t].J = J;
+    P[t].lo = J->lo + (int)((int64_t)n * t / T);
+    P[t].hi = J->lo + (int)((int64_t)n * (t + 1) / T);
+    ok[t] = t > 0 && pthread_create(&th[t], NULL, rows_thread, &P[t]) == 0;
+  }
+  for (int t = 0; t < T; t++)
+    if (!ok[t]) rows_run(J, P[t].lo, P[t].hi);  /* this thread's share, and any a thread could not take */
+  for (int t = 1; t < T; t++)
+    if (ok[t]) pthread_join(th[t], NULL);
+}
+
 /* flat <-> jagged */
+static double *pack_rows(ctx_t *c, REAL **rows, const size_t *start, int n, const char *where) {
+  rows_job J = {ROWS_PACK, next_buf(c, where), rows, start, NULL, NULL, 0, n};
+  rows_par(&J, start[n]);
+  return c->flat;
+}
 static double *pack_cells(ctx_t *c, const gridT *g, REAL **rows, int extra) {
-  size_t o = 0;
-  for (int i = 0; i < g->Nc; i++) {
-    size_t n = (size_t)(g->Nk[i] + extra);
-    memcpy(c->flat + o, rows[i], n * sizeof(double));
-    o += n;
-  }
-  return c->flat;
+  return pack_rows(c, rows, extra ? c->wst : c->cst, g->Nc, "pack");
 }
-static double *pack_edges(ctx_t *c, const gridT *g, REAL **rows) {
-  size_t o = 0;
-  for (int j = 0; j < g->Ne; j++) {
-    memcpy(c->flat + o, rows[j], (size_t)g->Nkc[j] * sizeof(double));
-    o += (size_t)g->Nkc[j];
-  }
-  return c->flat;
-}
-/* write back cells of cellp[celldist[lo]..celldist[hi]) */
+static double *pack_edges(ctx_t *c, const gridT *g, REAL **rows) { return pack_rows(c, rows, c->est, g->Ne, "pack"); }
+
+/* write back cells of cellp[celldist[lo]..celldist[hi]) from c->flat */
 static void unpack_cells(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  static size_t *start = NULL;
-  static int nstart = 0;
-  if (nstart < g->Nc + 1) {
-    free(start);
-    start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Nc + 1));
-    nstart = g->Nc + 1;
-  }
-  start[0] = 0;
-  for (int i = 0; i < g->Nc; i++) start[i + 1] = start[i] + (size_t)g->Nk[i];
-  for (int ip = g->celldist[lo]; ip < g->celldist[hi]; ip++) {
-    const int i = g->cellp[ip];
-    memcpy(rows[i], c->flat + start[i], (size_t)g->Nk[i] * sizeof(double));
-  }
+  rows_job J = {ROWS_UNPACK, c->flat, rows, c->cst, g->cellp, NULL, g->celldist[lo], g->celldist[hi]};
+  rows_par(&J, c->cst[g->Nc]);
 }
 
 static void upload_cells(ctx_t *c, gridT *g, const char *name, REAL **rows, int extra, const char *where) {
@@ -160,14 +268,30 @@ static void upload_cells(ctx_t *c, gridT *g, const char *name, REAL **rows, int 
 static void upload_edges(ctx_t *c, gridT *g, const char *name, REAL **rows, const char *where) {
   check(sunhip_upload(c->dev, name, pack_edges(c, g, rows)), where);
 }
+/* field name into the next pinned slot (c->flat) */
+static void download(ctx_t *c, const char *name, const char *where) {
+  check(sunhip_download(c->dev, name, next_buf(c, where)), where);
+}
+
+/* qprecond=1: 1 when c's rows are the residual rows phys->stmp3 CGSolveQ
+ * uses as r (main/phys.c:2419), as in Solve's call (main/phys.c:1280). */
+static int c_aliases_r(REAL **c, const gridT *g, const physT *p, const char *W) {
+  int same = 0, n = 0;
+  for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++, n++) {
+    const int i = g->cellp[ip];
+    same += c[i] == p->stmp3[i];
+  }
+  if (same != 0 && same != n) fatal(W, "c shares some rows, but not all, with the residual rows phys->stmp3");
+  return same != 0;
+}
 
 void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs,
               MPI_Comm comm) {
   static const char *W = "CGSolveQ";
-  (void)myproc; (void)comm; (void)c;
+  (void)myproc; (void)comm;
   if (numprocs != 1) fatal(W, "only numprocs=1 (the whole grid on one GPU) is supported");
-  if (prop->qprecond != 0 && prop->qprecond != 2)
-    fatal(W, "qprecond must be 0 or 2 (qprecond=1 is not built for the device)");
+  if (prop->qprecond != 0 && prop->qprecond != 1 && prop->qprecond != 2) fatal(W, "qprecond must be 0, 1 or 2");
+  if (prop->qprecond == 1 && !c) fatal(W, "qprecond=1 needs the conditioning rows c");
   ctx_t *C = get_ctx(grid, W);
   extents(C, grid, W);
   upload_cells(C, grid, "q", q, 0, W);
@@ -175,7 +299,12 @@ void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *p
   upload_cells(C, grid, "dzz", grid->dzz, 0, W);
   upload_edges(C, grid, "dzf", grid->dzf, W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
-  sunhip_cg_params P = {prop->qmaxiters, prop->qprecond, (int)prop->resnorm, prop->qepsilon, 0};
+  sunhip_cg_params P = {prop->qmaxiters, prop->qprecond, (int)prop->resnorm, prop->qepsilon, 0, 0};
+  if (prop->qprecond == 1) {
+    /* a boundary neighbour's c is read as the caller left it */
+    upload_cells(C, grid, "qcond", c, 0, W);
+    P.qc_alias = c_aliases_r(c, grid, phys, W);
+  }
   if (g_mixed) {
     if (prop->qprecond != 2) fatal(W, "the mixed-precision form needs qprecond=2");
     int inner = 0;
@@ -185,7 +314,7 @@ void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *p
     check(sunhip_q_solve(C->dev, &P, &g_last[0].st), W);
   }
   g_last[0].valid = 1;
-  check(sunhip_download(C->dev, "q", C->flat), W);
+  download(C, "q", W);
   unpack_cells(C, grid, q, 0, 1);
 }
 
@@ -199,7 +328,7 @@ void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MP
   check(sunhip_upload(C->dev, "h", phys->h), W);
   check(sunhip_upload(C->dev, "htmp", phys->htmp), W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
-  sunhip_cg_params P = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0};
+  sunhip_cg_params P = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0, 0};
   if (g_mixed) {
     int inner = 0;
     check(sunhip_h_solve_mixed(C->dev, &P, prop->grav, prop->theta, prop->dt, g_inner_tol, 50, &g_last[1].st,
@@ -209,7 +338,7 @@ void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MP
     check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
   }
   g_last[1].valid = 1;
-  check(sunhip_download(C->dev, "h", C->flat), W);
+  download(C, "h", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
     phys->h[i] = C->flat[i];
@@ -228,7 +357,7 @@ void OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *grid, physT *ph
   check(sunhip_upload(C->dev, "D", phys->D), W);
   upload_cells(C, grid, "opx", x, 0, W);
   check(sunhip_q_operator(C->dev, "opx", "opy"), W);
-  check(sunhip_download(C->dev, "opy", C->flat), W);
+  download(C, "opy", W);
   unpack_cells(C, grid, y, 0, 1);
 }
 
@@ -240,7 +369,7 @@ void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, 
   upload_cells(C, grid, "coef", coef, 0, W);
   upload_cells(C, grid, "opx", x, 0, W);
   check(sunhip_q_precondition(C->dev, "opx", "opy"), W);
-  check(sunhip_download(C->dev, "opy", C->flat), W);
+  download(C, "opy", W);
   unpack_cells(C, grid, xc, 0, 1);
 }
 
@@ -254,7 +383,7 @@ void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *ph
   check(sunhip_upload(C->dev, "hcoef", coef), W);
   check(sunhip_upload(C->dev, "hx", x), W);
   check(sunhip_h_operator_coef(C->dev, fcoef, "hx", "hy"), W);
-  check(sunhip_download(C->dev, "hy", C->flat), W);
+  download(C, "hy", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
     y[i] = C->flat[i];
@@ -265,36 +394,27 @@ void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *ph
 static void upload_brows(ctx_t *C, gridT *g, REAL **bscal, const char *name, const char *W) {
   const int lo = g->edgedist[2], hi = g->edgedist[5];
   if (!bscal || hi <= lo) return;
-  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
-  start[0] = 0;
-  for (int j = 0; j < g->Ne; j++) start[j + 1] = start[j] + (size_t)g->Nkc[j];
-  memset(C->flat, 0, start[g->Ne] * sizeof(double));
+  double *b = next_buf(C, W);
+  memset(b, 0, C->est[g->Ne] * sizeof(double));
   for (int jp = lo; jp < hi; jp++) {
     const int j = g->edgep[jp];
-    memcpy(C->flat + start[j], bscal[jp - lo], (size_t)g->Nke[j] * sizeof(double));
+    memcpy(b + C->est[j], bscal[jp - lo], (size_t)g->Nke[j] * sizeof(double));
   }
-  free(start);
-  check(sunhip_upload(C->dev, name, C->flat), W);
+  check(sunhip_upload(C->dev, name, b), W);
 }
 static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) { upload_brows(C, g, bscal, "bscal", W); }
 
 static void download_face_values(ctx_t *C, gridT *g, physT *p, const char *W) {
-  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Ne + 1));
-  start[0] = 0;
-  for (int j = 0; j < g->Ne; j++) start[j + 1] = start[j] + (size_t)g->Nkc[j];
   const char *names[2] = {"SfHp", "SfHm"};
   REAL **rows[2] = {p->SfHp, p->SfHm};
   for (int f = 0; f < 2; f++) {
-    check(sunhip_download(C->dev, names[f], C->flat), W);
+    download(C, names[f], W);
     for (int r = 0; r < 2; r++) {
-      const int lo = g->edgedist[r == 0 ? 0 : 2], hi = g->edgedist[r == 0 ? 1 : 3];
-      for (int jp = lo; jp < hi; jp++) {
-        const int j = g->edgep[jp];
-        memcpy(rows[f][j], C->flat + start[j], (size_t)g->Nke[j] * sizeof(double));
-      }
+      rows_job J = {ROWS_UNPACK_UPTO, C->flat, rows[f], C->est, g->edgep, g->Nke, g->edgedist[r == 0 ? 0 : 2],
+                    g->edgedist[r == 0 ? 1 : 3]};
+      rows_par(&J, C->est[g->Ne]);
     }
   }
-  free(start);
 }
 
 static void scalar_inputs(ctx_t *C, gridT *grid, physT *phys, REAL **scal, REAL **bscal, const char *W) {
@@ -315,9 +435,9 @@ void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, R
   sunhip_scalar_params S = {prop->theta, prop->dt, 0.0, TVD, prop->n, 0, 0};
   check(sunhip_face_scalars(C->dev, &S, "s"), W);
   download_face_values(C, grid, phys, W);
-  check(sunhip_download(C->dev, "gradSx", C->flat), W);
+  download(C, "gradSx", W);
   unpack_cells(C, grid, phys->gradSx, 0, 2);
-  check(sunhip_download(C->dev, "gradSy", C->flat), W);
+  download(C, "gradSy", W);
   unpack_cells(C, grid, phys->gradSy, 0, 2);
 }
 
@@ -343,46 +463,34 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   upload_cells(C, grid, "dzz", grid->dzz, 0, W);
   upload_cells(C, grid, "dzzold", grid->dzzold, 0, W);
   upload_cells(C, grid, "kappa_tv", kappa_tv, 0, W);
-  for (int i = 0; i < grid->Nc; i++) C->flat[i] = phys->active[i] ? 1.0 : 0.0;
-  check(sunhip_upload(C->dev, "active", C->flat), W);
+  double *act = next_buf(C, W);
+  for (int i = 0; i < grid->Nc; i++) act[i] = phys->active[i] ? 1.0 : 0.0;
+  check(sunhip_upload(C->dev, "active", act), W);
   sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
   check(sunhip_update_scalars(C->dev, &S, "s"), W);
-  check(sunhip_download(C->dev, "s", C->flat), W);
+  download(C, "s", W);
   unpack_cells(C, grid, scal, 0, 1);
   /* Cn: zero everywhere, then the computational columns */
   for (int i = 0; i < grid->Nc; i++) memset(Cn[i], 0, (size_t)grid->Nk[i] * sizeof(double));
-  check(sunhip_download(C->dev, "Cn", C->flat), W);
+  download(C, "Cn", W);
   unpack_cells(C, grid, Cn, 0, 1);
   if (TVDscheme) download_face_values(C, grid, phys, W);
 }
 
 /* write back rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
 static void unpack_cells_w(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  size_t *start = (size_t *)malloc(sizeof(size_t) * (size_t)(g->Nc + 1));
-  start[0] = 0;
-  for (int i = 0; i < g->Nc; i++) start[i + 1] = start[i] + (size_t)g->Nk[i] + 1;
-  for (int ip = g->celldist[lo]; ip < g->celldist[hi]; ip++) {
-    const int i = g->cellp[ip];
-    memcpy(rows[i], c->flat + start[i], ((size_t)g->Nk[i] + 1) * sizeof(double));
-  }
-  free(start);
+  rows_job J = {ROWS_UNPACK, c->flat, rows, c->wst, g->cellp, NULL, g->celldist[lo], g->celldist[hi]};
+  rows_par(&J, c->wst[g->Nc]);
 }
 /* write back levels [ctop, Nk) of every cell */
 static void unpack_cells_wet(ctx_t *c, const gridT *g, REAL **rows) {
-  size_t o = 0;
-  for (int i = 0; i < g->Nc; i++) {
-    const int kt = g->ctop[i];
-    if (kt < g->Nk[i]) memcpy(rows[i] + kt, c->flat + o + kt, (size_t)(g->Nk[i] - kt) * sizeof(double));
-    o += (size_t)g->Nk[i];
-  }
+  rows_job J = {ROWS_UNPACK_FROM, c->flat, rows, c->cst, NULL, g->ctop, 0, g->Nc};
+  rows_par(&J, c->cst[g->Nc]);
 }
 /* write back every edge row */
 static void unpack_edges(ctx_t *c, const gridT *g, REAL **rows) {
-  size_t o = 0;
-  for (int j = 0; j < g->Ne; j++) {
-    memcpy(rows[j], c->flat + o, (size_t)g->Nkc[j] * sizeof(double));
-    o += (size_t)g->Nkc[j];
-  }
+  rows_job J = {ROWS_UNPACK, c->flat, rows, c->est, NULL, NULL, 0, g->Ne};
+  rows_par(&J, c->est[g->Ne]);
 }
 
 void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs) {
@@ -397,9 +505,9 @@ void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int mypro
   upload_cells(C, grid, "w", phys->w, 1, W);
   upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
   check(sunhip_compute_qsource(C->dev, prop->theta, prop->dt, "src"), W);
-  check(sunhip_download(C->dev, "src", C->flat), W);
+  download(C, "src", W);
   unpack_cells_wet(C, grid, src);
-  check(sunhip_download(C->dev, "D", C->flat), W);
+  download(C, "D", W);
   memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
 }
 
@@ -416,11 +524,11 @@ void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int
   upload_edges(C, grid, "u", phys->u, W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
   check(sunhip_corrector(C->dev, prop->dt, "q", "qp"), W);
-  check(sunhip_download(C->dev, "u", C->flat), W);
+  download(C, "u", W);
   unpack_edges(C, grid, phys->u);
-  check(sunhip_download(C->dev, "w", C->flat), W);
+  download(C, "w", W);
   unpack_cells_w(C, grid, phys->w, 0, 1);
-  check(sunhip_download(C->dev, "qp", C->flat), W);
+  download(C, "qp", W);
   unpack_cells(C, grid, phys->q, 0, 1);
 }
 
@@ -433,17 +541,14 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
   upload_edges(C, grid, "dzf", grid->dzf, W);
   upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
   check(sunhip_continuity(C->dev, prop->theta, "w"), W);
-  check(sunhip_download(C->dev, "w", C->flat), W);
+  download(C, "w", W);
   unpack_cells_w(C, grid, w, 0, 1);
 }
 
 /* write back levels [0, Nke) of every edge row */
 static void unpack_edges_wet(ctx_t *c, const gridT *g, REAL **rows) {
-  size_t o = 0;
-  for (int j = 0; j < g->Ne; j++) {
-    memcpy(rows[j], c->flat + o, (size_t)g->Nke[j] * sizeof(double));
-    o += (size_t)g->Nkc[j];
-  }
+  rows_job J = {ROWS_UNPACK_UPTO, c->flat, rows, c->est, NULL, g->Nke, 0, g->Ne};
+  rows_par(&J, c->est[g->Ne]);
 }
 
 static sunhip_upred_params upred_params(const propT *prop) {
@@ -481,15 +586,15 @@ void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, 
   if (rc == -5) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(0); }  /* main/phys.c:2902 exits with 0 */
   if (rc == -6) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(EXIT_FAILURE); }
   check(rc, W);
-  check(sunhip_download(C->dev, "utmp", C->flat), W);
+  download(C, "utmp", W);
   unpack_edges(C, grid, phys->utmp);
-  check(sunhip_download(C->dev, "ue", C->flat), W);
+  download(C, "ue", W);
   unpack_edges(C, grid, phys->ut);
-  check(sunhip_download(C->dev, "utmp2", C->flat), W);
+  download(C, "utmp2", W);
   unpack_edges_wet(C, grid, phys->utmp2);
-  check(sunhip_download(C->dev, "wtmp2", C->flat), W);
+  download(C, "wtmp2", W);
   unpack_cells_w(C, grid, phys->wtmp2, 0, 2);
-  check(sunhip_download(C->dev, "D", C->flat), W);
+  download(C, "D", W);
   memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
 }
 
@@ -512,18 +617,18 @@ void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, 
   check(sunhip_upload(C->dev, "hcorr", phys->hcorr), W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
   sunhip_upred_params P = upred_params(prop);
-  sunhip_cg_params G = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0};
+  sunhip_cg_params G = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0, 0};
   check(sunhip_u_solve(C->dev, &P, &G, &g_last[1].st), W);
   g_last[1].valid = 1;
-  check(sunhip_download(C->dev, "h", C->flat), W);
+  download(C, "h", W);
   memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-  check(sunhip_download(C->dev, "hcorr", C->flat), W);
+  download(C, "hcorr", W);
   memcpy(phys->hcorr, C->flat, (size_t)grid->Nc * sizeof(double));
-  check(sunhip_download(C->dev, "active", C->flat), W);
+  download(C, "active", W);
   for (int i = 0; i < grid->Nc; i++) phys->active[i] = C->flat[i] != 0;
-  check(sunhip_download(C->dev, "u", C->flat), W);
+  download(C, "u", W);
   unpack_edges(C, grid, phys->u);
-  check(sunhip_download(C->dev, "s", C->flat), W);
+  download(C, "s", W);
   unpack_cells(C, grid, phys->s, 0, 2);
 }
 
@@ -543,23 +648,23 @@ void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option) {
   check(sunhip_upload(C->dev, "dv", grid->dv), W);
   sunhip_vgrid_params V = vgrid_params(grid, prop);
   check(sunhip_update_dz(C->dev, &V, option), W);
-  check(sunhip_download(C->dev, "dzz", C->flat), W);
+  download(C, "dzz", W);
   unpack_cells(C, grid, grid->dzz, 0, 2);
   check(sunhip_get_extents(C->dev, grid->ctop, option != -1 ? grid->ctopold : NULL, grid->etop,
                            option != -1 ? grid->etopold : NULL), W);
   if (option != -1) {
-    check(sunhip_download(C->dev, "dzzold", C->flat), W);
+    download(C, "dzzold", W);
     unpack_cells(C, grid, grid->dzzold, 0, 2);
   } else {
-    check(sunhip_download(C->dev, "h", C->flat), W);
+    download(C, "h", W);
     memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-    check(sunhip_download(C->dev, "dv", C->flat), W);
+    download(C, "dv", W);
     memcpy(grid->dv, C->flat, (size_t)grid->Nc * sizeof(double));
-    check(sunhip_download(C->dev, "dzbot", C->flat), W);
+    download(C, "dzbot", W);
     memcpy(grid->dzbot, C->flat, (size_t)grid->Nc * sizeof(double));
   }
   if (option == 1) {
-    check(sunhip_download(C->dev, "dzfB", C->flat), W);
+    download(C, "dzfB", W);
     memcpy(grid->dzfB, C->flat, (size_t)grid->Ne * sizeof(double));
   }
 }
@@ -575,10 +680,10 @@ void SetFluxHeight(gridT *grid, physT *phys, propT *prop) {
   if (grid->smoothbot) check(sunhip_upload(C->dev, "dzbot", grid->dzbot), W);
   sunhip_vgrid_params V = vgrid_params(grid, prop);
   check(sunhip_set_flux_height(C->dev, &V), W);
-  check(sunhip_download(C->dev, "dzf", C->flat), W);
+  download(C, "dzf", W);
   unpack_edges(C, grid, grid->dzf);
   if (grid->smoothbot) {
-    check(sunhip_download(C->dev, "dzz", C->flat), W);
+    download(C, "dzz", W);
     unpack_cells(C, grid, grid->dzz, 0, 2);
   }
 }
@@ -616,12 +721,13 @@ void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int nu
   upload_brows(C, grid, phys->boundary_w, "bw", W);
   if (scheme == 2) {
     /* main/phys.c:4272-4273 on the host, with the reference's own libm */
+    double *def1 = next_buf(C, W);
     for (int j = 0; j < grid->Ne; j++) {
       int nc1 = grid->grad[2 * j];
       if (nc1 == -1) nc1 = grid->grad[2 * j + 1];
-      C->flat[j] = sqrt(pow(grid->xv[nc1] - grid->xe[j], 2) + pow(grid->yv[nc1] - grid->ye[j], 2));
+      def1[j] = sqrt(pow(grid->xv[nc1] - grid->xe[j], 2) + pow(grid->yv[nc1] - grid->ye[j], 2));
     }
-    check(sunhip_set_face_distances(C->dev, C->flat), W);
+    check(sunhip_set_face_distances(C->dev, def1), W);
   }
   if (scheme == 5) {
     upload_edges(C, grid, "u", phys->u, W);
@@ -637,25 +743,25 @@ void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int nu
   const char *wnames[] = {"w", "wtmp", "Cn_W", "Cn_W2"};
   REAL **wrows[] = {phys->w, phys->wtmp, phys->Cn_W, phys->Cn_W2};
   for (int f = 0; f < 4; f++) {
-    check(sunhip_download(C->dev, wnames[f], C->flat), W);
+    download(C, wnames[f], W);
     unpack_cells_w(C, grid, wrows[f], 0, 2);
   }
-  check(sunhip_download(C->dev, "stmp", C->flat), W);
+  download(C, "stmp", W);
   unpack_cells(C, grid, phys->stmp, 0, 2);
   if (scheme) {
-    check(sunhip_download(C->dev, "wc", C->flat), W);
+    download(C, "wc", W);
     unpack_cells(C, grid, phys->wc, 0, 2);
-    check(sunhip_download(C->dev, "ue", C->flat), W);
+    download(C, "ue", W);
     unpack_edges(C, grid, phys->ut);
   }
   if (scheme == 5) {
-    check(sunhip_download(C->dev, "SfHp", C->flat), W);
+    download(C, "SfHp", W);
     unpack_edges(C, grid, phys->SfHp);
-    check(sunhip_download(C->dev, "SfHm", C->flat), W);
+    download(C, "SfHm", W);
     unpack_edges(C, grid, phys->SfHm);
-    check(sunhip_download(C->dev, "gradSx", C->flat), W);
+    download(C, "gradSx", W);
     unpack_cells(C, grid, phys->gradSx, 0, 2);
-    check(sunhip_download(C->dev, "gradSy", C->flat), W);
+    download(C, "gradSy", W);
     unpack_cells(C, grid, phys->gradSy, 0, 2);
   }
 }
@@ -669,10 +775,7 @@ int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) 
 }
 
 void sunhip_dropin_reset(void) {
-  for (int i = 0; i < g_nctx; i++) {
-    sunhip_grid_destroy(g_ctx[i].dev);
-    free(g_ctx[i].flat);
-  }
+  for (int i = 0; i < g_nctx; i++) ctx_free(&g_ctx[i]);
   memset(g_ctx, 0, sizeof g_ctx);
   g_nctx = 0;
 }

@@ -557,6 +557,10 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_ford};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
+  for (int i = 0; i < 2; i++) {
+    if (g->h_pin[i]) hipHostFree(g->h_pin[i]);
+    if (g->ev_pin[i]) hipEventDestroy(g->ev_pin[i]);
+  }
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->h_poll) hipHostFree(g->h_poll);
   for (hipEvent_t e : g->evp) if (e) hipEventDestroy(e);
@@ -646,6 +650,36 @@ static size_t flat_len(const sunhip_grid *g, Kind k) {
   return 0;
 }
 
+int sunhip_pinned(sunhip_grid *g, int slot, int64_t n, double **out) {
+  if (slot < 0 || slot > 1 || n < 0) {
+    set_error("sunhip_pinned: slot %d / size %lld", slot, (long long)n);
+    return -1;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  if (g->pin_pending[slot]) {
+    HIPCHK(hipEventSynchronize(g->ev_pin[slot]));
+    g->pin_pending[slot] = false;
+  }
+  if (!g->ev_pin[slot]) HIPCHK(hipEventCreateWithFlags(&g->ev_pin[slot], hipEventDisableTiming));
+  if (g->h_pin_n[slot] < (size_t)n) {
+    if (g->h_pin[slot]) HIPCHK(hipHostFree(g->h_pin[slot]));
+    g->h_pin[slot] = nullptr;
+    g->h_pin_n[slot] = 0;
+    HIPCHK(hipHostMalloc((void **)&g->h_pin[slot], (size_t)std::max<int64_t>(n, 1) * sizeof(double),
+                         hipHostMallocDefault));
+    g->h_pin_n[slot] = (size_t)std::max<int64_t>(n, 1);
+  }
+  *out = g->h_pin[slot];
+  return 0;
+}
+
+// The pinned slot flat lies in, or -1.
+static int pinned_slot(const sunhip_grid *g, const double *flat, size_t n) {
+  for (int i = 0; i < 2; i++)
+    if (g->h_pin[i] && flat >= g->h_pin[i] && flat + n <= g->h_pin[i] + g->h_pin_n[i]) return i;
+  return -1;
+}
+
 int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
   HIPCHK(hipSetDevice(g->device));
   Field *f;
@@ -653,6 +687,7 @@ int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
   if (rc) return rc;
   const size_t n = flat_len(g, f->kind);
   if ((rc = ensure_stage(g, n))) return rc;
+  const int pin = pinned_slot(g, flat, n);
   HIPCHK(hipMemcpyAsync(g->d_stage, flat, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
   switch (f->kind) {
     case CELL: case CELLW:
@@ -671,6 +706,11 @@ int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
       break;
   }
   HIPCHK(hipGetLastError());
+  if (pin >= 0) {  // the slot is reused only through sunhip_pinned, which waits for this copy
+    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
+    g->pin_pending[pin] = true;
+    return 0;
+  }
   HIPCHK(hipStreamSynchronize(g->stream));  // the caller may reuse its buffer
   return 0;
 }

@@ -140,6 +140,10 @@ struct sunhip_grid {
   size_t stage_n = 0;
   double *h_stage = nullptr;        // pinned host staging
   size_t h_stage_n = 0;
+  double *h_pin[2] = {nullptr, nullptr};   // pinned host slots (sunhip_pinned)
+  size_t h_pin_n[2] = {0, 0};
+  hipEvent_t ev_pin[2] = {nullptr, nullptr};  // last upload read from the slot
+  bool pin_pending[2] = {false, false};
   bool have_extents = false;
   // reductions
   double *d_part = nullptr;         // block partials

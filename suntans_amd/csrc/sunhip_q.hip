@@ -203,6 +203,50 @@ __global__ __launch_bounds__(256) void k_qfinish(Geo G, T *__restrict__ x, const
                   threadIdx.x & 63);
 }
 
+// ---- qprecond=1 (ConditionQ, its QCoefficients branch, OperatorQC) ------
+template <int NF>
+__global__ __launch_bounds__(256) void k_qcond(Geo G, const double *__restrict__ dzz, const double *__restrict__ D,
+                                               double *__restrict__ c) {
+  qcond_tile<NF>(G, dzz, D, c, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+template <int NF>
+__global__ __launch_bounds__(256) void k_qcoef_pc1(Geo G, const double *__restrict__ dzz,
+                                                   const double *__restrict__ D, const double *__restrict__ c,
+                                                   double *__restrict__ coef, double *__restrict__ fc, int64_t NT) {
+  qcoef_pc1_tile<NF>(G, dzz, D, c, coef, fc, NT, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+template <int MODE, int NF>
+__global__ __launch_bounds__(256) void k_qcapply(Geo G, QVec<double, double> V, CGScal *sc, Red R, int64_t NT) {
+  if (MODE == QM_FUSED && sc->done) return;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const double mu = MODE == QM_FUSED ? sc->mu : 0.0, nu = MODE == QM_FUSED ? sc->nu : 0.0;
+  const double dot = qcapply_tile<MODE, NF>(G, V, mu, nu, NT, t, threadIdx.x & 63);
+  if (MODE == QM_FUSED) {
+    double dv[1] = {dot};
+    reduce_out<1, 256>(dv, R, sc);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_qscale(Geo G, double *__restrict__ a, const double *__restrict__ b,
+                                                int op) {
+  qscale_tile(G, a, b, op, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// The iterations read a non-computational neighbour's p where the reference
+// reads it: p aliases src there (main/phys.c:2419, 2496-2522 update only the
+// computational cells), so p = rtmp + mu*p_old with p_old = 0 needs rtmp =
+// src on those columns.
+__global__ __launch_bounds__(256) void k_qbnd_src(Geo G, const double *__restrict__ src, double *__restrict__ rtmp) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] == 1) return;
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  for (int k = 0; k < n; k++) at32(rtmp, base + (uint32_t)k * 64u) = at32(src, base + (uint32_t)k * 64u);
+}
+
 template <typename T>
 __global__ void k_fill(T *p, int64_t n, T v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,6 +428,24 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
   return 0;
 }
 
+// OperatorQC over all tiles (qprecond=1).
+template <int MODE>
+int launch_qcapply(sunhip_grid *g, CGScal *sc, const QVec<double, double> &V, const Red &R, int pk = -1) {
+  const int nb = nb4(g);
+  const Geo G = g->geo();
+  const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
+  with_nf(g->mf, [&](auto nfc) {
+    constexpr int NF = decltype(nfc)::value;
+    k_qcapply<MODE, NF><<<nb, 256, 0, g->stream>>>(G, V, sc, R, g->NT);
+    return 0;
+  });
+  prof_end(g, ps);
+  HIPCHK(hipGetLastError());
+  if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, sc, R.op);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 template <typename T>
 int launch_noprec(sunhip_grid *g, const QSolver<T> &S, int update, int op) {
   const int nb = nb4(g);
@@ -422,8 +484,15 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
   V.pnew = (it & 1) ? S.p0 : S.p1;
   V.x = S.x;
   V.out = S.z;
-  int rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, Red{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY},
-                                        pk_apply);
+  V.fcoef = S.fcoef;
+  const Red RA{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY};
+  int rc;
+  if constexpr (std::is_same<T, double>::value) {
+    if (prm->precond == 1) rc = launch_qcapply<QM_FUSED>(g, S.sc, V, RA, pk_apply);
+    else rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, RA, pk_apply);
+  } else {
+    rc = launch_apply<QM_FUSED, T, T>(g, S.sc, V, RA, pk_apply);
+  }
   if (rc) return rc;
   if (prm->precond == 2)
     return launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1, pk_prec);
@@ -458,8 +527,8 @@ int check_q(sunhip_grid *g, const sunhip_cg_params *prm) {
     set_error("sunhip_q_*: call sunhip_set_extents first");
     return -1;
   }
-  if (prm->precond != 0 && prm->precond != 2) {
-    set_error("qprecond=%d is not supported on the device (0 or 2)", prm->precond);
+  if (prm->precond != 0 && prm->precond != 1 && prm->precond != 2) {
+    set_error("qprecond=%d is not supported (0, 1 or 2)", prm->precond);
     return -1;
   }
   if (g->maxdepth > 128) { set_error("column depth %d exceeds 128", g->maxdepth); return -1; }
@@ -482,7 +551,27 @@ QSolver<double> solver64(sunhip_grid *g, F64 &B) {
                          B.p0->d, B.p1->d, B.bq->d};
 }
 
-// coefficients, b', r = src - L x, then cg_start (main/phys.c:2441-2475)
+// The face coefficients of qprecond=1: one row of the cell layout per face
+// slot of the kernels' face-count class (>= maxfaces).
+int qc_fcoef(sunhip_grid *g, double **out) {
+  auto it = g->fields.find("#qfcoef");
+  if (it == g->fields.end()) {
+    Field f;
+    f.kind = CELL;
+    f.n = (size_t)nf_class(g->mf) * (size_t)g->NT;
+    if (int rc = dev_alloc(g, (void **)&f.d, f.n * sizeof(double))) return rc;
+    HIPCHK(hipMemsetAsync(f.d, 0, f.n * sizeof(double), g->stream));
+    it = g->fields.emplace("#qfcoef", f).first;
+  }
+  *out = it->second.d;
+  return 0;
+}
+
+// coefficients, b', r = src - L x, then cg_start (main/phys.c:2423-2475).
+// qprecond=1: c = ConditionQ into "qcond" (whose other columns keep what the
+// caller put there: the reference reads a boundary neighbour's c as it
+// finds it), src /= c and q *= c on the computational columns, the scaled
+// coefficients, and OperatorQC in place of OperatorQ.
 int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double> &S, CGScal *stage = nullptr,
             CGScal *sc = nullptr) {
   int rc = check_q(g, prm);
@@ -491,12 +580,47 @@ int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double>
   S = solver64(g, B);
   if (sc) S.sc = sc;
   if ((rc = upload_scal(g, S.sc, prm, stage))) return rc;
-  k_qcoef<<<nb4(g), 256, 0, g->stream>>>(g->geo(), B.dzz->d, B.coef->d);
-  if (prm->precond == 2 && (rc = launch_factor<double>(g, B.coef->d, B.bq->d))) return rc;
+  const Geo G = g->geo();
   QVec<double, double> V{};
   V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = B.r->d;
-  if ((rc = launch_apply<QM_RESID, double, double>(g, S.sc, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
-  return cg_start<double>(g, prm, S);
+  if (prm->precond == 1) {
+    Field *c;
+    double *fc;
+    if ((rc = field(g, "qcond", &c)) || (rc = qc_fcoef(g, &fc))) return rc;
+    with_nf(g->mf, [&](auto nfc) {
+      constexpr int NF = decltype(nfc)::value;
+      k_qcond<NF><<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.D->d, c->d);
+      k_qscale<<<nb4(g), 256, 0, g->stream>>>(G, B.src->d, c->d, 0);
+      k_qscale<<<nb4(g), 256, 0, g->stream>>>(G, B.q->d, c->d, 1);
+      k_qcoef_pc1<NF><<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.D->d, c->d, B.coef->d, fc, g->NT);
+      return 0;
+    });
+    HIPCHK(hipGetLastError());
+    S.fcoef = fc;
+    V.fcoef = fc;
+    if ((rc = launch_qcapply<QM_RESID>(g, S.sc, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
+  } else {
+    k_qcoef<<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
+    if (prm->precond == 2 && (rc = launch_factor<double>(g, B.coef->d, B.bq->d))) return rc;
+    if ((rc = launch_apply<QM_RESID, double, double>(g, S.sc, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
+  }
+  if ((rc = cg_start<double>(g, prm, S))) return rc;
+  k_qbnd_src<<<nb4(g), 256, 0, g->stream>>>(G, B.src->d, B.rtmp->d);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// After the last x update, qprecond=1 rescales x: by c (main/phys.c:2539-2546)
+// or, when the caller's c aliases the residual rows as Solve's does
+// (main/phys.c:1280 passes phys->stmp3, which CGSolveQ uses as r), by the
+// final residual, as the reference then does.
+int q_unscale(sunhip_grid *g, const sunhip_cg_params *prm, const F64 &B) {
+  if (prm->precond != 1) return 0;
+  Field *c;
+  if (int rc = field(g, "qcond", &c)) return rc;
+  k_qscale<<<nb4(g), 256, 0, g->stream>>>(g->geo(), B.q->d, prm->qc_alias ? B.r->d : c->d, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int q_stats(sunhip_grid *g, CGScal *d, sunhip_cg_stats *st) {
@@ -619,6 +743,7 @@ int sunhip_q_solve(sunhip_grid *g, const sunhip_cg_params *prm, sunhip_cg_stats 
   if ((rc = q_setup(g, prm, B, S))) return rc;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
   if ((rc = cg_loop<double>(g, prm, S, PK_QAPPLY, PK_QPREC))) return rc;
+  if ((rc = q_unscale(g, prm, B))) return rc;
   return q_stats(g, g->d_qs, st);
 }
 
@@ -637,6 +762,7 @@ int sunhip_q_solve_fixed(sunhip_grid *g, const sunhip_cg_params *prm, int n, sun
     if ((rc = cg_iteration<double>(g, &p, S, it, PK_QAPPLY, PK_QPREC))) return rc;
   k_qfinish<double><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
+  if ((rc = q_unscale(g, &p, B))) return rc;
   return q_stats(g, g->d_qs, st);
 }
 

@@ -33,6 +33,7 @@ struct QVec {
   const T *rtmp, *pold;    // FUSED: p = rtmp + mu*pold
   T *pnew, *x;             // FUSED
   TO *out;                 // L p, or src - L x
+  const T *fcoef;          // qprecond=1: face coefficients, maxfaces rows of the cell layout
 };
 
 __device__ inline void q_op(CGScal *sc, int op, const double *tot) {
@@ -245,6 +246,159 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
     vprev = v[U - 1];
   }
   return dot;
+}
+
+// ---- qprecond=1: the symmetrically scaled operator ------------------------
+// The face levels [kmn, kmx) and the face's D of every face of slot s with a
+// neighbour (nbb >= 0); kmx = 0 for the other faces.
+template <int NF>
+__device__ __forceinline__ void qc_faces(const Geo &G, const double *__restrict__ D, int s, uint32_t (&nbb)[NF],
+                                         int (&kmn)[NF], int (&kmx)[NF], double (&Df)[NF]) {
+  const int NS = G.NS, nf = G.nf[s];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int nb = f < nf ? G.nbb[f * NS + s] : -1;
+    if (nb >= 0) {
+      const int kr = G.kr[f * NS + s];
+      nbb[f] = (uint32_t)nb; kmn[f] = kr & 0xffff; kmx[f] = kr >> 16;
+      Df[f] = D[G.es[f * NS + s]];
+    } else {
+      nbb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
+    }
+  }
+}
+
+// ConditionQ (main/phys.c:3629-3681) of tile t's computational columns:
+// c = sqrt of the diagonal of L (faces dzz*D in face order, then the
+// vertical coefficients as QCoefficients forms them), 1 where the diagonal
+// is not positive.
+template <int NF>
+__device__ __forceinline__ void qcond_tile(const Geo &G, const double *__restrict__ dzz, const double *__restrict__ D,
+                                           double *__restrict__ c, int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int kt = G.ctop[s];
+  const double Ac = G.Ac[s];
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  uint32_t nbb[NF];
+  int kmn[NF], kmx[NF];
+  double Df[NF];
+  qc_faces<NF>(G, D, s, nbb, kmn, kmx, Df);
+  auto Z = [&](int k) { return at32(dzz, base + (uint32_t)k * 64u); };
+  // a[k] of main/phys.c:3667-3669
+  auto A = [&](int k) { return k == kt ? Ac / Z(kt) : 2 * Ac / (Z(k) + Z(k - 1)); };
+  for (int k = kt; k < n; k++) {
+    const double zk = Z(k);
+    double x = 0;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+      if (k >= kmn[f] && k < kmx[f]) x += zk * Df[f];
+    if (kt < n - 1) {
+      if (k == kt) x += 2 * A(kt) + A(kt + 1);
+      else if (k == n - 1) x += A(n - 1);
+      else x += (A(k) + A(k + 1));
+    }
+    if (x <= 0) x = 1;
+    at32(c, base + (uint32_t)k * 64u) = sqrt(x);
+  }
+}
+
+// QCoefficients with qprecond=1 (main/phys.c:3426-3454): coef scaled by the
+// conditioning c of the two levels it couples, the face coefficients
+// dzz*D/(c c_neighbour) into fc (NF rows of the cell layout, row f at
+// f*NT).  Every face coefficient of a level outside the face's range is 0.
+template <int NF>
+__device__ __forceinline__ void qcoef_pc1_tile(const Geo &G, const double *__restrict__ dzz,
+                                               const double *__restrict__ D, const double *__restrict__ c,
+                                               double *__restrict__ coef, double *__restrict__ fc, int64_t NT,
+                                               int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int kt = G.ctop[s];
+  const double Ac = G.Ac[s];
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  uint32_t nbb[NF];
+  int kmn[NF], kmx[NF];
+  double Df[NF];
+  qc_faces<NF>(G, D, s, nbb, kmn, kmx, Df);
+  auto Z = [&](int k) { return at32(dzz, base + (uint32_t)k * 64u); };
+  auto Cc = [&](int k) { return at32(c, base + (uint32_t)k * 64u); };
+  at32(coef, base + (uint32_t)kt * 64u) = Ac / Z(kt) / Cc(kt);
+  for (int k = kt + 1; k < n; k++) at32(coef, base + (uint32_t)k * 64u) = 2 * Ac / (Z(k) + Z(k - 1)) / (Cc(k) * Cc(k - 1));
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    double *row = fc + (int64_t)f * NT;
+    for (int k = 0; k < n; k++) {
+      const uint32_t idx = base + (uint32_t)k * 64u;
+      at32(row, idx) = (k >= kmn[f] && k < kmx[f]) ? Z(k) * Df[f] / (Cc(k) * at32(c, nbb[f] + (uint32_t)k * 64u))
+                                                  : 0.0;
+    }
+  }
+}
+
+// OperatorQC (main/phys.c:3364-3409) over tile t in the modes of
+// qapply_tile (PLAIN, RESID, FUSED): y = -v, plus v_neighbour * fcoef over
+// the faces in face order, plus the scaled vertical neighbours (top: the one
+// below; bottom: the one above).  In FUSED mode v = rtmp + mu*p_old is
+// formed on the fly for the column and its neighbours, p and x are written
+// as in qapply_tile and the lane's p.(L p) is returned.
+template <int MODE, int NF>
+__device__ __forceinline__ double qcapply_tile(const Geo &G, const QVec<double, double> &V, double mu, double nu,
+                                               int64_t NT, int t, int l) {
+  if (t >= G.Nt) return 0;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return 0;
+  const int kt = G.ctop[s];
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  uint32_t nbb[NF];
+  int kmn[NF], kmx[NF];
+  double Df[NF];
+  qc_faces<NF>(G, V.D, s, nbb, kmn, kmx, Df);
+  auto val = [&](uint32_t idx) {
+    return MODE == QM_FUSED ? at32(V.rtmp, idx) + mu * at32(V.pold, idx) : at32(V.vin, idx);
+  };
+  double dot = 0;
+  for (int k = kt; k < n; k++) {
+    const uint32_t idx = base + (uint32_t)k * 64u;
+    const double v0 = val(idx);
+    double y = -v0;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+      if (k >= kmn[f] && k < kmx[f]) y += val(nbb[f] + (uint32_t)k * 64u) * at32(V.fcoef + (int64_t)f * NT, idx);
+    if (kt < n - 1) {
+      if (k == kt) y += at32(V.coef, idx + 64u) * val(idx + 64u);
+      else if (k == n - 1) y += at32(V.coef, idx) * val(idx - 64u);
+      else y += at32(V.coef, idx) * val(idx - 64u) + at32(V.coef, idx + 64u) * val(idx + 64u);
+    }
+    if (MODE == QM_FUSED) {
+      const double po = at32(V.pold, idx);
+      at32(V.pnew, idx) = v0;
+      at32(V.x, idx) = at32((const double *)V.x, idx) + nu * po;
+      at32(V.out, idx) = y;
+      dot += v0 * y;
+    } else if (MODE == QM_RESID) {
+      at32(V.out, idx) = at32(V.src, idx) - y;
+    } else {
+      at32(V.out, idx) = y;
+    }
+  }
+  return dot;
+}
+
+// Element-wise a = a op b on tile t's computational levels: op 0 a /= b,
+// op 1 a *= b (the scalings of main/phys.c:2425-2433, 2539-2546).
+__device__ __forceinline__ void qscale_tile(const Geo &G, double *__restrict__ a, const double *__restrict__ b, int op,
+                                            int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  for (int k = G.ctop[s]; k < n; k++) {
+    const uint32_t idx = base + (uint32_t)k * 64u;
+    at32(a, idx) = op == 0 ? at32(a, idx) / at32(b, idx) : at32(a, idx) * at32(b, idx);
+  }
 }
 
 __device__ __forceinline__ int wave_min_i(int v) {

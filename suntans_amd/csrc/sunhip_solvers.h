@@ -16,6 +16,7 @@ struct QSolver {
   CGScal *sc;
   const T *coef, *dzf, *D;
   T *x, *r, *z, *rtmp, *p0, *p1, *bq;
+  const T *fcoef;  // qprecond=1: the scaled face coefficients
 };
 
 // One free-surface PCG instance.

@@ -31,7 +31,7 @@ class Extents(C.Structure):
 
 class CGParams(C.Structure):
     _fields_ = [("maxiters", C.c_int), ("precond", C.c_int), ("resnorm", C.c_int),
-                ("epsilon", C.c_double), ("check_every", C.c_int)]
+                ("epsilon", C.c_double), ("check_every", C.c_int), ("qc_alias", C.c_int)]
 
 
 class CGStats(C.Structure):
@@ -269,8 +269,11 @@ class DeviceGrid:
     def sync(self):
         check(lib().sunhip_sync(self.h), "sync")
 
-    def q_solve(self, maxiters=2000, precond=2, resnorm=0, epsilon=1e-10, check_every=0, fixed=None):
-        P = CGParams(maxiters, precond, resnorm, epsilon, check_every)
+    def q_solve(self, maxiters=2000, precond=2, resnorm=0, epsilon=1e-10, check_every=0, fixed=None, qc_alias=0):
+        """CGSolveQ on the resident fields.  precond 1: qc_alias = 1 divides
+        the scaled solution by the final residual at the end, as the
+        reference does when its c aliases r (Solve's call)."""
+        P = CGParams(maxiters, precond, resnorm, epsilon, check_every, qc_alias)
         S = CGStats()
         if fixed is None:
             check(lib().sunhip_q_solve(self.h, C.byref(P), C.byref(S)), "sunhip_q_solve")

@@ -47,8 +47,9 @@ class _Cpu:
         return getattr(self.L, self.pre + name)
 
     # calls mirror the reference call sites
-    def cgsolveq(self, st):
-        self.fn("CGSolveQ")(st.ptr("qc"), st.ptr("stmp"), st.ptr("stmp3"), st.g, st.p, st.prop)
+    def cgsolveq(self, st, c="stmp3"):
+        """c: the conditioning rows (Solve passes stmp3, the residual rows)."""
+        self.fn("CGSolveQ")(st.ptr("qc"), st.ptr("stmp"), st.ptr(c), st.g, st.p, st.prop)
 
     def cgsolve(self, st):
         self.fn("CGSolve")(st.g, st.p, st.prop)
@@ -169,8 +170,8 @@ class RefImpl(_Cpu):
         self._last[which] = (int(m.group(1)), float(m.group(2))) if m else (None, None)
         return out
 
-    def cgsolveq(self, st):
-        self._captured(lambda: super(RefImpl, self).cgsolveq(st), "q")
+    def cgsolveq(self, st, c="stmp3"):
+        self._captured(lambda: super(RefImpl, self).cgsolveq(st, c), "q")
 
     def cgsolve(self, st):
         self._captured(lambda: super(RefImpl, self).cgsolve(st), "h")
@@ -186,8 +187,8 @@ class DropinImpl:
     def __init__(self):
         self.L = native.dropin_lib()
 
-    def cgsolveq(self, st):
-        self.L.CGSolveQ(st.ptr("qc"), st.ptr("stmp"), st.ptr("stmp3"), st.g, st.p, st.prop, 0, 1, 0)
+    def cgsolveq(self, st, c="stmp3"):
+        self.L.CGSolveQ(st.ptr("qc"), st.ptr("stmp"), st.ptr(c), st.g, st.p, st.prop, 0, 1, 0)
 
     def cgsolve(self, st):
         self.L.CGSolve(st.g, st.p, st.prop, 0, 1, 0)
