@@ -135,6 +135,14 @@ int sunhip_field_scale(sunhip_grid *g, const char *dst, const char *src, double 
  * boundary neighbour's c as it finds it), and prm->qc_alias selects the
  * final rescaling. */
 int sunhip_q_solve(sunhip_grid *g, const sunhip_cg_params *prm, sunhip_cg_stats *st);
+/* The reference layer thicknesses grid->dz (Nkmax values, host memory).  No
+ * reference counterpart: with them the pressure solve finds, per column and
+ * face, the levels where QCoefficients' coef and the face heights dzf are
+ * what dz gives (the z-level grid's full cells) and computes them there
+ * instead of reading them; elsewhere it reads them.  Results are the same
+ * bits either way (tuning "zreg" 0 switches it off).  sunhip_update_dz and
+ * sunhip_set_flux_height set them too. */
+int sunhip_set_dz(sunhip_grid *g, const double *dz);
 /* y = OperatorQ(x) on named device fields (QCoefficients from "dzz"). */
 int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y);
 /* xc = Preconditioner(x) (vertical-line Thomas solve per column). */
@@ -167,7 +175,8 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * of each column whose Thomas pivots and forward values the preconditioner
  * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
  * at most 16; -1, the default: 24 from 1024 tiles of 64 columns up in FP64,
- * 16 from 2048 tiles up in FP32, else 0), "xchg" (the iteration's operator
+ * 16 from 2048 tiles up in FP32, else 0), "zreg" (1, the default: the z-level windows of sunhip_set_dz; 0 off),
+ * "xchg" (the iteration's operator
  * kernel exchanges neighbour values inside each wave through LDS, gathering
  * only cross-tile neighbours from memory: 0 off, 1 on, 2 on with a register
  * budget of 4 waves per SIMD, 3 on with 2 levels per load group; -1, the

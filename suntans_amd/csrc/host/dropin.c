@@ -148,6 +148,7 @@ static ctx_t *get_ctx(gridT *g, const char *where) {
   T.cellp = g->cellp; T.celldist = g->celldist; T.edgep = g->edgep; T.edgedist = g->edgedist;
   T.Ac = g->Ac; T.df = g->df; T.dg = g->dg; T.xv = g->xv; T.yv = g->yv;
   check(sunhip_grid_create(&c->dev, &T, g_device), where);
+  if (g->dz) check(sunhip_set_dz(c->dev, g->dz), where);  /* the z-level windows of the pressure solve */
   int64_t sz[6];
   sunhip_grid_sizes(c->dev, sz);
   size_t n = (size_t)sz[1];

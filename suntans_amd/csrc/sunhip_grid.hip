@@ -391,8 +391,11 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   const int NS = g->NS, NES = g->NES;
   auto cell_elem = [&](int c) { int s = g->slot_of_cell[c]; return g->toff[s / 64] + (s % 64); };
   auto edge_elem = [&](int e) { int s = g->slot_of_edge[e]; return g->eoff[s / 64] + (s % 64); };
-  std::vector<int> nk(NS, 0), nf(NS, 0), comp(NS, 0), nbb(mf * NS, -1), nbs(mf * NS, -1), eb(mf * NS, -1),
-      es(mf * NS, -1), nrm(mf * NS, 0);
+  // per-face tables: one row of NS per face slot of the kernels' face-count
+  // class (>= maxfaces), so a kernel may load every row of its class
+  const int mfc = nf_class(mf);
+  std::vector<int> nk(NS, 0), nf(NS, 0), comp(NS, 0), nbb(mfc * NS, -1), nbs(mfc * NS, -1), eb(mfc * NS, -1),
+      es(mfc * NS, -1), nrm(mfc * NS, 0);
   std::vector<double> Ac(NS, 0.0);
   std::vector<int64_t> cst(NS, -1), cstw(NS, -1);
   for (int s = 0; s < NS; s++) {
@@ -432,7 +435,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   for (int jp = T->edgedist[2]; jp < T->edgedist[3]; jp++) ecomp[g->slot_of_edge[T->edgep[jp]]] = 2;
   // each column's faces in the order their edges appear in edgep (the order
   // of the reference's edge loops that scatter onto cells)
-  std::vector<int> ford(mf * NS, -1);
+  std::vector<int> ford(mfc * NS, -1);
   {
     std::vector<int> pos(Ne, 0);
     for (int jp = 0; jp < Ne; jp++) pos[T->edgep[jp]] = jp;
@@ -466,7 +469,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   {
     // neighbour exchange: same-tile neighbours by lane, the others as a
     // compact list of cross-tile (face, slot) pairs per tile
-    std::vector<int> xcode(mf * NS, -1), pair((size_t)g->Nt * SUNHIP_XPAIRS, -1), npair(g->Nt, 0);
+    std::vector<int> xcode(mfc * NS, -1), pair((size_t)g->Nt * SUNHIP_XPAIRS, -1), npair(g->Nt, 0);
     for (int t = 0; t < g->Nt; t++) {
       int m = 0;
       for (int l = 0; l < SUNHIP_TILE; l++) {
@@ -511,7 +514,7 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
     rc |= upload_vec(g, &g->d_cellslot, cs);
     rc |= upload_vec(g, &g->d_edgeslot, esv);
   }
-  std::vector<int> zeros(NS, 0), ezeros(NES, 0), kr0(mf * NS, 0);
+  std::vector<int> zeros(NS, 0), ezeros(NES, 0), kr0(mfc * NS, 0);
   rc |= upload_vec(g, &g->d_ctop, zeros);
   rc |= upload_vec(g, &g->d_ctopold, zeros);
   rc |= upload_vec(g, &g->d_kr, kr0);
@@ -554,7 +557,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
                   g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold,
-                  g->d_ford};
+                  g->d_ford, g->d_zdzt, g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   for (int i = 0; i < 2; i++) {

@@ -125,6 +125,15 @@ struct sunhip_grid {
   int *d_xcode = nullptr, *d_pair = nullptr, *d_npair = nullptr;  // neighbour exchange (Geo)
   int *d_uerr = nullptr;             // UPredictor's exit conditions
   double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)
+  // z-level windows of the pressure solve (ZReg): dz[k] (Nkmax + 16
+  // entries, zero past Nkmax), the windows, exceptions and tile flags of the
+  // current solve
+  double *d_zdzt = nullptr;
+  double zden = 0;                   // the common dz[k] + dz[k-1], 0 when the layers differ
+  bool have_zdz = false;
+  int *d_cwin = nullptr, *d_fwin = nullptr, *d_tflag = nullptr;
+  double *d_cex = nullptr, *d_fex = nullptr;
+  int tune_zreg = 1;
   int *d_e_etopold = nullptr;        // [NES] etop before the last UpdateDZ
   int *d_ford = nullptr;             // [MAXF*NS] each column's faces in edgep order of their edges
   int n_type4 = 0;                   // no-slip (type-4) edges

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "sunhip_solvers.h"
 
@@ -115,13 +116,16 @@ __global__ __launch_bounds__(256) void k_qfactor(Geo G, const T *__restrict__ co
 // y = L v (qapply_tile's modes) over all tiles, TPB tiles per block.
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
 // compiler's choice).
-template <int MODE, int NF, int U, typename T, typename TO, int TPB = 4, bool NTS = false, int MINW = 0>
+template <int MODE, int NF, int U, typename T, typename TO, int TPB = 4, bool NTS = false, int MINW = 0,
+          bool ZR = false>
 __global__ __launch_bounds__(TPB * 64, MINW > 0 ? MINW : 1) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
   if (MODE == QM_FUSED && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * TPB + (threadIdx.x >> 6);
   const T mu = MODE == QM_FUSED ? (T)sc->mu : T(0);
   const T nu = MODE == QM_FUSED ? (T)sc->nu : T(0);
-  const double dot = qapply_tile<MODE, NF, U, T, TO, NTS>(G, V, mu, nu, t, threadIdx.x & 63);
+  const double dot = (ZR && t < G.Nt && V.Z.tflag[t])
+                         ? qapply_tile<MODE, NF, U, T, TO, NTS, true>(G, V, mu, nu, t, threadIdx.x & 63)
+                         : qapply_tile<MODE, NF, U, T, TO, NTS, false>(G, V, mu, nu, t, threadIdx.x & 63);
   if (MODE == QM_FUSED || MODE == QM_RESID_DOT) {
     double dv[1] = {dot};
     reduce_out<1, TPB * 64>(dv, R, sc);
@@ -156,27 +160,51 @@ __global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal 
 
 // The iteration's operator kernel with the neighbour exchange inside each
 // wave (qapply_xchg); 4 tiles per block, one exchange row per wave.
-template <int NF, int U, typename T, bool NTS, int MINW = 0>
+template <int NF, int U, typename T, bool NTS, int MINW = 0, bool ZR = false>
 __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_x(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
   if (sc->done) return;
   __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
   const int w = threadIdx.x >> 6;
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
-  const double dot = qapply_xchg<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w]);
+  const double dot =
+      (ZR && t < G.Nt && V.Z.tflag[t])
+          ? qapply_xchg<NF, U, T, NTS, true>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w])
+          : qapply_xchg<NF, U, T, NTS, false>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w]);
+  double v[1] = {dot};
+  reduce_out<1, 256>(v, R, sc);
+}
+
+// The pipelined exchange kernel (qapply_xchg_pipe) for tiles that fit
+// their z-level windows and have at most 64 cross-tile pairs, qapply_xchg
+// for the others.
+template <int NF, int U, typename T, bool NTS, int MINW = 0>
+__global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_xp(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
+  if (sc->done) return;
+  __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
+  const int w = threadIdx.x >> 6;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+  const double dot =
+      (t < G.Nt && V.Z.tflag[t] && G.npair[t] <= 64)
+          ? qapply_xchg_pipe<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w])
+          : qapply_xchg<NF, U, T, NTS, false>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w]);
   double v[1] = {dot};
   reduce_out<1, 256>(v, R, sc);
 }
 
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
-template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0>
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
-                                               CGScal *sc, Red R) {
+                                               CGScal *sc, Red R, ZReg Z) {
   if (UPDATE && sc->done) return;
   const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
-  qprec_tile<UPDATE, U, T, RZID, KR>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t, threadIdx.x & 63, rz,
-                                     rr);
+  if (ZR && t < G.Nt && Z.tflag[t])
+    qprec_tile<UPDATE, U, T, RZID, KR, true>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
+                                             threadIdx.x & 63, rz, rr, &Z);
+  else
+    qprec_tile<UPDATE, U, T, RZID, KR, false>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
+                                              threadIdx.x & 63, rz, rr, &Z);
   double v[2] = {rz, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -201,6 +229,76 @@ __global__ __launch_bounds__(256) void k_qfinish(Geo G, T *__restrict__ x, const
   if (sc->iter == 0) return;
   qfinish_tile<T>(G, x, (sc->iter & 1) ? p1 : p0, (T)sc->nu, blockIdx.x * 4 + (threadIdx.x >> 6),
                   threadIdx.x & 63);
+}
+
+// The z-level windows (ZReg) of every computational column and face, their
+// exceptions, and whether every column of the tile fits them.  A column's
+// window starts below its top cell at the first level whose stored coef is
+// 2 Ac/den bit for bit and runs while it is; it fits when at most two levels
+// lie above it and two below.  A face's window starts at its first level
+// where the stored dzf is dz[k] and runs while it is; it fits with at most
+// one level on either side.
+template <int NF>
+__global__ __launch_bounds__(256) void k_qwin(Geo G, const double *__restrict__ coef, const double *__restrict__ dzf,
+                                              double den, const double *__restrict__ dzt, int *__restrict__ cwin,
+                                              int *__restrict__ fwin, double *__restrict__ cex,
+                                              double *__restrict__ fex, int *__restrict__ tflag) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s], NS = G.NS;
+  int cw = 0, fw[NF];
+  double ce[4] = {0, 0, 0, 0}, fe[2][NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) { fw[f] = 0; fe[0][f] = 0; fe[1][f] = 0; }
+  bool fits = true;
+  if (n > 0 && G.comp[s] == 1) {
+    const int kt = G.ctop[s];
+    const uint32_t base = (uint32_t)(G.toff[t] + l);
+    auto cv = [&](int k) { return at32(coef, base + (uint32_t)k * 64u); };
+    if (den > 0) {
+      const double ci = 2 * G.Ac[s] / den;
+      int c0 = kt + 1;
+      while (c0 < n && !(cv(c0) == ci)) c0++;
+      int c1 = c0;
+      while (c1 < n && cv(c1) == ci) c1++;
+      fits = c0 - kt <= 2 && n - c1 <= 2;
+      cw = c0 | (c1 << 16);
+      ce[0] = cv(kt);
+      if (kt + 1 < n) ce[1] = cv(kt + 1);
+      if (n - 2 >= kt) ce[2] = cv(n - 2);
+      ce[3] = cv(n - 1);
+    } else {
+      fits = false;
+    }
+    const int nf = G.nf[s];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      if (f >= nf || G.nbb[f * NS + s] < 0) continue;
+      const int kr = G.kr[f * NS + s], kmn = kr & 0xffff, kmx = kr >> 16;
+      if (kmx <= kmn) continue;
+      const uint32_t eb = (uint32_t)G.eb[f * NS + s];
+      auto dv = [&](int k) { return at32(dzf, eb + (uint32_t)k * 64u); };
+      int e0 = kmn;
+      while (e0 < kmx && !(dv(e0) == dzt[e0])) e0++;
+      int e1 = e0;
+      while (e1 < kmx && dv(e1) == dzt[e1]) e1++;
+      fits = fits && e0 - kmn <= 1 && kmx - e1 <= 1;
+      fw[f] = e0 | (e1 << 16);
+      fe[0][f] = dv(kmn);
+      fe[1][f] = dv(kmx - 1);
+    }
+  }
+  cwin[s] = cw;
+#pragma unroll
+  for (int i = 0; i < 4; i++) cex[i * NS + s] = ce[i];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    fwin[f * NS + s] = fw[f];
+    fex[f * NS + s] = fe[0][f];
+    fex[(NF + f) * NS + s] = fe[1][f];
+  }
+  const int all = wave_min_i(fits ? 1 : 0);
+  if (l == 0) tflag[t] = all;
 }
 
 // ---- qprecond=1 (ConditionQ, its QCoefficients branch, OperatorQC) ------
@@ -294,17 +392,24 @@ int launch_factor(sunhip_grid *g, const T *coef, T *bq) {
 // (OP_NONE for the plain drop-in application).
 template <bool UPDATE, typename T>
 int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *out, const T *bq, int op,
-                unsigned *cnt, int pk = -1) {
+                unsigned *cnt, int pk = -1, const ZReg *Z = nullptr) {
   const Geo G = g->geo();
   const int nb = nb4(g);
   Red R{g->d_part, 0, (g->tune_reduce == 0 && op != OP_NONE) ? cnt : nullptr, op};
   // the timing pair brackets the column kernel alone, not the finisher
   const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
   const bool rzid = g->tune_rzid && op != OP_NONE;
+  const bool zr = Z && Z->cwin;
+  const ZReg ZZ = zr ? *Z : ZReg{};
 #define PREC(U, KR) \
   do { \
-    if (rzid) k_qprec<UPDATE, U, T, true, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
-    else k_qprec<UPDATE, U, T, false, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R); \
+    if (zr) { \
+      if (rzid) k_qprec<UPDATE, U, T, true, KR, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      else k_qprec<UPDATE, U, T, false, KR, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+    } else { \
+      if (rzid) k_qprec<UPDATE, U, T, true, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      else k_qprec<UPDATE, U, T, false, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+    } \
   } while (0)
   // FP32 columns keep at most 16 levels: beyond that the register head costs
   // more occupancy than it saves traffic
@@ -344,28 +449,62 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
   const int nba = (g->Nt + tpb - 1) / tpb;
   const Geo G = g->geo();
   const int ps = pk >= 0 ? prof_begin(g, pk) : -1;
-#define APPLY(NF, U) k_qapply<MODE, NF, U, T, TO><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
+  // the z-level windows, when the solve has them (the plain and exchange kernels)
+  const bool zr = V.Z.cwin != nullptr;
+#define APPLY(NF, U)                                                                              \
+  do {                                                                                            \
+    if (zr) k_qapply<MODE, NF, U, T, TO, 4, false, 0, true><<<nba, 256, 0, g->stream>>>(G, V, sc, R); \
+    else k_qapply<MODE, NF, U, T, TO><<<nba, 256, 0, g->stream>>>(G, V, sc, R);                      \
+  } while (0)
 #define APPLYV(TPB, NTS) k_qapply<MODE, 3, 4, T, TO, TPB, NTS><<<nba, TPB * 64, 0, g->stream>>>(G, V, sc, R)
 #define APPLYW(NTS, W) k_qapply<MODE, 3, 4, T, TO, 4, NTS, W><<<nba, 256, 0, g->stream>>>(G, V, sc, R)
   if constexpr (MODE == QM_FUSED && std::is_same<T, TO>::value) {
     // the exchange kernel, unless another launch shape was asked for
     const int xmode = g->tune_xchg >= 0 ? g->tune_xchg : (g->Nt >= 4096 ? 2 : 0);
+    if (zr && xmode >= 4 && g->max_pairs <= SUNHIP_XPAIRS && g->tune_unroll == 4) {
+      // the pipelined exchange kernel: 4 two-level groups (xchg 4), 4 four-level
+      // groups under a 4-wave register budget (5), or 2 two-level groups at
+      // 4 waves (6)
+      const int nbx = (g->Nt + 3) / 4;
+      with_nf(g->mf, [&](auto nfc) {
+        constexpr int NF = decltype(nfc)::value;
+        if (xmode == 5 && NF <= 4) {
+          if (g->tune_nts) k_qapply_xp<NF, 4, T, true, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_xp<NF, 4, T, false, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        } else if (xmode == 6) {
+          if (g->tune_nts) k_qapply_xp<NF, 2, T, true, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_xp<NF, 2, T, false, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        } else {
+          if (g->tune_nts) k_qapply_xp<NF, 2, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          else k_qapply_xp<NF, 2, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+        }
+        return 0;
+      });
+      prof_end(g, ps);
+      HIPCHK(hipGetLastError());
+      if (R.op != OP_NONE && !R.cnt) k_qfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nbx, sc, R.op);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
     const bool xchg = xmode && g->max_pairs <= SUNHIP_XPAIRS && g->tune_tiles == 4 && !g->tune_blk &&
                       !g->tune_band && !g->tune_minw && g->tune_unroll == 4;
     if (xchg) {
       const int nbx = (g->Nt + 3) / 4;
       with_nf(g->mf, [&](auto nfc) {
         constexpr int NF = decltype(nfc)::value, UX = NF <= 4 ? 4 : 2;
+#define XCH(U, NTS, W)                                                                          \
+  do {                                                                                          \
+    if (zr) k_qapply_x<NF, U, T, NTS, W, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);       \
+    else k_qapply_x<NF, U, T, NTS, W><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);                 \
+  } while (0)
         if (xmode == 2) {  // register budget of 4 waves per SIMD
-          if (g->tune_nts) k_qapply_x<NF, UX, T, true, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
-          else k_qapply_x<NF, UX, T, false, 4><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          if (g->tune_nts) XCH(UX, true, 4); else XCH(UX, false, 4);
         } else if (xmode == 3) {  // 2 levels per group
-          if (g->tune_nts) k_qapply_x<NF, 2, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
-          else k_qapply_x<NF, 2, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          if (g->tune_nts) XCH(2, true, 0); else XCH(2, false, 0);
         } else {
-          if (g->tune_nts) k_qapply_x<NF, UX, T, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
-          else k_qapply_x<NF, UX, T, false><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);
+          if (g->tune_nts) XCH(UX, true, 0); else XCH(UX, false, 0);
         }
+#undef XCH
         return 0;
       });
       prof_end(g, ps);
@@ -462,7 +601,7 @@ template <typename T>
 int cg_start(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S) {
   int rc;
   if (prm->precond == 2)
-    rc = launch_prec<false, T>(g, S.sc, S.coef, nullptr, S.r, S.rtmp, S.bq, OP_PREC_SETUP, g->d_count + 1);
+    rc = launch_prec<false, T>(g, S.sc, S.coef, nullptr, S.r, S.rtmp, S.bq, OP_PREC_SETUP, g->d_count + 1, -1, &S.Z);
   else
     rc = launch_noprec<T>(g, S, 0, OP_NOPREC_SETUP);
   if (rc) return rc;
@@ -485,6 +624,7 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
   V.x = S.x;
   V.out = S.z;
   V.fcoef = S.fcoef;
+  V.Z = S.Z;
   const Red RA{g->d_part, 0, g->tune_reduce == 0 ? g->d_count : nullptr, OP_APPLY};
   int rc;
   if constexpr (std::is_same<T, double>::value) {
@@ -495,7 +635,8 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
   }
   if (rc) return rc;
   if (prm->precond == 2)
-    return launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1, pk_prec);
+    return launch_prec<true, T>(g, S.sc, S.coef, S.z, S.r, S.rtmp, S.bq, OP_PREC_ITER, g->d_count + 1, pk_prec,
+                                &S.Z);
   const int ps = prof_begin(g, pk_prec);
   rc = launch_noprec<T>(g, S, 1, OP_NOPREC_ITER);
   prof_end(g, ps);
@@ -602,6 +743,8 @@ int q_setup(sunhip_grid *g, const sunhip_cg_params *prm, F64 &B, QSolver<double>
   } else {
     k_qcoef<<<nb4(g), 256, 0, g->stream>>>(G, B.dzz->d, B.coef->d);
     if (prm->precond == 2 && (rc = launch_factor<double>(g, B.coef->d, B.bq->d))) return rc;
+    if ((rc = q_windows(g, B.coef->d, B.dzf->d, &S.Z))) return rc;
+    V.Z = S.Z;
     if ((rc = launch_apply<QM_RESID, double, double>(g, S.sc, V, Red{g->d_part, 0, nullptr, OP_NONE}))) return rc;
   }
   if ((rc = cg_start<double>(g, prm, S))) return rc;
@@ -710,10 +853,55 @@ int q_mixed_prepare(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<float> 
   k_cast<double, float><<<(unsigned)((g->NES + 255) / 256), 256, 0, g->stream>>>(B.D->d, g->d_D32, g->NES);
   HIPCHK(hipGetLastError());
   if ((rc = launch_factor<float>(g, c32, b32))) return rc;
+  ZReg Z{};
+  if ((rc = q_windows(g, B.coef->d, B.dzf->d, &Z))) return rc;
   *S32 = QSolver<float>{g->d_ms, c32, f32, g->d_D32, x32, r32, z32, t32, pa, pb, b32};
+  S32->Z = Z;
   QVec<double, float> V{};
   V.coef = B.coef->d; V.dzf = B.dzf->d; V.D = B.D->d; V.vin = B.q->d; V.src = B.src->d; V.out = r32;
+  V.Z = Z;
   *R = V;
+  return 0;
+}
+
+int set_zdz(sunhip_grid *g, const double *dz) {
+  const size_t n = (size_t)g->Nkmax + 16;
+  if (!g->d_zdzt && dev_alloc(g, (void **)&g->d_zdzt, n * sizeof(double))) return -2;
+  std::vector<double> t(n, 0.0);
+  for (int k = 0; k < g->Nkmax; k++) t[k] = dz[k];
+  // the common dz[k] + dz[k-1] (the operands and order of QCoefficients'
+  // dzz[k] + dzz[k-1]), or 0 when the layers differ
+  double den = g->Nkmax > 1 ? dz[1] + dz[0] : 0.0;
+  for (int k = 2; k < g->Nkmax; k++)
+    if (!(dz[k] + dz[k - 1] == den)) den = 0.0;
+  if (!(den > 0)) den = 0.0;
+  HIPCHK(hipMemcpyAsync(g->d_zdzt, t.data(), n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));  // the host vector goes out of scope
+  g->zden = den;
+  g->have_zdz = true;
+  return 0;
+}
+
+int q_windows(sunhip_grid *g, const double *coef, const double *dzf, ZReg *Z) {
+  *Z = ZReg{};
+  if (!g->have_zdz || !g->tune_zreg) return 0;
+  const int NF = nf_class(g->mf);
+  if (!g->d_cwin) {
+    int rc = dev_alloc(g, (void **)&g->d_cwin, sizeof(int) * (size_t)g->NS);
+    if (!rc) rc = dev_alloc(g, (void **)&g->d_fwin, sizeof(int) * (size_t)g->NS * NF);
+    if (!rc) rc = dev_alloc(g, (void **)&g->d_cex, sizeof(double) * (size_t)g->NS * 4);
+    if (!rc) rc = dev_alloc(g, (void **)&g->d_fex, sizeof(double) * (size_t)g->NS * 2 * NF);
+    if (!rc) rc = dev_alloc(g, (void **)&g->d_tflag, sizeof(int) * (size_t)g->Nt);
+    if (rc) return rc;
+  }
+  with_nf(g->mf, [&](auto nfc) {
+    constexpr int NFC = decltype(nfc)::value;
+    k_qwin<NFC><<<nb4(g), 256, 0, g->stream>>>(g->geo(), coef, dzf, g->zden, g->d_zdzt, g->d_cwin, g->d_fwin,
+                                               g->d_cex, g->d_fex, g->d_tflag);
+    return 0;
+  });
+  HIPCHK(hipGetLastError());
+  *Z = ZReg{g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag, g->d_zdzt, g->zden};
   return 0;
 }
 
@@ -827,6 +1015,12 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
   return 0;
 }
 
+int sunhip_set_dz(sunhip_grid *g, const double *dz) {
+  HIPCHK(hipSetDevice(g->device));
+  if (!dz) { set_error("sunhip_set_dz: dz is required"); return -1; }
+  return set_zdz(g, dz);
+}
+
 int sunhip_q_operator(sunhip_grid *g, const char *x, const char *y) {
   HIPCHK(hipSetDevice(g->device));
   if (!g->have_extents) { set_error("sunhip_q_operator: call sunhip_set_extents first"); return -1; }
@@ -884,9 +1078,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     g->tune_phead = value;
   }
   else if (!strcmp(key, "xchg")) {
-    if (value < -1 || value > 3) { set_error("xchg must be -1..3"); return -1; }
+    if (value < -1 || value > 6) { set_error("xchg must be -1..6"); return -1; }
     g->tune_xchg = value;
   }
+  else if (!strcmp(key, "zreg")) g->tune_zreg = value != 0;
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {
     if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }

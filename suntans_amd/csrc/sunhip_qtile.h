@@ -21,6 +21,67 @@
 namespace sunhip {
 
 enum { QM_PLAIN = 0, QM_RESID = 1, QM_FUSED = 2, QM_RESID_DOT = 3 };
+
+// The levels of a z-level grid whose operator coefficients follow from the
+// reference layer thicknesses dz[k] (grid->dz).  Where the layers have one
+// thickness (dz[k] + dz[k-1] the same for every k >= 1), QCoefficients'
+// coef[k] = 2 Ac/(dz[k] + dz[k-1]) is one value per column below its top
+// cell, and a face height dzf[k] is dz[k].  A solve's set-up finds, per
+// column and per face, the window of levels where the stored value equals
+// that expression bit for bit, with at most two levels above and two below
+// it (the surface and bottom cells) for a column and one above and one below
+// for a face; those exceptions are copied per column.  A tile whose columns
+// all fit (tflag) reads no coef and no dzf: its kernels form them from the
+// window, the exceptions and dz.  Other tiles read the stored arrays.  The
+// results are the same bits either way.
+struct ZReg {
+  const int *cwin;     // [NS] c0 | c1 << 16: coef window of the column
+  const int *fwin;     // [NF*NS] f0 | f1 << 16: dzf window of face f (row f*NS)
+  const double *cex;   // [4*NS] coef at ctop, ctop+1, Nk-2, Nk-1 (row i*NS)
+  const double *fex;   // [2*NF*NS] dzf at the face's first and last level (rows f*NS, (NF+f)*NS)
+  const int *tflag;    // [Nt] 1: every computational column of the tile fits its windows
+  const double *dzt;   // dz[k], zero past Nkmax
+  double den;          // the common dz[k] + dz[k-1] (0: the layers differ)
+};
+
+// coef of a column of a regular tile: the interior value inside [c0, c1),
+// the copied exceptions outside it.
+template <typename T>
+__device__ __forceinline__ T zsel(int k, int kt, int n, int c0, int c1, T ci, T e0, T e1, T e2, T e3) {
+  const T lo = k == kt ? e0 : e1, hi = k == n - 1 ? e3 : e2;
+  return k < c0 ? lo : (k < c1 ? ci : hi);
+}
+template <typename T>
+struct ZCol {
+  int kt, n, c0, c1;
+  T ci, e0, e1, e2, e3;
+  __device__ __forceinline__ T at(int k) const { return zsel<T>(k, kt, n, c0, c1, ci, e0, e1, e2, e3); }
+};
+template <typename T>
+__device__ __forceinline__ ZCol<T> zcol(const ZReg &Z, const Geo &G, int s, int kt, int n) {
+  ZCol<T> c;
+  const int w = Z.cwin[s], NS = G.NS;
+  c.kt = kt; c.n = n; c.c0 = w & 0xffff; c.c1 = w >> 16;
+  c.ci = (T)(2 * G.Ac[s] / Z.den);  // QCoefficients' 2*Ac/(dzz[k]+dzz[k-1]) on the window
+  c.e0 = (T)Z.cex[s]; c.e1 = (T)Z.cex[NS + s]; c.e2 = (T)Z.cex[2 * NS + s]; c.e3 = (T)Z.cex[3 * NS + s];
+  return c;
+}
+// dzf of a face of a regular tile at level k of the face's range
+template <typename T>
+struct ZFace {
+  int f0, f1;
+  T top, bot;
+  __device__ __forceinline__ T at(int k, const double *dzt) const { return k < f0 ? top : (k < f1 ? (T)dzt[k] : bot); }
+};
+template <typename T, int NF>
+__device__ __forceinline__ ZFace<T> zface(const ZReg &Z, const Geo &G, int s, int f) {
+  ZFace<T> z;
+  const int NS = G.NS, w = Z.fwin[f * NS + s];
+  z.f0 = w & 0xffff; z.f1 = w >> 16;
+  z.top = (T)Z.fex[f * NS + s]; z.bot = (T)Z.fex[(NF + f) * NS + s];
+  return z;
+}
+
 // finish operations on the CG scalars
 enum { OP_NONE = 0, OP_APPLY = 1, OP_PREC_SETUP = 2, OP_PREC_ITER = 3, OP_NOPREC_SETUP = 4, OP_NOPREC_ITER = 5,
        OP_STORE = 6 };
@@ -34,6 +95,7 @@ struct QVec {
   T *pnew, *x;             // FUSED
   TO *out;                 // L p, or src - L x
   const T *fcoef;          // qprecond=1: face coefficients, maxfaces rows of the cell layout
+  ZReg Z;                  // z-level windows (the ZR kernels)
 };
 
 __device__ inline void q_op(CGScal *sc, int op, const double *tot) {
@@ -143,7 +205,7 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 // outputs with the non-temporal hint.  Returns the lane's share of the dot
 // product.
 
-template <int MODE, int NF, int U, typename T, typename TO, bool NTS = false>
+template <int MODE, int NF, int U, typename T, typename TO, bool NTS = false, bool ZR = false>
 __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V, T mu, T nu, int t, int l,
                                               int k0 = 0, int k1 = 0x7fffffff) {
   double dot = 0;
@@ -163,6 +225,7 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
   uint32_t nbb[NF], ebb[NF];
   int kmn[NF], kmx[NF], lim[NF];
   T Df[NF];
+  ZFace<T> zf[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     const int nbf = f < nf ? G.nbb[f * NS + s] : -1;
@@ -176,8 +239,10 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
     } else {
       nbb[f] = base; ebb[f] = 0; kmn[f] = 0; kmx[f] = 0; Df[f] = 0;
     }
+    if (ZR) zf[f] = zface<T, NF>(V.Z, G, s, f);
     lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
   }
+  const ZCol<T> zc = ZR ? zcol<T>(V.Z, G, s, kt, n) : ZCol<T>{};
   // levels [kbeg, kend) of the column: all of it, or one band [k0, k1)
   const int kbeg = max(kt, k0), kend = min(n, k1);
   T vprev = 0;
@@ -186,13 +251,17 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
     vprev = MODE == QM_FUSED ? at32(rt, idx) + mu * at32(po, idx) : at32(vin, idx);
   }
   for (int kb = kbeg; kb < kend; kb += U) {
-    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF], dzg[U];
+    if (ZR) {
+#pragma unroll
+      for (int j = 0; j < U; j++) dzg[j] = (T)V.Z.dzt[kb + j];  // the group's layer thicknesses (uniform)
+    }
 #pragma unroll
     for (int j = 0; j <= U; j++) {
       const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
       if (MODE == QM_FUSED) { pv[j] = at32(po, idx); v[j] = at32(rt, idx); }
       else { pv[j] = 0; v[j] = at32(vin, idx); }
-      c[j] = at32(coef, idx);
+      if (!ZR) c[j] = at32(coef, idx);  // ZR: formed where used
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -203,7 +272,7 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
         const uint32_t kk = (uint32_t)min(k, lim[f]) * 64u;
         const uint32_t ni = nbb[f] + kk;
         nv[j][f] = MODE == QM_FUSED ? at32(rt, ni) + mu * at32(po, ni) : at32(vin, ni);
-        dz[j][f] = at32(dzf, ebb[f] + kk);
+        if (!ZR) dz[j][f] = at32(dzf, ebb[f] + kk);
       }
     }
     if (MODE == QM_FUSED) {
@@ -215,11 +284,12 @@ __device__ __forceinline__ double qapply_tile(const Geo &G, const QVec<T, TO> &V
       const int k = kb + j;
       if (k < kend) {
         const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
-        const T c0 = c[j], cp1 = c[j + 1];
+        const T c0 = ZR ? zc.at(k) : c[j], cp1 = ZR ? zc.at(k + 1) : c[j + 1];
         T y = 0;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+          const T dzk = ZR ? (k < zf[f].f0 ? zf[f].top : (k < zf[f].f1 ? dzg[j] : zf[f].bot)) : dz[j][f];
+          const T term = (nv[j][f] - v0) * dzk * Df[f];
           y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
         }
         if (kt == n - 1) y -= T(2.0) * c0 * v0;
@@ -554,7 +624,7 @@ __device__ __forceinline__ double qapply_block(const Geo &G, const QVec<T, T> &V
 // order), so no barrier is needed.  Columns that are not computational but
 // have layers still publish their p.  Values are formed by the same
 // operations in the same order as in qapply_tile: bit-identical results.
-template <int NF, int U, typename T, bool NTS>
+template <int NF, int U, typename T, bool NTS, bool ZR = false>
 __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V, T mu, T nu, int t, int l,
                                               T (*row)[SUNHIP_TILE + SUNHIP_XPAIRS]) {
   if (t >= G.Nt) return 0;
@@ -566,16 +636,21 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
   const int n = G.nk[s];
   const bool own = n > 0 && G.comp[s] == 1;
   const int kt = n > 0 ? G.ctop[s] : 0;
-  const int kb0 = wave_min_i(own ? kt : 0x7fffffff), kb1 = wave_max_i(own ? n : 0);
+  const int kb0 = __builtin_amdgcn_readfirstlane(wave_min_i(own ? kt : 0x7fffffff));
+  const int kb1 = __builtin_amdgcn_readfirstlane(wave_max_i(own ? n : 0));
   if (kb1 <= kb0) return 0;  // no computational column in the tile (uniform)
   const uint32_t base = (uint32_t)(G.toff[t] + l);
   const int nf = n > 0 ? G.nf[s] : 0;
   int code[NF], kmn[NF], kmx[NF], lim[NF];
   uint32_t ebb[NF];
   T Df[NF];
+  ZFace<T> zf[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     code[f] = f < nf ? G.xcode[f * NS + s] : -1;
+    // the window kernel leaves faces without a neighbour (and face slots past
+    // nf, and columns that are not computational) at an empty window
+    if (ZR) zf[f] = zface<T, NF>(V.Z, G, s, f);
     if (f < nf) {
       const int kr = G.kr[f * NS + s];
       kmn[f] = kr & 0xffff;
@@ -587,6 +662,13 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
     }
     if (code[f] < 0) { kmn[f] = 0; kmx[f] = 0; }
     lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+  }
+  // every slot has its (possibly empty) window; plain locals, so they stay in registers
+  int zc0 = 0, zc1 = 0;
+  T zci = 0, ze0 = 0, ze1 = 0, ze2 = 0, ze3 = 0;
+  if (ZR) {
+    const ZCol<T> zc = zcol<T>(V.Z, G, s, kt, n);
+    zc0 = zc.c0; zc1 = zc.c1; zci = zc.ci; ze0 = zc.e0; ze1 = zc.e1; ze2 = zc.e2; ze3 = zc.e3;
   }
   // cross-tile pairs this lane gathers: pair l and pair 64 + l
   const int np = G.npair[t];
@@ -611,13 +693,17 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
   double dot = 0;
   T vprev = 0;
   for (int kb = kb0; kb < kb1; kb += U) {
-    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF];
+    T v[U + 1], c[U + 1], pv[U + 1], xv[U], nv[U][NF], dz[U][NF], dzg[U];
+    if (ZR) {
+#pragma unroll
+      for (int j = 0; j < U; j++) dzg[j] = (T)V.Z.dzt[kb + j];  // the group's layer thicknesses (uniform)
+    }
 #pragma unroll
     for (int j = 0; j <= U; j++) {
       const uint32_t idx = base + (uint32_t)min(kb + j, nn) * 64u;
       pv[j] = at32(po, idx);
       v[j] = at32(rt, idx);
-      c[j] = at32(coef, idx);
+      if (!ZR) c[j] = at32(coef, idx);  // ZR: formed where used
     }
 #pragma unroll
     for (int j = 0; j < U; j++) xv[j] = at32((const T *)V.x, base + (uint32_t)min(kb + j, nn) * 64u);
@@ -635,7 +721,8 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
         g1[j] = at32(rt, ni) + mu * at32(po, ni);
       }
 #pragma unroll
-      for (int f = 0; f < NF; f++) dz[j][f] = f < nf ? at32(dzf, ebb[f] + (uint32_t)min(k, lim[f]) * 64u) : T(0);
+      for (int f = 0; f < NF; f++)
+        if (!ZR) dz[j][f] = f < nf ? at32(dzf, ebb[f] + (uint32_t)min(k, lim[f]) * 64u) : T(0);
     }
 #pragma unroll
     for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
@@ -657,11 +744,13 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
         const int k = kb + j;
         if (k >= kt && k < n) {
           const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
-          const T c0 = c[j], cp1 = c[j + 1];
+          const T c0 = ZR ? zsel<T>(k, kt, n, zc0, zc1, zci, ze0, ze1, ze2, ze3) : c[j];
+          const T cp1 = ZR ? zsel<T>(k + 1, kt, n, zc0, zc1, zci, ze0, ze1, ze2, ze3) : c[j + 1];
           T y = 0;
 #pragma unroll
           for (int f = 0; f < NF; f++) {
-            const T term = (nv[j][f] - v0) * dz[j][f] * Df[f];
+            const T dzk = ZR ? (k < zf[f].f0 ? zf[f].top : (k < zf[f].f1 ? dzg[j] : zf[f].bot)) : dz[j][f];
+            const T term = (nv[j][f] - v0) * dzk * Df[f];
             y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
           }
           if (kt == n - 1) y -= T(2.0) * c0 * v0;
@@ -681,6 +770,142 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
   return dot;
 }
 
+// qapply_xchg for a tile whose columns fit their z-level windows, software
+// pipelined: the loads of the next group of U levels are issued before the
+// current group is exchanged, computed and stored, so its stores never stand
+// between a load and its use.  The level loop is unrolled twice over two
+// register buffers (A, B).  Same operations in the same order as
+// qapply_xchg: bit-identical results.
+template <int NF, int U, typename T, bool NTS>
+__device__ __forceinline__ double qapply_xchg_pipe(const Geo &G, const QVec<T, T> &V, T mu, T nu, int t, int l,
+                                                   T (*row)[SUNHIP_TILE + SUNHIP_XPAIRS]) {
+  if (t >= G.Nt) return 0;
+  const T *__restrict__ rt = V.rtmp;
+  const T *__restrict__ po = V.pold;
+  const int s = t * 64 + l, NS = G.NS;
+  const int n = G.nk[s];
+  const bool own = n > 0 && G.comp[s] == 1;
+  const int kt = n > 0 ? G.ctop[s] : 0;
+  const int kb0 = __builtin_amdgcn_readfirstlane(wave_min_i(own ? kt : 0x7fffffff));
+  const int kb1 = __builtin_amdgcn_readfirstlane(wave_max_i(own ? n : 0));
+  if (kb1 <= kb0) return 0;  // no computational column in the tile (uniform)
+  const uint32_t base = (uint32_t)(G.toff[t] + l);
+  const int nf = n > 0 ? G.nf[s] : 0;
+  int code[NF], kmn[NF], kmx[NF];
+  T Df[NF];
+  ZFace<T> zf[NF];
+  // the per-face tables have SUNHIP_MAXF rows, so every row of the class
+  // exists: load them all, then select (no branch between the loads)
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int xc = G.xcode[f * NS + s], kr = G.kr[f * NS + s], nb = G.nbb[f * NS + s], es = G.es[f * NS + s];
+    zf[f] = zface<T, NF>(V.Z, G, s, f);
+    const T d = V.D[max(es, 0)];
+    const bool on = f < nf;
+    code[f] = on ? xc : -1;
+    kmn[f] = on && xc >= 0 ? (kr & 0xffff) : 0;
+    kmx[f] = on && xc >= 0 ? (kr >> 16) : 0;
+    Df[f] = on && nb >= 0 ? d : T(0);
+  }
+  int zc0, zc1;
+  T zci, ze0, ze1, ze2, ze3;
+  {
+    const ZCol<T> zc = zcol<T>(V.Z, G, s, kt, n);
+    zc0 = zc.c0; zc1 = zc.c1; zci = zc.ci; ze0 = zc.e0; ze1 = zc.e1; ze2 = zc.e2; ze3 = zc.e3;
+  }
+  // at most 64 cross-tile pairs (the caller sends other tiles to qapply_xchg): pair l
+  const int np = G.npair[t];
+  const bool pact = l < np;
+  uint32_t pnb = base;
+  int plim = 0;
+  if (pact) {
+    const int fs = G.pair[(size_t)t * SUNHIP_XPAIRS + l];
+    pnb = (uint32_t)G.nbb[fs];
+    const int mx = G.kr[fs] >> 16;
+    plim = mx > 0 ? mx - 1 : 0;
+  }
+  const int nn = n > 0 ? n : 0;
+  double dot = 0;
+  T vprev = 0;
+  // the loads of the group at kb (raw values; combined where used)
+  auto load = [&](int kb, T (&pv)[U + 1], T (&v)[U + 1], T (&xv)[U], T (&r0)[U], T (&p0)[U], T (&dz)[U]) {
+#pragma unroll
+    for (int j = 0; j < U; j++) dz[j] = (T)V.Z.dzt[kb + j];  // the layer thicknesses of the group (uniform)
+#pragma unroll
+    for (int j = 0; j <= U; j++) {
+      const uint32_t idx = base + (uint32_t)min(kb + j, nn) * 64u;
+      pv[j] = at32(po, idx);
+      v[j] = at32(rt, idx);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = kb + j;
+      xv[j] = at32((const T *)V.x, base + (uint32_t)min(k, nn) * 64u);
+      r0[j] = 0; p0[j] = 0;
+      if (pact) {
+        const uint32_t ni = pnb + (uint32_t)min(k, plim) * 64u;
+        r0[j] = at32(rt, ni); p0[j] = at32(po, ni);
+      }
+    }
+  };
+  // exchange, compute and store the group at kb
+  auto process = [&](int kb, T (&pv)[U + 1], T (&v)[U + 1], T (&xv)[U], T (&r0)[U], T (&p0)[U], T (&dz)[U]) {
+    T nv[U][NF];
+#pragma unroll
+    for (int j = 0; j <= U; j++) v[j] = v[j] + mu * pv[j];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      row[j][l] = v[j];
+      row[j][64 + l] = pact ? r0[j] + mu * p0[j] : T(0);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < U; j++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) nv[j][f] = code[f] >= 0 ? row[j][code[f]] : T(0);
+    __builtin_amdgcn_wave_barrier();
+    if (own) {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = kb + j;
+        if (k >= kt && k < n) {
+          const T v0 = v[j], vm1 = j ? v[j - 1] : vprev, vp1 = v[j + 1];
+          const T c0 = zsel<T>(k, kt, n, zc0, zc1, zci, ze0, ze1, ze2, ze3);
+          const T cp1 = zsel<T>(k + 1, kt, n, zc0, zc1, zci, ze0, ze1, ze2, ze3);
+          T y = 0;
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            const T dzk = k < zf[f].f0 ? zf[f].top : (k < zf[f].f1 ? dz[j] : zf[f].bot);
+            const T term = (nv[j][f] - v0) * dzk * Df[f];
+            y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+          }
+          if (kt == n - 1) y -= T(2.0) * c0 * v0;
+          else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
+          else if (k == n - 1) y += c0 * vm1 - c0 * v0;
+          else y += c0 * vm1 - (c0 + cp1) * v0 + cp1 * vp1;
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          st_stream<NTS>(&at32(V.pnew, idx), v0);
+          st_stream<NTS>(&at32(V.x, idx), xv[j] + nu * pv[j]);
+          dot += (double)v0 * (double)y;
+          st_stream<NTS>(&at32(V.out, idx), y);
+        }
+      }
+    }
+    vprev = v[U - 1];
+  };
+  T pvA[U + 1], vA[U + 1], xvA[U], r0A[U], p0A[U], dzA[U];
+  T pvB[U + 1], vB[U + 1], xvB[U], r0B[U], p0B[U], dzB[U];
+  load(kb0, pvA, vA, xvA, r0A, p0A, dzA);
+  for (int kb = kb0; kb < kb1; kb += 2 * U) {
+    if (kb + U < kb1) load(kb + U, pvB, vB, xvB, r0B, p0B, dzB);
+    process(kb, pvA, vA, xvA, r0A, p0A, dzA);
+    if (kb + U >= kb1) break;
+    if (kb + 2 * U < kb1) load(kb + 2 * U, pvA, vA, xvA, r0A, p0A, dzA);
+    process(kb + U, pvB, vB, xvB, r0B, p0B, dzB);
+  }
+  return dot;
+}
+
 // Vertical-line preconditioner of tile t: (UPDATE) r -= nu*z, then
 // out = M^-1 r by Thomas down the column (b' recomputed, d' stored in out)
 // and back up it (b' from bq); rz and rr collect the lane's r.out and r.r.
@@ -691,19 +916,24 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
 // between the sweeps (their backward step reads nothing and their d' never
 // goes to memory); the levels below go through out and bq as above.  The
 // arithmetic and its order are the same for every KR.
-template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0>
+// ZR (a tile whose columns fit their z-level windows): coef formed from
+// the window and the exceptions (Z), as in qapply_tile.
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false>
 __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ coef, const T *__restrict__ z,
                                            T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq, T nu,
-                                           int t, int l, double &rz, double &rr) {
+                                           int t, int l, double &rz, double &rr, const ZReg *Zp = nullptr) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
   if (n == 0 || G.comp[s] != 1) return;
   const uint32_t base = (uint32_t)(G.toff[t] + l);
+  const ZCol<T> zc = ZR ? zcol<T>(*Zp, G, s, kt, n) : ZCol<T>{};
+  // coef at level k (k <= n; level n is never used but stays in the tile)
+  auto C = [&](int k) -> T { return ZR ? zc.at(k) : at32(coef, base + (uint32_t)k * 64u); };
   if (kt == n - 1) {
     const uint32_t idx = base + (uint32_t)kt * 64u;
     T rk = at32(r, idx);
     if (UPDATE) { rk = rk - nu * at32(z, idx); at32(r, idx) = rk; }
-    const T u = T(-0.5) * rk / at32(coef, idx);
+    const T u = T(-0.5) * rk / C(kt);
     at32(out, idx) = u;
     rz += (double)rk * (double)u;
     rr += (double)rk * (double)rk;
@@ -713,28 +943,47 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
   T hb[KH], hd[KH], hc[KH], hr[KH];
   T cprev = 0, bprev = 1, dprev = 0;
   if constexpr (KR > 0) {
+    // the head in groups of HG levels: a group's loads (clamped into the
+    // tile, which stores one level below its deepest column) are all issued
+    // before its arithmetic, so the forward sweep waits once per group
+    constexpr int HG = KR % 8 == 0 ? 8 : 4;
 #pragma unroll
-    for (int j = 0; j < KR; j++) {
-      const int k = kt + j;
-      if (k < n) {
-        const uint32_t idx = base + (uint32_t)k * 64u;
-        const T ck = at32(coef, idx), ck1 = at32(coef, idx + 64u);
-        T rk = at32((const T *)r, idx);
-        if (UPDATE) { rk = rk - nu * at32(z, idx); at32(r, idx) = rk; }
-        rr += (double)rk * (double)rk;
-        T a = 0, b, c = 0;
-        if (k == kt) { b = -2 * ck - ck1; c = ck1; }
-        else if (k == n - 1) { a = ck; b = -ck; }
-        else { a = ck; b = -ck - ck1; c = ck1; }
-        T d = rk;
-        if (k > kt) {
-          b -= a * cprev / bprev;
-          d -= a * dprev / bprev;
+    for (int g0 = 0; g0 < KR; g0 += HG) {
+      if (kt + g0 >= n) break;
+      T rg[HG], zg[HG], cg[HG + 1];
+#pragma unroll
+      for (int jj = 0; jj <= HG; jj++) {
+        const int k = min(kt + g0 + jj, n);
+        cg[jj] = C(k);
+        if (jj < HG) {
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          rg[jj] = at32((const T *)r, idx);
+          zg[jj] = UPDATE ? at32(z, idx) : T(0);
         }
-        if constexpr (RZID) rz += (double)d * (double)d / (double)b;
-        hb[j] = b; hd[j] = d; hc[j] = c;
-        if constexpr (!RZID) hr[j] = rk;
-        cprev = c; bprev = b; dprev = d;
+      }
+#pragma unroll
+      for (int jj = 0; jj < HG; jj++) {
+        const int j = g0 + jj, k = kt + j;
+        if (k < n) {
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          const T ck = cg[jj], ck1 = cg[jj + 1];
+          T rk = rg[jj];
+          if (UPDATE) { rk = rk - nu * zg[jj]; at32(r, idx) = rk; }
+          rr += (double)rk * (double)rk;
+          T a = 0, b, c = 0;
+          if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+          else if (k == n - 1) { a = ck; b = -ck; }
+          else { a = ck; b = -ck - ck1; c = ck1; }
+          T d = rk;
+          if (k > kt) {
+            b -= a * cprev / bprev;
+            d -= a * dprev / bprev;
+          }
+          if constexpr (RZID) rz += (double)d * (double)d / (double)b;
+          hb[j] = b; hd[j] = d; hc[j] = c;
+          if constexpr (!RZID) hr[j] = rk;
+          cprev = c; bprev = b; dprev = d;
+        }
       }
     }
   }
@@ -744,7 +993,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
 #pragma unroll
     for (int j = 0; j <= U; j++) {
       const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
-      cv[j] = at32(coef, idx);
+      cv[j] = C(min(kb + j, n));
       if (j < U) {
         rv[j] = at32((const T *)r, idx);
         zv[j] = UPDATE ? at32(z, idx) : T(0);
@@ -783,7 +1032,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       bv[j] = at32(bq, idx);
       dv[j] = at32((const T *)out, idx);
       rv[j] = RZID ? T(0) : at32((const T *)r, idx);
-      cv[j] = at32(coef, idx);
+      cv[j] = C(max(ke - j, klo));
     }
 #pragma unroll
     for (int j = 0; j < U; j++) {

@@ -17,6 +17,7 @@ struct QSolver {
   const T *coef, *dzf, *D;
   T *x, *r, *z, *rtmp, *p0, *p1, *bq;
   const T *fcoef;  // qprecond=1: the scaled face coefficients
+  ZReg Z;          // z-level windows of this solve (cwin null: none)
 };
 
 // One free-surface PCG instance.
@@ -26,6 +27,14 @@ struct HSolver {
   const T *coef, *fcoef;
   T *x, *r, *z, *rt, *p0, *p1;
 };
+
+// The layer thicknesses dz[0..Nkmax) the z-level windows use (host array).
+int set_zdz(sunhip_grid *g, const double *dz);
+// The z-level windows of the pressure operator for the coefficients coef
+// (FP64, from "dzz") and the face heights dzf now on the device; an empty
+// ZReg when the layer thicknesses are unknown or the windows are switched
+// off ("zreg" 0).
+int q_windows(sunhip_grid *g, const double *coef, const double *dzf, ZReg *Z);
 
 // Upload fresh CG scalars (tolerance, limits) to d from the pinned host slot
 // stage (g->h_sc when null).  The slot must not be rewritten before the

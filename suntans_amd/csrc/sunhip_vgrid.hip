@@ -173,6 +173,7 @@ static int vg_args(sunhip_grid *g, const sunhip_vgrid_params *vp, VGArgs *out) {
   }
   if (!g->d_dz && dev_alloc(g, (void **)&g->d_dz, sizeof(double) * (size_t)std::max(g->Nkmax, 1))) return -2;
   HIPCHK(hipMemcpyAsync(g->d_dz, vp->dz, sizeof(double) * (size_t)g->Nkmax, hipMemcpyHostToDevice, g->stream));
+  if (int rc = set_zdz(g, vp->dz)) return rc;
   VGArgs A{};
   A.dzz = ptr[0]; A.dzzold = ptr[1]; A.h = ptr[2]; A.dv = ptr[3]; A.dzbot = ptr[4]; A.dzfB = ptr[5];
   A.dzf = ptr[6]; A.u = ptr[7]; A.dz = g->d_dz;

@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         L.sunhip_error.restype = C.c_char_p
         L.sunhip_grid_create.argtypes = [C.POINTER(vp), C.POINTER(Topology), C.c_int]
         L.sunhip_validate.argtypes = [C.POINTER(Topology), C.POINTER(Extents)]
+        L.sunhip_set_dz.argtypes = [vp, _dp]
         L.sunhip_grid_destroy.argtypes = [vp]
         L.sunhip_set_extents.argtypes = [vp, C.POINTER(Extents)]
         L.sunhip_grid_bytes.argtypes = [vp]
@@ -226,6 +227,8 @@ class DeviceGrid:
         self.Nc, self.Ne = st.Nc, st.Ne
         self.st = st
         self.set_extents(st)
+        self._dz = np.ascontiguousarray(st.get("dz"), np.float64)  # grid->dz: the z-level windows
+        check(L.sunhip_set_dz(self.h, native.dptr(self._dz)), "sunhip_set_dz")
 
     def close(self):
         if getattr(self, "h", None):

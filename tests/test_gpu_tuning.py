@@ -1,6 +1,7 @@
 """Every kernel tuning variant (sunhip_set_tuning) solves the same problem.
-Unroll, store-hint, register-budget, register-head and block-staging variants are
-bit-identical to the default configuration (same operations in the same
+Unroll, store-hint, register-budget, register-head, block-staging and z-level
+window (zreg: coefficients computed from the layer thicknesses where they
+equal the stored ones) variants are bit-identical to the default configuration (same operations in the same
 order, same reduction order).  Variants that regroup a dot product only
 change its rounding: tiles per block (block partial sums of 8 or 16 tiles)
 rzid (r.M^-1 r as sum d'^2/b') and band (partial sums per level band); for
@@ -12,9 +13,11 @@ from suntans_amd import cases, device
 
 pytestmark = pytest.mark.gpu
 
-BITWISE = [("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1), ("prec_unroll", 4), ("ntstore", 0),
+BITWISE = [("xchg", 4), ("xchg", 5), ("xchg", 6), ("unroll", 1), ("unroll", 2), ("unroll", 8), ("prec_unroll", 1),
+           ("prec_unroll", 4), ("ntstore", 0),
            ("minwaves", 3), ("minwaves", 4), ("blocklds", 1), ("prec_head", 0), ("prec_head", 8),
-           ("prec_head", 16), ("prec_head", 24), ("prec_head", 32), ("xchg", 1), ("xchg", 2), ("xchg", 3)]
+           ("prec_head", 16), ("prec_head", 24), ("prec_head", 32), ("xchg", 1), ("xchg", 2), ("xchg", 3),
+           ("zreg", 0)]
 ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32)]
 
 
@@ -52,3 +55,34 @@ def test_variant_rounding(gpu_available, grids, key, value):
         q1, n1 = solve(st, [(key, value)])
         assert abs(n0 - n1) <= 1
         assert np.linalg.norm(q1 - q0) <= 1e-12 * np.linalg.norm(q0)
+
+
+def solve_mixed(st, tuning):
+    dg = device.DeviceGrid(st)
+    device.member_upload(dg, st)
+    for k, v in tuning:
+        dg.tune(k, v)
+    s = dg.q_solve_mixed(epsilon=1e-10, inner_tol=1e-4)
+    q = dg.download("q", dg.sizes()["Nflat"])
+    dg.close()
+    return q, s["niters"], s["inner_iters"]
+
+
+@pytest.mark.parametrize("xchg", [0, 2])
+def test_zreg_bitwise_mixed(gpu_available, grids, xchg):
+    """The z-level windows in the FP32 inner kernels and the FP64 residual."""
+    for st in grids:
+        a = solve_mixed(st, [("xchg", xchg), ("zreg", 0)])
+        b = solve_mixed(st, [("xchg", xchg), ("zreg", 1)])
+        assert a[1:] == b[1:]
+        assert np.array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("head", [0, 16, 24])
+def test_zreg_bitwise_heads(gpu_available, grids, head):
+    """The windows with every preconditioner register head, exchange kernel on."""
+    for st in grids:
+        q0, n0 = solve(st, [("xchg", 2), ("prec_head", head), ("zreg", 0)])
+        q1, n1 = solve(st, [("xchg", 2), ("prec_head", head), ("zreg", 1)])
+        assert n0 == n1
+        assert np.array_equal(q0, q1)
