@@ -16,6 +16,13 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// The tile group (4 tiles) a block of a column kernel works on: xcd_block,
+// then the grid's depth-first order within each XCD's run when it has one.
+__device__ __forceinline__ int tile_group(const Geo &G, int b, int nb) {
+  const int v = xcd_block(b, nb);
+  return G.b4perm ? G.b4perm[v] : v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -84,6 +84,7 @@ sunhip::Geo sunhip_grid::geo() const {
   G.e_nke = d_e_nke; G.e_nkc = d_e_nkc; G.e_etop = d_e_etop; G.e_mark = d_e_mark; G.e_comp = d_e_comp;
   G.df = d_df; G.dg = d_dg;
   G.xcode = d_xcode; G.pair = d_pair; G.npair = d_npair;
+  G.b4perm = tune_order ? d_b4perm : nullptr;
   return G;
 }
 
@@ -518,6 +519,24 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= upload_vec(g, &g->d_ctop, zeros);
   rc |= upload_vec(g, &g->d_ctopold, zeros);
   rc |= upload_vec(g, &g->d_kr, kr0);
+  {
+    // depth-first dispatch order of the 4-tile blocks: the groups of each
+    // XCD's contiguous run (xcd_block's split of nb4 blocks over 8 XCDs)
+    // sorted by their deepest tile, deepest first, Morton order among equals
+    const int nb4 = (g->Nt + 3) / 4, q = nb4 >> 3, r8 = nb4 & 7;
+    std::vector<int> perm(nb4), gdep(nb4, 0);
+    for (int b = 0; b < nb4; b++) {
+      perm[b] = b;
+      for (int t = 4 * b; t < std::min(4 * b + 4, g->Nt); t++) gdep[b] = std::max(gdep[b], g->tdep[t]);
+    }
+    for (int x = 0, start = 0; x < 8; x++) {
+      const int len = q + (x < r8 ? 1 : 0);
+      std::stable_sort(perm.begin() + start, perm.begin() + start + len,
+                       [&](int a, int b) { return gdep[a] > gdep[b]; });
+      start += len;
+    }
+    rc |= upload_vec(g, &g->d_b4perm, perm);
+  }
   rc |= upload_vec(g, &g->d_e_nke, ezeros);
   rc |= upload_vec(g, &g->d_e_etop, ezeros);
   rc |= upload_vec(g, &g->d_e_etopold, ezeros);
@@ -557,7 +576,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
                   g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
                   g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold,
-                  g->d_ford, g->d_zdzt, g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag};
+                  g->d_ford, g->d_zdzt, g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag, g->d_b4perm};
   for (void *p : ptrs) if (p) hipFree(p);
   if (g->h_stage) hipHostFree(g->h_stage);
   for (int i = 0; i < 2; i++) {

@@ -97,6 +97,11 @@ struct Geo {
   // a neighbour in the same tile; 64 + i: cross-tile pair i; -1 none), and
   // per tile the pairs' (face, slot) indices and count
   const int *xcode, *pair, *npair;
+  // dispatch order of the 4-tile blocks of the column kernels (b4perm[v] =
+  // the block group of virtual block v): within each XCD's run of groups the
+  // deepest go first, so the launch does not end on a few deep columns;
+  // null: identity
+  const int *b4perm;
 };
 
 }  // namespace sunhip
@@ -123,6 +128,8 @@ struct sunhip_grid {
   int *d_nk = nullptr, *d_ctop = nullptr, *d_ctopold = nullptr, *d_comp = nullptr, *d_nf = nullptr;
   int *d_nbb = nullptr, *d_nbs = nullptr, *d_eb = nullptr, *d_es = nullptr, *d_nrm = nullptr, *d_kr = nullptr;
   int *d_xcode = nullptr, *d_pair = nullptr, *d_npair = nullptr;  // neighbour exchange (Geo)
+  int *d_b4perm = nullptr;          // depth-first dispatch order of 4-tile blocks (Geo)
+  int tune_order = 1;               // 1: use it
   int *d_uerr = nullptr;             // UPredictor's exit conditions
   double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)
   // z-level windows of the pressure solve (ZReg): dz[k] (Nkmax + 16

@@ -120,7 +120,8 @@ template <int MODE, int NF, int U, typename T, typename TO, int TPB = 4, bool NT
           bool ZR = false>
 __global__ __launch_bounds__(TPB * 64, MINW > 0 ? MINW : 1) void k_qapply(Geo G, QVec<T, TO> V, CGScal *sc, Red R) {
   if (MODE == QM_FUSED && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * TPB + (threadIdx.x >> 6);
+  const int t = (TPB == 4 ? tile_group(G, blockIdx.x, gridDim.x) : xcd_block(blockIdx.x, gridDim.x)) * TPB +
+                (threadIdx.x >> 6);
   const T mu = MODE == QM_FUSED ? (T)sc->mu : T(0);
   const T nu = MODE == QM_FUSED ? (T)sc->nu : T(0);
   const double dot = (ZR && t < G.Nt && V.Z.tflag[t])
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_qapply_band(Geo G, QVec<T, T> V, CGScal
 template <int NF, int U, typename T, bool NTS>
 __global__ __launch_bounds__(256) void k_qapply_blk(Geo G, QVec<T, T> V, CGScal *sc, Red R) {
   if (sc->done) return;
-  const int t0 = xcd_block(blockIdx.x, gridDim.x) * 4;
+  const int t0 = tile_group(G, blockIdx.x, gridDim.x) * 4;
   const double dot = qapply_block<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t0);
   double dv[1] = {dot};
   reduce_out<1, 256>(dv, R, sc);
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_x(Geo G, QV
   if (sc->done) return;
   __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
   const int w = threadIdx.x >> 6;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+  const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + w;
   const double dot =
       (ZR && t < G.Nt && V.Z.tflag[t])
           ? qapply_xchg<NF, U, T, NTS, true>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w])
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_xp(Geo G, Q
   if (sc->done) return;
   __shared__ T rows[4][U][SUNHIP_TILE + SUNHIP_XPAIRS];
   const int w = threadIdx.x >> 6;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + w;
+  const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + w;
   const double dot =
       (t < G.Nt && V.Z.tflag[t] && G.npair[t] <= 64)
           ? qapply_xchg_pipe<NF, U, T, NTS>(G, V, (T)sc->mu, (T)sc->nu, t, threadIdx.x & 63, rows[w])
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
                                                CGScal *sc, Red R, ZReg Z) {
   if (UPDATE && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
   if (ZR && t < G.Nt && Z.tflag[t])
     qprec_tile<UPDATE, U, T, RZID, KR, true>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
@@ -214,7 +215,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_qnoprec(Geo G, const T *__restrict__ z, T *__restrict__ r,
                                                  T *__restrict__ out, CGScal *sc, Red R, int update) {
   if (update && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rr = 0;
   qnoprec_tile<T>(G, z, r, out, update ? (T)sc->nu : T(0), update != 0, t, threadIdx.x & 63, rr);
   double v[2] = {rr, rr};
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(256) void k_qcoef_pc1(Geo G, const double *__restri
 template <int MODE, int NF>
 __global__ __launch_bounds__(256) void k_qcapply(Geo G, QVec<double, double> V, CGScal *sc, Red R, int64_t NT) {
   if (MODE == QM_FUSED && sc->done) return;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   const double mu = MODE == QM_FUSED ? sc->mu : 0.0, nu = MODE == QM_FUSED ? sc->nu : 0.0;
   const double dot = qcapply_tile<MODE, NF>(G, V, mu, nu, NT, t, threadIdx.x & 63);
   if (MODE == QM_FUSED) {
@@ -1082,6 +1083,7 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     g->tune_xchg = value;
   }
   else if (!strcmp(key, "zreg")) g->tune_zreg = value != 0;
+  else if (!strcmp(key, "order")) g->tune_order = value != 0;
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {
     if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }

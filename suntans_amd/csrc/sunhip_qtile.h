@@ -707,18 +707,20 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
     }
 #pragma unroll
     for (int j = 0; j < U; j++) xv[j] = at32((const T *)V.x, base + (uint32_t)min(kb + j, nn) * 64u);
-    T g0[U], g1[U];
+    // cross-tile values loaded raw and combined after the group's loads,
+    // so no load of the group waits for another
+    T gr0[U], gp0[U], gr1[U], gp1[U];
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int k = kb + j;
-      g0[j] = 0; g1[j] = 0;
+      gr0[j] = 0; gp0[j] = 0; gr1[j] = 0; gp1[j] = 0;
       if (pact[0]) {
         const uint32_t ni = pnb[0] + (uint32_t)min(k, plim[0]) * 64u;
-        g0[j] = at32(rt, ni) + mu * at32(po, ni);
+        gr0[j] = at32(rt, ni); gp0[j] = at32(po, ni);
       }
       if (two && pact[1]) {
         const uint32_t ni = pnb[1] + (uint32_t)min(k, plim[1]) * 64u;
-        g1[j] = at32(rt, ni) + mu * at32(po, ni);
+        gr1[j] = at32(rt, ni); gp1[j] = at32(po, ni);
       }
 #pragma unroll
       for (int f = 0; f < NF; f++)
@@ -729,8 +731,8 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
 #pragma unroll
     for (int j = 0; j < U; j++) {
       row[j][l] = v[j];
-      row[j][64 + l] = g0[j];
-      if (two) row[j][128 + l] = g1[j];
+      row[j][64 + l] = pact[0] ? gr0[j] + mu * gp0[j] : T(0);
+      if (two) row[j][128 + l] = pact[1] ? gr1[j] + mu * gp1[j] : T(0);
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll

@@ -18,7 +18,7 @@ BITWISE = [("xchg", 4), ("xchg", 5), ("xchg", 6), ("unroll", 1), ("unroll", 2), 
            ("minwaves", 3), ("minwaves", 4), ("blocklds", 1), ("prec_head", 0), ("prec_head", 8),
            ("prec_head", 16), ("prec_head", 24), ("prec_head", 32), ("xchg", 1), ("xchg", 2), ("xchg", 3),
            ("zreg", 0)]
-ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32)]
+ROUNDING = [("tiles", 8), ("tiles", 16), ("rzid", 0), ("band", 8), ("band", 16), ("band", 32), ("order", 0)]
 
 
 def solve(st, tuning):
