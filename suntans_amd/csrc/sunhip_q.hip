@@ -492,7 +492,9 @@ int launch_apply(sunhip_grid *g, CGScal *sc, const QVec<T, TO> &V, const Red &R,
     if (xchg) {
       const int nbx = (g->Nt + 3) / 4;
       with_nf(g->mf, [&](auto nfc) {
-        constexpr int NF = decltype(nfc)::value, UX = NF <= 4 ? 4 : 2;
+        // FP32 values take half the registers: 8 levels per group on triangles and quads
+        constexpr int NF = decltype(nfc)::value,
+                      UX = NF <= 4 ? (std::is_same<T, float>::value ? 8 : 4) : 2;
 #define XCH(U, NTS, W)                                                                          \
   do {                                                                                          \
     if (zr) k_qapply_x<NF, U, T, NTS, W, true><<<nbx, 256, 0, g->stream>>>(G, V, sc, R);       \
