@@ -32,7 +32,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_large_solve.txt")
 
-KERNEL_NAMES = {"qapply": "k_qapply_x<3,4,double,true,4>", "qprec": "k_qprec<true,4,double,true,24>"}
+# the FP64 kernels of the workload's iteration (triangles, z-level windows)
+KERNEL_NAMES = {"qapply": "k_qapply_x<3, 4, double, true, 4, true>", "qprec": "k_qprec<true, 4, double, true, 24, true>"}
 
 
 def log(msg: str) -> None:
@@ -65,62 +66,162 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def cpu_baseline(st, gpu_iters: int, budget_s: float):
-    """The reference (oracle/_ref, compiled from its own sources) or, where it
-    is absent, the oracle restatement, timed on one host core on a bounded
-    sample: the same solve cut to 1 and to 3 PCG iterations; the per-iteration
-    time (difference / 2) and the set-up time give the time of a full solve
-    with the GPU's iteration count."""
+def _checkers():
+    """The CPU checker: the reference compiled from its own sources
+    (oracle/_ref) where present, else the oracle's restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _impls
     impl = _impls.reference()
-    kind = "reference"
-    if impl is None:
-        impl = _impls.oracle()
-        kind = "port"
+    if impl is not None:
+        return impl, "reference"
+    return _impls.oracle(), "port"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may run on, capped at 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _cpu_q_solve(impl, st, src0, maxiters, eps):
+    """One CGSolveQ (qprecond 2) from q = 0 by the CPU checker; returns
+    (seconds, iterations, q)."""
+    st.set("qc", 0.0)
+    st.set("stmp", src0)
+    st.set_prop(qprecond=2, qepsilon=eps, qmaxiters=maxiters, resnorm=0)
+    t = time.perf_counter()
+    impl.cgsolveq(st)
+    sec = time.perf_counter() - t
+    return sec, impl.stats("q")[0], st.get("qc")
+
+
+def cpu_baseline(st, x64, x_fixed, gpu_iters: int):
+    """The reference's CGSolveQ on the full workload on one host core (from
+    q = 0 to 1e-10), and the device's result against it: the converged q
+    (x64) and the q after exactly 1 and 3 PCG iterations (x_fixed)."""
+    impl, kind = _checkers()
     src0 = st.get("stmp")
-    times = {}
+    m = st.wet_mask()
+    fixed = {}
     for n in (1, 3):
-        st.set("qc", 0.0)
-        st.set("stmp", src0)
-        st.set_prop(qprecond=2, qepsilon=0.0, qmaxiters=n)
-        t = time.perf_counter()
-        if kind == "reference":
-            # the harness prints its convergence report at VERBOSE>2 only
-            impl.fn("CGSolveQ")(st.ptr("qc"), st.ptr("stmp"), st.ptr("stmp3"), st.g, st.p, st.prop)
-        else:
-            impl.cgsolveq(st)
-        times[n] = time.perf_counter() - t
-        log(f"cpu {kind}: {n} iteration(s) in {times[n]:.2f}s")
-        if times[n] > budget_s:
-            break
+        sec, _, qr = _cpu_q_solve(impl, st, src0, n, 0.0)
+        fixed[n] = {"cpu_s": sec, "rel_err_q": relerr(x_fixed[n][m], qr[m])}
+        log(f"cpu {kind}: {n} iteration(s) in {sec:.2f}s, device rel err {fixed[n]['rel_err_q']:.2e}")
+    sec, its, qr = _cpu_q_solve(impl, st, src0, 2000, 1e-10)
+    err = relerr(x64[m], qr[m])
+    log(f"cpu {kind}: full solve {its} iterations in {sec:.1f}s; device q rel err {err:.2e}")
     st.set("stmp", src0)
     st.set("qc", 0.0)
-    if 3 in times:
-        t_it = (times[3] - times[1]) / 2
-        t_setup = max(times[1] - t_it, 0.0)
-    else:
-        t_it = times[1] / 2
-        t_setup = t_it
-    t_solve = t_setup + gpu_iters * t_it
     return {
-        "value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": kind,
-        "sample": (f"full grid, PCG cut to 1 and 3 iterations ({times.get(1, 0):.1f}s, "
-                   f"{times.get(3, float('nan')):.1f}s); solve = setup {t_setup:.2f}s + "
-                   f"{gpu_iters} iterations x {t_it:.2f}s"),
-        "s_per_iteration": t_it,
+        "value": 1.0 / sec, "unit": "solves/s", "cores": 1, "kind": kind, "cpu_model": cpu_model(),
+        "sample": f"one full CGSolveQ of the workload from q=0 to 1e-10 on one core ({its} iterations, {sec:.1f} s)",
+        "s_per_solve": sec, "iterations": its, "device_iterations": gpu_iters,
+        "rel_err_q_vs_reference": err,
+        "fixed_iterations_rel_err_q": {str(n): v["rel_err_q"] for n, v in fixed.items()},
     }
 
 
-def traffic_for(kernel_key: str):
+def _core_worker(seed, nx, ny, target_N, barrier, out):
+    """One process of the all-core leg: build a reduced large-solve case,
+    wait for the others, solve it to 1e-10 by the CPU checker."""
+    from suntans_amd import cases
+    impl, _ = _checkers()
+    st = cases.large_solve(seed=seed, nx=nx, ny=ny, target_N=target_N)
+    st.set_prop(qprecond=2, qepsilon=1e-10, qmaxiters=2000, resnorm=0)
+    barrier.wait()
+    t0 = time.time()
+    impl.cgsolveq(st)
+    out.put((st.N, impl.stats("q")[0], t0, time.time()))
+
+
+def cpu_all_cores(full_N: int, full_iters: int, one_core_s: float, nx=250, ny=125):
+    """All-core throughput of the CPU checker: one process per core, each
+    solving its own reduced case of the workload (nx x ny quads, the same
+    bathymetry statistics) to 1e-10 at the same time.  Unknown-iterations per
+    second over the whole set, stated also as full-workload solves/s."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    P = host_cores()
+    tN = 5.0e7 * (nx * ny) / (1000 * 500)
+    barrier, out = ctx.Barrier(P), ctx.Queue()
+    procs = [ctx.Process(target=_core_worker, args=(1000 + i, nx, ny, tN, barrier, out)) for i in range(P)]
+    for p in procs:
+        p.start()
+    res = [out.get(timeout=900) for _ in procs]
+    for p in procs:
+        p.join()
+    start, end = min(r[2] for r in res), max(r[3] for r in res)
+    work = sum(r[0] * r[1] for r in res)
+    rate = work / (end - start)
+    one = full_N * full_iters / one_core_s
+    log(f"cpu all cores: {P} processes, {work:.3e} unknown-iterations in {end - start:.1f}s")
+    return {
+        "cores": P, "unknown_iterations_per_s": rate, "one_core_unknown_iterations_per_s": one,
+        "full_workload_solves_per_s": rate / (full_N * full_iters),
+        "sample": (f"{P} processes, each one CGSolveQ of its own {nx}x{ny}-quad case (N {min(r[0] for r in res)}"
+                   f"-{max(r[0] for r in res)}, seeded bathymetry) to 1e-10, run together; wall {end - start:.1f} s"),
+    }
+
+
+def lockexchange_leg():
+    """The lock-exchange trajectory (2000 cells x 50 levels, 20 steps, dt 5 s)
+    twice: once with every function from the CPU checker, once with CGSolve,
+    UpdateScalars and CGSolveQ from the drop-ins (libsuntans.so; host pack +
+    PCIe + device per call) and the rest of the step from the checker.  Per
+    call: the checker's time on one core and the drop-in's; the iteration
+    counts and final fields of the two trajectories compared."""
+    import _impls
+    import _trajectory
+    impl, kind = _checkers()
+    d = _impls.dropin()
+    a = _trajectory.initial()
+    ra = _trajectory.run(a, impl, impl)
+    b = _trajectory.initial()
+    rb = _trajectory.run(b, d, impl)
+    out = {"kind": kind, "steps": len(ra), "cells": a.Nc, "unknowns": a.N}
+    for key in ("cgsolve_s", "update_scalars_s", "cgsolveq_s"):
+        name = key[:-2]
+        out[name] = {"cpu_ms_p50": float(np.median([r[key] for r in ra])) * 1e3,
+                     "dropin_ms_p50": float(np.median([r[key] for r in rb[1:]])) * 1e3,
+                     "dropin_first_call_ms": rb[0][key] * 1e3}
+        out[name]["speedup"] = out[name]["cpu_ms_p50"] / out[name]["dropin_ms_p50"]
+    out["h_iterations_max_diff"] = int(max(abs(x["h_iters"] - y["h_iters"]) for x, y in zip(ra, rb)))
+    out["q_iterations_max_diff"] = int(max(abs(x["q_iters"] - y["q_iters"]) for x, y in zip(ra, rb)))
+    errs = {}
+    for name in _trajectory.FINAL_FIELDS:
+        ma = _trajectory.final_mask(a, name)
+        va, vb = a.get(name), b.get(name)
+        errs[name] = relerr(vb[ma], va[ma]) if ma is not None else relerr(vb, va)
+    out["final_rel_err"] = errs
+    log(f"lock exchange: {out}")
+    return out
+
+
+def traffic_for(kernel: str):
     """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
     summary (copy-calibrated FETCH_SIZE + WRITE_SIZE), None if absent."""
     from suntans_amd import profiling
-    t = profiling.load_traffic(PMC_SUMMARY)
-    name = KERNEL_NAMES[kernel_key]
-    if name not in t:
+    if not os.path.exists(PMC_SUMMARY):
         return None
-    return t[name][0] + t[name][1]
+    t = profiling.load_traffic(PMC_SUMMARY)
+    key = kernel.replace(" ", "")
+    for name, (f, w) in t.items():
+        if name.replace(" ", "") == key:
+            return f + w
+    return None
 
 
 def tvd_leg(st, dg, N, Ek, steps=5):
@@ -329,6 +430,86 @@ def step_leg(st, dg, steps=3, mixed=False, inner_tol=1e-4):
     return res
 
 
+def pcie_bandwidth(dg, nbytes=1 << 30, reps=5):
+    """Host-to-device and device-to-host copy rates (GB/s) between pinned host
+    memory and HBM."""
+    h2d, d2h = dg.pcie_probe(nbytes, reps)
+    return {"h2d": nbytes / h2d / 1e6, "d2h": nbytes / d2h / 1e6}
+
+
+def dropin_large_leg(st, dg, resident_ms):
+    """The reference signatures (libsuntans.so) on the full workload's host
+    state: CGSolveQ, CGSolve and UpdateScalars per call, each = pack the
+    reference's ragged arrays, copy to HBM, solve, copy back and unpack.
+    Beside each: the resident device time of the same call and the PCIe
+    floor (the bytes the call moves / the measured pinned copy rates).
+    Bytes per call (8 B each): CGSolveQ in q, src, dzz (N each), dzf (E_k),
+    D (Ne), out q (N); CGSolve in h, htmp, D (Nc, Nc, Ne), out h (Nc);
+    UpdateScalars in s, dzz, dzzold, kappa_tv (N), wnew, wtmp2 (N + Nc), u,
+    utmp2, dzf (E_k), active (Nc), out s, Cn (N), SfHp, SfHm (E_k)."""
+    import ctypes as C
+    from suntans_amd import cases, native
+    L = native.dropin_lib()
+    bw = pcie_bandwidth(dg)
+    N, Ek, Nc, Ne = st.N, st.scalar("Ek"), st.Nc, st.Ne
+    moved = {"CGSolveQ": (8 * (3 * N + Ek + Ne), 8 * N), "CGSolve": (8 * (2 * Nc + Ne), 8 * Nc),
+             "UpdateScalars": (8 * (6 * N + 2 * Nc + 3 * Ek + Nc), 8 * (2 * N + 2 * Ek))}
+    src0, D0, s0, h0 = st.get("stmp"), st.get("D"), st.get("s"), st.get("h")
+    Dh = cases.depth_integrated_D(st)
+    n_ = C.c_int()
+    r_ = C.c_double()
+    ms_ = C.c_double()
+
+    def q_call():
+        st.set("qc", 0.0)
+        st.set("stmp", src0)
+        st.set("D", D0)
+        t = time.perf_counter()
+        L.CGSolveQ(st.ptr("qc"), st.ptr("stmp"), st.ptr("stmp3"), st.g, st.p, st.prop, 0, 1, 0)
+        return time.perf_counter() - t
+
+    def h_call():
+        st.set("h", h0)
+        st.set("D", Dh)
+        t = time.perf_counter()
+        L.CGSolve(st.g, st.p, st.prop, 0, 1, 0)
+        return time.perf_counter() - t
+
+    def s_call():
+        st.set("s", s0)
+        g = st.get_prop
+        t = time.perf_counter()
+        L.UpdateScalars(st.g, st.p, st.prop, st.ptr("wnew"), st.ptr("s"), st.ptr("boundary_s"), st.ptr("Cn_R"),
+                        g("kappa_s"), g("kappa_sH"), st.ptr("kappa_tv"), g("theta"), None, None, None, None,
+                        0, 0, 0, 0, 1, 3)
+        return time.perf_counter() - t
+
+    st.set_prop(qprecond=2, qepsilon=1e-10, qmaxiters=2000, resnorm=0, hprecond=1, epsilon=1e-10, maxiters=1000,
+                n=3)
+    out = {"pcie_GBps": bw}
+    for name, fn, which in (("CGSolveQ", q_call, 0), ("CGSolve", h_call, 1), ("UpdateScalars", s_call, None)):
+        ts = [fn() for _ in range(4)]
+        dev = None
+        if which is not None:
+            L.sunhip_last_solve(which, C.byref(n_), C.byref(r_), C.byref(ms_))
+            dev = ms_.value
+        up, down = moved[name]
+        floor = (up / bw["h2d"] + down / bw["d2h"]) / 1e6
+        call = float(np.median(ts[1:])) * 1e3
+        res = resident_ms.get(name)
+        out[name] = {"call_ms_p50": call, "first_call_ms": ts[0] * 1e3, "resident_ms": res,
+                     "device_ms_last_call": dev, "bytes_h2d": up, "bytes_d2h": down, "pcie_floor_ms": floor,
+                     "overhead_ms": call - res if res is not None else None,
+                     "overhead_minus_floor_ms": call - res - floor if res is not None else None}
+        log(f"drop-in {name}: {out[name]}")
+    st.set("stmp", src0)
+    st.set("D", D0)
+    st.set("s", s0)
+    st.set("h", h0)
+    st.set("qc", 0.0)
+    return out
+
+
 def ensemble_specs(n: int, seed: int):
     """Member sizes of the ensemble: cells U[5e3, 2e5], aspect U[1, 4],
     levels U{20..100} (seeded)."""
@@ -496,12 +677,14 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="leave out the CPU baseline leg")
     ap.add_argument("--skip-extra", action="store_true",
                     help="only the FP64 pressure solves (for short profiled runs)")
-    ap.add_argument("--cpu-budget", type=float, default=40.0)
+    ap.add_argument("--skip-allcore", action="store_true", help="leave out the all-core CPU leg")
+    ap.add_argument("--skip-steps", action="store_true", help="leave out the whole-time-step legs")
+    ap.add_argument("--skip-dropin", action="store_true", help="leave out the drop-in per-call leg")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--skip-ensemble", action="store_true", help="leave out the ensemble and serving legs")
     ap.add_argument("--skip-serve", action="store_true", help="leave out the serving leg")
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10, 20, 50, 100, 200, 500, 1000, 2000])
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[10, 50, 200, 1000, 2000])
+    ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--build-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for HBM byte counters)")
@@ -556,7 +739,13 @@ def main():
     dom_ms = kt[dom][0] / max(kt[dom][1], 1)
     achieved = B[dom] / (dom_ms * 1e-3) / 1e9
     x64 = dg.download("q", nflat)
-    traffic = traffic_for(dom)
+    traffic = traffic_for(KERNEL_NAMES[dom])
+    # the device's q after exactly 1 and 3 iterations, for the comparison with the reference
+    x_fixed = {}
+    for n in (1, 3):
+        dg.copy("q", "q0")
+        dg.q_solve(maxiters=n, precond=2, epsilon=0.0, fixed=n)
+        x_fixed[n] = dg.download("q", nflat)
 
     out = {
         "metric": "Poisson-CG solves/s at 1e-10 rel resid (FP64 CGSolveQ, vertical-line PCG)",
@@ -576,6 +765,7 @@ def main():
                          f"{st.Nc} triangles, 100 levels of 2 m, depth 10-200 m, N={N} unknowns, "
                          "qprecond=2, qepsilon=1e-10, q0=0"),
             "N": N, "E_k": Ek, "Nc": st.Nc,
+            "tile_padding_ratio": dg.sizes()["NT"] / N,
             "parallelism": "single GPU",
         },
         "iterations": iters,
@@ -660,8 +850,13 @@ def main():
         out["tvd"] = tvd_leg(st, dg, N, Ek)
         out["upredictor"] = upred_leg(st, dg, N, Ek)
         out["wpredictor"] = wpred_leg(st, dg, N, Ek)
-        out["time_step"] = step_leg(st, dg)
-        out["time_step_mixed"] = step_leg(st, dg, mixed=True, inner_tol=args.inner_tol)
+        if not args.skip_steps:
+            out["time_step"] = step_leg(st, dg)
+            out["time_step_mixed"] = step_leg(st, dg, mixed=True, inner_tol=args.inner_tol)
+        if not args.skip_dropin:
+            out["dropin"] = dropin_large_leg(st, dg, {"CGSolveQ": ms_per_step,
+                                                      "CGSolve": out["free_surface"]["fp64_ms_p50"],
+                                                      "UpdateScalars": out["tvd"]["step_ms"]})
 
     if args.copy_probe:
         cp_ms = dg.copy_probe(1 << 30, 20)
@@ -683,7 +878,15 @@ def main():
             x.dg.close()
 
     if not args.skip_cpu:
-        out["cpu_baseline"] = cpu_baseline(st, int(round(its)), args.cpu_budget)
+        cb = cpu_baseline(st, x64, x_fixed, iters[-1])
+        out["rel_err_vs_reference"] = {"converged_q": cb["rel_err_q_vs_reference"],
+                                       **{f"q_after_{k}_iterations": v
+                                          for k, v in cb["fixed_iterations_rel_err_q"].items()},
+                                       "iterations_reference": cb["iterations"], "iterations_device": iters[-1]}
+        if not args.skip_allcore:
+            cb["all_cores"] = cpu_all_cores(N, cb["iterations"], cb["s_per_solve"])
+        cb["lockexchange"] = lockexchange_leg()
+        out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -197,6 +197,28 @@ int sunhip_kernel_times(sunhip_grid *g, double *ms, int *n);
 /* A plain device copy of nbytes, reps times (HBM byte-counter calibration);
  * *avg_ms is the mean duration. */
 int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
+/* No reference counterpart: transfers in parts, so that a caller packing or
+ * unpacking host rows overlaps that work with the copies.
+ *
+ * sunhip_upload_part copies values [off, off + n) of the field's flat array
+ * from flat (the whole array's base) into the device staging buffer; parts
+ * come in order, each starting where the last ended, and the part with
+ * last = 1 must end at the field's length: it then scatters the array into
+ * the field.  flat must not change until the last part has been called and,
+ * for a pinned slot (sunhip_pinned), until the slot is handed out again.
+ *
+ * sunhip_download_parts gathers the field and starts its copy into flat in
+ * nparts (1..SUNHIP_MAXPARTS) equal parts without waiting;
+ * sunhip_download_wait returns once values [0, upto) are on the host (upto
+ * at or beyond the length: all of them).  Every download in parts must be
+ * waited for in full before the next transfer of the grid. */
+#define SUNHIP_MAXPARTS 16
+int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int64_t off, int64_t n, int last);
+int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int nparts);
+int sunhip_download_wait(sunhip_grid *g, int64_t upto);
+/* No reference counterpart: host-to-device and device-to-host copy times (ms
+ * per copy) of nbytes between pinned host memory and HBM, averaged over reps. */
+int sunhip_pcie_probe(sunhip_grid *g, int64_t nbytes, int reps, double *h2d_ms, double *d2h_ms);
 
 /* ---- free surface (main/phys.c:3086-3217) ------------------------------ */
 /* Fields: "h" (in: boundary values; out: solution), "htmp" (RHS, clobbered),

@@ -588,16 +588,12 @@ int oracle_UPredictor_begin(gridT *g, physT *p, propT *prop) {
   return bad;
 }
 
-/* main/phys.c:2951-3018, the part of UPredictor after the open-boundary
- * hooks: the free-surface source htmp = Ac h - dt sum of the theta-weighted
- * face fluxes, CGSolve (cgsolver = 1), cells that dry below DRYCELLHEIGHT
- * clamped (h, active, bottom scalar), and u = U*** - g theta dt E grad h on
- * computational edges (zero on a dry one-layer edge).  Returns 1 for
- * cgsolver != 1 (GSSolve is not restated). */
-int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
+/* main/phys.c:2956-2971, UPredictor's free-surface source: htmp = Ac h -
+ * dt times the sum of the theta-weighted face fluxes of each computational
+ * cell. */
+void oracle_UPredictor_rhs(gridT *g, physT *p, propT *prop) {
   const double dt = prop->dt, theta = prop->theta;
   const int mf = g->maxfaces;
-  if (prop->cgsolver != 1) return 1;
   for (int ip = g->celldist[0]; ip < g->celldist[1]; ip++) {
     const int i = g->cellp[ip];
     double sum = 0;
@@ -608,7 +604,13 @@ int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
     }
     p->htmp[i] = g->Ac[i] * p->h[i] - dt * sum;
   }
-  oracle_CGSolve(g, p, prop);
+}
+
+/* main/phys.c:2991-3018, UPredictor after its CGSolve: cells that dry below
+ * DRYCELLHEIGHT clamped (h, active, bottom scalar), and u = U*** - g theta
+ * dt E grad h on computational edges (zero on a dry one-layer edge). */
+void oracle_UPredictor_finish(gridT *g, physT *p, propT *prop) {
+  const double dt = prop->dt, theta = prop->theta;
   for (int i = 0; i < g->Nc; i++)
     if (p->h[i] <= -g->dv[i] + 1e-10) {
       p->h[i] = -g->dv[i] + 1e-10;
@@ -626,6 +628,17 @@ int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
     if (g->etop[j] == g->Nke[j] - 1 && g->dzz[i1][g->etop[j]] == 0 && g->dzz[i2][g->etop[j]] == 0)
       p->u[j][g->etop[j]] = 0;
   }
+}
+
+/* main/phys.c:2951-3018, the part of UPredictor after the open-boundary
+ * hooks: the free-surface source, CGSolve (cgsolver = 1), then the drying
+ * clamp and the velocity update.  Returns 1 for cgsolver != 1 (GSSolve is
+ * not restated). */
+int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop) {
+  if (prop->cgsolver != 1) return 1;
+  oracle_UPredictor_rhs(g, p, prop);
+  oracle_CGSolve(g, p, prop);
+  oracle_UPredictor_finish(g, p, prop);
   return 0;
 }
 

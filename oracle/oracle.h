@@ -47,6 +47,8 @@ int oracle_UPredictor_begin(gridT *g, physT *p, propT *prop);
 void oracle_UpdateDZ(gridT *g, physT *p, propT *prop, int option);
 void oracle_SetFluxHeight(gridT *g, physT *p, propT *prop);
 int oracle_UPredictor_solve(gridT *g, physT *p, propT *prop);
+void oracle_UPredictor_rhs(gridT *g, physT *p, propT *prop);
+void oracle_UPredictor_finish(gridT *g, physT *p, propT *prop);
 double oracle_face_def1(gridT *g, int j);
 void oracle_MomentumFaceValues(REAL **uf, REAL **ui, REAL **bui, REAL **U, gridT *g, physT *p, propT *prop);
 void oracle_WPredictor(gridT *g, physT *p, propT *prop);

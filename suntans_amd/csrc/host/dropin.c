@@ -43,6 +43,9 @@ typedef struct {
   size_t flat_n;
   int slot;          /* the slot the next transfer takes */
   size_t *cst, *wst, *est;  /* flat offsets of cell (Nk), w-type (Nk+1) and edge (Nkc) rows */
+  int *cell_rank, *edge_rank;  /* position of cell i in cellp, of edge j in edgep */
+  int parts;         /* parts of the download in progress */
+  int *x_last;       /* the extents last sent (ctop, ctopold | Nke, etop), NULL: none */
 } ctx_t;
 
 static ctx_t g_ctx[MAXCTX];
@@ -107,6 +110,18 @@ static void ctx_free(ctx_t *c) {
   free(c->cst);
   free(c->wst);
   free(c->est);
+  free(c->cell_rank);
+  free(c->edge_rank);
+  free(c->x_last);
+}
+
+static int *ranks(const int *order, int n) {
+  int *r = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+  if (!r) return NULL;
+  for (int i = 0; i < n; i++) r[i] = -1;
+  for (int ip = 0; ip < n; ip++)
+    if (order[ip] >= 0 && order[ip] < n) r[order[ip]] = ip;
+  return r;
 }
 
 static size_t *offsets(const int *len, int n, int extra) {
@@ -158,7 +173,9 @@ static ctx_t *get_ctx(gridT *g, const char *where) {
   c->cst = offsets(g->Nk, g->Nc, 0);
   c->wst = offsets(g->Nk, g->Nc, 1);
   c->est = offsets(g->Nkc, g->Ne, 0);
-  if (!c->cst || !c->wst || !c->est) fatal(where, "out of host memory");
+  c->cell_rank = ranks(g->cellp, g->Nc);
+  c->edge_rank = ranks(g->edgep, g->Ne);
+  if (!c->cst || !c->wst || !c->est || !c->cell_rank || !c->edge_rank) fatal(where, "out of host memory");
   c->key = g; c->Nc = g->Nc; c->Ne = g->Ne;
   g_nctx++;
   return c;
@@ -172,30 +189,53 @@ static double *next_buf(ctx_t *c, const char *where) {
   return c->flat;
 }
 
+/* Send the moving vertical extents unless they equal the ones last sent. */
 static void extents(ctx_t *c, gridT *g, const char *where) {
+  const size_t nc = (size_t)g->Nc * sizeof(int), ne = (size_t)g->Ne * sizeof(int);
+  if (c->x_last && !memcmp(c->x_last, g->ctop, nc) && !memcmp(c->x_last + g->Nc, g->ctopold, nc) &&
+      !memcmp(c->x_last + 2 * (size_t)g->Nc, g->Nke, ne) && !memcmp(c->x_last + 2 * (size_t)g->Nc + g->Ne, g->etop, ne))
+    return;
   sunhip_extents X = {g->ctop, g->ctopold, g->Nke, g->etop};
   check(sunhip_set_extents(c->dev, &X), where);
+  if (!c->x_last) c->x_last = (int *)malloc(2 * (nc + ne));
+  if (!c->x_last) return;
+  memcpy(c->x_last, g->ctop, nc);
+  memcpy(c->x_last + g->Nc, g->ctopold, nc);
+  memcpy(c->x_last + 2 * (size_t)g->Nc, g->Nke, ne);
+  memcpy(c->x_last + 2 * (size_t)g->Nc + g->Ne, g->etop, ne);
+}
+/* The device's extents were changed on the device: send them next time. */
+static void extents_changed(ctx_t *c) {
+  free(c->x_last);
+  c->x_last = NULL;
 }
 
 /* ---- jagged rows <-> flat, split over threads -------------------------- */
-enum { ROWS_PACK, ROWS_UNPACK, ROWS_UNPACK_FROM, ROWS_UNPACK_UPTO };
+enum { ROWS_PACK, ROWS_UNPACK, ROWS_UNPACK_FROM, ROWS_UNPACK_UPTO, ROWS_UNPACK_ZERO };
 typedef struct {
   int mode;
   double *flat;
   REAL **rows;
   const size_t *start;     /* row i at flat + start[i], start[i+1] - start[i] values */
-  const int *order;        /* unpack: row of entry ip (NULL: ip itself) */
+  const int *rank;         /* unpack: row i's position in cellp / edgep (NULL: every row) */
+  int rlo, rhi, rlo2, rhi2;  /* ... only rows whose position is in [rlo, rhi) or [rlo2, rhi2) */
   const int *lvl;          /* FROM: first level written; UPTO: levels written */
-  int lo, hi;
+  int lo, hi;              /* rows lo..hi-1 */
 } rows_job;
 
 static void rows_run(const rows_job *J, int lo, int hi) {
-  for (int ip = lo; ip < hi; ip++) {
-    const int i = J->order ? J->order[ip] : ip;
+  for (int i = lo; i < hi; i++) {
     const size_t o = J->start[i], n = J->start[i + 1] - o;
+    if (J->rank) {
+      const int r = J->rank[i];
+      if ((r < J->rlo || r >= J->rhi) && (r < J->rlo2 || r >= J->rhi2)) {
+        if (J->mode == ROWS_UNPACK_ZERO) memset(J->rows[i], 0, n * sizeof(double));
+        continue;
+      }
+    }
     switch (J->mode) {
       case ROWS_PACK: memcpy(J->flat + o, J->rows[i], n * sizeof(double)); break;
-      case ROWS_UNPACK: memcpy(J->rows[i], J->flat + o, n * sizeof(double)); break;
+      case ROWS_UNPACK: case ROWS_UNPACK_ZERO: memcpy(J->rows[i], J->flat + o, n * sizeof(double)); break;
       case ROWS_UNPACK_FROM: {
         const size_t k0 = (size_t)J->lvl[i];
         if (k0 < n) memcpy(J->rows[i] + k0, J->flat + o + k0, (n - k0) * sizeof(double));
@@ -206,14 +246,8 @@ static void rows_run(const rows_job *J, int lo, int hi) {
   }
 }
 
-typedef struct { const rows_job *J; int lo, hi; } rows_part;
-static void *rows_thread(void *a) {
-  const rows_part *P = (const rows_part *)a;
-  rows_run(P->J, P->lo, P->hi);
-  return NULL;
-}
-
-/* Threads for packing: SUNHIP_DROPIN_THREADS, else the online cores, at most 16. */
+/* Threads for packing: SUNHIP_DROPIN_THREADS, else the online cores, at most
+ * 16.  They are started on first use and wait for work between jobs. */
 static int n_threads(void) {
   static int n = 0;
   if (!n) {
@@ -227,51 +261,160 @@ static int n_threads(void) {
   return n;
 }
 
+static struct {
+  pthread_mutex_t mu;
+  pthread_cond_t go, done;
+  int started, T;          /* threads running (besides the caller), shares per job */
+  unsigned gen;            /* job counter */
+  int pending;             /* shares not yet finished */
+  const rows_job *job;
+} g_pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 1, 0, 0, NULL};
+
+static void share(const rows_job *J, int t, int T, int *lo, int *hi) {
+  const int n = J->hi - J->lo;
+  *lo = J->lo + (int)((int64_t)n * t / T);
+  *hi = J->lo + (int)((int64_t)n * (t + 1) / T);
+}
+
+static void *pool_worker(void *arg) {
+  const int t = (int)(intptr_t)arg;
+  unsigned seen = 0;
+  for (;;) {
+    pthread_mutex_lock(&g_pool.mu);
+    while (g_pool.gen == seen) pthread_cond_wait(&g_pool.go, &g_pool.mu);
+    seen = g_pool.gen;
+    const rows_job *J = g_pool.job;
+    const int T = g_pool.T;
+    pthread_mutex_unlock(&g_pool.mu);
+    if (t < T) {
+      int lo, hi;
+      share(J, t, T, &lo, &hi);
+      rows_run(J, lo, hi);
+    }
+    pthread_mutex_lock(&g_pool.mu);
+    if (--g_pool.pending == 0) pthread_cond_signal(&g_pool.done);
+    pthread_mutex_unlock(&g_pool.mu);
+  }
+  return NULL;
+}
+
+/* Run J over the pool: share 0 on the calling thread, share t on worker t. */
 static void rows_par(const rows_job *J, size_t nvalues) {
   int T = n_threads();
-  if (nvalues < ((size_t)1 << 18) || J->hi - J->lo < 2 * T) T = 1;
-  pthread_t th[64];
-  rows_part P[64];
-  int ok[64];
-  const int n = J->hi - J->lo;
-  for (int t = 0; t < T; t++) {
-    P[t].J = J;
-    P[t].lo = J->lo + (int)((int64_t)n * t / T);
-    P[t].hi = J->lo + (int)((int64_t)n * (t + 1) / T);
-    ok[t] = t > 0 && pthread_create(&th[t], NULL, rows_thread, &P[t]) == 0;
+  if (nvalues < ((size_t)1 << 16) || J->hi - J->lo < 2 * T) T = 1;
+  if (T > 1 && g_pool.started < n_threads() - 1) {
+    while (g_pool.started < n_threads() - 1) {
+      pthread_t th;
+      if (pthread_create(&th, NULL, pool_worker, (void *)(intptr_t)(g_pool.started + 1))) break;
+      pthread_detach(th);
+      g_pool.started++;
+    }
   }
-  for (int t = 0; t < T; t++)
-    if (!ok[t]) rows_run(J, P[t].lo, P[t].hi);  /* this thread's share, and any a thread could not take */
-  for (int t = 1; t < T; t++)
-    if (ok[t]) pthread_join(th[t], NULL);
+  if (T > g_pool.started + 1) T = g_pool.started + 1;
+  if (T <= 1) {
+    rows_run(J, J->lo, J->hi);
+    return;
+  }
+  pthread_mutex_lock(&g_pool.mu);
+  g_pool.job = J;
+  g_pool.T = T;
+  g_pool.pending = g_pool.started;  /* every worker answers each job */
+  g_pool.gen++;
+  pthread_cond_broadcast(&g_pool.go);
+  pthread_mutex_unlock(&g_pool.mu);
+  int lo, hi;
+  share(J, 0, T, &lo, &hi);
+  rows_run(J, lo, hi);
+  pthread_mutex_lock(&g_pool.mu);
+  while (g_pool.pending) pthread_cond_wait(&g_pool.done, &g_pool.mu);
+  pthread_mutex_unlock(&g_pool.mu);
 }
 
-/* flat <-> jagged */
-static double *pack_rows(ctx_t *c, REAL **rows, const size_t *start, int n, const char *where) {
-  rows_job J = {ROWS_PACK, next_buf(c, where), rows, start, NULL, NULL, 0, n};
-  rows_par(&J, start[n]);
-  return c->flat;
-}
-static double *pack_cells(ctx_t *c, const gridT *g, REAL **rows, int extra) {
-  return pack_rows(c, rows, extra ? c->wst : c->cst, g->Nc, "pack");
-}
-static double *pack_edges(ctx_t *c, const gridT *g, REAL **rows) { return pack_rows(c, rows, c->est, g->Ne, "pack"); }
-
-/* write back cells of cellp[celldist[lo]..celldist[hi]) from c->flat */
-static void unpack_cells(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  rows_job J = {ROWS_UNPACK, c->flat, rows, c->cst, g->cellp, NULL, g->celldist[lo], g->celldist[hi]};
-  rows_par(&J, c->cst[g->Nc]);
+/* Parts a transfer of nvalues is split into: about 16 MB each, 1..8. */
+static int n_parts(size_t nvalues) {
+  size_t k = nvalues * sizeof(double) / ((size_t)16 << 20);
+  return k < 1 ? 1 : k > 8 ? 8 : (int)k;
 }
 
+/* Pack rows 0..n-1 into the next pinned slot part by part, each part's copy
+ * starting while the next is packed, and upload them into field name. */
+static void upload_rows(ctx_t *c, const char *name, REAL **rows, const size_t *start, int n, const char *where) {
+  double *flat = next_buf(c, where);
+  const int K = n_parts(start[n]);
+  for (int p = 0; p < K; p++) {
+    rows_job J = {ROWS_PACK, flat, rows, start, NULL, 0, 0, 0, 0, NULL, (int)((int64_t)n * p / K),
+                  (int)((int64_t)n * (p + 1) / K)};
+    rows_par(&J, start[J.hi] - start[J.lo]);
+    check(sunhip_upload_part(c->dev, name, flat, (int64_t)start[J.lo], (int64_t)(start[J.hi] - start[J.lo]),
+                             p == K - 1),
+          where);
+  }
+}
 static void upload_cells(ctx_t *c, gridT *g, const char *name, REAL **rows, int extra, const char *where) {
-  check(sunhip_upload(c->dev, name, pack_cells(c, g, rows, extra)), where);
+  upload_rows(c, name, rows, extra ? c->wst : c->cst, g->Nc, where);
 }
 static void upload_edges(ctx_t *c, gridT *g, const char *name, REAL **rows, const char *where) {
-  check(sunhip_upload(c->dev, name, pack_edges(c, g, rows)), where);
+  upload_rows(c, name, rows, c->est, g->Ne, where);
 }
-/* field name into the next pinned slot (c->flat) */
+
+/* Start the download of field name into the next pinned slot (c->flat) in
+ * parts; the unpack_* calls below wait for each part as they need it. */
 static void download(ctx_t *c, const char *name, const char *where) {
-  check(sunhip_download(c->dev, name, next_buf(c, where)), where);
+  double *flat = next_buf(c, where);
+  c->parts = n_parts(c->flat_n);
+  check(sunhip_download_parts(c->dev, name, flat, c->parts), where);
+}
+/* The whole of field name in c->flat. */
+static void download_all(ctx_t *c, const char *name, const char *where) {
+  download(c, name, where);
+  check(sunhip_download_wait(c->dev, INT64_MAX), where);
+}
+
+/* Write J's rows back from the download in progress, part by part. */
+static void unpack_parts(ctx_t *c, rows_job J) {
+  J.flat = c->flat;
+  const int lo = J.lo, n = J.hi - J.lo, K = c->parts;
+  for (int p = 0; p < K; p++) {
+    J.lo = lo + (int)((int64_t)n * p / K);
+    J.hi = lo + (int)((int64_t)n * (p + 1) / K);
+    check(sunhip_download_wait(c->dev, (int64_t)J.start[J.hi]), "unpack");
+    rows_par(&J, J.start[J.hi] - J.start[J.lo]);
+  }
+  check(sunhip_download_wait(c->dev, INT64_MAX), "unpack");
+}
+
+/* write back cells of cellp[celldist[lo]..celldist[hi]) */
+static void unpack_cells(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  rows_job J = {ROWS_UNPACK, NULL, rows, c->cst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL, 0,
+                g->Nc};
+  unpack_parts(c, J);
+}
+/* write back cells of cellp[celldist[lo]..celldist[hi]) and zero the other rows */
+static void unpack_cells_zero(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  rows_job J = {ROWS_UNPACK_ZERO, NULL, rows, c->cst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL,
+                0, g->Nc};
+  unpack_parts(c, J);
+}
+/* write back rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
+static void unpack_cells_w(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  rows_job J = {ROWS_UNPACK, NULL, rows, c->wst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL, 0,
+                g->Nc};
+  unpack_parts(c, J);
+}
+/* write back levels [ctop, Nk) of every cell */
+static void unpack_cells_wet(ctx_t *c, const gridT *g, REAL **rows) {
+  rows_job J = {ROWS_UNPACK_FROM, NULL, rows, c->cst, NULL, 0, 0, 0, 0, g->ctop, 0, g->Nc};
+  unpack_parts(c, J);
+}
+/* write back every edge row */
+static void unpack_edges(ctx_t *c, const gridT *g, REAL **rows) {
+  rows_job J = {ROWS_UNPACK, NULL, rows, c->est, NULL, 0, 0, 0, 0, NULL, 0, g->Ne};
+  unpack_parts(c, J);
+}
+/* write back levels [0, Nke) of every edge row */
+static void unpack_edges_wet(ctx_t *c, const gridT *g, REAL **rows) {
+  rows_job J = {ROWS_UNPACK_UPTO, NULL, rows, c->est, NULL, 0, 0, 0, 0, g->Nke, 0, g->Ne};
+  unpack_parts(c, J);
 }
 
 /* qprecond=1: 1 when c's rows are the residual rows phys->stmp3 CGSolveQ
@@ -339,7 +482,7 @@ void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MP
     check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
   }
   g_last[1].valid = 1;
-  download(C, "h", W);
+  download_all(C, "h", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
     phys->h[i] = C->flat[i];
@@ -384,7 +527,7 @@ void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *ph
   check(sunhip_upload(C->dev, "hcoef", coef), W);
   check(sunhip_upload(C->dev, "hx", x), W);
   check(sunhip_h_operator_coef(C->dev, fcoef, "hx", "hy"), W);
-  download(C, "hy", W);
+  download_all(C, "hy", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
     y[i] = C->flat[i];
@@ -409,12 +552,11 @@ static void download_face_values(ctx_t *C, gridT *g, physT *p, const char *W) {
   const char *names[2] = {"SfHp", "SfHm"};
   REAL **rows[2] = {p->SfHp, p->SfHm};
   for (int f = 0; f < 2; f++) {
+    /* levels [0, Nke) of the edges of edgep[edgedist[0..1)) and edgep[edgedist[2..3)) */
     download(C, names[f], W);
-    for (int r = 0; r < 2; r++) {
-      rows_job J = {ROWS_UNPACK_UPTO, C->flat, rows[f], C->est, g->edgep, g->Nke, g->edgedist[r == 0 ? 0 : 2],
-                    g->edgedist[r == 0 ? 1 : 3]};
-      rows_par(&J, C->est[g->Ne]);
-    }
+    rows_job J = {ROWS_UNPACK_UPTO, NULL, rows[f], C->est, C->edge_rank, g->edgedist[0], g->edgedist[1],
+                  g->edgedist[2], g->edgedist[3], g->Nke, 0, g->Ne};
+    unpack_parts(C, J);
   }
 }
 
@@ -471,28 +613,12 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   check(sunhip_update_scalars(C->dev, &S, "s"), W);
   download(C, "s", W);
   unpack_cells(C, grid, scal, 0, 1);
-  /* Cn: zero everywhere, then the computational columns */
-  for (int i = 0; i < grid->Nc; i++) memset(Cn[i], 0, (size_t)grid->Nk[i] * sizeof(double));
+  /* Cn: the computational columns, zero in the others */
   download(C, "Cn", W);
-  unpack_cells(C, grid, Cn, 0, 1);
+  unpack_cells_zero(C, grid, Cn, 0, 1);
   if (TVDscheme) download_face_values(C, grid, phys, W);
 }
 
-/* write back rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
-static void unpack_cells_w(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  rows_job J = {ROWS_UNPACK, c->flat, rows, c->wst, g->cellp, NULL, g->celldist[lo], g->celldist[hi]};
-  rows_par(&J, c->wst[g->Nc]);
-}
-/* write back levels [ctop, Nk) of every cell */
-static void unpack_cells_wet(ctx_t *c, const gridT *g, REAL **rows) {
-  rows_job J = {ROWS_UNPACK_FROM, c->flat, rows, c->cst, NULL, g->ctop, 0, g->Nc};
-  rows_par(&J, c->cst[g->Nc]);
-}
-/* write back every edge row */
-static void unpack_edges(ctx_t *c, const gridT *g, REAL **rows) {
-  rows_job J = {ROWS_UNPACK, c->flat, rows, c->est, NULL, NULL, 0, g->Ne};
-  rows_par(&J, c->est[g->Ne]);
-}
 
 void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs) {
   static const char *W = "ComputeQSource";
@@ -508,7 +634,7 @@ void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int mypro
   check(sunhip_compute_qsource(C->dev, prop->theta, prop->dt, "src"), W);
   download(C, "src", W);
   unpack_cells_wet(C, grid, src);
-  download(C, "D", W);
+  download_all(C, "D", W);
   memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
 }
 
@@ -546,11 +672,6 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
   unpack_cells_w(C, grid, w, 0, 1);
 }
 
-/* write back levels [0, Nke) of every edge row */
-static void unpack_edges_wet(ctx_t *c, const gridT *g, REAL **rows) {
-  rows_job J = {ROWS_UNPACK_UPTO, c->flat, rows, c->est, NULL, g->Nke, 0, g->Ne};
-  rows_par(&J, c->est[g->Ne]);
-}
 
 static sunhip_upred_params upred_params(const propT *prop) {
   sunhip_upred_params P = {prop->dt, prop->theta, prop->grav, prop->nu, prop->laxWendroff_Vertical, prop->thetaM,
@@ -595,7 +716,7 @@ void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, 
   unpack_edges_wet(C, grid, phys->utmp2);
   download(C, "wtmp2", W);
   unpack_cells_w(C, grid, phys->wtmp2, 0, 2);
-  download(C, "D", W);
+  download_all(C, "D", W);
   memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
 }
 
@@ -621,11 +742,11 @@ void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, 
   sunhip_cg_params G = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0, 0};
   check(sunhip_u_solve(C->dev, &P, &G, &g_last[1].st), W);
   g_last[1].valid = 1;
-  download(C, "h", W);
+  download_all(C, "h", W);
   memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-  download(C, "hcorr", W);
+  download_all(C, "hcorr", W);
   memcpy(phys->hcorr, C->flat, (size_t)grid->Nc * sizeof(double));
-  download(C, "active", W);
+  download_all(C, "active", W);
   for (int i = 0; i < grid->Nc; i++) phys->active[i] = C->flat[i] != 0;
   download(C, "u", W);
   unpack_edges(C, grid, phys->u);
@@ -649,6 +770,7 @@ void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option) {
   check(sunhip_upload(C->dev, "dv", grid->dv), W);
   sunhip_vgrid_params V = vgrid_params(grid, prop);
   check(sunhip_update_dz(C->dev, &V, option), W);
+  extents_changed(C);
   download(C, "dzz", W);
   unpack_cells(C, grid, grid->dzz, 0, 2);
   check(sunhip_get_extents(C->dev, grid->ctop, option != -1 ? grid->ctopold : NULL, grid->etop,
@@ -657,15 +779,15 @@ void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option) {
     download(C, "dzzold", W);
     unpack_cells(C, grid, grid->dzzold, 0, 2);
   } else {
-    download(C, "h", W);
+    download_all(C, "h", W);
     memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-    download(C, "dv", W);
+    download_all(C, "dv", W);
     memcpy(grid->dv, C->flat, (size_t)grid->Nc * sizeof(double));
-    download(C, "dzbot", W);
+    download_all(C, "dzbot", W);
     memcpy(grid->dzbot, C->flat, (size_t)grid->Nc * sizeof(double));
   }
   if (option == 1) {
-    download(C, "dzfB", W);
+    download_all(C, "dzfB", W);
     memcpy(grid->dzfB, C->flat, (size_t)grid->Ne * sizeof(double));
   }
 }

@@ -583,6 +583,8 @@ int sunhip_grid_destroy(sunhip_grid *g) {
     if (g->h_pin[i]) hipHostFree(g->h_pin[i]);
     if (g->ev_pin[i]) hipEventDestroy(g->ev_pin[i]);
   }
+  for (int i = 0; i < SUNHIP_MAXPARTS; i++)
+    if (g->ev_part[i]) hipEventDestroy(g->ev_part[i]);
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->h_poll) hipHostFree(g->h_poll);
   for (hipEvent_t e : g->evp) if (e) hipEventDestroy(e);
@@ -702,15 +704,9 @@ static int pinned_slot(const sunhip_grid *g, const double *flat, size_t n) {
   return -1;
 }
 
-int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
-  HIPCHK(hipSetDevice(g->device));
-  Field *f;
-  int rc = field(g, name, &f);
-  if (rc) return rc;
-  const size_t n = flat_len(g, f->kind);
-  if ((rc = ensure_stage(g, n))) return rc;
-  const int pin = pinned_slot(g, flat, n);
-  HIPCHK(hipMemcpyAsync(g->d_stage, flat, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+}  // extern "C"
+
+static void scatter_field(sunhip_grid *g, const Field *f) {
   switch (f->kind) {
     case CELL: case CELLW:
       k_scatter_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
@@ -727,23 +723,9 @@ int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
       k_scatter_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
       break;
   }
-  HIPCHK(hipGetLastError());
-  if (pin >= 0) {  // the slot is reused only through sunhip_pinned, which waits for this copy
-    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
-    g->pin_pending[pin] = true;
-    return 0;
-  }
-  HIPCHK(hipStreamSynchronize(g->stream));  // the caller may reuse its buffer
-  return 0;
 }
 
-int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
-  HIPCHK(hipSetDevice(g->device));
-  Field *f;
-  int rc = field(g, name, &f);
-  if (rc) return rc;
-  const size_t n = flat_len(g, f->kind);
-  if ((rc = ensure_stage(g, n))) return rc;
+static void gather_field(sunhip_grid *g, const Field *f) {
   switch (f->kind) {
     case CELL: case CELLW:
       k_gather_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
@@ -759,9 +741,124 @@ int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
       k_gather_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
       break;
   }
+}
+
+// A download in parts not yet waited for in full.
+static int parts_outstanding(sunhip_grid *g) {
+  if (g->parts_done < g->nparts) {
+    set_error("a download in parts has not been waited for in full");
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  if ((rc = parts_outstanding(g))) return rc;
+  if (g->up_filled) {
+    set_error("sunhip_upload: an upload in parts is in progress");
+    return -1;
+  }
+  const size_t n = flat_len(g, f->kind);
+  if ((rc = ensure_stage(g, n))) return rc;
+  const int pin = pinned_slot(g, flat, n);
+  HIPCHK(hipMemcpyAsync(g->d_stage, flat, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  scatter_field(g, f);
+  HIPCHK(hipGetLastError());
+  if (pin >= 0) {  // the slot is reused only through sunhip_pinned, which waits for this copy
+    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
+    g->pin_pending[pin] = true;
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));  // the caller may reuse its buffer
+  return 0;
+}
+
+int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int64_t off, int64_t n, int last) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  if ((rc = parts_outstanding(g))) return rc;
+  const int64_t len = (int64_t)flat_len(g, f->kind);
+  if (n < 0 || off != g->up_filled || off + n > len || (last && off + n != len)) {
+    set_error("sunhip_upload_part %s: part [%lld, %lld) after %lld of %lld values%s", name, (long long)off,
+              (long long)(off + n), (long long)g->up_filled, (long long)len, last ? " (last)" : "");
+    g->up_filled = 0;
+    return -1;
+  }
+  if ((rc = ensure_stage(g, (size_t)len))) return rc;
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(g->d_stage + off, flat + off, (size_t)n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  g->up_filled = off + n;
+  if (!last) return 0;
+  g->up_filled = 0;
+  scatter_field(g, f);
+  HIPCHK(hipGetLastError());
+  const int pin = pinned_slot(g, flat, (size_t)len);
+  if (pin >= 0) {
+    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
+    g->pin_pending[pin] = true;
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  if ((rc = parts_outstanding(g))) return rc;
+  const size_t n = flat_len(g, f->kind);
+  if ((rc = ensure_stage(g, n))) return rc;
+  gather_field(g, f);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(flat, g->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int nparts) {
+  HIPCHK(hipSetDevice(g->device));
+  Field *f;
+  int rc = field(g, name, &f);
+  if (rc) return rc;
+  if ((rc = parts_outstanding(g))) return rc;
+  if (nparts < 1 || nparts > SUNHIP_MAXPARTS) {
+    set_error("sunhip_download_parts: %d parts (1..%d)", nparts, SUNHIP_MAXPARTS);
+    return -1;
+  }
+  const int64_t n = (int64_t)flat_len(g, f->kind);
+  if ((rc = ensure_stage(g, (size_t)n))) return rc;
+  gather_field(g, f);
+  HIPCHK(hipGetLastError());
+  for (int p = 0; p <= nparts; p++) g->part_bound[p] = n * p / nparts;
+  for (int p = 0; p < nparts; p++) {
+    if (!g->ev_part[p]) HIPCHK(hipEventCreateWithFlags(&g->ev_part[p], hipEventDisableTiming));
+    const int64_t a = g->part_bound[p], b = g->part_bound[p + 1];
+    if (b > a)
+      HIPCHK(hipMemcpyAsync(flat + a, g->d_stage + a, (size_t)(b - a) * sizeof(double), hipMemcpyDeviceToHost,
+                            g->stream));
+    HIPCHK(hipEventRecord(g->ev_part[p], g->stream));
+  }
+  g->nparts = nparts;
+  g->parts_done = 0;
+  return 0;
+}
+
+int sunhip_download_wait(sunhip_grid *g, int64_t upto) {
+  HIPCHK(hipSetDevice(g->device));
+  while (g->parts_done < g->nparts && g->part_bound[g->parts_done] < upto) {
+    HIPCHK(hipEventSynchronize(g->ev_part[g->parts_done]));
+    g->parts_done++;
+  }
   return 0;
 }
 
@@ -879,6 +976,32 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
                                                     int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
+}
+
+extern "C" int sunhip_pcie_probe(sunhip_grid *g, int64_t nbytes, int reps, double *h2d_ms, double *d2h_ms) {
+  HIPCHK(hipSetDevice(g->device));
+  if (nbytes <= 0 || reps <= 0 || nbytes > ((int64_t)1 << 36)) { set_error("sunhip_pcie_probe: bad size"); return -1; }
+  void *h = nullptr, *d = nullptr;
+  HIPCHK(hipHostMalloc(&h, (size_t)nbytes, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&d, (size_t)nbytes));
+  memset(h, 0, (size_t)nbytes);
+  double *out[2] = {h2d_ms, d2h_ms};
+  for (int dir = 0; dir < 2; dir++) {
+    void *dst = dir == 0 ? d : h;
+    const void *src = dir == 0 ? h : d;
+    const hipMemcpyKind kind = dir == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    HIPCHK(hipMemcpyAsync(dst, src, (size_t)nbytes, kind, g->stream));  // warm
+    HIPCHK(hipEventRecord(g->ev0, g->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(hipMemcpyAsync(dst, src, (size_t)nbytes, kind, g->stream));
+    HIPCHK(hipEventRecord(g->ev1, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    *out[dir] = ms / reps;
+  }
+  HIPCHK(hipFree(d));
+  HIPCHK(hipHostFree(h));
+  return 0;
 }
 
 extern "C" int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms) {

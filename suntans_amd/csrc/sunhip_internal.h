@@ -160,6 +160,11 @@ struct sunhip_grid {
   size_t h_pin_n[2] = {0, 0};
   hipEvent_t ev_pin[2] = {nullptr, nullptr};  // last upload read from the slot
   bool pin_pending[2] = {false, false};
+  // transfers in parts (sunhip_upload_part, sunhip_download_parts)
+  int64_t up_filled = 0;                     // values of the upload in progress copied so far
+  int nparts = 0, parts_done = 0;            // download in progress: parts issued, parts waited for
+  int64_t part_bound[SUNHIP_MAXPARTS + 1] = {};
+  hipEvent_t ev_part[SUNHIP_MAXPARTS] = {};  // part p of the download is on the host
   bool have_extents = false;
   // reductions
   double *d_part = nullptr;         // block partials

@@ -125,6 +125,7 @@ def lib() -> C.CDLL:
         L.sunhip_kernel_times.argtypes = [vp, _dp, _ip]
         L.sunhip_kernel_times.restype = C.c_int
         L.sunhip_copy_probe.argtypes = [vp, C.c_int64, C.c_int, _dp]
+        L.sunhip_pcie_probe.argtypes = [vp, C.c_int64, C.c_int, _dp, _dp]
         L.sunhip_q_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_int,
                                            C.POINTER(CGStats), _ip]
         L.sunhip_h_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
@@ -404,6 +405,12 @@ class DeviceGrid:
         out = C.c_double()
         check(lib().sunhip_copy_probe(self.h, int(nbytes), int(reps), C.byref(out)), "copy_probe")
         return out.value
+
+    def pcie_probe(self, nbytes: int, reps: int):
+        """Pinned host <-> HBM copies of nbytes: (h2d ms, d2h ms) per copy."""
+        a, b = C.c_double(), C.c_double()
+        check(lib().sunhip_pcie_probe(self.h, int(nbytes), int(reps), C.byref(a), C.byref(b)), "pcie_probe")
+        return a.value, b.value
 
 
 class Batch:

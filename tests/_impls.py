@@ -95,6 +95,12 @@ class _Cpu:
     def upredictor_solve(self, st):
         return self.L.oracle_UPredictor_solve(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
 
+    def upredictor_rhs(self, st):
+        self.L.oracle_UPredictor_rhs(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
+    def upredictor_finish(self, st):
+        self.L.oracle_UPredictor_finish(C.c_void_p(st.g), C.c_void_p(st.p), C.c_void_p(st.prop))
+
     def upredictor(self, st):
         """The whole of UPredictor (reference), or its two device-covered
         parts in order (oracle); returns nonzero where the reference exits."""
@@ -146,19 +152,26 @@ class RefImpl(_Cpu):
         self.L.ref_set_verbose.argtypes = [C.c_int]
         self.L.ref_update_salinity.argtypes = [_vp] * 3
         self._last = {}
+        self._hist = {}
+        self.history = False  # also capture the per-iteration residuals
+        self.call_s = 0.0     # wall time of the last solve call alone
 
     def _captured(self, fn, which):
         """Run fn with the reference's convergence report captured from stdout
-        (it prints the iteration count at VERBOSE>2, main/phys.c:2527-2536)."""
+        (it prints the iteration count at VERBOSE>2, main/phys.c:2527-2536,
+        and every iteration's residual at VERBOSE>3, :2522 and :3200)."""
         import sys
+        import time
         sys.stdout.flush()
-        self.L.ref_set_verbose(3)
+        self.L.ref_set_verbose(4 if self.history else 3)
         libc = C.CDLL(None)
         fd = os.dup(1)
         with tempfile.TemporaryFile(mode="w+b") as tmp:
             os.dup2(tmp.fileno(), 1)
             try:
+                t = time.perf_counter()
                 fn()
+                self.call_s = time.perf_counter() - t
                 libc.fflush(None)
             finally:
                 os.dup2(fd, 1)
@@ -168,6 +181,7 @@ class RefImpl(_Cpu):
             out = tmp.read().decode()
         m = re.search(r"after (\d+) (?:iterations|steps)[^=]*(?:res|RES)=([-+0-9.eE]+|nan|inf)", out)
         self._last[which] = (int(m.group(1)), float(m.group(2))) if m else (None, None)
+        self._hist[which] = [float(v) for v in re.findall(r"Iteration: \d+, resid=([-+0-9.eE]+|nan|inf)", out)]
         return out
 
     def cgsolveq(self, st, c="stmp3"):
@@ -178,6 +192,11 @@ class RefImpl(_Cpu):
 
     def stats(self, which):
         return self._last.get(which, (None, None))
+
+    def residuals(self, which):
+        """Residuals sqrt(eps/eps0) the last solve printed after each
+        iteration (history mode only)."""
+        return list(self._hist.get(which, []))
 
 
 class DropinImpl:

@@ -14,9 +14,12 @@ oracle, iteration counters within +-1; exactly n PCG iterations within
 1e-10 relative to the initial one, and the error against a tight
 (epsilon 1e-14) oracle solve at most 1e-8; the measured errors are printed.
 """
+import os
+
 import numpy as np
 import pytest
 
+from golden_io import inputs_digest
 from suntans_amd import cases, device
 
 import _impls
@@ -195,3 +198,58 @@ def test_large_h_solve(gpu_available, hprecond):
     assert err <= 1e-10
     assert r["converged"] == 1
     assert e_mix <= 1e-8
+
+
+FULL_FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "large_solve.npz")
+
+
+def _sample_err(data, key, v):
+    idx = data[key + "_idx"]
+    assert len(v) == int(data[key + "_len"][0]), key
+    return relerr(v[idx], data[key + "_val"]), abs(np.linalg.norm(v) / data[key + "_norm"][0] - 1)
+
+
+@pytest.mark.gpu
+def test_full_size_vs_reference(gpu_available):
+    """The large solve at full size (1000x500 quads as 1.0M triangles, ~5.0e7
+    unknowns) against the reference's own solves of it
+    (tests/golden/large_solve.npz): CGSolveQ (qprecond 2) and CGSolve
+    (hprecond 1, D depth-integrated) to 1e-10, iteration counts within +-1,
+    the solutions within 1e-10 relative 2-norm over the fixture's 4096 seeded
+    samples and in 2-norm; and the residual after exactly k iterations equal
+    to the reference's printed history (7 digits) for several k."""
+    data = np.load(FULL_FIX)
+    st = cases.large_solve(seed=0)
+    assert st.N == int(data["N"][0])
+    assert inputs_digest(st) == data["inputs_sha256"].tobytes()
+    m = st.wet_mask()
+    dg = _dev(st, {})
+    dg.copy("q0", "q")
+    r = dg.q_solve(maxiters=2000, precond=2, epsilon=1e-10)
+    q = dg.download("q", len(m))[m]
+    es, en = _sample_err(data, "q", q)
+    print(f"full q: iterations reference={int(data['q_iters'][0])} device={r['niters']} sample err={es:.2e} "
+          f"norm err={en:.2e}")
+    assert abs(r["niters"] - int(data["q_iters"][0])) <= 1
+    assert es <= 1e-10 and en <= 1e-10
+    hist = data["q_hist"]
+    for k in (1, 2, 5, 20, 50):
+        dg.copy("q", "q0")
+        rk = dg.q_solve(maxiters=k, precond=2, epsilon=0.0, fixed=k)
+        print(f"full q after {k}: device resid {rk['resid']:.7e} reference {hist[k - 1]:.7e}")
+        assert abs(rk["resid"] / hist[k - 1] - 1) <= 2e-6, k
+    dg.close()
+    # free surface
+    st.set("D", cases.depth_integrated_D(st))
+    dh = device.DeviceGrid(st)
+    for n in ("h", "htmp", "D"):
+        dh.upload(n, st.get(n))
+    g, th, dt = st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt")
+    rh = dh.h_solve(g, th, dt, maxiters=1000, precond=1, epsilon=1e-10)
+    h = dh.download("h", st.Nc)
+    dh.close()
+    es, en = _sample_err(data, "h", h)
+    print(f"full h: iterations reference={int(data['h_iters'][0])} device={rh['niters']} sample err={es:.2e} "
+          f"norm err={en:.2e}")
+    assert abs(rh["niters"] - int(data["h_iters"][0])) <= 1
+    assert es <= 1e-10 and en <= 1e-10

@@ -14,6 +14,7 @@ from suntans_amd import cases, device
 
 import _grids
 import _impls
+from _trajectory import device_step, upload_state
 
 pytestmark = pytest.mark.gpu
 
@@ -40,35 +41,6 @@ def oracle_step(o, a):
     o.cgsolveq(a)
     o.corrector(a, qc="qc")
     o.continuity(a, w="w")
-
-
-def device_step(dg, a):
-    g = a.get_prop
-    th, dt = g("theta"), g("dt")
-    P = device.UPredParams(dt, th, g("grav"), g("nu"), g("laxWendroff_Vertical"), g("thetaM"), int(g("nonlinear")))
-    dg.set_flux_height(a)
-    dg.u_predict(P)
-    sh = dg.u_solve(P, maxiters=int(g("maxiters")), precond=int(g("hprecond")), epsilon=g("epsilon"))
-    dg.update_dz(a, 0)
-    dg.continuity(th, "wnew")
-    dg.update_scalars(th, dt, g("kappa_s"), int(g("TVDsalt")), step=int(g("n")))
-    dg.w_predict(device.WPredParams.of(a))
-    dg.compute_qsource(th, dt, "src")
-    sq = dg.q_solve(maxiters=int(g("qmaxiters")), precond=2, epsilon=g("qepsilon"))
-    dg.corrector(dt, "q", "qp")
-    dg.continuity(th, "w")
-    return sh, sq
-
-
-def upload_state(dg, a):
-    for f in ("u", "utmp", "utmp2", "dzf", "dzz", "dzzold", "s", "kappa_tv", "nu_tv", "nu_lax", "wc", "w", "wnew",
-              "wtmp2", "wtmp", "Cn_W", "Cn_W2", "h", "dv", "hcorr", "CdB", "CdT", "tau_T", "D"):
-        dg.upload(f, a.get(f))
-    dg.upload("ue", a.get("ut"))
-    dg.upload("q", a.get("qc"))
-    dg.upload("qp", a.get("q"))
-    dg.upload("active", a.active().astype(np.float64))
-    dg.set_face_distances(device.face_distances(a))
 
 
 @pytest.mark.parametrize("nonlinear", [1, 2])
