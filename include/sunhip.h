@@ -175,13 +175,17 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * of each column whose Thomas pivots and forward values the preconditioner
  * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
  * at most 16; -1, the default: 24 from 1024 tiles of 64 columns up in FP64,
- * 16 from 2048 tiles up in FP32, else 0), "zreg" (1, the default: the z-level windows of sunhip_set_dz; 0 off),
- * "xchg" (the iteration's operator
- * kernel exchanges neighbour values inside each wave through LDS, gathering
- * only cross-tile neighbours from memory: 0 off, 1 on, 2 on with a register
- * budget of 4 waves per SIMD, 3 on with 2 levels per load group; -1, the
- * default: 2 from 4096 tiles up, else 0).  Only "tiles", "reduce", "rzid" and "band" change the
- * rounding of a dot product; every other knob gives bit-identical results. */
+ * 16 from 2048 tiles up in FP32, else 0), "zreg" (1, the default: the
+ * z-level windows of sunhip_set_dz; 0 off), "order" (1, the default: blocks
+ * of four tiles dispatched deepest first within each XCD's run; 0 in tile
+ * order), "xchg" (the iteration's operator kernel exchanges neighbour values
+ * inside each wave through LDS, gathering only cross-tile neighbours from
+ * memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD, 3 on
+ * with 2 levels per load group, 4-6 the pipelined form; -1, the default: 2
+ * from 4096 tiles up, else 0), "tvd_group" (levels per load group of the
+ * scalar step's column update: 1, 2 (the default) or 4).  Only "tiles",
+ * "reduce", "rzid", "band" and "order" change the rounding of a dot product;
+ * every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event
  * pair around every solver kernel launch; sunhip_kernel_times fills ms[] and

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restri
   for (int w = first_item(); w < n; w += stride) {
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
-    update_column_tile<NF>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+    update_column_tile<NF, 2>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
 

@@ -32,9 +32,9 @@ __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
   tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-template <int NF>
+template <int NF, int GRP>
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  update_column_tile<NF>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+  update_column_tile<NF, GRP>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 namespace sunhip {
@@ -107,12 +107,22 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
   int rc = tvd_args(g, sp, scal, &T, &A);
   if (rc) return rc;
   if (sp->TVD && (rc = sunhip_face_scalars(g, sp, scal))) return rc;
-  // the column update reads the old values from stmp and writes scal
-  HIPCHK(hipMemcpyAsync(const_cast<double *>(A.s_old), A.s, g->fields.at(scal).n * sizeof(double),
-                        hipMemcpyDeviceToDevice, g->stream));
+  if (sp->TVD) {
+    // a column reads old values of its own cells only, all before it writes
+    // them: the update runs in place
+    A.s_old = A.s;
+  } else {
+    // first-order upwind reads the neighbours' old values: keep them in stmp
+    HIPCHK(hipMemcpyAsync(const_cast<double *>(A.s_old), A.s, g->fields.at(scal).n * sizeof(double),
+                          hipMemcpyDeviceToDevice, g->stream));
+  }
   const int ps = prof_begin(g, PK_TCOLUMN);
   with_nf(g->mf, [&](auto nf) {
-    k_update_column<decltype(nf)::value><<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(g->geo(), A);
+    constexpr int NF = decltype(nf)::value;
+    const unsigned nb = (unsigned)((g->Nt + 3) / 4);
+    if (g->tune_tvdgrp == 4) k_update_column<NF, 4><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else if (g->tune_tvdgrp == 2) k_update_column<NF, 2><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else k_update_column<NF, 1><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     return 0;
   });
   prof_end(g, ps);

@@ -241,7 +241,69 @@ __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF],
   return acc;
 }
 
+// The loads of one level k of the column walk (levels beyond the column
+// clamped into it, so every load is in bounds and none waits on a branch):
+// wnew and wtmp2 at k+1; the old scalar at k+3, dzz, dzzold and kappa_tv at
+// k+2 (0 outside the column); per face at k the edge's u, utmp2, dzf and
+// either SfHp and SfHm (TVD) or the old scalar of its two cells.
 template <int NF>
+struct LevelIn {
+  double wn, wo, sn, zn, on, qn;
+  double fu[NF], ft[NF], fz[NF], fa[NF], fb[NF];
+};
+
+template <int NF>
+__device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[NF], int64_t base, int nk, int k,
+                                           LevelIn<NF> &L) {
+  const int kw = k < nk ? k : nk - 1;  // w-type rows hold nk + 1 levels
+  L.wn = A.wnew[base + (int64_t)(kw + 1) * 64];
+  L.wo = A.wtmp2[base + (int64_t)(kw + 1) * 64];
+  auto cl = [&](const double *p, int kk) {
+    const double v = p[base + (int64_t)(kk < nk ? kk : nk) * 64];  // a tile holds one level below its deepest column
+    return kk < nk ? v : 0.0;
+  };
+  L.sn = cl(A.s_old, k + 3);
+  L.zn = cl(A.dzz, k + 2);
+  L.on = cl(A.dzzold, k + 2);
+  L.qn = cl(A.kappa_tv, k + 2);
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int ne = fc[f].nke;
+    const int ke = k < ne ? k : (ne > 0 ? ne - 1 : 0);
+    const uint32_t ei = fc[f].eb + (uint32_t)ke * 64u;
+    L.fu[f] = at32(A.u, ei);
+    L.ft[f] = at32(A.utmp2, ei);
+    L.fz[f] = at32(A.dzf, ei);
+    if (A.TVD) {
+      L.fa[f] = at32(A.SfHp, ei);
+      L.fb[f] = at32(A.SfHm, ei);
+    } else {
+      L.fa[f] = A.s_old[fc[f].c1 + (int64_t)ke * 64];
+      L.fb[f] = A.s_old[fc[f].c2 + (int64_t)ke * 64];
+    }
+  }
+}
+
+// hflux on the loaded values: the same products in the same order
+template <int NF>
+__device__ __forceinline__ double hflux_in(const ColArgs &A, const FaceC (&fc)[NF], int nf, int k,
+                                           const LevelIn<NF> &L) {
+  double acc = 0;
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    if (f >= nf || k >= fc[f].nke) continue;
+    const double ut2 = L.ft[f];
+    const double temp = A.TVD ? (ut2 > 0 ? L.fa[f] : L.fb[f]) : upwind_ref(ut2, L.fa[f], L.fb[f]);
+    acc += fc[f].cf * (A.theta * L.fu[f] + (1 - A.theta) * ut2) * temp * L.fz[f];
+  }
+  return acc;
+}
+
+// One column of UpdateScalars.  GRP > 1: the levels below the top row are
+// walked GRP at a time, every load of a group issued before its arithmetic,
+// and the back substitution reads its rows a group at a time; the values
+// are those of GRP = 1 bit for bit.
+template <int NF, int GRP = 1>
 __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, nk = G.nk[s];
@@ -296,7 +358,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   ap_am_old(ktop + 1, apO, amO);
   double bd_k1 = (ktop + 1 < nk) ? bd() : 0.0;  // bd at k+1
   double apN_k = 0, amN_k = 0, apO_k = 0, amO_k = 0, bd_k = 0;
-  for (int m = 0; m <= nb; m++) {
+  for (int m = 0; m <= (GRP > 1 ? 0 : nb); m++) {
     const int k = ktop + m;
     // shift the level-(k+1) quantities to level k, compute level k+1
     if (m > 0) {
@@ -363,6 +425,66 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
     o0 = o1; o1 = dzo.at(k + 2);
     q0 = q1; q1 = kap(k + 2);
   }
+  if (GRP > 1) {
+    // rows m = 1..nb, GRP levels at a time (the body of the loop above for
+    // m > 0, on loaded values)
+    for (int m0 = 1; m0 <= nb; m0 += GRP) {
+      LevelIn<NF> L[GRP];
+#pragma unroll
+      for (int j = 0; j < GRP; j++) {
+        const int k = ktop + m0 + j;
+        load_level(A, fc, base, nk, k < nk ? k : nk - 1, L[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < GRP; j++) {
+        const int m = m0 + j;
+        if (m > nb) break;
+        const int k = ktop + m;
+        apN_k = apN; amN_k = amN; apO_k = apO; amO_k = amO; bd_k = bd_k1;
+        if (tvdv) {
+          apam(L[j].wn, z1, z0, s0, s1, s2, s3, nk, ktop, k + 1, dt, A.TVD, apN, amN);
+          apam(L[j].wo, o1, o0, s0, s1, s2, s3, nk, ktop, k + 1, dt, A.TVD, apO, amO);
+        } else {
+          apN = 0.5 * (L[j].wn + fabs(L[j].wn));
+          amN = 0.5 * (L[j].wn - fabs(L[j].wn));
+          apO = 0.5 * (L[j].wo + fabs(L[j].wo));
+          amO = 0.5 * (L[j].wo - fabs(L[j].wo));
+        }
+        bd_k1 = (k + 1 < nk) ? bd() : 0.0;
+        const double sk = s1, skm = s0, skp = s2;
+        double a = td * amN_k;
+        double b = z0 + td * (apN_k - amN);
+        double c = -td * apN;
+        double d = o0 * sk;
+        if (m == nb) {
+          b += c;
+          a -= td * bd_k;
+          b += td * (bd_k + 2 * A.alpha_bot * bd_k);
+          d -= otd * (amO_k * skm + apO_k * sk) - otd * (bd_k * skm - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
+        } else {
+          a -= td * bd_k;
+          b += td * (bd_k + bd_k1);
+          c -= td * bd_k1;
+          d -= otd * (amO_k * skm + (apO_k - amO) * sk - apO * skp) -
+               otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
+        }
+        const double cnk = -hflux_in(A, fc, nfs, k, L[j]);
+        d += cnk;
+        A.Cn[base + (int64_t)k * 64] = cnk;
+        b -= a * cprev / bprev;
+        d -= a * dprev / bprev;
+        const int64_t idx = base + (int64_t)k * 64;
+        A.tb[idx] = b;
+        A.tc[idx] = c;
+        A.td[idx] = d;
+        bprev = b; cprev = c; dprev = d;
+        s0 = s1; s1 = s2; s2 = s3; s3 = L[j].sn;
+        z0 = z1; z1 = L[j].zn;
+        o0 = o1; o1 = L[j].on;
+        q0 = q1; q1 = L[j].qn;
+      }
+    }
+  }
   // Cn: zero above and at ktop, the folded top value on [ctop, ktop]
   for (int k = 0; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = 0;
   for (int k = ct; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = cn0 / (1 + abs(ct - ktop));
@@ -370,11 +492,32 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   if (nb > 0) {
     double un = A.td[base + (int64_t)(nk - 1) * 64] / A.tb[base + (int64_t)(nk - 1) * 64];
     A.s[base + (int64_t)(nk - 1) * 64] = un;
-    for (int k = nk - 2; k >= ktop; k--) {
-      const int64_t idx = base + (int64_t)k * 64;
-      const double bk = A.tb[idx];
-      un = A.td[idx] / bk - A.tc[idx] * un / bk;
-      A.s[idx] = un;
+    if (GRP > 1) {
+      for (int k0 = nk - 2; k0 >= ktop; k0 -= GRP) {
+        double rb[GRP], rc[GRP], rd[GRP];
+#pragma unroll
+        for (int j = 0; j < GRP; j++) {
+          const int k = k0 - j >= ktop ? k0 - j : ktop;
+          const int64_t idx = base + (int64_t)k * 64;
+          rb[j] = A.tb[idx];
+          rc[j] = A.tc[idx];
+          rd[j] = A.td[idx];
+        }
+#pragma unroll
+        for (int j = 0; j < GRP; j++) {
+          const int k = k0 - j;
+          if (k < ktop) break;
+          un = rd[j] / rb[j] - rc[j] * un / rb[j];
+          A.s[base + (int64_t)k * 64] = un;
+        }
+      }
+    } else {
+      for (int k = nk - 2; k >= ktop; k--) {
+        const int64_t idx = base + (int64_t)k * 64;
+        const double bk = A.tb[idx];
+        un = A.td[idx] / bk - A.tc[idx] * un / bk;
+        A.s[idx] = un;
+      }
     }
   } else if (A.step > 1) {
     const double b0 = A.tb[base + (int64_t)ktop * 64], d0 = A.td[base + (int64_t)ktop * 64];
