@@ -183,7 +183,8 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD, 3 on
  * with 2 levels per load group, 4-6 the pipelined form; -1, the default: 2
  * from 4096 tiles up, else 0), "tvd_group" (levels per load group of the
- * scalar step's column update: 1, 2 (the default) or 4).  Only "tiles",
+ * scalar step's column update: 1, 2 (the default) or 4), "tvd_sums_group"
+ * (the same for the TVD inflow sums: 1, 2 or 4, the default).  Only "tiles",
  * "reduce", "rzid", "band" and "order" change the rounding of a dot product;
  * every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);

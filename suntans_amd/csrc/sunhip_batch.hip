@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void kb_tvd_sums(const BTMember *__restrict__ 
   for (int w = first_item(); w < n; w += stride) {
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
-    if (B.T.TVD) tvd_sums_tile<NF>(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
+    if (B.T.TVD) tvd_sums_tile<NF, 4>(B.G, B.T, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
 

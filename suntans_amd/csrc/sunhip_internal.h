@@ -131,6 +131,7 @@ struct sunhip_grid {
   int *d_b4perm = nullptr;          // depth-first dispatch order of 4-tile blocks (Geo)
   int tune_order = 1;               // 1: use it
   int tune_tvdgrp = 2;              // levels per load group of the scalar column update (1, 2, 4)
+  int tune_sumsgrp = 4;             // levels per load group of the TVD inflow sums (1, 2, 4)
   int *d_uerr = nullptr;             // UPredictor's exit conditions
   double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)
   // z-level windows of the pressure solve (ZReg): dz[k] (Nkmax + 16

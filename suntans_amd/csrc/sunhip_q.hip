@@ -1090,6 +1090,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     if (value != 1 && value != 2 && value != 4) { set_error("tvd_group must be 1, 2 or 4"); return -1; }
     g->tune_tvdgrp = value;
   }
+  else if (!strcmp(key, "tvd_sums_group")) {
+    if (value != 1 && value != 2 && value != 4) { set_error("tvd_sums_group must be 1, 2 or 4"); return -1; }
+    g->tune_sumsgrp = value;
+  }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {
     if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }

@@ -23,9 +23,9 @@
 using namespace sunhip;
 
 
-template <int NF>
+template <int NF, int GRP>
 __global__ __launch_bounds__(256) void k_tvd_sums(Geo G, TvdArgs A) {
-  tvd_sums_tile<NF>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+  tvd_sums_tile<NF, GRP>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
@@ -85,7 +85,11 @@ int face_scalars_launch(sunhip_grid *g, const TvdArgs &A) {
   const Geo G = g->geo();
   int ps = prof_begin(g, PK_TSUMS);
   with_nf(g->mf, [&](auto nf) {
-    k_tvd_sums<decltype(nf)::value><<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
+    constexpr int NF = decltype(nf)::value;
+    const unsigned nb = (unsigned)((g->Nt + 3) / 4);
+    if (g->tune_sumsgrp == 4) k_tvd_sums<NF, 4><<<nb, 256, 0, g->stream>>>(G, A);
+    else if (g->tune_sumsgrp == 2) k_tvd_sums<NF, 2><<<nb, 256, 0, g->stream>>>(G, A);
+    else k_tvd_sums<NF, 1><<<nb, 256, 0, g->stream>>>(G, A);
     return 0;
   });
   prof_end(g, ps);

@@ -42,7 +42,7 @@ struct TvdArgs {
 // pass 1 (main/tvd.c:46-65), cells of celldist[0..2).  The faces' level-
 // invariant data (edge base, df, normal, neighbour base and depth) are read
 // once per column.  NF: faces per cell at most.
-template <int NF>
+template <int NF, int GRP = 1>
 __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s];
@@ -66,22 +66,46 @@ __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, in
       eb[f] = 0; dfe[f] = 0; nrm[f] = 0; nkn[f] = 0; nbb[f] = base;
     }
   }
-  for (int k = G.ctop[s]; k < n; k++) {
-    const uint32_t k64 = (uint32_t)k * 64u, idx = base + k64;
-    const double si = at32(A.s, idx);
-    double sq = 0, sqc = 0;
+  // GRP levels at a time: every load of the group first (levels past the
+  // column clamped to its last, the neighbour's scalar to its own bottom),
+  // then the sums in face order as below
+  const int k1 = G.ctop[s];
+  for (int k0 = k1; k0 < n; k0 += GRP) {
+    double vs[GRP], vu[GRP][NF], vt[GRP][NF], vz[GRP][NF], vn[GRP][NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      if (f < nf) {
+    for (int j = 0; j < GRP; j++) {
+      const int k = k0 + j < n ? k0 + j : n - 1;
+      const uint32_t k64 = (uint32_t)k * 64u;
+      vs[j] = at32(A.s, base + k64);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
         const uint32_t ei = eb[f] + k64;
-        const double un = nrm[f] * (th * at32(A.u, ei) + (1 - th) * at32(A.utmp2, ei));
-        const double Qm = 0.5 * at32(A.dzf, ei) * dfe[f] * fabs(un - fabs(un));
-        if (k < nkn[f]) sqc += Qm * (si - at32(A.s, nbb[f] + k64));
-        sq += Qm;
+        vu[j][f] = at32(A.u, ei);
+        vt[j][f] = at32(A.utmp2, ei);
+        vz[j][f] = at32(A.dzf, ei);
+        const int kn = k < nkn[f] ? k : (nkn[f] > 0 ? nkn[f] - 1 : 0);
+        vn[j][f] = at32(A.s, nbb[f] + (uint32_t)kn * 64u);
       }
     }
-    at32(A.oQ, idx) = sq;
-    at32(A.oQC, idx) = sqc;
+#pragma unroll
+    for (int j = 0; j < GRP; j++) {
+      const int k = k0 + j;
+      if (k >= n) break;
+      const uint32_t idx = base + (uint32_t)k * 64u;
+      const double si = vs[j];
+      double sq = 0, sqc = 0;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        if (f < nf) {
+          const double un = nrm[f] * (th * vu[j][f] + (1 - th) * vt[j][f]);
+          const double Qm = 0.5 * vz[j][f] * dfe[f] * fabs(un - fabs(un));
+          if (k < nkn[f]) sqc += Qm * (si - vn[j][f]);
+          sq += Qm;
+        }
+      }
+      at32(A.oQ, idx) = sq;
+      at32(A.oQC, idx) = sqc;
+    }
   }
 }
 

@@ -89,13 +89,14 @@ def test_zreg_bitwise_heads(gpu_available, grids, head):
 
 
 def scalar_step(st, group, tvd):
-    """One UpdateScalars on the device with tvd_group = group; returns the
-    new scalar and Cn."""
+    """One UpdateScalars on the device with tvd_group and tvd_sums_group =
+    group; returns the new scalar and Cn."""
     dg = device.DeviceGrid(st)
     for f in ("s", "u", "utmp2", "dzf", "dzz", "dzzold", "kappa_tv", "wnew", "wtmp2"):
         dg.upload(f, st.get(f))
     dg.upload("active", st.active().astype(np.float64))
     dg.tune("tvd_group", group)
+    dg.tune("tvd_sums_group", group)
     dg.update_scalars(st.get_prop("theta"), st.get_prop("dt"), 1e-4, tvd, step=3)
     n = len(st.get("s"))
     out = dg.download("s", n), dg.download("Cn", n)
@@ -113,8 +114,8 @@ SCALAR_GRIDS = [
 @pytest.mark.parametrize("tvd", [0, 1, 3, 4])
 @pytest.mark.parametrize("name,make", SCALAR_GRIDS, ids=[g[0] for g in SCALAR_GRIDS])
 def test_tvd_group_bitwise(gpu_available, name, make, tvd):
-    """The scalar column update walked 1, 2 or 4 levels per load group gives
-    the same bits (same operations in the same order)."""
+    """The inflow sums and the column update walked 1, 2 or 4 levels per
+    load group give the same bits (same operations in the same order)."""
     st = make()
     ref = scalar_step(st, 1, tvd)
     for group in (2, 4):
