@@ -641,27 +641,33 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
   if (kb1 <= kb0) return 0;  // no computational column in the tile (uniform)
   const uint32_t base = (uint32_t)(G.toff[t] + l);
   const int nf = n > 0 ? G.nf[s] : 0;
-  int code[NF], kmn[NF], kmx[NF], lim[NF];
+  // per face: the exchange code, the face's level range [kmn, kmx) and (ZR)
+  // its window [f0, f1) packed as 16-bit pairs, unpacked where compared, so
+  // the per-face state fits the register budget
+  int code[NF], krp[NF], lim[NF], zwp[NF];
   uint32_t ebb[NF];
-  T Df[NF];
-  ZFace<T> zf[NF];
+  T Df[NF], ztop[NF], zbot[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     code[f] = f < nf ? G.xcode[f * NS + s] : -1;
     // the window kernel leaves faces without a neighbour (and face slots past
     // nf, and columns that are not computational) at an empty window
-    if (ZR) zf[f] = zface<T, NF>(V.Z, G, s, f);
+    if (ZR) {
+      const ZFace<T> z = zface<T, NF>(V.Z, G, s, f);
+      zwp[f] = z.f0 | (z.f1 << 16);
+      ztop[f] = z.top;
+      zbot[f] = z.bot;
+    }
     if (f < nf) {
-      const int kr = G.kr[f * NS + s];
-      kmn[f] = kr & 0xffff;
-      kmx[f] = kr >> 16;
+      krp[f] = G.kr[f * NS + s];
       ebb[f] = (uint32_t)G.eb[f * NS + s];
       Df[f] = G.nbb[f * NS + s] >= 0 ? V.D[G.es[f * NS + s]] : T(0);
     } else {
-      kmn[f] = 0; kmx[f] = 0; ebb[f] = 0; Df[f] = 0;
+      krp[f] = 0; ebb[f] = 0; Df[f] = 0;
     }
-    if (code[f] < 0) { kmn[f] = 0; kmx[f] = 0; }
-    lim[f] = kmx[f] > 0 ? kmx[f] - 1 : 0;
+    if (code[f] < 0) krp[f] = 0;
+    const int kmx = krp[f] >> 16;
+    lim[f] = kmx > 0 ? kmx - 1 : 0;
   }
   // every slot has its (possibly empty) window; plain locals, so they stay in registers
   int zc0 = 0, zc1 = 0;
@@ -751,9 +757,9 @@ __device__ __forceinline__ double qapply_xchg(const Geo &G, const QVec<T, T> &V,
           T y = 0;
 #pragma unroll
           for (int f = 0; f < NF; f++) {
-            const T dzk = ZR ? (k < zf[f].f0 ? zf[f].top : (k < zf[f].f1 ? dzg[j] : zf[f].bot)) : dz[j][f];
+            const T dzk = ZR ? (k < (zwp[f] & 0xffff) ? ztop[f] : (k < (zwp[f] >> 16) ? dzg[j] : zbot[f])) : dz[j][f];
             const T term = (nv[j][f] - v0) * dzk * Df[f];
-            y = (k >= kmn[f] && k < kmx[f]) ? y + term : y;
+            y = (k >= (krp[f] & 0xffff) && k < (krp[f] >> 16)) ? y + term : y;
           }
           if (kt == n - 1) y -= T(2.0) * c0 * v0;
           else if (k == kt) y += (-2 * c0 - cp1) * v0 + cp1 * vp1;
