@@ -106,7 +106,7 @@ int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
  * "stmp"...; w-type fields take Nk+1 values per column (names starting
  * with 'w'); edge fields: "dzf","u","utmp2","u0","SfHp","SfHm","bscal" (the
  * open-boundary scalar, Nke values per type-2 edge, zero elsewhere); 2-D
- * cell fields: "h","htmp","htmp0","hcoef","active"...; 2-D edge fields:
+ * cell fields: "h","htmp","htmp0","hcoef","active","Ftop","Fbot"...; 2-D edge fields:
  * "D","Dq","Dh".  Upload/download move between a flat host array and the
  * device layout. */
 int sunhip_upload(sunhip_grid *g, const char *name, const double *flat);
@@ -266,6 +266,15 @@ int sunhip_face_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const ch
 /* One UpdateScalars step of field scal with "wnew","wtmp2","dzz","dzzold",
  * "kappa_tv","u","utmp2","dzf"; writes scal and "Cn". */
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
+/* The same step with UpdateScalars' source terms (main/scalars.c:151-183,
+ * 213-242): src1 and src2 name cell fields (the implicit and explicit
+ * sources, main/phys.c:1207-1209 and 1241-1243 pass uold and wtmp), ftop and
+ * fbot 2-D cell fields ("Ftop", "Fbot"); NULL leaves a term out.  Where the
+ * surface rose (ctopold > ctop) the reference reads the scalar at a stale
+ * level index past the row in the src1 term and writes src2's Cn below the
+ * row; the device uses the top cell's scalar and leaves those writes out. */
+int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
+                              const char *src2, const char *ftop, const char *fbot);
 
 /* ---- the vertical grid that moves with the free surface -------------------
  * UpdateDZ (main/phys.c:816-968) and SetFluxHeight (:4338-4435) on the

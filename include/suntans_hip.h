@@ -12,10 +12,10 @@
  * reference clobbers as a side effect (src, stmp2, stmp3, uold, wtmp, qtmp,
  * a..d, hold, htmp*, hcoef, hfcoef, and CGSolveQ's c with qprecond=1) are
  * left untouched: the reference reads none of them afterwards.
- * Unsupported configurations (numprocs>1, UpdateScalars with
- * src1/src2/Ftop/Fbot) print an error and exit, as the reference does on
- * fatal conditions; nonconvergence is not an error (query it with
- * sunhip_last_solve).
+ * Unsupported configurations (numprocs>1, first-order upwind scalars with
+ * open boundaries, and the branches named at each function) print an error
+ * and exit, as the reference does on fatal conditions; nonconvergence is not
+ * an error (query it with sunhip_last_solve).
  */
 #ifndef SUNTANS_HIP_H
 #define SUNTANS_HIP_H
@@ -68,7 +68,12 @@ void SetFluxHeight(gridT *grid, physT *phys, propT *prop);
 /* main/tvd.h:27-28 (definition main/tvd.c:36) */
 void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, REAL **boundary_scal,
                            int TVD, MPI_Comm comm, int myproc);
-/* main/scalars.h:19-22 (definition main/scalars.c:35) */
+/* main/scalars.h:19-22 (definition main/scalars.c:35).  The source terms
+ * src1, src2, Ftop and Fbot are applied as the reference applies them; where
+ * the surface rose (ctopold > ctop) and src1 or src2 is given, the reference
+ * reads the scalar at a stale level index past the row and writes src2's Cn
+ * below the row (main/scalars.c:153-154, 213-214): there the top cell's scalar
+ * is used and those writes are left out. */
 void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **scal, REAL **boundary_scal,
                    REAL **Cn, REAL kappa, REAL kappaH, REAL **kappa_tv, REAL theta, REAL **src1,
                    REAL **src2, REAL *Ftop, REAL *Fbot, int alpha_top, int alpha_bot, MPI_Comm comm,

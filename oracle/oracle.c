@@ -982,10 +982,15 @@ void oracle_UpdateScalars(gridT *g, physT *p, propT *prop, REAL **wnew, REAL **s
       for (int k = ct + 1; k < nk; k++) d[k - ct] += (1 - fab) * Cn[i][k];
     }
     for (int k = 0; k < ct; k++) Cn[i][k] = 0;
-    if (src2)
-      for (int k = ct; k < nk; k++) Cn[i][k - ktop] = dt * src2[i][k] * g->dzzold[i][k];
-    else
+    /* with src2 the reference writes Cn[i][k-ktop] from k = ctop: below row 0
+     * when ctop < ktop (the surface rose), outside the row; those writes
+     * are left out here */
+    if (src2) {
+      for (int k = ct; k < nk; k++)
+        if (k >= ktop) Cn[i][k - ktop] = dt * src2[i][k] * g->dzzold[i][k];
+    } else {
       for (int k = ct; k < nk; k++) Cn[i][k] = 0;
+    }
 
     /* horizontal flux divergence with the TVD face values */
     for (int k = 0; k < nk; k++) ap[k] = 0;
@@ -1034,6 +1039,15 @@ void oracle_solve_h(gridT *g, physT *p, propT *prop) { oracle_CGSolve(g, p, prop
 void oracle_update_salinity(gridT *g, physT *p, propT *prop) {
   oracle_UpdateScalars(g, p, prop, p->wnew, p->s, p->boundary_s, p->Cn_R, prop->kappa_s, prop->kappa_sH,
                        p->kappa_tv, prop->theta, NULL, NULL, NULL, NULL, 0, 0, 1, prop->TVDsalt);
+}
+/* the salinity update with source terms; flags as ref_update_salinity_src in
+ * oracle/ref_harness.c (1 src1 = uold, 2 src2 = wtmp, 4 Ftop = htmp, 8 Fbot
+ * = htmp2) */
+void oracle_update_salinity_src(gridT *g, physT *p, propT *prop, int flags, int alpha_top, int alpha_bot) {
+  oracle_UpdateScalars(g, p, prop, p->wnew, p->s, p->boundary_s, p->Cn_R, prop->kappa_s, prop->kappa_sH,
+                       p->kappa_tv, prop->theta, (flags & 1) ? p->uold : NULL, (flags & 2) ? p->wtmp : NULL,
+                       (flags & 4) ? p->htmp : NULL, (flags & 8) ? p->htmp2 : NULL, alpha_top, alpha_bot, 1,
+                       prop->TVDsalt);
 }
 
 /* ------------------------------------------------------------------------ */

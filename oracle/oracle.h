@@ -66,6 +66,7 @@ void oracle_UpdateScalars(gridT *g, physT *p, propT *prop, REAL **wnew, REAL **s
 void oracle_solve_q(gridT *g, physT *p, propT *prop);
 void oracle_solve_h(gridT *g, physT *p, propT *prop);
 void oracle_update_salinity(gridT *g, physT *p, propT *prop);
+void oracle_update_salinity_src(gridT *g, physT *p, propT *prop, int flags, int alpha_top, int alpha_bot);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,14 @@ EXPORT void ref_update_salinity(gridT *g, physT *p, propT *pr) {
   UpdateScalars(g, p, pr, p->wnew, p->s, p->boundary_s, p->Cn_R, pr->kappa_s, pr->kappa_sH, p->kappa_tv,
                 pr->theta, NULL, NULL, NULL, NULL, 0, 0, 0, 0, 1, pr->TVDsalt);
 }
+/* The salinity update with the source terms of the reference's own call
+ * sites (main/phys.c:1207-1209, 1241-1243: src1 = uold, src2 = wtmp) and the
+ * flux arguments: flags bit 1 src1, 2 src2, 4 Ftop = htmp, 8 Fbot = htmp2. */
+EXPORT void ref_update_salinity_src(gridT *g, physT *p, propT *pr, int flags, int alpha_top, int alpha_bot) {
+  UpdateScalars(g, p, pr, p->wnew, p->s, p->boundary_s, p->Cn_R, pr->kappa_s, pr->kappa_sH, p->kappa_tv,
+                pr->theta, (flags & 1) ? p->uold : NULL, (flags & 2) ? p->wtmp : NULL, (flags & 4) ? p->htmp : NULL,
+                (flags & 8) ? p->htmp2 : NULL, alpha_top, alpha_bot, 0, 0, 1, pr->TVDsalt);
+}
 EXPORT void ref_solve_q(gridT *g, physT *p, propT *pr) {
   CGSolveQ(p->qc, p->stmp, p->stmp3, g, p, pr, 0, 1, 0);
 }

@@ -590,8 +590,6 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
                    int checkflag, int TVDscheme) {
   static const char *W = "UpdateScalars";
   (void)kappaH; (void)comm; (void)myproc; (void)checkflag;
-  if (src1 || src2 || Ftop || Fbot)
-    fatal(W, "source terms (src1, src2, Ftop, Fbot) are not built for the device");
   if (!TVDscheme) {
     for (int jp = grid->edgedist[2]; jp < grid->edgedist[5]; jp++)
       if (boundary_scal) fatal(W, "first-order upwind with open boundaries is not built for the device");
@@ -609,8 +607,15 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   double *act = next_buf(C, W);
   for (int i = 0; i < grid->Nc; i++) act[i] = phys->active[i] ? 1.0 : 0.0;
   check(sunhip_upload(C->dev, "active", act), W);
+  /* source terms: cell rows (Nk values each; src2 may be w-type rows, of
+   * which the step reads Nk) and the two per-column fluxes */
+  if (src1) upload_cells(C, grid, "src1", src1, 0, W);
+  if (src2) upload_cells(C, grid, "src2", src2, 0, W);
+  if (Ftop) check(sunhip_upload(C->dev, "Ftop", Ftop), W);
+  if (Fbot) check(sunhip_upload(C->dev, "Fbot", Fbot), W);
   sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
-  check(sunhip_update_scalars(C->dev, &S, "s"), W);
+  check(sunhip_update_scalars_src(C->dev, &S, "s", src1 ? "src1" : NULL, src2 ? "src2" : NULL,
+                                  Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL), W);
   download(C, "s", W);
   unpack_cells(C, grid, scal, 0, 1);
   /* Cn: the computational columns, zero in the others */

@@ -39,7 +39,7 @@ Kind kind_of(const char *name) {
   static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB", "dzfB", "def1"};
   static const char *cellw[] = {"Cn_W", "Cn_W2"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",
-                                "hx", "hy", "active", "htmp0", "dv", "hcorr", "dzbot"};
+                                "hx", "hy", "active", "htmp0", "dv", "hcorr", "dzbot", "Ftop", "Fbot"};
   if (!strncmp(name, "up_", 3)) return EDGE;  // UPredictor's per-edge scratch
   if (!strcmp(name, "wc")) return CELL;       // cell-centred w: Nk values per column
   for (auto n : cellw) if (!strcmp(name, n)) return CELLW;

@@ -32,9 +32,10 @@ __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
   tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-template <int NF, int GRP>
+template <int NF, int GRP, bool SRC = false>
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  update_column_tile<NF, GRP>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+  update_column_tile<NF, GRP, SRC>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6),
+                                   threadIdx.x & 63);
 }
 
 namespace sunhip {
@@ -105,11 +106,30 @@ int face_scalars_launch(sunhip_grid *g, const TvdArgs &A) {
 extern "C" {
 
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
+  return sunhip_update_scalars_src(g, sp, scal, nullptr, nullptr, nullptr, nullptr);
+}
+
+int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
+                              const char *src2, const char *ftop, const char *fbot) {
   HIPCHK(hipSetDevice(g->device));
   TvdArgs T;
   ColArgs A;
   int rc = tvd_args(g, sp, scal, &T, &A);
   if (rc) return rc;
+  // the source fields: cell fields for src1/src2, 2-D cell fields for the fluxes
+  const char *names[4] = {src1, src2, ftop, fbot};
+  const double **dst[4] = {&A.src1, &A.src2, &A.Ftop, &A.Fbot};
+  for (int i = 0; i < 4; i++) {
+    if (!names[i]) continue;
+    const Kind want = i < 2 ? CELL : CELL2;
+    if (kind_of(names[i]) != want) {
+      set_error("sunhip_update_scalars_src: %s must name a %s field", names[i], i < 2 ? "cell" : "2-D cell");
+      return -1;
+    }
+    Field *f;
+    if ((rc = field(g, names[i], &f))) return rc;
+    *dst[i] = f->d;
+  }
   if (sp->TVD && (rc = sunhip_face_scalars(g, sp, scal))) return rc;
   if (sp->TVD) {
     // a column reads old values of its own cells only, all before it writes
@@ -124,7 +144,9 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
   with_nf(g->mf, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
     const unsigned nb = (unsigned)((g->Nt + 3) / 4);
-    if (g->tune_tvdgrp == 4) k_update_column<NF, 4><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    if (A.src1 || A.src2 || A.Ftop || A.Fbot)
+      k_update_column<NF, 1, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);  // sources: a variant of its own
+    else if (g->tune_tvdgrp == 4) k_update_column<NF, 4><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (g->tune_tvdgrp == 2) k_update_column<NF, 2><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else k_update_column<NF, 1><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     return 0;

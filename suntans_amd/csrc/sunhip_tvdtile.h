@@ -154,6 +154,10 @@ struct ColArgs {
   const double *active;                  // 2-D cell field (0/1)
   double *Cn;
   double *tb, *tc, *td;                  // Thomas scratch (cell fields)
+  // source terms (null: absent): src1 (implicit, per cell-layer), src2
+  // (explicit, per cell-layer), Ftop/Fbot (surface and bottom fluxes, per
+  // column), main/scalars.c:151-183, 213-242
+  const double *src1, *src2, *Ftop, *Fbot;
   double theta, dt, kappa;
   int TVD, step, alpha_top, alpha_bot;
 };
@@ -272,11 +276,11 @@ __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF],
 // either SfHp and SfHm (TVD) or the old scalar of its two cells.
 template <int NF>
 struct LevelIn {
-  double wn, wo, sn, zn, on, qn;
+  double wn, wo, sn, zn, on, qn, g1, g2;  // g1, g2: src1 and src2 at k
   double fu[NF], ft[NF], fz[NF], fa[NF], fb[NF];
 };
 
-template <int NF>
+template <int NF, bool SRC>
 __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[NF], int64_t base, int nk, int k,
                                            LevelIn<NF> &L) {
   const int kw = k < nk ? k : nk - 1;  // w-type rows hold nk + 1 levels
@@ -290,6 +294,8 @@ __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[N
   L.zn = cl(A.dzz, k + 2);
   L.on = cl(A.dzzold, k + 2);
   L.qn = cl(A.kappa_tv, k + 2);
+  L.g1 = SRC && A.src1 ? A.src1[base + (int64_t)k * 64] : 0.0;
+  L.g2 = SRC && A.src2 ? A.src2[base + (int64_t)k * 64] : 0.0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     const int ne = fc[f].nke;
@@ -326,8 +332,9 @@ __device__ __forceinline__ double hflux_in(const ColArgs &A, const FaceC (&fc)[N
 // One column of UpdateScalars.  GRP > 1: the levels below the top row are
 // walked GRP at a time, every load of a group issued before its arithmetic,
 // and the back substitution reads its rows a group at a time; the values
-// are those of GRP = 1 bit for bit.
-template <int NF, int GRP = 1>
+// are those of GRP = 1 bit for bit.  SRC: the source terms of ColArgs are
+// read (without it they are taken as absent, and cost no registers).
+template <int NF, int GRP = 1, bool SRC = false>
 __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, nk = G.nk[s];
@@ -370,8 +377,12 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   // vertical diffusion coefficient of the face above level k+1
   auto bd = [&]() { return (2.0 * A.kappa + q0 + q1) / (z0 + z1); };
   const double td = theta * dt, otd = (1 - theta) * dt;
-  // horizontal source of the top row: levels 0..ktop fold into row 0
-  double cn0 = 0;
+  const bool hs1 = SRC && A.src1 != nullptr, hs2 = SRC && A.src2 != nullptr;
+  const bool hft = SRC && A.Ftop != nullptr, hfb = SRC && A.Fbot != nullptr;
+  auto S1 = [&](int k) { return A.src1[base + (int64_t)k * 64]; };
+  // horizontal source of the top row: levels 0..ktop fold into row 0, after
+  // the top cell's explicit source dt*src2*dzzold
+  double cn0 = hs2 ? dt * A.src2[base + (int64_t)ktop * 64] * dzo.at(ktop) : 0.0;
   for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux(A, fc, nfs, k);
 
   // forward: assemble row m (level k = ktop+m) and eliminate
@@ -398,6 +409,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
       b = dznew - td * amN;
       c = -td * apN;
       if (nb == 0) b += c;
+      if (hs1) b += z0 * S1(ktop) * theta * dt;
       if (ktop < nk - 1) {
         b += td * (bd_k1 + 2 * A.alpha_top * bd_k1);
         c -= td * bd_k1;
@@ -405,11 +417,17 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
       if (cto <= ct) {
         d = 0;
         for (int kk = cto; kk <= ct; kk++) d += dzo.at(kk) * so.at(kk);
+        if (hs1)
+          for (int kk = cto; kk <= ct; kk++) d -= S1(kk) * (1 - theta) * dt * dzo.at(kk) * so.at(kk);
       } else {
         d = o0 * sk;
+        // the reference multiplies by the scalar at a stale level index past
+        // the row here (main/scalars.c:153-154); the top cell's, as the oracle
+        if (hs1) d -= S1(ktop) * (1 - theta) * dt * o0 * sk;
       }
       if (ktop < nk - 1) {
         d = d - otd * (-amO * sk - apO * skp) + otd * (-(2 * A.alpha_top * bd_k1 + bd_k1) * sk + bd_k1 * skp);
+        if (hft) d += dt * (1 - A.alpha_top + 2 * A.alpha_top * bd_k1) * A.Ftop[s];
       }
       d += cn0;
     } else {
@@ -417,20 +435,25 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
       b = z0 + td * (apN_k - amN);
       c = -td * apN;
       d = o0 * sk;
+      if (hs1) d -= S1(k) * (1 - theta) * dt * o0 * sk;
       if (m == nb) {
         b += c;
+        if (hs1) b += z0 * S1(k) * theta * dt;
         a -= td * bd_k;
         b += td * (bd_k + 2 * A.alpha_bot * bd_k);
         d -= otd * (amO_k * skm + apO_k * sk) - otd * (bd_k * skm - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
+        if (hfb) d += dt * (-1 + A.alpha_bot + 2 * A.alpha_bot * bd_k) * A.Fbot[s];
       } else {
         a -= td * bd_k;
         b += td * (bd_k + bd_k1);
         c -= td * bd_k1;
+        if (hs1) b += z0 * S1(k) * theta * dt;
         d -= otd * (amO_k * skm + (apO_k - amO) * sk - apO * skp) -
              otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
       }
-      // Cn[m] = 0 - ap[k]; d[m] += fab*Cn[m] with fab = 1
-      const double cnk = -hflux(A, fc, nfs, k);
+      // Cn[m] = dt*src2*dzzold (0 without src2) - ap[k]; d[m] += fab*Cn[m] with fab = 1
+      const double cnk = hs2 ? dt * A.src2[base + (int64_t)k * 64] * o0 - hflux(A, fc, nfs, k)
+                             : -hflux(A, fc, nfs, k);
       d += cnk;
       A.Cn[base + (int64_t)k * 64] = cnk;
     }
@@ -457,7 +480,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
 #pragma unroll
       for (int j = 0; j < GRP; j++) {
         const int k = ktop + m0 + j;
-        load_level(A, fc, base, nk, k < nk ? k : nk - 1, L[j]);
+        load_level<NF, SRC>(A, fc, base, nk, k < nk ? k : nk - 1, L[j]);
       }
 #pragma unroll
       for (int j = 0; j < GRP; j++) {
@@ -480,19 +503,24 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
         double b = z0 + td * (apN_k - amN);
         double c = -td * apN;
         double d = o0 * sk;
+        if (hs1) d -= L[j].g1 * (1 - theta) * dt * o0 * sk;
         if (m == nb) {
           b += c;
+          if (hs1) b += z0 * L[j].g1 * theta * dt;
           a -= td * bd_k;
           b += td * (bd_k + 2 * A.alpha_bot * bd_k);
           d -= otd * (amO_k * skm + apO_k * sk) - otd * (bd_k * skm - (bd_k + 2 * A.alpha_bot * bd_k) * sk);
+          if (hfb) d += dt * (-1 + A.alpha_bot + 2 * A.alpha_bot * bd_k) * A.Fbot[s];
         } else {
           a -= td * bd_k;
           b += td * (bd_k + bd_k1);
           c -= td * bd_k1;
+          if (hs1) b += z0 * L[j].g1 * theta * dt;
           d -= otd * (amO_k * skm + (apO_k - amO) * sk - apO * skp) -
                otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
         }
-        const double cnk = -hflux_in(A, fc, nfs, k, L[j]);
+        const double cnk = hs2 ? dt * L[j].g2 * o0 - hflux_in(A, fc, nfs, k, L[j])
+                               : -hflux_in(A, fc, nfs, k, L[j]);
         d += cnk;
         A.Cn[base + (int64_t)k * 64] = cnk;
         b -= a * cprev / bprev;

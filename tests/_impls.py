@@ -125,6 +125,13 @@ class _Cpu:
         getattr(self.L, self.pre + "update_salinity")(C.c_void_p(st.g), C.c_void_p(st.p),
                                                       C.c_void_p(st.prop))
 
+    def update_salinity_src(self, st, flags, alpha_top=0, alpha_bot=0):
+        """UpdateScalars on salinity with source terms: flags 1 src1 = uold,
+        2 src2 = wtmp, 4 Ftop = htmp, 8 Fbot = htmp2."""
+        f = getattr(self.L, self.pre + "update_salinity_src")
+        f.argtypes = [_vp] * 3 + [C.c_int] * 3
+        f(st.g, st.p, st.prop, int(flags), int(alpha_top), int(alpha_bot))
+
     def inner3(self, st, x, y):
         return self.fn("InnerProduct3")(st.ptr(x), st.ptr(y), st.g)
 
@@ -256,6 +263,16 @@ class DropinImpl:
         self.L.UpdateScalars(st.g, st.p, st.prop, st.ptr("wnew"), st.ptr("s"), st.ptr("boundary_s"),
                              st.ptr("Cn_R"), kappa, kappaH, st.ptr("kappa_tv"), theta, None, None, None, None,
                              0, 0, 0, 0, 1, tvd)
+
+    def update_salinity_src(self, st, flags, alpha_top=0, alpha_bot=0):
+        """As the CPU adapters: flags 1 src1 = uold, 2 src2 = wtmp, 4 Ftop =
+        htmp, 8 Fbot = htmp2."""
+        g = st.get_prop
+        self.L.UpdateScalars(st.g, st.p, st.prop, st.ptr("wnew"), st.ptr("s"), st.ptr("boundary_s"),
+                             st.ptr("Cn_R"), g("kappa_s"), g("kappa_sH"), st.ptr("kappa_tv"), g("theta"),
+                             st.ptr("uold") if flags & 1 else None, st.ptr("wtmp") if flags & 2 else None,
+                             st.ptr("htmp") if flags & 4 else None, st.ptr("htmp2") if flags & 8 else None,
+                             int(alpha_top), int(alpha_bot), 0, 0, 1, int(g("TVDsalt")))
 
     def stats(self, which):
         n = C.c_int()

@@ -198,3 +198,40 @@ def test_update_scalars_bitwise(gpu_available, name, make, tvd):
     sa, sb = a.get("s"), b.get("s")
     assert np.array_equal(sa, sb), f"max diff {np.abs(sa - sb).max()}"
     assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))
+
+
+SRC_GRIDS = [("tiny0", lambda: cases.tiny(0)), ("tiny_wide", lambda: cases.tiny(11, nx=23, ny=9, Nkmax=13)),
+             ("lockexchange", _lock),
+             ("hexquad7", lambda: cases.tiny(24, nx=14, ny=9, Nkmax=11, planform="hex", split_every=4, maxfaces=7))]
+
+
+@pytest.mark.parametrize("name,make", SRC_GRIDS, ids=[g[0] for g in SRC_GRIDS])
+@pytest.mark.parametrize("flags", [1, 2, 3, 12, 15])
+@pytest.mark.parametrize("tvd", [0, 3, 4])
+@pytest.mark.parametrize("rose", [False, True], ids=["ctop_kept", "surface_rose"])
+def test_update_scalars_sources_bitwise(gpu_available, name, make, flags, tvd, rose):
+    """The drop-in UpdateScalars with its source terms (src1 = uold, src2 =
+    wtmp, Ftop = htmp, Fbot = htmp2, alpha_top = alpha_bot = 1) bit for bit
+    against the oracle.  ctop_kept: ctopold lowered to ctop, where the
+    oracle is pinned to the reference (test_oracle_vs_ref.py); surface_rose:
+    the state as built, with columns whose surface rose, where both follow
+    the choice stated in include/suntans_hip.h."""
+    o, d = _impls.oracle(), _impls.dropin()
+    a = make()
+    rng = np.random.default_rng(41)
+    a.set("uold", rng.normal(0, 1e-4, a.get("uold").shape))
+    a.set("wtmp", rng.normal(0, 1e-3, a.get("wtmp").shape))
+    a.set("htmp", rng.normal(0, 1e-2, a.Nc))
+    a.set("htmp2", rng.normal(0, 1e-2, a.Nc))
+    if not rose:
+        a.set_ints("ctopold", np.minimum(a.ints("ctopold"), a.ints("ctop")))
+    elif not (a.ints("ctopold") > a.ints("ctop")).any():
+        pytest.skip("no column of this state has a risen surface")
+    b = a.clone()
+    for st in (a, b):
+        st.set_prop(TVDsalt=tvd, n=2)
+    o.update_salinity_src(a, flags, 1, 1)
+    d.update_salinity_src(b, flags, 1, 1)
+    sa, sb = a.get("s"), b.get("s")
+    assert np.array_equal(sa, sb), f"max diff {np.abs(sa - sb).max()}"
+    assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))

@@ -294,3 +294,43 @@ def test_momentum_face_values(name, make, scheme):
     ref.momentum_faces(b)
     for f in ("ut", "SfHp", "SfHm"):
         assert np.array_equal(a.get(f).view(np.int64), b.get(f).view(np.int64)), f
+
+
+def source_inputs(st, seed):
+    """Source terms of UpdateScalars on a state: src1 (uold, a decay rate),
+    src2 (wtmp, a volume source), Ftop (htmp) and Fbot (htmp2), seeded.
+    ctopold is lowered to ctop where it lay below it: where the surface rose
+    (ctopold > ctop) the reference's src1 branch reads its scalar at a stale
+    level index past the row and its src2 branch writes Cn below row 0
+    (main/scalars.c:153-154, 213-214), so those cells have no defined
+    reference result."""
+    rng = np.random.default_rng(seed)
+    st.set("uold", rng.normal(0, 1e-4, st.get("uold").shape))
+    st.set("wtmp", rng.normal(0, 1e-3, st.get("wtmp").shape))
+    st.set("htmp", rng.normal(0, 1e-2, st.Nc))
+    st.set("htmp2", rng.normal(0, 1e-2, st.Nc))
+    st.set_ints("ctopold", np.minimum(st.ints("ctopold"), st.ints("ctop")))
+
+
+SRC_MAKERS = [m for m in MAKERS if m[0] in ("tiny0", "tiny1", "tiny2", "tiny_wide", "lockexchange", "quadtri",
+                                            "hexquad7")]
+
+
+@pytest.mark.parametrize("name,make", SRC_MAKERS, ids=[m[0] for m in SRC_MAKERS])
+@pytest.mark.parametrize("flags", [1, 2, 12, 15])
+@pytest.mark.parametrize("alpha", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("tvd", [0, 4])
+def test_update_scalars_sources(name, make, flags, alpha, tvd):
+    """UpdateScalars with its source terms (src1 implicit, src2 explicit,
+    surface and bottom fluxes, main/scalars.c:151-183, 213-242) and the flux
+    boundary weights alpha_top/alpha_bot."""
+    o = _impls.oracle()
+    a = make()
+    source_inputs(a, 31)
+    b = a.clone()
+    for st in (a, b):
+        st.set_prop(TVDsalt=tvd, n=2)
+    o.update_salinity_src(a, flags, *alpha)
+    ref.update_salinity_src(b, flags, *alpha)
+    assert np.array_equal(a.get("s"), b.get("s"))
+    assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))
