@@ -272,9 +272,15 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
  * fbot 2-D cell fields ("Ftop", "Fbot"); NULL leaves a term out.  Where the
  * surface rose (ctopold > ctop) the reference reads the scalar at a stale
  * level index past the row in the src1 term and writes src2's Cn below the
- * row; the device uses the top cell's scalar and leaves those writes out. */
+ * row; the device uses the top cell's scalar and leaves those writes out.
+ * bcell (first-order upwind, TVD 0, with open boundaries) names a cell field
+ * holding the rows the reference builds in stmp2 (main/scalars.c:84-96: the
+ * boundary values in the rows of the cells next to edges of edgedist[2..5),
+ * the last such edge's where a cell has several); faces of type-2 and type-3
+ * edges read their outside value there (:260-263).  NULL: they read the
+ * cell's own value, as the reference does without boundary_scal. */
 int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
-                              const char *src2, const char *ftop, const char *fbot);
+                              const char *src2, const char *ftop, const char *fbot, const char *bcell);
 
 /* ---- the vertical grid that moves with the free surface -------------------
  * UpdateDZ (main/phys.c:816-968) and SetFluxHeight (:4338-4435) on the

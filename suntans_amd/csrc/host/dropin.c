@@ -590,10 +590,6 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
                    int checkflag, int TVDscheme) {
   static const char *W = "UpdateScalars";
   (void)kappaH; (void)comm; (void)myproc; (void)checkflag;
-  if (!TVDscheme) {
-    for (int jp = grid->edgedist[2]; jp < grid->edgedist[5]; jp++)
-      if (boundary_scal) fatal(W, "first-order upwind with open boundaries is not built for the device");
-  }
   prop->TVD = TVDscheme;
   prop->horiTVD = 1;
   prop->vertTVD = 1;
@@ -613,9 +609,23 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   if (src2) upload_cells(C, grid, "src2", src2, 0, W);
   if (Ftop) check(sunhip_upload(C->dev, "Ftop", Ftop), W);
   if (Fbot) check(sunhip_upload(C->dev, "Fbot", Fbot), W);
+  /* first-order upwind with open boundaries: the rows the reference builds
+   * in stmp2 (main/scalars.c:76-96), zero in the computational columns, then
+   * each open-boundary edge's values in its cell's rows, in edge order */
+  const int nob = grid->edgedist[5] - grid->edgedist[2];
+  const int bcell = !TVDscheme && boundary_scal && nob > 0;
+  if (bcell) {
+    double *b = next_buf(C, W);
+    memset(b, 0, C->cst[grid->Nc] * sizeof(double));
+    for (int jp = grid->edgedist[2]; jp < grid->edgedist[5]; jp++) {
+      const int ib = grid->grad[2 * grid->edgep[jp]];
+      for (int k = grid->ctop[ib]; k < grid->Nk[ib]; k++) b[C->cst[ib] + k] = boundary_scal[jp - grid->edgedist[2]][k];
+    }
+    check(sunhip_upload(C->dev, "bcell", b), W);
+  }
   sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
   check(sunhip_update_scalars_src(C->dev, &S, "s", src1 ? "src1" : NULL, src2 ? "src2" : NULL,
-                                  Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL), W);
+                                  Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL, bcell ? "bcell" : NULL), W);
   download(C, "s", W);
   unpack_cells(C, grid, scal, 0, 1);
   /* Cn: the computational columns, zero in the others */

@@ -106,24 +106,24 @@ int face_scalars_launch(sunhip_grid *g, const TvdArgs &A) {
 extern "C" {
 
 int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
-  return sunhip_update_scalars_src(g, sp, scal, nullptr, nullptr, nullptr, nullptr);
+  return sunhip_update_scalars_src(g, sp, scal, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
-                              const char *src2, const char *ftop, const char *fbot) {
+                              const char *src2, const char *ftop, const char *fbot, const char *bcell) {
   HIPCHK(hipSetDevice(g->device));
   TvdArgs T;
   ColArgs A;
   int rc = tvd_args(g, sp, scal, &T, &A);
   if (rc) return rc;
   // the source fields: cell fields for src1/src2, 2-D cell fields for the fluxes
-  const char *names[4] = {src1, src2, ftop, fbot};
-  const double **dst[4] = {&A.src1, &A.src2, &A.Ftop, &A.Fbot};
-  for (int i = 0; i < 4; i++) {
+  const char *names[5] = {src1, src2, ftop, fbot, bcell};
+  const double **dst[5] = {&A.src1, &A.src2, &A.Ftop, &A.Fbot, &A.bcell};
+  for (int i = 0; i < 5; i++) {
     if (!names[i]) continue;
-    const Kind want = i < 2 ? CELL : CELL2;
+    const Kind want = (i < 2 || i == 4) ? CELL : CELL2;
     if (kind_of(names[i]) != want) {
-      set_error("sunhip_update_scalars_src: %s must name a %s field", names[i], i < 2 ? "cell" : "2-D cell");
+      set_error("sunhip_update_scalars_src: %s must name a %s field", names[i], want == CELL ? "cell" : "2-D cell");
       return -1;
     }
     Field *f;

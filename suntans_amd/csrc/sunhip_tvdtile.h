@@ -158,6 +158,11 @@ struct ColArgs {
   // (explicit, per cell-layer), Ftop/Fbot (surface and bottom fluxes, per
   // column), main/scalars.c:151-183, 213-242
   const double *src1, *src2, *Ftop, *Fbot;
+  // first-order upwind only: the reference's stmp2 rows (the boundary values
+  // in the rows of cells next to open-boundary edges, main/scalars.c:84-96),
+  // read on faces of type-2 and type-3 edges without a second cell; null:
+  // those faces read the cell's own old value, as without boundary_scal
+  const double *bcell;
   double theta, dt, kappa;
   int TVD, step, alpha_top, alpha_bot;
 };
@@ -219,7 +224,9 @@ __device__ __forceinline__ double upwind_ref(double u, double a, double b) {
 // A face's level-invariant part of the horizontal flux of main/scalars.c:
 // 251-290: the edge's layer count and base, the upwind cell bases, and
 // cf = dt*df*normal/Ac, the leading factors of the flux in the reference's
-// order (so the product below rounds as the reference's does).
+// order (so the product below rounds as the reference's does).  c2 < 0 marks
+// an open-boundary face whose second value comes from ColArgs::bcell at
+// base -c2 - 1.
 struct FaceC {
   uint32_t eb;
   int nke;
@@ -238,7 +245,10 @@ __device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int 
       fc[f].nke = G.e_nke[es];
       int64_t c1 = G.e_c1[es], c2 = G.e_c2[es];
       if (c1 < 0) c1 = c2;
-      if (c2 < 0) c2 = c1;
+      if (c2 < 0) {
+        const int mk = G.e_mark[es];
+        c2 = A.bcell && (mk == 2 || mk == 3) ? -c1 - 1 : c1;
+      }
       fc[f].c1 = c1;
       fc[f].c2 = c2;
       fc[f].cf = A.dt * G.df[es] * G.nrm[f * NS + s] / Ac;
@@ -262,7 +272,9 @@ __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF],
     if (A.TVD) {
       temp = ut2 > 0 ? at32(A.SfHp, ei) : at32(A.SfHm, ei);
     } else {
-      temp = upwind_ref(ut2, A.s_old[fc[f].c1 + (int64_t)k * 64], A.s_old[fc[f].c2 + (int64_t)k * 64]);
+      const int64_t c2 = fc[f].c2;
+      const double sp = c2 >= 0 ? A.s_old[c2 + (int64_t)k * 64] : A.bcell[-c2 - 1 + (int64_t)k * 64];
+      temp = upwind_ref(ut2, A.s_old[fc[f].c1 + (int64_t)k * 64], sp);
     }
     acc += fc[f].cf * (A.theta * at32(A.u, ei) + (1 - A.theta) * ut2) * temp * at32(A.dzf, ei);
   }
@@ -308,8 +320,10 @@ __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[N
       L.fa[f] = at32(A.SfHp, ei);
       L.fb[f] = at32(A.SfHm, ei);
     } else {
+      const int64_t c2 = fc[f].c2;
+      const double *pb = c2 >= 0 ? A.s_old : A.bcell;
       L.fa[f] = A.s_old[fc[f].c1 + (int64_t)ke * 64];
-      L.fb[f] = A.s_old[fc[f].c2 + (int64_t)ke * 64];
+      L.fb[f] = pb[(c2 >= 0 ? c2 : -c2 - 1) + (int64_t)ke * 64];
     }
   }
 }

@@ -127,7 +127,7 @@ def lib() -> C.CDLL:
         L.sunhip_copy_probe.argtypes = [vp, C.c_int64, C.c_int, _dp]
         L.sunhip_pcie_probe.argtypes = [vp, C.c_int64, C.c_int, _dp, _dp]
         L.sunhip_update_scalars_src.argtypes = [vp, vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
-                                                C.c_char_p]
+                                                C.c_char_p, C.c_char_p]
         L.sunhip_q_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_int,
                                            C.POINTER(CGStats), _ip]
         L.sunhip_h_solve_mixed.argtypes = [vp, C.POINTER(CGParams), C.c_double, C.c_double, C.c_double,
@@ -330,14 +330,14 @@ class DeviceGrid:
         return d
 
     def update_scalars(self, theta, dt, kappa, TVD, step=2, scal="s", src1=None, src2=None, ftop=None, fbot=None,
-                       alpha_top=0, alpha_bot=0):
+                       alpha_top=0, alpha_bot=0, bcell=None):
         """One UpdateScalars step of field scal; src1/src2 (cell fields) and
         ftop/fbot (2-D cell fields) name the source terms, None leaves one
-        out."""
+        out; bcell names the open-boundary rows of first-order upwind."""
         S = ScalarParams(theta, dt, kappa, TVD, step, alpha_top, alpha_bot)
         enc = lambda n: None if n is None else n.encode()
         check(lib().sunhip_update_scalars_src(self.h, C.byref(S), scal.encode(), enc(src1), enc(src2), enc(ftop),
-                                              enc(fbot)), "sunhip_update_scalars")
+                                              enc(fbot), enc(bcell)), "sunhip_update_scalars")
 
     def _vgrid(self, st):
         self._dz = np.ascontiguousarray(st.get("dz"), np.float64)

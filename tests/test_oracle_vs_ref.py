@@ -334,3 +334,29 @@ def test_update_scalars_sources(name, make, flags, alpha, tvd):
     ref.update_salinity_src(b, flags, *alpha)
     assert np.array_equal(a.get("s"), b.get("s"))
     assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))
+
+
+OPEN_SCAL = [("open_tri", lambda: cases.tiny(3, nx=8, ny=5, open_marks=OPEN)),
+             ("open_hex", lambda: cases.tiny(25, nx=7, ny=6, planform="hex", split_every=4, maxfaces=7,
+                                             open_marks=OPEN)),
+             ("open_lock", lambda: cases.lock_exchange(seed=8, nx=20, ny=5, Nkmax=10, open_marks=OPEN))]
+
+
+@pytest.mark.parametrize("name,make", OPEN_SCAL, ids=[m[0] for m in OPEN_SCAL])
+@pytest.mark.parametrize("tvd", [0, 1, 3, 4])
+def test_update_scalars_open(name, make, tvd):
+    """UpdateScalars on grids with open boundaries (type-2 and type-3 edges)
+    and seeded boundary values: first-order upwind reads them through the
+    rows it builds in stmp2 (main/scalars.c:76-96, 260-263), the TVD face
+    values through HorizontalFaceScalars."""
+    o = _impls.oracle()
+    a = make()
+    rng = np.random.default_rng(43)
+    a.set("boundary_s", rng.uniform(0, 1, a.get("boundary_s").shape))
+    b = a.clone()
+    for st in (a, b):
+        st.set_prop(TVDsalt=tvd, n=2)
+    o.update_salinity(a)
+    ref.update_salinity(b)
+    assert np.array_equal(a.get("s"), b.get("s"))
+    assert np.array_equal(a.get("Cn_R"), b.get("Cn_R"))
