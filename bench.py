@@ -78,6 +78,22 @@ def _checkers():
 
 
 def cpu_model() -> str:
+    """The host CPU as lscpu names it (model, family/model/stepping, sockets
+    x cores, maximum clock), else /proc/cpuinfo's model name."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        f = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        f = {k.strip(): v.strip() for k, v in f.items()}
+        parts = [f.get("Model name", "unknown"),
+                 f"family {f.get('CPU family', '?')} model {f.get('Model', '?')} stepping {f.get('Stepping', '?')}",
+                 f"{f.get('Socket(s)', '?')} socket(s) x {f.get('Core(s) per socket', '?')} cores x "
+                 f"{f.get('Thread(s) per core', '?')} threads"]
+        if "CPU max MHz" in f:
+            parts.append(f"max {f['CPU max MHz']} MHz")
+        return ", ".join(parts)
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -108,10 +124,12 @@ def _cpu_q_solve(impl, st, src0, maxiters, eps):
     return sec, impl.stats("q")[0], st.get("qc")
 
 
-def cpu_baseline(st, x64, x_fixed, gpu_iters: int):
+def cpu_baseline(st, x64, x_fixed, gpu_iters: int, x_h=None):
     """The reference's CGSolveQ on the full workload on one host core (from
     q = 0 to 1e-10), and the device's result against it: the converged q
-    (x64) and the q after exactly 1 and 3 PCG iterations (x_fixed)."""
+    (x64) and the q after exactly 1 and 3 PCG iterations (x_fixed).  Also
+    the free-surface CGSolve of the same grid (hprecond 1, D depth-
+    integrated, h = 0, 1e-10), against the device's h (x_h) when given."""
     impl, kind = _checkers()
     src0 = st.get("stmp")
     m = st.wet_mask()
@@ -125,8 +143,23 @@ def cpu_baseline(st, x64, x_fixed, gpu_iters: int):
     log(f"cpu {kind}: full solve {its} iterations in {sec:.1f}s; device q rel err {err:.2e}")
     st.set("stmp", src0)
     st.set("qc", 0.0)
+    from suntans_amd import cases
+    D0, h0 = st.get("D"), st.get("h")
+    st.set("D", cases.depth_integrated_D(st))
+    st.set("h", 0.0)
+    st.set_prop(hprecond=1, epsilon=1e-10, maxiters=1000, resnorm=0)
+    t = time.perf_counter()
+    impl.cgsolve(st)
+    hsec = time.perf_counter() - t
+    hits = impl.stats("h")[0]
+    herr = relerr(x_h, st.get("h")) if x_h is not None else None
+    log(f"cpu {kind}: CGSolve {hits} iterations in {hsec:.2f}s; device h rel err {herr}")
+    st.set("D", D0)
+    st.set("h", h0)
     return {
         "value": 1.0 / sec, "unit": "solves/s", "cores": 1, "kind": kind, "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "cgsolve_s": hsec, "cgsolve_iterations": hits, "rel_err_h_vs_reference": herr,
         "sample": f"one full CGSolveQ of the workload from q=0 to 1e-10 on one core ({its} iterations, {sec:.1f} s)",
         "s_per_solve": sec, "iterations": its, "device_iterations": gpu_iters,
         "rel_err_q_vs_reference": err,
@@ -740,6 +773,7 @@ def main():
     achieved = B[dom] / (dom_ms * 1e-3) / 1e9
     x64 = dg.download("q", nflat)
     traffic = traffic_for(KERNEL_NAMES[dom])
+    x_h = None
     # the device's q after exactly 1 and 3 iterations, for the comparison with the reference
     x_fixed = {}
     for n in (1, 3):
@@ -836,6 +870,7 @@ def main():
             return res, dg.download("h", st.Nc)
 
         h64, xh = hrun(dg.h_solve, maxiters=1000, precond=1, epsilon=1e-10)
+        x_h = xh
         ht, xht = hrun(dg.h_solve, n=1, maxiters=10000, precond=1, epsilon=1e-14)
         hm, xhm = hrun(dg.h_solve_mixed, epsilon=1e-10, inner_tol=args.inner_tol, precond=1)
         ms = lambda rs: float(np.median([s["setup_ms"] + s["device_ms"] for s in rs]))
@@ -878,7 +913,7 @@ def main():
             x.dg.close()
 
     if not args.skip_cpu:
-        cb = cpu_baseline(st, x64, x_fixed, iters[-1])
+        cb = cpu_baseline(st, x64, x_fixed, iters[-1], x_h)
         out["rel_err_vs_reference"] = {"converged_q": cb["rel_err_q_vs_reference"],
                                        **{f"q_after_{k}_iterations": v
                                           for k, v in cb["fixed_iterations_rel_err_q"].items()},
