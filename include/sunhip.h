@@ -214,9 +214,16 @@ int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
  *
  * sunhip_download_parts gathers the field and starts its copy into flat in
  * nparts (1..SUNHIP_MAXPARTS) equal parts without waiting;
- * sunhip_download_wait returns once values [0, upto) are on the host (upto
- * at or beyond the length: all of them).  Every download in parts must be
- * waited for in full before the next transfer of the grid. */
+ * sunhip_download_wait returns once values [0, upto) of the oldest download
+ * in flight are on the host (upto at or beyond the length: all of them, and
+ * the next one in flight becomes the oldest).  Two downloads in parts may be
+ * in flight, into different host buffers: the second's gather and copies
+ * then overlap the caller's work on the first.  Uploads and whole downloads
+ * need none in flight.
+ *
+ * The copies run on a stream of the grid's own, through two device staging
+ * buffers taken in turn, so one field's copy overlaps the scatter or gather
+ * kernel of the field before it. */
 #define SUNHIP_MAXPARTS 16
 int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int64_t off, int64_t n, int last);
 int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int nparts);

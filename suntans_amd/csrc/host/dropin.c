@@ -16,6 +16,9 @@
 #include <math.h>
 #include <pthread.h>
 #include <unistd.h>
+#ifdef __SSE2__
+#include <emmintrin.h>
+#endif
 
 #include "sunhip.h"
 #include "suntans_hip.h"
@@ -44,7 +47,8 @@ typedef struct {
   int slot;          /* the slot the next transfer takes */
   size_t *cst, *wst, *est;  /* flat offsets of cell (Nk), w-type (Nk+1) and edge (Nkc) rows */
   int *cell_rank, *edge_rank;  /* position of cell i in cellp, of edge j in edgep */
-  int parts;         /* parts of the download in progress */
+  struct { double *flat; int parts; } dl[2];  /* downloads in flight, oldest first */
+  int dl_head, dl_n;
   int *x_last;       /* the extents last sent (ctop, ctopold | Nke, etop), NULL: none */
 } ctx_t;
 
@@ -223,6 +227,26 @@ typedef struct {
   int lo, hi;              /* rows lo..hi-1 */
 } rows_job;
 
+/* Copy n doubles with streaming stores: the destination is written whole and
+ * not read again soon, so its lines need not be fetched first (on x86 a plain
+ * copy reads every destination line before writing it).  The caller ends a
+ * run of these with copy_fence. */
+static void copy_stream(double *d, const double *s, size_t n) {
+#ifdef __SSE2__
+  size_t i = 0;
+  if (((uintptr_t)d & 15) && n) { d[0] = s[0]; i = 1; }
+  for (; i + 2 <= n; i += 2) _mm_stream_pd(d + i, _mm_loadu_pd(s + i));
+  if (i < n) d[i] = s[i];
+#else
+  memcpy(d, s, n * sizeof(double));
+#endif
+}
+static void copy_fence(void) {
+#ifdef __SSE2__
+  _mm_sfence();
+#endif
+}
+
 static void rows_run(const rows_job *J, int lo, int hi) {
   for (int i = lo; i < hi; i++) {
     const size_t o = J->start[i], n = J->start[i + 1] - o;
@@ -234,16 +258,17 @@ static void rows_run(const rows_job *J, int lo, int hi) {
       }
     }
     switch (J->mode) {
-      case ROWS_PACK: memcpy(J->flat + o, J->rows[i], n * sizeof(double)); break;
-      case ROWS_UNPACK: case ROWS_UNPACK_ZERO: memcpy(J->rows[i], J->flat + o, n * sizeof(double)); break;
+      case ROWS_PACK: copy_stream(J->flat + o, J->rows[i], n); break;
+      case ROWS_UNPACK: case ROWS_UNPACK_ZERO: copy_stream(J->rows[i], J->flat + o, n); break;
       case ROWS_UNPACK_FROM: {
         const size_t k0 = (size_t)J->lvl[i];
-        if (k0 < n) memcpy(J->rows[i] + k0, J->flat + o + k0, (n - k0) * sizeof(double));
+        if (k0 < n) copy_stream(J->rows[i] + k0, J->flat + o + k0, n - k0);
         break;
       }
-      case ROWS_UNPACK_UPTO: memcpy(J->rows[i], J->flat + o, (size_t)J->lvl[i] * sizeof(double)); break;
+      case ROWS_UNPACK_UPTO: copy_stream(J->rows[i], J->flat + o, (size_t)J->lvl[i]); break;
     }
   }
+  copy_fence();  /* this thread's stores are visible before its share is reported done */
 }
 
 /* Threads for packing: SUNHIP_DROPIN_THREADS, else the online cores, at most
@@ -357,23 +382,34 @@ static void upload_edges(ctx_t *c, gridT *g, const char *name, REAL **rows, cons
   upload_rows(c, name, rows, c->est, g->Ne, where);
 }
 
-/* Start the download of field name into the next pinned slot (c->flat) in
- * parts; the unpack_* calls below wait for each part as they need it. */
+/* Start the download of field name into the next pinned slot in parts.
+ * Two may be in flight: the device gathers and copies the second while the
+ * host unpacks the first.  unpack_parts takes them in order. */
 static void download(ctx_t *c, const char *name, const char *where) {
+  if (c->dl_n == 2) fatal(where, "a third download in flight");
   double *flat = next_buf(c, where);
-  c->parts = n_parts(c->flat_n);
-  check(sunhip_download_parts(c->dev, name, flat, c->parts), where);
+  if (c->dl_n && c->dl[c->dl_head].flat == flat) fatal(where, "a download into a slot not yet unpacked");
+  const int parts = n_parts(c->flat_n);
+  check(sunhip_download_parts(c->dev, name, flat, parts), where);
+  const int i = (c->dl_head + c->dl_n) & 1;
+  c->dl[i].flat = flat;
+  c->dl[i].parts = parts;
+  c->dl_n++;
 }
-/* The whole of field name in c->flat. */
-static void download_all(ctx_t *c, const char *name, const char *where) {
+/* The whole of field name, in a pinned slot (valid until the slot's next use). */
+static double *download_all(ctx_t *c, const char *name, const char *where) {
+  if (c->dl_n) fatal(where, "a whole download with others in flight");
   download(c, name, where);
   check(sunhip_download_wait(c->dev, INT64_MAX), where);
+  c->dl_n = 0;
+  return c->dl[c->dl_head].flat;
 }
 
-/* Write J's rows back from the download in progress, part by part. */
+/* Write J's rows back from the oldest download in flight, part by part. */
 static void unpack_parts(ctx_t *c, rows_job J) {
-  J.flat = c->flat;
-  const int lo = J.lo, n = J.hi - J.lo, K = c->parts;
+  if (!c->dl_n) fatal("unpack", "no download in flight");
+  J.flat = c->dl[c->dl_head].flat;
+  const int lo = J.lo, n = J.hi - J.lo, K = c->dl[c->dl_head].parts;
   for (int p = 0; p < K; p++) {
     J.lo = lo + (int)((int64_t)n * p / K);
     J.hi = lo + (int)((int64_t)n * (p + 1) / K);
@@ -381,40 +417,67 @@ static void unpack_parts(ctx_t *c, rows_job J) {
     rows_par(&J, J.start[J.hi] - J.start[J.lo]);
   }
   check(sunhip_download_wait(c->dev, INT64_MAX), "unpack");
+  c->dl_head ^= 1;
+  c->dl_n--;
 }
 
-/* write back cells of cellp[celldist[lo]..celldist[hi]) */
-static void unpack_cells(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+/* What a download writes back: the rows of J from field name. */
+typedef struct {
+  const char *name;
+  rows_job J;
+} fetch_t;
+
+/* Download and write back F[0..n-1] with two in flight. */
+static void fetch(ctx_t *c, const fetch_t *F, int n, const char *where) {
+  int issued = 0;
+  for (int i = 0; i < n; i++) {
+    for (; issued < n && issued < i + 2; issued++) download(c, F[issued].name, where);
+    unpack_parts(c, F[i].J);
+  }
+}
+
+/* cells of cellp[celldist[lo]..celldist[hi]) */
+static rows_job cells_job(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
   rows_job J = {ROWS_UNPACK, NULL, rows, c->cst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL, 0,
                 g->Nc};
-  unpack_parts(c, J);
+  return J;
 }
-/* write back cells of cellp[celldist[lo]..celldist[hi]) and zero the other rows */
-static void unpack_cells_zero(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  rows_job J = {ROWS_UNPACK_ZERO, NULL, rows, c->cst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL,
-                0, g->Nc};
-  unpack_parts(c, J);
+/* ... and zero the other rows */
+static rows_job cells_zero_job(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  rows_job J = cells_job(c, g, rows, lo, hi);
+  J.mode = ROWS_UNPACK_ZERO;
+  return J;
 }
-/* write back rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
-static void unpack_cells_w(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
-  rows_job J = {ROWS_UNPACK, NULL, rows, c->wst, c->cell_rank, g->celldist[lo], g->celldist[hi], 0, 0, NULL, 0,
-                g->Nc};
-  unpack_parts(c, J);
+/* rows of w-type cells (Nk+1 values) of cellp[celldist[lo]..celldist[hi]) */
+static rows_job cells_w_job(ctx_t *c, const gridT *g, REAL **rows, int lo, int hi) {
+  rows_job J = cells_job(c, g, rows, lo, hi);
+  J.start = c->wst;
+  return J;
 }
-/* write back levels [ctop, Nk) of every cell */
-static void unpack_cells_wet(ctx_t *c, const gridT *g, REAL **rows) {
+/* levels [ctop, Nk) of every cell */
+static rows_job cells_wet_job(ctx_t *c, const gridT *g, REAL **rows) {
   rows_job J = {ROWS_UNPACK_FROM, NULL, rows, c->cst, NULL, 0, 0, 0, 0, g->ctop, 0, g->Nc};
-  unpack_parts(c, J);
+  return J;
 }
-/* write back every edge row */
-static void unpack_edges(ctx_t *c, const gridT *g, REAL **rows) {
+/* every edge row */
+static rows_job edges_job(ctx_t *c, const gridT *g, REAL **rows) {
   rows_job J = {ROWS_UNPACK, NULL, rows, c->est, NULL, 0, 0, 0, 0, NULL, 0, g->Ne};
-  unpack_parts(c, J);
+  return J;
 }
-/* write back levels [0, Nke) of every edge row */
-static void unpack_edges_wet(ctx_t *c, const gridT *g, REAL **rows) {
+/* levels [0, Nke) of every edge row */
+static rows_job edges_wet_job(ctx_t *c, const gridT *g, REAL **rows) {
   rows_job J = {ROWS_UNPACK_UPTO, NULL, rows, c->est, NULL, 0, 0, 0, 0, g->Nke, 0, g->Ne};
-  unpack_parts(c, J);
+  return J;
+}
+/* levels [0, Nke) of the edges of edgep[edgedist[0..1)) and edgep[edgedist[2..3)) */
+static rows_job face_values_job(ctx_t *c, const gridT *g, REAL **rows) {
+  rows_job J = {ROWS_UNPACK_UPTO, NULL, rows, c->est, c->edge_rank, g->edgedist[0], g->edgedist[1],
+                g->edgedist[2], g->edgedist[3], g->Nke, 0, g->Ne};
+  return J;
+}
+static void fetch1(ctx_t *c, const char *name, rows_job J, const char *where) {
+  fetch_t F = {name, J};
+  fetch(c, &F, 1, where);
 }
 
 /* qprecond=1: 1 when c's rows are the residual rows phys->stmp3 CGSolveQ
@@ -458,8 +521,7 @@ void CGSolveQ(REAL **q, REAL **src, REAL **c, gridT *grid, physT *phys, propT *p
     check(sunhip_q_solve(C->dev, &P, &g_last[0].st), W);
   }
   g_last[0].valid = 1;
-  download(C, "q", W);
-  unpack_cells(C, grid, q, 0, 1);
+  fetch1(C, "q", cells_job(C, grid, q, 0, 1), W);
 }
 
 void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
@@ -482,10 +544,10 @@ void CGSolve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MP
     check(sunhip_h_solve(C->dev, &P, prop->grav, prop->theta, prop->dt, &g_last[1].st), W);
   }
   g_last[1].valid = 1;
-  download_all(C, "h", W);
+  const double *h = download_all(C, "h", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
-    phys->h[i] = C->flat[i];
+    phys->h[i] = h[i];
   }
 }
 
@@ -501,8 +563,7 @@ void OperatorQ(REAL **coef, REAL **x, REAL **y, REAL **c, gridT *grid, physT *ph
   check(sunhip_upload(C->dev, "D", phys->D), W);
   upload_cells(C, grid, "opx", x, 0, W);
   check(sunhip_q_operator(C->dev, "opx", "opy"), W);
-  download(C, "opy", W);
-  unpack_cells(C, grid, y, 0, 1);
+  fetch1(C, "opy", cells_job(C, grid, y, 0, 1), W);
 }
 
 void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, propT *prop) {
@@ -513,8 +574,7 @@ void Preconditioner(REAL **x, REAL **xc, REAL **coef, gridT *grid, physT *phys, 
   upload_cells(C, grid, "coef", coef, 0, W);
   upload_cells(C, grid, "opx", x, 0, W);
   check(sunhip_q_precondition(C->dev, "opx", "opy"), W);
-  download(C, "opy", W);
-  unpack_cells(C, grid, xc, 0, 1);
+  fetch1(C, "opy", cells_job(C, grid, xc, 0, 1), W);
 }
 
 int sunhip_h_operator_coef(sunhip_grid *g, const double *hfcoef_flat, const char *x, const char *y);
@@ -527,10 +587,10 @@ void OperatorH(REAL *x, REAL *y, REAL *coef, REAL *fcoef, gridT *grid, physT *ph
   check(sunhip_upload(C->dev, "hcoef", coef), W);
   check(sunhip_upload(C->dev, "hx", x), W);
   check(sunhip_h_operator_coef(C->dev, fcoef, "hx", "hy"), W);
-  download_all(C, "hy", W);
+  const double *hy = download_all(C, "hy", W);
   for (int ip = grid->celldist[0]; ip < grid->celldist[1]; ip++) {
     const int i = grid->cellp[ip];
-    y[i] = C->flat[i];
+    y[i] = hy[i];
   }
 }
 
@@ -547,18 +607,6 @@ static void upload_brows(ctx_t *C, gridT *g, REAL **bscal, const char *name, con
   check(sunhip_upload(C->dev, name, b), W);
 }
 static void upload_bscal(ctx_t *C, gridT *g, REAL **bscal, const char *W) { upload_brows(C, g, bscal, "bscal", W); }
-
-static void download_face_values(ctx_t *C, gridT *g, physT *p, const char *W) {
-  const char *names[2] = {"SfHp", "SfHm"};
-  REAL **rows[2] = {p->SfHp, p->SfHm};
-  for (int f = 0; f < 2; f++) {
-    /* levels [0, Nke) of the edges of edgep[edgedist[0..1)) and edgep[edgedist[2..3)) */
-    download(C, names[f], W);
-    rows_job J = {ROWS_UNPACK_UPTO, NULL, rows[f], C->est, C->edge_rank, g->edgedist[0], g->edgedist[1],
-                  g->edgedist[2], g->edgedist[3], g->Nke, 0, g->Ne};
-    unpack_parts(C, J);
-  }
-}
 
 static void scalar_inputs(ctx_t *C, gridT *grid, physT *phys, REAL **scal, REAL **bscal, const char *W) {
   extents(C, grid, W);
@@ -577,11 +625,11 @@ void HorizontalFaceScalars(gridT *grid, physT *phys, propT *prop, REAL **scal, R
   scalar_inputs(C, grid, phys, scal, boundary_scal, W);
   sunhip_scalar_params S = {prop->theta, prop->dt, 0.0, TVD, prop->n, 0, 0};
   check(sunhip_face_scalars(C->dev, &S, "s"), W);
-  download_face_values(C, grid, phys, W);
-  download(C, "gradSx", W);
-  unpack_cells(C, grid, phys->gradSx, 0, 2);
-  download(C, "gradSy", W);
-  unpack_cells(C, grid, phys->gradSy, 0, 2);
+  const fetch_t F[] = {{"SfHp", face_values_job(C, grid, phys->SfHp)},
+                       {"SfHm", face_values_job(C, grid, phys->SfHm)},
+                       {"gradSx", cells_job(C, grid, phys->gradSx, 0, 2)},
+                       {"gradSy", cells_job(C, grid, phys->gradSy, 0, 2)}};
+  fetch(C, F, 4, W);
 }
 
 void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **scal, REAL **boundary_scal,
@@ -626,12 +674,12 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
   check(sunhip_update_scalars_src(C->dev, &S, "s", src1 ? "src1" : NULL, src2 ? "src2" : NULL,
                                   Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL, bcell ? "bcell" : NULL), W);
-  download(C, "s", W);
-  unpack_cells(C, grid, scal, 0, 1);
   /* Cn: the computational columns, zero in the others */
-  download(C, "Cn", W);
-  unpack_cells_zero(C, grid, Cn, 0, 1);
-  if (TVDscheme) download_face_values(C, grid, phys, W);
+  const fetch_t F[] = {{"s", cells_job(C, grid, scal, 0, 1)},
+                       {"Cn", cells_zero_job(C, grid, Cn, 0, 1)},
+                       {"SfHp", face_values_job(C, grid, phys->SfHp)},
+                       {"SfHm", face_values_job(C, grid, phys->SfHm)}};
+  fetch(C, F, TVDscheme ? 4 : 2, W);
 }
 
 
@@ -647,10 +695,8 @@ void ComputeQSource(REAL **src, gridT *grid, physT *phys, propT *prop, int mypro
   upload_cells(C, grid, "w", phys->w, 1, W);
   upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
   check(sunhip_compute_qsource(C->dev, prop->theta, prop->dt, "src"), W);
-  download(C, "src", W);
-  unpack_cells_wet(C, grid, src);
-  download_all(C, "D", W);
-  memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
+  fetch1(C, "src", cells_wet_job(C, grid, src), W);
+  memcpy(phys->D, download_all(C, "D", W), (size_t)grid->Ne * sizeof(double));
 }
 
 void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
@@ -666,12 +712,10 @@ void Corrector(REAL **qc, gridT *grid, physT *phys, propT *prop, int myproc, int
   upload_edges(C, grid, "u", phys->u, W);
   check(sunhip_upload(C->dev, "D", phys->D), W);
   check(sunhip_corrector(C->dev, prop->dt, "q", "qp"), W);
-  download(C, "u", W);
-  unpack_edges(C, grid, phys->u);
-  download(C, "w", W);
-  unpack_cells_w(C, grid, phys->w, 0, 1);
-  download(C, "qp", W);
-  unpack_cells(C, grid, phys->q, 0, 1);
+  const fetch_t F[] = {{"u", edges_job(C, grid, phys->u)},
+                       {"w", cells_w_job(C, grid, phys->w, 0, 1)},
+                       {"qp", cells_job(C, grid, phys->q, 0, 1)}};
+  fetch(C, F, 3, W);
 }
 
 void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
@@ -683,8 +727,7 @@ void Continuity(REAL **w, gridT *grid, physT *phys, propT *prop) {
   upload_edges(C, grid, "dzf", grid->dzf, W);
   upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
   check(sunhip_continuity(C->dev, prop->theta, "w"), W);
-  download(C, "w", W);
-  unpack_cells_w(C, grid, w, 0, 1);
+  fetch1(C, "w", cells_w_job(C, grid, w, 0, 1), W);
 }
 
 
@@ -723,16 +766,12 @@ void sunhip_UPredictor_begin(gridT *grid, physT *phys, propT *prop, int myproc, 
   if (rc == -5) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(0); }  /* main/phys.c:2902 exits with 0 */
   if (rc == -6) { fprintf(stderr, "%s: %s\n", W, sunhip_error()); exit(EXIT_FAILURE); }
   check(rc, W);
-  download(C, "utmp", W);
-  unpack_edges(C, grid, phys->utmp);
-  download(C, "ue", W);
-  unpack_edges(C, grid, phys->ut);
-  download(C, "utmp2", W);
-  unpack_edges_wet(C, grid, phys->utmp2);
-  download(C, "wtmp2", W);
-  unpack_cells_w(C, grid, phys->wtmp2, 0, 2);
-  download_all(C, "D", W);
-  memcpy(phys->D, C->flat, (size_t)grid->Ne * sizeof(double));
+  const fetch_t F[] = {{"utmp", edges_job(C, grid, phys->utmp)},
+                       {"ue", edges_job(C, grid, phys->ut)},
+                       {"utmp2", edges_wet_job(C, grid, phys->utmp2)},
+                       {"wtmp2", cells_w_job(C, grid, phys->wtmp2, 0, 2)}};
+  fetch(C, F, 4, W);
+  memcpy(phys->D, download_all(C, "D", W), (size_t)grid->Ne * sizeof(double));
 }
 
 void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, int numprocs, MPI_Comm comm) {
@@ -757,16 +796,12 @@ void sunhip_UPredictor_solve(gridT *grid, physT *phys, propT *prop, int myproc, 
   sunhip_cg_params G = {prop->maxiters, prop->hprecond, (int)prop->resnorm, prop->epsilon, 0, 0};
   check(sunhip_u_solve(C->dev, &P, &G, &g_last[1].st), W);
   g_last[1].valid = 1;
-  download_all(C, "h", W);
-  memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-  download_all(C, "hcorr", W);
-  memcpy(phys->hcorr, C->flat, (size_t)grid->Nc * sizeof(double));
-  download_all(C, "active", W);
-  for (int i = 0; i < grid->Nc; i++) phys->active[i] = C->flat[i] != 0;
-  download(C, "u", W);
-  unpack_edges(C, grid, phys->u);
-  download(C, "s", W);
-  unpack_cells(C, grid, phys->s, 0, 2);
+  memcpy(phys->h, download_all(C, "h", W), (size_t)grid->Nc * sizeof(double));
+  memcpy(phys->hcorr, download_all(C, "hcorr", W), (size_t)grid->Nc * sizeof(double));
+  const double *act = download_all(C, "active", W);
+  for (int i = 0; i < grid->Nc; i++) phys->active[i] = act[i] != 0;
+  const fetch_t F[] = {{"u", edges_job(C, grid, phys->u)}, {"s", cells_job(C, grid, phys->s, 0, 2)}};
+  fetch(C, F, 2, W);
 }
 
 static sunhip_vgrid_params vgrid_params(const gridT *grid, const propT *prop) {
@@ -786,25 +821,17 @@ void UpdateDZ(gridT *grid, physT *phys, propT *prop, int option) {
   sunhip_vgrid_params V = vgrid_params(grid, prop);
   check(sunhip_update_dz(C->dev, &V, option), W);
   extents_changed(C);
-  download(C, "dzz", W);
-  unpack_cells(C, grid, grid->dzz, 0, 2);
   check(sunhip_get_extents(C->dev, grid->ctop, option != -1 ? grid->ctopold : NULL, grid->etop,
                            option != -1 ? grid->etopold : NULL), W);
-  if (option != -1) {
-    download(C, "dzzold", W);
-    unpack_cells(C, grid, grid->dzzold, 0, 2);
-  } else {
-    download_all(C, "h", W);
-    memcpy(phys->h, C->flat, (size_t)grid->Nc * sizeof(double));
-    download_all(C, "dv", W);
-    memcpy(grid->dv, C->flat, (size_t)grid->Nc * sizeof(double));
-    download_all(C, "dzbot", W);
-    memcpy(grid->dzbot, C->flat, (size_t)grid->Nc * sizeof(double));
+  const fetch_t F[] = {{"dzz", cells_job(C, grid, grid->dzz, 0, 2)},
+                       {"dzzold", cells_job(C, grid, grid->dzzold, 0, 2)}};
+  fetch(C, F, option != -1 ? 2 : 1, W);
+  if (option == -1) {
+    memcpy(phys->h, download_all(C, "h", W), (size_t)grid->Nc * sizeof(double));
+    memcpy(grid->dv, download_all(C, "dv", W), (size_t)grid->Nc * sizeof(double));
+    memcpy(grid->dzbot, download_all(C, "dzbot", W), (size_t)grid->Nc * sizeof(double));
   }
-  if (option == 1) {
-    download_all(C, "dzfB", W);
-    memcpy(grid->dzfB, C->flat, (size_t)grid->Ne * sizeof(double));
-  }
+  if (option == 1) memcpy(grid->dzfB, download_all(C, "dzfB", W), (size_t)grid->Ne * sizeof(double));
 }
 
 void SetFluxHeight(gridT *grid, physT *phys, propT *prop) {
@@ -818,12 +845,8 @@ void SetFluxHeight(gridT *grid, physT *phys, propT *prop) {
   if (grid->smoothbot) check(sunhip_upload(C->dev, "dzbot", grid->dzbot), W);
   sunhip_vgrid_params V = vgrid_params(grid, prop);
   check(sunhip_set_flux_height(C->dev, &V), W);
-  download(C, "dzf", W);
-  unpack_edges(C, grid, grid->dzf);
-  if (grid->smoothbot) {
-    download(C, "dzz", W);
-    unpack_cells(C, grid, grid->dzz, 0, 2);
-  }
+  const fetch_t F[] = {{"dzf", edges_job(C, grid, grid->dzf)}, {"dzz", cells_job(C, grid, grid->dzz, 0, 2)}};
+  fetch(C, F, grid->smoothbot ? 2 : 1, W);
 }
 
 /* WPredictor (main/phys.c:2025-2290), file-static in the reference: the
@@ -878,30 +901,18 @@ void sunhip_WPredictor(gridT *grid, physT *phys, propT *prop, int myproc, int nu
   sunhip_wpred_params P = {prop->dt, prop->theta, prop->nu, prop->nu_H, prop->CdW, prop->laxWendroff_Vertical,
                            scheme, prop->n, prop->AB, prop->TVDmomentum, prop->conserveMomentum};
   check(sunhip_w_predict(C->dev, &P), W);
-  const char *wnames[] = {"w", "wtmp", "Cn_W", "Cn_W2"};
-  REAL **wrows[] = {phys->w, phys->wtmp, phys->Cn_W, phys->Cn_W2};
-  for (int f = 0; f < 4; f++) {
-    download(C, wnames[f], W);
-    unpack_cells_w(C, grid, wrows[f], 0, 2);
-  }
-  download(C, "stmp", W);
-  unpack_cells(C, grid, phys->stmp, 0, 2);
-  if (scheme) {
-    download(C, "wc", W);
-    unpack_cells(C, grid, phys->wc, 0, 2);
-    download(C, "ue", W);
-    unpack_edges(C, grid, phys->ut);
-  }
-  if (scheme == 5) {
-    download(C, "SfHp", W);
-    unpack_edges(C, grid, phys->SfHp);
-    download(C, "SfHm", W);
-    unpack_edges(C, grid, phys->SfHm);
-    download(C, "gradSx", W);
-    unpack_cells(C, grid, phys->gradSx, 0, 2);
-    download(C, "gradSy", W);
-    unpack_cells(C, grid, phys->gradSy, 0, 2);
-  }
+  const fetch_t F[] = {{"w", cells_w_job(C, grid, phys->w, 0, 2)},
+                       {"wtmp", cells_w_job(C, grid, phys->wtmp, 0, 2)},
+                       {"Cn_W", cells_w_job(C, grid, phys->Cn_W, 0, 2)},
+                       {"Cn_W2", cells_w_job(C, grid, phys->Cn_W2, 0, 2)},
+                       {"stmp", cells_job(C, grid, phys->stmp, 0, 2)},
+                       {"wc", cells_job(C, grid, phys->wc, 0, 2)},
+                       {"ue", edges_job(C, grid, phys->ut)},
+                       {"SfHp", edges_job(C, grid, phys->SfHp)},
+                       {"SfHm", edges_job(C, grid, phys->SfHm)},
+                       {"gradSx", cells_job(C, grid, phys->gradSx, 0, 2)},
+                       {"gradSy", cells_job(C, grid, phys->gradSy, 0, 2)}};
+  fetch(C, F, scheme == 5 ? 11 : scheme ? 7 : 5, W);
 }
 
 int sunhip_last_solve(int which, int *niters, double *resid, double *device_ms) {

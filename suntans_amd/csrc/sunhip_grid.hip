@@ -566,6 +566,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   if (!g) return 0;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
+  if (g->xstream) hipStreamSynchronize(g->xstream);
   if (g->batch) batch_forget_grid(g->batch, g);
   for (auto &kv : g->fields) hipFree(kv.second.d);
   for (auto &kv : g->fields32) hipFree(kv.second);
@@ -574,17 +575,19 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_xcode, g->d_pair, g->d_npair,
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
-                  g->d_cellslot, g->d_edgeslot, g->d_stage, g->d_part, g->d_count, g->d_qs, g->d_hs,
+                  g->d_cellslot, g->d_edgeslot, g->d_stage[0], g->d_stage[1], g->d_part, g->d_count, g->d_qs, g->d_hs,
                   g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold,
                   g->d_ford, g->d_zdzt, g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag, g->d_b4perm};
   for (void *p : ptrs) if (p) hipFree(p);
-  if (g->h_stage) hipHostFree(g->h_stage);
   for (int i = 0; i < 2; i++) {
     if (g->h_pin[i]) hipHostFree(g->h_pin[i]);
     if (g->ev_pin[i]) hipEventDestroy(g->ev_pin[i]);
+    if (g->ev_stage[i]) hipEventDestroy(g->ev_stage[i]);
+    for (int p = 0; p < SUNHIP_MAXPARTS; p++)
+      if (g->dl[i].ev_part[p]) hipEventDestroy(g->dl[i].ev_part[p]);
   }
-  for (int i = 0; i < SUNHIP_MAXPARTS; i++)
-    if (g->ev_part[i]) hipEventDestroy(g->ev_part[i]);
+  if (g->ev_hand) hipEventDestroy(g->ev_hand);
+  if (g->xstream) hipStreamDestroy(g->xstream);
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->h_poll) hipHostFree(g->h_poll);
   for (hipEvent_t e : g->evp) if (e) hipEventDestroy(e);
@@ -651,15 +654,44 @@ int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
   return 0;
 }
 
-static int ensure_stage(sunhip_grid *g, size_t n) {
-  if (g->stage_n < n) {
-    if (g->d_stage) { hipFree(g->d_stage); g->bytes -= (int64_t)(g->stage_n * sizeof(double)); }
-    g->d_stage = nullptr;
-    g->stage_n = 0;
-    int rc = dev_alloc(g, (void **)&g->d_stage, n * sizeof(double));
-    if (rc) return rc;
-    g->stage_n = n;
+// Take staging buffer g->stage_next (at least n values) for a transfer whose
+// first step runs on stream s: s waits until the buffer's last work is over.
+// The copy stream and its events are made on first use.
+static int take_stage(sunhip_grid *g, size_t n, hipStream_t s, int *out) {
+  if (!g->xstream) {
+    HIPCHK(hipStreamCreateWithFlags(&g->xstream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&g->ev_hand, hipEventDisableTiming));
+    for (int i = 0; i < 2; i++) HIPCHK(hipEventCreateWithFlags(&g->ev_stage[i], hipEventDisableTiming));
   }
+  const int b = g->stage_next;
+  if (g->stage_n[b] < n) {
+    if (g->stage_busy[b]) HIPCHK(hipEventSynchronize(g->ev_stage[b]));
+    g->stage_busy[b] = false;
+    if (g->d_stage[b]) { hipFree(g->d_stage[b]); g->bytes -= (int64_t)(g->stage_n[b] * sizeof(double)); }
+    g->d_stage[b] = nullptr;
+    g->stage_n[b] = 0;
+    int rc = dev_alloc(g, (void **)&g->d_stage[b], n * sizeof(double));
+    if (rc) return rc;
+    g->stage_n[b] = n;
+  }
+  if (g->stage_busy[b]) HIPCHK(hipStreamWaitEvent(s, g->ev_stage[b], 0));
+  g->stage_next = b ^ 1;
+  *out = b;
+  return 0;
+}
+
+// Work on staging buffer b, last enqueued on stream s, is over when s gets here.
+static int release_stage(sunhip_grid *g, int b, hipStream_t s) {
+  HIPCHK(hipEventRecord(g->ev_stage[b], s));
+  g->stage_busy[b] = true;
+  g->stage_uses[b]++;
+  return 0;
+}
+
+// Stream b continues only after what is enqueued on stream a so far.
+static int hand_over(sunhip_grid *g, hipStream_t a, hipStream_t b) {
+  HIPCHK(hipEventRecord(g->ev_hand, a));
+  HIPCHK(hipStreamWaitEvent(b, g->ev_hand, 0));
   return 0;
 }
 
@@ -706,49 +738,67 @@ static int pinned_slot(const sunhip_grid *g, const double *flat, size_t n) {
 
 }  // extern "C"
 
-static void scatter_field(sunhip_grid *g, const Field *f) {
+static void scatter_field(sunhip_grid *g, const Field *f, const double *st) {
   switch (f->kind) {
     case CELL: case CELLW:
-      k_scatter_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
+      k_scatter_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(st, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
                                                            g->d_nk, f->kind == CELLW, g->d_toff, g->d_tdep);
       break;
     case EDGE:
-      k_scatter_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff,
+      k_scatter_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(st, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff,
                                                             g->d_edep);
       break;
     case CELL2:
-      k_scatter_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_cellslot, g->Nc);
+      k_scatter_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(st, f->d, g->d_cellslot, g->Nc);
       break;
     case EDGE2:
-      k_scatter_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
+      k_scatter_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(st, f->d, g->d_edgeslot, g->Ne);
       break;
   }
 }
 
-static void gather_field(sunhip_grid *g, const Field *f) {
+static void gather_field(sunhip_grid *g, const Field *f, double *st) {
   switch (f->kind) {
     case CELL: case CELLW:
-      k_gather_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
+      k_gather_cols<<<g->Nt, SUNHIP_TILE, 0, g->stream>>>(st, f->d, f->kind == CELL ? g->d_cstart : g->d_cstartw,
                                                           g->d_nk, f->kind == CELLW, g->d_toff);
       break;
     case EDGE:
-      k_gather_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(g->d_stage, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff);
+      k_gather_cols<<<g->Net, SUNHIP_TILE, 0, g->stream>>>(st, f->d, g->d_estart, g->d_e_nkc, 0, g->d_eoff);
       break;
     case CELL2:
-      k_gather_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_cellslot, g->Nc);
+      k_gather_2d<<<(g->Nc + 255) / 256, 256, 0, g->stream>>>(st, f->d, g->d_cellslot, g->Nc);
       break;
     case EDGE2:
-      k_gather_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(g->d_stage, f->d, g->d_edgeslot, g->Ne);
+      k_gather_2d<<<(g->Ne + 255) / 256, 256, 0, g->stream>>>(st, f->d, g->d_edgeslot, g->Ne);
       break;
   }
 }
 
 // A download in parts not yet waited for in full.
 static int parts_outstanding(sunhip_grid *g) {
-  if (g->parts_done < g->nparts) {
+  if (g->dl_count) {
     set_error("a download in parts has not been waited for in full");
     return -1;
   }
+  return 0;
+}
+
+// The last copy of an upload from staging buffer b is enqueued on the copy
+// stream: scatter b into f on the compute stream after it, and mark what the
+// caller may reuse when.
+static int finish_upload(sunhip_grid *g, const Field *f, int b, const double *flat, size_t n) {
+  const int pin = pinned_slot(g, flat, n);
+  if (pin >= 0) {  // the slot is reused only through sunhip_pinned, which waits for the copy
+    HIPCHK(hipEventRecord(g->ev_pin[pin], g->xstream));
+    g->pin_pending[pin] = true;
+  }
+  int rc = hand_over(g, g->xstream, g->stream);
+  if (rc) return rc;
+  scatter_field(g, f, g->d_stage[b]);
+  HIPCHK(hipGetLastError());
+  if ((rc = release_stage(g, b, g->stream))) return rc;
+  if (pin < 0) HIPCHK(hipStreamSynchronize(g->xstream));  // the caller may reuse its buffer
   return 0;
 }
 
@@ -765,18 +815,10 @@ int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
     return -1;
   }
   const size_t n = flat_len(g, f->kind);
-  if ((rc = ensure_stage(g, n))) return rc;
-  const int pin = pinned_slot(g, flat, n);
-  HIPCHK(hipMemcpyAsync(g->d_stage, flat, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
-  scatter_field(g, f);
-  HIPCHK(hipGetLastError());
-  if (pin >= 0) {  // the slot is reused only through sunhip_pinned, which waits for this copy
-    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
-    g->pin_pending[pin] = true;
-    return 0;
-  }
-  HIPCHK(hipStreamSynchronize(g->stream));  // the caller may reuse its buffer
-  return 0;
+  int b;
+  if ((rc = take_stage(g, n, g->xstream, &b))) return rc;
+  HIPCHK(hipMemcpyAsync(g->d_stage[b], flat, n * sizeof(double), hipMemcpyHostToDevice, g->xstream));
+  return finish_upload(g, f, b, flat, n);
 }
 
 int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int64_t off, int64_t n, int last) {
@@ -792,22 +834,15 @@ int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int
     g->up_filled = 0;
     return -1;
   }
-  if ((rc = ensure_stage(g, (size_t)len))) return rc;
+  if (off == 0 && (rc = take_stage(g, (size_t)len, g->xstream, &g->up_stage))) return rc;
+  const int b = g->up_stage;
   if (n > 0)
-    HIPCHK(hipMemcpyAsync(g->d_stage + off, flat + off, (size_t)n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(g->d_stage[b] + off, flat + off, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                          g->xstream));
   g->up_filled = off + n;
   if (!last) return 0;
   g->up_filled = 0;
-  scatter_field(g, f);
-  HIPCHK(hipGetLastError());
-  const int pin = pinned_slot(g, flat, (size_t)len);
-  if (pin >= 0) {
-    HIPCHK(hipEventRecord(g->ev_pin[pin], g->stream));
-    g->pin_pending[pin] = true;
-    return 0;
-  }
-  HIPCHK(hipStreamSynchronize(g->stream));
-  return 0;
+  return finish_upload(g, f, b, flat, (size_t)len);
 }
 
 int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
@@ -817,11 +852,14 @@ int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
   if (rc) return rc;
   if ((rc = parts_outstanding(g))) return rc;
   const size_t n = flat_len(g, f->kind);
-  if ((rc = ensure_stage(g, n))) return rc;
-  gather_field(g, f);
+  int b;
+  if ((rc = take_stage(g, n, g->stream, &b))) return rc;
+  gather_field(g, f, g->d_stage[b]);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(flat, g->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipStreamSynchronize(g->stream));
+  if ((rc = hand_over(g, g->stream, g->xstream))) return rc;
+  HIPCHK(hipMemcpyAsync(flat, g->d_stage[b], n * sizeof(double), hipMemcpyDeviceToHost, g->xstream));
+  if ((rc = release_stage(g, b, g->xstream))) return rc;
+  HIPCHK(hipStreamSynchronize(g->xstream));
   return 0;
 }
 
@@ -830,34 +868,57 @@ int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int np
   Field *f;
   int rc = field(g, name, &f);
   if (rc) return rc;
-  if ((rc = parts_outstanding(g))) return rc;
+  if (g->dl_count >= 2) {
+    set_error("sunhip_download_parts: two downloads in parts are outstanding");
+    return -1;
+  }
+  if (g->up_filled) {
+    set_error("sunhip_download_parts: an upload in parts is in progress");
+    return -1;
+  }
   if (nparts < 1 || nparts > SUNHIP_MAXPARTS) {
     set_error("sunhip_download_parts: %d parts (1..%d)", nparts, SUNHIP_MAXPARTS);
     return -1;
   }
   const int64_t n = (int64_t)flat_len(g, f->kind);
-  if ((rc = ensure_stage(g, (size_t)n))) return rc;
-  gather_field(g, f);
+  int b;
+  if ((rc = take_stage(g, (size_t)n, g->stream, &b))) return rc;
+  gather_field(g, f, g->d_stage[b]);
   HIPCHK(hipGetLastError());
-  for (int p = 0; p <= nparts; p++) g->part_bound[p] = n * p / nparts;
+  if ((rc = hand_over(g, g->stream, g->xstream))) return rc;
+  sunhip_grid::Download &D = g->dl[(g->dl_head + g->dl_count) & 1];
+  for (int p = 0; p <= nparts; p++) D.part_bound[p] = n * p / nparts;
   for (int p = 0; p < nparts; p++) {
-    if (!g->ev_part[p]) HIPCHK(hipEventCreateWithFlags(&g->ev_part[p], hipEventDisableTiming));
-    const int64_t a = g->part_bound[p], b = g->part_bound[p + 1];
-    if (b > a)
-      HIPCHK(hipMemcpyAsync(flat + a, g->d_stage + a, (size_t)(b - a) * sizeof(double), hipMemcpyDeviceToHost,
-                            g->stream));
-    HIPCHK(hipEventRecord(g->ev_part[p], g->stream));
+    if (!D.ev_part[p]) HIPCHK(hipEventCreateWithFlags(&D.ev_part[p], hipEventDisableTiming));
+    const int64_t a = D.part_bound[p], e = D.part_bound[p + 1];
+    if (e > a)
+      HIPCHK(hipMemcpyAsync(flat + a, g->d_stage[b] + a, (size_t)(e - a) * sizeof(double), hipMemcpyDeviceToHost,
+                            g->xstream));
+    HIPCHK(hipEventRecord(D.ev_part[p], g->xstream));
   }
-  g->nparts = nparts;
-  g->parts_done = 0;
+  if ((rc = release_stage(g, b, g->xstream))) return rc;
+  D.stage = b;
+  D.use = g->stage_uses[b];
+  D.nparts = nparts;
+  D.parts_done = 0;
+  g->dl_count++;
   return 0;
 }
 
 int sunhip_download_wait(sunhip_grid *g, int64_t upto) {
   HIPCHK(hipSetDevice(g->device));
-  while (g->parts_done < g->nparts && g->part_bound[g->parts_done] < upto) {
-    HIPCHK(hipEventSynchronize(g->ev_part[g->parts_done]));
-    g->parts_done++;
+  if (!g->dl_count) return 0;
+  sunhip_grid::Download &D = g->dl[g->dl_head];
+  while (D.parts_done < D.nparts && D.part_bound[D.parts_done] < upto) {
+    HIPCHK(hipEventSynchronize(D.ev_part[D.parts_done]));
+    D.parts_done++;
+  }
+  if (D.parts_done == D.nparts) {
+    // every copy out of its staging buffer is over: the next transfer through
+    // that buffer need not wait on the device for the copy stream
+    if (g->stage_uses[D.stage] == D.use) g->stage_busy[D.stage] = false;
+    g->dl_head ^= 1;
+    g->dl_count--;
   }
   return 0;
 }
@@ -903,6 +964,7 @@ int sunhip_field_scale(sunhip_grid *g, const char *dst, const char *src, double 
 int sunhip_sync(sunhip_grid *g) {
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipStreamSynchronize(g->stream));
+  if (g->xstream) HIPCHK(hipStreamSynchronize(g->xstream));
   return 0;
 }
 

@@ -154,19 +154,33 @@ struct sunhip_grid {
   int *d_e_nke = nullptr, *d_e_nkc = nullptr, *d_e_etop = nullptr, *d_e_mark = nullptr, *d_e_comp = nullptr;
   int64_t *d_cstart = nullptr, *d_cstartw = nullptr, *d_estart = nullptr;  // per slot flat start, -1 padding
   int *d_cellslot = nullptr, *d_edgeslot = nullptr;  // host index -> slot
-  double *d_stage = nullptr;        // flat staging buffer (device)
-  size_t stage_n = 0;
-  double *h_stage = nullptr;        // pinned host staging
-  size_t h_stage_n = 0;
+  // Transfers: two device staging buffers taken in turn, the host<->device
+  // copies on a stream of their own, so that one field's copy overlaps the
+  // scatter or gather kernel of the one before it.
+  double *d_stage[2] = {nullptr, nullptr};
+  size_t stage_n[2] = {0, 0};
+  int stage_next = 0;                            // the buffer the next transfer takes
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};   // the last work on buffer b is over
+  bool stage_busy[2] = {false, false};           // ... recorded and maybe not reached yet
+  unsigned stage_uses[2] = {0, 0};               // transfers through buffer b so far
+  hipStream_t xstream = nullptr;                 // the copies (default priority)
+  hipEvent_t ev_hand = nullptr;                  // hand-over between xstream and stream
   double *h_pin[2] = {nullptr, nullptr};   // pinned host slots (sunhip_pinned)
   size_t h_pin_n[2] = {0, 0};
   hipEvent_t ev_pin[2] = {nullptr, nullptr};  // last upload read from the slot
   bool pin_pending[2] = {false, false};
   // transfers in parts (sunhip_upload_part, sunhip_download_parts)
   int64_t up_filled = 0;                     // values of the upload in progress copied so far
-  int nparts = 0, parts_done = 0;            // download in progress: parts issued, parts waited for
-  int64_t part_bound[SUNHIP_MAXPARTS + 1] = {};
-  hipEvent_t ev_part[SUNHIP_MAXPARTS] = {};  // part p of the download is on the host
+  int up_stage = 0;                          // its staging buffer
+  // downloads in parts not yet waited for in full, oldest first (at most 2)
+  struct Download {
+    int nparts = 0, parts_done = 0;            // parts issued, parts waited for
+    int64_t part_bound[SUNHIP_MAXPARTS + 1] = {};
+    hipEvent_t ev_part[SUNHIP_MAXPARTS] = {};  // part p is on the host
+    int stage = 0;                             // the staging buffer its copies read
+    unsigned use = 0;                          // ... and which use of it
+  } dl[2];
+  int dl_head = 0, dl_count = 0;
   bool have_extents = false;
   // reductions
   double *d_part = nullptr;         // block partials
