@@ -184,8 +184,11 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * with 2 levels per load group, 4-6 the pipelined form; -1, the default: 2
  * from 4096 tiles up, else 0), "tvd_group" (levels per load group of the
  * scalar step's column update: 1, 2 (the default) or 4), "tvd_sums_group"
- * (the same for the TVD inflow sums: 1, 2 or 4, the default).  Only "tiles",
- * "reduce", "rzid", "band" and "order" change the rounding of a dot product;
+ * (the same for the TVD inflow sums: 1, 2 or 4, the default), "h_blocks"
+ * (most blocks of the free-surface iteration kernels, each adding all the
+ * others' sums: 256, 512, 1024 (the default) or 2048).  Only "tiles",
+ * "reduce", "rzid", "band", "order" and "h_blocks" change the rounding of a
+ * dot product;
  * every other knob gives bit-identical results. */
 int sunhip_set_tuning(sunhip_grid *g, const char *key, int value);
 /* Per-kernel device time: sunhip_profile(g, 1) starts recording an event

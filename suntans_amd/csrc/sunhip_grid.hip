@@ -546,8 +546,8 @@ int sunhip_grid_create(sunhip_grid **out, const sunhip_topology *T, int device) 
   rc |= dev_alloc(g, (void **)&g->d_part, sizeof(double) * (size_t)g->part_n);
   rc |= dev_alloc(g, (void **)&g->d_count, sizeof(unsigned) * 64);
   rc |= dev_alloc(g, (void **)&g->d_qs, sizeof(CGScal));
-  rc |= dev_alloc(g, (void **)&g->d_hs, sizeof(CGScal));
-  rc |= dev_alloc(g, (void **)&g->d_ms, sizeof(CGScal));
+  rc |= dev_alloc(g, (void **)&g->d_hs, 2 * sizeof(CGScal));  // two: the free-surface iteration alternates them
+  rc |= dev_alloc(g, (void **)&g->d_ms, 2 * sizeof(CGScal));
   rc |= dev_alloc(g, (void **)&g->d_mo, sizeof(CGScal));
   if (rc) { sunhip_grid_destroy(g); return rc; }
   HIPCHK(hipMemset(g->d_count, 0, sizeof(unsigned) * 64));

@@ -101,6 +101,215 @@ __global__ __launch_bounds__(256) void k_happly(Geo G, HVec<T, TO> V, CGScal *sc
   }
 }
 
+// ---- the iteration: two kernels, each finishing the other's dot products ----
+//
+// Each iteration kernel starts by adding the previous kernel's block sums
+// (every block the same sums in the same order, so every block holds the same
+// scalars) and applying the pending step of the reference's loop to the
+// scalars: HOP_ITER (main/phys.c:3175-3203) before the operator, HOP_APPLY
+// (alpha / p.z) before the update.  The scalars alternate between two slots:
+// a kernel reads one and its block 0 writes the other, with its own step as
+// the one pending.  The block sums alternate between two areas likewise.
+// At most 1024 blocks (tuning key "h_blocks"; a few cells per thread), so
+// that each block reads a few KB of sums.  (Issuing a block's cell loads before the sums, or blocks
+// of 1024 threads with one cell each, measured slower at Nc = 1e6.)
+constexpr int HBS = 256;
+
+// h_partials for HBS-thread blocks
+template <int NV>
+__device__ void h_partials_it(const double (&v)[NV], double *part) {
+  __shared__ double sh[NV][HBS / 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double s = wave_sum(v[i]);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0;
+      for (int j = 0; j < HBS / 64; j++) s += sh[i][j];
+      part[(size_t)blockIdx.x * NV + i] = s;
+    }
+  }
+}
+
+// sum over the nparts block sums of part (NV per block), every block alike
+template <int NV>
+__device__ void h_sum_parts(const double *__restrict__ part, int nparts, double (&tot)[NV]) {
+  __shared__ double sh[NV][HBS / 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nparts; b += HBS) s += part[(size_t)b * NV + i];
+    s = wave_sum(s);
+    if (l == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = 0;
+    for (int j = 0; j < HBS / 64; j++) s += sh[i][j];
+    tot[i] = s;
+  }
+}
+
+// The scalars after in's pending step on the block sums of part (NV per
+// block, written by the kernel before); block 0 stores them to out with
+// next_op pending unless the solve is done.  out may be in (one block).
+template <int NV>
+__device__ void h_state(const CGScal *in, CGScal *out, const double *__restrict__ part, int nparts, int next_op,
+                        CGScal &S) {
+  __shared__ CGScal sS;
+  const int pend = in->pend, done = in->done;
+  __syncthreads();  // every thread has read pend before block 0 may overwrite it
+  double tot[NV];
+  if (pend && !done) h_sum_parts<NV>(part, nparts, tot);
+  if (threadIdx.x == 0) {
+    CGScal c = *in;
+    if (pend && !done) h_op(&c, pend, tot);
+    c.pend = 0;
+    sS = c;
+    if (blockIdx.x == 0) {
+      CGScal o = c;
+      o.pend = c.done ? 0 : next_op;
+      *out = o;
+    }
+  }
+  __syncthreads();
+  S = sS;
+}
+
+// One cell of the fused operator (happly_cell, mode HM_FUSED) in two halves:
+// the loads, unconditional at clamped indices, then the arithmetic in
+// happly_cell's order.  A thread loads its first HCPT cells before it works
+// on any, so their loads are in flight together (stores of one cell would
+// otherwise keep the compiler from issuing the next cell's loads early).
+constexpr int HCPT = 4;
+
+template <typename T, int NF>
+struct HCellIn {
+  bool ok;
+  int nf;
+  T po0, r0, c, x0;
+  int nb[NF];
+  T fc[NF], rn[NF], pn[NF];
+};
+
+template <typename T, int NF>
+__device__ __forceinline__ void hcell_direct(const Geo &G, const HVec<T, T> &V, int s, HCellIn<T, NF> &I) {
+  const int sc = s < G.NS ? s : 0;
+  I.ok = (s < G.NS) & (G.nk[sc] > 0) & (G.comp[sc] == 1);
+  I.nf = G.nf[sc];
+  I.po0 = V.pold[sc];
+  I.r0 = V.rtmp[sc];
+  I.c = V.coef[sc];
+  I.x0 = V.x[sc];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int row = f < G.mf ? f : G.mf - 1;  // rows past maxfaces are not stored
+    I.nb[f] = G.nbs[row * G.NS + sc];
+    I.fc[f] = V.fcoef[row * G.NS + sc];
+  }
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ void hcell_gather(const HVec<T, T> &V, int sc, HCellIn<T, NF> &I) {
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    if (!(f < I.nf && I.nb[f] >= 0)) I.nb[f] = -1;
+    const int n = I.nb[f] >= 0 ? I.nb[f] : sc;
+    I.rn[f] = V.rtmp[n];
+    I.pn[f] = V.pold[n];
+  }
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ double hcell_apply(const HVec<T, T> &V, const HCellIn<T, NF> &I, T mu, T nu, int s) {
+  if (!I.ok) return 0;
+  const T v0 = I.r0 + mu * I.po0;
+  T y = I.c * v0;
+#pragma unroll
+  for (int f = 0; f < NF; f++)
+    if (I.nb[f] >= 0) y -= I.fc[f] * (I.rn[f] + mu * I.pn[f]);
+  V.pnew[s] = v0;
+  V.x[s] = I.x0 + nu * I.po0;
+  V.out[s] = y;
+  return (double)v0 * (double)y;
+}
+
+// p = rtmp + mu p_old ; x += nu p_old ; z = L p ; block sums of p.z
+template <typename T, int NF>
+__global__ __launch_bounds__(HBS) void k_happly_it(Geo G, HVec<T, T> V, const CGScal *sin, CGScal *sout,
+                                                   const double *pin, double *pout, int nparts) {
+  CGScal S;
+  h_state<2>(sin, sout, pin, nparts, HOP_APPLY, S);
+  if (S.done) return;
+  const T mu = (T)S.mu, nu = (T)S.nu;
+  const int stride = gridDim.x * HBS;
+  const int s0 = xcd_block(blockIdx.x, gridDim.x) * HBS + threadIdx.x;
+  HCellIn<T, NF> I[HCPT];
+#pragma unroll
+  for (int j = 0; j < HCPT; j++) hcell_direct<T, NF>(G, V, s0 + j * stride, I[j]);
+#pragma unroll
+  for (int j = 0; j < HCPT; j++) hcell_gather<T, NF>(V, s0 + j * stride < G.NS ? s0 + j * stride : 0, I[j]);
+  double dot = 0;
+#pragma unroll
+  for (int j = 0; j < HCPT; j++) dot += hcell_apply<T, NF>(V, I[j], mu, nu, s0 + j * stride);
+  for (int s = s0 + HCPT * stride; s < G.NS; s += stride) dot += happly_cell<HM_FUSED, T, T>(G, V, mu, nu, s);
+  double v[1] = {dot};
+  h_partials_it<1>(v, pout);
+}
+
+// r -= nu z ; rtmp = M^-1 r ; block sums of r.rtmp and r.r (hprec_cell's
+// arithmetic; the first HCPT cells' loads issued together)
+template <typename T>
+__global__ __launch_bounds__(HBS) void k_hprec_it(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
+                                                  T *__restrict__ r, T *__restrict__ rt, const CGScal *sin,
+                                                  CGScal *sout, const double *pin, double *pout, int nparts) {
+  CGScal S;
+  h_state<1>(sin, sout, pin, nparts, HOP_ITER, S);
+  if (S.done) return;
+  const T nu = (T)S.nu;
+  const int pc = S.precond;
+  const int stride = gridDim.x * HBS;
+  const int s0 = xcd_block(blockIdx.x, gridDim.x) * HBS + threadIdx.x;
+  bool ok[HCPT];
+  T rk[HCPT], zk[HCPT], ck[HCPT];
+#pragma unroll
+  for (int j = 0; j < HCPT; j++) {
+    const int s = s0 + j * stride, sc = s < G.NS ? s : 0;
+    ok[j] = (s < G.NS) & (G.nk[sc] > 0) & (G.comp[sc] == 1);
+    rk[j] = r[sc];
+    zk[j] = z[sc];
+    ck[j] = coef[sc];
+  }
+  double rz = 0, rr = 0;
+#pragma unroll
+  for (int j = 0; j < HCPT; j++) {
+    if (!ok[j]) continue;
+    const int s = s0 + j * stride;
+    const T rn = rk[j] - nu * zk[j];
+    r[s] = rn;
+    const T u = pc == 1 ? rn / ck[j] : rn;
+    rt[s] = u;
+    rz += (double)rn * (double)u;
+    rr += (double)rn * (double)rn;
+  }
+  for (int s = s0 + HCPT * stride; s < G.NS; s += stride) hprec_cell<T>(G, coef, z, r, rt, nu, pc, true, s, rz, rr);
+  double v[2] = {rz, rr};
+  h_partials_it<2>(v, pout);
+}
+
+// Apply the pending step in place (one block): the scalars the host reads.
+__global__ __launch_bounds__(HBS) void k_hsettle(CGScal *sc, const double *pin, int nparts) {
+  CGScal S;
+  h_state<2>(sc, sc, pin, nparts, 0, S);
+}
+
 // r -= nu*z ; rtmp = r/hcoef (hprecond=1) or r ; dots.
 template <typename T>
 __global__ __launch_bounds__(256) void k_hprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
@@ -187,6 +396,12 @@ int h_start(sunhip_grid *g, HSolver<T> &S) {
   return 0;
 }
 
+// blocks of the iteration kernels; their sums go to d_part (p.z, nb values)
+// and d_part + nb (r.rtmp and r.r, 2 nb values)
+int nbh_it(const sunhip_grid *g) { return std::min(g->tune_hblk, (g->NS + HBS - 1) / HBS); }
+
+// One iteration: S.sc[0] holds the scalars before and after it (with the
+// step HOP_ITER pending), S.sc[1] those between its two kernels.
 template <typename T>
 int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
   const Geo G = g->geo();
@@ -195,15 +410,26 @@ int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
   V.pold = (it & 1) ? S.p1 : S.p0;
   V.pnew = (it & 1) ? S.p0 : S.p1;
   V.x = S.x; V.out = S.z;
-  const int nb = nbh(g);
+  const int nb = nbh_it(g);
+  double *pz = g->d_part, *prz = g->d_part + nb;
   int ps = prof_begin(g, pk_a);
-  k_happly<HM_FUSED, T, T><<<nb, 256, 0, g->stream>>>(G, V, S.sc, g->d_part);
+  with_nf(g->mf, [&](auto nfc) {
+    constexpr int NF = decltype(nfc)::value;
+    k_happly_it<T, NF><<<nb, HBS, 0, g->stream>>>(G, V, S.sc, S.sc + 1, prz, pz, nb);
+    return 0;
+  });
   prof_end(g, ps);
-  k_hfinalize<1><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_APPLY);
   ps = prof_begin(g, pk_p);
-  k_hprec<T><<<nb, 256, 0, g->stream>>>(G, S.coef, S.z, S.r, S.rt, S.sc, g->d_part, 1);
+  k_hprec_it<T><<<nb, HBS, 0, g->stream>>>(G, S.coef, S.z, S.r, S.rt, S.sc + 1, S.sc, pz, prz, nb);
   prof_end(g, ps);
-  k_hfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, S.sc, HOP_ITER);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The last iteration's HOP_ITER applied to S.sc[0].
+int h_settle(sunhip_grid *g, CGScal *sc) {
+  const int nb = nbh_it(g);
+  k_hsettle<<<1, HBS, 0, g->stream>>>(sc, g->d_part + nb, nb);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -218,6 +444,7 @@ int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a,
   while (!done && it < prm->maxiters) {
     for (int b = 0; b < every && it < prm->maxiters; b++, it++)
       if ((rc = h_iteration<T>(g, S, it, pk_a, pk_p))) return rc;
+    if ((rc = h_settle(g, S.sc))) return rc;
     if ((rc = poll_chunk(g, S.sc, slot, pending, &done))) return rc;
   }
   k_hfinish<T><<<nbh(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);

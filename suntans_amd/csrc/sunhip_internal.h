@@ -65,7 +65,8 @@ struct CGScal {
   double alpha, alpha0, eps, eps0, nu, mu, muinv, pz, tol, resid;
   int iter;       // loop iterations completed
   int nloop;      // the reference's loop counter n at exit
-  int maxit, done, converged, precond, resnorm, pad;
+  int maxit, done, converged, precond, resnorm;
+  int pend;       // op still to apply to the last kernel's block sums (free-surface iteration), 0 none
 };
 
 // Per-call geometry the kernels read, gathered into one argument block.
@@ -131,6 +132,7 @@ struct sunhip_grid {
   int *d_b4perm = nullptr;          // depth-first dispatch order of 4-tile blocks (Geo)
   int tune_order = 1;               // 1: use it
   int tune_tvdgrp = 2;              // levels per load group of the scalar column update (1, 2, 4)
+  int tune_hblk = 1024;             // most blocks of the free-surface iteration kernels
   int tune_sumsgrp = 4;             // levels per load group of the TVD inflow sums (1, 2, 4)
   int *d_uerr = nullptr;             // UPredictor's exit conditions
   double *d_dz = nullptr;            // [Nkmax] reference layer thicknesses (vertical grid updates)

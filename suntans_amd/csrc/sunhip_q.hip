@@ -1094,6 +1094,13 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     if (value != 1 && value != 2 && value != 4) { set_error("tvd_sums_group must be 1, 2 or 4"); return -1; }
     g->tune_sumsgrp = value;
   }
+  else if (!strcmp(key, "h_blocks")) {
+    if (value != 256 && value != 512 && value != 1024 && value != 2048) {
+      set_error("h_blocks must be 256, 512, 1024 or 2048");
+      return -1;
+    }
+    g->tune_hblk = value;
+  }
   else if (!strcmp(key, "reduce")) g->tune_reduce = value;
   else if (!strcmp(key, "tiles")) {
     if (value != 4 && value != 8 && value != 16) { set_error("tiles must be 4, 8 or 16"); return -1; }
