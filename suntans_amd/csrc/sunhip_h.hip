@@ -2,10 +2,11 @@
 //
 // Reference: CGSolve main/phys.c:3086-3217, HCoefficients :3252-3269,
 // OperatorH :3335-3349, HPreconditioner :3227-3235, InnerProduct :3280-3294.
-// Same two-kernel iteration as the pressure solve, one thread per cell, with
-// the scalars on the device and the block sums finished by a one-block
-// kernel in a fixed order.  Templated on the storage type like the pressure
-// solve: T=double is the drop-in, T=float the inner solve of the
+// Two kernels per iteration with the scalars on the device; each kernel
+// adds the previous one's block sums itself, in one fixed order, so that no
+// one-block finishing kernel runs between them (at Nc = 1e6 a launch costs
+// about as much as the work).  Templated on the storage type like the
+// pressure solve: T=double is the drop-in, T=float the inner solve of the
 // mixed-precision form (sunhip_h_solve_mixed).
 #include <hip/hip_runtime.h>
 

@@ -121,3 +121,34 @@ def test_tvd_group_bitwise(gpu_available, name, make, tvd):
     for group in (2, 4):
         got = scalar_step(st, group, tvd)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), group
+
+
+def h_solve(st, tuning):
+    dg = device.DeviceGrid(st)
+    dg.upload("D", cases.depth_integrated_D(st))
+    dg.upload("htmp", st.get("htmp"))
+    dg.upload("h", np.zeros(st.Nc))
+    for k, v in tuning:
+        dg.tune(k, v)
+    g, th, dt = st.get_prop("grav"), st.get_prop("theta"), st.get_prop("dt")
+    s = dg.h_solve(g, th, dt, maxiters=2000, precond=1, epsilon=1e-10)
+    h = dg.download("h", st.Nc)
+    dg.close()
+    return h, s["niters"]
+
+
+@pytest.fixture(scope="module")
+def h_grid():
+    # 320,000 cells: 256 blocks of 256 threads give each thread 4 or 5 cells,
+    # so the four batched cells and the loop after them both run
+    return cases.large_solve(seed=3, nx=400, ny=400, Nkmax=3, target_N=6.0e5)
+
+
+@pytest.mark.parametrize("blocks", [256, 512, 2048])
+def test_h_blocks_rounding(gpu_available, h_grid, blocks):
+    """h_blocks regroups the free-surface dot products' block sums: counts
+    within +-1 and h within 1e-12 of the default."""
+    h0, n0 = h_solve(h_grid, [])
+    h1, n1 = h_solve(h_grid, [("h_blocks", blocks)])
+    assert abs(n0 - n1) <= 1
+    assert np.linalg.norm(h1 - h0) <= 1e-12 * np.linalg.norm(h0)
