@@ -173,9 +173,9 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * four tiles walking the levels together), "band" (levels per workgroup of
  * the operator kernel, 8 or more; 0 whole columns), "prec_head" (top levels
  * of each column whose Thomas pivots and forward values the preconditioner
- * keeps in registers between its sweeps: 0, 8, 16, 24, 32; FP32 solves use
- * at most 16; -1, the default: 24 from 1024 tiles of 64 columns up in FP64,
- * 16 from 2048 tiles up in FP32, else 0), "zreg" (1, the default: the
+ * keeps in registers between its sweeps: 0, 8, 16, 24, 32; -1, the
+ * default: 24 from 1024 tiles of 64 columns up in FP64, 16 from 2048 tiles
+ * up in FP32, else 0), "zreg" (1, the default: the
  * z-level windows of sunhip_set_dz; 0 off), "order" (1, the default: blocks
  * of four tiles dispatched deepest first within each XCD's run; 0 in tile
  * order), "xchg" (the iteration's operator kernel exchanges neighbour values

@@ -363,6 +363,15 @@ __global__ __launch_bounds__(256) void k_mix_update(Geo G, double *__restrict__ 
   mix_update_tile(G, x, e, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
+// The inner solve's k_qfinish and k_mix_update in one launch (mixed form).
+__global__ __launch_bounds__(256) void k_mix_finish(Geo G, double *__restrict__ x, const float *__restrict__ e,
+                                                    const float *__restrict__ p0, const float *__restrict__ p1,
+                                                    const CGScal *sc) {
+  const int it = sc->iter;
+  mix_finish_tile(G, x, e, (it & 1) ? p1 : p0, (float)sc->nu, it != 0, blockIdx.x * 4 + (threadIdx.x >> 6),
+                  threadIdx.x & 63);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -412,12 +421,11 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
       else k_qprec<UPDATE, U, T, false, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
     } \
   } while (0)
-  // FP32 columns keep at most 16 levels: beyond that the register head costs
-  // more occupancy than it saves traffic
+  // FP32 columns keep 16 levels by default: 24 and 32 measured no faster on
+  // the 5e7-unknown solve (the register head costs occupancy)
   const bool fp32 = std::is_same<T, float>::value;
   int head = g->tune_phead;
   if (head < 0) head = fp32 ? (g->Nt >= 2048 ? 16 : 0) : (g->Nt >= 1024 ? 24 : 0);
-  if (fp32) head = std::min(head, 16);
   if (head == 0) {
     switch (g->tune_punroll) {
       case 1: PREC(1, 0); break;
@@ -649,8 +657,10 @@ int cg_iteration(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int
 // Iterate until the device flag says done; the host polls every few
 // iterations and kernels of a batch that start after convergence return at
 // once.
+// finish = false leaves the last x update to the caller (k_mix_finish).
 template <typename T>
-int cg_loop(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int pk_apply, int pk_prec) {
+int cg_loop(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int pk_apply, int pk_prec,
+            bool finish = true) {
   int every = prm->check_every > 0 ? prm->check_every : (g->NT > (1 << 22) ? 4 : 8);
   if (every & 1) every++;
   int it = 0, rc, slot = 0;
@@ -661,7 +671,7 @@ int cg_loop(sunhip_grid *g, const sunhip_cg_params *prm, QSolver<T> &S, int pk_a
       if ((rc = cg_iteration<T>(g, prm, S, it, pk_apply, pk_prec))) return rc;
     if ((rc = poll_chunk(g, S.sc, slot, pending, &done))) return rc;
   }
-  k_qfinish<T><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
+  if (finish) k_qfinish<T><<<nb4(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -977,12 +987,16 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
   double eps0 = -1, eps = 0;
   int outer = 0, total_inner = 0, converged = 0;
   HIPCHK(hipEventRecord(g->ev1, g->stream));
-  for (;;) {
-    // FP64 true residual, cast to FP32 for the inner solve
-    rc = launch_apply<QM_RESID_DOT, double, float>(g, so, R, Red{g->d_part, 0, nullptr, OP_STORE}, PK_RESID);
-    if (rc) return rc;
+  // FP64 true residual, cast to FP32 for the inner solve; its r.r on the host
+  auto residual = [&]() -> int {
+    int e = launch_apply<QM_RESID_DOT, double, float>(g, so, R, Red{g->d_part, 0, nullptr, OP_STORE}, PK_RESID);
+    if (e) return e;
     HIPCHK(hipMemcpyAsync(g->h_sc, so, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
+    return 0;
+  };
+  if ((rc = residual())) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (;;) {
     eps = g->h_sc->pz;
     if (eps0 < 0) eps0 = eps;
     const double ratio = eps == 0 ? 0.0 : sqrt(eps / eps0);
@@ -992,11 +1006,15 @@ int sunhip_q_solve_mixed(sunhip_grid *g, const sunhip_cg_params *prm, double inn
     // target still needs (with a margin for the FP32/FP64 residual gap)
     inner.epsilon = std::max(inner_tol, 0.25 * prm->epsilon / ratio);
     if ((rc = q_inner_begin(g, &inner, S, nullptr))) return rc;
-    if ((rc = cg_loop<float>(g, &inner, S, PK_MAPPLY, PK_MPREC))) return rc;
-    HIPCHK(hipMemcpyAsync(g->h_sc, S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    if ((rc = cg_loop<float>(g, &inner, S, PK_MAPPLY, PK_MPREC, false))) return rc;
+    // x += e (with the inner solve's last update), the next residual, then
+    // one wait for both the inner count and the residual
+    k_mix_finish<<<nb4(g), 256, 0, g->stream>>>(g->geo(), const_cast<double *>(R.vin), S.x, S.p0, S.p1, S.sc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&g->h_poll[0], S.sc, sizeof(CGScal), hipMemcpyDeviceToHost, g->stream));
+    if ((rc = residual())) return rc;
     HIPCHK(hipStreamSynchronize(g->stream));
-    total_inner += g->h_sc->iter;
-    if ((rc = q_mix_update(g, const_cast<double *>(R.vin), S.x))) return rc;
+    total_inner += g->h_poll[0].iter;
     outer++;
   }
   HIPCHK(hipEventRecord(g->ev2, g->stream));

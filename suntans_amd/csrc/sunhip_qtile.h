@@ -1114,6 +1114,23 @@ __device__ __forceinline__ void mix_update_tile(const Geo &G, double *__restrict
   }
 }
 
+// The inner FP32 solve's last x update (qfinish_tile, when upd) and then
+// x64 += e32 (mix_update_tile) in one pass over tile t, with the same
+// roundings as the two passes.
+__device__ __forceinline__ void mix_finish_tile(const Geo &G, double *__restrict__ x, const float *__restrict__ e,
+                                                const float *__restrict__ p, float nu, bool upd, int t, int l) {
+  if (t >= G.Nt) return;
+  const int s = t * 64 + l, n = G.nk[s];
+  if (n == 0 || G.comp[s] != 1) return;
+  const int64_t base = G.toff[t] + l;
+  for (int k = G.ctop[s]; k < n; k++) {
+    const int64_t idx = base + (int64_t)k * 64;
+    float ek = e[idx];
+    if (upd) ek = ek + nu * p[idx];
+    x[idx] = x[idx] + (double)ek;
+  }
+}
+
 // Fill / cast over tile t's stored levels (padding included).
 template <typename T>
 __device__ __forceinline__ void fill_tile(const Geo &G, T *__restrict__ p, T v, int t, int l) {
