@@ -35,7 +35,7 @@ int dev_alloc(sunhip_grid *g, void **p, size_t bytes) {
 
 Kind kind_of(const char *name) {
   static const char *edge[] = {"dzf", "u", "utmp", "utmp2", "u0", "ue", "SfHp", "SfHm", "ef0", "ef1", "bscal",
-                               "bw"};
+                               "bw", "tvdP1", "tvdP2"};
   static const char *edge2[] = {"D", "df", "dg", "Dq", "Dh", "tau_T", "CdT", "CdB", "dzfB", "def1"};
   static const char *cellw[] = {"Cn_W", "Cn_W2"};
   static const char *cell2[] = {"h", "htmp", "hold", "htmp2", "htmp3", "hcoef", "hr", "hz", "hrt", "hp0", "hp1",

@@ -32,10 +32,10 @@ __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
   tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-template <int NF, int GRP, bool SRC = false>
+template <int NF, int GRP, bool SRC = false, bool PRE = false>
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  update_column_tile<NF, GRP, SRC>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6),
-                                   threadIdx.x & 63);
+  update_column_tile<NF, GRP, SRC, PRE>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6),
+                                        threadIdx.x & 63);
 }
 
 namespace sunhip {
@@ -54,7 +54,7 @@ int tvd_args(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, T
   TvdArgs A{};
   A.s = fs->d; A.u = u->d; A.utmp2 = ut->d; A.dzf = dzf->d; A.sumQ = sq->d; A.sumQC = sqc->d;
   A.oQ = sq->d; A.oQC = sqc->d; A.SfHp = hp->d; A.SfHm = hm->d; A.bscal = bs ? bs->d : nullptr;
-  A.theta = sp->theta; A.TVD = sp->TVD;
+  A.theta = sp->theta; A.dt = sp->dt; A.TVD = sp->TVD;
   *T = A;
   ColArgs C{};
   C.s_old = so->d; C.s = fs->d; C.wnew = wn->d; C.wtmp2 = w2->d; C.dzz = dzz->d; C.dzzold = dzo->d;
@@ -130,7 +130,16 @@ int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, co
     if ((rc = field(g, names[i], &f))) return rc;
     *dst[i] = f->d;
   }
-  if (sp->TVD && (rc = sunhip_face_scalars(g, sp, scal))) return rc;
+  // without source terms the faces pass also forms the column's flux terms
+  const bool srcs = A.src1 || A.src2 || A.Ftop || A.Fbot;
+  const bool pre = sp->TVD && !srcs;
+  if (pre) {
+    Field *p1, *p2;
+    if ((rc = field(g, "tvdP1", &p1) | field(g, "tvdP2", &p2))) return rc;
+    T.P1 = p1->d; T.P2 = p2->d;
+    A.P1 = p1->d; A.P2 = p2->d;
+  }
+  if (sp->TVD && (rc = face_scalars_launch(g, T))) return rc;
   if (sp->TVD) {
     // a column reads old values of its own cells only, all before it writes
     // them: the update runs in place
@@ -144,8 +153,11 @@ int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, co
   with_nf(g->mf, [&](auto nf) {
     constexpr int NF = decltype(nf)::value;
     const unsigned nb = (unsigned)((g->Nt + 3) / 4);
-    if (A.src1 || A.src2 || A.Ftop || A.Fbot)
+    if (srcs)
       k_update_column<NF, 1, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);  // sources: a variant of its own
+    else if (pre && g->tune_tvdgrp == 4) k_update_column<NF, 4, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else if (pre && g->tune_tvdgrp == 2) k_update_column<NF, 2, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else if (pre) k_update_column<NF, 1, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (g->tune_tvdgrp == 4) k_update_column<NF, 4><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (g->tune_tvdgrp == 2) k_update_column<NF, 2><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else k_update_column<NF, 1><<<nb, 256, 0, g->stream>>>(g->geo(), A);

@@ -35,7 +35,10 @@ struct TvdArgs {
   double *oQ, *oQC;
   double *SfHp, *SfHm;             // edge fields
   const double *bscal;             // edge field, type-2 edges (may be null)
-  double theta;
+  // the column update's horizontal flux terms, one per edge-level and side
+  // (null: not formed; see tvd_faces_tile)
+  double *P1, *P2;
+  double theta, dt;
   int TVD;
 };
 
@@ -109,35 +112,93 @@ __device__ __forceinline__ void tvd_sums_tile(const Geo &G, const TvdArgs &A, in
   }
 }
 
-// pass 2 (main/tvd.c:69-119), one lane per edge
+// The horizontal flux term of the scalar column update (main/scalars.c:
+// 283-285, dt*df*normal/Ac*(theta*u+(1-theta)*utmp2)*temp*dzf) at level k
+// of an edge for one of its cells, with the normal taken as +1: the product
+// with normal -1 is the same value negated, exactly, so the column adds
+// normal * P.  c = dt*df/Ac of that cell, w = theta*u+(1-theta)*utmp2, temp
+// SfHp or SfHm by the sign of utmp2.  The faces pass stores it per edge-level
+// for each side (P1: the grad[0] cell's, P2: the grad[1] cell's), and the
+// column then reads one value per face and level instead of u, utmp2, dzf,
+// SfHp and SfHm.
+__device__ __forceinline__ double face_term(double c, double w, double temp, double fz) { return c * w * temp * fz; }
+
+// Side constants of the flux terms of edge slot s (0 for a missing side).
+__device__ __forceinline__ void face_term_consts(const Geo &G, const TvdArgs &A, int s, double &c1, double &c2) {
+  const int s1 = G.e_s1[s], s2 = G.e_s2[s];
+  const double dd = A.dt * G.df[s];
+  c1 = s1 >= 0 ? dd / G.Ac[s1] : 0.0;
+  c2 = s2 >= 0 ? dd / G.Ac[s2] : 0.0;
+}
+
+__device__ __forceinline__ void store_terms(const Geo &G, const TvdArgs &A, int s, int64_t ei, double c1, double c2,
+                                            double w, double ut2, double hp, double hm) {
+  const double temp = ut2 > 0 ? hp : hm, fz = A.dzf[ei];
+  // streamed: the column kernel reads them after this launch
+  if (G.e_s1[s] >= 0) __builtin_nontemporal_store(face_term(c1, w, temp, fz), &A.P1[ei]);
+  if (G.e_s2[s] >= 0) __builtin_nontemporal_store(face_term(c2, w, temp, fz), &A.P2[ei]);
+}
+
+// pass 2 (main/tvd.c:69-119), one lane per edge; with TvdArgs::P1 set, the
+// column's flux terms of each level too (store_terms), from the values in
+// registers.  Edges outside the pass (closed boundaries) get theirs from the
+// stored face values, which the column would otherwise read.
 __device__ __forceinline__ void tvd_faces_tile(const Geo &G, const TvdArgs &A, int t, int l) {
   if (t >= G.Net) return;
   const int s = t * 64 + l, ec = G.e_comp[s];
-  if (ec == 0) return;
   const int64_t base = G.eoff[t] + l;
   const int et = G.e_etop[s], nke = G.e_nke[s];
-  for (int k = 0; k < et; k++) A.SfHp[base + (int64_t)k * 64] = A.SfHm[base + (int64_t)k * 64] = 0;
   const double th = A.theta;
+  const bool terms = A.P1 != nullptr;
+  double c1 = 0, c2 = 0;
+  if (terms) face_term_consts(G, A, s, c1, c2);
+  if (ec == 0) {
+    if (terms)
+      for (int k = 0; k < nke; k++) {
+        const int64_t ei = base + (int64_t)k * 64;
+        const double ut2 = A.utmp2[ei];
+        store_terms(G, A, s, ei, c1, c2, th * A.u[ei] + (1 - th) * ut2, ut2, A.SfHp[ei], A.SfHm[ei]);
+      }
+    return;
+  }
+  for (int k = 0; k < et; k++) {
+    const int64_t ei = base + (int64_t)k * 64;
+    A.SfHp[ei] = A.SfHm[ei] = 0;
+    if (terms) {
+      const double ut2 = A.utmp2[ei];
+      store_terms(G, A, s, ei, c1, c2, th * A.u[ei] + (1 - th) * ut2, ut2, 0.0, 0.0);
+    }
+  }
   if (ec == 1) {
-    const int64_t c1 = G.e_c1[s], c2 = G.e_c2[s];
+    const int64_t e1 = G.e_c1[s], e2 = G.e_c2[s];
     for (int k = et; k < nke; k++) {
       const int64_t ei = base + (int64_t)k * 64;
-      const double un = th * A.u[ei] + (1 - th) * A.utmp2[ei];
-      const double s1 = A.s[c1 + (int64_t)k * 64], s2 = A.s[c2 + (int64_t)k * 64];
-      const int64_t up = (un > 0 ? c2 : c1) + (int64_t)k * 64;
+      const double ut2 = A.utmp2[ei];
+      const double un = th * A.u[ei] + (1 - th) * ut2;
+      const double s1 = A.s[e1 + (int64_t)k * 64], s2 = A.s[e2 + (int64_t)k * 64];
+      const int64_t up = (un > 0 ? e2 : e1) + (int64_t)k * 64;
       const double si = un > 0 ? s2 : s1, sm = un > 0 ? s1 : s2;
       double r = 0;
       const double q = A.sumQ[up];
       if (q != 0 && sm != si) r = A.sumQC[up] / (q * (sm - si));
-      A.SfHp[ei] = s2 + 0.5 * psi(r, A.TVD) * (s1 - s2);
-      A.SfHm[ei] = s1 - 0.5 * psi(r, A.TVD) * (s1 - s2);
+      const double hp = s2 + 0.5 * psi(r, A.TVD) * (s1 - s2);
+      const double hm = s1 - 0.5 * psi(r, A.TVD) * (s1 - s2);
+      A.SfHp[ei] = hp;
+      A.SfHm[ei] = hm;
+      if (terms) store_terms(G, A, s, ei, c1, c2, un, ut2, hp, hm);
     }
   } else {
     const int64_t ib = G.e_c1[s];
     for (int k = et; k < nke; k++) {
       const int64_t ei = base + (int64_t)k * 64;
-      A.SfHp[ei] = A.bscal ? A.bscal[ei] : 0.0;
-      A.SfHm[ei] = A.s[ib + (int64_t)k * 64];
+      const double hp = A.bscal ? A.bscal[ei] : 0.0;
+      const double hm = A.s[ib + (int64_t)k * 64];
+      A.SfHp[ei] = hp;
+      A.SfHm[ei] = hm;
+      if (terms) {
+        const double ut2 = A.utmp2[ei];
+        store_terms(G, A, s, ei, c1, c2, th * A.u[ei] + (1 - th) * ut2, ut2, hp, hm);
+      }
     }
   }
 }
@@ -163,6 +224,9 @@ struct ColArgs {
   // read on faces of type-2 and type-3 edges without a second cell; null:
   // those faces read the cell's own old value, as without boundary_scal
   const double *bcell;
+  // the flux terms formed by the faces pass (TvdArgs::P1, P2), read instead
+  // of u, utmp2, dzf, SfHp and SfHm when the column is instantiated with PRE
+  const double *P1, *P2;
   double theta, dt, kappa;
   int TVD, step, alpha_top, alpha_bot;
 };
@@ -232,6 +296,8 @@ struct FaceC {
   int nke;
   int64_t c1, c2;
   double cf;
+  const double *pp;  // this cell's side of the precomputed flux terms
+  double nd;         // the face's normal sign
 };
 
 template <int NF>
@@ -252,14 +318,17 @@ __device__ __forceinline__ void face_consts(const Geo &G, const ColArgs &A, int 
       fc[f].c1 = c1;
       fc[f].c2 = c2;
       fc[f].cf = A.dt * G.df[es] * G.nrm[f * NS + s] / Ac;
+      fc[f].pp = G.e_s1[es] == s ? A.P1 : A.P2;
+      fc[f].nd = (double)G.nrm[f * NS + s];
     } else {
       fc[f].eb = 0; fc[f].nke = 0; fc[f].c1 = 0; fc[f].c2 = 0; fc[f].cf = 0;
+      fc[f].pp = A.P1; fc[f].nd = 0;
     }
   }
 }
 
 // horizontal TVD flux divergence at level k (main/scalars.c:251-290)
-template <int NF>
+template <int NF, bool PRE = false>
 __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF], int nf, int k) {
   double acc = 0;
   const uint32_t k64 = (uint32_t)k * 64u;
@@ -267,6 +336,10 @@ __device__ __forceinline__ double hflux(const ColArgs &A, const FaceC (&fc)[NF],
   for (int f = 0; f < NF; f++) {
     if (f >= nf || k >= fc[f].nke) continue;
     const uint32_t ei = fc[f].eb + k64;
+    if (PRE) {
+      acc += fc[f].nd * at32(fc[f].pp, ei);
+      continue;
+    }
     const double ut2 = at32(A.utmp2, ei);
     double temp;
     if (A.TVD) {
@@ -290,9 +363,10 @@ template <int NF>
 struct LevelIn {
   double wn, wo, sn, zn, on, qn, g1, g2;  // g1, g2: src1 and src2 at k
   double fu[NF], ft[NF], fz[NF], fa[NF], fb[NF];
+  double fp[NF];  // PRE: the precomputed flux term
 };
 
-template <int NF, bool SRC>
+template <int NF, bool SRC, bool PRE = false>
 __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[NF], int64_t base, int nk, int k,
                                            LevelIn<NF> &L) {
   const int kw = k < nk ? k : nk - 1;  // w-type rows hold nk + 1 levels
@@ -313,6 +387,10 @@ __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[N
     const int ne = fc[f].nke;
     const int ke = k < ne ? k : (ne > 0 ? ne - 1 : 0);
     const uint32_t ei = fc[f].eb + (uint32_t)ke * 64u;
+    if (PRE) {
+      L.fp[f] = at32(fc[f].pp, ei);
+      continue;
+    }
     L.fu[f] = at32(A.u, ei);
     L.ft[f] = at32(A.utmp2, ei);
     L.fz[f] = at32(A.dzf, ei);
@@ -329,13 +407,17 @@ __device__ __forceinline__ void load_level(const ColArgs &A, const FaceC (&fc)[N
 }
 
 // hflux on the loaded values: the same products in the same order
-template <int NF>
+template <int NF, bool PRE = false>
 __device__ __forceinline__ double hflux_in(const ColArgs &A, const FaceC (&fc)[NF], int nf, int k,
                                            const LevelIn<NF> &L) {
   double acc = 0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     if (f >= nf || k >= fc[f].nke) continue;
+    if (PRE) {
+      acc += fc[f].nd * L.fp[f];
+      continue;
+    }
     const double ut2 = L.ft[f];
     const double temp = A.TVD ? (ut2 > 0 ? L.fa[f] : L.fb[f]) : upwind_ref(ut2, L.fa[f], L.fb[f]);
     acc += fc[f].cf * (A.theta * L.fu[f] + (1 - A.theta) * ut2) * temp * L.fz[f];
@@ -347,8 +429,10 @@ __device__ __forceinline__ double hflux_in(const ColArgs &A, const FaceC (&fc)[N
 // walked GRP at a time, every load of a group issued before its arithmetic,
 // and the back substitution reads its rows a group at a time; the values
 // are those of GRP = 1 bit for bit.  SRC: the source terms of ColArgs are
-// read (without it they are taken as absent, and cost no registers).
-template <int NF, int GRP = 1, bool SRC = false>
+// read (without it they are taken as absent, and cost no registers).  PRE:
+// the horizontal flux terms are read from ColArgs::P1/P2 (TVD schemes, the
+// faces pass having formed them), one value per face and level.
+template <int NF, int GRP = 1, bool SRC = false, bool PRE = false>
 __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, nk = G.nk[s];
@@ -397,7 +481,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   // horizontal source of the top row: levels 0..ktop fold into row 0, after
   // the top cell's explicit source dt*src2*dzzold
   double cn0 = hs2 ? dt * A.src2[base + (int64_t)ktop * 64] * dzo.at(ktop) : 0.0;
-  for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux(A, fc, nfs, k);
+  for (int k = 0; k <= ktop && k < nk; k++) cn0 -= hflux<NF, PRE>(A, fc, nfs, k);
 
   // forward: assemble row m (level k = ktop+m) and eliminate
   double bprev = 1, cprev = 0, dprev = 0;
@@ -466,8 +550,8 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
              otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
       }
       // Cn[m] = dt*src2*dzzold (0 without src2) - ap[k]; d[m] += fab*Cn[m] with fab = 1
-      const double cnk = hs2 ? dt * A.src2[base + (int64_t)k * 64] * o0 - hflux(A, fc, nfs, k)
-                             : -hflux(A, fc, nfs, k);
+      const double cnk = hs2 ? dt * A.src2[base + (int64_t)k * 64] * o0 - hflux<NF, PRE>(A, fc, nfs, k)
+                             : -hflux<NF, PRE>(A, fc, nfs, k);
       d += cnk;
       A.Cn[base + (int64_t)k * 64] = cnk;
     }
@@ -494,7 +578,7 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
 #pragma unroll
       for (int j = 0; j < GRP; j++) {
         const int k = ktop + m0 + j;
-        load_level<NF, SRC>(A, fc, base, nk, k < nk ? k : nk - 1, L[j]);
+        load_level<NF, SRC, PRE>(A, fc, base, nk, k < nk ? k : nk - 1, L[j]);
       }
 #pragma unroll
       for (int j = 0; j < GRP; j++) {
@@ -533,8 +617,8 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
           d -= otd * (amO_k * skm + (apO_k - amO) * sk - apO * skp) -
                otd * (bd_k * skm - (bd_k + bd_k1) * sk + bd_k1 * skp);
         }
-        const double cnk = hs2 ? dt * L[j].g2 * o0 - hflux_in(A, fc, nfs, k, L[j])
-                               : -hflux_in(A, fc, nfs, k, L[j]);
+        const double cnk = hs2 ? dt * L[j].g2 * o0 - hflux_in<NF, PRE>(A, fc, nfs, k, L[j])
+                               : -hflux_in<NF, PRE>(A, fc, nfs, k, L[j]);
         d += cnk;
         A.Cn[base + (int64_t)k * 64] = cnk;
         b -= a * cprev / bprev;
