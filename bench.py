@@ -283,7 +283,9 @@ def tvd_leg(st, dg, N, Ek, steps=5):
     wall = (time.perf_counter() - t) / steps
     kt = dg.kernel_times()
     dg.profile(False)
-    nb = {"tvd_sums": 8 * (3 * Ek + 3 * N), "tvd_faces": 8 * (4 * Ek + 3 * N), "tvd_column": 8 * (9 * N + 5 * Ek)}
+    # faces: u, utmp2, dzf, the two cells' values and inflow sums in, SfHp, SfHm and the column's
+    # flux terms (one per edge-level and side) out; column: its nine cell streams and those terms
+    nb = {"tvd_sums": 8 * (3 * Ek + 3 * N), "tvd_faces": 8 * (7 * Ek + 3 * N), "tvd_column": 8 * (9 * N + 2 * Ek)}
     res = {"step_ms": wall * 1e3, "steps_per_s": 1 / wall}
     for k, b in nb.items():
         if k in kt:
