@@ -260,8 +260,8 @@ def traffic_for(kernel: str):
 def tvd_leg(st, dg, N, Ek, steps=5):
     """One TVD superbee scalar step (HorizontalFaceScalars + UpdateScalars)
     on the large grid, all fields resident; per-kernel times against their
-    algorithmic bytes (8 B each): face sums 3E_k + 3N, face values
-    4E_k + 3N, column update 9N + 5E_k."""
+    algorithmic bytes (8 B each): face sums 3E_k + 3N, face values and the
+    column's flux terms 7E_k + 3N, column update 9N + 2E_k."""
     u, w = st.get("u"), st.get("w")
     dg.upload("s", st.get("s"))
     dg.upload("u", u)

@@ -111,12 +111,13 @@ int sunhip_grid_sizes(const sunhip_grid *g, int64_t *out6);
  * device layout. */
 int sunhip_upload(sunhip_grid *g, const char *name, const double *flat);
 int sunhip_download(sunhip_grid *g, const char *name, double *flat);
-/* Pinned host buffer slot (0 or 1) of the context, at least n doubles,
- * for transfers at the full host-link rate.  sunhip_upload from a slot
- * returns once the copy is enqueued (no wait); the next sunhip_pinned of
+/* Pinned host buffer slot (0..SUNHIP_PIN_SLOTS-1) of the context, at least
+ * n doubles, for transfers at the full host-link rate.  sunhip_upload from a
+ * slot returns once the copy is enqueued (no wait); the next sunhip_pinned of
  * that slot waits until the device has read it, so a caller can fill one
- * slot while the other is in flight.  Valid until the next call for the
- * same slot or sunhip_grid_destroy.  No reference counterpart. */
+ * slot while another is in flight.  Valid until the next call for the same
+ * slot or sunhip_grid_destroy.  No reference counterpart. */
+#define SUNHIP_PIN_SLOTS 4
 int sunhip_pinned(sunhip_grid *g, int slot, int64_t n, double **out);
 /* Device-side copy of one field into another of the same kind. */
 int sunhip_copy_field(sunhip_grid *g, const char *dst, const char *src);
@@ -221,12 +222,14 @@ int sunhip_copy_probe(sunhip_grid *g, int64_t nbytes, int reps, double *avg_ms);
  * in flight are on the host (upto at or beyond the length: all of them, and
  * the next one in flight becomes the oldest).  Two downloads in parts may be
  * in flight, into different host buffers: the second's gather and copies
- * then overlap the caller's work on the first.  Uploads and whole downloads
- * need none in flight.
+ * then overlap the caller's work on the first.  Uploads may be issued while
+ * downloads are in flight (they take other staging buffers and another copy
+ * stream); whole downloads need none in flight.
  *
- * The copies run on a stream of the grid's own, through two device staging
- * buffers taken in turn, so one field's copy overlaps the scatter or gather
- * kernel of the field before it. */
+ * The copies run on streams of the grid's own, one per direction, through
+ * device staging buffers taken in turn (two per direction), so one field's
+ * copy overlaps the scatter or gather kernel of the field before it, and a
+ * download overlaps the uploads issued after it. */
 #define SUNHIP_MAXPARTS 16
 int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int64_t off, int64_t n, int last);
 int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int nparts);
@@ -291,6 +294,18 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
  * cell's own value, as the reference does without boundary_scal. */
 int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
                               const char *src2, const char *ftop, const char *fbot, const char *bcell);
+/* No reference counterpart: sunhip_update_scalars (TVD >= 1, no source
+ * terms) in two calls, so that a caller can move data between them.
+ * sunhip_update_scalars_faces runs the faces pass (HorizontalFaceScalars:
+ * "SfHp", "SfHm" and the column's flux terms) from scal, "u", "utmp2", "dzf"
+ * and "bscal", enqueued without a wait; sunhip_update_scalars_column then
+ * runs the column update, which reads "wnew", "wtmp2", "dzz", "dzzold",
+ * "kappa_tv", "active" and the flux terms, also enqueued without a wait (a
+ * download or sunhip_sync waits for it).  The pair gives
+ * the result of sunhip_update_scalars bit for bit provided scal, "u",
+ * "utmp2", "dzf" and "bscal" do not change between the two calls. */
+int sunhip_update_scalars_faces(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
+int sunhip_update_scalars_column(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal);
 
 /* ---- the vertical grid that moves with the free surface -------------------
  * UpdateDZ (main/phys.c:816-968) and SetFluxHeight (:4338-4435) on the

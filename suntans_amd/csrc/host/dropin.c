@@ -44,7 +44,7 @@ typedef struct {
   sunhip_grid *dev;
   double *flat;      /* the pinned slot in use (sunhip_pinned) */
   size_t flat_n;
-  int slot;          /* the slot the next transfer takes */
+  int slot, dslot;   /* the slot the next upload (0, 1) and download (2, 3) takes */
   size_t *cst, *wst, *est;  /* flat offsets of cell (Nk), w-type (Nk+1) and edge (Nkc) rows */
   int *cell_rank, *edge_rank;  /* position of cell i in cellp, of edge j in edgep */
   struct { double *flat; int parts; } dl[2];  /* downloads in flight, oldest first */
@@ -185,12 +185,20 @@ static ctx_t *get_ctx(gridT *g, const char *where) {
   return c;
 }
 
-/* The next pinned slot for a transfer (it waits until the device has read
- * what was last uploaded from it); it becomes c->flat. */
+/* The next pinned slot for an upload, 0 or 1 (it waits until the device has
+ * read what was last uploaded from it); it becomes c->flat. */
 static double *next_buf(ctx_t *c, const char *where) {
   check(sunhip_pinned(c->dev, c->slot, (int64_t)c->flat_n, &c->flat), where);
   c->slot ^= 1;
   return c->flat;
+}
+/* The next pinned slot for a download, 2 or 3, so that downloads in flight
+ * leave the upload slots free. */
+static double *next_dbuf(ctx_t *c, const char *where) {
+  double *p;
+  check(sunhip_pinned(c->dev, 2 + c->dslot, (int64_t)c->flat_n, &p), where);
+  c->dslot ^= 1;
+  return p;
 }
 
 /* Send the moving vertical extents unless they equal the ones last sent. */
@@ -387,7 +395,7 @@ static void upload_edges(ctx_t *c, gridT *g, const char *name, REAL **rows, cons
  * host unpacks the first.  unpack_parts takes them in order. */
 static void download(ctx_t *c, const char *name, const char *where) {
   if (c->dl_n == 2) fatal(where, "a third download in flight");
-  double *flat = next_buf(c, where);
+  double *flat = next_dbuf(c, where);
   if (c->dl_n && c->dl[c->dl_head].flat == flat) fatal(where, "a download into a slot not yet unpacked");
   const int parts = n_parts(c->flat_n);
   check(sunhip_download_parts(c->dev, name, flat, parts), where);
@@ -643,6 +651,16 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
   prop->vertTVD = 1;
   ctx_t *C = get_ctx(grid, W);
   scalar_inputs(C, grid, phys, scal, boundary_scal, W);
+  sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
+  /* TVD without source terms: the faces pass runs as soon as its inputs are
+   * on the device, and its face values come back while the column update's
+   * inputs go over (the link carries both directions at once) */
+  const int split = TVDscheme && !src1 && !src2 && !Ftop && !Fbot;
+  if (split) {
+    check(sunhip_update_scalars_faces(C->dev, &S, "s"), W);
+    download(C, "SfHp", W);
+    download(C, "SfHm", W);
+  }
   upload_cells(C, grid, "wnew", wnew, 1, W);
   upload_cells(C, grid, "wtmp2", phys->wtmp2, 1, W);
   upload_cells(C, grid, "dzz", grid->dzz, 0, W);
@@ -671,14 +689,21 @@ void UpdateScalars(gridT *grid, physT *phys, propT *prop, REAL **wnew, REAL **sc
     }
     check(sunhip_upload(C->dev, "bcell", b), W);
   }
-  sunhip_scalar_params S = {theta, prop->dt, kappa, TVDscheme, prop->n, alpha_top, alpha_bot};
-  check(sunhip_update_scalars_src(C->dev, &S, "s", src1 ? "src1" : NULL, src2 ? "src2" : NULL,
-                                  Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL, bcell ? "bcell" : NULL), W);
   /* Cn: the computational columns, zero in the others */
   const fetch_t F[] = {{"s", cells_job(C, grid, scal, 0, 1)},
                        {"Cn", cells_zero_job(C, grid, Cn, 0, 1)},
                        {"SfHp", face_values_job(C, grid, phys->SfHp)},
                        {"SfHm", face_values_job(C, grid, phys->SfHm)}};
+  if (split) {
+    /* the column update is enqueued before the face values are unpacked */
+    check(sunhip_update_scalars_column(C->dev, &S, "s"), W);
+    unpack_parts(C, F[2].J);
+    unpack_parts(C, F[3].J);
+    fetch(C, F, 2, W);
+    return;
+  }
+  check(sunhip_update_scalars_src(C->dev, &S, "s", src1 ? "src1" : NULL, src2 ? "src2" : NULL,
+                                  Ftop ? "Ftop" : NULL, Fbot ? "Fbot" : NULL, bcell ? "bcell" : NULL), W);
   fetch(C, F, TVDscheme ? 4 : 2, W);
 }
 

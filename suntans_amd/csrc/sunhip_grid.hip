@@ -567,6 +567,7 @@ int sunhip_grid_destroy(sunhip_grid *g) {
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
   if (g->xstream) hipStreamSynchronize(g->xstream);
+  if (g->xdown) hipStreamSynchronize(g->xdown);
   if (g->batch) batch_forget_grid(g->batch, g);
   for (auto &kv : g->fields) hipFree(kv.second.d);
   for (auto &kv : g->fields32) hipFree(kv.second);
@@ -575,19 +576,22 @@ int sunhip_grid_destroy(sunhip_grid *g) {
                   g->d_xcode, g->d_pair, g->d_npair,
                   g->d_dg, g->d_e_c1, g->d_e_c2, g->d_e_s1, g->d_e_s2, g->d_e_nke, g->d_e_nkc,
                   g->d_e_etop, g->d_e_mark, g->d_e_comp, g->d_cstart, g->d_cstartw, g->d_estart,
-                  g->d_cellslot, g->d_edgeslot, g->d_stage[0], g->d_stage[1], g->d_part, g->d_count, g->d_qs, g->d_hs,
+                  g->d_cellslot, g->d_edgeslot, g->d_stage[0], g->d_stage[1], g->d_stage[2], g->d_stage[3], g->d_part, g->d_count, g->d_qs, g->d_hs,
                   g->d_class_tiles, g->d_ms, g->d_mo, g->d_D32, g->d_h32, g->d_uerr, g->d_dz, g->d_e_etopold,
                   g->d_ford, g->d_zdzt, g->d_cwin, g->d_fwin, g->d_cex, g->d_fex, g->d_tflag, g->d_b4perm};
   for (void *p : ptrs) if (p) hipFree(p);
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < SUNHIP_PIN_SLOTS; i++) {
     if (g->h_pin[i]) hipHostFree(g->h_pin[i]);
     if (g->ev_pin[i]) hipEventDestroy(g->ev_pin[i]);
+  }
+  for (int i = 0; i < 4; i++)
     if (g->ev_stage[i]) hipEventDestroy(g->ev_stage[i]);
+  for (int i = 0; i < 2; i++)
     for (int p = 0; p < SUNHIP_MAXPARTS; p++)
       if (g->dl[i].ev_part[p]) hipEventDestroy(g->dl[i].ev_part[p]);
-  }
   if (g->ev_hand) hipEventDestroy(g->ev_hand);
   if (g->xstream) hipStreamDestroy(g->xstream);
+  if (g->xdown) hipStreamDestroy(g->xdown);
   if (g->h_sc) hipHostFree(g->h_sc);
   if (g->h_poll) hipHostFree(g->h_poll);
   for (hipEvent_t e : g->evp) if (e) hipEventDestroy(e);
@@ -654,16 +658,12 @@ int sunhip_set_extents(sunhip_grid *g, const sunhip_extents *X) {
   return 0;
 }
 
-// Take staging buffer g->stage_next (at least n values) for a transfer whose
-// first step runs on stream s: s waits until the buffer's last work is over.
-// The copy stream and its events are made on first use.
-static int take_stage(sunhip_grid *g, size_t n, hipStream_t s, int *out) {
-  if (!g->xstream) {
-    HIPCHK(hipStreamCreateWithFlags(&g->xstream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&g->ev_hand, hipEventDisableTiming));
-    for (int i = 0; i < 2; i++) HIPCHK(hipEventCreateWithFlags(&g->ev_stage[i], hipEventDisableTiming));
-  }
-  const int b = g->stage_next;
+// Take the next staging buffer of direction dir (0 upload, 1 download; at
+// least n values) for a transfer whose first step runs on stream s: s waits
+// until the buffer's last work is over.  The copy streams and their events
+// are made on first use.
+static int take_stage(sunhip_grid *g, size_t n, int dir, hipStream_t s, int *out) {
+  const int b = g->stage_next[dir];
   if (g->stage_n[b] < n) {
     if (g->stage_busy[b]) HIPCHK(hipEventSynchronize(g->ev_stage[b]));
     g->stage_busy[b] = false;
@@ -675,8 +675,17 @@ static int take_stage(sunhip_grid *g, size_t n, hipStream_t s, int *out) {
     g->stage_n[b] = n;
   }
   if (g->stage_busy[b]) HIPCHK(hipStreamWaitEvent(s, g->ev_stage[b], 0));
-  g->stage_next = b ^ 1;
+  g->stage_next[dir] = b ^ 1;
   *out = b;
+  return 0;
+}
+
+static int copy_streams(sunhip_grid *g) {
+  if (g->xstream) return 0;
+  HIPCHK(hipStreamCreateWithFlags(&g->xstream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&g->xdown, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g->ev_hand, hipEventDisableTiming));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreateWithFlags(&g->ev_stage[i], hipEventDisableTiming));
   return 0;
 }
 
@@ -707,7 +716,7 @@ static size_t flat_len(const sunhip_grid *g, Kind k) {
 }
 
 int sunhip_pinned(sunhip_grid *g, int slot, int64_t n, double **out) {
-  if (slot < 0 || slot > 1 || n < 0) {
+  if (slot < 0 || slot >= SUNHIP_PIN_SLOTS || n < 0) {
     set_error("sunhip_pinned: slot %d / size %lld", slot, (long long)n);
     return -1;
   }
@@ -731,7 +740,7 @@ int sunhip_pinned(sunhip_grid *g, int slot, int64_t n, double **out) {
 
 // The pinned slot flat lies in, or -1.
 static int pinned_slot(const sunhip_grid *g, const double *flat, size_t n) {
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < SUNHIP_PIN_SLOTS; i++)
     if (g->h_pin[i] && flat >= g->h_pin[i] && flat + n <= g->h_pin[i] + g->h_pin_n[i]) return i;
   return -1;
 }
@@ -809,14 +818,13 @@ int sunhip_upload(sunhip_grid *g, const char *name, const double *flat) {
   Field *f;
   int rc = field(g, name, &f);
   if (rc) return rc;
-  if ((rc = parts_outstanding(g))) return rc;
   if (g->up_filled) {
     set_error("sunhip_upload: an upload in parts is in progress");
     return -1;
   }
   const size_t n = flat_len(g, f->kind);
   int b;
-  if ((rc = take_stage(g, n, g->xstream, &b))) return rc;
+  if ((rc = copy_streams(g)) || (rc = take_stage(g, n, 0, g->xstream, &b))) return rc;
   HIPCHK(hipMemcpyAsync(g->d_stage[b], flat, n * sizeof(double), hipMemcpyHostToDevice, g->xstream));
   return finish_upload(g, f, b, flat, n);
 }
@@ -826,7 +834,6 @@ int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int
   Field *f;
   int rc = field(g, name, &f);
   if (rc) return rc;
-  if ((rc = parts_outstanding(g))) return rc;
   const int64_t len = (int64_t)flat_len(g, f->kind);
   if (n < 0 || off != g->up_filled || off + n > len || (last && off + n != len)) {
     set_error("sunhip_upload_part %s: part [%lld, %lld) after %lld of %lld values%s", name, (long long)off,
@@ -834,7 +841,8 @@ int sunhip_upload_part(sunhip_grid *g, const char *name, const double *flat, int
     g->up_filled = 0;
     return -1;
   }
-  if (off == 0 && (rc = take_stage(g, (size_t)len, g->xstream, &g->up_stage))) return rc;
+  if (off == 0 && ((rc = copy_streams(g)) || (rc = take_stage(g, (size_t)len, 0, g->xstream, &g->up_stage))))
+    return rc;
   const int b = g->up_stage;
   if (n > 0)
     HIPCHK(hipMemcpyAsync(g->d_stage[b] + off, flat + off, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
@@ -853,13 +861,13 @@ int sunhip_download(sunhip_grid *g, const char *name, double *flat) {
   if ((rc = parts_outstanding(g))) return rc;
   const size_t n = flat_len(g, f->kind);
   int b;
-  if ((rc = take_stage(g, n, g->stream, &b))) return rc;
+  if ((rc = copy_streams(g)) || (rc = take_stage(g, n, 1, g->stream, &b))) return rc;
   gather_field(g, f, g->d_stage[b]);
   HIPCHK(hipGetLastError());
-  if ((rc = hand_over(g, g->stream, g->xstream))) return rc;
-  HIPCHK(hipMemcpyAsync(flat, g->d_stage[b], n * sizeof(double), hipMemcpyDeviceToHost, g->xstream));
-  if ((rc = release_stage(g, b, g->xstream))) return rc;
-  HIPCHK(hipStreamSynchronize(g->xstream));
+  if ((rc = hand_over(g, g->stream, g->xdown))) return rc;
+  HIPCHK(hipMemcpyAsync(flat, g->d_stage[b], n * sizeof(double), hipMemcpyDeviceToHost, g->xdown));
+  if ((rc = release_stage(g, b, g->xdown))) return rc;
+  HIPCHK(hipStreamSynchronize(g->xdown));
   return 0;
 }
 
@@ -882,10 +890,10 @@ int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int np
   }
   const int64_t n = (int64_t)flat_len(g, f->kind);
   int b;
-  if ((rc = take_stage(g, (size_t)n, g->stream, &b))) return rc;
+  if ((rc = copy_streams(g)) || (rc = take_stage(g, (size_t)n, 1, g->stream, &b))) return rc;
   gather_field(g, f, g->d_stage[b]);
   HIPCHK(hipGetLastError());
-  if ((rc = hand_over(g, g->stream, g->xstream))) return rc;
+  if ((rc = hand_over(g, g->stream, g->xdown))) return rc;
   sunhip_grid::Download &D = g->dl[(g->dl_head + g->dl_count) & 1];
   for (int p = 0; p <= nparts; p++) D.part_bound[p] = n * p / nparts;
   for (int p = 0; p < nparts; p++) {
@@ -893,10 +901,10 @@ int sunhip_download_parts(sunhip_grid *g, const char *name, double *flat, int np
     const int64_t a = D.part_bound[p], e = D.part_bound[p + 1];
     if (e > a)
       HIPCHK(hipMemcpyAsync(flat + a, g->d_stage[b] + a, (size_t)(e - a) * sizeof(double), hipMemcpyDeviceToHost,
-                            g->xstream));
-    HIPCHK(hipEventRecord(D.ev_part[p], g->xstream));
+                            g->xdown));
+    HIPCHK(hipEventRecord(D.ev_part[p], g->xdown));
   }
-  if ((rc = release_stage(g, b, g->xstream))) return rc;
+  if ((rc = release_stage(g, b, g->xdown))) return rc;
   D.stage = b;
   D.use = g->stage_uses[b];
   D.nparts = nparts;
@@ -965,6 +973,7 @@ int sunhip_sync(sunhip_grid *g) {
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipStreamSynchronize(g->stream));
   if (g->xstream) HIPCHK(hipStreamSynchronize(g->xstream));
+  if (g->xdown) HIPCHK(hipStreamSynchronize(g->xdown));
   return 0;
 }
 

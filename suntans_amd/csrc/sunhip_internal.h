@@ -156,21 +156,23 @@ struct sunhip_grid {
   int *d_e_nke = nullptr, *d_e_nkc = nullptr, *d_e_etop = nullptr, *d_e_mark = nullptr, *d_e_comp = nullptr;
   int64_t *d_cstart = nullptr, *d_cstartw = nullptr, *d_estart = nullptr;  // per slot flat start, -1 padding
   int *d_cellslot = nullptr, *d_edgeslot = nullptr;  // host index -> slot
-  // Transfers: two device staging buffers taken in turn, the host<->device
-  // copies on a stream of their own, so that one field's copy overlaps the
-  // scatter or gather kernel of the one before it.
-  double *d_stage[2] = {nullptr, nullptr};
-  size_t stage_n[2] = {0, 0};
-  int stage_next = 0;                            // the buffer the next transfer takes
-  hipEvent_t ev_stage[2] = {nullptr, nullptr};   // the last work on buffer b is over
-  bool stage_busy[2] = {false, false};           // ... recorded and maybe not reached yet
-  unsigned stage_uses[2] = {0, 0};               // transfers through buffer b so far
-  hipStream_t xstream = nullptr;                 // the copies (default priority)
-  hipEvent_t ev_hand = nullptr;                  // hand-over between xstream and stream
-  double *h_pin[2] = {nullptr, nullptr};   // pinned host slots (sunhip_pinned)
-  size_t h_pin_n[2] = {0, 0};
-  hipEvent_t ev_pin[2] = {nullptr, nullptr};  // last upload read from the slot
-  bool pin_pending[2] = {false, false};
+  // Transfers: device staging buffers taken in turn, two for uploads (0, 1)
+  // and two for downloads (2, 3), the copies of each direction on a stream of
+  // their own, so that one field's copy overlaps the scatter or gather kernel
+  // of the one before it and uploads overlap downloads.
+  double *d_stage[4] = {};
+  size_t stage_n[4] = {};
+  int stage_next[2] = {0, 2};                    // the buffer the next upload / download takes
+  hipEvent_t ev_stage[4] = {};                   // the last work on buffer b is over
+  bool stage_busy[4] = {};                       // ... recorded and maybe not reached yet
+  unsigned stage_uses[4] = {};                   // transfers through buffer b so far
+  hipStream_t xstream = nullptr;                 // host-to-device copies (default priority)
+  hipStream_t xdown = nullptr;                   // device-to-host copies (default priority)
+  hipEvent_t ev_hand = nullptr;                  // hand-over between a copy stream and stream
+  double *h_pin[SUNHIP_PIN_SLOTS] = {};      // pinned host slots (sunhip_pinned)
+  size_t h_pin_n[SUNHIP_PIN_SLOTS] = {};
+  hipEvent_t ev_pin[SUNHIP_PIN_SLOTS] = {};  // last upload read from the slot
+  bool pin_pending[SUNHIP_PIN_SLOTS] = {};
   // transfers in parts (sunhip_upload_part, sunhip_download_parts)
   int64_t up_filled = 0;                     // values of the upload in progress copied so far
   int up_stage = 0;                          // its staging buffer

@@ -109,8 +109,13 @@ int sunhip_update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
   return sunhip_update_scalars_src(g, sp, scal, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
-int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
-                              const char *src2, const char *ftop, const char *fbot, const char *bcell) {
+}  // extern "C"
+
+// The scalar step in phases: 1 the faces pass (TVD >= 1; with the column's
+// flux terms when there are no source terms), 2 the column update (after
+// phase 1 when TVD >= 1 and there are no source terms), 3 both.
+static int update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
+                          const char *src2, const char *ftop, const char *fbot, const char *bcell, int phases) {
   HIPCHK(hipSetDevice(g->device));
   TvdArgs T;
   ColArgs A;
@@ -139,7 +144,8 @@ int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, co
     T.P1 = p1->d; T.P2 = p2->d;
     A.P1 = p1->d; A.P2 = p2->d;
   }
-  if (sp->TVD && (rc = face_scalars_launch(g, T))) return rc;
+  if ((phases & 1) && sp->TVD && (rc = face_scalars_launch(g, T))) return rc;
+  if (!(phases & 2)) return 0;
   if (sp->TVD) {
     // a column reads old values of its own cells only, all before it writes
     // them: the update runs in place
@@ -165,8 +171,31 @@ int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, co
   });
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g->stream));
+  if (phases == 3) HIPCHK(hipStreamSynchronize(g->stream));
   return 0;
+}
+
+extern "C" {
+
+int sunhip_update_scalars_src(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal, const char *src1,
+                              const char *src2, const char *ftop, const char *fbot, const char *bcell) {
+  return update_scalars(g, sp, scal, src1, src2, ftop, fbot, bcell, 3);
+}
+
+int sunhip_update_scalars_faces(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
+  if (!sp || !sp->TVD) {
+    set_error("sunhip_update_scalars_faces: TVD schemes only (TVD >= 1)");
+    return -1;
+  }
+  return update_scalars(g, sp, scal, nullptr, nullptr, nullptr, nullptr, nullptr, 1);
+}
+
+int sunhip_update_scalars_column(sunhip_grid *g, const sunhip_scalar_params *sp, const char *scal) {
+  if (!sp || !sp->TVD) {
+    set_error("sunhip_update_scalars_column: TVD schemes only (TVD >= 1)");
+    return -1;
+  }
+  return update_scalars(g, sp, scal, nullptr, nullptr, nullptr, nullptr, nullptr, 2);
 }
 
 }  // extern "C"
