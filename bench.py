@@ -478,7 +478,9 @@ def dropin_large_leg(st, dg, resident_ms):
     state: CGSolveQ, CGSolve and UpdateScalars per call, each = pack the
     reference's ragged arrays, copy to HBM, solve, copy back and unpack.
     Beside each: the resident device time of the same call and the PCIe
-    floor (the bytes the call moves / the measured pinned copy rates).
+    floor (the bytes the call moves / the measured pinned copy rates, one
+    direction after the other: a call whose downloads overlap its uploads,
+    as UpdateScalars' face values do, can come in under it).
     Bytes per call (8 B each): CGSolveQ in q, src, dzz (N each), dzf (E_k),
     D (Ne), out q (N); CGSolve in h, htmp, D (Nc, Nc, Ne), out h (Nc);
     UpdateScalars in s, dzz, dzzold, kappa_tv (N), wnew, wtmp2 (N + Nc), u,
