@@ -248,7 +248,9 @@ __global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restri
   for (int w = first_item(); w < n; w += stride) {
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
-    update_column_tile<NF, 2>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+    // members with the faces pass's flux terms read those (TVD >= 1)
+    if (B.C.P1) update_column_tile<NF, 2, false, true>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+    else update_column_tile<NF, 2>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
 
@@ -1091,6 +1093,15 @@ int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **gs, int n, const 
     if ((rc = sunhip_batch_attach(b, g))) return rc;
     tab[i].G = g->geo();
     if ((rc = tvd_args(g, sp + i, scal, &tab[i].T, &tab[i].C))) return rc;
+    if (sp[i].TVD) {
+      // as sunhip_update_scalars: the faces pass forms the column's flux
+      // terms, and the column update runs in place
+      Field *p1, *p2;
+      if ((rc = field(g, "tvdP1", &p1) | field(g, "tvdP2", &p2))) return rc;
+      tab[i].T.P1 = p1->d; tab[i].T.P2 = p2->d;
+      tab[i].C.P1 = p1->d; tab[i].C.P2 = p2->d;
+      tab[i].C.s_old = tab[i].C.s;
+    }
     for (int t = 0; t < g->Nt; t++) wc.push_back(make_int2(i, t));
     for (int t = 0; t < g->Net; t++) we.push_back(make_int2(i, t));
   }
@@ -1118,8 +1129,9 @@ int sunhip_batch_update_scalars(sunhip_batch *b, sunhip_grid **gs, int n, const 
   if (ne) kb_tvd_faces<<<(ne + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wte.d, ne);
   HIPCHK(hipGetLastError());
   for (int i = 0; i < n; i++)
-    HIPCHK(hipMemcpyAsync(const_cast<double *>(tab[i].C.s_old), tab[i].C.s, gs[i]->fields.at(scal).n * sizeof(double),
-                          hipMemcpyDeviceToDevice, b->stream));
+    if (!sp[i].TVD)  // first-order upwind reads the neighbours' old values: keep them in stmp
+      HIPCHK(hipMemcpyAsync(const_cast<double *>(tab[i].C.s_old), tab[i].C.s,
+                            gs[i]->fields.at(scal).n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
   with_nf(mf, [&](auto nf) {
     kb_update_column<decltype(nf)::value><<<(nc + BW - 1) / BW, 256, 0, b->stream>>>(b->ttvd.d, b->wtc.d, nc);
     return 0;
