@@ -305,10 +305,13 @@ __global__ __launch_bounds__(HBS) void k_hprec_it(Geo G, const T *__restrict__ c
   h_partials_it<2>(v, pout);
 }
 
-// Apply the pending step in place (one block): the scalars the host reads.
-__global__ __launch_bounds__(HBS) void k_hsettle(CGScal *sc, const double *pin, int nparts) {
+// Apply the pending step in place (one block): the scalars the host reads,
+// also written to host (pinned host memory, may be null) so that polling
+// needs no copy.
+__global__ __launch_bounds__(HBS) void k_hsettle(CGScal *sc, const double *pin, int nparts, CGScal *host) {
   CGScal S;
   h_state<2>(sc, sc, pin, nparts, 0, S);
+  if (host && threadIdx.x == 0) *host = S;
 }
 
 // r -= nu*z ; rtmp = r/hcoef (hprecond=1) or r ; dots.
@@ -427,11 +430,28 @@ int h_iteration(sunhip_grid *g, HSolver<T> &S, int it, int pk_a, int pk_p) {
   return 0;
 }
 
-// The last iteration's HOP_ITER applied to S.sc[0].
-int h_settle(sunhip_grid *g, CGScal *sc) {
+// The last iteration's HOP_ITER applied to S.sc[0]; with host, the result
+// also lands there.
+int h_settle(sunhip_grid *g, CGScal *sc, CGScal *host = nullptr) {
   const int nb = nbh_it(g);
-  k_hsettle<<<1, HBS, 0, g->stream>>>(sc, g->d_part + nb, nb);
+  k_hsettle<<<1, HBS, 0, g->stream>>>(sc, g->d_part + nb, nb, host);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// poll_chunk for the free surface: the settle kernel writes the scalars
+// into the pinned poll slot itself
+int h_poll(sunhip_grid *g, CGScal *sc, int &slot, bool &pending, bool *done) {
+  int rc = h_settle(g, sc, &g->h_poll[slot]);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(g->evp[slot], g->stream));
+  *done = false;
+  if (pending) {
+    HIPCHK(hipEventSynchronize(g->evp[slot ^ 1]));
+    *done = g->h_poll[slot ^ 1].done != 0;
+  }
+  pending = true;
+  slot ^= 1;
   return 0;
 }
 
@@ -445,8 +465,7 @@ int h_loop(sunhip_grid *g, const sunhip_cg_params *prm, HSolver<T> &S, int pk_a,
   while (!done && it < prm->maxiters) {
     for (int b = 0; b < every && it < prm->maxiters; b++, it++)
       if ((rc = h_iteration<T>(g, S, it, pk_a, pk_p))) return rc;
-    if ((rc = h_settle(g, S.sc))) return rc;
-    if ((rc = poll_chunk(g, S.sc, slot, pending, &done))) return rc;
+    if ((rc = h_poll(g, S.sc, slot, pending, &done))) return rc;
   }
   k_hfinish<T><<<nbh(g), 256, 0, g->stream>>>(g->geo(), S.x, S.p0, S.p1, S.sc);
   HIPCHK(hipGetLastError());
