@@ -22,6 +22,9 @@ __global__ __launch_bounds__(256) void k_upred_cells(Geo G, UPArgs A) {
 __global__ __launch_bounds__(256) void k_upred_edges(Geo G, UPArgs A) {
   upred_edge_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
+__global__ __launch_bounds__(256) void k_upred_hterm(Geo G, UPArgs A) {
+  upred_hterm_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
 template <int NF>
 __global__ __launch_bounds__(256) void k_upred_htmp(Geo G, UPArgs A) {
   upred_htmp_tile<NF>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
@@ -43,11 +46,11 @@ static int upred_args(sunhip_grid *g, const sunhip_upred_params *p, UPArgs *out)
       {"up_c", nullptr}, {"up_bp", nullptr}, {"up_ep", nullptr}, {"D", nullptr}, {"tau_T", nullptr},
       {"CdT", nullptr}, {"CdB", nullptr}, {"dzz", nullptr}, {"nu_tv", nullptr}, {"nu_lax", nullptr},
       {"w", nullptr}, {"wtmp2", nullptr}, {"s", nullptr}, {"h", nullptr}, {"htmp", nullptr},
-      {"active", nullptr}, {"hcorr", nullptr}, {"dv", nullptr}};
+      {"active", nullptr}, {"hcorr", nullptr}, {"dv", nullptr}, {"up_hterm", nullptr}};
   UPArgs A{};
   double **dst[] = {&A.u, &A.utmp, &A.utmp2, &A.E, (double **)&A.dzf, &A.sc, &A.sbp, &A.sep, &A.D, (double **)&A.tau_T, (double **)&A.CdT, (double **)&A.CdB,
                     (double **)&A.dzz, (double **)&A.nu_tv, (double **)&A.nu_lax, (double **)&A.w, &A.wtmp2, &A.s,
-                    &A.h, &A.htmp, &A.active, &A.hcorr, (double **)&A.dv};
+                    &A.h, &A.htmp, &A.active, &A.hcorr, (double **)&A.dv, &A.hterm};
   for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) {
     Field *fp;
     int rc = field(g, f[i].name, &fp);
@@ -90,6 +93,7 @@ int sunhip_u_solve(sunhip_grid *g, const sunhip_upred_params *p, const sunhip_cg
   int rc = upred_args(g, p, &A);
   if (rc) return rc;
   const Geo G = g->geo();
+  k_upred_hterm<<<(g->Net + 3) / 4, 256, 0, g->stream>>>(G, A);
   with_nf(g->mf, [&](auto nf) {
     k_upred_htmp<decltype(nf)::value><<<(g->Nt + 3) / 4, 256, 0, g->stream>>>(G, A);
     return 0;

@@ -21,6 +21,7 @@ struct UPArgs {
   double *u, *utmp, *utmp2, *E;
   const double *dzf;
   double *sc, *sbp, *sep;  // per-edge scratch (up_*): upper diagonal, eliminated diagonal and ones
+  double *hterm;           // per edge-level: the free-surface source term, see upred_hterm_tile
   // edge 2-D (by edge slot)
   double *D;
   const double *tau_T, *CdT, *CdB;
@@ -240,25 +241,40 @@ __device__ __forceinline__ void upred_edge_tile(const Geo &G, const UPArgs &A, i
   nan_check();
 }
 
-// htmp = Ac h - dt * sum of theta-weighted face fluxes (main/phys.c:2955-2971).
+// The term of htmp's face sum at every level of edge slot s (main/phys.c:
+// 2965-2967, ((1-theta)*u+theta*utmp)*df*normal*dzf) with the normal taken
+// as +1: with -1 it is the same value negated, exactly, so a cell adds
+// normal * hterm.  One lane per edge walks its levels (coalesced), and the
+// cell sum reads one value per face and level instead of gathering u, utmp
+// and dzf.
+__device__ __forceinline__ void upred_hterm_tile(const Geo &G, const UPArgs &A, int t, int l) {
+  if (t >= G.Net) return;
+  const int s = t * 64 + l, ne = G.e_nke[s];
+  if (ne == 0) return;
+  const uint32_t base = (uint32_t)(G.eoff[t] + l);
+  const double th = A.theta, df = G.df[s];
+  for (int k = G.e_etop[s]; k < ne; k++) {
+    const uint32_t ei = base + (uint32_t)k * 64u;
+    at32(A.hterm, ei) = ((1 - th) * at32(A.u, ei) + th * at32(A.utmp, ei)) * df * at32(A.dzf, ei);
+  }
+}
+
+// htmp = Ac h - dt * sum of theta-weighted face fluxes (main/phys.c:2955-2971),
+// the faces' terms added in the reference's order (faces, then levels).
 template <int NF>
 __device__ __forceinline__ void upred_htmp_tile(const Geo &G, const UPArgs &A, int t, int l) {
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s];
   if (n == 0 || G.comp[s] != 1) return;
   const int NS = G.NS, nf = G.nf[s];
-  const double th = A.theta;
   double sum = 0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     if (f < nf) {
       const int es = G.es[f * NS + s];
       const uint32_t eb = (uint32_t)G.eb[f * NS + s];
-      const double df = G.df[es], nrm = (double)G.nrm[f * NS + s];
-      for (int k = G.e_etop[es]; k < G.e_nke[es]; k++) {
-        const uint32_t ei = eb + (uint32_t)k * 64u;
-        sum += ((1 - th) * at32(A.u, ei) + th * at32(A.utmp, ei)) * df * nrm * at32(A.dzf, ei);
-      }
+      const double nrm = (double)G.nrm[f * NS + s];
+      for (int k = G.e_etop[es]; k < G.e_nke[es]; k++) sum += nrm * at32(A.hterm, eb + (uint32_t)k * 64u);
     }
   }
   A.htmp[s] = G.Ac[s] * A.h[s] - A.dt * sum;
