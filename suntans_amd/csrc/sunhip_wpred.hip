@@ -227,14 +227,19 @@ __global__ __launch_bounds__(256) void k_wp_column(Geo G, WPArgs A) {
   const bool multi = n - ct > 1;
   double zp = 0, sp = 0;       // dzz and stmp of the level above
   double cp = 0, bp = 0, dp = 0;  // eliminated diagonal, scaled upper diagonal and right-hand side above
+  // w at k-1, k, k+1, q and the viscosity of the level above slide down the
+  // column in registers: each is loaded once (the values and their order of
+  // operations are those of loading them at every use)
+  double wm = 0, w0 = R(A.w, ct), qp = 0, visp = 0;
   for (int k = ct; k < n; k++) {
     const double z = R(A.dzz, k), st = R(A.stmp, k);
-    const double cn = R(A.Cn, k);
-    double wt = R(A.w, k) + A.f2 * cn + A.f3 * R(A.Cn2, k);
+    const double cn = R(A.Cn, k), c2 = R(A.Cn2, k), qk = R(A.q, k);
+    const double w1 = R(A.w, k + 1);  // w-type rows hold n + 1 levels
+    double wt = w0 + A.f2 * cn + A.f3 * c2;
     X(A.Cn2, k) = cn;
     double c = 0;
-    if (k > ct) wt -= 2.0 * dt / (zp + z) * (R(A.q, k - 1) - R(A.q, k));
-    else wt += 2.0 * dt / z * R(A.q, k);
+    if (k > ct) wt -= 2.0 * dt / (zp + z) * (qp - qk);
+    else wt += 2.0 * dt / z * qk;
     if (k > ct) c -= dt * (zp * sp + z * st) / (zp + z);
     else c -= dt * st;
     X(A.Cn, k) = c;
@@ -243,14 +248,15 @@ __global__ __launch_bounds__(256) void k_wp_column(Geo G, WPArgs A) {
       double a, b;
       const double vis = nu + lw * R(A.nu_lax, k) + R(A.nu_tv, k);
       if (k > ct) {
-        a = 2 * (nu + lw * R(A.nu_lax, k - 1) + R(A.nu_tv, k - 1)) / zp / (z + zp);
+        a = 2 * visp / zp / (z + zp);
         b = 2 * vis / z / (z + zp);
-        wt += dt * (1 - th) * (a * R(A.w, k - 1) - (a + b) * R(A.w, k) + b * R(A.w, k + 1));
+        wt += dt * (1 - th) * (a * wm - (a + b) * w0 + b * w1);
       } else {
         b = vis / (z * z);
         a = b;
-        wt += dt * (1 - th) * (-(a + b) * R(A.w, k) + (a + b) * R(A.w, k + 1));
+        wt += dt * (1 - th) * (-(a + b) * w0 + (a + b) * w1);
       }
+      visp = vis;
       double cd = 1 + dt * th * (a + b);
       const double as = a * (-dt * th);
       double bs = b * (-dt * th);
@@ -265,6 +271,7 @@ __global__ __launch_bounds__(256) void k_wp_column(Geo G, WPArgs A) {
     }
     X(A.wtmp, k) = wt;
     zp = z; sp = st;
+    qp = qk; wm = w0; w0 = w1;
   }
   if (!multi) {
     for (int k = ct; k < n; k++) X(A.w, k) = R(A.wtmp, k);
