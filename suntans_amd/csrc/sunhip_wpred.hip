@@ -231,10 +231,16 @@ __global__ __launch_bounds__(256) void k_wp_column(Geo G, WPArgs A) {
   // column in registers: each is loaded once (the values and their order of
   // operations are those of loading them at every use)
   double wm = 0, w0 = R(A.w, ct), qp = 0, visp = 0;
+  // the next level's loads are issued at the top of each level, before its
+  // stores, so they are in flight while it computes
+  double zN = R(A.dzz, ct), stN = R(A.stmp, ct), cnN = R(A.Cn, ct), c2N = R(A.Cn2, ct), qN = R(A.q, ct);
+  double w1N = R(A.w, ct + 1), nlN = R(A.nu_lax, ct), ntN = R(A.nu_tv, ct);  // w-type rows hold n + 1 levels
   for (int k = ct; k < n; k++) {
-    const double z = R(A.dzz, k), st = R(A.stmp, k);
-    const double cn = R(A.Cn, k), c2 = R(A.Cn2, k), qk = R(A.q, k);
-    const double w1 = R(A.w, k + 1);  // w-type rows hold n + 1 levels
+    const double z = zN, st = stN, cn = cnN, c2 = c2N, qk = qN, w1 = w1N, nlk = nlN, ntk = ntN;
+    if (k + 1 < n) {
+      zN = R(A.dzz, k + 1); stN = R(A.stmp, k + 1); cnN = R(A.Cn, k + 1); c2N = R(A.Cn2, k + 1);
+      qN = R(A.q, k + 1); w1N = R(A.w, k + 2); nlN = R(A.nu_lax, k + 1); ntN = R(A.nu_tv, k + 1);
+    }
     double wt = w0 + A.f2 * cn + A.f3 * c2;
     X(A.Cn2, k) = cn;
     double c = 0;
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(256) void k_wp_column(Geo G, WPArgs A) {
     wt += A.f1 * c;
     if (multi) {
       double a, b;
-      const double vis = nu + lw * R(A.nu_lax, k) + R(A.nu_tv, k);
+      const double vis = nu + lw * nlk + ntk;
       if (k > ct) {
         a = 2 * visp / zp / (z + zp);
         b = 2 * vis / z / (z + zp);
