@@ -976,7 +976,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
           const uint32_t idx = base + (uint32_t)k * 64u;
           const T ck = cg[jj], ck1 = cg[jj + 1];
           T rk = rg[jj];
-          if (UPDATE) { rk = rk - nu * zg[jj]; at32(r, idx) = rk; }
+          if (UPDATE) { rk = rk - nu * zg[jj]; st_stream<true>(&at32(r, idx), rk); }
           rr += (double)rk * (double)rk;
           T a = 0, b, c = 0;
           if (k == kt) { b = -2 * ck - ck1; c = ck1; }
@@ -1013,7 +1013,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       if (k < n) {
         const uint32_t idx = base + (uint32_t)k * 64u;
         T rk = rv[j];
-        if (UPDATE) { rk = rk - nu * zv[j]; at32(r, idx) = rk; }
+        if (UPDATE) { rk = rk - nu * zv[j]; st_stream<true>(&at32(r, idx), rk); }
         rr += (double)rk * (double)rk;
         const T ck = cv[j], ck1 = cv[j + 1];
         T a = 0, b, c = 0;
@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       if (k >= klo) {
         const T b = bv[j], d = dv[j];
         const T u = (k == n - 1) ? d / b : d / b - cnext * unext / b;
-        at32(out, base + (uint32_t)k * 64u) = u;
+        st_stream<true>(&at32(out, base + (uint32_t)k * 64u), u);
         if constexpr (!RZID) rz += (double)rv[j] * (double)u;
         unext = u;
         cnext = cv[j];  // c at level k-1 is coef[k]
@@ -1062,7 +1062,7 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       if (k < n) {
         const T b = hb[j], d = hd[j];
         const T u = (k == n - 1) ? d / b : d / b - hc[j] * unext / b;
-        at32(out, base + (uint32_t)k * 64u) = u;
+        st_stream<true>(&at32(out, base + (uint32_t)k * 64u), u);
         if constexpr (!RZID) rz += (double)hr[j] * (double)u;
         unext = u;
       }
