@@ -184,8 +184,11 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD, 3 on
  * with 2 levels per load group, 4-6 the pipelined form; -1, the default: 2
  * from 4096 tiles up, else 0), "tvd_group" (levels per load group of the
- * scalar step's column update: 1, 2 (the default) or 4), "tvd_sums_group"
- * (the same for the TVD inflow sums: 1, 2 or 4, the default), "h_blocks"
+ * scalar step's column update: 1, 2 (the default) or 4), "tvd_head" (Thomas
+ * rows below the top row that the TVD column update keeps in registers
+ * between its sweeps, with load groups of 2: 0, 8 (the default) or 16),
+ * "tvd_sums_group" (the same as "tvd_group" for the TVD inflow sums: 1, 2 or
+ * 4, the default), "h_blocks"
  * (most blocks of the free-surface iteration kernels, each adding all the
  * others' sums: 256, 512, 1024 (the default) or 2048).  Only "tiles",
  * "reduce", "rzid", "band", "order" and "h_blocks" change the rounding of a

@@ -1108,6 +1108,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
     if (value != 1 && value != 2 && value != 4) { set_error("tvd_group must be 1, 2 or 4"); return -1; }
     g->tune_tvdgrp = value;
   }
+  else if (!strcmp(key, "tvd_head")) {
+    if (value != 0 && value != 8 && value != 16) { set_error("tvd_head must be 0, 8 or 16"); return -1; }
+    g->tune_tvdhead = value;
+  }
   else if (!strcmp(key, "tvd_sums_group")) {
     if (value != 1 && value != 2 && value != 4) { set_error("tvd_sums_group must be 1, 2 or 4"); return -1; }
     g->tune_sumsgrp = value;

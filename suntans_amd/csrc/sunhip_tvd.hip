@@ -32,10 +32,10 @@ __global__ __launch_bounds__(256) void k_tvd_faces(Geo G, TvdArgs A) {
   tvd_faces_tile(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
-template <int NF, int GRP, bool SRC = false, bool PRE = false>
+template <int NF, int GRP, bool SRC = false, bool PRE = false, int KR = 0>
 __global__ __launch_bounds__(256) void k_update_column(Geo G, ColArgs A) {
-  update_column_tile<NF, GRP, SRC, PRE>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6),
-                                        threadIdx.x & 63);
+  update_column_tile<NF, GRP, SRC, PRE, KR>(G, A, xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6),
+                                            threadIdx.x & 63);
 }
 
 namespace sunhip {
@@ -162,6 +162,10 @@ static int update_scalars(sunhip_grid *g, const sunhip_scalar_params *sp, const 
     if (srcs)
       k_update_column<NF, 1, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);  // sources: a variant of its own
     else if (pre && g->tune_tvdgrp == 4) k_update_column<NF, 4, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else if (pre && g->tune_tvdgrp == 2 && g->tune_tvdhead == 16)
+      k_update_column<NF, 2, false, true, 16><<<nb, 256, 0, g->stream>>>(g->geo(), A);
+    else if (pre && g->tune_tvdgrp == 2 && g->tune_tvdhead == 8)
+      k_update_column<NF, 2, false, true, 8><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (pre && g->tune_tvdgrp == 2) k_update_column<NF, 2, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (pre) k_update_column<NF, 1, false, true><<<nb, 256, 0, g->stream>>>(g->geo(), A);
     else if (g->tune_tvdgrp == 4) k_update_column<NF, 4><<<nb, 256, 0, g->stream>>>(g->geo(), A);

@@ -432,8 +432,9 @@ __device__ __forceinline__ double hflux_in(const ColArgs &A, const FaceC (&fc)[N
 // read (without it they are taken as absent, and cost no registers).  PRE:
 // the horizontal flux terms are read from ColArgs::P1/P2 (TVD schemes, the
 // faces pass having formed them), one value per face and level.
-template <int NF, int GRP = 1, bool SRC = false, bool PRE = false>
+template <int NF, int GRP = 1, bool SRC = false, bool PRE = false, int KR = 0>
 __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &A, int t, int l) {
+  static_assert(KR == 0 || (GRP > 1 && KR % GRP == 0), "a register head needs load groups that divide it");
   if (t >= G.Nt) return;
   const int s = t * 64 + l, nk = G.nk[s];
   if (nk == 0 || G.comp[s] != 1) return;
@@ -491,6 +492,12 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
   ap_am_old(ktop + 1, apO, amO);
   double bd_k1 = (ktop + 1 < nk) ? bd() : 0.0;  // bd at k+1
   double apN_k = 0, amN_k = 0, apO_k = 0, amO_k = 0, bd_k = 0;
+  // KR > 0: rows 0..KR (the top KR+1 levels) keep b, c and d in registers
+  // between the sweeps, so they never go to tb/tc/td; the arithmetic and its
+  // order are the same for every KR
+  constexpr int KH = KR > 0 ? KR : 1;
+  double hb[KH], hc[KH], hd[KH];
+  double h0b = 1, h0c = 0, h0d = 0;
   for (int m = 0; m <= (GRP > 1 ? 0 : nb); m++) {
     const int k = ktop + m;
     // shift the level-(k+1) quantities to level k, compute level k+1
@@ -560,9 +567,13 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
       d -= a * dprev / bprev;
     }
     const int64_t idx = base + (int64_t)k * 64;
-    A.tb[idx] = b;
-    A.tc[idx] = c;
-    A.td[idx] = d;
+    if (KR > 0) {
+      h0b = b; h0c = c; h0d = d;
+    } else {
+      A.tb[idx] = b;
+      A.tc[idx] = c;
+      A.td[idx] = d;
+    }
     bprev = b; cprev = c; dprev = d;
     // slide the windows down one level
     s0 = s1; s1 = s2; s2 = s3; s3 = so.at(k + 3);
@@ -571,9 +582,9 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
     q0 = q1; q1 = kap(k + 2);
   }
   if (GRP > 1) {
-    // rows m = 1..nb, GRP levels at a time (the body of the loop above for
-    // m > 0, on loaded values)
-    for (int m0 = 1; m0 <= nb; m0 += GRP) {
+    // rows m0..m0+GRP-1 (up to nb) on loaded values (the body of the loop
+    // above for m > 0); put(j, idx, b, c, d) keeps row m0+j
+    auto group = [&](int m0, auto &&put) {
       LevelIn<NF> L[GRP];
 #pragma unroll
       for (int j = 0; j < GRP; j++) {
@@ -623,31 +634,51 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
         A.Cn[base + (int64_t)k * 64] = cnk;
         b -= a * cprev / bprev;
         d -= a * dprev / bprev;
-        const int64_t idx = base + (int64_t)k * 64;
-        A.tb[idx] = b;
-        A.tc[idx] = c;
-        A.td[idx] = d;
+        put(j, base + (int64_t)k * 64, b, c, d);
         bprev = b; cprev = c; dprev = d;
         s0 = s1; s1 = s2; s2 = s3; s3 = L[j].sn;
         z0 = z1; z1 = L[j].zn;
         o0 = o1; o1 = L[j].on;
         q0 = q1; q1 = L[j].qn;
       }
+    };
+    if constexpr (KR > 0) {
+      // rows 1..KR: unrolled, so that every head index is a constant
+#pragma unroll
+      for (int g0 = 0; g0 < KR; g0 += GRP) {
+        if (1 + g0 > nb) break;
+        group(1 + g0, [&](int j, int64_t, double b, double c, double d) {
+          hb[g0 + j] = b;
+          hc[g0 + j] = c;
+          hd[g0 + j] = d;
+        });
+      }
     }
+    for (int m0 = 1 + KR; m0 <= nb; m0 += GRP)
+      group(m0, [&](int, int64_t idx, double b, double c, double d) {
+        A.tb[idx] = b;
+        A.tc[idx] = c;
+        A.td[idx] = d;
+      });
   }
   // Cn: zero above and at ktop, the folded top value on [ctop, ktop]
   for (int k = 0; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = 0;
   for (int k = ct; k <= ktop && k < nk; k++) A.Cn[base + (int64_t)k * 64] = cn0 / (1 + abs(ct - ktop));
-  // back substitution into scal[ktop..Nk-1]
+  // back substitution into scal[ktop..Nk-1]: the rows in memory (levels
+  // klo..Nk-1), then those of the register head
   if (nb > 0) {
-    double un = A.td[base + (int64_t)(nk - 1) * 64] / A.tb[base + (int64_t)(nk - 1) * 64];
-    A.s[base + (int64_t)(nk - 1) * 64] = un;
+    const int klo = KR > 0 ? ktop + 1 + KR : ktop;
+    double un = 0;
+    if (nk - 1 >= klo) {
+      un = A.td[base + (int64_t)(nk - 1) * 64] / A.tb[base + (int64_t)(nk - 1) * 64];
+      A.s[base + (int64_t)(nk - 1) * 64] = un;
+    }
     if (GRP > 1) {
-      for (int k0 = nk - 2; k0 >= ktop; k0 -= GRP) {
+      for (int k0 = nk - 2; k0 >= klo; k0 -= GRP) {
         double rb[GRP], rc[GRP], rd[GRP];
 #pragma unroll
         for (int j = 0; j < GRP; j++) {
-          const int k = k0 - j >= ktop ? k0 - j : ktop;
+          const int k = k0 - j >= klo ? k0 - j : klo;
           const int64_t idx = base + (int64_t)k * 64;
           rb[j] = A.tb[idx];
           rc[j] = A.tc[idx];
@@ -656,21 +687,34 @@ __device__ __forceinline__ void update_column_tile(const Geo &G, const ColArgs &
 #pragma unroll
         for (int j = 0; j < GRP; j++) {
           const int k = k0 - j;
-          if (k < ktop) break;
+          if (k < klo) break;
           un = rd[j] / rb[j] - rc[j] * un / rb[j];
           A.s[base + (int64_t)k * 64] = un;
         }
       }
     } else {
-      for (int k = nk - 2; k >= ktop; k--) {
+      for (int k = nk - 2; k >= klo; k--) {
         const int64_t idx = base + (int64_t)k * 64;
         const double bk = A.tb[idx];
         un = A.td[idx] / bk - A.tc[idx] * un / bk;
         A.s[idx] = un;
       }
     }
+    if constexpr (KR > 0) {
+#pragma unroll
+      for (int j = KR - 1; j >= 0; j--) {
+        const int k = ktop + 1 + j;
+        if (k <= nk - 1) {
+          un = k == nk - 1 ? hd[j] / hb[j] : hd[j] / hb[j] - hc[j] * un / hb[j];
+          A.s[base + (int64_t)k * 64] = un;
+        }
+      }
+      un = h0d / h0b - h0c * un / h0b;
+      A.s[base + (int64_t)ktop * 64] = un;
+    }
   } else if (A.step > 1) {
-    const double b0 = A.tb[base + (int64_t)ktop * 64], d0 = A.td[base + (int64_t)ktop * 64];
+    const double b0 = KR > 0 ? h0b : A.tb[base + (int64_t)ktop * 64];
+    const double d0 = KR > 0 ? h0d : A.td[base + (int64_t)ktop * 64];
     A.s[base + (int64_t)ktop * 64] = (b0 > 0 && A.active[s] != 0) ? d0 / b0 : 0;
   }
   for (int k = 0; k < ct; k++) A.s[base + (int64_t)k * 64] = 0;

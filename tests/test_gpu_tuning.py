@@ -88,15 +88,16 @@ def test_zreg_bitwise_heads(gpu_available, grids, head):
         assert np.array_equal(q0, q1)
 
 
-def scalar_step(st, group, tvd):
+def scalar_step(st, group, tvd, head=0):
     """One UpdateScalars on the device with tvd_group and tvd_sums_group =
-    group; returns the new scalar and Cn."""
+    group and tvd_head = head; returns the new scalar and Cn."""
     dg = device.DeviceGrid(st)
     for f in ("s", "u", "utmp2", "dzf", "dzz", "dzzold", "kappa_tv", "wnew", "wtmp2"):
         dg.upload(f, st.get(f))
     dg.upload("active", st.active().astype(np.float64))
     dg.tune("tvd_group", group)
     dg.tune("tvd_sums_group", group)
+    dg.tune("tvd_head", head)
     dg.update_scalars(st.get_prop("theta"), st.get_prop("dt"), 1e-4, tvd, step=3)
     n = len(st.get("s"))
     out = dg.download("s", n), dg.download("Cn", n)
@@ -121,6 +122,19 @@ def test_tvd_group_bitwise(gpu_available, name, make, tvd):
     for group in (2, 4):
         got = scalar_step(st, group, tvd)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), group
+
+
+@pytest.mark.parametrize("tvd", [1, 4])
+@pytest.mark.parametrize("name,make", SCALAR_GRIDS, ids=[g[0] for g in SCALAR_GRIDS])
+def test_tvd_head_bitwise(gpu_available, name, make, tvd):
+    """The column update with its top 9 or 17 Thomas rows kept in registers
+    (columns shallower and deeper than the head) gives the same bits as with
+    every row through memory."""
+    st = make()
+    ref = scalar_step(st, 2, tvd, 0)
+    for head in (8, 16):
+        got = scalar_step(st, 2, tvd, head)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), head
 
 
 def h_solve(st, tuning):
