@@ -184,7 +184,9 @@ int sunhip_q_coefficients(sunhip_grid *g);
  * memory: 0 off, 1 on, 2 on with a register budget of 4 waves per SIMD, 3 on
  * with 2 levels per load group, 4-6 the pipelined form; -1, the default: 2
  * from 4096 tiles up, else 0), "tvd_group" (levels per load group of the
- * scalar step's column update: 1, 2 (the default) or 4), "tvd_head" (Thomas
+ * scalar step's column update: 1, 2 (the default) or 4), "prec_lds" (levels
+ * below the preconditioner's register head of 16 or 24 that keep their
+ * Thomas rows in LDS between the sweeps: 0 or 12, the default), "tvd_head" (Thomas
  * rows below the top row that the TVD column update keeps in registers
  * between its sweeps, with load groups of 2: 0, 8 (the default) or 16),
  * "tvd_sums_group" (the same as "tvd_group" for the TVD inflow sums: 1, 2 or

@@ -132,6 +132,7 @@ struct sunhip_grid {
   int *d_b4perm = nullptr;          // depth-first dispatch order of 4-tile blocks (Geo)
   int tune_order = 1;               // 1: use it
   int tune_tvdgrp = 2;              // levels per load group of the scalar column update (1, 2, 4)
+  int tune_plds = 12;               // levels below the preconditioner's register head kept in LDS (0, 12)
   int tune_tvdhead = 8;             // rows of the scalar column update kept in registers (0, 8, 16)
   int tune_hblk = 1024;             // most blocks of the free-surface iteration kernels
   int tune_sumsgrp = 4;             // levels per load group of the TVD inflow sums (1, 2, 4)

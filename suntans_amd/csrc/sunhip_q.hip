@@ -193,19 +193,21 @@ __global__ __launch_bounds__(256, MINW > 0 ? MINW : 1) void k_qapply_xp(Geo G, Q
 }
 
 // Vertical-line preconditioner (qprec_tile) over all tiles, 4 per block.
-template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false>
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false, int KL = 0>
 __global__ __launch_bounds__(256) void k_qprec(Geo G, const T *__restrict__ coef, const T *__restrict__ z,
                                                T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq,
                                                CGScal *sc, Red R, ZReg Z) {
   if (UPDATE && sc->done) return;
+  // rows of the LDS stage: [level][b', c, d'][lane of the block]
+  __shared__ T lrows[KL > 0 ? 3 * KL * 256 : 1];
   const int t = tile_group(G, blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   double rz = 0, rr = 0;
   if (ZR && t < G.Nt && Z.tflag[t])
-    qprec_tile<UPDATE, U, T, RZID, KR, true>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
-                                             threadIdx.x & 63, rz, rr, &Z);
+    qprec_tile<UPDATE, U, T, RZID, KR, true, KL>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
+                                                 threadIdx.x & 63, rz, rr, &Z, lrows + threadIdx.x);
   else
-    qprec_tile<UPDATE, U, T, RZID, KR, false>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
-                                              threadIdx.x & 63, rz, rr, &Z);
+    qprec_tile<UPDATE, U, T, RZID, KR, false, KL>(G, coef, z, r, out, bq, UPDATE ? (T)sc->nu : T(0), t,
+                                                  threadIdx.x & 63, rz, rr, &Z, lrows + threadIdx.x);
   double v[2] = {rz, rr};
   reduce_out<2, 256>(v, R, sc);
 }
@@ -411,16 +413,21 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   const bool rzid = g->tune_rzid && op != OP_NONE;
   const bool zr = Z && Z->cwin;
   const ZReg ZZ = zr ? *Z : ZReg{};
-#define PREC(U, KR) \
+#define PREC3(U, KR, KL) \
   do { \
     if (zr) { \
-      if (rzid) k_qprec<UPDATE, U, T, true, KR, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
-      else k_qprec<UPDATE, U, T, false, KR, true><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      if (rzid) \
+        k_qprec<UPDATE, U, T, true, KR, true, KL><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      else \
+        k_qprec<UPDATE, U, T, false, KR, true, KL><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
     } else { \
-      if (rzid) k_qprec<UPDATE, U, T, true, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
-      else k_qprec<UPDATE, U, T, false, KR><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      if (rzid) \
+        k_qprec<UPDATE, U, T, true, KR, false, KL><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
+      else \
+        k_qprec<UPDATE, U, T, false, KR, false, KL><<<nb, 256, 0, g->stream>>>(G, coef, z, r, out, bq, sc, R, ZZ); \
     } \
   } while (0)
+#define PREC(U, KR) PREC3(U, KR, 0)
   // FP32 columns keep 16 levels by default: 24 and 32 measured no faster on
   // the 5e7-unknown solve (the register head costs occupancy)
   const bool fp32 = std::is_same<T, float>::value;
@@ -436,12 +443,19 @@ int launch_prec(sunhip_grid *g, CGScal *sc, const T *coef, const T *z, T *r, T *
   } else {  // register head: the memory part at 4 levels per group
     switch (head) {
       case 8: PREC(4, 8); break;
-      case 16: PREC(4, 16); break;
-      case 24: PREC(4, 24); break;
+      case 16:
+        if (g->tune_plds == 12) PREC3(4, 16, 12);
+        else PREC(4, 16);
+        break;
+      case 24:
+        if (g->tune_plds == 12) PREC3(4, 24, 12);
+        else PREC(4, 24);
+        break;
       default: PREC(4, 32); break;
     }
   }
 #undef PREC
+#undef PREC3
   prof_end(g, ps);
   HIPCHK(hipGetLastError());
   if (op != OP_NONE && g->tune_reduce == 1) k_qfinalize<2><<<1, 1024, 0, g->stream>>>(g->d_part, nb, sc, op);
@@ -1107,6 +1121,10 @@ int sunhip_set_tuning(sunhip_grid *g, const char *key, int value) {
   else if (!strcmp(key, "tvd_group")) {
     if (value != 1 && value != 2 && value != 4) { set_error("tvd_group must be 1, 2 or 4"); return -1; }
     g->tune_tvdgrp = value;
+  }
+  else if (!strcmp(key, "prec_lds")) {
+    if (value != 0 && value != 12) { set_error("prec_lds must be 0 or 12"); return -1; }
+    g->tune_plds = value;
   }
   else if (!strcmp(key, "tvd_head")) {
     if (value != 0 && value != 8 && value != 16) { set_error("tvd_head must be 0, 8 or 16"); return -1; }

@@ -926,10 +926,16 @@ __device__ __forceinline__ double qapply_xchg_pipe(const Geo &G, const QVec<T, T
 // arithmetic and its order are the same for every KR.
 // ZR (a tile whose columns fit their z-level windows): coef formed from
 // the window and the exceptions (Z), as in qapply_tile.
-template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false>
+// KL > 0 (with KR > 0): the KL levels below the head keep b', c and d' in
+// LDS between the sweeps (lrow: this lane's first word, the block's lanes
+// interleaved, 3*KL words of stride 256), so they neither write d' to out
+// nor read out, bq and coef back; the arithmetic is the same again.
+template <bool UPDATE, int U, typename T, bool RZID = false, int KR = 0, bool ZR = false, int KL = 0>
 __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ coef, const T *__restrict__ z,
                                            T *__restrict__ r, T *__restrict__ out, const T *__restrict__ bq, T nu,
-                                           int t, int l, double &rz, double &rr, const ZReg *Zp = nullptr) {
+                                           int t, int l, double &rz, double &rr, const ZReg *Zp = nullptr,
+                                           T *lrow = nullptr) {
+  static_assert(KL == 0 || (KR > 0 && KL % U == 0), "an LDS stage follows a register head, in whole load groups");
   if (t >= G.Nt) return;
   const int s = t * 64 + l, n = G.nk[s], kt = G.ctop[s];
   if (n == 0 || G.comp[s] != 1) return;
@@ -995,7 +1001,50 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
       }
     }
   }
-  const int klo = kt + KR;  // first level that goes through memory
+  // the LDS stage: levels kt+KR .. kt+KR+KL-1, as the memory loop below but
+  // with the row kept in lrow
+  const int kls = kt + KR, kle = min(n, kt + KR + KL);
+  if constexpr (KL > 0) {
+    for (int kb = kls; kb < kle; kb += U) {
+      T rv[U], zv[U], cv[U + 1];
+#pragma unroll
+      for (int j = 0; j <= U; j++) {
+        const uint32_t idx = base + (uint32_t)min(kb + j, n) * 64u;
+        cv[j] = C(min(kb + j, n));
+        if (j < U) {
+          rv[j] = at32((const T *)r, idx);
+          zv[j] = UPDATE ? at32(z, idx) : T(0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = kb + j;
+        if (k < kle) {
+          const uint32_t idx = base + (uint32_t)k * 64u;
+          T rk = rv[j];
+          if (UPDATE) { rk = rk - nu * zv[j]; st_stream<true>(&at32(r, idx), rk); }
+          rr += (double)rk * (double)rk;
+          const T ck = cv[j], ck1 = cv[j + 1];
+          T a = 0, b, c = 0;
+          if (k == kt) { b = -2 * ck - ck1; c = ck1; }
+          else if (k == n - 1) { a = ck; b = -ck; }
+          else { a = ck; b = -ck - ck1; c = ck1; }
+          T d = rk;
+          if (k > kt) {
+            b -= a * cprev / bprev;
+            d -= a * dprev / bprev;
+          }
+          if constexpr (RZID) rz += (double)d * (double)d / (double)b;
+          T *w = lrow + (3 * (k - kls)) * 256;
+          w[0] = b;
+          w[256] = c;
+          w[512] = d;
+          cprev = c; bprev = b; dprev = d;
+        }
+      }
+    }
+  }
+  const int klo = kt + KR + KL;  // first level that goes through memory
   for (int kb = klo; kb < n; kb += U) {
     T rv[U], zv[U], cv[U + 1];
 #pragma unroll
@@ -1053,6 +1102,17 @@ __device__ __forceinline__ void qprec_tile(const Geo &G, const T *__restrict__ c
         unext = u;
         cnext = cv[j];  // c at level k-1 is coef[k]
       }
+    }
+  }
+  if constexpr (KL > 0) {
+    for (int k = kle - 1; k >= kls; k--) {
+      const T *w = lrow + (3 * (k - kls)) * 256;
+      const T b = w[0], c = w[256], d = w[512];
+      const T u = (k == n - 1) ? d / b : d / b - c * unext / b;
+      const uint32_t idx = base + (uint32_t)k * 64u;
+      st_stream<true>(&at32(out, idx), u);
+      if constexpr (!RZID) rz += (double)at32((const T *)r, idx) * (double)u;
+      unext = u;
     }
   }
   if constexpr (KR > 0) {

@@ -48,6 +48,25 @@ def test_variant_bitwise(gpu_available, grids, key, value):
         assert np.array_equal(q0, q1)
 
 
+@pytest.mark.parametrize("head", [16, 24])
+@pytest.mark.parametrize("rzid", [0, 1])
+def test_prec_lds_bitwise(gpu_available, grids, head, rzid):
+    """The preconditioner's 12 levels below its register head kept in LDS
+    (columns of 13 to 50 levels: shorter than the head, ending inside the
+    LDS stage and below it) give the same bits, in the FP64 solve and in the
+    mixed form's FP32 inner solve."""
+    for st in grids:
+        base = [("prec_head", head), ("rzid", rzid)]
+        q0, n0 = solve(st, base + [("prec_lds", 0)])
+        q1, n1 = solve(st, base + [("prec_lds", 12)])
+        assert n0 == n1
+        assert np.array_equal(q0, q1)
+        a = solve_mixed(st, base + [("prec_lds", 0)])
+        b = solve_mixed(st, base + [("prec_lds", 12)])
+        assert a[1:] == b[1:]
+        assert np.array_equal(a[0], b[0])
+
+
 @pytest.mark.parametrize("key,value", ROUNDING, ids=[f"{k}={v}" for k, v in ROUNDING])
 def test_variant_rounding(gpu_available, grids, key, value):
     for st in grids:
