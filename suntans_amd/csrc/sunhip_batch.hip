@@ -129,6 +129,9 @@ template <int U, typename T>
 __global__ __launch_bounds__(256) void kb_qprec(const BQMember<T, T> *__restrict__ M, const int2 *__restrict__ W,
                                                 int nwork, double *__restrict__ part) {
   const int l = threadIdx.x & 63, stride = gridDim.x * BW;
+  // each lane's rows of the LDS stage (its own words only)
+  constexpr int KL = 12;
+  __shared__ T lrows[3 * KL * 256];
   for (int w = first_item(); w < nwork; w += stride) {
     const int2 it = W[w];
     const int m = __builtin_amdgcn_readfirstlane(it.x), t = __builtin_amdgcn_readfirstlane(it.y);
@@ -136,9 +139,11 @@ __global__ __launch_bounds__(256) void kb_qprec(const BQMember<T, T> *__restrict
     if (B.sc->done) continue;
     double rz = 0, rr = 0;
     // the top 24 (FP64) or 16 (FP32) levels of each column stay in registers
-    // between the Thomas sweeps, as in the single-grid kernel
-    qprec_tile<true, U, T, true, sizeof(T) == 8 ? 24 : 16>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq, (T)B.sc->nu, t, l,
-                                                          rz, rr);
+    // between the Thomas sweeps and the next 12 in LDS, as in the
+    // single-grid kernel
+    qprec_tile<true, U, T, true, sizeof(T) == 8 ? 24 : 16, false, KL>(B.G, B.coef, B.z, B.r, B.rtmp, B.bq,
+                                                                      (T)B.sc->nu, t, l, rz, rr, nullptr,
+                                                                      lrows + threadIdx.x);
     rz = wave_sum(rz);
     rr = wave_sum(rr);
     if (l == 0) { part[2 * (size_t)w] = rz; part[2 * (size_t)w + 1] = rr; }
@@ -249,7 +254,8 @@ __global__ __launch_bounds__(256) void kb_update_column(const BTMember *__restri
     const int2 it = W[w];
     const BTMember &B = M[__builtin_amdgcn_readfirstlane(it.x)];
     // members with the faces pass's flux terms read those (TVD >= 1)
-    if (B.C.P1) update_column_tile<NF, 2, false, true>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
+    // (and keep their top 9 Thomas rows in registers)
+    if (B.C.P1) update_column_tile<NF, 2, false, true, 8>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
     else update_column_tile<NF, 2>(B.G, B.C, __builtin_amdgcn_readfirstlane(it.y), l);
   }
 }
