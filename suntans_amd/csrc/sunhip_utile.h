@@ -224,21 +224,25 @@ __device__ __forceinline__ void upred_edge_tile(const Geo &G, const UPArgs &A, i
     else { h1p = 0; h2p = 0; up = 0; scp = 0; }
   }
   if (dry) { A.err[0] = 1; return; }
-  // substitution, bottom up
+  // substitution, bottom up; it forms the final U*** of every wet level
+  // (et..kb), so the nan test of U*** is taken on the values as they are
+  // formed instead of reading them back
   double un = dprev / bprev, en = eprev / bprev;
+  bool bad = un != un;
   E(A.utmp, kb) = un;
   E(A.E, kb) = en;
   for (int k = kb - 1; k >= et; k--) {
     const double bk = E(A.sbp, k), ck = E(A.sc, k);
     un = E(A.utmp, k) / bk - ck * un / bk;
     en = E(A.sep, k) / bk - ck * en / bk;
+    bad |= un != un;
     E(A.utmp, k) = un;
     E(A.E, k) = en;
   }
   double Dj = 0;
   for (int k = et; k < ne; k++) Dj += E(A.E, k) * at32(A.dzf, base + (uint32_t)k * 64u);
   A.D[s] = Dj;
-  nan_check();
+  if (bad) A.err[1] = 1;
 }
 
 // The term of htmp's face sum at every level of edge slot s (main/phys.c:

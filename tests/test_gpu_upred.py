@@ -11,10 +11,11 @@ hooks, which do nothing on these closed grids), against the oracle.
 import numpy as np
 import pytest
 
-from suntans_amd import cases
+from suntans_amd import cases, device
 
 import _grids
 import _impls
+import _trajectory
 
 pytestmark = pytest.mark.gpu
 
@@ -76,3 +77,32 @@ def test_upredictor_solve(gpu_available, name, make, variant):
     assert np.array_equal(a.get("hcorr"), b.get("hcorr"))
     if variant == "drying":
         assert (a.active() == 0).sum() > 0
+
+
+@pytest.mark.parametrize("where", ["top", "bottom"])
+def test_upredictor_nan_refused(gpu_available, where):
+    """U*** = nan on a wet level of an interior edge (the explicit source
+    utmp planted with nan at its top or bottom level) is refused with an
+    error (main/phys.c:2945-2950); the same call without it goes through."""
+    st = cases.lock_exchange(seed=7)
+    cases.timestep_inputs(st, np.random.default_rng(31), nonlinear=2)
+    g = st.get_prop
+    P = device.UPredParams(g("dt"), g("theta"), g("grav"), g("nu"), g("laxWendroff_Vertical"), g("thetaM"),
+                           int(g("nonlinear")))
+    dg = device.DeviceGrid(st)
+    _trajectory.upload_state(dg, st)
+    dg.set_flux_height(st)
+    dg.u_predict(P)
+    edge, k = st.edge_layers()
+    wet = st.edge_wet_mask() & (st.ints("mark")[edge] == 0)
+    e = edge[np.flatnonzero(wet)[len(np.flatnonzero(wet)) // 2]]
+    levels = np.flatnonzero(wet & (edge == e))
+    assert len(levels) > 2
+    bad = st.get("utmp").copy()
+    bad[levels[0] if where == "top" else levels[-1]] = np.nan
+    _trajectory.upload_state(dg, st)
+    dg.upload("utmp", bad)
+    dg.set_flux_height(st)
+    with pytest.raises(RuntimeError, match="nan"):
+        dg.u_predict(P)
+    dg.close()
