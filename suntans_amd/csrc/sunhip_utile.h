@@ -60,7 +60,8 @@ __device__ __forceinline__ void upred_edge_tile(const Geo &G, const UPArgs &A, i
       if (E(A.utmp, k) != E(A.utmp, k)) A.err[1] = 1;
   };
   A.D[s] = 0;
-  for (int k = 0; k < ne; k++) E(A.utmp2, k) = E(A.u, k);
+  // streamed: utmp2 is read by later launches only
+  for (int k = 0; k < ne; k++) __builtin_nontemporal_store(E(A.u, k), &E(A.utmp2, k));
   if (comp == 2)
     for (int k = et; k < ne; k++) E(A.utmp, k) = E(A.u, k);
   if (comp != 1) { nan_check(); return; }
@@ -236,7 +237,7 @@ __device__ __forceinline__ void upred_edge_tile(const Geo &G, const UPArgs &A, i
     un = E(A.utmp, k) / bk - ck * un / bk;
     en = E(A.sep, k) / bk - ck * en / bk;
     bad |= un != un;
-    E(A.utmp, k) = un;
+    __builtin_nontemporal_store(un, &E(A.utmp, k));
     E(A.E, k) = en;
   }
   double Dj = 0;
